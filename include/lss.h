@@ -1,0 +1,135 @@
+/*
+ * lss.h — C ABI of the MI355X hybrid solver for locally stiff SPD systems.
+ *
+ * Drop-in boundary for the reference's GPU hybrid solver
+ * polysolve::linear::GPUHybrid (src/polysolve/linear/GPUHybrid.hpp:27-192,
+ * src/polysolve/linear/GPUHybrid.cu:79-1543) and the polysolve::linear::Solver
+ * surface it implements (src/polysolve/linear/Solver.hpp:32-149).  Plain
+ * pointers and sizes only; every entry point returns an lss_status and never
+ * exits the process (the reference calls exit(EXIT_FAILURE) on device errors,
+ * GPUHybrid.cu:34-63).
+ *
+ * Matrix input: the compressed-sparse-column arrays of polysolve's
+ * StiffnessMatrix (Eigen::SparseMatrix<double, ColMajor, int>,
+ * src/polysolve/Types.hpp:162-166): 0-based int32 indices, full symmetric
+ * storage, ascending row indices per column.  Like the reference (which hands
+ * the CSC arrays to hypre as CSR rows, GPUHybrid.cu:401-433) the solver reads
+ * column j as row j of the operator, so the matrix must be symmetric.
+ *
+ * Threading: one handle per host thread; calls on one handle must not overlap.
+ * Device: the HIP device current at lss_create.
+ */
+#ifndef LSS_H
+#define LSS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum lss_status {
+    LSS_OK = 0,
+    LSS_ERR_CONFIG = 1,          /* invalid JSON / key / type (Solver.cpp:157-165 throws)          */
+    LSS_ERR_SIZE = 2,            /* size mismatch (MASSolver.cu:486-489 throws)                     */
+    LSS_ERR_NOT_FACTORIZED = 3,  /* solve before factorize                                          */
+    LSS_ERR_HIP = 4,             /* HIP runtime error (GPUHybrid.cu:34-63 exits instead)            */
+    LSS_ERR_NOT_SPD = 5,         /* non-SPD stiff block or MAS bank (MASPreconditioner.cu:934)      */
+    LSS_ERR_OOM = 6,             /* device allocation failed (CudaUtils.cuh:41-124 diagnoses)       */
+    LSS_ERR_UNSUPPORTED = 7,     /* input outside what this build implements                        */
+    LSS_ERR_ARG = 8              /* null handle / pointer, negative size                            */
+} lss_status;
+
+typedef struct lss_solver lss_solver;
+
+/*
+ * Create a solver from a JSON document shaped like spec/GPUHybrid.json:
+ *   {"solver": "GPUHybrid" | ["GPUHybrid", ...], "GPUHybrid": {key: value, ...}}
+ * Keys and defaults follow linear-solver-spec.json:646-747 (the defaults
+ * Solver::create injects, Solver.cpp:143-173); keys parsed as in
+ * GPUHybrid::set_parameters (GPUHybrid.cu:106-179).  Additional keys of this
+ * build are listed in INTEGRATION.md.  json may be NULL or "" for all defaults.
+ * Replaces: Solver::create(json, logger) + GPUHybrid::GPUHybrid() (GPUHybrid.cu:79-104).
+ */
+int lss_create(const char *json, lss_solver **out);
+
+/* Re-apply parameters.  Replaces GPUHybrid::set_parameters (GPUHybrid.cu:106-179). */
+int lss_set_parameters(lss_solver *s, const char *json);
+
+/* User-supplied stiff DOFs, used when select_bad_dofs_from_l1_norm is false.
+ * Replaces GPUHybrid::set_problematic_dofs (GPUHybrid.hpp:69). */
+int lss_set_problematic_dofs(lss_solver *s, const int32_t *dofs, int64_t n);
+
+/* No-op kept for the Solver surface.  Replaces Solver::analyze_pattern (Solver.hpp:97). */
+int lss_analyze_pattern(lss_solver *s, int32_t n, int64_t nnz, const int32_t *colptr, const int32_t *rowidx);
+
+/* Factorize from host CSC arrays (copied; the caller keeps ownership).
+ * Replaces GPUHybrid::factorize (GPUHybrid.cu:195-282). */
+int lss_factorize_csc(lss_solver *s, int32_t n, int64_t nnz, const int32_t *colptr, const int32_t *rowidx,
+                      const double *vals);
+
+/* Factorize from device-resident CSC arrays (not retained after return).
+ * stream: hipStream_t or NULL for the solver's own stream.  Same semantics as
+ * lss_factorize_csc without the host-to-device copy. */
+int lss_factorize_csc_device(lss_solver *s, int32_t n, int64_t nnz, const int32_t *d_colptr,
+                             const int32_t *d_rowidx, const double *d_vals, void *stream);
+
+/* Solve A x = b with host vectors; x is in/out (initial guess).
+ * Replaces GPUHybrid::solve (GPUHybrid.cu:342-399) and pcg_solve (1413-1512). */
+int lss_solve(lss_solver *s, const double *b, double *x);
+
+/* Same with device vectors (n doubles each), enqueued on stream (or the solver's
+ * own stream when NULL); returns after the solve has finished. */
+int lss_solve_device(lss_solver *s, const double *d_b, double *d_x, void *stream);
+
+/* Solve k right-hand sides (column-major n x k device arrays) as one fused
+ * block PCG that shares every matrix and preconditioner read; each column keeps
+ * its own scalars and stopping test.  No reference counterpart (the reference
+ * solves one right-hand side per call, GPUHybrid.cu:342). */
+int lss_solve_multi_device(lss_solver *s, int32_t k, const double *d_B, double *d_X, void *stream);
+
+/* JSON info: {"num_iterations", "final_res_norm"} as GPUHybrid::get_info
+ * (GPUHybrid.cu:181-185) plus phase timings and stiff-set statistics.
+ * Writes at most cap bytes (NUL-terminated); *needed (optional) gets the full length. */
+int lss_get_info(const lss_solver *s, char *buf, size_t cap, size_t *needed);
+
+/* Solver name with the reference's rules (GPUHybrid.hpp:57-67):
+ * "GPUHybrid", "GPUAMGF" or "GPUAMG". */
+const char *lss_name(const lss_solver *s);
+
+/* Stiff DOF components selected at factorize (ascending DOFs per component,
+ * components ordered by smallest DOF).  dofs[cap], comp_off[comp_cap] (ncomp+1
+ * entries).  The reference keeps them in bad_indices_arrays (GPUHybrid.cu:219-262). */
+int lss_get_stiff_dofs(const lss_solver *s, int32_t *dofs, int64_t cap, int64_t *n_dofs, int64_t *comp_off,
+                       int64_t comp_cap, int64_t *n_comp);
+
+/* Last error message of this handle (empty string when none). */
+const char *lss_last_error(const lss_solver *s);
+
+/* Release all device memory.  Replaces GPUHybrid::~GPUHybrid (GPUHybrid.cu:1514-1530). */
+void lss_destroy(lss_solver *s);
+
+/* ---- inspection entry points (parity tests and tools) --------------------
+ * Each runs the named device stage of the factorized solver on host data.  No
+ * reference counterpart as separate calls: they expose the stages of
+ * GPUHybrid.cu named in each comment. */
+
+/* Row l1 norms computed on the device at factorize (GPUHybrid.cu:753-765). */
+int lss_debug_row_norms(const lss_solver *s, double *out, int64_t n);
+/* GMM result: params = {global_mean, global_var, mean0, mean1, var0, var1, w0, w1},
+ * iters, jumped, selected (GPUHybrid.cu:767-883). */
+int lss_debug_gmm(const lss_solver *s, double *params, int32_t *iters, int32_t *jumped, int64_t *selected);
+/* MAS partition: perm[old_node] = new_node (nb entries), part_off (nparts+1). */
+int lss_debug_partition(const lss_solver *s, int32_t *perm, int64_t nb, int32_t *part_off, int64_t cap,
+                        int32_t *nparts);
+/* y = A x with the device SpMV (FP64). */
+int lss_debug_spmv(lss_solver *s, const double *x, double *y);
+/* z = B r (base preconditioner, undamped) and z = M^{-1} r (full hybrid preconditioner). */
+int lss_debug_base_apply(lss_solver *s, const double *r, double *z);
+int lss_debug_precond_apply(lss_solver *s, const double *r, double *z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LSS_H */

@@ -1,0 +1,249 @@
+// C ABI (include/lss.h) over HybridSolver.  Every entry point catches all
+// exceptions and returns a status code; the message is kept per handle.
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+
+#include "lss.h"
+#include "config.hpp"
+#include "solver.hpp"
+
+struct lss_solver {
+    std::unique_ptr<lss::HybridSolver> impl;
+    std::string last_error;
+};
+
+namespace {
+
+template <class F>
+int guarded(lss_solver *s, F &&f)
+{
+    if (!s) return LSS_ERR_ARG;
+    try {
+        f();
+        s->last_error.clear();
+        return LSS_OK;
+    } catch (const lss::Error &e) {
+        s->last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        s->last_error = "host allocation failed";
+        return LSS_ERR_OOM;
+    } catch (const std::exception &e) {
+        s->last_error = e.what();
+        return LSS_ERR_HIP;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int lss_create(const char *json, lss_solver **out)
+{
+    if (!out) return LSS_ERR_ARG;
+    *out = nullptr;
+    auto s = new (std::nothrow) lss_solver();
+    if (!s) return LSS_ERR_OOM;
+    int rc = guarded(s, [&] {
+        lss::Config cfg = lss::parse_config(json, lss::Config{});
+        s->impl = std::make_unique<lss::HybridSolver>(cfg);
+    });
+    if (rc != LSS_OK) {
+        // keep the handle so the caller can read the message, but mark it unusable
+        *out = s;
+        return rc;
+    }
+    *out = s;
+    return LSS_OK;
+}
+
+int lss_set_parameters(lss_solver *s, const char *json)
+{
+    return guarded(s, [&] {
+        if (!s->impl) lss::fail(LSS_ERR_CONFIG, "solver was not created");
+        s->impl->set_config(lss::parse_config(json, s->impl->config()));
+    });
+}
+
+int lss_set_problematic_dofs(lss_solver *s, const int32_t *dofs, int64_t n)
+{
+    return guarded(s, [&] {
+        if (!s->impl) lss::fail(LSS_ERR_CONFIG, "solver was not created");
+        if (n < 0 || (n > 0 && !dofs)) lss::fail(LSS_ERR_ARG, "invalid DOF list");
+        s->impl->set_problematic_dofs(dofs, n);
+    });
+}
+
+int lss_analyze_pattern(lss_solver *s, int32_t, int64_t, const int32_t *, const int32_t *)
+{
+    return guarded(s, [&] {
+        if (!s->impl) lss::fail(LSS_ERR_CONFIG, "solver was not created");
+    });
+}
+
+int lss_factorize_csc(lss_solver *s, int32_t n, int64_t nnz, const int32_t *colptr, const int32_t *rowidx,
+                      const double *vals)
+{
+    return guarded(s, [&] {
+        if (!s->impl) lss::fail(LSS_ERR_CONFIG, "solver was not created");
+        if (n <= 0 || nnz < 0 || !colptr || (nnz > 0 && (!rowidx || !vals))) lss::fail(LSS_ERR_ARG, "invalid matrix");
+        if (int64_t(colptr[n]) != nnz) lss::fail(LSS_ERR_SIZE, "colptr[n] does not match nnz");
+        s->impl->factorize_host(n, nnz, colptr, rowidx, vals);
+    });
+}
+
+int lss_factorize_csc_device(lss_solver *s, int32_t n, int64_t nnz, const int32_t *d_colptr, const int32_t *d_rowidx,
+                             const double *d_vals, void *stream)
+{
+    return guarded(s, [&] {
+        if (!s->impl) lss::fail(LSS_ERR_CONFIG, "solver was not created");
+        if (n <= 0 || nnz < 0 || !d_colptr || (nnz > 0 && (!d_rowidx || !d_vals)))
+            lss::fail(LSS_ERR_ARG, "invalid matrix");
+        s->impl->factorize_device(n, nnz, d_colptr, d_rowidx, d_vals, static_cast<hipStream_t>(stream));
+    });
+}
+
+int lss_solve(lss_solver *s, const double *b, double *x)
+{
+    return guarded(s, [&] {
+        if (!s->impl) lss::fail(LSS_ERR_CONFIG, "solver was not created");
+        if (!b || !x) lss::fail(LSS_ERR_ARG, "null vector");
+        s->impl->solve_host(b, x);
+    });
+}
+
+int lss_solve_device(lss_solver *s, const double *d_b, double *d_x, void *stream)
+{
+    return guarded(s, [&] {
+        if (!s->impl) lss::fail(LSS_ERR_CONFIG, "solver was not created");
+        if (!d_b || !d_x) lss::fail(LSS_ERR_ARG, "null vector");
+        s->impl->solve_device(d_b, d_x, static_cast<hipStream_t>(stream));
+    });
+}
+
+int lss_solve_multi_device(lss_solver *s, int32_t k, const double *d_B, double *d_X, void *stream)
+{
+    return guarded(s, [&] {
+        if (!s->impl) lss::fail(LSS_ERR_CONFIG, "solver was not created");
+        if (k < 0 || (k > 0 && (!d_B || !d_X))) lss::fail(LSS_ERR_ARG, "invalid right-hand sides");
+        s->impl->solve_multi_device(k, d_B, d_X, static_cast<hipStream_t>(stream));
+    });
+}
+
+int lss_get_info(const lss_solver *s, char *buf, size_t cap, size_t *needed)
+{
+    if (!s || !s->impl) return LSS_ERR_ARG;
+    try {
+        const std::string j = s->impl->info_json();
+        if (needed) *needed = j.size() + 1;
+        if (buf && cap > 0) {
+            const size_t m = std::min(cap - 1, j.size());
+            std::memcpy(buf, j.data(), m);
+            buf[m] = '\0';
+        }
+        return LSS_OK;
+    } catch (...) {
+        return LSS_ERR_HIP;
+    }
+}
+
+const char *lss_name(const lss_solver *s)
+{
+    if (!s || !s->impl) return "";
+    return s->impl->name();
+}
+
+int lss_get_stiff_dofs(const lss_solver *s, int32_t *dofs, int64_t cap, int64_t *n_dofs, int64_t *comp_off,
+                       int64_t comp_cap, int64_t *n_comp)
+{
+    if (!s || !s->impl) return LSS_ERR_ARG;
+    const auto &comps = s->impl->components();
+    int64_t w = 0;
+    if (comp_off && comp_cap > 0) comp_off[0] = 0;
+    for (size_t c = 0; c < comps.size(); ++c) {
+        for (int32_t d : comps[c]) {
+            if (dofs && w < cap) dofs[w] = d;
+            ++w;
+        }
+        if (comp_off && int64_t(c + 1) < comp_cap) comp_off[c + 1] = w;
+    }
+    if (n_dofs) *n_dofs = w;
+    if (n_comp) *n_comp = int64_t(comps.size());
+    return LSS_OK;
+}
+
+const char *lss_last_error(const lss_solver *s)
+{
+    if (!s) return "null handle";
+    return s->last_error.c_str();
+}
+
+void lss_destroy(lss_solver *s) { delete s; }
+
+int lss_debug_row_norms(const lss_solver *s, double *out, int64_t n)
+{
+    return guarded(const_cast<lss_solver *>(s), [&] {
+        if (!s->impl || n != s->impl->size()) lss::fail(LSS_ERR_SIZE, "size mismatch");
+        s->impl->debug_row_norms(out);
+    });
+}
+
+int lss_debug_gmm(const lss_solver *s, double *params, int32_t *iters, int32_t *jumped, int64_t *selected)
+{
+    return guarded(const_cast<lss_solver *>(s), [&] {
+        if (!s->impl) lss::fail(LSS_ERR_CONFIG, "solver was not created");
+        const auto &g = s->impl->gmm();
+        const double v[8] = {g.global_mean, g.global_var, g.mean0, g.mean1, g.var0, g.var1, g.w0, g.w1};
+        if (params) std::memcpy(params, v, sizeof(v));
+        if (iters) *iters = g.iters;
+        if (jumped) *jumped = g.jumped;
+        if (selected) *selected = g.selected;
+    });
+}
+
+int lss_debug_partition(const lss_solver *s, int32_t *perm, int64_t nb, int32_t *part_off, int64_t cap,
+                        int32_t *nparts)
+{
+    return guarded(const_cast<lss_solver *>(s), [&] {
+        if (!s->impl) lss::fail(LSS_ERR_CONFIG, "solver was not created");
+        const auto &p = s->impl->perm();
+        const auto &o = s->impl->part_off();
+        if (nparts) *nparts = o.empty() ? 0 : int32_t(o.size()) - 1;
+        if (perm) {
+            if (nb != int64_t(p.size())) lss::fail(LSS_ERR_SIZE, "perm size mismatch");
+            std::memcpy(perm, p.data(), p.size() * sizeof(int32_t));
+        }
+        if (part_off) {
+            if (cap < int64_t(o.size())) lss::fail(LSS_ERR_SIZE, "part_off capacity too small");
+            std::memcpy(part_off, o.data(), o.size() * sizeof(int32_t));
+        }
+    });
+}
+
+int lss_debug_spmv(lss_solver *s, const double *x, double *y)
+{
+    return guarded(s, [&] {
+        if (!s->impl) lss::fail(LSS_ERR_CONFIG, "solver was not created");
+        s->impl->debug_spmv(x, y);
+    });
+}
+
+int lss_debug_base_apply(lss_solver *s, const double *r, double *z)
+{
+    return guarded(s, [&] {
+        if (!s->impl) lss::fail(LSS_ERR_CONFIG, "solver was not created");
+        s->impl->debug_base(r, z);
+    });
+}
+
+int lss_debug_precond_apply(lss_solver *s, const double *r, double *z)
+{
+    return guarded(s, [&] {
+        if (!s->impl) lss::fail(LSS_ERR_CONFIG, "solver was not created");
+        s->impl->debug_precond(r, z);
+    });
+}
+
+}  // extern "C"

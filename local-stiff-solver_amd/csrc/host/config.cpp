@@ -1,0 +1,106 @@
+#include "config.hpp"
+
+#include <cmath>
+#include <string>
+
+#include "../common/json.hpp"
+#include "../common/runtime.hpp"
+
+namespace lss {
+namespace {
+
+[[noreturn]] void bad(const std::string &m) { fail(LSS_ERR_CONFIG, "invalid configuration: " + m); }
+
+int64_t as_int(const json::Value &v, const std::string &k)
+{
+    if (v.kind != json::Value::Number || !v.is_int || std::floor(v.num) != v.num) bad(k + " must be an integer");
+    return static_cast<int64_t>(v.num);
+}
+double as_float(const json::Value &v, const std::string &k)
+{
+    if (v.kind != json::Value::Number) bad(k + " must be a number");
+    return v.num;
+}
+bool as_bool(const json::Value &v, const std::string &k)
+{
+    if (v.kind != json::Value::Bool) bad(k + " must be a boolean");
+    return v.b;
+}
+std::string as_str(const json::Value &v, const std::string &k)
+{
+    if (v.kind != json::Value::String) bad(k + " must be a string");
+    return v.str;
+}
+
+}  // namespace
+
+Config parse_config(const char *text, const Config &base)
+{
+    Config c = base;
+    if (!text) return c;
+    std::string s(text);
+    bool blank = true;
+    for (char ch : s)
+        if (!std::isspace(static_cast<unsigned char>(ch))) blank = false;
+    if (blank) return c;
+    json::Value doc;
+    try {
+        doc = json::parse(s);
+    } catch (const std::exception &e) {
+        bad(e.what());
+    }
+    if (doc.kind != json::Value::Object) bad("document must be an object");
+    // solver selection (Solver.cpp:99-141): a name or a priority list that must contain GPUHybrid
+    if (const json::Value *sv = doc.find("solver")) {
+        bool ok = false;
+        if (sv->kind == json::Value::String) ok = sv->str == "GPUHybrid";
+        if (sv->kind == json::Value::Array)
+            for (auto &e : sv->arr)
+                if (e.kind == json::Value::String && e.str == "GPUHybrid") ok = true;
+        if (!ok) bad("\"solver\" must select GPUHybrid");
+    }
+    const json::Value *g = doc.find("GPUHybrid");
+    if (!g) return c;
+    if (g->kind != json::Value::Object) bad("\"GPUHybrid\" must be an object");
+    for (auto &kv : g->obj) {
+        const std::string &k = kv.first;
+        const json::Value &v = kv.second;
+        if (k == "max_iter") c.max_iter = int(as_int(v, k));
+        else if (k == "relative_tolerance") c.relative_tolerance = as_float(v, k);
+        else if (k == "absolute_tolerance") c.absolute_tolerance = as_float(v, k);
+        else if (k == "block_size") c.block_size = int(as_int(v, k));
+        else if (k == "theta") c.theta = as_float(v, k);
+        else if (k == "do_hybrid_precond") c.do_hybrid_precond = as_bool(v, k);
+        else if (k == "select_bad_dofs_from_l1_norm") c.select_bad_dofs_from_l1_norm = as_bool(v, k);
+        else if (k == "conditioning_threshold") c.conditioning_threshold = as_float(v, k);
+        else if (k == "gmm_jump_threshold") c.gmm_jump_threshold = as_float(v, k);
+        else if (k == "gmm_tol") c.gmm_tol = as_float(v, k);
+        else if (k == "max_gmm_iterations") c.max_gmm_iterations = int(as_int(v, k));
+        else if (k == "decompose_subdomains") c.decompose_subdomains = as_bool(v, k);
+        else if (k == "min_subdomain_size") c.min_subdomain_size = as_int(v, k);
+        else if (k == "max_subdomain_size") c.max_subdomain_size = as_int(v, k);
+        else if (k == "expand_subdomains") c.expand_subdomains = as_bool(v, k);
+        else if (k == "max_dense_size") c.max_dense_size = as_int(v, k);
+        else if (k == "use_float_on_subdomains") c.use_float_on_subdomains = as_bool(v, k);
+        else if (k == "precision") c.precision = as_str(v, k);
+        else if (k == "inner_spmv") c.inner_spmv = as_str(v, k);
+        else if (k == "base_preconditioner") c.base_preconditioner = as_str(v, k);
+        else if (k == "check_every") c.check_every = int(as_int(v, k));
+        else if (k == "damping") c.damping = as_float(v, k);
+        else if (k == "power_iterations") c.power_iterations = int(as_int(v, k));
+        else if (k == "max_dense_component") c.max_dense_component = as_int(v, k);
+        else if (k == "profile_kernels") c.profile_kernels = as_bool(v, k);
+        else bad("unknown key GPUHybrid/" + k);
+    }
+    if (c.block_size < 1 || c.block_size > 3) bad("block_size must be 1, 2 or 3");
+    if (c.max_iter < 0) bad("max_iter must be >= 0");
+    if (c.precision != "fp64" && c.precision != "fp32" && c.precision != "bf16") bad("precision must be fp64|fp32|bf16");
+    if (c.inner_spmv != "auto" && c.inner_spmv != "fp64" && c.inner_spmv != "fp32") bad("inner_spmv must be auto|fp64|fp32");
+    if (c.base_preconditioner != "MAS" && c.base_preconditioner != "Jacobi") bad("base_preconditioner must be MAS|Jacobi");
+    if (c.check_every < 1) bad("check_every must be >= 1");
+    if (!(c.damping > 0.0) || c.damping >= 2.0) bad("damping must be in (0, 2)");
+    if (c.power_iterations < 1) bad("power_iterations must be >= 1");
+    return c;
+}
+
+}  // namespace lss

@@ -1,0 +1,46 @@
+// Solver configuration: the keys of spec/GPUHybrid.json with the defaults that
+// Solver::create injects from linear-solver-spec.json:646-747, parsed as in
+// GPUHybrid::set_parameters (GPUHybrid.cu:106-179), plus this build's own keys.
+#pragma once
+#include <cstdint>
+#include <string>
+
+namespace lss {
+
+struct Config {
+    // reference keys (linear-solver-spec.json defaults)
+    int max_iter = 10000;
+    double relative_tolerance = 1e-10;
+    double absolute_tolerance = 0.0;
+    int block_size = 1;
+    double theta = 0.5;  // AMG strength threshold; accepted, unused (MAS has no strength graph)
+    bool do_hybrid_precond = true;
+    bool select_bad_dofs_from_l1_norm = true;
+    double conditioning_threshold = 100.0;
+    double gmm_jump_threshold = 100.0;
+    double gmm_tol = 1e-3;
+    int max_gmm_iterations = 20;
+    bool decompose_subdomains = true;
+    int64_t min_subdomain_size = 1;
+    int64_t max_subdomain_size = 10000000;
+    bool expand_subdomains = true;
+    int64_t max_dense_size = 0;           // accepted; every component uses the dense device Cholesky
+    bool use_float_on_subdomains = false;  // accepted; the stiff factor stays FP64
+    // keys of this build
+    std::string precision = "fp64";     // MAS bank inverses: "fp64" | "fp32" | "bf16"
+    std::string inner_spmv = "auto";    // preconditioner-internal SpMV values: "auto" | "fp64" | "fp32"
+    std::string base_preconditioner = "MAS";  // "MAS" | "Jacobi"
+    int check_every = 8;                // PCG iterations between host polls of the device stop flag
+    double damping = 1.0;               // omega = damping / lambda_max(B A) inside the B-P-B sweep
+    int power_iterations = 12;          // iterations of the lambda_max(B A) estimate
+    int64_t max_dense_component = 16384;  // largest stiff component for the dense Cholesky
+    bool profile_kernels = false;       // time SpMV / MAS launches with HIP events
+
+    bool fp32_inner() const { return inner_spmv == "fp32" || (inner_spmv == "auto" && precision != "fp64"); }
+};
+
+// Parse a configuration document; throws lss::Error(LSS_ERR_CONFIG) on bad input.
+// `base` supplies the values of keys that are absent.
+Config parse_config(const char *json, const Config &base);
+
+}  // namespace lss

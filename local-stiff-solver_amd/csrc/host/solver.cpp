@@ -1,0 +1,807 @@
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <sstream>
+
+#include "../kernels/util_kernels.hpp"
+#include "topology.hpp"
+
+namespace lss {
+namespace {
+
+using clk = std::chrono::steady_clock;
+double since(clk::time_point t0) { return std::chrono::duration<double>(clk::now() - t0).count(); }
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kMaxCoarseLevels = 4;  // MAS_MAX_COARSE_LEVEL, MASPreconditioner.hpp:10-12
+constexpr size_t kPartialsCap = size_t(1) << 20;
+
+template <class T>
+std::vector<T> to_host(const DBuf<T> &d, size_t n, hipStream_t s)
+{
+    std::vector<T> h(n);
+    d.download(h.data(), n, s);
+    LSS_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+double d2h_scalar(const double *d, hipStream_t s)
+{
+    double v = 0;
+    LSS_HIP(hipMemcpyAsync(&v, d, sizeof(double), hipMemcpyDeviceToHost, s));
+    LSS_HIP(hipStreamSynchronize(s));
+    return v;
+}
+
+}  // namespace
+
+HybridSolver::HybridSolver(const Config &cfg) : cfg_(cfg)
+{
+    LSS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    void *p = nullptr;
+    LSS_HIP(hipHostMalloc(&p, sizeof(dev::PcgState), hipHostMallocDefault));
+    h_st_ = static_cast<dev::PcgState *>(p);
+    std::memset(h_st_, 0, sizeof(dev::PcgState));
+    st_.resize(1);
+    st_.zero(stream_);  // reduction tickets start at zero and are reset by every reduction
+    partials_.resize(kPartialsCap);
+    d_status_.resize(1);
+    sync(stream_);
+}
+
+HybridSolver::~HybridSolver()
+{
+    release();
+    for (auto e : ev_pool_) (void)hipEventDestroy(e);
+    if (h_st_) (void)hipHostFree(h_st_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+const char *HybridSolver::name() const
+{
+    // GPUHybrid.hpp:57-67
+    if (cfg_.do_hybrid_precond) return cfg_.select_bad_dofs_from_l1_norm ? "GPUHybrid" : "GPUAMGF";
+    return "GPUAMG";
+}
+
+k::Precision HybridSolver::prec() const
+{
+    if (cfg_.precision == "fp32") return k::Precision::FP32;
+    if (cfg_.precision == "bf16") return k::Precision::BF16;
+    return k::Precision::FP64;
+}
+
+void HybridSolver::set_problematic_dofs(const int32_t *dofs, int64_t n)
+{
+    user_set_.assign(dofs, dofs + n);
+    std::sort(user_set_.begin(), user_set_.end());
+    user_set_.erase(std::unique(user_set_.begin(), user_set_.end()), user_set_.end());
+    user_set_given_ = true;
+}
+
+void HybridSolver::release()
+{
+    factorized_ = false;
+    mas_.clear();
+    comps_.clear();
+    d_perm_.release();
+    d_rowptr_.release();
+    d_cols_.release();
+    d_bvals_.release();
+    d_slice_ptr_.release();
+    d_scols_.release();
+    d_svals_.release();
+    d_svals32_.release();
+    d_dinv_.release();
+    d_comp_off_.release();
+    d_mat_off_.release();
+    d_pdofs_.release();
+    d_L_.release();
+    d_swork_.release();
+    xp_.release(), bp_.release(), r_.release(), p_.release(), q_.release(), z_.release(), u_.release();
+    ncomp_ = 0;
+    ns_ = 0;
+    omega_ = 1.0;
+}
+
+// ------------------------------------------------------------------ factorize
+void HybridSolver::factorize_host(int32_t n, int64_t nnz, const int32_t *colptr, const int32_t *rowidx,
+                                  const double *vals)
+{
+    DBuf<int32_t> dp, di;
+    DBuf<double> dv;
+    dp.upload(colptr, size_t(n) + 1, stream_);
+    di.upload(rowidx, size_t(nnz), stream_);
+    dv.upload(vals, size_t(nnz), stream_);
+    factorize_device(n, nnz, dp.get(), di.get(), dv.get(), stream_);
+    sync(stream_);
+}
+
+void HybridSolver::factorize_device(int32_t n, int64_t nnz, const int32_t *ptr, const int32_t *idx, const double *vals,
+                                    hipStream_t s)
+{
+    // GPUHybrid::factorize, GPUHybrid.cu:195-282
+    if (!s) s = stream_;
+    if (n <= 0) fail(LSS_ERR_SIZE, "matrix must have at least one row");
+    auto t0 = clk::now();
+    release();
+    h_ptr_.clear();
+    n_ = n;
+    bs_ = cfg_.block_size;
+    nb_ = (n + bs_ - 1) / bs_;
+    nnz_ = nnz;
+    int32_t last = 0;
+    LSS_HIP(hipMemcpyAsync(&last, ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    sync(s);
+    if (int64_t(last) != nnz) fail(LSS_ERR_SIZE, "colptr[n] does not match nnz");
+    gmm_ = GmmResult{};
+    filter_ = FilterStats{};
+    n_selected_ = n_filtered_ = n_expanded_ = 0;
+
+    auto t = clk::now();
+    if (cfg_.do_hybrid_precond) select_stiff(n, ptr, idx, vals, s);
+    t_select_ = since(t);
+
+    t = clk::now();
+    build_operator(n, nnz, ptr, idx, vals, s);
+    sync(s);
+    t_operator_ = since(t) - t_partition_;
+
+    t = clk::now();
+    if (cfg_.base_preconditioner == "MAS") {
+        build_mas(s);
+    } else {
+        d_dinv_.resize(size_t(nb_) * bs_);
+        k::diag_inverse(nb_, bs_, d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), d_dinv_.get(), s);
+    }
+    sync(s);
+    t_mas_ = since(t);
+
+    t = clk::now();
+    if (!comps_.empty()) build_stiff(n, ptr, idx, vals, s);
+    sync(s);
+    t_stiff_ = since(t);
+
+    alloc_vectors();
+    t = clk::now();
+    estimate_omega(s);
+    t_omega_ = since(t);
+    factorized_ = true;
+    h_ptr_.clear();
+    h_ptr_.shrink_to_fit();
+    t_factorize_ = since(t0);
+}
+
+RowSet HybridSolver::fetch_rows(const std::vector<int32_t> &rows, const int32_t *ptr, const int32_t *idx,
+                                const double *vals, bool with_vals, hipStream_t s)
+{
+    RowSet rs;
+    rs.rows = rows;
+    const int64_t m = int64_t(rows.size());
+    rs.ptr.assign(m + 1, 0);
+    if (m == 0) return rs;
+    DBuf<int32_t> d_rows;
+    d_rows.upload(rows.data(), m, s);
+    if (h_ptr_.size() != size_t(n_) + 1) {
+        h_ptr_.resize(size_t(n_) + 1);
+        LSS_HIP(hipMemcpyAsync(h_ptr_.data(), ptr, sizeof(int32_t) * (size_t(n_) + 1), hipMemcpyDeviceToHost, s));
+        sync(s);
+    }
+    std::vector<int32_t> lo(m), hi(m);
+    for (int64_t k = 0; k < m; ++k) {
+        lo[k] = h_ptr_[rows[k]];
+        hi[k] = h_ptr_[rows[k] + 1];
+    }
+    for (int64_t k = 0; k < m; ++k) rs.ptr[k + 1] = rs.ptr[k] + (hi[k] - lo[k]);
+    const int64_t tot = rs.ptr[m];
+    DBuf<int64_t> d_optr;
+    d_optr.upload(rs.ptr.data(), m + 1, s);
+    DBuf<int32_t> d_oidx(std::max<int64_t>(tot, 1));
+    DBuf<double> d_ovals;
+    if (with_vals) d_ovals.resize(std::max<int64_t>(tot, 1));
+    k::gather_rows(m, d_rows.get(), ptr, idx, vals, d_optr.get(), d_oidx.get(), with_vals ? d_ovals.get() : nullptr,
+                   s);
+    rs.idx.resize(tot);
+    d_oidx.download(rs.idx.data(), tot, s);
+    if (with_vals) {
+        rs.vals.resize(tot);
+        d_ovals.download(rs.vals.data(), tot, s);
+    }
+    sync(s);
+    return rs;
+}
+
+void HybridSolver::select_stiff(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s)
+{
+    std::vector<int32_t> S;
+    if (cfg_.select_bad_dofs_from_l1_norm) {
+        // GPUHybrid::select_bad_dofs, GPUHybrid.cu:737-883
+        d_norms_.resize(n);
+        k::row_l1(n, ptr, vals, d_norms_.get(), s);
+        DBuf<double> g0(n), g1(n), part(4 * k::kCsumChunks), sums(8);
+        const double *x = d_norms_.get();
+        k::canonical_sum(n, x, part.get(), sums.get(), s);
+        k::minmax(n, x, sums.get() + 1, s);
+        double h[8];
+        sums.download(h, 3, s);
+        sync(s);
+        const double global_mean = h[0] / n;
+        k::gmm_var_global(n, x, global_mean, part.get(), s);
+        k::csum_finish(1, part.get(), sums.get(), s);
+        const double global_var = d2h_scalar(sums.get(), s) / n;
+        double mean0 = h[1], mean1 = h[2], var0 = global_var, var1 = global_var, w0 = 0.5, w1 = 0.5;
+        const double var_reg = 1e-6;
+        int it;
+        for (it = 0; it < cfg_.max_gmm_iterations; ++it) {
+            k::GmmParams p;
+            p.mean0 = mean0;
+            p.mean1 = mean1;
+            p.var0 = var0;
+            p.var1 = var1;
+            p.log_w0 = std::log(w0);
+            p.log_w1 = std::log(w1);
+            p.c0 = -0.5 * std::log(2.0 * kPi * var0);
+            p.c1 = -0.5 * std::log(2.0 * kPi * var1);
+            k::gmm_estep(n, x, p, g0.get(), g1.get(), part.get(), s);
+            k::csum_finish(4, part.get(), sums.get(), s);
+            sums.download(h, 4, s);
+            sync(s);
+            const double sum_g0 = h[0], sum_g1 = h[1];
+            w0 = sum_g0 / n;
+            w1 = sum_g1 / n;
+            const double om0 = mean0, om1 = mean1, ov0 = var0, ov1 = var1;
+            mean0 = h[2] / sum_g0;
+            mean1 = h[3] / sum_g1;
+            k::gmm_var(n, x, g0.get(), g1.get(), mean0, mean1, part.get(), s);
+            k::csum_finish(2, part.get(), sums.get(), s);
+            sums.download(h, 2, s);
+            sync(s);
+            var0 = h[0] / sum_g0 + var_reg;
+            var1 = h[1] / sum_g1 + var_reg;
+            if (std::fabs(mean0 - om0) / std::fabs(om0) < cfg_.gmm_tol &&
+                std::fabs(mean1 - om1) / std::fabs(om1) < cfg_.gmm_tol &&
+                std::fabs(var0 - ov0) / std::fabs(ov0) < cfg_.gmm_tol &&
+                std::fabs(var1 - ov1) / std::fabs(ov1) < cfg_.gmm_tol)
+                break;
+        }
+        gmm_.global_mean = global_mean;
+        gmm_.global_var = global_var;
+        gmm_.mean0 = mean0;
+        gmm_.mean1 = mean1;
+        gmm_.var0 = var0;
+        gmm_.var1 = var1;
+        gmm_.w0 = w0;
+        gmm_.w1 = w1;
+        gmm_.iters = it;
+        gmm_.jumped = std::fabs(mean1) / std::fabs(mean0) > cfg_.gmm_jump_threshold;
+        if (gmm_.jumped) {
+            DBuf<uint8_t> fl(n);
+            k::gmm_flags(n, g0.get(), g1.get(), fl.get(), s);
+            auto hf = to_host(fl, n, s);
+            for (int32_t i = 0; i < n; ++i)
+                if (hf[i]) S.push_back(i);
+        }
+        gmm_.selected = int64_t(S.size());
+    } else {
+        for (int32_t d : user_set_)
+            if (d >= 0 && d < n) S.push_back(d);
+    }
+    n_selected_ = int64_t(S.size());
+    if (cfg_.decompose_subdomains && !S.empty()) {
+        RowSet rows = fetch_rows(S, ptr, idx, vals, true, s);
+        S = filter_subdomains(n, rows, S, cfg_.min_subdomain_size, cfg_.max_subdomain_size,
+                              cfg_.conditioning_threshold, filter_);
+    }
+    n_filtered_ = int64_t(S.size());
+    if (cfg_.expand_subdomains && !S.empty()) {
+        RowSet rows = fetch_rows(S, ptr, idx, nullptr, false, s);
+        S = expand_subdomains(n, rows, S);
+    }
+    n_expanded_ = int64_t(S.size());
+    comps_.clear();
+    if (cfg_.decompose_subdomains) {
+        if (!S.empty()) {
+            RowSet rows = fetch_rows(S, ptr, idx, nullptr, false, s);
+            comps_ = decompose_subdomains(n, rows, S, cfg_.max_subdomain_size);
+        }
+    } else if (!S.empty()) {
+        comps_.push_back(S);
+    }
+}
+
+void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, const int32_t *idx, const double *vals,
+                                  hipStream_t s)
+{
+    const int bb = bs_ * bs_;
+    // scalar rows -> block rows (original numbering)
+    DBuf<int32_t> cnt(nb_), flag(1);
+    flag.zero(s);
+    k::bsr_count(n, bs_, ptr, idx, cnt.get(), flag.get(), s);
+    auto hc = to_host(cnt, nb_, s);
+    int32_t hflag = 0;
+    flag.download(&hflag, 1, s);
+    sync(s);
+    if (hflag) fail(LSS_ERR_ARG, "row indices must be strictly ascending within each column");
+    Pattern g;
+    g.n = nb_;
+    g.rowptr.assign(nb_ + 1, 0);
+    int64_t acc = 0;
+    for (int32_t I = 0; I < nb_; ++I) {
+        acc += hc[I];
+        if (acc >= (int64_t(1) << 31)) fail(LSS_ERR_UNSUPPORTED, "more than 2^31 nonzero blocks");
+        g.rowptr[I + 1] = int32_t(acc);
+    }
+    nnzb_ = acc;
+    DBuf<int32_t> d_rp0, d_c0(std::max<int64_t>(nnzb_, 1));
+    DBuf<double> d_v0(std::max<int64_t>(nnzb_ * bb, 1));
+    d_rp0.upload(g.rowptr.data(), nb_ + 1, s);
+    d_v0.zero(s);
+    k::bsr_fill(n, bs_, ptr, idx, vals, d_rp0.get(), d_c0.get(), d_v0.get(), s);
+    g.cols.resize(nnzb_);
+    d_c0.download(g.cols.data(), nnzb_, s);
+    sync(s);
+
+    // partition (MAS banks) and permutation
+    auto tp = clk::now();
+    if (cfg_.base_preconditioner == "MAS") {
+        std::vector<int32_t> part_of;
+        int32_t nparts = 0;
+        partition_graph(g, 32, part_of, nparts);
+        partition_permutation(part_of, nparts, perm_, part_off_);
+    } else {
+        perm_.resize(nb_);
+        std::iota(perm_.begin(), perm_.end(), 0);
+        part_off_.clear();
+    }
+    Pattern gp = permute_pattern(g, perm_);
+    t_partition_ = since(tp);
+    d_perm_.upload(perm_.data(), nb_, s);
+    d_rowptr_.upload(gp.rowptr.data(), nb_ + 1, s);
+    d_cols_.resize(std::max<int64_t>(nnzb_, 1));
+    d_bvals_.resize(std::max<int64_t>(nnzb_ * bb, 1));
+    k::bsr_permute(nb_, bs_, d_rp0.get(), d_c0.get(), d_v0.get(), d_perm_.get(), d_rowptr_.get(), d_cols_.get(),
+                   d_bvals_.get(), s);
+
+    // sliced block-ELL copy for SpMV
+    const int H = 64 / bs_;
+    const int32_t nsl = (nb_ + H - 1) / H;
+    std::vector<int64_t> sp(nsl + 1, 0);
+    for (int32_t sl = 0; sl < nsl; ++sl) {
+        int32_t w = 0;
+        for (int32_t r = sl * H; r < std::min(nb_, (sl + 1) * H); ++r) w = std::max(w, gp.rowptr[r + 1] - gp.rowptr[r]);
+        sp[sl + 1] = sp[sl] + int64_t(w) * H;
+    }
+    sell_slots_ = sp[nsl];
+    d_slice_ptr_.upload(sp.data(), nsl + 1, s);
+    d_scols_.resize(std::max<int64_t>(sell_slots_, 1));
+    d_svals_.resize(std::max<int64_t>(sell_slots_ * bb, 1));
+    d_svals_.zero(s);
+    if (cfg_.fp32_inner()) {
+        d_svals32_.resize(std::max<int64_t>(sell_slots_ * bb, 1));
+        d_svals32_.zero(s);
+    }
+    k::sell_fill(nb_, bs_, d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), d_slice_ptr_.get(), d_scols_.get(),
+                 d_svals_.get(), cfg_.fp32_inner() ? d_svals32_.get() : nullptr, s);
+    sell_.nb = nb_;
+    sell_.bs = bs_;
+    sell_.nslices = nsl;
+    sell_.slice_ptr = d_slice_ptr_.get();
+    sell_.cols = d_scols_.get();
+    sell_.vals = d_svals_.get();
+    sell_.vals32 = cfg_.fp32_inner() ? d_svals32_.get() : nullptr;
+    sync(s);
+    // the permuted pattern is kept for the MAS hierarchy
+    level_nodes_.clear();
+    mas_.clear();
+    if (cfg_.base_preconditioner == "MAS") {
+        std::vector<MasLevel> levels;
+        build_mas_levels(gp, part_off_, kMaxCoarseLevels, levels);
+        mas_.resize(levels.size());
+        for (size_t l = 0; l < levels.size(); ++l) {
+            const MasLevel &h = levels[l];
+            DevLevel &d = mas_[l];
+            d.nodes = h.nodes;
+            d.nbanks = int32_t(h.bank_off.size()) - 1;
+            d.bank_off.upload(h.bank_off.data(), h.bank_off.size(), s);
+            if (l >= 1) {
+                d.rowptr.upload(h.op.rowptr.data(), h.op.rowptr.size(), s);
+                d.cols.upload(h.op.cols.data(), std::max<size_t>(h.op.cols.size(), 1), s);
+                const int64_t nslots = int64_t(h.op.cols.size());
+                d.vals.resize(std::max<int64_t>(nslots * bb, 1));
+                DBuf<int64_t> sptr;
+                DBuf<int32_t> sblk;
+                sptr.upload(h.src_ptr.data(), h.src_ptr.size(), s);
+                sblk.upload(h.src_blk.data(), std::max<size_t>(h.src_blk.size(), 1), s);
+                const double *fine_vals = (l == 1) ? d_bvals_.get() : mas_[l - 1].vals.get();
+                k::mas_galerkin(nslots, bs_, sptr.get(), sblk.get(), fine_vals, d.vals.get(), s);
+                d.child_ptr.upload(h.child_ptr.data(), h.child_ptr.size(), s);
+                d.child.upload(h.child.data(), std::max<size_t>(h.child.size(), 1), s);
+                d.r.resize(size_t(h.nodes) * bs_);
+                d.w.resize(size_t(h.nodes) * bs_);
+                sync(s);  // sptr / sblk are released at scope end
+            }
+            if (!h.parent.empty()) d.parent.upload(h.parent.data(), h.parent.size(), s);
+            level_nodes_.push_back(h.nodes);
+        }
+    }
+    sync(s);
+}
+
+void HybridSolver::build_mas(hipStream_t s)
+{
+    const int D = 32 * bs_;
+    const int64_t S = int64_t(D) * (D + 1) / 2;
+    const size_t eb = k::precision_bytes(prec());
+    d_status_.zero(s);
+    mas_banks_ = 0;
+    mas_bytes_ = 0;
+    for (size_t l = 0; l < mas_.size(); ++l) {
+        DevLevel &d = mas_[l];
+        d.inv.resize(std::max<size_t>(size_t(d.nbanks) * S * eb, 16));
+        const int32_t *rp = l == 0 ? d_rowptr_.get() : d.rowptr.get();
+        const int32_t *cl = l == 0 ? d_cols_.get() : d.cols.get();
+        const double *vl = l == 0 ? d_bvals_.get() : d.vals.get();
+        k::mas_bank_invert(prec(), bs_, d.nbanks, d.bank_off.get(), rp, cl, vl, d.inv.get(), d_status_.get(), s);
+        mas_banks_ += d.nbanks;
+        mas_bytes_ += double(d.nbanks) * S * eb;
+    }
+    int st = 0;
+    d_status_.download(&st, 1, s);
+    sync(s);
+    // level operators of the coarse levels are only needed to build the banks
+    for (size_t l = 1; l < mas_.size(); ++l) {
+        mas_[l].rowptr.release();
+        mas_[l].cols.release();
+        mas_[l].vals.release();
+    }
+    mas_levels_ = int(mas_.size());
+    if (st) fail(LSS_ERR_NOT_SPD, "MAS bank inverse failed (zero pivot or non-finite value)");
+}
+
+void HybridSolver::build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s)
+{
+    // factorize_submatrix, GPUHybrid.cu:993-1411, with a dense FP64 Cholesky per component
+    ncomp_ = int32_t(comps_.size());
+    std::vector<int64_t> off(ncomp_ + 1, 0), moff(ncomp_ + 1, 0);
+    max_comp_ = 0;
+    for (int32_t c = 0; c < ncomp_; ++c) {
+        const int64_t m = int64_t(comps_[c].size());
+        if (m > cfg_.max_dense_component)
+            fail(LSS_ERR_UNSUPPORTED, "stiff component of " + std::to_string(m) +
+                                          " DOFs exceeds max_dense_component (" +
+                                          std::to_string(cfg_.max_dense_component) + ")");
+        max_comp_ = std::max(max_comp_, m);
+        off[c + 1] = off[c] + m;
+        moff[c + 1] = moff[c] + m * m;
+    }
+    ns_ = off[ncomp_];
+    std::vector<int32_t> dofs(ns_), pd(ns_);
+    for (int32_t c = 0; c < ncomp_; ++c)
+        for (size_t r = 0; r < comps_[c].size(); ++r) {
+            const int32_t d = comps_[c][r];
+            dofs[off[c] + r] = d;
+            pd[off[c] + r] = perm_[d / bs_] * bs_ + d % bs_;
+        }
+    d_comp_off_.upload(off.data(), ncomp_ + 1, s);
+    d_mat_off_.upload(moff.data(), ncomp_ + 1, s);
+    d_pdofs_.upload(pd.data(), ns_, s);
+    DBuf<int32_t> d_dofs, g2l(n), comp_of(n);
+    d_dofs.upload(dofs.data(), ns_, s);
+    k::stiff_maps(ncomp_, n, d_comp_off_.get(), d_dofs.get(), g2l.get(), comp_of.get(), s);
+    d_L_.resize(moff[ncomp_]);
+    d_L_.zero(s);
+    k::stiff_extract(ncomp_, d_comp_off_.get(), d_mat_off_.get(), d_dofs.get(), g2l.get(), comp_of.get(), ptr, idx,
+                     vals, d_L_.get(), s);
+    d_status_.zero(s);
+    k::stiff_cholesky(ncomp_, d_comp_off_.get(), d_mat_off_.get(), d_L_.get(), d_status_.get(), s);
+    int st = 0;
+    d_status_.download(&st, 1, s);
+    sync(s);
+    if (st) fail(LSS_ERR_NOT_SPD, "stiff block Cholesky failed: a selected sub-block is not SPD");
+    d_swork_.resize(std::max<int64_t>(ns_, 1));
+}
+
+void HybridSolver::alloc_vectors()
+{
+    const size_t np = size_t(nb_) * bs_;
+    xp_.resize(np), bp_.resize(np), r_.resize(np), p_.resize(np), q_.resize(np), z_.resize(np), u_.resize(np);
+}
+
+void HybridSolver::estimate_omega(hipStream_t s)
+{
+    omega_ = 1.0;
+    if (!cfg_.do_hybrid_precond || ncomp_ == 0) return;
+    // lambda_max(B A) by power iteration in the A inner product; omega = damping / lambda
+    const int64_t np = int64_t(nb_) * bs_;
+    k::hash_vector(n_, r_.get(), s);
+    k::fill(np, p_.get(), 0.0, s);
+    k::permute_in(nb_, bs_, n_, d_perm_.get(), r_.get(), p_.get(), s);
+    DBuf<double> sc(3);
+    double lam = 1.0;
+    for (int it = 0; it < cfg_.power_iterations; ++it) {
+        k::spmv(sell_, p_.get(), q_.get(), nullptr, s);
+        base_apply(q_.get(), 1.0, nullptr, z_.get(), nullptr, nullptr, s);
+        k::dot(np, q_.get(), z_.get(), partials_.get(), &st_.get()->ticket[0], sc.get(), s);
+        k::dot(np, p_.get(), q_.get(), partials_.get(), &st_.get()->ticket[0], sc.get() + 1, s);
+        k::dot(np, z_.get(), z_.get(), partials_.get(), &st_.get()->ticket[0], sc.get() + 2, s);
+        double h[3];
+        sc.download(h, 3, s);
+        sync(s);
+        lam = h[0] / h[1];
+        k::copy(np, z_.get(), p_.get(), s);
+        k::scale(np, p_.get(), 1.0 / std::sqrt(h[2]), s);
+    }
+    if (!(lam > 0.0) || !std::isfinite(lam)) fail(LSS_ERR_NOT_SPD, "lambda_max(BA) estimate is not positive");
+    omega_ = cfg_.damping / lam;
+}
+
+// ------------------------------------------------------------------ preconditioner
+void HybridSolver::base_apply(const double *r, double scale, const double *add, double *z, dev::PcgState *st,
+                              const int *stop, hipStream_t s)
+{
+    const int64_t np = int64_t(nb_) * bs_;
+    if (cfg_.base_preconditioner != "MAS") {
+        k::jacobi_apply(np, d_dinv_.get(), r, scale, add, z, st ? r_.get() : nullptr, st, partials_.get(), stop, s);
+        return;
+    }
+    const int L = int(mas_.size()) - 1;
+    const double *src = r;
+    for (int l = 1; l <= L; ++l) {
+        k::mas_restrict(mas_[l].nodes, bs_, mas_[l].child_ptr.get(), mas_[l].child.get(), src, mas_[l].r.get(), stop,
+                        s);
+        src = mas_[l].r.get();
+    }
+    for (int l = L; l >= 1; --l) {
+        const bool top = (l == L);
+        k::mas_coarse_apply(prec(), bs_, mas_[l].nodes, mas_[l].inv.get(), mas_[l].r.get(),
+                            top ? nullptr : mas_[l].parent.get(), top ? nullptr : mas_[l + 1].w.get(),
+                            mas_[l].w.get(), stop, s);
+    }
+    const bool prof = cfg_.profile_kernels && st;
+    if (prof) prof_begin(2, s);
+    k::mas_level0_apply(prec(), bs_, mas_[0].nbanks, mas_[0].bank_off.get(), mas_[0].inv.get(), r,
+                        L >= 1 ? mas_[0].parent.get() : nullptr, L >= 1 ? mas_[1].w.get() : nullptr, scale, add, z,
+                        st ? r_.get() : nullptr, st, partials_.get(), stop, s);
+    if (prof) prof_end(s);
+}
+
+void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, const int *stop, hipStream_t s)
+{
+    // custom_mixed_precond_iter, GPUHybrid.cu:521-542
+    if (!cfg_.do_hybrid_precond || ncomp_ == 0) {
+        base_apply(r, 1.0, nullptr, z, st, stop, s);
+        return;
+    }
+    base_apply(r, omega_, nullptr, z, nullptr, stop, s);  // z1 = B r
+    // z2 = z1 + P (r - A z1)  (dss_precond_iter, GPUHybrid.cu:544-675)
+    k::stiff_correct(ncomp_, ns_, bs_, d_comp_off_.get(), d_mat_off_.get(), d_pdofs_.get(), d_L_.get(),
+                     d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z, d_swork_.get(), stop, s);
+    // u = r - A z2
+    const bool prof = cfg_.profile_kernels && st;
+    if (prof) prof_begin(1, s);
+    k::spmv_residual(sell_, z, r, u_.get(), stop, s, cfg_.fp32_inner());
+    if (prof) prof_end(s);
+    // z = z2 + B u
+    base_apply(u_.get(), omega_, z, z, st, stop, s);
+}
+
+// ------------------------------------------------------------------ PCG
+void HybridSolver::prof_begin(int kind, hipStream_t s)
+{
+    int *cnt = kind == 0 ? &prof_spmv_outer_n_ : (kind == 1 ? &prof_spmv_inner_n_ : &prof_mas0_n_);
+    if (*cnt >= 200) {
+        pending_skip_ = true;
+        return;
+    }
+    pending_skip_ = false;
+    ++*cnt;
+    if (ev_used_ + 2 > ev_pool_.size()) {
+        for (int q = 0; q < 64; ++q) {
+            hipEvent_t e;
+            LSS_HIP(hipEventCreate(&e));
+            ev_pool_.push_back(e);
+        }
+    }
+    LSS_HIP(hipEventRecord(ev_pool_[ev_used_], s));
+    ev_marks_.emplace_back(kind, int(ev_used_));
+    ev_used_ += 2;
+}
+
+void HybridSolver::prof_end(hipStream_t s)
+{
+    // closes the pair opened by the matching prof_begin (kinds never nest)
+    if (pending_skip_ || ev_marks_.empty()) return;
+    LSS_HIP(hipEventRecord(ev_pool_[ev_marks_.back().second + 1], s));
+}
+
+void HybridSolver::prof_collect()
+{
+    double sum[3] = {0, 0, 0};
+    int cnt[3] = {0, 0, 0};
+    for (auto &m : ev_marks_) {
+        float ms = 0;
+        LSS_HIP(hipEventElapsedTime(&ms, ev_pool_[m.second], ev_pool_[m.second + 1]));
+        sum[m.first] += ms;
+        cnt[m.first]++;
+    }
+    prof_spmv_outer_ms_ = cnt[0] ? sum[0] / cnt[0] : 0;
+    prof_spmv_inner_ms_ = cnt[1] ? sum[1] / cnt[1] : 0;
+    prof_mas0_ms_ = cnt[2] ? sum[2] / cnt[2] : 0;
+    prof_spmv_outer_n_ = cnt[0];
+    prof_spmv_inner_n_ = cnt[1];
+    prof_mas0_n_ = cnt[2];
+}
+
+void HybridSolver::run_pcg(hipStream_t s)
+{
+    // GPUHybrid::pcg_solve, GPUHybrid.cu:1413-1512, scalars on the device
+    const int64_t np = int64_t(nb_) * bs_;
+    dev::PcgState *st = st_.get();
+    DBuf<double> sc(1);
+    k::dot(np, bp_.get(), bp_.get(), partials_.get(), &st->ticket[0], sc.get(), s);
+    // st->ticket is zero only after the first use; reset the whole state below
+    const double bb = d2h_scalar(sc.get(), s);
+    if (!(bb > 0.0)) {
+        k::fill(np, xp_.get(), 0.0, s);
+        num_iterations_ = 0;
+        final_res_norm_ = 0.0;
+        stop_code_ = 0;
+        return;
+    }
+    std::memset(h_st_, 0, sizeof(dev::PcgState));
+    h_st_->bb = bb;
+    h_st_->rel_eps = cfg_.relative_tolerance * cfg_.relative_tolerance;
+    h_st_->abs_eps = cfg_.absolute_tolerance * cfg_.absolute_tolerance;
+    LSS_HIP(hipMemcpyAsync(st, h_st_, sizeof(dev::PcgState), hipMemcpyHostToDevice, s));
+    ev_marks_.clear();
+    ev_used_ = 0;
+    prof_spmv_outer_n_ = prof_spmv_inner_n_ = prof_mas0_n_ = 0;
+
+    k::spmv_residual(sell_, xp_.get(), bp_.get(), r_.get(), nullptr, s);  // r = b - A x
+    precond_apply(r_.get(), z_.get(), st, nullptr, s);                     // z = M r, gamma = r.z
+    k::copy(np, z_.get(), p_.get(), s);
+    int k = 0;
+    bool stopped = false;
+    while (k < cfg_.max_iter && !stopped) {
+        const int batch = std::min(cfg_.check_every, cfg_.max_iter - k);
+        for (int b = 0; b < batch; ++b, ++k) {
+            const bool prof = cfg_.profile_kernels;
+            if (prof) prof_begin(0, s);
+            k::spmv_pq(sell_, p_.get(), q_.get(), st, partials_.get(), k, s);
+            if (prof) prof_end(s);
+            k::pcg_update_xr(np, xp_.get(), r_.get(), p_.get(), q_.get(), st, partials_.get(), s);
+            precond_apply(r_.get(), z_.get(), st, &st->stop, s);
+            k::pcg_update_p(np, p_.get(), z_.get(), st, s);
+        }
+        LSS_HIP(hipMemcpyAsync(h_st_, st, sizeof(dev::PcgState), hipMemcpyDeviceToHost, s));
+        sync(s);
+        stopped = h_st_->stop != 0;
+    }
+    LSS_HIP(hipMemcpyAsync(h_st_, st, sizeof(dev::PcgState), hipMemcpyDeviceToHost, s));
+    sync(s);
+    num_iterations_ = h_st_->iters;
+    stop_code_ = h_st_->stop;
+    // true residual ||A x - b|| (GPUHybrid.cu:383-386)
+    k::spmv_residual(sell_, xp_.get(), bp_.get(), q_.get(), nullptr, s);
+    k::dot(np, q_.get(), q_.get(), partials_.get(), &st->ticket[0], sc.get(), s);
+    final_res_norm_ = std::sqrt(d2h_scalar(sc.get(), s));
+    if (cfg_.profile_kernels) prof_collect();
+}
+
+void HybridSolver::solve_device(const double *b, double *x, hipStream_t s)
+{
+    if (!factorized_) fail(LSS_ERR_NOT_FACTORIZED, "solve called before factorize");
+    if (!s) s = stream_;
+    auto t0 = clk::now();
+    const int64_t np = int64_t(nb_) * bs_;
+    if (np != n_) {
+        k::fill(np, bp_.get(), 0.0, s);
+        k::fill(np, xp_.get(), 0.0, s);
+    }
+    k::permute_in(nb_, bs_, n_, d_perm_.get(), b, bp_.get(), s);
+    k::permute_in(nb_, bs_, n_, d_perm_.get(), x, xp_.get(), s);
+    run_pcg(s);
+    k::permute_out(nb_, bs_, n_, d_perm_.get(), xp_.get(), x, s);
+    sync(s);
+    t_solve_ = since(t0);
+}
+
+void HybridSolver::solve_host(const double *b, double *x)
+{
+    DBuf<double> db, dx;
+    db.upload(b, n_, stream_);
+    dx.upload(x, n_, stream_);
+    solve_device(db.get(), dx.get(), stream_);
+    dx.download(x, n_, stream_);
+    sync(stream_);
+}
+
+void HybridSolver::solve_multi_device(int32_t k, const double *B, double *X, hipStream_t s)
+{
+    for (int32_t c = 0; c < k; ++c) solve_device(B + int64_t(c) * n_, X + int64_t(c) * n_, s);
+}
+
+// ------------------------------------------------------------------ inspection
+void HybridSolver::debug_row_norms(double *out) const
+{
+    if (!d_norms_) fail(LSS_ERR_NOT_FACTORIZED, "row norms are computed by factorize when stiff DOFs are selected");
+    d_norms_.download(out, n_, stream_);
+    LSS_HIP(hipStreamSynchronize(stream_));
+}
+
+void HybridSolver::debug_spmv(const double *x, double *y)
+{
+    if (!factorized_) fail(LSS_ERR_NOT_FACTORIZED, "not factorized");
+    DBuf<double> dx;
+    dx.upload(x, n_, stream_);
+    k::fill(int64_t(nb_) * bs_, xp_.get(), 0.0, stream_);
+    k::permute_in(nb_, bs_, n_, d_perm_.get(), dx.get(), xp_.get(), stream_);
+    k::spmv(sell_, xp_.get(), q_.get(), nullptr, stream_);
+    k::permute_out(nb_, bs_, n_, d_perm_.get(), q_.get(), dx.get(), stream_);
+    dx.download(y, n_, stream_);
+    sync(stream_);
+}
+
+void HybridSolver::debug_base(const double *r, double *z)
+{
+    if (!factorized_) fail(LSS_ERR_NOT_FACTORIZED, "not factorized");
+    DBuf<double> dr;
+    dr.upload(r, n_, stream_);
+    k::fill(int64_t(nb_) * bs_, bp_.get(), 0.0, stream_);
+    k::permute_in(nb_, bs_, n_, d_perm_.get(), dr.get(), bp_.get(), stream_);
+    base_apply(bp_.get(), 1.0, nullptr, z_.get(), nullptr, nullptr, stream_);
+    k::permute_out(nb_, bs_, n_, d_perm_.get(), z_.get(), dr.get(), stream_);
+    dr.download(z, n_, stream_);
+    sync(stream_);
+}
+
+void HybridSolver::debug_precond(const double *r, double *z)
+{
+    if (!factorized_) fail(LSS_ERR_NOT_FACTORIZED, "not factorized");
+    DBuf<double> dr;
+    dr.upload(r, n_, stream_);
+    k::fill(int64_t(nb_) * bs_, r_.get(), 0.0, stream_);
+    k::permute_in(nb_, bs_, n_, d_perm_.get(), dr.get(), r_.get(), stream_);
+    precond_apply(r_.get(), z_.get(), nullptr, nullptr, stream_);
+    k::permute_out(nb_, bs_, n_, d_perm_.get(), z_.get(), dr.get(), stream_);
+    dr.download(z, n_, stream_);
+    sync(stream_);
+}
+
+std::string HybridSolver::info_json() const
+{
+    static const char *stops[] = {"max_iter", "converged_rel", "converged_abs", "err_zero_sdotp",
+                                  "err_negative_alpha", "err_subnormal_alpha"};
+    int64_t nst = 0;
+    for (auto &c : comps_) nst += int64_t(c.size());
+    std::ostringstream o;
+    o.precision(17);
+    o << "{\"num_iterations\":" << num_iterations_ << ",\"final_res_norm\":" << final_res_norm_ << ",\"solver\":\""
+      << name() << "\",\"stop_reason\":\"" << stops[(stop_code_ >= 0 && stop_code_ < 6) ? stop_code_ : 0]
+      << "\",\"n\":" << n_ << ",\"nnz\":" << nnz_ << ",\"block_size\":" << bs_ << ",\"nnz_blocks\":" << nnzb_
+      << ",\"sell_slots\":" << sell_slots_ << ",\"precision\":\"" << cfg_.precision << "\",\"inner_spmv_fp32\":"
+      << (cfg_.fp32_inner() ? "true" : "false") << ",\"omega\":" << omega_ << ",\"timings\":{\"factorize\":"
+      << t_factorize_ << ",\"select\":" << t_select_ << ",\"operator\":" << t_operator_ << ",\"partition\":"
+      << t_partition_ << ",\"mas\":" << t_mas_ << ",\"stiff\":" << t_stiff_ << ",\"omega\":" << t_omega_
+      << ",\"solve\":" << t_solve_ << "},\"stiff\":{\"gmm\":{\"global_mean\":" << gmm_.global_mean
+      << ",\"global_var\":" << gmm_.global_var << ",\"mean0\":" << gmm_.mean0 << ",\"mean1\":" << gmm_.mean1
+      << ",\"var0\":" << gmm_.var0 << ",\"var1\":" << gmm_.var1 << ",\"w0\":" << gmm_.w0 << ",\"w1\":" << gmm_.w1
+      << ",\"iters\":" << gmm_.iters << ",\"jumped\":" << gmm_.jumped << ",\"selected\":" << gmm_.selected
+      << "},\"n_selected\":" << n_selected_ << ",\"n_filtered\":" << n_filtered_ << ",\"n_expanded\":" << n_expanded_
+      << ",\"filter\":{\"too_small\":" << filter_.too_small << ",\"too_large\":" << filter_.too_large
+      << ",\"not_poorly_conditioned\":" << filter_.not_poorly_conditioned << "},\"n_components\":" << comps_.size()
+      << ",\"n_stiff_dofs\":" << nst << ",\"max_component\":" << max_comp_ << "},\"mas\":{\"levels\":" << mas_levels_
+      << ",\"banks\":" << mas_banks_ << ",\"inverse_bytes\":" << mas_bytes_ << ",\"parts\":"
+      << (part_off_.empty() ? 0 : part_off_.size() - 1) << ",\"level_nodes\":[";
+    for (size_t l = 0; l < level_nodes_.size(); ++l) o << (l ? "," : "") << level_nodes_[l];
+    o << "]},\"kernel_ms\":{\"spmv_outer\":" << prof_spmv_outer_ms_ << ",\"spmv_outer_n\":" << prof_spmv_outer_n_
+      << ",\"spmv_inner\":" << prof_spmv_inner_ms_ << ",\"spmv_inner_n\":" << prof_spmv_inner_n_
+      << ",\"mas_level0\":" << prof_mas0_ms_ << ",\"mas_level0_n\":" << prof_mas0_n_ << "}}";
+    return o.str();
+}
+
+}  // namespace lss

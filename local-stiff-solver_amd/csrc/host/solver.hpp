@@ -1,0 +1,163 @@
+// HybridSolver: the MI355X implementation of GPUHybrid factorize()/solve()
+// (src/polysolve/linear/GPUHybrid.hpp:27-192) with B = MAS.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../common/runtime.hpp"
+#include "../kernels/kernels.hpp"
+#include "../kernels/mas_kernels.hpp"
+#include "config.hpp"
+#include "stiff_select.hpp"
+
+namespace lss {
+
+struct GmmResult {
+    double global_mean = 0, global_var = 0, mean0 = 0, mean1 = 0, var0 = 0, var1 = 0, w0 = 0, w1 = 0;
+    int iters = 0;
+    int jumped = 0;
+    int64_t selected = 0;
+};
+
+class HybridSolver {
+public:
+    explicit HybridSolver(const Config &cfg);
+    ~HybridSolver();
+    HybridSolver(const HybridSolver &) = delete;
+    HybridSolver &operator=(const HybridSolver &) = delete;
+
+    void set_config(const Config &cfg) { cfg_ = cfg; }
+    const Config &config() const { return cfg_; }
+    void set_problematic_dofs(const int32_t *dofs, int64_t n);
+
+    void factorize_host(int32_t n, int64_t nnz, const int32_t *colptr, const int32_t *rowidx, const double *vals);
+    void factorize_device(int32_t n, int64_t nnz, const int32_t *ptr, const int32_t *idx, const double *vals,
+                          hipStream_t s);
+    void solve_host(const double *b, double *x);
+    void solve_device(const double *b, double *x, hipStream_t s);
+    void solve_multi_device(int32_t k, const double *B, double *X, hipStream_t s);
+
+    std::string info_json() const;
+    const char *name() const;
+    bool factorized() const { return factorized_; }
+    int32_t size() const { return n_; }
+
+    // inspection
+    void debug_row_norms(double *out) const;
+    const GmmResult &gmm() const { return gmm_; }
+    const std::vector<int32_t> &perm() const { return perm_; }
+    const std::vector<int32_t> &part_off() const { return part_off_; }
+    const std::vector<std::vector<int32_t>> &components() const { return comps_; }
+    void debug_spmv(const double *x, double *y);
+    void debug_base(const double *r, double *z);
+    void debug_precond(const double *r, double *z);
+
+private:
+    struct DevLevel {
+        int32_t nodes = 0, nbanks = 0;
+        DBuf<int32_t> bank_off;
+        DBuf<uint8_t> inv;         // packed inverses, precision-dependent element type
+        DBuf<int32_t> rowptr, cols;  // level operator (levels >= 1)
+        DBuf<double> vals;
+        DBuf<int64_t> child_ptr;
+        DBuf<int32_t> child;
+        DBuf<int32_t> parent;      // to the next level
+        DBuf<double> r, w;         // restricted residual / prolongated correction
+    };
+
+    void release();
+    void select_stiff(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s);
+    RowSet fetch_rows(const std::vector<int32_t> &rows, const int32_t *ptr, const int32_t *idx, const double *vals,
+                      bool with_vals, hipStream_t s);
+    void build_operator(int32_t n, int64_t nnz, const int32_t *ptr, const int32_t *idx, const double *vals,
+                        hipStream_t s);
+    void build_mas(hipStream_t s);
+    void build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s);
+    void estimate_omega(hipStream_t s);
+    void alloc_vectors();
+
+    // preconditioner pieces on permuted vectors
+    void base_apply(const double *r, double scale, const double *add, double *z, dev::PcgState *st, const int *stop,
+                    hipStream_t s);
+    void precond_apply(const double *r, double *z, dev::PcgState *st, const int *stop, hipStream_t s);
+    void run_pcg(hipStream_t s);
+
+    k::Precision prec() const;
+    void sync(hipStream_t s) const { LSS_HIP(hipStreamSynchronize(s)); }
+
+    Config cfg_;
+    hipStream_t stream_ = nullptr;
+    bool factorized_ = false;
+    int32_t n_ = 0, nb_ = 0;
+    int bs_ = 1;
+    int64_t nnz_ = 0, nnzb_ = 0;
+
+    // stiff selection
+    std::vector<int32_t> user_set_;
+    bool user_set_given_ = false;
+    GmmResult gmm_;
+    FilterStats filter_;
+    int64_t n_selected_ = 0, n_filtered_ = 0, n_expanded_ = 0;
+    std::vector<std::vector<int32_t>> comps_;
+    DBuf<double> d_norms_;
+
+    // permutation / operator
+    std::vector<int32_t> perm_, part_off_;
+    DBuf<int32_t> d_perm_;
+    DBuf<int32_t> d_rowptr_, d_cols_;
+    DBuf<double> d_bvals_;
+    DBuf<int64_t> d_slice_ptr_;
+    DBuf<int32_t> d_scols_;
+    DBuf<double> d_svals_;
+    DBuf<float> d_svals32_;
+    k::SellView sell_;
+    int64_t sell_slots_ = 0;
+
+    // base preconditioner
+    std::vector<DevLevel> mas_;
+    std::vector<int32_t> level_nodes_;
+    DBuf<double> d_dinv_;
+
+    // stiff blocks
+    int32_t ncomp_ = 0;
+    int64_t ns_ = 0;
+    int64_t max_comp_ = 0;
+    DBuf<int64_t> d_comp_off_, d_mat_off_;
+    DBuf<int32_t> d_pdofs_;
+    DBuf<double> d_L_, d_swork_;
+    double omega_ = 1.0;
+
+    // PCG
+    DBuf<double> xp_, bp_, r_, p_, q_, z_, u_;
+    DBuf<double> partials_;
+    DBuf<dev::PcgState> st_;
+    dev::PcgState *h_st_ = nullptr;  // pinned mirror
+    DBuf<int> d_status_;
+
+    // info
+    int num_iterations_ = 0;
+    double final_res_norm_ = 0.0;
+    int stop_code_ = 0;
+    double t_factorize_ = 0, t_select_ = 0, t_operator_ = 0, t_partition_ = 0, t_mas_ = 0, t_stiff_ = 0,
+           t_omega_ = 0, t_solve_ = 0;
+    int mas_levels_ = 0;
+    int64_t mas_banks_ = 0;
+    double mas_bytes_ = 0;
+    // kernel timing (profile_kernels)
+    double prof_spmv_outer_ms_ = 0, prof_spmv_inner_ms_ = 0, prof_mas0_ms_ = 0;
+    int prof_spmv_outer_n_ = 0, prof_spmv_inner_n_ = 0, prof_mas0_n_ = 0;
+    std::vector<int32_t> h_ptr_;  // host copy of the row pointer during factorize
+    bool pending_skip_ = false;
+    std::vector<hipEvent_t> ev_pool_;
+    std::vector<std::pair<int, int>> ev_marks_;  // (kind, event index)
+    size_t ev_used_ = 0;
+    void prof_begin(int kind, hipStream_t s);
+    void prof_end(hipStream_t s);
+    void prof_collect();
+};
+
+}  // namespace lss

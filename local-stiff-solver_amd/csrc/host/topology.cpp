@@ -1,0 +1,229 @@
+#include "topology.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <stdexcept>
+
+namespace lss {
+
+void partition_graph(const Pattern &g, int max_part, std::vector<int32_t> &part_of, int32_t &nparts)
+{
+    const int32_t n = g.n;
+    part_of.assign(n, -1);
+    std::vector<int32_t> gain(n, 0);
+    std::vector<int32_t> frontier;
+    frontier.reserve(1024);
+    int32_t p = 0, seed = 0;
+    while (true) {
+        while (seed < n && part_of[seed] != -1) ++seed;
+        if (seed >= n) break;
+        frontier.clear();
+        int size = 0;
+        int32_t u = seed;
+        while (true) {
+            part_of[u] = p;
+            ++size;
+            for (int32_t e = g.rowptr[u]; e < g.rowptr[u + 1]; ++e) {
+                const int32_t w = g.cols[e];
+                if (w == u || part_of[w] != -1) continue;
+                if (gain[w] == 0) frontier.push_back(w);
+                ++gain[w];
+            }
+            if (size >= max_part) break;
+            int best = -1;
+            int32_t bw = 0, bg = 0;
+            for (int f = 0; f < int(frontier.size()); ++f) {
+                const int32_t w = frontier[f];
+                if (part_of[w] != -1) continue;
+                const int32_t gw = gain[w];
+                if (best < 0 || gw > bg || (gw == bg && w < bw)) {
+                    best = f;
+                    bw = w;
+                    bg = gw;
+                }
+            }
+            if (best < 0) break;
+            u = bw;
+        }
+        for (int32_t w : frontier) gain[w] = 0;
+        ++p;
+    }
+    nparts = p;
+}
+
+void partition_permutation(const std::vector<int32_t> &part_of, int32_t nparts, std::vector<int32_t> &perm,
+                           std::vector<int32_t> &part_off)
+{
+    const int32_t n = int32_t(part_of.size());
+    part_off.assign(nparts + 1, 0);
+    for (int32_t i = 0; i < n; ++i) part_off[part_of[i] + 1]++;
+    for (int32_t p = 0; p < nparts; ++p) part_off[p + 1] += part_off[p];
+    std::vector<int32_t> next(part_off.begin(), part_off.end() - 1);
+    perm.assign(n, 0);
+    for (int32_t i = 0; i < n; ++i) perm[i] = next[part_of[i]]++;
+}
+
+Pattern permute_pattern(const Pattern &g, const std::vector<int32_t> &perm)
+{
+    Pattern out;
+    out.n = g.n;
+    out.rowptr.assign(g.n + 1, 0);
+    for (int32_t i = 0; i < g.n; ++i) out.rowptr[perm[i] + 1] = g.rowptr[i + 1] - g.rowptr[i];
+    for (int32_t i = 0; i < g.n; ++i) out.rowptr[i + 1] += out.rowptr[i];
+    out.cols.resize(g.cols.size());
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int32_t i = 0; i < g.n; ++i) {
+        int32_t *d = out.cols.data() + out.rowptr[perm[i]];
+        int32_t k = 0;
+        for (int32_t e = g.rowptr[i]; e < g.rowptr[i + 1]; ++e) d[k++] = perm[g.cols[e]];
+        std::sort(d, d + k);
+    }
+    return out;
+}
+
+namespace {
+
+// Connected components of one bank (<= 32 nodes) given neighbour masks that
+// include the node itself; rank = order of the component's smallest member.
+int bank_components(int m, const uint32_t *nbr, int32_t *rank)
+{
+    uint32_t conn[32];
+    for (int l = 0; l < m; ++l) {
+        uint32_t c = nbr[l], vis = 1u << l, todo = c ^ vis;
+        while (todo) {
+            const int v = __builtin_ctz(todo);
+            c |= nbr[v];
+            vis |= 1u << v;
+            todo = c ^ vis;
+        }
+        conn[l] = c;
+    }
+    uint32_t leads = 0;
+    for (int l = 0; l < m; ++l)
+        if (__builtin_ctz(conn[l]) == l) leads |= 1u << l;
+    for (int l = 0; l < m; ++l) {
+        const int lead = __builtin_ctz(conn[l]);
+        rank[l] = __builtin_popcount(leads & ((1u << lead) - 1u));
+    }
+    return __builtin_popcount(leads);
+}
+
+// Bank-local components of a level: returns parent[] and the next level's node count.
+int32_t level_components(const Pattern &op, const std::vector<int32_t> &bank_off, std::vector<int32_t> &parent)
+{
+    const int32_t nbanks = int32_t(bank_off.size()) - 1;
+    std::vector<int32_t> count(nbanks);
+    parent.assign(op.n, -1);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int32_t b = 0; b < nbanks; ++b) {
+        const int32_t lo = bank_off[b], hi = bank_off[b + 1];
+        const int m = hi - lo;
+        if (m > 32) throw std::runtime_error("bank larger than 32 nodes");
+        uint32_t nbr[32];
+        for (int l = 0; l < m; ++l) {
+            uint32_t mask = 1u << l;
+            const int32_t v = lo + l;
+            for (int32_t e = op.rowptr[v]; e < op.rowptr[v + 1]; ++e) {
+                const int32_t w = op.cols[e];
+                if (w >= lo && w < hi) mask |= 1u << (w - lo);
+            }
+            nbr[l] = mask;
+        }
+        int32_t rank[32];
+        count[b] = bank_components(m, nbr, rank);
+        for (int l = 0; l < m; ++l) parent[lo + l] = rank[l];  // local rank for now
+    }
+    std::vector<int32_t> base(nbanks + 1, 0);
+    for (int32_t b = 0; b < nbanks; ++b) base[b + 1] = base[b] + count[b];
+#pragma omp parallel for schedule(static)
+    for (int32_t b = 0; b < nbanks; ++b)
+        for (int32_t v = bank_off[b]; v < bank_off[b + 1]; ++v) parent[v] += base[b];
+    return base[nbanks];
+}
+
+// Galerkin pattern R A R^T and per-slot source lists (fine slots in ascending order).
+void galerkin(const Pattern &fine, const std::vector<int32_t> &parent, int32_t coarse_n, MasLevel &out)
+{
+    // children
+    out.child_ptr.assign(coarse_n + 1, 0);
+    for (int32_t v = 0; v < fine.n; ++v) out.child_ptr[parent[v] + 1]++;
+    for (int32_t a = 0; a < coarse_n; ++a) out.child_ptr[a + 1] += out.child_ptr[a];
+    out.child.resize(fine.n);
+    {
+        std::vector<int64_t> fill(out.child_ptr.begin(), out.child_ptr.end() - 1);
+        for (int32_t v = 0; v < fine.n; ++v) out.child[fill[parent[v]]++] = v;
+    }
+    // per coarse row: (coarse col, fine slot) pairs sorted
+    std::vector<std::vector<int32_t>> rcols(coarse_n);
+    std::vector<std::vector<int32_t>> rsrc(coarse_n);
+    std::vector<std::vector<int32_t>> rcnt(coarse_n);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int32_t a = 0; a < coarse_n; ++a) {
+        std::vector<std::pair<int32_t, int32_t>> pr;
+        for (int64_t q = out.child_ptr[a]; q < out.child_ptr[a + 1]; ++q) {
+            const int32_t v = out.child[q];
+            for (int32_t e = fine.rowptr[v]; e < fine.rowptr[v + 1]; ++e) pr.emplace_back(parent[fine.cols[e]], e);
+        }
+        std::sort(pr.begin(), pr.end());
+        auto &c = rcols[a];
+        auto &s = rsrc[a];
+        auto &k = rcnt[a];
+        for (size_t t = 0; t < pr.size(); ++t) {
+            if (c.empty() || c.back() != pr[t].first) {
+                c.push_back(pr[t].first);
+                k.push_back(0);
+            }
+            ++k.back();
+            s.push_back(pr[t].second);
+        }
+    }
+    out.op.n = coarse_n;
+    out.op.rowptr.assign(coarse_n + 1, 0);
+    for (int32_t a = 0; a < coarse_n; ++a) out.op.rowptr[a + 1] = out.op.rowptr[a] + int32_t(rcols[a].size());
+    const int64_t nslots = out.op.rowptr[coarse_n];
+    out.op.cols.resize(nslots);
+    out.src_ptr.assign(nslots + 1, 0);
+    out.src_blk.resize(fine.cols.size());
+    // slot source counts, then prefix
+    for (int32_t a = 0; a < coarse_n; ++a)
+        for (size_t t = 0; t < rcols[a].size(); ++t) {
+            out.op.cols[out.op.rowptr[a] + t] = rcols[a][t];
+            out.src_ptr[out.op.rowptr[a] + t + 1] = rcnt[a][t];
+        }
+    for (int64_t t = 0; t < nslots; ++t) out.src_ptr[t + 1] += out.src_ptr[t];
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int32_t a = 0; a < coarse_n; ++a) {
+        const int64_t base = out.src_ptr[out.op.rowptr[a]];
+        std::copy(rsrc[a].begin(), rsrc[a].end(), out.src_blk.begin() + base);
+    }
+}
+
+}  // namespace
+
+void build_mas_levels(const Pattern &fine, const std::vector<int32_t> &part_off, int max_coarse,
+                      std::vector<MasLevel> &levels)
+{
+    levels.clear();
+    levels.emplace_back();
+    MasLevel &l0 = levels.back();
+    l0.nodes = fine.n;
+    l0.bank_off = part_off;
+    l0.op = fine;
+    for (int lv = 0; lv < max_coarse; ++lv) {
+        MasLevel &cur = levels.back();
+        std::vector<int32_t> parent;
+        const int32_t next_n = level_components(cur.op, cur.bank_off, parent);
+        // stop when nothing merges (MASPreconditioner.cu:449-454); level 0 always coarsens
+        if (lv > 0 && next_n == cur.nodes) break;
+        MasLevel nxt;
+        nxt.nodes = next_n;
+        galerkin(cur.op, parent, next_n, nxt);
+        const int32_t nb = (next_n + 31) / 32;
+        nxt.bank_off.resize(nb + 1);
+        for (int32_t b = 0; b <= nb; ++b) nxt.bank_off[b] = std::min<int32_t>(b * 32, next_n);
+        levels.back().parent = std::move(parent);
+        levels.push_back(std::move(nxt));
+    }
+}
+
+}  // namespace lss

@@ -1,0 +1,181 @@
+// BLAS-1 and PCG kernels with device-resident scalars.
+//
+// The reference keeps alpha/beta/gamma on the host and synchronizes on every
+// dot product (GPUHybrid.cu:1455-1511); here each reduction ends in the kernel
+// itself (deterministic last-workgroup sum) and the scalar update of
+// GPUHybrid.cu:1461-1507 runs there, so the PCG loop issues no host sync.
+#include "kernels.hpp"
+#include "../common/runtime.hpp"
+
+namespace lss::k {
+namespace {
+
+constexpr int kT = 256;
+
+inline int red_grid(int64_t n) { return int(std::min<int64_t>(std::max<int64_t>(div_up(n, kT * 4), 1), 4096)); }
+
+__global__ __launch_bounds__(kT) void k_dot(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
+                                            double *partials, unsigned *ticket, double *out)
+{
+    __shared__ double sh[kT / 64];
+    double v = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT) v += a[i] * b[i];
+    const double part = dev::block_sum<kT>(v, sh);
+    double total;
+    if (dev::grid_sum_last<kT>(part, partials, ticket, sh, total) && threadIdx.x == 0) *out = total;
+}
+
+__global__ __launch_bounds__(kT) void k_update_xr(int64_t n, double *__restrict__ x, double *__restrict__ r,
+                                                  const double *__restrict__ p, const double *__restrict__ q,
+                                                  dev::PcgState *st, double *partials)
+{
+    if (st->stop) return;
+    __shared__ double sh[kT / 64];
+    const double alpha = st->alpha;
+    double v = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT) {
+        x[i] = x[i] + alpha * p[i];
+        const double ri = r[i] + (-alpha) * q[i];
+        r[i] = ri;
+        v += ri * ri;
+    }
+    const double part = dev::block_sum<kT>(v, sh);
+    double total;
+    if (dev::grid_sum_last<kT>(part, partials, &st->ticket[0], sh, total) && threadIdx.x == 0) {
+        // GPUHybrid.cu:1484-1496
+        st->rr = total;
+        if (st->rel_eps > 0 && (total / st->bb) < st->rel_eps)
+            st->stop = dev::kConvergedRel;
+        else if (st->abs_eps > 0 && total < st->abs_eps)
+            st->stop = dev::kConvergedAbs;
+    }
+}
+
+__global__ __launch_bounds__(kT) void k_rz(int64_t n, const double *__restrict__ r, const double *__restrict__ z,
+                                           dev::PcgState *st, double *partials, int first)
+{
+    if (st->stop) return;
+    __shared__ double sh[kT / 64];
+    double v = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT) v += r[i] * z[i];
+    const double part = dev::block_sum<kT>(v, sh);
+    double total;
+    if (dev::grid_sum_last<kT>(part, partials, &st->ticket[0], sh, total) && threadIdx.x == 0) {
+        // GPUHybrid.cu:1450-1451 (first) and 1501-1504
+        st->gamma = total;
+        if (first) {
+            st->gamma_old = total;
+        } else {
+            st->beta = total / st->gamma_old;
+            st->gamma_old = total;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kT) void k_update_p(int64_t n, double *__restrict__ p, const double *__restrict__ z,
+                                                 const dev::PcgState *st)
+{
+    if (st->stop) return;
+    const double beta = st->beta;
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT)
+        p[i] = beta * p[i] + z[i];
+}
+
+__global__ void k_copy(int64_t n, const double *__restrict__ a, double *__restrict__ b)
+{
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT) b[i] = a[i];
+}
+__global__ void k_fill(int64_t n, double *__restrict__ a, double v)
+{
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT) a[i] = v;
+}
+__global__ void k_scale(int64_t n, double *__restrict__ a, double v)
+{
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT) a[i] *= v;
+}
+
+__global__ void k_permute_in(int32_t nb, int bs, int64_t n, const int32_t *__restrict__ perm,
+                             const double *__restrict__ src, double *__restrict__ dst)
+{
+    const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
+    if (i >= int64_t(nb) * bs) return;
+    const int32_t old = int32_t(i / bs);
+    const int c = int(i - int64_t(old) * bs);
+    dst[int64_t(perm[old]) * bs + c] = i < n ? src[i] : 0.0;
+}
+
+__global__ void k_permute_out(int32_t nb, int bs, int64_t n, const int32_t *__restrict__ perm,
+                              const double *__restrict__ src, double *__restrict__ dst)
+{
+    const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
+    if (i >= n) return;
+    const int32_t old = int32_t(i / bs);
+    const int c = int(i - int64_t(old) * bs);
+    dst[i] = src[int64_t(perm[old]) * bs + c];
+}
+
+inline int ew_grid(int64_t n) { return int(std::min<int64_t>(std::max<int64_t>(div_up(n, kT * 4), 1), 65536)); }
+
+}  // namespace
+
+void dot(int64_t n, const double *a, const double *b, double *partials, unsigned *ticket, double *out, hipStream_t s)
+{
+    k_dot<<<red_grid(n), kT, 0, s>>>(n, a, b, partials, ticket, out);
+    LSS_LAUNCH_CHECK();
+}
+
+void pcg_update_xr(int64_t n, double *x, double *r, const double *p, const double *q, dev::PcgState *st,
+                   double *partials, hipStream_t s)
+{
+    k_update_xr<<<red_grid(n), kT, 0, s>>>(n, x, r, p, q, st, partials);
+    LSS_LAUNCH_CHECK();
+}
+
+void pcg_rz(int64_t n, const double *r, const double *z, dev::PcgState *st, double *partials, bool first,
+            hipStream_t s)
+{
+    k_rz<<<red_grid(n), kT, 0, s>>>(n, r, z, st, partials, first ? 1 : 0);
+    LSS_LAUNCH_CHECK();
+}
+
+void pcg_update_p(int64_t n, double *p, const double *z, const dev::PcgState *st, hipStream_t s)
+{
+    k_update_p<<<ew_grid(n), kT, 0, s>>>(n, p, z, st);
+    LSS_LAUNCH_CHECK();
+}
+
+void copy(int64_t n, const double *a, double *b, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_copy<<<ew_grid(n), kT, 0, s>>>(n, a, b);
+    LSS_LAUNCH_CHECK();
+}
+void fill(int64_t n, double *a, double v, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_fill<<<ew_grid(n), kT, 0, s>>>(n, a, v);
+    LSS_LAUNCH_CHECK();
+}
+void scale(int64_t n, double *a, double v, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_scale<<<ew_grid(n), kT, 0, s>>>(n, a, v);
+    LSS_LAUNCH_CHECK();
+}
+
+void permute_in(int32_t nb, int bs, int64_t n, const int32_t *perm, const double *src, double *dst, hipStream_t s)
+{
+    const int64_t m = int64_t(nb) * bs;
+    if (m == 0) return;
+    k_permute_in<<<div_up(m, kT), kT, 0, s>>>(nb, bs, n, perm, src, dst);
+    LSS_LAUNCH_CHECK();
+}
+
+void permute_out(int32_t nb, int bs, int64_t n, const int32_t *perm, const double *src, double *dst, hipStream_t s)
+{
+    if (n == 0) return;
+    k_permute_out<<<div_up(n, kT), kT, 0, s>>>(nb, bs, n, perm, src, dst);
+    LSS_LAUNCH_CHECK();
+}
+
+}  // namespace lss::k

@@ -1,0 +1,84 @@
+// Launchers of the hand-written gfx950 kernels.  All take an explicit stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_utils.hpp"
+
+namespace lss::k {
+
+// ---------------------------------------------------------------- formats
+// Scalar CSR (operator rows) -> block rows: number of distinct block columns per block row.
+void bsr_count(int32_t n, int bs, const int32_t *ptr, const int32_t *idx, int32_t *cnt, int32_t *unsorted_flag,
+               hipStream_t s);
+// Fill block rows (block-major, row-major inside a block).  Missing tail scalars of
+// the last block row get a unit diagonal (reference BSRMatrix.cu:123-134, 189-191).
+void bsr_fill(int32_t n, int bs, const int32_t *ptr, const int32_t *idx, const double *vals, const int32_t *rowptr,
+              int32_t *cols, double *bvals, hipStream_t s);
+// Relabel block rows/cols: new row perm[I] <- old row I, columns mapped and sorted.
+void bsr_permute(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols, const double *vals,
+                 const int32_t *perm, const int32_t *new_rowptr, int32_t *new_cols, double *new_vals, hipStream_t s);
+// Sliced block-ELL layout (one wave per slice of 64/bs block rows).
+void sell_width(int32_t nb, int bs, const int32_t *rowptr, int32_t *slice_width, hipStream_t s);
+void sell_fill(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols, const double *vals,
+               const int64_t *slice_ptr, int32_t *scols, double *svals, float *svals32, hipStream_t s);
+
+// ---------------------------------------------------------------- SpMV
+struct SellView {
+    int32_t nb = 0;
+    int bs = 1;
+    int32_t nslices = 0;
+    const int64_t *slice_ptr = nullptr;  // nslices + 1, in slots (slot = one block)
+    const int32_t *cols = nullptr;       // per slot, -1 = padding
+    const double *vals = nullptr;        // per slot bs*bs
+    const float *vals32 = nullptr;       // optional FP32 copy
+};
+
+// y = A x (sub == nullptr) or y = sub - A x.  If dot_w != nullptr a
+// deterministic grid sum of dot_w . y goes to *dot_out via the ticket reduction.
+// If stop != nullptr and *stop != 0 the kernel does nothing.
+void spmv(const SellView &A, const double *x, double *y, const double *sub, hipStream_t s, bool fp32 = false);
+void spmv_pq(const SellView &A, const double *p, double *q, dev::PcgState *st, double *partials, int iter,
+             hipStream_t s);
+void spmv_residual(const SellView &A, const double *x, const double *b, double *r, const int *stop, hipStream_t s,
+                   bool fp32 = false);
+
+// ---------------------------------------------------------------- BLAS-1 / PCG
+constexpr int kCsumChunks = 1024;  // canonical reduction tree (shared with the oracle)
+constexpr int kCsumLanes = 256;
+// Canonical-order sums used by the stiff DOF selection.
+void canonical_sum(int64_t n, const double *v, double *partials, double *out, hipStream_t s);
+
+void dot(int64_t n, const double *a, const double *b, double *partials, unsigned *ticket, double *out,
+         hipStream_t s);
+// x += alpha p; r -= alpha q; rr = r.r; convergence test.
+void pcg_update_xr(int64_t n, double *x, double *r, const double *p, const double *q, dev::PcgState *st,
+                   double *partials, hipStream_t s);
+// gamma = r.z; beta = gamma / gamma_old; p = beta p + z.
+void pcg_rz(int64_t n, const double *r, const double *z, dev::PcgState *st, double *partials, bool first,
+            hipStream_t s);
+void pcg_update_p(int64_t n, double *p, const double *z, const dev::PcgState *st, hipStream_t s);
+void copy(int64_t n, const double *a, double *b, hipStream_t s);
+void fill(int64_t n, double *a, double v, hipStream_t s);
+void scale(int64_t n, double *a, double v, hipStream_t s);
+// permuted[i] = src[perm-mapped], for node permutation with block size bs
+void permute_in(int32_t nb, int bs, int64_t n, const int32_t *perm, const double *src, double *dst, hipStream_t s);
+void permute_out(int32_t nb, int bs, int64_t n, const int32_t *perm, const double *src, double *dst, hipStream_t s);
+
+// ---------------------------------------------------------------- stiff DOF selection
+void row_l1(int32_t n, const int32_t *ptr, const double *vals, double *out, hipStream_t s);
+void minmax(int64_t n, const double *v, double *out2, hipStream_t s);
+struct GmmParams {
+    double mean0, mean1, var0, var1, log_w0, log_w1, c0, c1;
+};
+// E-step: gamma0/gamma1 + canonical partial sums of {g0, g1, g0 x, g1 x}.
+void gmm_estep(int64_t n, const double *x, GmmParams p, double *g0, double *g1, double *partials4, hipStream_t s);
+// canonical partial sums of {g0 (x-m0)^2, g1 (x-m1)^2}
+void gmm_var(int64_t n, const double *x, const double *g0, const double *g1, double m0, double m1,
+             double *partials2, hipStream_t s);
+void gmm_var_global(int64_t n, const double *x, double mean, double *partials1, hipStream_t s);
+void csum_finish(int nsums, const double *partials, double *out, hipStream_t s);
+void gmm_flags(int64_t n, const double *g0, const double *g1, uint8_t *flags, hipStream_t s);
+
+}  // namespace lss::k

@@ -1,0 +1,410 @@
+// Multilevel additive Schwarz (MAS) preconditioner on gfx950.
+//
+// Semantics follow MAS/MASPreconditioner.cu:33-1164: level-0 banks are the
+// graph-partition parts (<= 32 nodes); coarse level l+1 nodes are the bank-local
+// connected components of level l; every bank matrix (dim 32*bs, zero diagonal
+// entries set to 1) is inverted and stored packed-upper; the apply is
+//     z = sum_l R_l^T  A_{l,bank}^{-1}  R_l r.
+// Layout and schedule are this build's own:
+//  * parts are contiguous ranges of the partition-permuted numbering, so the
+//    level-0 gather/scatter of the reference (padded copy, 593-659 / 745-777)
+//    disappears: a bank reads and writes one contiguous slice of r and z;
+//  * coarse operators are the Galerkin products A_{l+1} = R A_l R^T, summed in
+//    a fixed source order (no FP atomics, reference 494-591 uses atomicAdd);
+//  * one workgroup assembles AND inverts a bank in LDS (reference: separate
+//    fill, pad and invert kernels, 939-979);
+//  * the apply runs restriction level by level, the coarse banks from the
+//    coarsest level down (each adding its parent's prolongated correction), and
+//    finally the level-0 banks, whose epilogue adds the coarse correction,
+//    scales, optionally adds a vector and reduces r.z.
+#include "mas_kernels.hpp"
+#include "../common/runtime.hpp"
+
+namespace lss::k {
+namespace {
+
+__device__ __forceinline__ int packed_index(int D, int i, int j)
+{
+    // row-major packed upper triangle, i <= j (MASPreconditioner.cu:483-491)
+    return i * D - (i * (i + 1)) / 2 + j;
+}
+
+template <class T>
+__device__ __forceinline__ T to_store(double v);
+template <>
+__device__ __forceinline__ double to_store<double>(double v) { return v; }
+template <>
+__device__ __forceinline__ float to_store<float>(double v) { return static_cast<float>(v); }
+template <>
+__device__ __forceinline__ __hip_bfloat16 to_store<__hip_bfloat16>(double v)
+{
+    return __float2bfloat16(static_cast<float>(v));
+}
+
+template <class T>
+__device__ __forceinline__ float to_lds(T v);
+template <>
+__device__ __forceinline__ float to_lds<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ float to_lds<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
+
+constexpr int kInvThreads = 256;
+
+// Assemble bank matrices of one level from its block operator and invert them.
+template <int BS, class TS>
+__global__ __launch_bounds__(kInvThreads) void k_bank_invert(int32_t nbanks, const int32_t *__restrict__ bank_off,
+                                                             const int32_t *__restrict__ rowptr,
+                                                             const int32_t *__restrict__ cols,
+                                                             const double *__restrict__ vals, TS *__restrict__ out,
+                                                             int *__restrict__ status)
+{
+    constexpr int D = 32 * BS;
+    constexpr int S = D * (D + 1) / 2;
+    constexpr int BB = BS * BS;
+    extern __shared__ double lds[];
+    double *M = lds;          // D x D row-major
+    double *colk = lds + D * D;
+    __shared__ double inv_piv;
+    const int32_t bank = blockIdx.x;
+    if (bank >= nbanks) return;
+    const int32_t lo = bank_off[bank], hi = bank_off[bank + 1];
+    for (int t = threadIdx.x; t < D * D; t += kInvThreads) M[t] = 0.0;
+    __syncthreads();
+    // assemble: block (i, J) with both nodes in the bank
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int32_t i = lo + wid; i < hi; i += kInvThreads / 64) {
+        for (int32_t k = rowptr[i] + lane; k < rowptr[i + 1]; k += 64) {
+            const int32_t J = cols[k];
+            if (J < lo || J >= hi) continue;
+            const int r0 = (i - lo) * BS, c0 = (J - lo) * BS;
+#pragma unroll
+            for (int a = 0; a < BS; ++a)
+#pragma unroll
+                for (int b = 0; b < BS; ++b) M[(r0 + a) * D + c0 + b] = vals[int64_t(k) * BB + a * BS + b];
+        }
+    }
+    __syncthreads();
+    // zero diagonal -> 1 (MASPreconditioner.cu:779-796): padding nodes and empty rows
+    for (int d = threadIdx.x; d < D; d += kInvThreads)
+        if (M[d * D + d] == 0.0) M[d * D + d] = 1.0;
+    __syncthreads();
+    // in-place Gauss-Jordan inverse without pivoting (SPD)
+    bool ok = true;
+    for (int k = 0; k < D; ++k) {
+        if (threadIdx.x == 0) {
+            const double piv = M[k * D + k];
+            if (!(piv != 0.0) || !isfinite(piv)) ok = false;
+            inv_piv = 1.0 / piv;
+        }
+        __syncthreads();
+        const double ip = inv_piv;
+        for (int t = threadIdx.x; t < D; t += kInvThreads) {
+            colk[t] = M[t * D + k];
+            if (t != k) M[k * D + t] *= ip;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < D * D; t += kInvThreads) {
+            const int i = t / D, j = t - i * D;
+            if (i == k) continue;
+            if (j == k)
+                M[t] = -colk[i] * ip;
+            else
+                M[t] -= colk[i] * M[k * D + j];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) M[k * D + k] = ip;
+        __syncthreads();
+    }
+    // packed upper, symmetrized
+    TS *dst = out + int64_t(bank) * S;
+    for (int t = threadIdx.x; t < S; t += kInvThreads) {
+        // invert packed index: row i with i*D - i(i-1)/2 <= t
+        int i = int((2.0 * D + 1.0 - sqrt((2.0 * D + 1.0) * (2.0 * D + 1.0) - 8.0 * t)) * 0.5);
+        while (i > 0 && packed_index(D, i, i) > t) --i;
+        while (i + 1 < D && packed_index(D, i + 1, i + 1) <= t) ++i;
+        const int j = t - packed_index(D, i, i) + i;
+        const double v = 0.5 * (M[i * D + j] + M[j * D + i]);
+        if (!isfinite(v)) ok = false;
+        dst[t] = to_store<TS>(v);
+    }
+    if (!ok) atomicOr(status, 1);
+}
+
+// Galerkin coarse operator values: target slot t sums its source blocks in order.
+__global__ void k_galerkin(int64_t nslots, int bb, const int64_t *__restrict__ src_ptr,
+                           const int32_t *__restrict__ src_blk, const double *__restrict__ fine_vals,
+                           double *__restrict__ coarse_vals)
+{
+    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= nslots * bb) return;
+    const int64_t t = g / bb;
+    const int e = int(g - t * bb);
+    double s = 0.0;
+    for (int64_t q = src_ptr[t]; q < src_ptr[t + 1]; ++q) s += fine_vals[int64_t(src_blk[q]) * bb + e];
+    coarse_vals[g] = s;
+}
+
+// r_l[a] = sum of children (level l-1 nodes) in index order.
+__global__ void k_restrict(int32_t nodes, int bs, const int64_t *__restrict__ child_ptr,
+                           const int32_t *__restrict__ child, const double *__restrict__ fine,
+                           double *__restrict__ coarse, const int *__restrict__ stop)
+{
+    if (stop && *stop) return;
+    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= int64_t(nodes) * bs) return;
+    const int32_t a = int32_t(g / bs);
+    const int c = int(g - int64_t(a) * bs);
+    double s = 0.0;
+    for (int64_t q = child_ptr[a]; q < child_ptr[a + 1]; ++q) s += fine[int64_t(child[q]) * bs + c];
+    coarse[g] = s;
+}
+
+// Packed symmetric matvec out of LDS: thread i (< D) returns row i of M x.
+template <int D, class TL>
+__device__ __forceinline__ double sym_row(const TL *__restrict__ A, const double *__restrict__ x, int i)
+{
+    double acc = 0.0;
+    // j < i: column i of the upper triangle, consecutive across threads
+    for (int j = 0; j < i; ++j) acc += double(A[packed_index(D, j, j) + (i - j)]) * x[j];
+    // j >= i: row i of the upper triangle, contiguous
+    const TL *row = A + packed_index(D, i, i) - i;
+    for (int j = i; j < D; ++j) acc += double(row[j]) * x[j];
+    return acc;
+}
+
+template <class TS>
+struct LdsType {
+    using type = double;
+};
+template <>
+struct LdsType<float> {
+    using type = float;
+};
+template <>
+struct LdsType<__hip_bfloat16> {
+    using type = float;
+};
+
+template <class TS, class TL>
+__device__ __forceinline__ void stage_matrix(const TS *__restrict__ src, TL *__restrict__ dst, int S, int nthreads)
+{
+    if constexpr (std::is_same<TS, double>::value) {
+        const double2 *s2 = reinterpret_cast<const double2 *>(src);
+        double2 *d2 = reinterpret_cast<double2 *>(dst);
+        for (int t = threadIdx.x; t < S / 2; t += nthreads) d2[t] = s2[t];
+        if ((S & 1) && threadIdx.x == 0) dst[S - 1] = src[S - 1];
+    } else if constexpr (std::is_same<TS, float>::value) {
+        const float4 *s4 = reinterpret_cast<const float4 *>(src);
+        float4 *d4 = reinterpret_cast<float4 *>(dst);
+        for (int t = threadIdx.x; t < S / 4; t += nthreads) d4[t] = s4[t];
+        for (int t = (S / 4) * 4 + threadIdx.x; t < S; t += nthreads) dst[t] = src[t];
+    } else {
+        for (int t = threadIdx.x; t < S; t += nthreads) dst[t] = to_lds<TS>(src[t]);
+    }
+}
+
+constexpr int kApplyThreads = 128;
+
+// Coarse level: y = A^{-1} r per bank of 32 nodes; w = y + w_parent[parent].
+template <int BS, class TS>
+__global__ __launch_bounds__(kApplyThreads) void k_coarse_apply(int32_t nodes, const TS *__restrict__ inv,
+                                                                const double *__restrict__ r,
+                                                                const int32_t *__restrict__ parent,
+                                                                const double *__restrict__ w_parent,
+                                                                double *__restrict__ w, const int *__restrict__ stop)
+{
+    constexpr int D = 32 * BS;
+    constexpr int S = D * (D + 1) / 2;
+    using TL = typename LdsType<TS>::type;
+    if (stop && *stop) return;
+    __shared__ __attribute__((aligned(16))) TL A[S + 4];
+    __shared__ double x[D];
+    const int32_t bank = blockIdx.x;
+    const int32_t lo = bank * 32;
+    const int32_t cnt = min(32, nodes - lo);
+    stage_matrix<TS, TL>(inv + int64_t(bank) * S, A, S, kApplyThreads);
+    for (int t = threadIdx.x; t < D; t += kApplyThreads) x[t] = (t < cnt * BS) ? r[int64_t(lo) * BS + t] : 0.0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt * BS; i += kApplyThreads) {
+        double y = sym_row<D, TL>(A, x, i);
+        if (w_parent) {
+            const int32_t node = lo + i / BS;
+            y += w_parent[int64_t(parent[node]) * BS + (i % BS)];
+        }
+        w[int64_t(lo) * BS + i] = y;
+    }
+}
+
+// Level 0: z = scale * (A_bank^{-1} r_bank + w1[parent]) (+ add); optional r.z reduction.
+template <int BS, class TS, bool DOT>
+__global__ __launch_bounds__(kApplyThreads) void k_level0_apply(int32_t nbanks, const int32_t *__restrict__ part_off,
+                                                                const TS *__restrict__ inv,
+                                                                const double *__restrict__ r,
+                                                                const int32_t *__restrict__ parent,
+                                                                const double *__restrict__ w1, double scale,
+                                                                const double *__restrict__ add, double *__restrict__ z,
+                                                                const double *__restrict__ dotv, dev::PcgState *st,
+                                                                double *partials, const int *__restrict__ stop)
+{
+    constexpr int D = 32 * BS;
+    constexpr int S = D * (D + 1) / 2;
+    using TL = typename LdsType<TS>::type;
+    if (stop && *stop) return;
+    __shared__ __attribute__((aligned(16))) TL A[S + 4];
+    __shared__ double x[D];
+    const int32_t bank = blockIdx.x;
+    const int32_t lo = part_off[bank];
+    const int32_t cnt = part_off[bank + 1] - lo;
+    stage_matrix<TS, TL>(inv + int64_t(bank) * S, A, S, kApplyThreads);
+    for (int t = threadIdx.x; t < D; t += kApplyThreads) x[t] = (t < cnt * BS) ? r[int64_t(lo) * BS + t] : 0.0;
+    __syncthreads();
+    double part = 0.0;
+    for (int i = threadIdx.x; i < cnt * BS; i += kApplyThreads) {
+        double y = sym_row<D, TL>(A, x, i);
+        const int64_t gi = int64_t(lo) * BS + i;
+        if (w1) y += w1[int64_t(parent[lo + i / BS]) * BS + (i % BS)];
+        y *= scale;
+        if (add) y += add[gi];
+        z[gi] = y;
+        if constexpr (DOT) part += dotv[gi] * y;
+    }
+    if constexpr (DOT) {
+        __shared__ double sh[kApplyThreads / 64];
+        const double p = dev::block_sum<kApplyThreads>(part, sh);
+        double total;
+        if (dev::grid_sum_last<kApplyThreads>(p, partials, &st->ticket[0], sh, total) && threadIdx.x == 0) {
+            // gamma = r . z; beta = gamma / gamma_old (GPUHybrid.cu:1501-1504)
+            st->gamma = total;
+            st->beta = total / st->gamma_old;
+            st->gamma_old = total;
+        }
+    }
+}
+
+template <class TS>
+void launch_bank_invert(int bs, int32_t nbanks, const int32_t *bank_off, const int32_t *rowptr, const int32_t *cols,
+                        const double *vals, TS *out, int *status, hipStream_t s)
+{
+    if (nbanks == 0) return;
+    const int D = 32 * bs;
+    const size_t lds = size_t(D * D + D) * sizeof(double);
+    switch (bs) {
+        case 1:
+            k_bank_invert<1, TS><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out, status);
+            break;
+        case 2:
+            k_bank_invert<2, TS><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out, status);
+            break;
+        case 3:
+            k_bank_invert<3, TS><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out, status);
+            break;
+        default: fail(LSS_ERR_UNSUPPORTED, "block_size must be 1, 2 or 3");
+    }
+    LSS_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void mas_set_lds_limits()
+{
+    static bool done = false;
+    if (done) return;
+    const size_t lds3 = size_t(96 * 96 + 96) * sizeof(double);
+    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_invert<3, double>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
+    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_invert<3, float>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
+    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_invert<3, __hip_bfloat16>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
+    done = true;
+}
+
+void mas_bank_invert(Precision prec, int bs, int32_t nbanks, const int32_t *bank_off, const int32_t *rowptr,
+                     const int32_t *cols, const double *vals, void *out, int *status, hipStream_t s)
+{
+    mas_set_lds_limits();
+    switch (prec) {
+        case Precision::FP64:
+            launch_bank_invert<double>(bs, nbanks, bank_off, rowptr, cols, vals, static_cast<double *>(out), status, s);
+            break;
+        case Precision::FP32:
+            launch_bank_invert<float>(bs, nbanks, bank_off, rowptr, cols, vals, static_cast<float *>(out), status, s);
+            break;
+        case Precision::BF16:
+            launch_bank_invert<__hip_bfloat16>(bs, nbanks, bank_off, rowptr, cols, vals,
+                                               static_cast<__hip_bfloat16 *>(out), status, s);
+            break;
+    }
+}
+
+void mas_galerkin(int64_t nslots, int bs, const int64_t *src_ptr, const int32_t *src_blk, const double *fine_vals,
+                  double *coarse_vals, hipStream_t s)
+{
+    const int64_t work = nslots * bs * bs;
+    if (work == 0) return;
+    k_galerkin<<<div_up(work, 256), 256, 0, s>>>(nslots, bs * bs, src_ptr, src_blk, fine_vals, coarse_vals);
+    LSS_LAUNCH_CHECK();
+}
+
+void mas_restrict(int32_t nodes, int bs, const int64_t *child_ptr, const int32_t *child, const double *fine,
+                  double *coarse, const int *stop, hipStream_t s)
+{
+    const int64_t work = int64_t(nodes) * bs;
+    if (work == 0) return;
+    k_restrict<<<div_up(work, 256), 256, 0, s>>>(nodes, bs, child_ptr, child, fine, coarse, stop);
+    LSS_LAUNCH_CHECK();
+}
+
+void mas_coarse_apply(Precision prec, int bs, int32_t nodes, const void *inv, const double *r, const int32_t *parent,
+                      const double *w_parent, double *w, const int *stop, hipStream_t s)
+{
+    const int32_t nbanks = div_up(nodes, 32);
+    if (nbanks == 0) return;
+#define LSS_CA(BS, TS)                                                                                         \
+    k_coarse_apply<BS, TS><<<nbanks, kApplyThreads, 0, s>>>(nodes, static_cast<const TS *>(inv), r, parent,    \
+                                                            w_parent, w, stop)
+#define LSS_CA_BS(TS)                                                                                          \
+    switch (bs) {                                                                                              \
+        case 1: LSS_CA(1, TS); break;                                                                          \
+        case 2: LSS_CA(2, TS); break;                                                                          \
+        case 3: LSS_CA(3, TS); break;                                                                          \
+        default: fail(LSS_ERR_UNSUPPORTED, "block_size must be 1, 2 or 3");                                    \
+    }
+    switch (prec) {
+        case Precision::FP64: LSS_CA_BS(double); break;
+        case Precision::FP32: LSS_CA_BS(float); break;
+        case Precision::BF16: LSS_CA_BS(__hip_bfloat16); break;
+    }
+#undef LSS_CA_BS
+#undef LSS_CA
+    LSS_LAUNCH_CHECK();
+}
+
+void mas_level0_apply(Precision prec, int bs, int32_t nbanks, const int32_t *part_off, const void *inv,
+                      const double *r, const int32_t *parent, const double *w1, double scale, const double *add,
+                      double *z, const double *dotv, dev::PcgState *st, double *partials, const int *stop,
+                      hipStream_t s)
+{
+    if (nbanks == 0) return;
+    const bool dot = st != nullptr;
+#define LSS_L0(BS, TS, DOT)                                                                                    \
+    k_level0_apply<BS, TS, DOT><<<nbanks, kApplyThreads, 0, s>>>(nbanks, part_off, static_cast<const TS *>(inv), \
+                                                                 r, parent, w1, scale, add, z, dotv, st, partials, stop)
+#define LSS_L0_BS(TS)                                                                                          \
+    switch (bs) {                                                                                              \
+        case 1: if (dot) LSS_L0(1, TS, true); else LSS_L0(1, TS, false); break;                                \
+        case 2: if (dot) LSS_L0(2, TS, true); else LSS_L0(2, TS, false); break;                                \
+        case 3: if (dot) LSS_L0(3, TS, true); else LSS_L0(3, TS, false); break;                                \
+        default: fail(LSS_ERR_UNSUPPORTED, "block_size must be 1, 2 or 3");                                    \
+    }
+    switch (prec) {
+        case Precision::FP64: LSS_L0_BS(double); break;
+        case Precision::FP32: LSS_L0_BS(float); break;
+        case Precision::BF16: LSS_L0_BS(__hip_bfloat16); break;
+    }
+#undef LSS_L0_BS
+#undef LSS_L0
+    LSS_LAUNCH_CHECK();
+}
+
+}  // namespace lss::k

@@ -1,0 +1,48 @@
+// Launchers of the MAS preconditioner and stiff-block kernels.
+#pragma once
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "device_utils.hpp"
+
+namespace lss::k {
+
+// Storage precision of the MAS bank inverses.
+enum class Precision : int { FP64 = 0, FP32 = 1, BF16 = 2 };
+
+inline size_t precision_bytes(Precision p) { return p == Precision::FP64 ? 8 : (p == Precision::FP32 ? 4 : 2); }
+
+// Assemble + invert the banks of one level.  bank_off: nbanks + 1 node offsets.
+void mas_bank_invert(Precision prec, int bs, int32_t nbanks, const int32_t *bank_off, const int32_t *rowptr,
+                     const int32_t *cols, const double *vals, void *out, int *status, hipStream_t s);
+// Galerkin values: coarse slot t = sum of fine blocks src_blk[src_ptr[t] .. src_ptr[t+1]).
+void mas_galerkin(int64_t nslots, int bs, const int64_t *src_ptr, const int32_t *src_blk, const double *fine_vals,
+                  double *coarse_vals, hipStream_t s);
+void mas_restrict(int32_t nodes, int bs, const int64_t *child_ptr, const int32_t *child, const double *fine,
+                  double *coarse, const int *stop, hipStream_t s);
+void mas_coarse_apply(Precision prec, int bs, int32_t nodes, const void *inv, const double *r, const int32_t *parent,
+                      const double *w_parent, double *w, const int *stop, hipStream_t s);
+// dotv != nullptr with st: gamma = dotv . z is reduced into the PCG state.
+void mas_level0_apply(Precision prec, int bs, int32_t nbanks, const int32_t *part_off, const void *inv,
+                      const double *r, const int32_t *parent, const double *w1, double scale, const double *add,
+                      double *z, const double *dotv, dev::PcgState *st, double *partials, const int *stop,
+                      hipStream_t s);
+
+// ---------------------------------------------------------------- stiff blocks
+// Dense S_c A S_c^T per component from operator rows (original numbering).
+void stiff_extract(int32_t ncomp, const int64_t *comp_off, const int64_t *mat_off, const int32_t *dofs,
+                   const int32_t *g2l, const int32_t *comp_of, const int32_t *ptr, const int32_t *idx,
+                   const double *vals, double *mats, hipStream_t s);
+// In-place dense Cholesky (lower, row-major) of every component; status |= 2 on failure.
+void stiff_cholesky(int32_t ncomp, const int64_t *comp_off, const int64_t *mat_off, double *mats, int *status,
+                    hipStream_t s);
+// z[p] += (S_c A S_c^T)^{-1} (r - A z)[p] over the stiff rows p of each component,
+// (A z) taken from the block operator in the permuted numbering.
+void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, const int64_t *mat_off,
+                   const int32_t *pdofs, const double *L, const int32_t *rowptr, const int32_t *cols,
+                   const double *bvals, const double *r, double *z, double *work, const int *stop, hipStream_t s);
+
+}  // namespace lss::k

@@ -1,0 +1,331 @@
+// Block sparse formats and SpMV for gfx950.
+//
+// Matrix layout in HBM (sliced block-ELL, "SELL-H/bs"): block rows are grouped
+// into slices of H = 64/bs rows, one wave64 per slice.  Lane l of the wave owns
+// scalar row r = l % bs of block row h = l / bs, so a wave covers 64 (bs = 1),
+// 64 (bs = 2) or 63 (bs = 3) scalar rows and every lane accumulates its own row:
+// no cross-lane reduction, no atomics.  Slot j of slice s holds block j of each
+// of the H rows; the bs*bs values of the H blocks of a slot are contiguous, so
+// the wave's value loads for one slot cover one contiguous 8*H*bs*bs-byte span
+// (1512 B for bs = 3).  Rows shorter than the slice width are padded with
+// column -1 and zero values.
+//
+// Reference counterparts: BSR layout MAS/BSRMatrix.hpp:10-57 (row-major blocks,
+// tail block diagonal padded with 1, BSRMatrix.cu:123-134, 189-191); SpMV
+// cuSPARSE BSR (MASSolver.cu:356-375) / hypre ParCSR matvec (GPUHybrid.cu:454-466).
+#include "kernels.hpp"
+#include "../common/runtime.hpp"
+
+namespace lss::k {
+namespace {
+
+constexpr int kSpmvThreads = 256;  // 4 waves = 4 slices per workgroup
+
+__global__ void k_bsr_count(int32_t n, int bs, const int32_t *__restrict__ ptr, const int32_t *__restrict__ idx,
+                            int32_t *__restrict__ cnt, int32_t *__restrict__ unsorted)
+{
+    const int32_t nb = (n + bs - 1) / bs;
+    const int32_t I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nb) return;
+    int32_t p[3], e[3];
+    const int rows = min(bs, n - I * bs);
+    for (int r = 0; r < bs; ++r) {
+        if (r < rows) {
+            p[r] = ptr[I * bs + r];
+            e[r] = ptr[I * bs + r + 1];
+            for (int32_t k = p[r] + 1; k < e[r]; ++k)
+                if (idx[k] <= idx[k - 1]) *unsorted = 1;
+        } else {
+            p[r] = e[r] = 0;
+        }
+    }
+    int32_t c = 0;
+    bool has_diag = false;
+    while (true) {
+        int32_t cur = 0x7fffffff;
+        for (int r = 0; r < rows; ++r)
+            if (p[r] < e[r]) cur = min(cur, idx[p[r]] / bs);
+        if (cur == 0x7fffffff) break;
+        ++c;
+        has_diag |= (cur == I);
+        for (int r = 0; r < rows; ++r)
+            while (p[r] < e[r] && idx[p[r]] / bs == cur) ++p[r];
+    }
+    if (rows < bs && !has_diag) ++c;  // padded tail diagonal block
+    cnt[I] = c;
+}
+
+__global__ void k_bsr_fill(int32_t n, int bs, const int32_t *__restrict__ ptr, const int32_t *__restrict__ idx,
+                           const double *__restrict__ vals, const int32_t *__restrict__ rowptr,
+                           int32_t *__restrict__ cols, double *__restrict__ bvals)
+{
+    const int32_t nb = (n + bs - 1) / bs;
+    const int32_t I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nb) return;
+    const int bb = bs * bs;
+    int32_t p[3], e[3];
+    const int rows = min(bs, n - I * bs);
+    for (int r = 0; r < bs; ++r) {
+        p[r] = r < rows ? ptr[I * bs + r] : 0;
+        e[r] = r < rows ? ptr[I * bs + r + 1] : 0;
+    }
+    int32_t w = rowptr[I];
+    int32_t diag_slot = -1;
+    bool tail_pending = rows < bs;
+    while (true) {
+        int32_t cur = 0x7fffffff;
+        for (int r = 0; r < rows; ++r)
+            if (p[r] < e[r]) cur = min(cur, idx[p[r]] / bs);
+        if (tail_pending && I < cur && diag_slot < 0) {  // insert missing diagonal in order
+            cols[w] = I;
+            diag_slot = w++;
+            continue;
+        }
+        if (cur == 0x7fffffff) break;
+        if (cur == I) diag_slot = w;
+        cols[w] = cur;
+        for (int r = 0; r < rows; ++r)
+            while (p[r] < e[r] && idx[p[r]] / bs == cur) {
+                bvals[int64_t(w) * bb + r * bs + (idx[p[r]] - cur * bs)] = vals[p[r]];
+                ++p[r];
+            }
+        ++w;
+    }
+    if (tail_pending && diag_slot >= 0)
+        for (int r = rows; r < bs; ++r) bvals[int64_t(diag_slot) * bb + r * bs + r] = 1.0;
+}
+
+__global__ void k_bsr_permute(int32_t nb, int bs, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
+                              const double *__restrict__ vals, const int32_t *__restrict__ perm,
+                              const int32_t *__restrict__ new_rowptr, int32_t *__restrict__ new_cols,
+                              double *__restrict__ new_vals)
+{
+    const int32_t I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nb) return;
+    const int bb = bs * bs;
+    const int32_t b = rowptr[I], e = rowptr[I + 1];
+    const int32_t dst = new_rowptr[perm[I]];
+    for (int32_t k = b; k < e; ++k) {
+        const int32_t ck = perm[cols[k]];
+        int32_t rank = 0;
+        for (int32_t q = b; q < e; ++q) rank += perm[cols[q]] < ck;
+        new_cols[dst + rank] = ck;
+        for (int t = 0; t < bb; ++t) new_vals[int64_t(dst + rank) * bb + t] = vals[int64_t(k) * bb + t];
+    }
+}
+
+__global__ void k_sell_width(int32_t nb, int H, const int32_t *__restrict__ rowptr, int32_t *__restrict__ width)
+{
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t ns = (nb + H - 1) / H;
+    if (s >= ns) return;
+    int32_t w = 0;
+    for (int32_t r = s * H; r < min(nb, (s + 1) * H); ++r) w = max(w, rowptr[r + 1] - rowptr[r]);
+    width[s] = w;
+}
+
+__global__ void k_sell_fill(int32_t nb, int bs, int H, const int32_t *__restrict__ rowptr,
+                            const int32_t *__restrict__ cols, const double *__restrict__ vals,
+                            const int64_t *__restrict__ slice_ptr, int32_t *__restrict__ scols,
+                            double *__restrict__ svals, float *__restrict__ svals32)
+{
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nb) return;
+    const int bb = bs * bs;
+    const int32_t s = r / H, h = r % H;
+    const int64_t base = slice_ptr[s];
+    const int32_t W = int32_t((slice_ptr[s + 1] - base) / H);
+    const int32_t b = rowptr[r], len = rowptr[r + 1] - b;
+    for (int32_t j = 0; j < W; ++j) {
+        const int64_t slot = base + int64_t(j) * H + h;
+        if (j < len) {
+            scols[slot] = cols[b + j];
+            for (int t = 0; t < bb; ++t) {
+                const double v = vals[int64_t(b + j) * bb + t];
+                svals[slot * bb + t] = v;
+                if (svals32) svals32[slot * bb + t] = static_cast<float>(v);
+            }
+        } else {
+            scols[slot] = -1;
+        }
+    }
+}
+
+// One wave per slice.  MODE 0: y = A x; MODE 1: y = sub - A x.
+// DOT: per-lane partial of x_i * y_i (p . A p) reduced deterministically.
+template <int BS, int MODE, bool DOT, bool F32>
+__global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslices, const int64_t *__restrict__ slice_ptr,
+                                                        const int32_t *__restrict__ cols, const double *__restrict__ vals,
+                                                        const float *__restrict__ vals32, const double *__restrict__ x,
+                                                        double *__restrict__ y, const double *__restrict__ sub,
+                                                        const int *__restrict__ stop, dev::PcgState *st,
+                                                        double *__restrict__ partials, int iter)
+{
+    constexpr int H = 64 / BS;
+    constexpr int BB = BS * BS;
+    if (stop && *stop) return;
+    const int lane = threadIdx.x & 63;
+    const int32_t slice = blockIdx.x * (kSpmvThreads / 64) + (threadIdx.x >> 6);
+    const int h = lane / BS, r = lane - h * BS;
+    const int32_t row = slice * H + h;
+    const bool active = (h < H) && (slice < nslices) && (row < nb);
+    double acc = 0.0;
+    if (slice < nslices) {
+        const int64_t base = slice_ptr[slice];
+        const int32_t W = int32_t((slice_ptr[slice + 1] - base) / H);
+        int32_t j = 0;
+        for (; j + 2 <= W; j += 2) {
+            const int64_t s0 = base + int64_t(j) * H + h, s1 = s0 + H;
+            int32_t c0 = active ? cols[s0] : -1;
+            int32_t c1 = active ? cols[s1] : -1;
+            const bool v0 = c0 >= 0, v1 = c1 >= 0;
+            c0 = v0 ? c0 : 0;
+            c1 = v1 ? c1 : 0;
+            double a0[BS], a1[BS], x0[BS], x1[BS];
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                if constexpr (F32) {
+                    a0[c] = v0 ? double(vals32[s0 * BB + r * BS + c]) : 0.0;
+                    a1[c] = v1 ? double(vals32[s1 * BB + r * BS + c]) : 0.0;
+                } else {
+                    a0[c] = v0 ? vals[s0 * BB + r * BS + c] : 0.0;
+                    a1[c] = v1 ? vals[s1 * BB + r * BS + c] : 0.0;
+                }
+                x0[c] = x[int64_t(c0) * BS + c];
+                x1[c] = x[int64_t(c1) * BS + c];
+            }
+#pragma unroll
+            for (int c = 0; c < BS; ++c) acc += a0[c] * x0[c];
+#pragma unroll
+            for (int c = 0; c < BS; ++c) acc += a1[c] * x1[c];
+        }
+        if (j < W) {
+            const int64_t s0 = base + int64_t(j) * H + h;
+            int32_t c0 = active ? cols[s0] : -1;
+            if (c0 >= 0) {
+#pragma unroll
+                for (int c = 0; c < BS; ++c) {
+                    const double a = F32 ? double(vals32[s0 * BB + r * BS + c]) : vals[s0 * BB + r * BS + c];
+                    acc += a * x[int64_t(c0) * BS + c];
+                }
+            }
+        }
+    }
+    double out = 0.0;
+    if (active) {
+        const int64_t i = int64_t(row) * BS + r;
+        out = (MODE == 1) ? sub[i] - acc : acc;
+        y[i] = out;
+    }
+    if constexpr (DOT) {
+        __shared__ double sh[kSpmvThreads / 64];
+        const double part = dev::block_sum<kSpmvThreads>(active ? x[int64_t(row) * BS + r] * out : 0.0, sh);
+        double total;
+        if (dev::grid_sum_last<kSpmvThreads>(part, partials, &st->ticket[0], sh, total) && threadIdx.x == 0) {
+            // GPUHybrid.cu:1458-1480
+            st->iters = iter + 1;
+            st->pq = total;
+            if (total == 0.0) {
+                st->stop = dev::kZeroSdotp;
+            } else {
+                const double alpha = st->gamma / total;
+                st->alpha = alpha;
+                if (alpha <= 0.0)
+                    st->stop = dev::kNegativeAlpha;
+                else if (alpha < 2.2250738585072014e-308)
+                    st->stop = dev::kSubnormalAlpha;
+            }
+        }
+    }
+}
+
+template <int MODE, bool DOT, bool F32>
+void launch_spmv(const SellView &A, const double *x, double *y, const double *sub, const int *stop,
+                 dev::PcgState *st, double *partials, int iter, hipStream_t s)
+{
+    const int grid = div_up(A.nslices, kSpmvThreads / 64);
+    if (grid == 0) return;
+#define LSS_SPMV_CASE(BS)                                                                                      \
+    k_spmv<BS, MODE, DOT, F32><<<grid, kSpmvThreads, 0, s>>>(A.nb, A.nslices, A.slice_ptr, A.cols, A.vals,     \
+                                                             A.vals32, x, y, sub, stop, st, partials, iter)
+    switch (A.bs) {
+        case 1: LSS_SPMV_CASE(1); break;
+        case 2: LSS_SPMV_CASE(2); break;
+        case 3: LSS_SPMV_CASE(3); break;
+        default: fail(LSS_ERR_UNSUPPORTED, "block_size must be 1, 2 or 3");
+    }
+#undef LSS_SPMV_CASE
+    LSS_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void bsr_count(int32_t n, int bs, const int32_t *ptr, const int32_t *idx, int32_t *cnt, int32_t *unsorted_flag,
+               hipStream_t s)
+{
+    const int32_t nb = (n + bs - 1) / bs;
+    if (nb == 0) return;
+    k_bsr_count<<<div_up(nb, 256), 256, 0, s>>>(n, bs, ptr, idx, cnt, unsorted_flag);
+    LSS_LAUNCH_CHECK();
+}
+
+void bsr_fill(int32_t n, int bs, const int32_t *ptr, const int32_t *idx, const double *vals, const int32_t *rowptr,
+              int32_t *cols, double *bvals, hipStream_t s)
+{
+    const int32_t nb = (n + bs - 1) / bs;
+    if (nb == 0) return;
+    k_bsr_fill<<<div_up(nb, 256), 256, 0, s>>>(n, bs, ptr, idx, vals, rowptr, cols, bvals);
+    LSS_LAUNCH_CHECK();
+}
+
+void bsr_permute(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols, const double *vals,
+                 const int32_t *perm, const int32_t *new_rowptr, int32_t *new_cols, double *new_vals, hipStream_t s)
+{
+    if (nb == 0) return;
+    k_bsr_permute<<<div_up(nb, 256), 256, 0, s>>>(nb, bs, rowptr, cols, vals, perm, new_rowptr, new_cols, new_vals);
+    LSS_LAUNCH_CHECK();
+}
+
+void sell_width(int32_t nb, int bs, const int32_t *rowptr, int32_t *slice_width, hipStream_t s)
+{
+    const int H = 64 / bs;
+    const int32_t ns = div_up(nb, H);
+    if (ns == 0) return;
+    k_sell_width<<<div_up(ns, 256), 256, 0, s>>>(nb, H, rowptr, slice_width);
+    LSS_LAUNCH_CHECK();
+}
+
+void sell_fill(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols, const double *vals,
+               const int64_t *slice_ptr, int32_t *scols, double *svals, float *svals32, hipStream_t s)
+{
+    if (nb == 0) return;
+    k_sell_fill<<<div_up(nb, 256), 256, 0, s>>>(nb, bs, 64 / bs, rowptr, cols, vals, slice_ptr, scols, svals,
+                                                  svals32);
+    LSS_LAUNCH_CHECK();
+}
+
+void spmv(const SellView &A, const double *x, double *y, const double *sub, hipStream_t s, bool fp32)
+{
+    if (fp32) {
+        if (sub) launch_spmv<1, false, true>(A, x, y, sub, nullptr, nullptr, nullptr, 0, s);
+        else launch_spmv<0, false, true>(A, x, y, nullptr, nullptr, nullptr, nullptr, 0, s);
+    } else {
+        if (sub) launch_spmv<1, false, false>(A, x, y, sub, nullptr, nullptr, nullptr, 0, s);
+        else launch_spmv<0, false, false>(A, x, y, nullptr, nullptr, nullptr, nullptr, 0, s);
+    }
+}
+
+void spmv_pq(const SellView &A, const double *p, double *q, dev::PcgState *st, double *partials, int iter,
+             hipStream_t s)
+{
+    launch_spmv<0, true, false>(A, p, q, nullptr, &st->stop, st, partials, iter, s);
+}
+
+void spmv_residual(const SellView &A, const double *x, const double *b, double *r, const int *stop, hipStream_t s,
+                   bool fp32)
+{
+    if (fp32) launch_spmv<1, false, true>(A, x, r, b, stop, nullptr, nullptr, 0, s);
+    else launch_spmv<1, false, false>(A, x, r, b, stop, nullptr, nullptr, 0, s);
+}
+
+}  // namespace lss::k

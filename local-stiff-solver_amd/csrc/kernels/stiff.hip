@@ -1,0 +1,247 @@
+// Stiff-block direct solves on gfx950, FP64 throughout.
+//
+// Reference: factorize_submatrix (GPUHybrid.cu:993-1411) extracts S_c A S_c^T
+// per stiff component and factorizes it with cuDSS (sparse LDL^T) or cuBLAS
+// (dense LU + explicit inverse); dss_precond_iter (544-675) applies
+// z2 = z1 + S^T (S A S^T)^{-1} S (r - A z1) with gather / solve / scatter.
+// Here: dense Cholesky per component (one workgroup each, blocked, diagonal
+// blocks in LDS), and the correction computes (r - A z1) only on the stiff
+// rows (the other rows of the reference's full SpMV are never read), then
+// solves with blocked triangular sweeps and adds into z in place.
+#include "mas_kernels.hpp"
+#include "../common/runtime.hpp"
+
+namespace lss::k {
+namespace {
+
+constexpr int NB = 32;
+constexpr int kT = 256;
+
+__global__ __launch_bounds__(kT) void k_extract(const int64_t *__restrict__ comp_off, const int64_t *__restrict__ mat_off,
+                                                const int32_t *__restrict__ dofs, const int32_t *__restrict__ g2l,
+                                                const int32_t *__restrict__ comp_of, const int32_t *__restrict__ ptr,
+                                                const int32_t *__restrict__ idx, const double *__restrict__ vals,
+                                                double *__restrict__ mats)
+{
+    // GPUHybrid.cu:1351-1383 semantics: row r of the block = operator row dofs[r]
+    const int32_t c = blockIdx.x;
+    const int64_t off = comp_off[c];
+    const int32_t m = int32_t(comp_off[c + 1] - off);
+    double *M = mats + mat_off[c];
+    for (int32_t r = threadIdx.x; r < m; r += kT) {
+        const int32_t d = dofs[off + r];
+        for (int32_t e = ptr[d]; e < ptr[d + 1]; ++e) {
+            const int32_t j = idx[e];
+            if (comp_of[j] == c) M[int64_t(r) * m + g2l[j]] = vals[e];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kT) void k_cholesky(const int64_t *__restrict__ comp_off, const int64_t *__restrict__ mat_off,
+                                                 double *__restrict__ mats, int *__restrict__ status)
+{
+    __shared__ double Lkk[NB][NB + 1];
+    __shared__ int bad;
+    const int32_t c = blockIdx.x;
+    const int32_t m = int32_t(comp_off[c + 1] - comp_off[c]);
+    double *A = mats + mat_off[c];
+    if (threadIdx.x == 0) bad = 0;
+    for (int32_t k0 = 0; k0 < m; k0 += NB) {
+        const int nb = min(NB, m - k0);
+        // 1. diagonal block in LDS
+        for (int t = threadIdx.x; t < NB * NB; t += kT) {
+            const int i = t / NB, j = t % NB;
+            Lkk[i][j] = (i < nb && j <= i) ? A[int64_t(k0 + i) * m + k0 + j] : 0.0;
+        }
+        __syncthreads();
+        for (int j = 0; j < nb; ++j) {
+            if (threadIdx.x == 0) {
+                const double d = Lkk[j][j];
+                if (!(d > 0.0) || !isfinite(d)) bad = 1;
+                Lkk[j][j] = sqrt(d);
+            }
+            __syncthreads();
+            const double dj = Lkk[j][j];
+            for (int i = j + 1 + threadIdx.x; i < nb; i += kT) Lkk[i][j] /= dj;
+            __syncthreads();
+            for (int t = threadIdx.x; t < NB * NB; t += kT) {
+                const int i = t / NB, l = t % NB;
+                if (i < nb && l > j && l <= i) Lkk[i][l] -= Lkk[i][j] * Lkk[l][j];
+            }
+            __syncthreads();
+        }
+        for (int t = threadIdx.x; t < NB * NB; t += kT) {
+            const int i = t / NB, j = t % NB;
+            if (i < nb && j < nb) A[int64_t(k0 + i) * m + k0 + j] = (j <= i) ? Lkk[i][j] : 0.0;
+        }
+        // 2. panel rows below: L[i, blk] = A[i, blk] Lkk^{-T}
+        for (int32_t i = k0 + nb + threadIdx.x; i < m; i += kT) {
+            double v[NB];
+#pragma unroll
+            for (int l = 0; l < NB; ++l) {
+                if (l < nb) {
+                    double s = A[int64_t(i) * m + k0 + l];
+                    for (int t = 0; t < l; ++t) s -= v[t] * Lkk[l][t];
+                    v[l] = s / Lkk[l][l];
+                }
+            }
+#pragma unroll
+            for (int l = 0; l < NB; ++l)
+                if (l < nb) A[int64_t(i) * m + k0 + l] = v[l];
+        }
+        __syncthreads();
+        // 3. trailing lower update
+        const int32_t T = m - k0 - nb;
+        const int64_t pairs = int64_t(T) * (T + 1) / 2;
+        for (int64_t q = threadIdx.x; q < pairs; q += kT) {
+            // row-major lower enumeration: q -> (i, j), j <= i
+            int32_t i = int32_t((sqrt(8.0 * double(q) + 1.0) - 1.0) * 0.5);
+            while (int64_t(i) * (i + 1) / 2 > q) --i;
+            while (int64_t(i + 1) * (i + 2) / 2 <= q) ++i;
+            const int32_t j = int32_t(q - int64_t(i) * (i + 1) / 2);
+            const double *Li = A + int64_t(k0 + nb + i) * m + k0;
+            const double *Lj = A + int64_t(k0 + nb + j) * m + k0;
+            double s = 0.0;
+            for (int t = 0; t < nb; ++t) s += Li[t] * Lj[t];
+            A[int64_t(k0 + nb + i) * m + k0 + nb + j] -= s;
+        }
+        __syncthreads();
+    }
+    // zero the strict upper triangle (never read, kept clean)
+    for (int64_t q = threadIdx.x; q < int64_t(m) * m; q += kT) {
+        const int32_t i = int32_t(q / m), j = int32_t(q % m);
+        if (j > i) A[q] = 0.0;
+    }
+    if (threadIdx.x == 0 && bad) atomicOr(status, 2);
+}
+
+__global__ void k_stiff_residual(int64_t ns, int bs, const int32_t *__restrict__ pdofs,
+                                 const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
+                                 const double *__restrict__ bvals, const double *__restrict__ r,
+                                 const double *__restrict__ z, double *__restrict__ work, const int *__restrict__ stop)
+{
+    if (stop && *stop) return;
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    const int32_t p = pdofs[k];
+    const int32_t I = p / bs;
+    const int a = p - I * bs;
+    const int bb = bs * bs;
+    double acc = 0.0;
+    for (int32_t e = rowptr[I]; e < rowptr[I + 1]; ++e) {
+        const int32_t J = cols[e];
+        for (int c = 0; c < bs; ++c) acc += bvals[int64_t(e) * bb + a * bs + c] * z[int64_t(J) * bs + c];
+    }
+    work[k] = r[p] - acc;
+}
+
+__global__ __launch_bounds__(kT) void k_stiff_solve(const int64_t *__restrict__ comp_off,
+                                                    const int64_t *__restrict__ mat_off, const int32_t *__restrict__ pdofs,
+                                                    const double *__restrict__ Lall, double *__restrict__ work,
+                                                    double *__restrict__ z, const int *__restrict__ stop)
+{
+    if (stop && *stop) return;
+    __shared__ double Lkk[NB][NB + 1];
+    __shared__ double yb[NB];
+    const int32_t c = blockIdx.x;
+    const int64_t off = comp_off[c];
+    const int32_t m = int32_t(comp_off[c + 1] - off);
+    const double *L = Lall + mat_off[c];
+    double *y = work + off;
+    const int lane = threadIdx.x & 63;
+    // forward: L y = t
+    for (int32_t k0 = 0; k0 < m; k0 += NB) {
+        const int nb = min(NB, m - k0);
+        for (int t = threadIdx.x; t < NB * NB; t += kT) {
+            const int i = t / NB, j = t % NB;
+            Lkk[i][j] = (i < nb && j < nb) ? L[int64_t(k0 + i) * m + k0 + j] : 0.0;
+        }
+        if (threadIdx.x < nb) yb[threadIdx.x] = y[k0 + threadIdx.x];
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            double yl = lane < nb ? yb[lane] : 0.0;
+            for (int j = 0; j < nb; ++j) {
+                if (lane == j) yl /= Lkk[j][j];
+                const double yj = __shfl(yl, j, 64);
+                if (lane > j && lane < nb) yl -= Lkk[lane][j] * yj;
+            }
+            if (lane < nb) {
+                yb[lane] = yl;
+                y[k0 + lane] = yl;
+            }
+        }
+        __syncthreads();
+        for (int32_t i = k0 + nb + threadIdx.x; i < m; i += kT) {
+            const double *Li = L + int64_t(i) * m + k0;
+            double s = 0.0;
+            for (int t = 0; t < nb; ++t) s += Li[t] * yb[t];
+            y[i] -= s;
+        }
+        __syncthreads();
+    }
+    // backward: L^T x = y
+    const int32_t nblk = (m + NB - 1) / NB;
+    for (int32_t b = nblk - 1; b >= 0; --b) {
+        const int32_t k0 = b * NB;
+        const int nb = min(NB, m - k0);
+        for (int t = threadIdx.x; t < NB * NB; t += kT) {
+            const int i = t / NB, j = t % NB;
+            Lkk[i][j] = (i < nb && j < nb) ? L[int64_t(k0 + i) * m + k0 + j] : 0.0;
+        }
+        if (threadIdx.x < nb) yb[threadIdx.x] = y[k0 + threadIdx.x];
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            double xl = lane < nb ? yb[lane] : 0.0;
+            for (int j = nb - 1; j >= 0; --j) {
+                if (lane == j) xl /= Lkk[j][j];
+                const double xj = __shfl(xl, j, 64);
+                if (lane < j) xl -= Lkk[j][lane] * xj;
+            }
+            if (lane < nb) {
+                yb[lane] = xl;
+                y[k0 + lane] = xl;
+            }
+        }
+        __syncthreads();
+        for (int32_t i = threadIdx.x; i < k0; i += kT) {
+            double s = 0.0;
+            for (int t = 0; t < nb; ++t) s += L[int64_t(k0 + t) * m + i] * yb[t];
+            y[i] -= s;
+        }
+        __syncthreads();
+    }
+    // z2 = z1 + S^T x (GPUHybrid.cu:621-671)
+    for (int32_t i = threadIdx.x; i < m; i += kT) z[pdofs[off + i]] += y[i];
+}
+
+}  // namespace
+
+void stiff_extract(int32_t ncomp, const int64_t *comp_off, const int64_t *mat_off, const int32_t *dofs,
+                   const int32_t *g2l, const int32_t *comp_of, const int32_t *ptr, const int32_t *idx,
+                   const double *vals, double *mats, hipStream_t s)
+{
+    if (ncomp == 0) return;
+    k_extract<<<ncomp, kT, 0, s>>>(comp_off, mat_off, dofs, g2l, comp_of, ptr, idx, vals, mats);
+    LSS_LAUNCH_CHECK();
+}
+
+void stiff_cholesky(int32_t ncomp, const int64_t *comp_off, const int64_t *mat_off, double *mats,
+                    int *status, hipStream_t s)
+{
+    if (ncomp == 0) return;
+    k_cholesky<<<ncomp, kT, 0, s>>>(comp_off, mat_off, mats, status);
+    LSS_LAUNCH_CHECK();
+}
+
+void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, const int64_t *mat_off,
+                   const int32_t *pdofs, const double *L, const int32_t *rowptr, const int32_t *cols,
+                   const double *bvals, const double *r, double *z, double *work, const int *stop, hipStream_t s)
+{
+    if (ncomp == 0 || ns == 0) return;
+    k_stiff_residual<<<div_up(ns, 256), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r, z, work, stop);
+    LSS_LAUNCH_CHECK();
+    k_stiff_solve<<<ncomp, kT, 0, s>>>(comp_off, mat_off, pdofs, L, work, z, stop);
+    LSS_LAUNCH_CHECK();
+}
+
+}  // namespace lss::k

@@ -1,0 +1,106 @@
+"""HIP path vs the oracle (CPU restatement of the reference GPUHybrid path)."""
+import numpy as np
+import pytest
+
+from lss_amd import problems
+from lss_amd.solver import Solver
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(bs, **kw):
+    g = {"block_size": bs, "relative_tolerance": 1e-8, "gmm_jump_threshold": 10.0, "min_subdomain_size": 4}
+    g.update(kw)
+    return {"solver": "GPUHybrid", "GPUHybrid": g}
+
+
+def _ocfg(bs, **kw):
+    base = dict(block_size=bs, rel_tol=1e-8, gmm_jump_threshold=10.0, min_subdomain_size=4)
+    base.update(kw)
+    return O.config(**base)
+
+
+CASES = [("laplace_sphere", 12), ("elastic_contact", 10), ("elastic_slivers", 8), ("elastic_logE", 8)]
+
+
+@pytest.mark.parametrize("name,N", CASES)
+def test_row_norms_bit_exact(gpu, name, N):
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    s = Solver.create(_cfg(prob.block_size))
+    s.factorize(A)
+    dev = s.row_norms()
+    ref = O.row_l1(A.colptr, A.vals)
+    assert np.array_equal(dev, ref)
+
+
+@pytest.mark.parametrize("name,N", CASES)
+def test_stiff_set_matches_oracle(gpu, name, N):
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    s = Solver.create(_cfg(prob.block_size))
+    s.factorize(A)
+    g = s.gmm()
+    ref = O.gmm_select(O.row_l1(A.colptr, A.vals), 1e-3, 20, 10.0)
+    for k in ("global_mean", "global_var", "mean0", "mean1", "var0", "var1", "w0", "w1"):
+        assert g[k] == pytest.approx(ref[k], rel=1e-12, abs=1e-300), k
+    assert g["iters"] == ref["iters"]
+    assert g["selected"] == ref["set"].size
+    H = O.Hybrid(_ocfg(prob.block_size))
+    H.factorize(A.colptr, A.rowidx, A.vals)
+    oc = H.stiff_components()
+    dc = s.stiff_components()
+    assert len(dc) == len(oc)
+    for a, b in zip(dc, oc):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name,N", CASES)
+def test_spmv_matches(gpu, name, N):
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    s = Solver.create(_cfg(prob.block_size))
+    s.factorize(A)
+    x = problems.random_vector(A.n, 7)
+    y = s.spmv(x)
+    yr = O.spmv(A.colptr, A.rowidx, A.vals, x)
+    assert np.max(np.abs(y - yr)) <= 1e-13 * np.max(np.abs(yr))
+
+
+@pytest.mark.parametrize("name,N", CASES)
+def test_mas_apply_matches(gpu, name, N):
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    bs = prob.block_size
+    s = Solver.create(_cfg(bs, do_hybrid_precond=False))
+    s.factorize(A)
+    nb = A.n // bs
+    perm, off = s.partition(nb)
+    mas = O.Mas(A.colptr, A.rowidx, A.vals, bs, perm, off)
+    r = problems.random_vector(A.n, 3)
+    z = s.base_apply(r)
+    zr = mas.apply(r)
+    assert np.linalg.norm(z - zr) <= 1e-9 * np.linalg.norm(zr)
+    info = s.get_info()
+    assert info["mas"]["levels"] == mas.levels
+
+
+@pytest.mark.parametrize("name,N", CASES)
+def test_solve_matches_oracle(gpu, name, N):
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    b = prob.rhs(0)
+    s = Solver.create(_cfg(prob.block_size))
+    s.factorize(A)
+    x = s.solve(b)
+    info = s.get_info()
+    H = O.Hybrid(_ocfg(prob.block_size))
+    H.factorize(A.colptr, A.rowidx, A.vals)
+    xr = H.solve(b)
+    oi = H.info()
+    S = A.to_scipy()
+    assert np.linalg.norm(S @ x - b) <= 1.01e-8 * np.linalg.norm(b)
+    assert info["omega"] == pytest.approx(oi["omega"], rel=1e-6)
+    assert abs(info["num_iterations"] - oi["num_iterations"]) <= max(2, 0.05 * oi["num_iterations"])
+    assert np.linalg.norm(x - xr) <= 1e-5 * np.linalg.norm(xr)
