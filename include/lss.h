@@ -70,8 +70,9 @@ int lss_factorize_csc(lss_solver *s, int32_t n, int64_t nnz, const int32_t *colp
                       const double *vals);
 
 /* Factorize from device-resident CSC arrays (not retained after return).
- * stream: hipStream_t or NULL for the solver's own stream.  Same semantics as
- * lss_factorize_csc without the host-to-device copy. */
+ * stream: hipStream_t, or NULL for the solver's own stream (a blocking stream,
+ * ordered with the legacy default stream).  Same semantics as lss_factorize_csc
+ * without the host-to-device copy. */
 int lss_factorize_csc_device(lss_solver *s, int32_t n, int64_t nnz, const int32_t *d_colptr,
                              const int32_t *d_rowidx, const double *d_vals, void *stream);
 

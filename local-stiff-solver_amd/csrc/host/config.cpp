@@ -87,9 +87,10 @@ Config parse_config(const char *text, const Config &base)
         else if (k == "base_preconditioner") c.base_preconditioner = as_str(v, k);
         else if (k == "check_every") c.check_every = int(as_int(v, k));
         else if (k == "damping") c.damping = as_float(v, k);
-        else if (k == "power_iterations") c.power_iterations = int(as_int(v, k));
+        else if (k == "lanczos_steps") c.lanczos_steps = int(as_int(v, k));
         else if (k == "max_dense_component") c.max_dense_component = as_int(v, k);
         else if (k == "profile_kernels") c.profile_kernels = as_bool(v, k);
+        else if (k == "true_residual_check") c.true_residual_check = as_bool(v, k);
         else bad("unknown key GPUHybrid/" + k);
     }
     if (c.block_size < 1 || c.block_size > 3) bad("block_size must be 1, 2 or 3");
@@ -99,7 +100,7 @@ Config parse_config(const char *text, const Config &base)
     if (c.base_preconditioner != "MAS" && c.base_preconditioner != "Jacobi") bad("base_preconditioner must be MAS|Jacobi");
     if (c.check_every < 1) bad("check_every must be >= 1");
     if (!(c.damping > 0.0) || c.damping >= 2.0) bad("damping must be in (0, 2)");
-    if (c.power_iterations < 1) bad("power_iterations must be >= 1");
+    if (c.lanczos_steps < 1) bad("lanczos_steps must be >= 1");
     return c;
 }
 

@@ -31,12 +31,16 @@ struct Config {
     std::string inner_spmv = "auto";    // preconditioner-internal SpMV values: "auto" | "fp64" | "fp32"
     std::string base_preconditioner = "MAS";  // "MAS" | "Jacobi"
     int check_every = 8;                // PCG iterations between host polls of the device stop flag
-    double damping = 1.0;               // omega = damping / lambda_max(B A) inside the B-P-B sweep
-    int power_iterations = 12;          // iterations of the lambda_max(B A) estimate
+    double damping = 1.5;               // omega = damping / lambda_max(B A) inside the B-P-B sweep
+    int lanczos_steps = 20;          // CG-Lanczos steps of the lambda_max(B A) estimate
     int64_t max_dense_component = 16384;  // largest stiff component for the dense Cholesky
     bool profile_kernels = false;       // time SpMV / MAS launches with HIP events
+    bool true_residual_check = false;   // on convergence verify ||b - A x|| and restart PCG from it if needed
 
-    bool fp32_inner() const { return inner_spmv == "fp32" || (inner_spmv == "auto" && precision != "fp64"); }
+    // FP32 matrix values in the preconditioner-internal SpMV only on request: next to
+    // an FP64 stiff-block solve they make the sweep inconsistent (an FP32 rounding of
+    // a 1e8 contact entry exceeds the soft entries)
+    bool fp32_inner() const { return inner_spmv == "fp32"; }
 };
 
 // Parse a configuration document; throws lss::Error(LSS_ERR_CONFIG) on bad input.

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <sstream>
@@ -41,13 +42,15 @@ double d2h_scalar(const double *d, hipStream_t s)
 
 HybridSolver::HybridSolver(const Config &cfg) : cfg_(cfg)
 {
-    LSS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    // a blocking stream: work on it is ordered with the legacy default stream, so
+    // callers that pass NULL (e.g. torch's default stream) need no extra sync
+    LSS_HIP(hipStreamCreate(&stream_));
     void *p = nullptr;
     LSS_HIP(hipHostMalloc(&p, sizeof(dev::PcgState), hipHostMallocDefault));
     h_st_ = static_cast<dev::PcgState *>(p);
     std::memset(h_st_, 0, sizeof(dev::PcgState));
     st_.resize(1);
-    st_.zero(stream_);  // reduction tickets start at zero and are reset by every reduction
+    st_.zero(stream_);
     partials_.resize(kPartialsCap);
     d_status_.resize(1);
     sync(stream_);
@@ -511,41 +514,100 @@ void HybridSolver::alloc_vectors()
     xp_.resize(np), bp_.resize(np), r_.resize(np), p_.resize(np), q_.resize(np), z_.resize(np), u_.resize(np);
 }
 
+namespace {
+
+// Largest eigenvalue of the symmetric tridiagonal (d, e) by Sturm-count bisection.
+double tridiag_max_eig(const std::vector<double> &d, const std::vector<double> &e)
+{
+    const int m = int(d.size());
+    double lo = 0.0, hi = 0.0;
+    for (int i = 0; i < m; ++i) {
+        const double rad = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(e[i]) : 0.0);
+        hi = std::max(hi, d[i] + rad);
+        lo = std::min(lo, d[i] - rad);
+    }
+    auto count_below = [&](double x) {
+        int cnt = 0;
+        double q = d[0] - x;
+        if (q < 0) ++cnt;
+        for (int i = 1; i < m; ++i) {
+            if (q == 0.0) q = 1e-300;
+            q = d[i] - x - e[i - 1] * e[i - 1] / q;
+            if (q < 0) ++cnt;
+        }
+        return cnt;
+    };
+    for (int it = 0; it < 200 && hi - lo > 1e-13 * std::max(1.0, std::fabs(hi)); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (count_below(mid) >= m) hi = mid;
+        else lo = mid;
+    }
+    return hi;
+}
+
+}  // namespace
+
 void HybridSolver::estimate_omega(hipStream_t s)
 {
     omega_ = 1.0;
     if (!cfg_.do_hybrid_precond || ncomp_ == 0) return;
-    // lambda_max(B A) by power iteration in the A inner product; omega = damping / lambda
+    // The additive MAS can have lambda_max(B A) > 2, which makes the multiplicative
+    // sweep (I - BA)(I - PA)(I - BA) non-contractive and the composed preconditioner
+    // indefinite.  omega = damping / theta, theta the largest Ritz value of B A after
+    // lanczos_steps steps of B-preconditioned CG (CG-Lanczos) on A x = v, v hashed;
+    // theta <= lambda_max(B A) and converges to it quickly.
     const int64_t np = int64_t(nb_) * bs_;
-    k::hash_vector(n_, r_.get(), s);
-    k::fill(np, p_.get(), 0.0, s);
-    k::permute_in(nb_, bs_, n_, d_perm_.get(), r_.get(), p_.get(), s);
-    DBuf<double> sc(3);
-    double lam = 1.0;
-    for (int it = 0; it < cfg_.power_iterations; ++it) {
-        k::spmv(sell_, p_.get(), q_.get(), nullptr, s);
-        base_apply(q_.get(), 1.0, nullptr, z_.get(), nullptr, nullptr, s);
-        k::dot(np, q_.get(), z_.get(), partials_.get(), &st_.get()->ticket[0], sc.get(), s);
-        k::dot(np, p_.get(), q_.get(), partials_.get(), &st_.get()->ticket[0], sc.get() + 1, s);
-        k::dot(np, z_.get(), z_.get(), partials_.get(), &st_.get()->ticket[0], sc.get() + 2, s);
-        double h[3];
-        sc.download(h, 3, s);
-        sync(s);
-        lam = h[0] / h[1];
-        k::copy(np, z_.get(), p_.get(), s);
-        k::scale(np, p_.get(), 1.0 / std::sqrt(h[2]), s);
+    double *r = r_.get(), *z = z_.get(), *p = p_.get(), *q = q_.get();
+    k::hash_vector(n_, u_.get(), s);
+    k::fill(np, r, 0.0, s);
+    k::permute_in(nb_, bs_, n_, d_perm_.get(), u_.get(), r, s);
+    DBuf<double> sc(1);
+    base_apply(r, 1.0, nullptr, z, nullptr, nullptr, s);
+    k::copy(np, z, p, s);
+    k::dot(np, r, z, partials_.get(), sc.get(), s);
+    double rz = d2h_scalar(sc.get(), s);
+    std::vector<double> alpha, beta;
+    for (int j = 0; j < cfg_.lanczos_steps && rz > 0.0; ++j) {
+        k::spmv(sell_, p, q, nullptr, s);
+        k::dot(np, p, q, partials_.get(), sc.get(), s);
+        const double pq = d2h_scalar(sc.get(), s);
+        if (!(pq > 0.0)) break;
+        const double a = rz / pq;
+        k::axpy(np, -a, q, r, s);
+        base_apply(r, 1.0, nullptr, z, nullptr, nullptr, s);
+        k::dot(np, r, z, partials_.get(), sc.get(), s);
+        const double rz2 = d2h_scalar(sc.get(), s);
+        const double b = rz2 / rz;
+        alpha.push_back(a);
+        beta.push_back(b);
+        if (!(rz2 > 0.0)) break;
+        rz = rz2;
+        k::xpby(np, z, b, p, s);
     }
-    if (!(lam > 0.0) || !std::isfinite(lam)) fail(LSS_ERR_NOT_SPD, "lambda_max(BA) estimate is not positive");
-    omega_ = cfg_.damping / lam;
+    const int m = int(alpha.size());
+    if (m == 0) fail(LSS_ERR_NOT_SPD, "lambda_max(BA) estimate failed: B or A is not positive definite");
+    std::vector<double> d(m), e(m > 0 ? m - 1 : 0);
+    for (int j = 0; j < m; ++j) {
+        d[j] = 1.0 / alpha[j] + (j > 0 ? beta[j - 1] / alpha[j - 1] : 0.0);
+        if (j + 1 < m) e[j] = std::sqrt(beta[j]) / alpha[j];
+    }
+    const double theta = tridiag_max_eig(d, e);
+    if (!(theta > 0.0) || !std::isfinite(theta))
+        fail(LSS_ERR_NOT_SPD, "lambda_max(BA) estimate is not positive: " + std::to_string(theta));
+    lambda_max_ = theta;
+    omega_ = cfg_.damping / theta;
 }
 
 // ------------------------------------------------------------------ preconditioner
 void HybridSolver::base_apply(const double *r, double scale, const double *add, double *z, dev::PcgState *st,
-                              const int *stop, hipStream_t s)
+                              const int *stop, hipStream_t s, bool first)
 {
     const int64_t np = int64_t(nb_) * bs_;
+    const int op = first ? dev::kFinishRzFirst : dev::kFinishRz;
     if (cfg_.base_preconditioner != "MAS") {
-        k::jacobi_apply(np, d_dinv_.get(), r, scale, add, z, st ? r_.get() : nullptr, st, partials_.get(), stop, s);
+        const int g =
+            k::jacobi_apply(np, d_dinv_.get(), r, scale, add, z, st ? r_.get() : nullptr, st, partials_.get(), stop, s);
+        if (st) k::finish(g, partials_.get(), op, st, nullptr, 0, s);
         return;
     }
     const int L = int(mas_.size()) - 1;
@@ -567,13 +629,15 @@ void HybridSolver::base_apply(const double *r, double scale, const double *add, 
                         L >= 1 ? mas_[0].parent.get() : nullptr, L >= 1 ? mas_[1].w.get() : nullptr, scale, add, z,
                         st ? r_.get() : nullptr, st, partials_.get(), stop, s);
     if (prof) prof_end(s);
+    if (st) k::finish(mas_[0].nbanks, partials_.get(), op, st, nullptr, 0, s);
 }
 
-void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, const int *stop, hipStream_t s)
+void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, const int *stop, hipStream_t s,
+                                 bool first)
 {
     // custom_mixed_precond_iter, GPUHybrid.cu:521-542
     if (!cfg_.do_hybrid_precond || ncomp_ == 0) {
-        base_apply(r, 1.0, nullptr, z, st, stop, s);
+        base_apply(r, 1.0, nullptr, z, st, stop, s, first);
         return;
     }
     base_apply(r, omega_, nullptr, z, nullptr, stop, s);  // z1 = B r
@@ -586,7 +650,7 @@ void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, 
     k::spmv_residual(sell_, z, r, u_.get(), stop, s, cfg_.fp32_inner());
     if (prof) prof_end(s);
     // z = z2 + B u
-    base_apply(u_.get(), omega_, z, z, st, stop, s);
+    base_apply(u_.get(), omega_, z, z, st, stop, s, first);
 }
 
 // ------------------------------------------------------------------ PCG
@@ -642,8 +706,7 @@ void HybridSolver::run_pcg(hipStream_t s)
     const int64_t np = int64_t(nb_) * bs_;
     dev::PcgState *st = st_.get();
     DBuf<double> sc(1);
-    k::dot(np, bp_.get(), bp_.get(), partials_.get(), &st->ticket[0], sc.get(), s);
-    // st->ticket is zero only after the first use; reset the whole state below
+    k::dot(np, bp_.get(), bp_.get(), partials_.get(), sc.get(), s);
     const double bb = d2h_scalar(sc.get(), s);
     if (!(bb > 0.0)) {
         k::fill(np, xp_.get(), 0.0, s);
@@ -662,24 +725,41 @@ void HybridSolver::run_pcg(hipStream_t s)
     prof_spmv_outer_n_ = prof_spmv_inner_n_ = prof_mas0_n_ = 0;
 
     k::spmv_residual(sell_, xp_.get(), bp_.get(), r_.get(), nullptr, s);  // r = b - A x
-    precond_apply(r_.get(), z_.get(), st, nullptr, s);                     // z = M r, gamma = r.z
+    precond_apply(r_.get(), z_.get(), st, nullptr, s, true);               // z = M r, gamma = r.z
     k::copy(np, z_.get(), p_.get(), s);
     int k = 0;
-    bool stopped = false;
-    while (k < cfg_.max_iter && !stopped) {
-        const int batch = std::min(cfg_.check_every, cfg_.max_iter - k);
-        for (int b = 0; b < batch; ++b, ++k) {
-            const bool prof = cfg_.profile_kernels;
-            if (prof) prof_begin(0, s);
-            k::spmv_pq(sell_, p_.get(), q_.get(), st, partials_.get(), k, s);
-            if (prof) prof_end(s);
-            k::pcg_update_xr(np, xp_.get(), r_.get(), p_.get(), q_.get(), st, partials_.get(), s);
-            precond_apply(r_.get(), z_.get(), st, &st->stop, s);
-            k::pcg_update_p(np, p_.get(), z_.get(), st, s);
+    restarts_ = 0;
+    while (true) {
+        bool stopped = false;
+        while (k < cfg_.max_iter && !stopped) {
+            const int batch = std::min(cfg_.check_every, cfg_.max_iter - k);
+            for (int b = 0; b < batch; ++b, ++k) {
+                const bool prof = cfg_.profile_kernels;
+                if (prof) prof_begin(0, s);
+                k::spmv_pq(sell_, p_.get(), q_.get(), st, partials_.get(), k, s);
+                if (prof) prof_end(s);
+                k::pcg_update_xr(np, xp_.get(), r_.get(), p_.get(), q_.get(), st, partials_.get(), s);
+                precond_apply(r_.get(), z_.get(), st, &st->stop, s);
+                k::pcg_update_p(np, p_.get(), z_.get(), st, s);
+            }
+            LSS_HIP(hipMemcpyAsync(h_st_, st, sizeof(dev::PcgState), hipMemcpyDeviceToHost, s));
+            sync(s);
+            stopped = h_st_->stop != 0;
         }
-        LSS_HIP(hipMemcpyAsync(h_st_, st, sizeof(dev::PcgState), hipMemcpyDeviceToHost, s));
-        sync(s);
-        stopped = h_st_->stop != 0;
+        const int code = h_st_->stop;
+        if (!cfg_.true_residual_check || (code != dev::kConvergedRel && code != dev::kConvergedAbs)) break;
+        // The recurrence residual drifts from b - A x on badly conditioned systems;
+        // accept only a true residual inside the tolerance, else restart from it.
+        k::spmv_residual(sell_, xp_.get(), bp_.get(), r_.get(), nullptr, s);
+        k::dot(np, r_.get(), r_.get(), partials_.get(), sc.get(), s);
+        const double rr = d2h_scalar(sc.get(), s);
+        if ((h_st_->rel_eps > 0 && rr / bb < h_st_->rel_eps) || (h_st_->abs_eps > 0 && rr < h_st_->abs_eps)) break;
+        if (k >= cfg_.max_iter || restarts_ >= 50) break;
+        ++restarts_;
+        h_st_->stop = 0;
+        LSS_HIP(hipMemcpyAsync(&st->stop, &h_st_->stop, sizeof(int), hipMemcpyHostToDevice, s));
+        precond_apply(r_.get(), z_.get(), st, nullptr, s, true);
+        k::copy(np, z_.get(), p_.get(), s);
     }
     LSS_HIP(hipMemcpyAsync(h_st_, st, sizeof(dev::PcgState), hipMemcpyDeviceToHost, s));
     sync(s);
@@ -687,7 +767,7 @@ void HybridSolver::run_pcg(hipStream_t s)
     stop_code_ = h_st_->stop;
     // true residual ||A x - b|| (GPUHybrid.cu:383-386)
     k::spmv_residual(sell_, xp_.get(), bp_.get(), q_.get(), nullptr, s);
-    k::dot(np, q_.get(), q_.get(), partials_.get(), &st->ticket[0], sc.get(), s);
+    k::dot(np, q_.get(), q_.get(), partials_.get(), sc.get(), s);
     final_res_norm_ = std::sqrt(d2h_scalar(sc.get(), s));
     if (cfg_.profile_kernels) prof_collect();
 }
@@ -780,11 +860,12 @@ std::string HybridSolver::info_json() const
     for (auto &c : comps_) nst += int64_t(c.size());
     std::ostringstream o;
     o.precision(17);
-    o << "{\"num_iterations\":" << num_iterations_ << ",\"final_res_norm\":" << final_res_norm_ << ",\"solver\":\""
+    o << "{\"num_iterations\":" << num_iterations_ << ",\"final_res_norm\":" << final_res_norm_
+      << ",\"restarts\":" << restarts_ << ",\"solver\":\""
       << name() << "\",\"stop_reason\":\"" << stops[(stop_code_ >= 0 && stop_code_ < 6) ? stop_code_ : 0]
       << "\",\"n\":" << n_ << ",\"nnz\":" << nnz_ << ",\"block_size\":" << bs_ << ",\"nnz_blocks\":" << nnzb_
       << ",\"sell_slots\":" << sell_slots_ << ",\"precision\":\"" << cfg_.precision << "\",\"inner_spmv_fp32\":"
-      << (cfg_.fp32_inner() ? "true" : "false") << ",\"omega\":" << omega_ << ",\"timings\":{\"factorize\":"
+      << (cfg_.fp32_inner() ? "true" : "false") << ",\"omega\":" << omega_ << ",\"lambda_max_BA\":" << lambda_max_ << ",\"timings\":{\"factorize\":"
       << t_factorize_ << ",\"select\":" << t_select_ << ",\"operator\":" << t_operator_ << ",\"partition\":"
       << t_partition_ << ",\"mas\":" << t_mas_ << ",\"stiff\":" << t_stiff_ << ",\"omega\":" << t_omega_
       << ",\"solve\":" << t_solve_ << "},\"stiff\":{\"gmm\":{\"global_mean\":" << gmm_.global_mean

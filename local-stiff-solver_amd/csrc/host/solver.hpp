@@ -81,9 +81,11 @@ private:
     void alloc_vectors();
 
     // preconditioner pieces on permuted vectors
+    // st != nullptr: gamma = r . z is reduced into the PCG state (first: before the loop)
     void base_apply(const double *r, double scale, const double *add, double *z, dev::PcgState *st, const int *stop,
-                    hipStream_t s);
-    void precond_apply(const double *r, double *z, dev::PcgState *st, const int *stop, hipStream_t s);
+                    hipStream_t s, bool first = false);
+    void precond_apply(const double *r, double *z, dev::PcgState *st, const int *stop, hipStream_t s,
+                       bool first = false);
     void run_pcg(hipStream_t s);
 
     k::Precision prec() const;
@@ -130,6 +132,7 @@ private:
     DBuf<int32_t> d_pdofs_;
     DBuf<double> d_L_, d_swork_;
     double omega_ = 1.0;
+    double lambda_max_ = 0.0;
 
     // PCG
     DBuf<double> xp_, bp_, r_, p_, q_, z_, u_;
@@ -142,6 +145,7 @@ private:
     int num_iterations_ = 0;
     double final_res_norm_ = 0.0;
     int stop_code_ = 0;
+    int restarts_ = 0;
     double t_factorize_ = 0, t_select_ = 0, t_operator_ = 0, t_partition_ = 0, t_mas_ = 0, t_stiff_ = 0,
            t_omega_ = 0, t_solve_ = 0;
     int mas_levels_ = 0;

@@ -6,49 +6,94 @@
 
 namespace lss {
 
-void partition_graph(const Pattern &g, int max_part, std::vector<int32_t> &part_of, int32_t &nparts)
+namespace {
+
+// One bisection step of the recursive partitioner: order the nodes of `set`
+// (all labelled `label` in lab[]) by breadth-first search inside the subgraph,
+// starting from `start`, continuing from the smallest remaining node when the
+// subgraph is disconnected; the first `left` nodes form the left half.
+void bfs_order(const Pattern &g, const std::vector<int32_t> &set, int32_t start, int32_t label,
+               std::vector<int32_t> &lab, std::vector<int32_t> &order)
 {
-    const int32_t n = g.n;
-    part_of.assign(n, -1);
-    std::vector<int32_t> gain(n, 0);
-    std::vector<int32_t> frontier;
-    frontier.reserve(1024);
-    int32_t p = 0, seed = 0;
-    while (true) {
-        while (seed < n && part_of[seed] != -1) ++seed;
-        if (seed >= n) break;
-        frontier.clear();
-        int size = 0;
-        int32_t u = seed;
-        while (true) {
-            part_of[u] = p;
-            ++size;
+    order.clear();
+    order.reserve(set.size());
+    const int32_t done = -2 - label;  // visited marker, distinct per subproblem
+    auto visit_from = [&](int32_t s0) {
+        size_t head = order.size();
+        lab[s0] = done;
+        order.push_back(s0);
+        while (head < order.size()) {
+            const int32_t u = order[head++];
             for (int32_t e = g.rowptr[u]; e < g.rowptr[u + 1]; ++e) {
                 const int32_t w = g.cols[e];
-                if (w == u || part_of[w] != -1) continue;
-                if (gain[w] == 0) frontier.push_back(w);
-                ++gain[w];
-            }
-            if (size >= max_part) break;
-            int best = -1;
-            int32_t bw = 0, bg = 0;
-            for (int f = 0; f < int(frontier.size()); ++f) {
-                const int32_t w = frontier[f];
-                if (part_of[w] != -1) continue;
-                const int32_t gw = gain[w];
-                if (best < 0 || gw > bg || (gw == bg && w < bw)) {
-                    best = f;
-                    bw = w;
-                    bg = gw;
+                if (lab[w] == label) {
+                    lab[w] = done;
+                    order.push_back(w);
                 }
             }
-            if (best < 0) break;
-            u = bw;
         }
-        for (int32_t w : frontier) gain[w] = 0;
-        ++p;
+    };
+    visit_from(start);
+    if (order.size() < set.size())
+        for (int32_t v : set)  // set is ascending
+            if (lab[v] == label) visit_from(v);
+}
+
+void bisect(const Pattern &g, std::vector<int32_t> set, int32_t start, int max_part, int32_t part_base,
+            int32_t label, std::vector<int32_t> &lab, std::vector<int32_t> &part_of, int depth)
+{
+    const int64_t m = int64_t(set.size());
+    if (m <= max_part) {
+        for (int32_t v : set) part_of[v] = part_base;
+        return;
     }
-    nparts = p;
+    const int64_t k = (m + max_part - 1) / max_part;  // parts in this subtree
+    const int64_t kl = (k + 1) / 2;
+    const int64_t nl = kl * max_part;                   // left half holds kl full parts
+    for (int32_t v : set) lab[v] = label;
+    std::vector<int32_t> order;
+    bfs_order(g, set, start, label, lab, order);
+    std::vector<int32_t> L(order.begin(), order.begin() + nl), R(order.begin() + nl, order.end());
+    // the right half restarts from the node farthest from `start`, the left from `start`
+    const int32_t rs = order.back();
+    std::sort(L.begin(), L.end());
+    std::sort(R.begin(), R.end());
+    const int32_t lbl_l = 2 * label + 1, lbl_r = 2 * label + 2;
+    if (depth < 12) {
+#pragma omp task default(none) firstprivate(L, start, max_part, part_base, lbl_l, depth) shared(g, lab, part_of)
+        bisect(g, std::move(L), start, max_part, part_base, lbl_l, lab, part_of, depth + 1);
+#pragma omp task default(none) firstprivate(R, rs, max_part, part_base, kl, lbl_r, depth) shared(g, lab, part_of)
+        bisect(g, std::move(R), rs, max_part, int32_t(part_base + kl), lbl_r, lab, part_of, depth + 1);
+#pragma omp taskwait
+    } else {
+        bisect(g, std::move(L), start, max_part, part_base, lbl_l, lab, part_of, depth + 1);
+        bisect(g, std::move(R), rs, max_part, int32_t(part_base + kl), lbl_r, lab, part_of, depth + 1);
+    }
+}
+
+}  // namespace
+
+void partition_graph(const Pattern &g, int max_part, std::vector<int32_t> &part_of, int32_t &nparts)
+{
+    // Recursive breadth-first bisection into parts of max_part nodes (the last
+    // part of a subtree may be smaller).  Part ids follow the recursion from
+    // left to right, so every run of consecutive parts is one subtree: compact
+    // at every scale, which is what the bank-local coarse levels of MAS need.
+    const int32_t n = g.n;
+    part_of.assign(n, 0);
+    nparts = int32_t((int64_t(n) + max_part - 1) / max_part);
+    if (n == 0) return;
+    std::vector<int32_t> all(n);
+    std::iota(all.begin(), all.end(), 0);
+    // pseudo-peripheral start: the last node reached by a search from node 0
+    std::vector<int32_t> lab(n, 0), order;
+    bfs_order(g, all, 0, 0, lab, order);
+    const int32_t start = order.back();
+    // labels: subproblem ids (heap numbering); visited markers are negative
+    lab.assign(n, 0);
+#pragma omp parallel
+#pragma omp single
+    bisect(g, std::move(all), start, max_part, 0, 0, lab, part_of, 0);
 }
 
 void partition_permutation(const std::vector<int32_t> &part_of, int32_t nparts, std::vector<int32_t> &perm,

@@ -13,9 +13,9 @@ struct Pattern {
     std::vector<int32_t> cols;    // ascending per row
 };
 
-// Greedy graph-growing partition into parts of at most max_part nodes: seed =
-// smallest unassigned node, grow by the frontier node with the most edges into
-// the part (ties: smallest id).  Replaces KaMinPar (MAS/GraphPartition.cpp:176-289).
+// Recursive breadth-first bisection into ceil(n / max_part) parts of at most
+// max_part nodes, numbered in recursion order (consecutive parts are close at
+// every scale).  Replaces KaMinPar (MAS/GraphPartition.cpp:176-289).
 void partition_graph(const Pattern &g, int max_part, std::vector<int32_t> &part_of, int32_t &nparts);
 
 // perm[old] = new with nodes sorted by part, stable in node id (MASSolver.cu:90-118).

@@ -1,9 +1,10 @@
 // BLAS-1 and PCG kernels with device-resident scalars.
 //
 // The reference keeps alpha/beta/gamma on the host and synchronizes on every
-// dot product (GPUHybrid.cu:1455-1511); here each reduction ends in the kernel
-// itself (deterministic last-workgroup sum) and the scalar update of
-// GPUHybrid.cu:1461-1507 runs there, so the PCG loop issues no host sync.
+// dot product (GPUHybrid.cu:1455-1511); here every reduction ends in a
+// one-workgroup finisher kernel that adds the per-workgroup partials in index
+// order (deterministic) and applies the scalar update of GPUHybrid.cu:1461-1507
+// on the device, so the PCG loop issues no host sync.
 #include "kernels.hpp"
 #include "../common/runtime.hpp"
 
@@ -15,14 +16,64 @@ constexpr int kT = 256;
 inline int red_grid(int64_t n) { return int(std::min<int64_t>(std::max<int64_t>(div_up(n, kT * 4), 1), 4096)); }
 
 __global__ __launch_bounds__(kT) void k_dot(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
-                                            double *partials, unsigned *ticket, double *out)
+                                            double *partials)
 {
     __shared__ double sh[kT / 64];
     double v = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT) v += a[i] * b[i];
-    const double part = dev::block_sum<kT>(v, sh);
-    double total;
-    if (dev::grid_sum_last<kT>(part, partials, ticket, sh, total) && threadIdx.x == 0) *out = total;
+    dev::store_partial(dev::block_sum<kT>(v, sh), partials);
+}
+
+constexpr int kFinT = 1024;
+
+// Sum `count` partials in index order and apply the scalar update of `op`.
+__global__ __launch_bounds__(kFinT) void k_finish(int count, const double *__restrict__ partials, int op,
+                                                  dev::PcgState *st, double *out, int iter)
+{
+    if (op != dev::kFinishStore && st->stop) return;
+    __shared__ double sh[kFinT / 64];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < count; i += kFinT) v += partials[i];
+    const double total = dev::block_sum<kFinT>(v, sh);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+        case dev::kFinishStore: *out = total; break;
+        case dev::kFinishPq: {
+            // GPUHybrid.cu:1458-1480
+            st->iters = iter + 1;
+            st->pq = total;
+            if (total == 0.0) {
+                st->stop = dev::kZeroSdotp;
+            } else {
+                const double alpha = st->gamma / total;
+                st->alpha = alpha;
+                if (alpha <= 0.0)
+                    st->stop = dev::kNegativeAlpha;
+                else if (alpha < 2.2250738585072014e-308)
+                    st->stop = dev::kSubnormalAlpha;
+            }
+            break;
+        }
+        case dev::kFinishRr:
+            // GPUHybrid.cu:1484-1496
+            st->rr = total;
+            if (st->rel_eps > 0 && (total / st->bb) < st->rel_eps)
+                st->stop = dev::kConvergedRel;
+            else if (st->abs_eps > 0 && total < st->abs_eps)
+                st->stop = dev::kConvergedAbs;
+            break;
+        case dev::kFinishRzFirst:
+            // GPUHybrid.cu:1450-1451
+            st->gamma = total;
+            st->gamma_old = total;
+            break;
+        case dev::kFinishRz:
+            // GPUHybrid.cu:1501-1504
+            st->gamma = total;
+            st->beta = total / st->gamma_old;
+            st->gamma_old = total;
+            break;
+    }
 }
 
 __global__ __launch_bounds__(kT) void k_update_xr(int64_t n, double *__restrict__ x, double *__restrict__ r,
@@ -39,37 +90,7 @@ __global__ __launch_bounds__(kT) void k_update_xr(int64_t n, double *__restrict_
         r[i] = ri;
         v += ri * ri;
     }
-    const double part = dev::block_sum<kT>(v, sh);
-    double total;
-    if (dev::grid_sum_last<kT>(part, partials, &st->ticket[0], sh, total) && threadIdx.x == 0) {
-        // GPUHybrid.cu:1484-1496
-        st->rr = total;
-        if (st->rel_eps > 0 && (total / st->bb) < st->rel_eps)
-            st->stop = dev::kConvergedRel;
-        else if (st->abs_eps > 0 && total < st->abs_eps)
-            st->stop = dev::kConvergedAbs;
-    }
-}
-
-__global__ __launch_bounds__(kT) void k_rz(int64_t n, const double *__restrict__ r, const double *__restrict__ z,
-                                           dev::PcgState *st, double *partials, int first)
-{
-    if (st->stop) return;
-    __shared__ double sh[kT / 64];
-    double v = 0.0;
-    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT) v += r[i] * z[i];
-    const double part = dev::block_sum<kT>(v, sh);
-    double total;
-    if (dev::grid_sum_last<kT>(part, partials, &st->ticket[0], sh, total) && threadIdx.x == 0) {
-        // GPUHybrid.cu:1450-1451 (first) and 1501-1504
-        st->gamma = total;
-        if (first) {
-            st->gamma_old = total;
-        } else {
-            st->beta = total / st->gamma_old;
-            st->gamma_old = total;
-        }
-    }
+    dev::store_partial(dev::block_sum<kT>(v, sh), partials);
 }
 
 __global__ __launch_bounds__(kT) void k_update_p(int64_t n, double *__restrict__ p, const double *__restrict__ z,
@@ -92,6 +113,16 @@ __global__ void k_fill(int64_t n, double *__restrict__ a, double v)
 __global__ void k_scale(int64_t n, double *__restrict__ a, double v)
 {
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT) a[i] *= v;
+}
+
+__global__ void k_axpy(int64_t n, double a, const double *__restrict__ x, double *__restrict__ y)
+{
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT) y[i] += a * x[i];
+}
+__global__ void k_xpby(int64_t n, const double *__restrict__ x, double b, double *__restrict__ y)
+{
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT)
+        y[i] = x[i] + b * y[i];
 }
 
 __global__ void k_permute_in(int32_t nb, int bs, int64_t n, const int32_t *__restrict__ perm,
@@ -118,24 +149,28 @@ inline int ew_grid(int64_t n) { return int(std::min<int64_t>(std::max<int64_t>(d
 
 }  // namespace
 
-void dot(int64_t n, const double *a, const double *b, double *partials, unsigned *ticket, double *out, hipStream_t s)
+void finish(int count, const double *partials, int op, dev::PcgState *st, double *out, int iter, hipStream_t s)
 {
-    k_dot<<<red_grid(n), kT, 0, s>>>(n, a, b, partials, ticket, out);
+    if (count <= 0) return;
+    k_finish<<<1, kFinT, 0, s>>>(count, partials, op, st, out, iter);
     LSS_LAUNCH_CHECK();
+}
+
+void dot(int64_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s)
+{
+    const int g = red_grid(n);
+    k_dot<<<g, kT, 0, s>>>(n, a, b, partials);
+    LSS_LAUNCH_CHECK();
+    finish(g, partials, dev::kFinishStore, nullptr, out, 0, s);
 }
 
 void pcg_update_xr(int64_t n, double *x, double *r, const double *p, const double *q, dev::PcgState *st,
                    double *partials, hipStream_t s)
 {
-    k_update_xr<<<red_grid(n), kT, 0, s>>>(n, x, r, p, q, st, partials);
+    const int g = red_grid(n);
+    k_update_xr<<<g, kT, 0, s>>>(n, x, r, p, q, st, partials);
     LSS_LAUNCH_CHECK();
-}
-
-void pcg_rz(int64_t n, const double *r, const double *z, dev::PcgState *st, double *partials, bool first,
-            hipStream_t s)
-{
-    k_rz<<<red_grid(n), kT, 0, s>>>(n, r, z, st, partials, first ? 1 : 0);
-    LSS_LAUNCH_CHECK();
+    finish(g, partials, dev::kFinishRr, st, nullptr, 0, s);
 }
 
 void pcg_update_p(int64_t n, double *p, const double *z, const dev::PcgState *st, hipStream_t s)
@@ -160,6 +195,19 @@ void scale(int64_t n, double *a, double v, hipStream_t s)
 {
     if (n <= 0) return;
     k_scale<<<ew_grid(n), kT, 0, s>>>(n, a, v);
+    LSS_LAUNCH_CHECK();
+}
+
+void axpy(int64_t n, double a, const double *x, double *y, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_axpy<<<ew_grid(n), kT, 0, s>>>(n, a, x, y);
+    LSS_LAUNCH_CHECK();
+}
+void xpby(int64_t n, const double *x, double b, double *y, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_xpby<<<ew_grid(n), kT, 0, s>>>(n, x, b, y);
     LSS_LAUNCH_CHECK();
 }
 

@@ -70,33 +70,23 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
     return r;
 }
 
-__device__ __forceinline__ double load_acquire(const double *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Grid reductions are split in two launches: every workgroup of the producer
+// kernel stores its partial (plain store), and a one-workgroup finisher kernel
+// adds the partials in index order and applies the scalar update.  The kernel
+// boundary makes the partials visible; a per-workgroup agent-scope release
+// (needed by a single-kernel "last workgroup" scheme) writes back the XCD's L2
+// on gfx950 and costs more than the extra launch.
+enum FinishOp : int {
+    kFinishStore = 0,  // *out = total
+    kFinishPq = 1,     // p.Ap: alpha, breakdown checks, iteration count
+    kFinishRr = 2,     // r.r: convergence test
+    kFinishRzFirst = 3,  // r.z before the loop: gamma = gamma_old
+    kFinishRz = 4,     // r.z in the loop: beta
+};
 
-// Deterministic grid-wide sum: every workgroup publishes its partial, the last
-// one to arrive (by ticket) adds the partials in index order.  Returns true in
-// the last workgroup, where *total (thread 0) holds the sum.  The ticket is
-// reset by the last workgroup so the kernel can be launched again.
-template <int NT>
-__device__ __forceinline__ bool grid_sum_last(double partial, double *partials, unsigned *ticket, double *sh,
-                                              double &total)
+__device__ __forceinline__ void store_partial(double block_total, double *partials)
 {
-    __shared__ bool is_last;
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[blockIdx.x], partial, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
-    double v = 0.0;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += NT) v += load_acquire(&partials[i]);
-    total = block_sum<NT>(v, sh);
-    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
+    if (threadIdx.x == 0) partials[blockIdx.x] = block_total;
 }
 
 }  // namespace lss::dev

@@ -35,9 +35,9 @@ struct SellView {
     const float *vals32 = nullptr;       // optional FP32 copy
 };
 
-// y = A x (sub == nullptr) or y = sub - A x.  If dot_w != nullptr a
-// deterministic grid sum of dot_w . y goes to *dot_out via the ticket reduction.
-// If stop != nullptr and *stop != 0 the kernel does nothing.
+// y = A x (sub == nullptr) or y = sub - A x.  spmv_pq also reduces p . A p and
+// updates alpha / the breakdown flags of the PCG state.  With stop != nullptr and
+// *stop != 0 the kernels do nothing.
 void spmv(const SellView &A, const double *x, double *y, const double *sub, hipStream_t s, bool fp32 = false);
 void spmv_pq(const SellView &A, const double *p, double *q, dev::PcgState *st, double *partials, int iter,
              hipStream_t s);
@@ -50,18 +50,19 @@ constexpr int kCsumLanes = 256;
 // Canonical-order sums used by the stiff DOF selection.
 void canonical_sum(int64_t n, const double *v, double *partials, double *out, hipStream_t s);
 
-void dot(int64_t n, const double *a, const double *b, double *partials, unsigned *ticket, double *out,
-         hipStream_t s);
+// Sum count partials in index order and apply the scalar update `op` (dev::FinishOp).
+void finish(int count, const double *partials, int op, dev::PcgState *st, double *out, int iter, hipStream_t s);
+void dot(int64_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s);
 // x += alpha p; r -= alpha q; rr = r.r; convergence test.
 void pcg_update_xr(int64_t n, double *x, double *r, const double *p, const double *q, dev::PcgState *st,
                    double *partials, hipStream_t s);
-// gamma = r.z; beta = gamma / gamma_old; p = beta p + z.
-void pcg_rz(int64_t n, const double *r, const double *z, dev::PcgState *st, double *partials, bool first,
-            hipStream_t s);
+// p = beta p + z.
 void pcg_update_p(int64_t n, double *p, const double *z, const dev::PcgState *st, hipStream_t s);
 void copy(int64_t n, const double *a, double *b, hipStream_t s);
 void fill(int64_t n, double *a, double v, hipStream_t s);
 void scale(int64_t n, double *a, double v, hipStream_t s);
+void axpy(int64_t n, double a, const double *x, double *y, hipStream_t s);  // y += a x
+void xpby(int64_t n, const double *x, double b, double *y, hipStream_t s);  // y = x + b y
 // permuted[i] = src[perm-mapped], for node permutation with block size bs
 void permute_in(int32_t nb, int bs, int64_t n, const int32_t *perm, const double *src, double *dst, hipStream_t s);
 void permute_out(int32_t nb, int bs, int64_t n, const int32_t *perm, const double *src, double *dst, hipStream_t s);

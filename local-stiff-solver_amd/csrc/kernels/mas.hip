@@ -3,8 +3,14 @@
 // Semantics follow MAS/MASPreconditioner.cu:33-1164: level-0 banks are the
 // graph-partition parts (<= 32 nodes); coarse level l+1 nodes are the bank-local
 // connected components of level l; every bank matrix (dim 32*bs, zero diagonal
-// entries set to 1) is inverted and stored packed-upper; the apply is
+// entries set to 1) is inverted; the apply is
 //     z = sum_l R_l^T  A_{l,bank}^{-1}  R_l r.
+// The reference stores A_b^{-1} packed-upper from a Gauss-Jordan sweep
+// (799-936); here each bank keeps its inverse Cholesky factor G = L^{-1}
+// (A_b = L L^T), packed lower, the same number of values, and the apply computes
+// G^T (G x).  The bank operator is then SPD by construction in every storage
+// precision, where an explicit inverse of a bank with condition number 1e10-1e11
+// (contact stiffness) can lose definiteness already in FP64.
 // Layout and schedule are this build's own:
 //  * parts are contiguous ranges of the partition-permuted numbering, so the
 //    level-0 gather/scatter of the reference (padded copy, 593-659 / 745-777)
@@ -42,6 +48,11 @@ __device__ __forceinline__ __hip_bfloat16 to_store<__hip_bfloat16>(double v)
 }
 
 template <class T>
+__device__ __forceinline__ double to_lds_any(T v) { return double(v); }
+template <>
+__device__ __forceinline__ double to_lds_any<__hip_bfloat16>(__hip_bfloat16 v) { return double(__bfloat162float(v)); }
+
+template <class T>
 __device__ __forceinline__ float to_lds(T v);
 template <>
 __device__ __forceinline__ float to_lds<float>(float v) { return v; }
@@ -50,9 +61,16 @@ __device__ __forceinline__ float to_lds<__hip_bfloat16>(__hip_bfloat16 v) { retu
 
 constexpr int kInvThreads = 256;
 
-// Assemble bank matrices of one level from its block operator and invert them.
+__device__ __forceinline__ int packed_lower(int i, int j)
+{
+    // row-major packed lower triangle, j <= i
+    return (i * (i + 1)) / 2 + j;
+}
+
+// Assemble the bank matrices of one level from its block operator, factor
+// A_b = L L^T and invert L in LDS; store G = L^{-1} packed lower.
 template <int BS, class TS>
-__global__ __launch_bounds__(kInvThreads) void k_bank_invert(int32_t nbanks, const int32_t *__restrict__ bank_off,
+__global__ __launch_bounds__(kInvThreads) void k_bank_factor(int32_t nbanks, const int32_t *__restrict__ bank_off,
                                                              const int32_t *__restrict__ rowptr,
                                                              const int32_t *__restrict__ cols,
                                                              const double *__restrict__ vals, TS *__restrict__ out,
@@ -62,12 +80,13 @@ __global__ __launch_bounds__(kInvThreads) void k_bank_invert(int32_t nbanks, con
     constexpr int S = D * (D + 1) / 2;
     constexpr int BB = BS * BS;
     extern __shared__ double lds[];
-    double *M = lds;          // D x D row-major
-    double *colk = lds + D * D;
-    __shared__ double inv_piv;
+    double *M = lds;         // D x D row-major, lower triangle used
+    double *tmp = lds + D * D;
+    __shared__ int bad;
     const int32_t bank = blockIdx.x;
     if (bank >= nbanks) return;
     const int32_t lo = bank_off[bank], hi = bank_off[bank + 1];
+    if (threadIdx.x == 0) bad = 0;
     for (int t = threadIdx.x; t < D * D; t += kInvThreads) M[t] = 0.0;
     __syncthreads();
     // assemble: block (i, J) with both nodes in the bank
@@ -88,46 +107,51 @@ __global__ __launch_bounds__(kInvThreads) void k_bank_invert(int32_t nbanks, con
     for (int d = threadIdx.x; d < D; d += kInvThreads)
         if (M[d * D + d] == 0.0) M[d * D + d] = 1.0;
     __syncthreads();
-    // in-place Gauss-Jordan inverse without pivoting (SPD)
-    bool ok = true;
+    // Cholesky, right-looking, lower triangle in place
     for (int k = 0; k < D; ++k) {
         if (threadIdx.x == 0) {
-            const double piv = M[k * D + k];
-            if (!(piv != 0.0) || !isfinite(piv)) ok = false;
-            inv_piv = 1.0 / piv;
+            const double d = M[k * D + k];
+            if (!(d > 0.0) || !isfinite(d)) bad = 1;
+            M[k * D + k] = sqrt(d);
         }
         __syncthreads();
-        const double ip = inv_piv;
-        for (int t = threadIdx.x; t < D; t += kInvThreads) {
-            colk[t] = M[t * D + k];
-            if (t != k) M[k * D + t] *= ip;
-        }
+        const double dk = M[k * D + k];
+        for (int i = k + 1 + threadIdx.x; i < D; i += kInvThreads) M[i * D + k] /= dk;
         __syncthreads();
-        for (int t = threadIdx.x; t < D * D; t += kInvThreads) {
-            const int i = t / D, j = t - i * D;
-            if (i == k) continue;
-            if (j == k)
-                M[t] = -colk[i] * ip;
-            else
-                M[t] -= colk[i] * M[k * D + j];
+        const int T = D - k - 1;
+        for (int t = threadIdx.x; t < T * T; t += kInvThreads) {
+            const int i = k + 1 + t / T, j = k + 1 + t % T;
+            if (j <= i) M[i * D + j] -= M[i * D + k] * M[j * D + k];
         }
-        __syncthreads();
-        if (threadIdx.x == 0) M[k * D + k] = ip;
         __syncthreads();
     }
-    // packed upper, symmetrized
+    // G = L^{-1} in place, columns from the last to the first (LAPACK trti2, lower)
+    for (int j = D - 1; j >= 0; --j) {
+        if (threadIdx.x == 0) M[j * D + j] = 1.0 / M[j * D + j];
+        __syncthreads();
+        const double ajj = -M[j * D + j];
+        // x(j+1:D) := G(j+1:D, j+1:D) * L(j+1:D, j), then scale by ajj
+        for (int i = j + 1 + threadIdx.x; i < D; i += kInvThreads) {
+            double acc = 0.0;
+            for (int q = j + 1; q <= i; ++q) acc += M[i * D + q] * M[q * D + j];
+            tmp[i] = acc * ajj;
+        }
+        __syncthreads();
+        for (int i = j + 1 + threadIdx.x; i < D; i += kInvThreads) M[i * D + j] = tmp[i];
+        __syncthreads();
+    }
     TS *dst = out + int64_t(bank) * S;
+    bool ok = true;
     for (int t = threadIdx.x; t < S; t += kInvThreads) {
-        // invert packed index: row i with i*D - i(i-1)/2 <= t
-        int i = int((2.0 * D + 1.0 - sqrt((2.0 * D + 1.0) * (2.0 * D + 1.0) - 8.0 * t)) * 0.5);
-        while (i > 0 && packed_index(D, i, i) > t) --i;
-        while (i + 1 < D && packed_index(D, i + 1, i + 1) <= t) ++i;
-        const int j = t - packed_index(D, i, i) + i;
-        const double v = 0.5 * (M[i * D + j] + M[j * D + i]);
+        int i = int((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while (i > 0 && packed_lower(i, 0) > t) --i;
+        while (i + 1 < D && packed_lower(i + 1, 0) <= t) ++i;
+        const int j = t - packed_lower(i, 0);
+        const double v = M[i * D + j];
         if (!isfinite(v)) ok = false;
         dst[t] = to_store<TS>(v);
     }
-    if (!ok) atomicOr(status, 1);
+    if (!ok || (threadIdx.x == 0 && bad)) atomicOr(status, 1);
 }
 
 // Galerkin coarse operator values: target slot t sums its source blocks in order.
@@ -159,17 +183,26 @@ __global__ void k_restrict(int32_t nodes, int bs, const int64_t *__restrict__ ch
     coarse[g] = s;
 }
 
-// Packed symmetric matvec out of LDS: thread i (< D) returns row i of M x.
-template <int D, class TL>
-__device__ __forceinline__ double sym_row(const TL *__restrict__ A, const double *__restrict__ x, int i)
+// y = G^T (G x) with G packed lower in LDS.  Phase 1: thread i forms
+// t_i = G(i, 0:i) x (row i, contiguous); phase 2: thread j forms
+// y_j = G(j:D, j)^T t (column j: consecutive addresses across threads).
+template <int D, int NT, class TL>
+__device__ __forceinline__ void bank_apply(const TL *__restrict__ G, const double *__restrict__ x,
+                                           double *__restrict__ t, double *__restrict__ y, int rows)
 {
-    double acc = 0.0;
-    // j < i: column i of the upper triangle, consecutive across threads
-    for (int j = 0; j < i; ++j) acc += double(A[packed_index(D, j, j) + (i - j)]) * x[j];
-    // j >= i: row i of the upper triangle, contiguous
-    const TL *row = A + packed_index(D, i, i) - i;
-    for (int j = i; j < D; ++j) acc += double(row[j]) * x[j];
-    return acc;
+    for (int i = threadIdx.x; i < rows; i += NT) {
+        const TL *g = G + packed_lower(i, 0);
+        double acc = 0.0;
+        for (int j = 0; j <= i; ++j) acc += double(g[j]) * x[j];
+        t[i] = acc;
+    }
+    for (int i = rows + threadIdx.x; i < D; i += NT) t[i] = 0.0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < rows; j += NT) {
+        double acc = 0.0;
+        for (int i = j; i < rows; ++i) acc += double(G[packed_lower(i, j)]) * t[i];
+        y[j] = acc;
+    }
 }
 
 template <class TS>
@@ -185,25 +218,36 @@ struct LdsType<__hip_bfloat16> {
     using type = float;
 };
 
-template <class TS, class TL>
-__device__ __forceinline__ void stage_matrix(const TS *__restrict__ src, TL *__restrict__ dst, int S, int nthreads)
+// Stage one bank's packed factor into LDS: every thread issues all of its
+// 16-byte global loads (non-temporal: each factor is read once per apply) before
+// the first LDS store, so a workgroup keeps the whole bank in flight.
+template <class TS, class TL, int S, int NT>
+__device__ __forceinline__ void stage_matrix(const TS *__restrict__ src, TL *__restrict__ dst)
 {
-    if constexpr (std::is_same<TS, double>::value) {
-        const double2 *s2 = reinterpret_cast<const double2 *>(src);
-        double2 *d2 = reinterpret_cast<double2 *>(dst);
-        for (int t = threadIdx.x; t < S / 2; t += nthreads) d2[t] = s2[t];
-        if ((S & 1) && threadIdx.x == 0) dst[S - 1] = src[S - 1];
-    } else if constexpr (std::is_same<TS, float>::value) {
-        const float4 *s4 = reinterpret_cast<const float4 *>(src);
-        float4 *d4 = reinterpret_cast<float4 *>(dst);
-        for (int t = threadIdx.x; t < S / 4; t += nthreads) d4[t] = s4[t];
-        for (int t = (S / 4) * 4 + threadIdx.x; t < S; t += nthreads) dst[t] = src[t];
-    } else {
-        for (int t = threadIdx.x; t < S; t += nthreads) dst[t] = to_lds<TS>(src[t]);
+    constexpr int VEC = 16 / sizeof(TS);           // elements per 16-byte load
+    constexpr int NV = S / VEC;                    // full vectors
+    constexpr int PER = (NV + NT - 1) / NT;
+    typedef unsigned int V __attribute__((ext_vector_type(4)));
+    const V *sv = reinterpret_cast<const V *>(src);
+    V reg[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int t = threadIdx.x + q * NT;
+        if (t < NV) reg[q] = __builtin_nontemporal_load(&sv[t]);
     }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int t = threadIdx.x + q * NT;
+        if (t < NV) {
+            const TS *e = reinterpret_cast<const TS *>(&reg[q]);
+#pragma unroll
+            for (int u = 0; u < VEC; ++u) dst[t * VEC + u] = static_cast<TL>(to_lds_any<TS>(e[u]));
+        }
+    }
+    for (int t = NV * VEC + threadIdx.x; t < S; t += NT) dst[t] = static_cast<TL>(to_lds_any<TS>(src[t]));
 }
 
-constexpr int kApplyThreads = 128;
+constexpr int kApplyThreads = 256;
 
 // Coarse level: y = A^{-1} r per bank of 32 nodes; w = y + w_parent[parent].
 template <int BS, class TS>
@@ -218,15 +262,17 @@ __global__ __launch_bounds__(kApplyThreads) void k_coarse_apply(int32_t nodes, c
     using TL = typename LdsType<TS>::type;
     if (stop && *stop) return;
     __shared__ __attribute__((aligned(16))) TL A[S + 4];
-    __shared__ double x[D];
+    __shared__ double x[D], tv[D], yv[D];
     const int32_t bank = blockIdx.x;
     const int32_t lo = bank * 32;
     const int32_t cnt = min(32, nodes - lo);
-    stage_matrix<TS, TL>(inv + int64_t(bank) * S, A, S, kApplyThreads);
+    stage_matrix<TS, TL, S, kApplyThreads>(inv + int64_t(bank) * S, A);
     for (int t = threadIdx.x; t < D; t += kApplyThreads) x[t] = (t < cnt * BS) ? r[int64_t(lo) * BS + t] : 0.0;
     __syncthreads();
+    // rows beyond the bank's nodes are identity padding: they do not couple to real rows
+    bank_apply<D, kApplyThreads, TL>(A, x, tv, yv, cnt * BS);
     for (int i = threadIdx.x; i < cnt * BS; i += kApplyThreads) {
-        double y = sym_row<D, TL>(A, x, i);
+        double y = yv[i];
         if (w_parent) {
             const int32_t node = lo + i / BS;
             y += w_parent[int64_t(parent[node]) * BS + (i % BS)];
@@ -251,16 +297,17 @@ __global__ __launch_bounds__(kApplyThreads) void k_level0_apply(int32_t nbanks, 
     using TL = typename LdsType<TS>::type;
     if (stop && *stop) return;
     __shared__ __attribute__((aligned(16))) TL A[S + 4];
-    __shared__ double x[D];
+    __shared__ double x[D], tv[D], yv[D];
     const int32_t bank = blockIdx.x;
     const int32_t lo = part_off[bank];
     const int32_t cnt = part_off[bank + 1] - lo;
-    stage_matrix<TS, TL>(inv + int64_t(bank) * S, A, S, kApplyThreads);
+    stage_matrix<TS, TL, S, kApplyThreads>(inv + int64_t(bank) * S, A);
     for (int t = threadIdx.x; t < D; t += kApplyThreads) x[t] = (t < cnt * BS) ? r[int64_t(lo) * BS + t] : 0.0;
     __syncthreads();
+    bank_apply<D, kApplyThreads, TL>(A, x, tv, yv, cnt * BS);
     double part = 0.0;
     for (int i = threadIdx.x; i < cnt * BS; i += kApplyThreads) {
-        double y = sym_row<D, TL>(A, x, i);
+        double y = yv[i];
         const int64_t gi = int64_t(lo) * BS + i;
         if (w1) y += w1[int64_t(parent[lo + i / BS]) * BS + (i % BS)];
         y *= scale;
@@ -270,14 +317,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_level0_apply(int32_t nbanks, 
     }
     if constexpr (DOT) {
         __shared__ double sh[kApplyThreads / 64];
-        const double p = dev::block_sum<kApplyThreads>(part, sh);
-        double total;
-        if (dev::grid_sum_last<kApplyThreads>(p, partials, &st->ticket[0], sh, total) && threadIdx.x == 0) {
-            // gamma = r . z; beta = gamma / gamma_old (GPUHybrid.cu:1501-1504)
-            st->gamma = total;
-            st->beta = total / st->gamma_old;
-            st->gamma_old = total;
-        }
+        dev::store_partial(dev::block_sum<kApplyThreads>(part, sh), partials);
     }
 }
 
@@ -290,13 +330,13 @@ void launch_bank_invert(int bs, int32_t nbanks, const int32_t *bank_off, const i
     const size_t lds = size_t(D * D + D) * sizeof(double);
     switch (bs) {
         case 1:
-            k_bank_invert<1, TS><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out, status);
+            k_bank_factor<1, TS><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out, status);
             break;
         case 2:
-            k_bank_invert<2, TS><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out, status);
+            k_bank_factor<2, TS><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out, status);
             break;
         case 3:
-            k_bank_invert<3, TS><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out, status);
+            k_bank_factor<3, TS><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out, status);
             break;
         default: fail(LSS_ERR_UNSUPPORTED, "block_size must be 1, 2 or 3");
     }
@@ -310,11 +350,11 @@ void mas_set_lds_limits()
     static bool done = false;
     if (done) return;
     const size_t lds3 = size_t(96 * 96 + 96) * sizeof(double);
-    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_invert<3, double>),
+    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, double>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
-    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_invert<3, float>),
+    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, float>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
-    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_invert<3, __hip_bfloat16>),
+    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, __hip_bfloat16>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
     done = true;
 }

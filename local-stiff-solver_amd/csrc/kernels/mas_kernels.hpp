@@ -25,7 +25,7 @@ void mas_restrict(int32_t nodes, int bs, const int64_t *child_ptr, const int32_t
                   double *coarse, const int *stop, hipStream_t s);
 void mas_coarse_apply(Precision prec, int bs, int32_t nodes, const void *inv, const double *r, const int32_t *parent,
                       const double *w_parent, double *w, const int *stop, hipStream_t s);
-// dotv != nullptr with st: gamma = dotv . z is reduced into the PCG state.
+// st != nullptr: per-bank partials of dotv . z go to partials[bank] (reduced by k::finish).
 void mas_level0_apply(Precision prec, int bs, int32_t nbanks, const int32_t *part_off, const void *inv,
                       const double *r, const int32_t *parent, const double *w1, double scale, const double *add,
                       double *z, const double *dotv, dev::PcgState *st, double *partials, const int *stop,

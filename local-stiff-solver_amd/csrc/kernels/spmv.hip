@@ -173,42 +173,36 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslic
     if (slice < nslices) {
         const int64_t base = slice_ptr[slice];
         const int32_t W = int32_t((slice_ptr[slice + 1] - base) / H);
-        int32_t j = 0;
-        for (; j + 2 <= W; j += 2) {
-            const int64_t s0 = base + int64_t(j) * H + h, s1 = s0 + H;
-            int32_t c0 = active ? cols[s0] : -1;
-            int32_t c1 = active ? cols[s1] : -1;
-            const bool v0 = c0 >= 0, v1 = c1 >= 0;
-            c0 = v0 ? c0 : 0;
-            c1 = v1 ? c1 : 0;
-            double a0[BS], a1[BS], x0[BS], x1[BS];
+        // chunks of C slots: column indices first, then values and x, so every
+        // lane has 2*C*BS independent loads in flight; the matrix is streamed
+        // with non-temporal loads to leave the caches to x
+        constexpr int C = 4;
+        for (int32_t j0 = 0; j0 < W; j0 += C) {
+            int32_t c[C];
 #pragma unroll
-            for (int c = 0; c < BS; ++c) {
-                if constexpr (F32) {
-                    a0[c] = v0 ? double(vals32[s0 * BB + r * BS + c]) : 0.0;
-                    a1[c] = v1 ? double(vals32[s1 * BB + r * BS + c]) : 0.0;
-                } else {
-                    a0[c] = v0 ? vals[s0 * BB + r * BS + c] : 0.0;
-                    a1[c] = v1 ? vals[s1 * BB + r * BS + c] : 0.0;
-                }
-                x0[c] = x[int64_t(c0) * BS + c];
-                x1[c] = x[int64_t(c1) * BS + c];
+            for (int u = 0; u < C; ++u) {
+                const int64_t slot = base + int64_t(j0 + u) * H + h;
+                c[u] = (active && j0 + u < W) ? __builtin_nontemporal_load(&cols[slot]) : -1;
             }
+            double a[C][BS], xv[C][BS];
 #pragma unroll
-            for (int c = 0; c < BS; ++c) acc += a0[c] * x0[c];
+            for (int u = 0; u < C; ++u) {
+                const bool v = c[u] >= 0;
+                const int64_t slot = base + int64_t(j0 + u) * H + h;
+                const int64_t xc = int64_t(v ? c[u] : 0) * BS;
 #pragma unroll
-            for (int c = 0; c < BS; ++c) acc += a1[c] * x1[c];
-        }
-        if (j < W) {
-            const int64_t s0 = base + int64_t(j) * H + h;
-            int32_t c0 = active ? cols[s0] : -1;
-            if (c0 >= 0) {
-#pragma unroll
-                for (int c = 0; c < BS; ++c) {
-                    const double a = F32 ? double(vals32[s0 * BB + r * BS + c]) : vals[s0 * BB + r * BS + c];
-                    acc += a * x[int64_t(c0) * BS + c];
+                for (int q = 0; q < BS; ++q) {
+                    if constexpr (F32)
+                        a[u][q] = v ? double(__builtin_nontemporal_load(&vals32[slot * BB + r * BS + q])) : 0.0;
+                    else
+                        a[u][q] = v ? __builtin_nontemporal_load(&vals[slot * BB + r * BS + q]) : 0.0;
+                    xv[u][q] = x[xc + q];
                 }
             }
+#pragma unroll
+            for (int u = 0; u < C; ++u)
+#pragma unroll
+                for (int q = 0; q < BS; ++q) acc += a[u][q] * xv[u][q];
         }
     }
     double out = 0.0;
@@ -219,23 +213,8 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslic
     }
     if constexpr (DOT) {
         __shared__ double sh[kSpmvThreads / 64];
-        const double part = dev::block_sum<kSpmvThreads>(active ? x[int64_t(row) * BS + r] * out : 0.0, sh);
-        double total;
-        if (dev::grid_sum_last<kSpmvThreads>(part, partials, &st->ticket[0], sh, total) && threadIdx.x == 0) {
-            // GPUHybrid.cu:1458-1480
-            st->iters = iter + 1;
-            st->pq = total;
-            if (total == 0.0) {
-                st->stop = dev::kZeroSdotp;
-            } else {
-                const double alpha = st->gamma / total;
-                st->alpha = alpha;
-                if (alpha <= 0.0)
-                    st->stop = dev::kNegativeAlpha;
-                else if (alpha < 2.2250738585072014e-308)
-                    st->stop = dev::kSubnormalAlpha;
-            }
-        }
+        dev::store_partial(dev::block_sum<kSpmvThreads>(active ? x[int64_t(row) * BS + r] * out : 0.0, sh),
+                           partials);
     }
 }
 
@@ -319,6 +298,7 @@ void spmv_pq(const SellView &A, const double *p, double *q, dev::PcgState *st, d
              hipStream_t s)
 {
     launch_spmv<0, true, false>(A, p, q, nullptr, &st->stop, st, partials, iter, s);
+    finish(div_up(A.nslices, kSpmvThreads / 64), partials, dev::kFinishPq, st, nullptr, iter, s);
 }
 
 void spmv_residual(const SellView &A, const double *x, const double *b, double *r, const int *stop, hipStream_t s,

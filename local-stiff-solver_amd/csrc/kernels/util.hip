@@ -76,13 +76,7 @@ __global__ __launch_bounds__(kT) void k_jacobi(int64_t n, const double *__restri
     }
     if constexpr (DOT) {
         __shared__ double sh[kT / 64];
-        const double part = dev::block_sum<kT>(v, sh);
-        double total;
-        if (dev::grid_sum_last<kT>(part, partials, &st->ticket[0], sh, total) && threadIdx.x == 0) {
-            st->gamma = total;
-            st->beta = total / st->gamma_old;
-            st->gamma_old = total;
-        }
+        dev::store_partial(dev::block_sum<kT>(v, sh), partials);
     }
 }
 
@@ -124,8 +118,8 @@ void diag_inverse(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols
     LSS_LAUNCH_CHECK();
 }
 
-void jacobi_apply(int64_t n, const double *dinv, const double *r, double scale, const double *add, double *z,
-                  const double *dotv, dev::PcgState *st, double *partials, const int *stop, hipStream_t s)
+int jacobi_apply(int64_t n, const double *dinv, const double *r, double scale, const double *add, double *z,
+                 const double *dotv, dev::PcgState *st, double *partials, const int *stop, hipStream_t s)
 {
     const int grid = int(std::min<int64_t>(std::max<int64_t>(div_up(n, kT * 4), 1), 4096));
     if (st)
@@ -133,6 +127,7 @@ void jacobi_apply(int64_t n, const double *dinv, const double *r, double scale, 
     else
         k_jacobi<false><<<grid, kT, 0, s>>>(n, dinv, r, scale, add, z, dotv, st, partials, stop);
     LSS_LAUNCH_CHECK();
+    return grid;
 }
 
 }  // namespace lss::k

@@ -18,8 +18,9 @@ void stiff_maps(int32_t ncomp, int32_t n, const int64_t *comp_off, const int32_t
 void hash_vector(int64_t n, double *out, hipStream_t s);
 void diag_inverse(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols, const double *vals, double *dinv,
                   hipStream_t s);
-// z = scale * D^{-1} r (+ add); with st: gamma = dotv . z update of the PCG state.
-void jacobi_apply(int64_t n, const double *dinv, const double *r, double scale, const double *add, double *z,
+// z = scale * D^{-1} r (+ add); with st: per-workgroup partials of dotv . z.
+// Returns the number of partials written.
+int jacobi_apply(int64_t n, const double *dinv, const double *r, double scale, const double *add, double *z,
                   const double *dotv, dev::PcgState *st, double *partials, const int *stop, hipStream_t s);
 
 }  // namespace lss::k

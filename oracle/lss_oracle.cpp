@@ -25,7 +25,7 @@ double since(clk::time_point t0) { return std::chrono::duration<double>(clk::now
 
 constexpr int kChunks = 1024;  // reduction tree shape shared with the device kernels
 constexpr int kLanes = 256;
-constexpr int kPowerIters = 12;  // lambda_max(BA) estimate for the damped sweep
+constexpr int kLanczosSteps = 20;  // CG-Lanczos steps of the lambda_max(BA) estimate
 
 template <class F>
 double canonical_sum(int64_t n, F f)
@@ -157,23 +157,6 @@ void chol_solve(int m, const double *L, double *x)
         for (int k = i + 1; k < m; ++k) s -= L[size_t(k) * m + i] * x[k];
         x[i] = s / L[size_t(i) * m + i];
     }
-}
-
-// SPD inverse through Cholesky; falls back to Gauss-Jordan with partial pivoting.
-bool spd_inverse(int m, std::vector<double> &a)
-{
-    std::vector<double> L = a;
-    if (cholesky(m, L)) {
-        std::vector<double> col(m);
-        for (int c = 0; c < m; ++c) {
-            std::fill(col.begin(), col.end(), 0.0);
-            col[c] = 1.0;
-            chol_solve(m, L.data(), col.data());
-            for (int r = 0; r < m; ++r) a[size_t(r) * m + c] = col[r];
-        }
-        return true;
-    }
-    return false;
 }
 
 }  // namespace
@@ -361,46 +344,72 @@ extern "C" int orc_decompose(int32_t n, const int32_t *ptr, const int32_t *idx, 
 }
 
 // ===================================================================== partition
+namespace {
+
+// breadth-first order of `set` inside its induced subgraph, from `start`, then
+// from the smallest unvisited member for every further component
+std::vector<int32_t> bfs_in(const int64_t *bptr, const int32_t *bidx, const std::vector<int32_t> &set, int32_t start,
+                            std::vector<char> &in, std::vector<char> &seen)
+{
+    for (int32_t v : set) in[v] = 1, seen[v] = 0;
+    std::vector<int32_t> order;
+    order.reserve(set.size());
+    auto from = [&](int32_t s0) {
+        size_t head = order.size();
+        seen[s0] = 1;
+        order.push_back(s0);
+        while (head < order.size()) {
+            const int32_t u = order[head++];
+            for (int64_t e = bptr[u]; e < bptr[u + 1]; ++e) {
+                const int32_t w = bidx[e];
+                if (in[w] && !seen[w]) {
+                    seen[w] = 1;
+                    order.push_back(w);
+                }
+            }
+        }
+    };
+    from(start);
+    for (int32_t v : set)
+        if (!seen[v]) from(v);
+    for (int32_t v : set) in[v] = 0;
+    return order;
+}
+
+void split(const int64_t *bptr, const int32_t *bidx, std::vector<int32_t> set, int32_t start, int max_part,
+           int32_t base, int32_t *part_of, std::vector<char> &in, std::vector<char> &seen)
+{
+    if (int64_t(set.size()) <= max_part) {
+        for (int32_t v : set) part_of[v] = base;
+        return;
+    }
+    const int64_t k = (int64_t(set.size()) + max_part - 1) / max_part, kl = (k + 1) / 2;
+    auto order = bfs_in(bptr, bidx, set, start, in, seen);
+    std::vector<int32_t> L(order.begin(), order.begin() + kl * max_part), R(order.begin() + kl * max_part, order.end());
+    const int32_t rs = order.back();
+    std::sort(L.begin(), L.end());
+    std::sort(R.begin(), R.end());
+    split(bptr, bidx, std::move(L), start, max_part, base, part_of, in, seen);
+    split(bptr, bidx, std::move(R), rs, max_part, int32_t(base + kl), part_of, in, seen);
+}
+
+}  // namespace
+
 extern "C" int orc_partition(int32_t nb, const int64_t *bptr, const int32_t *bidx, int max_part, int32_t *part_of,
                              int32_t *nparts)
 {
-    // Greedy graph growing: seed = smallest unassigned node; grow by the
-    // frontier node with the most edges into the part (ties: smallest id).
-    std::fill(part_of, part_of + nb, -1);
-    std::vector<int32_t> gain(nb, 0);
-    std::vector<int32_t> frontier;
-    int32_t p = 0, seed = 0;
-    while (true) {
-        while (seed < nb && part_of[seed] != -1) ++seed;
-        if (seed >= nb) break;
-        frontier.clear();
-        int size = 0;
-        int32_t u = seed;
-        while (true) {
-            part_of[u] = p;
-            ++size;
-            for (int64_t e = bptr[u]; e < bptr[u + 1]; ++e) {
-                const int32_t w = bidx[e];
-                if (w == u || part_of[w] != -1) continue;
-                if (gain[w] == 0) frontier.push_back(w);
-                ++gain[w];
-            }
-            if (size >= max_part) break;
-            int best = -1;
-            for (int f = 0; f < int(frontier.size()); ++f) {
-                const int32_t w = frontier[f];
-                if (part_of[w] != -1) continue;
-                if (best < 0 || gain[w] > gain[frontier[best]] ||
-                    (gain[w] == gain[frontier[best]] && w < frontier[best]))
-                    best = f;
-            }
-            if (best < 0) break;
-            u = frontier[best];
-        }
-        for (int32_t w : frontier) gain[w] = 0;
-        ++p;
-    }
-    *nparts = p;
+    // Recursive breadth-first bisection: a subtree of k parts sends its first
+    // ceil(k/2)*max_part nodes in search order left (searched again from the
+    // same start) and the rest right (searched from the last node reached);
+    // parts are numbered left to right.  The first start is the last node
+    // reached by a search from node 0.
+    *nparts = int32_t((int64_t(nb) + max_part - 1) / max_part);
+    if (nb == 0) return 0;
+    std::vector<char> in(nb, 0), seen(nb, 0);
+    std::vector<int32_t> all(nb);
+    std::iota(all.begin(), all.end(), 0);
+    const int32_t start = bfs_in(bptr, bidx, all, 0, in, seen).back();
+    split(bptr, bidx, std::move(all), start, max_part, 0, part_of, in, seen);
     return 0;
 }
 
@@ -422,7 +431,7 @@ struct orc_mas {
     std::vector<int32_t> counts;    // matrices per level
     std::vector<int32_t> cco;       // coarse nodes per coarse level
     std::vector<int32_t> map;       // new node * MAX_COARSE + lv
-    std::vector<std::vector<double>> inv;  // per level: counts[l] * dim * dim (dense row-major)
+    std::vector<std::vector<double>> inv;  // per level: counts[l] * dim * dim Cholesky factors (row-major lower)
     int dim = 32;
 };
 
@@ -574,7 +583,8 @@ extern "C" orc_mas *orc_mas_create(int32_t n, const int32_t *ptr, const int32_t 
             std::vector<double> a(M->inv[L].begin() + size_t(k) * D * D, M->inv[L].begin() + size_t(k + 1) * D * D);
             for (int d = 0; d < D; ++d)
                 if (a[size_t(d) * D + d] == 0.0) a[size_t(d) * D + d] = 1.0;
-            if (!spd_inverse(D, a)) ok = false;
+            // A_b^{-1} is applied through its Cholesky factor (two triangular solves)
+            if (!cholesky(D, a)) ok = false;
             std::copy(a.begin(), a.end(), M->inv[L].begin() + size_t(k) * D * D);
         }
         if (!ok) {
@@ -603,7 +613,14 @@ extern "C" int orc_mas_inverse(const orc_mas *m, int level, int32_t bank, double
 {
     if (level < 0 || level >= m->levels || bank < 0 || bank >= m->counts[level]) return -1;
     const size_t D = m->dim;
-    std::copy(m->inv[level].begin() + bank * D * D, m->inv[level].begin() + (bank + 1) * D * D, out);
+    const double *L = m->inv[level].data() + bank * D * D;
+    std::vector<double> col(D);
+    for (size_t c = 0; c < D; ++c) {
+        std::fill(col.begin(), col.end(), 0.0);
+        col[c] = 1.0;
+        chol_solve(int(D), L, col.data());
+        for (size_t r = 0; r < D; ++r) out[r * D + c] = col[r];
+    }
     return 0;
 }
 
@@ -637,12 +654,10 @@ extern "C" void orc_mas_apply(const orc_mas *m, const double *r, double *z)
         const int32_t cnt = m->counts[L];
 #pragma omp parallel for schedule(static)
         for (int32_t k = 0; k < cnt; ++k) {
-            const double *Ainv = &m->inv[L][size_t(k) * D * D];
-            for (int i = 0; i < D; ++i) {
-                double s = 0.0;
-                for (int j = 0; j < D; ++j) s += Ainv[size_t(i) * D + j] * rl[L][size_t(k) * D + j];
-                yl[L][size_t(k) * D + i] = s;
-            }
+            const double *Lf = &m->inv[L][size_t(k) * D * D];
+            double *y = &yl[L][size_t(k) * D];
+            std::copy(&rl[L][size_t(k) * D], &rl[L][size_t(k) * D] + D, y);
+            chol_solve(D, Lf, y);
         }
     }
     for (int32_t p = 0; p < nparts; ++p)
@@ -720,6 +735,28 @@ struct orc_hybrid {
             base_apply(r, z);
             return;
         }
+        if (cfg.composition == 1) {
+            // z = P r + y - P A y,  y = B (r - A P r)
+            std::vector<double> pr(n, 0.0), t(n), y(n), pay(n, 0.0);
+            stiff_apply(r, pr.data());
+            spmv(A, pr.data(), t.data());
+#pragma omp parallel for schedule(static)
+            for (int32_t i = 0; i < n; ++i) t[i] = r[i] - t[i];
+            base_apply_raw(t.data(), y.data());
+            spmv(A, y.data(), t.data());
+            stiff_apply(t.data(), pay.data());
+#pragma omp parallel for schedule(static)
+            for (int32_t i = 0; i < n; ++i) z[i] = pr[i] + y[i] - pay[i];
+            return;
+        }
+        if (cfg.composition == 2) {
+            std::vector<double> pr(n, 0.0);
+            stiff_apply(r, pr.data());
+            base_apply_raw(r, z);
+#pragma omp parallel for schedule(static)
+            for (int32_t i = 0; i < n; ++i) z[i] += pr[i];
+            return;
+        }
         base_apply(r, z);  // z1 = B r
         // dss_precond_iter: next_z = z1 + P (r - A z1)
         spmv(A, z, z2.data());
@@ -738,6 +775,74 @@ struct orc_hybrid {
         for (int32_t i = 0; i < n; ++i) z[i] += z2[i];
     }
 };
+
+namespace {
+
+// Largest eigenvalue of the symmetric tridiagonal (d, e) by Sturm-count bisection.
+double tridiag_max_eig(const std::vector<double> &d, const std::vector<double> &e)
+{
+    const int m = int(d.size());
+    double lo = 0.0, hi = 0.0;
+    for (int i = 0; i < m; ++i) {
+        const double rad = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(e[i]) : 0.0);
+        hi = std::max(hi, d[i] + rad);
+        lo = std::min(lo, d[i] - rad);
+    }
+    auto count_below = [&](double x) {  // eigenvalues < x
+        int cnt = 0;
+        double q = d[0] - x;
+        if (q < 0) ++cnt;
+        for (int i = 1; i < m; ++i) {
+            if (q == 0.0) q = 1e-300;
+            q = d[i] - x - e[i - 1] * e[i - 1] / q;
+            if (q < 0) ++cnt;
+        }
+        return cnt;
+    };
+    for (int it = 0; it < 200 && hi - lo > 1e-13 * std::max(1.0, std::fabs(hi)); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (count_below(mid) >= m) hi = mid;
+        else lo = mid;
+    }
+    return hi;
+}
+
+}  // namespace
+
+double lanczos_lambda_max(const orc_hybrid &h, int32_t n)
+{
+    // CG-Lanczos on A with preconditioner B (undamped), x0 = 0, b = hashed vector
+    std::vector<double> r(n), z(n), p(n), q(n);
+    for (int32_t i = 0; i < n; ++i) r[i] = double(uint32_t(uint32_t(i) * 2654435761u)) * 0x1p-32 - 0.5;
+    h.base_apply_raw(r.data(), z.data());
+    p = z;
+    double rz = dot(n, r.data(), z.data());
+    std::vector<double> alpha, beta;
+    for (int j = 0; j < kLanczosSteps; ++j) {
+        spmv(h.A, p.data(), q.data());
+        const double pq = dot(n, p.data(), q.data());
+        if (!(pq > 0.0)) break;
+        const double a = rz / pq;
+#pragma omp parallel for schedule(static)
+        for (int32_t i = 0; i < n; ++i) r[i] -= a * q[i];
+        h.base_apply_raw(r.data(), z.data());
+        const double rz2 = dot(n, r.data(), z.data());
+        const double b = rz2 / rz;
+        alpha.push_back(a);
+        beta.push_back(b);
+        if (!(rz2 > 0.0)) break;
+        rz = rz2;
+#pragma omp parallel for schedule(static)
+        for (int32_t i = 0; i < n; ++i) p[i] = z[i] + b * p[i];
+    }
+    const int m = int(alpha.size());
+    std::vector<double> d(m), e(m > 0 ? m - 1 : 0);
+    for (int j = 0; j < m; ++j) {
+        d[j] = 1.0 / alpha[j] + (j > 0 ? beta[j - 1] / alpha[j - 1] : 0.0);
+        if (j + 1 < m) e[j] = std::sqrt(beta[j]) / alpha[j];
+    }
+    return m > 0 ? tridiag_max_eig(d, e) : 1.0;
+}
 
 extern "C" orc_hybrid *orc_hybrid_create(const orc_config *cfg)
 {
@@ -864,22 +969,13 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
     }
     // The additive base preconditioner B can have lambda_max(BA) > 2, which makes the
     // multiplicative sweep (I - BA)(I - PA)(I - BA) non-contractive and the composed
-    // preconditioner indefinite; damp B by omega = 1 / lambda_max(BA), estimated by a
-    // fixed power iteration in the A inner product from a hashed start vector.
+    // preconditioner indefinite; damp B by omega = damping / theta, theta the largest
+    // Ritz value of B A after kLanczosSteps steps of B-preconditioned CG on A x = v
+    // (v a hashed vector), which never exceeds lambda_max(BA) and approaches it fast.
     h->omega = 1.0;
-    if (c.do_hybrid_precond && !h->comps.empty()) {
-        std::vector<double> v(n), w(n), u(n);
-        for (int32_t i = 0; i < n; ++i) v[i] = double(uint32_t(uint32_t(i) * 2654435761u)) * 0x1p-32 - 0.5;
-        double lam = 1.0;
-        for (int k = 0; k < kPowerIters; ++k) {
-            spmv(A, v.data(), w.data());
-            h->base_apply_raw(w.data(), u.data());
-            lam = dot(n, w.data(), u.data()) / dot(n, v.data(), w.data());
-            const double nu = 1.0 / std::sqrt(dot(n, u.data(), u.data()));
-#pragma omp parallel for schedule(static)
-            for (int32_t i = 0; i < n; ++i) v[i] = u[i] * nu;
-        }
-        h->omega = 1.0 / lam;
+    if (c.do_hybrid_precond && !h->comps.empty() && c.composition == 0) {
+        const double theta = lanczos_lambda_max(*h, n);
+        h->omega = (c.damping > 0 ? c.damping : 1.0) / theta;
     }
     h->t_setup = since(t0);
     return 0;

@@ -22,10 +22,11 @@
  *   orc_expand          GPUHybrid.cu:972-991   pinned: one-ring closure in numpy
  *   orc_decompose       GPUHybrid.cu:691-735   pinned: scipy connected_components
  *   disjoint set        src/polysolve/linear/Hybrid.hpp:223-260
- *   orc_partition       the build's greedy graph-growing partitioner (the reference
- *                       uses KaMinPar, src/polysolve/linear/mas_utils/GraphPartition.cpp:176-289,
+ *   orc_partition       the build's recursive breadth-first bisection partitioner
+ *                       (the reference uses KaMinPar, MAS/GraphPartition.cpp:176-289,
  *                       whose output is not reproducible here: parity unpinned vs
- *                       the reference; pinned: part-size and contiguity invariants)
+ *                       the reference; pinned: part-size, count and coverage
+ *                       invariants, and bit-equality with the product's partition)
  *   orc_mas_*           src/polysolve/linear/mas_utils/MASPreconditioner.cu:33-1164
  *                       (bank-local CCO coarse spaces, per-bank matrices, zero
  *                       diagonal padding, inverse, restrict/apply/prolong)
@@ -119,6 +120,8 @@ typedef struct orc_config {
     double conditioning_threshold;
     int base_precond;   /* 0 = MAS, 1 = Jacobi, 2 = identity */
     int threads;        /* OpenMP threads (0 = runtime default) */
+    double damping;     /* omega = damping / lambda_max(BA) in the B-P-B sweep (0 -> 1) */
+    int composition;    /* 0 = B-P-B (GPUHybrid.cu:521-542), 1 = P + (I-PA) B (I-AP), 2 = B + P */
 } orc_config;
 
 typedef struct orc_hybrid orc_hybrid;

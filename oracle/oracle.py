@@ -36,7 +36,7 @@ class Config(ctypes.Structure):
         ("max_subdomain_size", ctypes.c_int64), ("gmm_jump_threshold", ctypes.c_double),
         ("gmm_tol", ctypes.c_double), ("max_gmm_iterations", ctypes.c_int),
         ("conditioning_threshold", ctypes.c_double), ("base_precond", ctypes.c_int),
-        ("threads", ctypes.c_int),
+        ("threads", ctypes.c_int), ("damping", ctypes.c_double), ("composition", ctypes.c_int),
     ]
 
 
@@ -226,6 +226,8 @@ def config(block_size=1, rel_tol=1e-8, abs_tol=0.0, max_iter=10000, base="mas", 
     c.conditioning_threshold = kw.get("conditioning_threshold", 100.0)
     c.base_precond = {"mas": 0, "jacobi": 1, "identity": 2}[base]
     c.threads = threads
+    c.damping = kw.get("damping", 1.5)
+    c.composition = {"BPB": 0, "PBP": 1, "additive": 2}[kw.get("composition", "BPB")]
     return c
 
 

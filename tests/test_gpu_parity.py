@@ -104,3 +104,26 @@ def test_solve_matches_oracle(gpu, name, N):
     assert info["omega"] == pytest.approx(oi["omega"], rel=1e-6)
     assert abs(info["num_iterations"] - oi["num_iterations"]) <= max(2, 0.05 * oi["num_iterations"])
     assert np.linalg.norm(x - xr) <= 1e-5 * np.linalg.norm(xr)
+
+
+@pytest.mark.parametrize("name,N", CASES)
+def test_partition_matches_oracle(gpu, name, N):
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    bs = prob.block_size
+    s = Solver.create(_cfg(bs))
+    s.factorize(A)
+    nb = A.n // bs
+    perm, off = s.partition(nb)
+    # block graph of the operator, as the solver builds it
+    import scipy.sparse as sp
+    # stored pattern (explicit zeros included), read as operator rows
+    G = sp.csr_matrix((np.ones(A.nnz), A.rowidx, A.colptr), shape=(A.n, A.n))
+    Rb = sp.csr_matrix((np.ones(A.n), (np.arange(A.n) // bs, np.arange(A.n))), shape=(nb, A.n))
+    Bg = (Rb @ G @ Rb.T).tocsr()
+    Bg.sort_indices()
+    part, nparts = O.partition(Bg.indptr.astype(np.int64), Bg.indices, 32)
+    assert nparts == off.size - 1
+    # product numbering: new = perm[old], parts are ranges of new ids
+    part_dev = np.searchsorted(off, perm, side="right") - 1
+    assert np.array_equal(part_dev, part)
