@@ -169,7 +169,10 @@ def main():
         "valid": valid,
         "phases_mixed_s": mixed["info"]["timings"],
         "stiff": {"n_stiff_dofs": mixed["info"]["stiff"]["n_stiff_dofs"],
-                  "n_components": mixed["info"]["stiff"]["n_components"]},
+                  "n_components": mixed["info"]["stiff"]["n_components"],
+                  "max_component": mixed["info"]["stiff"]["max_component"]},
+        "mas": mixed["info"]["mas"],
+        "sell_padding": round(mixed["info"]["sell_slots"] / mixed["info"]["nnz_blocks"] - 1.0, 4),
         "roofline": {"kernel": "spmv_sell_bs3_fp64", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "bytes_per_launch": by, "avg_launch_ms": spmv_ms,

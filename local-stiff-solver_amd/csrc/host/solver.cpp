@@ -103,6 +103,7 @@ void HybridSolver::release()
     d_comp_off_.release();
     d_mat_off_.release();
     d_pdofs_.release();
+    d_row_comp_.release();
     d_L_.release();
     d_swork_.release();
     xp_.release(), bp_.release(), r_.release(), p_.release(), q_.release(), z_.release(), u_.release();
@@ -381,6 +382,8 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     sell_slots_ = sp[nsl];
     d_slice_ptr_.upload(sp.data(), nsl + 1, s);
     d_scols_.resize(std::max<int64_t>(sell_slots_, 1));
+    // -1 everywhere first: slots of rows past the last block row stay padding
+    LSS_HIP(hipMemsetAsync(d_scols_.get(), 0xFF, d_scols_.bytes(), s));
     d_svals_.resize(std::max<int64_t>(sell_slots_ * bb, 1));
     d_svals_.zero(s);
     if (cfg_.fp32_inner()) {
@@ -492,6 +495,10 @@ void HybridSolver::build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx
     d_comp_off_.upload(off.data(), ncomp_ + 1, s);
     d_mat_off_.upload(moff.data(), ncomp_ + 1, s);
     d_pdofs_.upload(pd.data(), ns_, s);
+    std::vector<int32_t> rc(ns_);
+    for (int32_t c = 0; c < ncomp_; ++c)
+        for (int64_t k = off[c]; k < off[c + 1]; ++k) rc[k] = c;
+    d_row_comp_.upload(rc.data(), ns_, s);
     DBuf<int32_t> d_dofs, g2l(n), comp_of(n);
     d_dofs.upload(dofs.data(), ns_, s);
     k::stiff_maps(ncomp_, n, d_comp_off_.get(), d_dofs.get(), g2l.get(), comp_of.get(), s);
@@ -505,7 +512,11 @@ void HybridSolver::build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx
     d_status_.download(&st, 1, s);
     sync(s);
     if (st) fail(LSS_ERR_NOT_SPD, "stiff block Cholesky failed: a selected sub-block is not SPD");
-    d_swork_.resize(std::max<int64_t>(ns_, 1));
+    DBuf<double> G(moff[ncomp_]);
+    k::stiff_inverse_factor(ncomp_, int32_t(max_comp_), d_comp_off_.get(), d_mat_off_.get(), d_L_.get(), G.get(), s);
+    sync(s);
+    d_L_ = std::move(G);  // only the inverse factor is needed by the apply
+    d_swork_.resize(std::max<int64_t>(2 * ns_, 1));
 }
 
 void HybridSolver::alloc_vectors()
@@ -642,8 +653,10 @@ void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, 
     }
     base_apply(r, omega_, nullptr, z, nullptr, stop, s);  // z1 = B r
     // z2 = z1 + P (r - A z1)  (dss_precond_iter, GPUHybrid.cu:544-675)
-    k::stiff_correct(ncomp_, ns_, bs_, d_comp_off_.get(), d_mat_off_.get(), d_pdofs_.get(), d_L_.get(),
-                     d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z, d_swork_.get(), stop, s);
+    k::stiff_correct(ncomp_, ns_, bs_, d_comp_off_.get(), d_mat_off_.get(), d_row_comp_.get(), d_pdofs_.get(),
+                     d_L_.get(),
+                     d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z, d_swork_.get(), d_swork_.get() + ns_, stop,
+                     s);
     // u = r - A z2
     const bool prof = cfg_.profile_kernels && st;
     if (prof) prof_begin(1, s);

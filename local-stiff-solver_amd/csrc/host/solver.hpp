@@ -129,7 +129,7 @@ private:
     int64_t ns_ = 0;
     int64_t max_comp_ = 0;
     DBuf<int64_t> d_comp_off_, d_mat_off_;
-    DBuf<int32_t> d_pdofs_;
+    DBuf<int32_t> d_pdofs_, d_row_comp_;
     DBuf<double> d_L_, d_swork_;
     double omega_ = 1.0;
     double lambda_max_ = 0.0;

@@ -183,25 +183,58 @@ __global__ void k_restrict(int32_t nodes, int bs, const int64_t *__restrict__ ch
     coarse[g] = s;
 }
 
-// y = G^T (G x) with G packed lower in LDS.  Phase 1: thread i forms
-// t_i = G(i, 0:i) x (row i, contiguous); phase 2: thread j forms
-// y_j = G(j:D, j)^T t (column j: consecutive addresses across threads).
+// y = G^T (G x) with G packed lower in LDS (rows >= `rows` are identity
+// padding and skipped).  Phase 1: thread i forms t_i = G(i, 0:i) x (row i,
+// contiguous); phase 2: thread j forms y_j = G(j:rows, j)^T t (column j:
+// consecutive addresses across threads).  The dot loops are unrolled by 8 with
+// all LDS loads of a step issued before the multiply-adds and four independent
+// accumulators: the bank apply is bound by LDS latency, not by its flops.
 template <int D, int NT, class TL>
 __device__ __forceinline__ void bank_apply(const TL *__restrict__ G, const double *__restrict__ x,
                                            double *__restrict__ t, double *__restrict__ y, int rows)
 {
     for (int i = threadIdx.x; i < rows; i += NT) {
         const TL *g = G + packed_lower(i, 0);
-        double acc = 0.0;
-        for (int j = 0; j <= i; ++j) acc += double(g[j]) * x[j];
-        t[i] = acc;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int j = 0;
+        for (; j + 8 <= i + 1; j += 8) {
+            double gv[8], xv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                gv[u] = double(g[j + u]);
+                xv[u] = x[j + u];
+            }
+            a0 += gv[0] * xv[0] + gv[4] * xv[4];
+            a1 += gv[1] * xv[1] + gv[5] * xv[5];
+            a2 += gv[2] * xv[2] + gv[6] * xv[6];
+            a3 += gv[3] * xv[3] + gv[7] * xv[7];
+        }
+        for (; j <= i; ++j) a0 += double(g[j]) * x[j];
+        t[i] = (a0 + a1) + (a2 + a3);
     }
-    for (int i = rows + threadIdx.x; i < D; i += NT) t[i] = 0.0;
     __syncthreads();
     for (int j = threadIdx.x; j < rows; j += NT) {
-        double acc = 0.0;
-        for (int i = j; i < rows; ++i) acc += double(G[packed_lower(i, j)]) * t[i];
-        y[j] = acc;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int i = j;
+        int idx = packed_lower(j, j);  // advances by i + 1 from row i to row i + 1
+        for (; i + 8 <= rows; i += 8) {
+            double gv[8], tv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                gv[u] = double(G[idx]);
+                tv[u] = t[i + u];
+                idx += i + u + 1;
+            }
+            a0 += gv[0] * tv[0] + gv[4] * tv[4];
+            a1 += gv[1] * tv[1] + gv[5] * tv[5];
+            a2 += gv[2] * tv[2] + gv[6] * tv[6];
+            a3 += gv[3] * tv[3] + gv[7] * tv[7];
+        }
+        for (; i < rows; ++i) {
+            a0 += double(G[idx]) * t[i];
+            idx += i + 1;
+        }
+        y[j] = (a0 + a1) + (a2 + a3);
     }
 }
 

@@ -39,10 +39,15 @@ void stiff_extract(int32_t ncomp, const int64_t *comp_off, const int64_t *mat_of
 // In-place dense Cholesky (lower, row-major) of every component; status |= 2 on failure.
 void stiff_cholesky(int32_t ncomp, const int64_t *comp_off, const int64_t *mat_off, double *mats, int *status,
                     hipStream_t s);
-// z[p] += (S_c A S_c^T)^{-1} (r - A z)[p] over the stiff rows p of each component,
-// (A z) taken from the block operator in the permuted numbering.
+// G = L^{-1} per component from the Cholesky factors; stored with G^T in the strict upper triangle.
+void stiff_inverse_factor(int32_t ncomp, int32_t max_m, const int64_t *comp_off, const int64_t *mat_off,
+                          const double *L, double *G, hipStream_t s);
+// z[p] += (S_c A S_c^T)^{-1} (r - A z)[p] = G^T G (r - A z)[p] over the stiff rows p of
+// each component, (A z) taken from the block operator in the permuted numbering.
 void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, const int64_t *mat_off,
-                   const int32_t *pdofs, const double *L, const int32_t *rowptr, const int32_t *cols,
-                   const double *bvals, const double *r, double *z, double *work, const int *stop, hipStream_t s);
+                   const int32_t *row_comp, const int32_t *pdofs, const double *G, const int32_t *rowptr,
+                   const int32_t *cols,
+                   const double *bvals, const double *r, double *z, double *work, double *work2, const int *stop,
+                   hipStream_t s);
 
 }  // namespace lss::k

@@ -5,10 +5,10 @@
 // scalar row r = l % bs of block row h = l / bs, so a wave covers 64 (bs = 1),
 // 64 (bs = 2) or 63 (bs = 3) scalar rows and every lane accumulates its own row:
 // no cross-lane reduction, no atomics.  Slot j of slice s holds block j of each
-// of the H rows; the bs*bs values of the H blocks of a slot are contiguous, so
-// the wave's value loads for one slot cover one contiguous 8*H*bs*bs-byte span
-// (1512 B for bs = 3).  Rows shorter than the slice width are padded with
-// column -1 and zero values.
+// of the H rows; its H*bs*bs values are stored lane-major: value c of lane l
+// (column c of scalar row l) sits at c*H*bs + l, so each of the bs value loads
+// of a wave reads one contiguous span of 8*H*bs bytes (504 B for bs = 3).
+// Rows shorter than the slice width are padded with column -1 and zero values.
 //
 // Reference counterparts: BSR layout MAS/BSRMatrix.hpp:10-57 (row-major blocks,
 // tail block diagonal padded with 1, BSRMatrix.cu:123-134, 189-191); SpMV
@@ -140,11 +140,15 @@ __global__ void k_sell_fill(int32_t nb, int bs, int H, const int32_t *__restrict
         const int64_t slot = base + int64_t(j) * H + h;
         if (j < len) {
             scols[slot] = cols[b + j];
-            for (int t = 0; t < bb; ++t) {
-                const double v = vals[int64_t(b + j) * bb + t];
-                svals[slot * bb + t] = v;
-                if (svals32) svals32[slot * bb + t] = static_cast<float>(v);
-            }
+            // value (rr, c) of the block -> lane l = h*bs + rr, component c
+            const int64_t vbase = (base + int64_t(j) * H) * bb;
+            for (int rr = 0; rr < bs; ++rr)
+                for (int c = 0; c < bs; ++c) {
+                    const double v = vals[int64_t(b + j) * bb + rr * bs + c];
+                    const int64_t at = vbase + int64_t(c) * (H * bs) + h * bs + rr;
+                    svals[at] = v;
+                    if (svals32) svals32[at] = static_cast<float>(v);
+                }
         } else {
             scols[slot] = -1;
         }
@@ -173,29 +177,29 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslic
     if (slice < nslices) {
         const int64_t base = slice_ptr[slice];
         const int32_t W = int32_t((slice_ptr[slice + 1] - base) / H);
-        // chunks of C slots: column indices first, then values and x, so every
-        // lane has 2*C*BS independent loads in flight; the matrix is streamed
-        // with non-temporal loads to leave the caches to x
+        // Every load is unconditional: padding slots hold column -1 and zero
+        // values, lanes past the last row read the slot of row H-1 (results
+        // discarded), so the C-slot chunk issues all its column, value and x
+        // loads back to back.  The matrix is streamed with non-temporal loads.
+        const int hh = h < H ? h : H - 1;
+        const int ll = hh * BS + r;  // lane within the slot's value layout
         constexpr int C = 4;
-        for (int32_t j0 = 0; j0 < W; j0 += C) {
+        int32_t j0 = 0;
+        for (; j0 + C <= W; j0 += C) {
             int32_t c[C];
 #pragma unroll
-            for (int u = 0; u < C; ++u) {
-                const int64_t slot = base + int64_t(j0 + u) * H + h;
-                c[u] = (active && j0 + u < W) ? __builtin_nontemporal_load(&cols[slot]) : -1;
-            }
+            for (int u = 0; u < C; ++u) c[u] = __builtin_nontemporal_load(&cols[base + int64_t(j0 + u) * H + hh]);
             double a[C][BS], xv[C][BS];
 #pragma unroll
             for (int u = 0; u < C; ++u) {
-                const bool v = c[u] >= 0;
-                const int64_t slot = base + int64_t(j0 + u) * H + h;
-                const int64_t xc = int64_t(v ? c[u] : 0) * BS;
+                const int64_t vb = (base + int64_t(j0 + u) * H) * BB + ll;
+                const int64_t xc = int64_t(c[u] < 0 ? 0 : c[u]) * BS;
 #pragma unroll
                 for (int q = 0; q < BS; ++q) {
                     if constexpr (F32)
-                        a[u][q] = v ? double(__builtin_nontemporal_load(&vals32[slot * BB + r * BS + q])) : 0.0;
+                        a[u][q] = double(__builtin_nontemporal_load(&vals32[vb + q * (H * BS)]));
                     else
-                        a[u][q] = v ? __builtin_nontemporal_load(&vals[slot * BB + r * BS + q]) : 0.0;
+                        a[u][q] = __builtin_nontemporal_load(&vals[vb + q * (H * BS)]);
                     xv[u][q] = x[xc + q];
                 }
             }
@@ -203,6 +207,18 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslic
             for (int u = 0; u < C; ++u)
 #pragma unroll
                 for (int q = 0; q < BS; ++q) acc += a[u][q] * xv[u][q];
+        }
+        for (; j0 < W; ++j0) {
+            const int64_t slot = base + int64_t(j0) * H + hh;
+            const int32_t c0 = __builtin_nontemporal_load(&cols[slot]);
+            const int64_t xc = int64_t(c0 < 0 ? 0 : c0) * BS;
+            const int64_t vb = (base + int64_t(j0) * H) * BB + ll;
+#pragma unroll
+            for (int q = 0; q < BS; ++q) {
+                const double a = F32 ? double(__builtin_nontemporal_load(&vals32[vb + q * (H * BS)]))
+                                     : __builtin_nontemporal_load(&vals[vb + q * (H * BS)]);
+                acc += a * x[xc + q];
+            }
         }
     }
     double out = 0.0;

@@ -4,10 +4,13 @@
 // per stiff component and factorizes it with cuDSS (sparse LDL^T) or cuBLAS
 // (dense LU + explicit inverse); dss_precond_iter (544-675) applies
 // z2 = z1 + S^T (S A S^T)^{-1} S (r - A z1) with gather / solve / scatter.
-// Here: dense Cholesky per component (one workgroup each, blocked, diagonal
-// blocks in LDS), and the correction computes (r - A z1) only on the stiff
-// rows (the other rows of the reference's full SpMV are never read), then
-// solves with blocked triangular sweeps and adds into z in place.
+// Here: dense FP64 Cholesky per component (one workgroup each, blocked,
+// diagonal blocks in LDS), then the inverse factor G = L^{-1} (column-parallel
+// forward substitution), so that each preconditioner application is two
+// dependency-free triangular matrix-vector products x = G^T (G t) instead of
+// 2*m/32 dependent block steps of a triangular solve.  The correction computes
+// (r - A z1) only on the stiff rows (the other rows of the reference's full
+// SpMV are never read) and adds x into z in place.
 #include "mas_kernels.hpp"
 #include "../common/runtime.hpp"
 
@@ -135,83 +138,61 @@ __global__ void k_stiff_residual(int64_t ns, int bs, const int32_t *__restrict__
     work[k] = r[p] - acc;
 }
 
-__global__ __launch_bounds__(kT) void k_stiff_solve(const int64_t *__restrict__ comp_off,
+// G = L^{-1}: one lane per column j, G(i, j) = -(sum_{k=j}^{i-1} L(i,k) G(k,j)) / L(i,i).
+// Stored as M with M(i, j) = G(i, j) for j <= i and M(j, i) = G(i, j) above the
+// diagonal (G and G^T share one square array).
+__global__ __launch_bounds__(64) void k_tri_inverse(const int64_t *__restrict__ comp_off,
+                                                    const int64_t *__restrict__ mat_off, const double *__restrict__ Lall,
+                                                    double *__restrict__ Gall)
+{
+    const int32_t c = blockIdx.x;
+    const int32_t m = int32_t(comp_off[c + 1] - comp_off[c]);
+    const int32_t j = blockIdx.y * 64 + threadIdx.x;
+    if (j >= m) return;
+    const double *L = Lall + mat_off[c];
+    double *G = Gall + mat_off[c];
+    G[int64_t(j) * m + j] = 1.0 / L[int64_t(j) * m + j];
+    for (int32_t i = j + 1; i < m; ++i) {
+        const double *Li = L + int64_t(i) * m;
+        double s = 0.0;
+        for (int32_t k = j; k < i; ++k) s += Li[k] * G[int64_t(k) * m + j];
+        const double g = -s / Li[i];
+        G[int64_t(i) * m + j] = g;  // lower triangle: G
+        G[int64_t(j) * m + i] = g;  // strict upper triangle: G^T, so both products read rows
+    }
+}
+
+// One wave per stiff row (all components at once).  Phase G: u_i = M(i, 0:i) t;
+// phase GT: x_j = M(j, j) u_j + M(j, j+1:m) u_{j+1:m}.  Lanes read a row with
+// coalesced loads, a wave reduction finishes the dot product.
+template <bool TRANSPOSED>
+__global__ __launch_bounds__(256) void k_stiff_rows(int64_t ns, const int32_t *__restrict__ row_comp,
+                                                    const int64_t *__restrict__ comp_off,
                                                     const int64_t *__restrict__ mat_off, const int32_t *__restrict__ pdofs,
-                                                    const double *__restrict__ Lall, double *__restrict__ work,
-                                                    double *__restrict__ z, const int *__restrict__ stop)
+                                                    const double *__restrict__ M, const double *__restrict__ in,
+                                                    double *__restrict__ out, double *__restrict__ z,
+                                                    const int *__restrict__ stop)
 {
     if (stop && *stop) return;
-    __shared__ double Lkk[NB][NB + 1];
-    __shared__ double yb[NB];
-    const int32_t c = blockIdx.x;
+    const int64_t k = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= ns) return;
+    const int32_t c = row_comp[k];
     const int64_t off = comp_off[c];
     const int32_t m = int32_t(comp_off[c + 1] - off);
-    const double *L = Lall + mat_off[c];
-    double *y = work + off;
-    const int lane = threadIdx.x & 63;
-    // forward: L y = t
-    for (int32_t k0 = 0; k0 < m; k0 += NB) {
-        const int nb = min(NB, m - k0);
-        for (int t = threadIdx.x; t < NB * NB; t += kT) {
-            const int i = t / NB, j = t % NB;
-            Lkk[i][j] = (i < nb && j < nb) ? L[int64_t(k0 + i) * m + k0 + j] : 0.0;
-        }
-        if (threadIdx.x < nb) yb[threadIdx.x] = y[k0 + threadIdx.x];
-        __syncthreads();
-        if (threadIdx.x < 64) {
-            double yl = lane < nb ? yb[lane] : 0.0;
-            for (int j = 0; j < nb; ++j) {
-                if (lane == j) yl /= Lkk[j][j];
-                const double yj = __shfl(yl, j, 64);
-                if (lane > j && lane < nb) yl -= Lkk[lane][j] * yj;
-            }
-            if (lane < nb) {
-                yb[lane] = yl;
-                y[k0 + lane] = yl;
-            }
-        }
-        __syncthreads();
-        for (int32_t i = k0 + nb + threadIdx.x; i < m; i += kT) {
-            const double *Li = L + int64_t(i) * m + k0;
-            double s = 0.0;
-            for (int t = 0; t < nb; ++t) s += Li[t] * yb[t];
-            y[i] -= s;
-        }
-        __syncthreads();
+    const int32_t i = int32_t(k - off);
+    const double *row = M + mat_off[c] + int64_t(i) * m;
+    const double *v = in + off;
+    const int32_t lo = TRANSPOSED ? i : 0, hi = TRANSPOSED ? m : i + 1;
+    double acc = 0.0;
+    for (int32_t j = lo + lane; j < hi; j += 64) acc += row[j] * v[j];
+    acc = dev::wave_sum(acc);
+    if (lane == 0) {
+        if (TRANSPOSED)
+            z[pdofs[k]] += acc;  // z2 = z1 + S^T x (GPUHybrid.cu:621-671)
+        else
+            out[k] = acc;
     }
-    // backward: L^T x = y
-    const int32_t nblk = (m + NB - 1) / NB;
-    for (int32_t b = nblk - 1; b >= 0; --b) {
-        const int32_t k0 = b * NB;
-        const int nb = min(NB, m - k0);
-        for (int t = threadIdx.x; t < NB * NB; t += kT) {
-            const int i = t / NB, j = t % NB;
-            Lkk[i][j] = (i < nb && j < nb) ? L[int64_t(k0 + i) * m + k0 + j] : 0.0;
-        }
-        if (threadIdx.x < nb) yb[threadIdx.x] = y[k0 + threadIdx.x];
-        __syncthreads();
-        if (threadIdx.x < 64) {
-            double xl = lane < nb ? yb[lane] : 0.0;
-            for (int j = nb - 1; j >= 0; --j) {
-                if (lane == j) xl /= Lkk[j][j];
-                const double xj = __shfl(xl, j, 64);
-                if (lane < j) xl -= Lkk[j][lane] * xj;
-            }
-            if (lane < nb) {
-                yb[lane] = xl;
-                y[k0 + lane] = xl;
-            }
-        }
-        __syncthreads();
-        for (int32_t i = threadIdx.x; i < k0; i += kT) {
-            double s = 0.0;
-            for (int t = 0; t < nb; ++t) s += L[int64_t(k0 + t) * m + i] * yb[t];
-            y[i] -= s;
-        }
-        __syncthreads();
-    }
-    // z2 = z1 + S^T x (GPUHybrid.cu:621-671)
-    for (int32_t i = threadIdx.x; i < m; i += kT) z[pdofs[off + i]] += y[i];
 }
 
 }  // namespace
@@ -233,14 +214,27 @@ void stiff_cholesky(int32_t ncomp, const int64_t *comp_off, const int64_t *mat_o
     LSS_LAUNCH_CHECK();
 }
 
+void stiff_inverse_factor(int32_t ncomp, int32_t max_m, const int64_t *comp_off, const int64_t *mat_off,
+                          const double *L, double *G, hipStream_t s)
+{
+    if (ncomp == 0) return;
+    dim3 grid(ncomp, div_up(max_m, 64));
+    k_tri_inverse<<<grid, 64, 0, s>>>(comp_off, mat_off, L, G);
+    LSS_LAUNCH_CHECK();
+}
+
 void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, const int64_t *mat_off,
-                   const int32_t *pdofs, const double *L, const int32_t *rowptr, const int32_t *cols,
-                   const double *bvals, const double *r, double *z, double *work, const int *stop, hipStream_t s)
+                   const int32_t *row_comp, const int32_t *pdofs, const double *G, const int32_t *rowptr,
+                   const int32_t *cols,
+                   const double *bvals, const double *r, double *z, double *work, double *work2, const int *stop,
+                   hipStream_t s)
 {
     if (ncomp == 0 || ns == 0) return;
     k_stiff_residual<<<div_up(ns, 256), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r, z, work, stop);
     LSS_LAUNCH_CHECK();
-    k_stiff_solve<<<ncomp, kT, 0, s>>>(comp_off, mat_off, pdofs, L, work, z, stop);
+    k_stiff_rows<false><<<div_up(ns, 4), 256, 0, s>>>(ns, row_comp, comp_off, mat_off, pdofs, G, work, work2, z, stop);
+    LSS_LAUNCH_CHECK();
+    k_stiff_rows<true><<<div_up(ns, 4), 256, 0, s>>>(ns, row_comp, comp_off, mat_off, pdofs, G, work2, nullptr, z, stop);
     LSS_LAUNCH_CHECK();
 }
 
