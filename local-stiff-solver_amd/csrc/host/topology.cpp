@@ -8,10 +8,9 @@ namespace lss {
 
 namespace {
 
-// One bisection step of the recursive partitioner: order the nodes of `set`
-// (all labelled `label` in lab[]) by breadth-first search inside the subgraph,
-// starting from `start`, continuing from the smallest remaining node when the
-// subgraph is disconnected; the first `left` nodes form the left half.
+// Breadth-first order of `set` (all labelled `label` in lab[]) inside its
+// induced subgraph, from `start`, continuing from the smallest unvisited member
+// when the subgraph is disconnected.
 void bfs_order(const Pattern &g, const std::vector<int32_t> &set, int32_t start, int32_t label,
                std::vector<int32_t> &lab, std::vector<int32_t> &order)
 {
@@ -39,8 +38,8 @@ void bfs_order(const Pattern &g, const std::vector<int32_t> &set, int32_t start,
             if (lab[v] == label) visit_from(v);
 }
 
-void bisect(const Pattern &g, std::vector<int32_t> set, int32_t start, int max_part, int32_t part_base,
-            int32_t label, std::vector<int32_t> &lab, std::vector<int32_t> &part_of, int depth)
+void bisect(const Pattern &g, std::vector<int32_t> set, int max_part, int32_t part_base, int32_t label,
+            std::vector<int32_t> &lab, std::vector<int32_t> &part_of, int depth)
 {
     const int64_t m = int64_t(set.size());
     if (m <= max_part) {
@@ -50,24 +49,31 @@ void bisect(const Pattern &g, std::vector<int32_t> set, int32_t start, int max_p
     const int64_t k = (m + max_part - 1) / max_part;  // parts in this subtree
     const int64_t kl = (k + 1) / 2;
     const int64_t nl = kl * max_part;                   // left half holds kl full parts
+    // pseudo-peripheral start: the last node reached from the smallest member
     for (int32_t v : set) lab[v] = label;
     std::vector<int32_t> order;
+    bfs_order(g, set, set.front(), label, lab, order);
+    const int32_t start = order.back();
+    for (int32_t v : set) lab[v] = label;
     bfs_order(g, set, start, label, lab, order);
-    std::vector<int32_t> L(order.begin(), order.begin() + nl), R(order.begin() + nl, order.end());
-    // the right half restarts from the node farthest from `start`, the left from `start`
-    const int32_t rs = order.back();
-    std::sort(L.begin(), L.end());
-    std::sort(R.begin(), R.end());
+    // halves in ascending order: mark the first nl nodes, then one scan of `set`
     const int32_t lbl_l = 2 * label + 1, lbl_r = 2 * label + 2;
+    for (int64_t q = 0; q < nl; ++q) lab[order[q]] = lbl_l;
+    std::vector<int32_t> L, R;
+    L.reserve(nl);
+    R.reserve(m - nl);
+    for (int32_t v : set) (lab[v] == lbl_l ? L : R).push_back(v);
+    std::vector<int32_t>().swap(order);
+    std::vector<int32_t>().swap(set);
     if (depth < 12) {
-#pragma omp task default(none) firstprivate(L, start, max_part, part_base, lbl_l, depth) shared(g, lab, part_of)
-        bisect(g, std::move(L), start, max_part, part_base, lbl_l, lab, part_of, depth + 1);
-#pragma omp task default(none) firstprivate(R, rs, max_part, part_base, kl, lbl_r, depth) shared(g, lab, part_of)
-        bisect(g, std::move(R), rs, max_part, int32_t(part_base + kl), lbl_r, lab, part_of, depth + 1);
+#pragma omp task default(none) firstprivate(L, max_part, part_base, lbl_l, depth) shared(g, lab, part_of)
+        bisect(g, std::move(L), max_part, part_base, lbl_l, lab, part_of, depth + 1);
+#pragma omp task default(none) firstprivate(R, max_part, part_base, kl, lbl_r, depth) shared(g, lab, part_of)
+        bisect(g, std::move(R), max_part, int32_t(part_base + kl), lbl_r, lab, part_of, depth + 1);
 #pragma omp taskwait
     } else {
-        bisect(g, std::move(L), start, max_part, part_base, lbl_l, lab, part_of, depth + 1);
-        bisect(g, std::move(R), rs, max_part, int32_t(part_base + kl), lbl_r, lab, part_of, depth + 1);
+        bisect(g, std::move(L), max_part, part_base, lbl_l, lab, part_of, depth + 1);
+        bisect(g, std::move(R), max_part, int32_t(part_base + kl), lbl_r, lab, part_of, depth + 1);
     }
 }
 
@@ -75,25 +81,32 @@ void bisect(const Pattern &g, std::vector<int32_t> set, int32_t start, int max_p
 
 void partition_graph(const Pattern &g, int max_part, std::vector<int32_t> &part_of, int32_t &nparts)
 {
-    // Recursive breadth-first bisection into parts of max_part nodes (the last
-    // part of a subtree may be smaller).  Part ids follow the recursion from
-    // left to right, so every run of consecutive parts is one subtree: compact
-    // at every scale, which is what the bank-local coarse levels of MAS need.
+    // Recursive breadth-first bisection.  Nodes without off-diagonal entries
+    // (e.g. eliminated Dirichlet rows) are set aside into trailing parts: left
+    // in place they become singleton components that never merge and fill the
+    // coarse levels.  Each subset is split by a search from a pseudo-peripheral
+    // node (last node reached from its smallest member): the first
+    // ceil(k/2)*max_part nodes in search order go left.  Part ids follow the
+    // recursion from left to right, so every run of consecutive parts is one
+    // subtree, compact at every scale, as the bank-local coarse levels need.
     const int32_t n = g.n;
     part_of.assign(n, 0);
-    nparts = int32_t((int64_t(n) + max_part - 1) / max_part);
-    if (n == 0) return;
-    std::vector<int32_t> all(n);
-    std::iota(all.begin(), all.end(), 0);
-    // pseudo-peripheral start: the last node reached by a search from node 0
-    std::vector<int32_t> lab(n, 0), order;
-    bfs_order(g, all, 0, 0, lab, order);
-    const int32_t start = order.back();
-    // labels: subproblem ids (heap numbering); visited markers are negative
-    lab.assign(n, 0);
+    std::vector<int32_t> real, iso;
+    for (int32_t v = 0; v < n; ++v) {
+        bool coupled = false;
+        for (int32_t e = g.rowptr[v]; e < g.rowptr[v + 1] && !coupled; ++e) coupled = g.cols[e] != v;
+        (coupled ? real : iso).push_back(v);
+    }
+    const int32_t real_parts = int32_t((int64_t(real.size()) + max_part - 1) / max_part);
+    nparts = real_parts + int32_t((int64_t(iso.size()) + max_part - 1) / max_part);
+    for (size_t q = 0; q < iso.size(); ++q) part_of[iso[q]] = real_parts + int32_t(q / max_part);
+    if (real.empty()) return;
+    // labels: subproblem ids (heap numbering) >= 0, visited markers < -1, isolated -1
+    std::vector<int32_t> lab(n, 0);
+    for (int32_t v : iso) lab[v] = -1;
 #pragma omp parallel
 #pragma omp single
-    bisect(g, std::move(all), start, max_part, 0, 0, lab, part_of, 0);
+    bisect(g, std::move(real), max_part, 0, 0, lab, part_of, 0);
 }
 
 void partition_permutation(const std::vector<int32_t> &part_of, int32_t nparts, std::vector<int32_t> &perm,

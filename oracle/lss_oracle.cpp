@@ -376,21 +376,21 @@ std::vector<int32_t> bfs_in(const int64_t *bptr, const int32_t *bidx, const std:
     return order;
 }
 
-void split(const int64_t *bptr, const int32_t *bidx, std::vector<int32_t> set, int32_t start, int max_part,
-           int32_t base, int32_t *part_of, std::vector<char> &in, std::vector<char> &seen)
+void split(const int64_t *bptr, const int32_t *bidx, std::vector<int32_t> set, int max_part, int32_t base,
+           int32_t *part_of, std::vector<char> &in, std::vector<char> &seen)
 {
     if (int64_t(set.size()) <= max_part) {
         for (int32_t v : set) part_of[v] = base;
         return;
     }
     const int64_t k = (int64_t(set.size()) + max_part - 1) / max_part, kl = (k + 1) / 2;
+    const int32_t start = bfs_in(bptr, bidx, set, set.front(), in, seen).back();
     auto order = bfs_in(bptr, bidx, set, start, in, seen);
     std::vector<int32_t> L(order.begin(), order.begin() + kl * max_part), R(order.begin() + kl * max_part, order.end());
-    const int32_t rs = order.back();
     std::sort(L.begin(), L.end());
     std::sort(R.begin(), R.end());
-    split(bptr, bidx, std::move(L), start, max_part, base, part_of, in, seen);
-    split(bptr, bidx, std::move(R), rs, max_part, int32_t(base + kl), part_of, in, seen);
+    split(bptr, bidx, std::move(L), max_part, base, part_of, in, seen);
+    split(bptr, bidx, std::move(R), max_part, int32_t(base + kl), part_of, in, seen);
 }
 
 }  // namespace
@@ -398,18 +398,23 @@ void split(const int64_t *bptr, const int32_t *bidx, std::vector<int32_t> set, i
 extern "C" int orc_partition(int32_t nb, const int64_t *bptr, const int32_t *bidx, int max_part, int32_t *part_of,
                              int32_t *nparts)
 {
-    // Recursive breadth-first bisection: a subtree of k parts sends its first
-    // ceil(k/2)*max_part nodes in search order left (searched again from the
-    // same start) and the rest right (searched from the last node reached);
-    // parts are numbered left to right.  The first start is the last node
-    // reached by a search from node 0.
-    *nparts = int32_t((int64_t(nb) + max_part - 1) / max_part);
-    if (nb == 0) return 0;
+    // Recursive breadth-first bisection.  Nodes without off-diagonal entries go
+    // to trailing parts (32 per part, ascending).  A subset of k parts is
+    // searched from the last node reached by a search from its smallest member;
+    // its first ceil(k/2)*max_part nodes in that order go left; parts are
+    // numbered left to right.
+    std::vector<int32_t> real, iso;
+    for (int32_t v = 0; v < nb; ++v) {
+        bool coupled = false;
+        for (int64_t e = bptr[v]; e < bptr[v + 1]; ++e) coupled |= bidx[e] != v;
+        (coupled ? real : iso).push_back(v);
+    }
+    const int32_t real_parts = int32_t((int64_t(real.size()) + max_part - 1) / max_part);
+    *nparts = real_parts + int32_t((int64_t(iso.size()) + max_part - 1) / max_part);
+    for (size_t q = 0; q < iso.size(); ++q) part_of[iso[q]] = real_parts + int32_t(q / max_part);
+    if (real.empty()) return 0;
     std::vector<char> in(nb, 0), seen(nb, 0);
-    std::vector<int32_t> all(nb);
-    std::iota(all.begin(), all.end(), 0);
-    const int32_t start = bfs_in(bptr, bidx, all, 0, in, seen).back();
-    split(bptr, bidx, std::move(all), start, max_part, 0, part_of, in, seen);
+    split(bptr, bidx, std::move(real), max_part, 0, part_of, in, seen);
     return 0;
 }
 
@@ -690,6 +695,7 @@ struct orc_hybrid {
     orc_mas *mas = nullptr;
     std::vector<double> jacobi;
     std::vector<int32_t> perm, part_off;
+    std::vector<int32_t> given_perm, given_part_off;  // optional partition supplied by the caller
     double omega = 1.0;  // damping of B inside the multiplicative sweep
     int iters = 0;
     double final_res = 0.0, t_setup = 0.0, t_solve = 0.0;
@@ -951,7 +957,14 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
         }
         std::vector<int32_t> part_of(nb);
         int32_t np = 0;
-        orc_partition(nb, bptr.data(), bidx.data(), 32, part_of.data(), &np);
+        if (!h->given_perm.empty() && int32_t(h->given_perm.size()) == nb) {
+            np = int32_t(h->given_part_off.size()) - 1;
+            for (int32_t I = 0; I < nb; ++I)
+                part_of[I] = int32_t(std::upper_bound(h->given_part_off.begin(), h->given_part_off.end(),
+                                                      h->given_perm[I]) - h->given_part_off.begin()) - 1;
+        } else {
+            orc_partition(nb, bptr.data(), bidx.data(), 32, part_of.data(), &np);
+        }
         // permutation sorted by part, stable in node id (MASSolver.cu:90-118)
         h->part_off.assign(np + 1, 0);
         for (int32_t I = 0; I < nb; ++I) h->part_off[part_of[I] + 1]++;
@@ -1040,6 +1053,13 @@ extern "C" int orc_hybrid_solve(orc_hybrid *h, const double *b, double *xio)
 }
 
 extern "C" double orc_hybrid_omega(const orc_hybrid *h) { return h->omega; }
+
+extern "C" void orc_hybrid_set_partition(orc_hybrid *h, const int32_t *perm, int32_t nb, const int32_t *part_off,
+                                         int32_t nparts)
+{
+    h->given_perm.assign(perm, perm + nb);
+    h->given_part_off.assign(part_off, part_off + nparts + 1);
+}
 
 extern "C" void orc_hybrid_info(const orc_hybrid *h, int *iters, double *final_res, double *t_setup, double *t_solve,
                                 int64_t *n_stiff, int64_t *n_comp)

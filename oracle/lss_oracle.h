@@ -134,6 +134,9 @@ int orc_hybrid_solve(orc_hybrid *h, const double *b, double *x);
 /* iters, final_res_norm, setup seconds, solve seconds, n stiff dofs, n components */
 void orc_hybrid_info(const orc_hybrid *h, int *iters, double *final_res, double *t_setup, double *t_solve,
                      int64_t *n_stiff, int64_t *n_comp);
+/* Use this partition (perm[old] = new, parts = ranges of part_off) instead of orc_partition. */
+void orc_hybrid_set_partition(orc_hybrid *h, const int32_t *perm, int32_t nb, const int32_t *part_off,
+                              int32_t nparts);
 /* Damping omega = 1 / lambda_max(BA) of the base preconditioner in the B-P-B sweep. */
 double orc_hybrid_omega(const orc_hybrid *h);
 /* Export the stiff components (sorted by smallest DOF). Returns count written. */

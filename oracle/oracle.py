@@ -77,6 +77,7 @@ def lib():
         L.orc_hybrid_stiff.restype = ctypes.c_int64
         L.orc_hybrid_stiff.argtypes = [P, P, ctypes.c_int64, P, ctypes.c_int64]
         L.orc_hybrid_precond.argtypes = [P, P, P]
+        L.orc_hybrid_set_partition.argtypes = [P, P, ctypes.c_int32, P, ctypes.c_int32]
         L.orc_hybrid_omega.restype = ctypes.c_double
         L.orc_hybrid_omega.argtypes = [P]
         L.orc_hybrid_partition.restype = ctypes.c_int32
@@ -242,6 +243,11 @@ class Hybrid:
     def __del__(self):
         if getattr(self, "_h", None) is not None and _lib is not None:
             _lib.orc_hybrid_destroy(self._h)
+
+    def set_partition(self, perm, part_off):
+        perm, part_off = _i32(perm), _i32(part_off)
+        self._keep = (perm, part_off)
+        lib().orc_hybrid_set_partition(self._h, _p(perm), perm.size, _p(part_off), part_off.size - 1)
 
     def set_problematic_dofs(self, dofs):
         d = _i32(dofs)
