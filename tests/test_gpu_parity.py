@@ -127,3 +127,65 @@ def test_partition_matches_oracle(gpu, name, N):
     # product numbering: new = perm[old], parts are ranges of new ids
     part_dev = np.searchsorted(off, perm, side="right") - 1
     assert np.array_equal(part_dev, part)
+
+
+# ---------------------------------------------------------------- reference example system
+def _contact_fixture():
+    import os
+
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    d = np.load(os.path.join(gold, "A_contact.npz"))
+    g = np.load(os.path.join(gold, "A_contact_golden.npz"))
+    A = problems.CSC(int(d["n"]), d["colptr"], d["rowidx"], d["vals"])
+    return A, d["rhs"], g
+
+
+def test_contact_fixture_selection(gpu):
+    A, b, g = _contact_fixture()
+    s = Solver.create(_cfg(3))
+    s.factorize(A)
+    assert np.array_equal(s.row_norms(), g["row_l1"])
+    comps = s.stiff_components()
+    assert np.array_equal(np.cumsum([0] + [c.size for c in comps]), g["comp_off"])
+    assert np.array_equal(np.concatenate(comps), g["comp_dofs"])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_contact_fixture_solve(gpu, precision):
+    A, b, g = _contact_fixture()
+    s = Solver.create(_cfg(3, precision=precision))
+    s.factorize(A)
+    x = s.solve(b)
+    info = s.get_info()
+    assert info["stop_reason"] == "converged_rel"
+    # operator = CSC arrays read as rows; row sums reach 6e11, so the FP64 floor of
+    # the true residual is near 1e-6 (the direct solution leaves 9e-7)
+    R = A.to_scipy().T
+    assert np.linalg.norm(R @ x - b) <= 2e-6 * np.linalg.norm(b)
+    xd = g["x_direct"]
+    assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
+    H = O.Hybrid(_ocfg(3))
+    H.factorize(A.colptr, A.rowidx, A.vals)
+    H.solve(b)
+    oi = H.info()
+    assert abs(info["num_iterations"] - oi["num_iterations"]) <= max(3, 0.05 * oi["num_iterations"])
+
+
+def test_zero_rhs_and_initial_guess(gpu):
+    prob = problems.make("elastic_contact", N=8)
+    A = prob.csc()
+    s = Solver.create(_cfg(3))
+    s.factorize(A)
+    x0 = problems.random_vector(A.n, 3)
+    x = s.solve(np.zeros(A.n), x0)
+    assert np.all(x == 0) and s.get_info()["num_iterations"] == 0
+    b = prob.rhs(1)
+    x1 = s.solve(b)
+    it1 = s.get_info()["num_iterations"]
+    # a converged initial guess stops after the first residual check
+    x2 = s.solve(b, x1.copy())
+    assert s.get_info()["num_iterations"] <= 1
+    assert np.linalg.norm(x2 - x1) <= 1e-8 * np.linalg.norm(x1)
+    # the handle keeps its factorization across solves
+    x3 = s.solve(b)
+    assert np.array_equal(x3, x1) and s.get_info()["num_iterations"] == it1
