@@ -66,9 +66,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="elastic_contact")
     ap.add_argument("--N", type=int, default=None, help="grid nodes per axis (default: the workload's)")
-    ap.add_argument("--cpu-sample-iters", type=int, default=20)
+    ap.add_argument("--cpu-sample-iters", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--max-iter", type=int, default=20000,
+                    help="PCG iteration cap (profiling runs cut it; such a line is marked invalid)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solver has no multi-GPU path.",
@@ -83,7 +87,18 @@ def main():
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
+    if args.copy_probe_mib:
+        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE per dispatch of this
+        # kernel calibrate the counters against exact byte counts
+        src = torch.full((args.copy_probe_mib << 17,), 1.0, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
     prob = problems.make(args.workload, N=args.N)
+    log("workload", args.workload, "N", prob.N)
     A = prob.csc()
     b = prob.rhs(0)
     n = A.n
@@ -96,7 +111,7 @@ def main():
     bnorm = float(np.linalg.norm(b))
     stream = torch.cuda.current_stream()
     base = {"block_size": prob.block_size, "relative_tolerance": 1e-8, "gmm_jump_threshold": 10.0,
-            "min_subdomain_size": 4, "max_iter": 20000}
+            "min_subdomain_size": 4, "max_iter": args.max_iter}
     paths = {"fp64": dict(base, precision="fp64"), "mixed": dict(base, precision="fp32")}
     res = {}
     for name, g in paths.items():
@@ -119,6 +134,7 @@ def main():
         rel = info["final_res_norm"] / bnorm
         res[name] = {"s": dt, "iters": info["num_iterations"], "rel_res": rel, "info": info,
                      "x": x.cpu().numpy().copy(), "stop": info["stop_reason"]}
+        log(name, f"{dt:.3f} s", info["num_iterations"], "iterations", info["stop_reason"])
         del s
     # roofline of the outer FP64 SpMV, timed with HIP events on the solver's stream
     g = dict(paths["mixed"], profile_kernels=True)
@@ -143,6 +159,7 @@ def main():
     x_diff = float(np.linalg.norm(mixed["x"] - fp64["x"]) / np.linalg.norm(fp64["x"]))
     cpu = None
     if not args.no_cpu:
+        log("cpu baseline sample")
         cpu = cpu_baseline(prob, A, b, fp64["iters"], args.cpu_sample_iters, args.cpu_threads)
     line = {
         "metric": "time-to-1e-8 residual (s), mixed vs FP64 GPU, 6.3M-DOF; SpMV HBM GB/s",

@@ -23,6 +23,14 @@ namespace {
 using clk = std::chrono::steady_clock;
 double since(clk::time_point t0) { return std::chrono::duration<double>(clk::now() - t0).count(); }
 
+// ORC_TIMING=1 prints the set-up phases to stderr (for sizing CPU baselines)
+void phase(const char *what, clk::time_point &t)
+{
+    static const bool on = std::getenv("ORC_TIMING") != nullptr;
+    if (on) std::fprintf(stderr, "[oracle] %-12s %.3f s\n", what, since(t));
+    t = clk::now();
+}
+
 constexpr int kChunks = 1024;  // reduction tree shape shared with the device kernels
 constexpr int kLanes = 256;
 constexpr int kLanczosSteps = 20;  // CG-Lanczos steps of the lambda_max(BA) estimate
@@ -100,9 +108,18 @@ void spmv(const Csr &A, const double *x, double *y)
 
 double dot(int64_t n, const double *a, const double *b)
 {
+    // fixed chunking, partials added in chunk order: the same bits for any thread count
+    constexpr int kChunks = 256;
+    double part[kChunks];
+    const int64_t len = (n + kChunks - 1) / kChunks;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < kChunks; ++c) {
+        double s = 0.0;
+        for (int64_t i = c * len; i < std::min(n, (c + 1) * len); ++i) s += a[i] * b[i];
+        part[c] = s;
+    }
     double s = 0.0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-    for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+    for (int c = 0; c < kChunks; ++c) s += part[c];
     return s;
 }
 
@@ -438,6 +455,11 @@ struct orc_mas {
     std::vector<int32_t> map;       // new node * MAX_COARSE + lv
     std::vector<std::vector<double>> inv;  // per level: counts[l] * dim * dim Cholesky factors (row-major lower)
     int dim = 32;
+    std::vector<int32_t> inv_perm;  // new node -> old node
+    // per coarse level: members (new fine nodes, ascending) of each coarse node
+    std::vector<std::vector<int64_t>> mem_ptr;
+    std::vector<std::vector<int32_t>> mem;
+    mutable std::vector<std::vector<double>> rl, yl;  // apply work vectors
 };
 
 extern "C" orc_mas *orc_mas_create(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, int bs,
@@ -452,21 +474,24 @@ extern "C" orc_mas *orc_mas_create(int32_t n, const int32_t *ptr, const int32_t 
     M->part_off.assign(part_off, part_off + nparts + 1);
     M->dim = B * bs;
     const int32_t nb = M->nb;
-    std::vector<int32_t> inv_perm(nb);
-    for (int32_t o = 0; o < nb; ++o) inv_perm[perm[o]] = o;
+    auto tp = clk::now();
+    M->inv_perm.assign(nb, 0);
+    for (int32_t o = 0; o < nb; ++o) M->inv_perm[perm[o]] = o;
     // block adjacency in the new numbering (pattern of the operator, MASPreconditioner.cu:128-213)
     std::vector<std::vector<int32_t>> adj(nb);
-    for (int32_t i = 0; i < n; ++i) {
-        const int32_t I = perm[i / bs];
-        for (int32_t e = ptr[i]; e < ptr[i + 1]; ++e) {
-            const int32_t J = perm[idx[e] / bs];
-            if (J != I) adj[I].push_back(J);
-        }
-    }
-    for (auto &a : adj) {
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int32_t ob = 0; ob < nb; ++ob) {
+        const int32_t I = perm[ob];
+        auto &a = adj[I];
+        for (int32_t i = ob * bs; i < std::min<int64_t>(int64_t(ob + 1) * bs, n); ++i)
+            for (int32_t e = ptr[i]; e < ptr[i + 1]; ++e) {
+                const int32_t J = perm[idx[e] / bs];
+                if (J != I) a.push_back(J);
+            }
         std::sort(a.begin(), a.end());
         a.erase(std::unique(a.begin(), a.end()), a.end());
     }
+    phase("mas.adj", tp);
     std::vector<int32_t> bank0(nb), local0(nb);
     for (int32_t p = 0; p < nparts; ++p)
         for (int32_t v = part_off[p]; v < part_off[p + 1]; ++v) {
@@ -557,29 +582,60 @@ extern "C" orc_mas *orc_mas_create(int32_t n, const int32_t *ptr, const int32_t 
 
     // matrices (MASPreconditioner.cu:494-591, 779-796) and inverses (799-936)
     const int D = M->dim;
+    phase("mas.cco", tp);
     M->inv.resize(M->levels);
     for (int L = 0; L < M->levels; ++L) M->inv[L].assign(size_t(M->counts[L]) * D * D, 0.0);
+    // level 0: every scalar row fills its own bank row
+#pragma omp parallel for schedule(static, 4096)
     for (int32_t i = 0; i < n; ++i) {
         const int32_t I = perm[i / bs];
         const int a = i % bs;
+        double *m = &M->inv[0][size_t(bank0[I]) * D * D + size_t(local0[I] * bs + a) * D];
         for (int32_t e = ptr[i]; e < ptr[i + 1]; ++e) {
-            const int32_t j = idx[e];
-            const int32_t J = perm[j / bs];
-            const int b = j % bs;
-            const double v = vals[e];
-            if (bank0[I] == bank0[J]) {
-                double *m = &M->inv[0][size_t(bank0[I]) * D * D];
-                m[size_t(local0[I] * bs + a) * D + local0[J] * bs + b] += v;
-            }
-            for (int lv = 0; lv < level_num; ++lv) {
-                const int32_t ci = M->map[size_t(I) * orc_mas::MAX_COARSE + lv];
-                const int32_t cj = M->map[size_t(J) * orc_mas::MAX_COARSE + lv];
-                if (ci / B != cj / B) continue;
-                double *m = &M->inv[lv + 1][size_t(ci / B) * D * D];
-                m[size_t((ci % B) * bs + a) * D + (cj % B) * bs + b] += v;
+            const int32_t J = perm[idx[e] / bs];
+            if (bank0[I] == bank0[J]) m[local0[J] * bs + idx[e] % bs] += vals[e];
+        }
+    }
+    // members of each coarse node, ascending in the new numbering
+    M->mem_ptr.assign(level_num, {});
+    M->mem.assign(level_num, {});
+    for (int lv = 0; lv < level_num; ++lv) {
+        auto &mp = M->mem_ptr[lv];
+        auto &mm = M->mem[lv];
+        mp.assign(size_t(M->cco[lv]) + 1, 0);
+        for (int32_t v = 0; v < nb; ++v) mp[M->map[size_t(v) * orc_mas::MAX_COARSE + lv] + 1]++;
+        for (int32_t c = 0; c < M->cco[lv]; ++c) mp[c + 1] += mp[c];
+        mm.resize(nb);
+        std::vector<int64_t> nx(mp.begin(), mp.end() - 1);
+        for (int32_t v = 0; v < nb; ++v) mm[nx[M->map[size_t(v) * orc_mas::MAX_COARSE + lv]]++] = v;
+    }
+    // coarse levels: Galerkin sums accumulated in ascending scalar row order, one
+    // coarse bank per task (its fine rows gathered in that order)
+    for (int lv = 0; lv < level_num; ++lv) {
+        const int32_t nbank = M->counts[lv + 1];
+        std::vector<std::vector<int32_t>> rows(nbank);
+        for (int32_t ob = 0; ob < nb; ++ob) {
+            const int32_t ci = M->map[size_t(perm[ob]) * orc_mas::MAX_COARSE + lv];
+            rows[ci / B].push_back(ob);
+        }
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int32_t k = 0; k < nbank; ++k) {
+            double *mk = &M->inv[lv + 1][size_t(k) * D * D];
+            for (int32_t ob : rows[k]) {
+                const int32_t ci = M->map[size_t(perm[ob]) * orc_mas::MAX_COARSE + lv];
+                for (int32_t i = ob * bs; i < std::min<int64_t>(int64_t(ob + 1) * bs, n); ++i) {
+                    const int a = i % bs;
+                    for (int32_t e = ptr[i]; e < ptr[i + 1]; ++e) {
+                        const int32_t J = perm[idx[e] / bs];
+                        const int32_t cj = M->map[size_t(J) * orc_mas::MAX_COARSE + lv];
+                        if (ci / B != cj / B) continue;
+                        mk[size_t((ci % B) * bs + a) * D + (cj % B) * bs + idx[e] % bs] += vals[e];
+                    }
+                }
             }
         }
     }
+    phase("mas.assemble", tp);
     for (int L = 0; L < M->levels; ++L) {
         const int32_t cnt = M->counts[L];
         bool ok = true;
@@ -597,6 +653,7 @@ extern "C" orc_mas *orc_mas_create(int32_t n, const int32_t *ptr, const int32_t 
             return nullptr;
         }
     }
+    phase("mas.factor", tp);
     return M;
 }
 
@@ -633,28 +690,35 @@ extern "C" void orc_mas_apply(const orc_mas *m, const double *r, double *z)
 {
     // MASPreconditioner.cu:1099-1164 (gather, per-bank inverse, prolongation)
     const int B = orc_mas::BANK, bs = m->bs, D = m->dim;
-    const int32_t nb = m->nb;
-    std::vector<int32_t> inv_perm(nb);
-    for (int32_t o = 0; o < nb; ++o) inv_perm[m->perm[o]] = o;
-    std::vector<std::vector<double>> rl(m->levels), yl(m->levels);
+    const auto &inv_perm = m->inv_perm;
+    auto &rl = m->rl, &yl = m->yl;
+    rl.resize(m->levels);
+    yl.resize(m->levels);
     for (int L = 0; L < m->levels; ++L) {
         rl[L].assign(size_t(m->counts[L]) * D, 0.0);
-        yl[L].assign(size_t(m->counts[L]) * D, 0.0);
+        yl[L].resize(size_t(m->counts[L]) * D);
     }
     const int32_t nparts = m->counts[0];
+    auto fine = [&](int32_t v, int a) {
+        const int64_t s = int64_t(inv_perm[v]) * bs + a;
+        return s < m->n ? r[s] : 0.0;
+    };
+#pragma omp parallel for schedule(static, 64)
     for (int32_t p = 0; p < nparts; ++p)
-        for (int32_t v = m->part_off[p]; v < m->part_off[p + 1]; ++v) {
-            const int32_t old = inv_perm[v];
+        for (int32_t v = m->part_off[p]; v < m->part_off[p + 1]; ++v)
+            for (int a = 0; a < bs; ++a) rl[0][size_t(p) * D + (v - m->part_off[p]) * bs + a] = fine(v, a);
+    // restriction: members summed in ascending order
+    for (int lv = 0; lv + 1 < m->levels; ++lv) {
+        const auto &mp = m->mem_ptr[lv];
+        const auto &mm = m->mem[lv];
+#pragma omp parallel for schedule(static, 1024)
+        for (int32_t c = 0; c < m->cco[lv]; ++c)
             for (int a = 0; a < bs; ++a) {
-                const int64_t s = int64_t(old) * bs + a;
-                const double val = s < m->n ? r[s] : 0.0;
-                rl[0][size_t(p) * D + (v - m->part_off[p]) * bs + a] = val;
-                for (int lv = 0; lv + 1 < m->levels; ++lv) {
-                    const int32_t c = m->map[size_t(v) * orc_mas::MAX_COARSE + lv];
-                    rl[lv + 1][size_t(c) * bs + a] += val;
-                }
+                double acc = 0.0;
+                for (int64_t q = mp[c]; q < mp[c + 1]; ++q) acc += fine(mm[q], a);
+                rl[lv + 1][size_t(c) * bs + a] = acc;
             }
-        }
+    }
     for (int L = 0; L < m->levels; ++L) {
         const int32_t cnt = m->counts[L];
 #pragma omp parallel for schedule(static)
@@ -665,6 +729,7 @@ extern "C" void orc_mas_apply(const orc_mas *m, const double *r, double *z)
             chol_solve(D, Lf, y);
         }
     }
+#pragma omp parallel for schedule(static, 64)
     for (int32_t p = 0; p < nparts; ++p)
         for (int32_t v = m->part_off[p]; v < m->part_off[p + 1]; ++v) {
             const int32_t old = inv_perm[v];
@@ -881,6 +946,7 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
     const orc_config &c = h->cfg;
     h->comps.clear();
     h->chol.clear();
+    auto tp = clk::now();
     if (c.do_hybrid_precond) {
         // GPUHybrid.cu:214-264
         std::vector<int32_t> S;
@@ -917,6 +983,7 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
         } else if (!S.empty()) {
             h->comps.push_back(S);
         }
+        phase("select", tp);
         // dense sub-blocks S_c A S_c^T (GPUHybrid.cu:1351-1383 semantics) + Cholesky
         h->chol.resize(h->comps.size());
         bool ok = true;
@@ -936,6 +1003,7 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
             h->chol[k] = std::move(D);
         }
         if (!ok) return -5;
+        phase("stiff", tp);
     }
     // base preconditioner
     orc_mas_destroy(h->mas);
@@ -944,17 +1012,19 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
     if (c.base_precond == 0) {
         const int32_t nb = (n + bs - 1) / bs;
         std::vector<std::vector<int32_t>> adj(nb);
-        for (int32_t i = 0; i < n; ++i)
-            for (int32_t e = A.ptr[i]; e < A.ptr[i + 1]; ++e) adj[i / bs].push_back(A.idx[e] / bs);
-        std::vector<int64_t> bptr(nb + 1, 0);
-        std::vector<int32_t> bidx;
+#pragma omp parallel for schedule(dynamic, 1024)
         for (int32_t I = 0; I < nb; ++I) {
             auto &a = adj[I];
+            for (int32_t i = I * bs; i < std::min<int64_t>(int64_t(I + 1) * bs, n); ++i)
+                for (int32_t e = A.ptr[i]; e < A.ptr[i + 1]; ++e) a.push_back(A.idx[e] / bs);
             std::sort(a.begin(), a.end());
             a.erase(std::unique(a.begin(), a.end()), a.end());
-            bidx.insert(bidx.end(), a.begin(), a.end());
-            bptr[I + 1] = int64_t(bidx.size());
         }
+        std::vector<int64_t> bptr(nb + 1, 0);
+        for (int32_t I = 0; I < nb; ++I) bptr[I + 1] = bptr[I] + int64_t(adj[I].size());
+        std::vector<int32_t> bidx(bptr[nb]);
+#pragma omp parallel for schedule(static, 4096)
+        for (int32_t I = 0; I < nb; ++I) std::copy(adj[I].begin(), adj[I].end(), bidx.begin() + bptr[I]);
         std::vector<int32_t> part_of(nb);
         int32_t np = 0;
         if (!h->given_perm.empty() && int32_t(h->given_perm.size()) == nb) {
@@ -963,7 +1033,9 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
                 part_of[I] = int32_t(std::upper_bound(h->given_part_off.begin(), h->given_part_off.end(),
                                                       h->given_perm[I]) - h->given_part_off.begin()) - 1;
         } else {
+            phase("block_graph", tp);
             orc_partition(nb, bptr.data(), bidx.data(), 32, part_of.data(), &np);
+            phase("partition", tp);
         }
         // permutation sorted by part, stable in node id (MASSolver.cu:90-118)
         h->part_off.assign(np + 1, 0);
@@ -974,6 +1046,7 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
         for (int32_t I = 0; I < nb; ++I) h->perm[I] = next[part_of[I]]++;
         h->mas = orc_mas_create(n, A.ptr, A.idx, A.vals, bs, h->perm.data(), h->part_off.data(), np);
         if (!h->mas) return -5;
+        phase("mas", tp);
     } else if (c.base_precond == 1) {
         h->jacobi.assign(n, 1.0);
         for (int32_t i = 0; i < n; ++i)
@@ -989,6 +1062,7 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
     if (c.do_hybrid_precond && !h->comps.empty() && c.composition == 0) {
         const double theta = lanczos_lambda_max(*h, n);
         h->omega = (c.damping > 0 ? c.damping : 1.0) / theta;
+        phase("lanczos", tp);
     }
     h->t_setup = since(t0);
     return 0;
