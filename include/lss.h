@@ -105,11 +105,32 @@ const char *lss_name(const lss_solver *s);
 int lss_get_stiff_dofs(const lss_solver *s, int32_t *dofs, int64_t cap, int64_t *n_dofs, int64_t *comp_off,
                        int64_t comp_cap, int64_t *n_comp);
 
-/* Last error message of this handle (empty string when none). */
+/* Last error message of this handle (empty string when none).  With s == NULL:
+ * the message of the last failed handle-free call (lss_adjoint_*) on this thread. */
 const char *lss_last_error(const lss_solver *s);
 
 /* Release all device memory.  Replaces GPUHybrid::~GPUHybrid (GPUHybrid.cu:1514-1530). */
 void lss_destroy(lss_solver *s);
+
+/* ---- adjoint of the solve ------------------------------------------------
+ * The reference has no adjoint.  For a loss L(x) with A x = b solved by a
+ * handle, the caller solves A^T lam = dL/dx with the same handle (the operator
+ * is symmetric, so A^T = A and the factorization is reused: lss_solve_device
+ * with the right-hand side dL/dx); then dL/db = lam and the two calls below
+ * give the matrix and per-element gradients.  Both are handle-free, run
+ * asynchronously on `stream` (NULL = the null stream) and take device pointers.
+ *
+ * dL/dA on the stored pattern: d_grad[e] = -lam[j] * x[d_rowidx[e]] for every
+ * entry e of CSC column j, i.e. the gradient with respect to operator entry
+ * (j, rowidx[e]) (CSC column j is operator row j, GPUHybrid.cu:401-433).
+ * d_grad has nnz entries in the order of d_rowidx. */
+int lss_adjoint_pattern_grad(int32_t n, int64_t nnz, const int32_t *d_colptr, const int32_t *d_rowidx,
+                             const double *d_lam, const double *d_x, double *d_grad, void *stream);
+/* Chain rule to parameters p_s through the sparse Jacobian d vals / d p:
+ * d_out[s] = sum over k in [d_seg_ptr[s], d_seg_ptr[s+1]) of d_coef[k] * d_grad_vals[d_slot[k]],
+ * e.g. per-element coefficients with (slot, unit-coefficient value) pairs per element. */
+int lss_adjoint_chain(int64_t nseg, const int64_t *d_seg_ptr, const int32_t *d_slot, const double *d_coef,
+                      const double *d_grad_vals, double *d_out, void *stream);
 
 /* ---- inspection entry points (parity tests and tools) --------------------
  * Each runs the named device stage of the factorized solver on host data.  No

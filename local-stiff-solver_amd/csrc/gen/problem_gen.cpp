@@ -552,6 +552,83 @@ int lssgen_grad_coef(const lssgen_problem *P, const double *lam, const double *x
     return 0;
 }
 
+extern "C++" {
+namespace {
+// Visit the unmasked entries (row, col, value) of element e's unit matrix in a
+// fixed order (a, i, b, j); masking as in lssgen_grad_coef.
+template <class F>
+void for_element_entries(const lssgen_problem *P, int64_t e, const double *K, F &&f)
+{
+    const int bs = P->bs, m = 4 * bs;
+    const auto cs = tet_corners(static_cast<int>(e % 6));
+    int nodes[4];
+    for (int a = 0; a < 4; ++a) nodes[a] = P->corner_node(e / 6, cs[a]);
+    for (int a = 0; a < 4; ++a)
+        for (int i = 0; i < bs; ++i)
+            for (int b = 0; b < 4; ++b) {
+                const bool masked = P->dirichlet[nodes[a]] || P->dirichlet[nodes[b]];
+                if (masked && nodes[a] != nodes[b]) continue;
+                for (int j = 0; j < bs; ++j) {
+                    if (masked && i != j) continue;
+                    f(int64_t(nodes[a]) * bs + i, int64_t(nodes[b]) * bs + j, K[(a * bs + i) * m + (b * bs + j)]);
+                }
+            }
+}
+}  // namespace
+}  // extern "C++"
+
+int64_t lssgen_element_jacobian_nnz(const lssgen_problem *P)
+{
+    const int m = 4 * P->bs;
+    int64_t total = 0;
+#pragma omp parallel
+    {
+        std::vector<double> K(m * m, 0.0);
+#pragma omp for schedule(static) reduction(+ : total)
+        for (int64_t e = 0; e < P->n_elem; ++e)
+            for_element_entries(P, e, K.data(), [&](int64_t, int64_t, double) { ++total; });
+    }
+    return total;
+}
+
+int lssgen_element_jacobian(const lssgen_problem *P, int64_t *seg_ptr, int32_t *slot, double *val)
+{
+    const int m = 4 * P->bs;
+    const int64_t n = P->n_nodes * P->bs;
+    std::vector<int32_t> colptr(n + 1), rowidx(P->nnz);
+    std::vector<double> tmp(P->nnz);
+    lssgen_fill_csc(P, colptr.data(), rowidx.data(), tmp.data());
+    seg_ptr[0] = 0;
+    {
+        std::vector<double> K(m * m, 0.0);
+        for (int64_t e = 0; e < P->n_elem; ++e) {
+            int64_t c = 0;
+            for_element_entries(P, e, K.data(), [&](int64_t, int64_t, double) { ++c; });
+            seg_ptr[e + 1] = seg_ptr[e] + c;
+        }
+    }
+    int bad = 0;
+#pragma omp parallel
+    {
+        std::vector<double> K(m * m);
+#pragma omp for schedule(static) reduction(| : bad)
+        for (int64_t e = 0; e < P->n_elem; ++e) {
+            P->unit_element(e, K.data());
+            int64_t w = seg_ptr[e];
+            // the CSC column of operator row r holds that row's entries
+            for_element_entries(P, e, K.data(), [&](int64_t r, int64_t c, double v) {
+                const int32_t *b = rowidx.data() + colptr[r], *en = rowidx.data() + colptr[r + 1];
+                const int32_t *it = std::lower_bound(b, en, static_cast<int32_t>(c));
+                if (it == en || *it != c) bad = 1;
+                slot[w] = static_cast<int32_t>(it - rowidx.data());
+                val[w] = v;
+                ++w;
+            });
+        }
+    }
+    return bad ? -1 : 0;
+}
+
 int lssgen_fill_vals_with_coef(lssgen_problem *P, const double *coef, double *vals)
 {
     std::copy(coef, coef + P->n_elem, P->coef.begin());

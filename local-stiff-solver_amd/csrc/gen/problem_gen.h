@@ -81,6 +81,16 @@ int lssgen_element_nodes(const lssgen_problem *p, int32_t *out);
 int lssgen_grad_coef(const lssgen_problem *p, const double *lam, const double *x, double *out);
 
 /*
+ * Sparse Jacobian d vals / d coef in element-major form: element e owns
+ * entries [seg_ptr[e], seg_ptr[e+1]) of (slot into the CSC arrays of
+ * lssgen_fill_csc, value of its unit-coefficient element matrix), with the
+ * Dirichlet masking applied.  vals = J coef + (contact terms).
+ * lssgen_element_jacobian_nnz gives the entry count; returns 0 on success.
+ */
+int64_t lssgen_element_jacobian_nnz(const lssgen_problem *p);
+int lssgen_element_jacobian(const lssgen_problem *p, int64_t *seg_ptr, int32_t *slot, double *val);
+
+/*
  * Rebuild the matrix values with per-element coefficients `coef` (same pattern),
  * then refill vals[nnz] in the CSC order of lssgen_fill_csc.
  */

@@ -8,6 +8,7 @@
 #include "lss.h"
 #include "config.hpp"
 #include "solver.hpp"
+#include "../kernels/kernels.hpp"
 
 struct lss_solver {
     std::unique_ptr<lss::HybridSolver> impl;
@@ -15,6 +16,25 @@ struct lss_solver {
 };
 
 namespace {
+
+// message of the last failed handle-free call on this thread
+thread_local std::string g_error;
+
+template <class F>
+int guarded_free(F &&f)
+{
+    try {
+        f();
+        g_error.clear();
+        return LSS_OK;
+    } catch (const lss::Error &e) {
+        g_error = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return LSS_ERR_HIP;
+    }
+}
 
 template <class F>
 int guarded(lss_solver *s, F &&f)
@@ -176,8 +196,28 @@ int lss_get_stiff_dofs(const lss_solver *s, int32_t *dofs, int64_t cap, int64_t 
 
 const char *lss_last_error(const lss_solver *s)
 {
-    if (!s) return "null handle";
+    if (!s) return g_error.empty() ? "null handle" : g_error.c_str();
     return s->last_error.c_str();
+}
+
+int lss_adjoint_pattern_grad(int32_t n, int64_t nnz, const int32_t *d_colptr, const int32_t *d_rowidx,
+                             const double *d_lam, const double *d_x, double *d_grad, void *stream)
+{
+    return guarded_free([&] {
+        if (n < 0 || nnz < 0 || (n > 0 && (!d_colptr || !d_lam || !d_x)) || (nnz > 0 && (!d_rowidx || !d_grad)))
+            lss::fail(LSS_ERR_ARG, "invalid arguments");
+        lss::k::pattern_grad(n, d_colptr, d_rowidx, d_lam, d_x, d_grad, static_cast<hipStream_t>(stream));
+    });
+}
+
+int lss_adjoint_chain(int64_t nseg, const int64_t *d_seg_ptr, const int32_t *d_slot, const double *d_coef,
+                      const double *d_grad_vals, double *d_out, void *stream)
+{
+    return guarded_free([&] {
+        if (nseg < 0 || (nseg > 0 && (!d_seg_ptr || !d_slot || !d_coef || !d_grad_vals || !d_out)))
+            lss::fail(LSS_ERR_ARG, "invalid arguments");
+        lss::k::segment_dot(nseg, d_seg_ptr, d_slot, d_coef, d_grad_vals, d_out, static_cast<hipStream_t>(stream));
+    });
 }
 
 void lss_destroy(lss_solver *s) { delete s; }

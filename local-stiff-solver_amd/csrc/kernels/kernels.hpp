@@ -82,4 +82,12 @@ void gmm_var_global(int64_t n, const double *x, double mean, double *partials1, 
 void csum_finish(int nsums, const double *partials, double *out, hipStream_t s);
 void gmm_flags(int64_t n, const double *g0, const double *g1, uint8_t *flags, hipStream_t s);
 
+// ---------------------------------------------------------------- adjoint
+// grad[e] = -lam[j] * x[rowidx[e]] for every entry e of CSC column j (operator entry (j, rowidx[e])).
+void pattern_grad(int32_t n, const int32_t *colptr, const int32_t *rowidx, const double *lam, const double *x,
+                  double *grad, hipStream_t s);
+// out[s] = sum over k in [ptr[s], ptr[s+1]) of coef[k] * g[slot[k]].
+void segment_dot(int64_t nseg, const int64_t *ptr, const int32_t *slot, const double *coef, const double *g,
+                 double *out, hipStream_t s);
+
 }  // namespace lss::k

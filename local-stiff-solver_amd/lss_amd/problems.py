@@ -75,6 +75,9 @@ def _load():
     lib.lssgen_element_nodes.argtypes = [P, P]
     lib.lssgen_grad_coef.argtypes = [P, P, P, P]
     lib.lssgen_fill_vals_with_coef.argtypes = [P, P, P]
+    lib.lssgen_element_jacobian_nnz.restype = ctypes.c_int64
+    lib.lssgen_element_jacobian_nnz.argtypes = [P]
+    lib.lssgen_element_jacobian.argtypes = [P, P, P, P]
     _lib = lib
     return lib
 
@@ -183,6 +186,17 @@ class Problem:
         out = np.empty(self.n_elements, np.float64)
         _load().lssgen_grad_coef(self._h, _ptr(lam), _ptr(x), _ptr(out))
         return out
+
+    def element_jacobian(self):
+        """d vals / d coef, element-major: (seg_ptr[n_elem+1] int64, slot int32, value float64)."""
+        lib = _load()
+        m = lib.lssgen_element_jacobian_nnz(self._h)
+        seg = np.empty(self.n_elements + 1, np.int64)
+        slot = np.empty(m, np.int32)
+        val = np.empty(m, np.float64)
+        if lib.lssgen_element_jacobian(self._h, _ptr(seg), _ptr(slot), _ptr(val)) != 0:
+            raise RuntimeError("element Jacobian: an element entry is missing from the pattern")
+        return seg, slot, val
 
     def vals_with_coef(self, coef: np.ndarray) -> np.ndarray:
         """Matrix values (CSC order) re-assembled with per-element coefficients."""

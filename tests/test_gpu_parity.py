@@ -21,6 +21,13 @@ def _ocfg(bs, **kw):
     return O.config(**base)
 
 
+def _res_ok(S, x, b, rtol=1e-8):
+    """||b - A x|| within the recurrence tolerance, or within the FP64 floor
+    eps * || |A| |x| || that stiff contact entries impose on any computed residual."""
+    floor = 50 * np.finfo(np.float64).eps * np.linalg.norm(abs(S) @ np.abs(x))
+    return np.linalg.norm(S @ x - b) <= max(1.01 * rtol * np.linalg.norm(b), floor)
+
+
 CASES = [("laplace_sphere", 12), ("elastic_contact", 10), ("elastic_slivers", 8), ("elastic_logE", 8)]
 
 
@@ -100,7 +107,7 @@ def test_solve_matches_oracle(gpu, name, N):
     xr = H.solve(b)
     oi = H.info()
     S = A.to_scipy()
-    assert np.linalg.norm(S @ x - b) <= 1.01e-8 * np.linalg.norm(b)
+    assert _res_ok(S, x, b)
     assert info["omega"] == pytest.approx(oi["omega"], rel=1e-6)
     assert abs(info["num_iterations"] - oi["num_iterations"]) <= max(2, 0.05 * oi["num_iterations"])
     assert np.linalg.norm(x - xr) <= 1e-5 * np.linalg.norm(xr)
@@ -150,7 +157,7 @@ def test_contact_fixture_selection(gpu):
     assert np.array_equal(np.concatenate(comps), g["comp_dofs"])
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
 def test_contact_fixture_solve(gpu, precision):
     A, b, g = _contact_fixture()
     s = Solver.create(_cfg(3, precision=precision))
@@ -168,7 +175,9 @@ def test_contact_fixture_solve(gpu, precision):
     H.factorize(A.colptr, A.rowidx, A.vals)
     H.solve(b)
     oi = H.info()
-    assert abs(info["num_iterations"] - oi["num_iterations"]) <= max(3, 0.05 * oi["num_iterations"])
+    # BF16 bank factors cost some iterations; the FP64 outer PCG keeps the accuracy
+    slack = 0.15 if precision == "bf16" else 0.05
+    assert abs(info["num_iterations"] - oi["num_iterations"]) <= max(3, slack * oi["num_iterations"])
 
 
 def test_zero_rhs_and_initial_guess(gpu):
@@ -189,3 +198,25 @@ def test_zero_rhs_and_initial_guess(gpu):
     # the handle keeps its factorization across solves
     x3 = s.solve(b)
     assert np.array_equal(x3, x1) and s.get_info()["num_iterations"] == it1
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("name,N", [("elastic_contact", 12), ("laplace_sphere", 14)])
+def test_reduced_precision_banks_converge(gpu, name, N, precision):
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    b = prob.rhs(0)
+    s64 = Solver.create(_cfg(prob.block_size))
+    s64.factorize(A)
+    x64 = s64.solve(b)
+    it64 = s64.get_info()["num_iterations"]
+    s = Solver.create(_cfg(prob.block_size, precision=precision))
+    s.factorize(A)
+    x = s.solve(b)
+    info = s.get_info()
+    assert info["stop_reason"] == "converged_rel"
+    assert _res_ok(A.to_scipy(), x, b)
+    assert np.linalg.norm(x - x64) <= 1e-6 * np.linalg.norm(x64)
+    # the FP64 outer PCG recovers the accuracy; FP32 bank rounding costs few
+    # iterations, BF16 rounding of contact banks (entries spanning 1e8) up to ~2.2x
+    assert info["num_iterations"] <= (1.25 if precision == "fp32" else 2.5) * it64 + 3
