@@ -91,6 +91,7 @@ Config parse_config(const char *text, const Config &base)
         else if (k == "max_dense_component") c.max_dense_component = as_int(v, k);
         else if (k == "profile_kernels") c.profile_kernels = as_bool(v, k);
         else if (k == "true_residual_check") c.true_residual_check = as_bool(v, k);
+        else if (k == "partitioner") c.partitioner = as_str(v, k);
         else bad("unknown key GPUHybrid/" + k);
     }
     if (c.block_size < 1 || c.block_size > 3) bad("block_size must be 1, 2 or 3");
@@ -99,6 +100,7 @@ Config parse_config(const char *text, const Config &base)
     if (c.inner_spmv != "auto" && c.inner_spmv != "fp64" && c.inner_spmv != "fp32") bad("inner_spmv must be auto|fp64|fp32");
     if (c.base_preconditioner != "MAS" && c.base_preconditioner != "Jacobi") bad("base_preconditioner must be MAS|Jacobi");
     if (c.check_every < 1) bad("check_every must be >= 1");
+    if (c.partitioner != "device" && c.partitioner != "host") bad("partitioner must be device|host");
     if (!(c.damping > 0.0) || c.damping >= 2.0) bad("damping must be in (0, 2)");
     if (c.lanczos_steps < 1) bad("lanczos_steps must be >= 1");
     return c;

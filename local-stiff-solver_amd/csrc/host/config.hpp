@@ -36,6 +36,7 @@ struct Config {
     int64_t max_dense_component = 16384;  // largest stiff component for the dense Cholesky
     bool profile_kernels = false;       // time SpMV / MAS launches with HIP events
     bool true_residual_check = false;   // on convergence verify ||b - A x|| and restart PCG from it if needed
+    std::string partitioner = "device";  // MAS partition: "device" (HIP kernels) | "host" (OpenMP)
 
     // FP32 matrix values in the preconditioner-internal SpMV only on request: next to
     // an FP64 stiff-block solve they make the sweep inconsistent (an FP32 rounding of

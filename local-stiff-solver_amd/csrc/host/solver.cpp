@@ -351,7 +351,16 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
 
     // partition (MAS banks) and permutation
     auto tp = clk::now();
-    if (cfg_.base_preconditioner == "MAS") {
+    if (cfg_.base_preconditioner == "MAS" && cfg_.partitioner == "device") {
+        DBuf<int32_t> d_off;
+        d_perm_.resize(nb_);
+        const int32_t nparts = k::device_partition(nb_, d_rp0.get(), d_c0.get(), 32, d_perm_.get(), d_off, s);
+        perm_.resize(nb_);
+        part_off_.resize(size_t(nparts) + 1);
+        d_perm_.download(perm_.data(), nb_, s);
+        d_off.download(part_off_.data(), part_off_.size(), s);
+        sync(s);
+    } else if (cfg_.base_preconditioner == "MAS") {
         std::vector<int32_t> part_of;
         int32_t nparts = 0;
         partition_graph(g, 32, part_of, nparts);
@@ -363,7 +372,7 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     }
     Pattern gp = permute_pattern(g, perm_);
     t_partition_ = since(tp);
-    d_perm_.upload(perm_.data(), nb_, s);
+    if (!(cfg_.base_preconditioner == "MAS" && cfg_.partitioner == "device")) d_perm_.upload(perm_.data(), nb_, s);
     d_rowptr_.upload(gp.rowptr.data(), nb_ + 1, s);
     d_cols_.resize(std::max<int64_t>(nnzb_, 1));
     d_bvals_.resize(std::max<int64_t>(nnzb_ * bb, 1));

@@ -5,6 +5,7 @@
 #include <cstdint>
 
 #include "device_utils.hpp"
+#include "../common/runtime.hpp"
 
 namespace lss::k {
 
@@ -81,6 +82,13 @@ void gmm_var(int64_t n, const double *x, const double *g0, const double *g1, dou
 void gmm_var_global(int64_t n, const double *x, double mean, double *partials1, hipStream_t s);
 void csum_finish(int nsums, const double *partials, double *out, hipStream_t s);
 void gmm_flags(int64_t n, const double *g0, const double *g1, uint8_t *flags, hipStream_t s);
+
+// ---------------------------------------------------------------- partition
+// Recursive breadth-first bisection of the block graph (rowptr/cols, rows
+// ascending) into parts of at most max_part nodes; writes perm[old] = new and
+// part_off (nparts + 1 entries, resized); returns nparts.  Synchronizes s.
+int32_t device_partition(int32_t nb, const int32_t *rowptr, const int32_t *cols, int max_part, int32_t *perm,
+                         DBuf<int32_t> &part_off, hipStream_t s);
 
 // ---------------------------------------------------------------- adjoint
 // grad[e] = -lam[j] * x[rowidx[e]] for every entry e of CSC column j (operator entry (j, rowidx[e])).
