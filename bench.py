@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-iter", type=int, default=20000,
                     help="PCG iteration cap (profiling runs cut it; such a line is marked invalid)")
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=JSON",
+                    help="extra GPUHybrid configuration entry for both paths (experiments)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (HBM counter calibration)")
     args = ap.parse_args()
@@ -112,6 +114,9 @@ def main():
     stream = torch.cuda.current_stream()
     base = {"block_size": prob.block_size, "relative_tolerance": 1e-8, "gmm_jump_threshold": 10.0,
             "min_subdomain_size": 4, "max_iter": args.max_iter}
+    for kv in args.set:
+        k, v = kv.split("=", 1)
+        base[k] = json.loads(v)
     paths = {"fp64": dict(base, precision="fp64"), "mixed": dict(base, precision="fp32")}
     res = {}
     for name, g in paths.items():

@@ -8,38 +8,51 @@ namespace lss {
 
 namespace {
 
-// Breadth-first order of `set` (all labelled `label` in lab[]) inside its
-// induced subgraph, from `start`, continuing from the smallest unvisited member
-// when the subgraph is disconnected.
+// Members of `set` (all labelled `label` in lab[]) ordered by (breadth-first
+// distance from `start` inside the induced subgraph, node id).  A search that
+// runs out restarts from the smallest unreached member, one level beyond the
+// deepest level so far.  dist[] is scratch for the members of `set`.
 void bfs_order(const Pattern &g, const std::vector<int32_t> &set, int32_t start, int32_t label,
-               std::vector<int32_t> &lab, std::vector<int32_t> &order)
+               std::vector<int32_t> &lab, std::vector<int32_t> &dist, std::vector<int32_t> &order)
 {
-    order.clear();
-    order.reserve(set.size());
     const int32_t done = -2 - label;  // visited marker, distinct per subproblem
-    auto visit_from = [&](int32_t s0) {
-        size_t head = order.size();
+    std::vector<int32_t> queue;
+    queue.reserve(set.size());
+    int32_t deepest = 0;
+    auto visit_from = [&](int32_t s0, int32_t d0) {
+        size_t head = queue.size();
         lab[s0] = done;
-        order.push_back(s0);
-        while (head < order.size()) {
-            const int32_t u = order[head++];
+        dist[s0] = d0;
+        deepest = std::max(deepest, d0);
+        queue.push_back(s0);
+        while (head < queue.size()) {
+            const int32_t u = queue[head++];
+            const int32_t du = dist[u] + 1;
             for (int32_t e = g.rowptr[u]; e < g.rowptr[u + 1]; ++e) {
                 const int32_t w = g.cols[e];
                 if (lab[w] == label) {
                     lab[w] = done;
-                    order.push_back(w);
+                    dist[w] = du;
+                    deepest = std::max(deepest, du);
+                    queue.push_back(w);
                 }
             }
         }
     };
-    visit_from(start);
-    if (order.size() < set.size())
+    visit_from(start, 0);
+    if (queue.size() < set.size())
         for (int32_t v : set)  // set is ascending
-            if (lab[v] == label) visit_from(v);
+            if (lab[v] == label) visit_from(v, deepest + 1);
+    // stable counting sort of the ascending set by distance
+    std::vector<int64_t> cnt(size_t(deepest) + 2, 0);
+    for (int32_t v : set) cnt[dist[v] + 1]++;
+    for (int32_t d = 0; d <= deepest; ++d) cnt[d + 1] += cnt[d];
+    order.resize(set.size());
+    for (int32_t v : set) order[cnt[dist[v]]++] = v;
 }
 
 void bisect(const Pattern &g, std::vector<int32_t> set, int max_part, int32_t part_base, int32_t label,
-            std::vector<int32_t> &lab, std::vector<int32_t> &part_of, int depth)
+            std::vector<int32_t> &lab, std::vector<int32_t> &dist, std::vector<int32_t> &part_of, int depth)
 {
     const int64_t m = int64_t(set.size());
     if (m <= max_part) {
@@ -49,13 +62,13 @@ void bisect(const Pattern &g, std::vector<int32_t> set, int max_part, int32_t pa
     const int64_t k = (m + max_part - 1) / max_part;  // parts in this subtree
     const int64_t kl = (k + 1) / 2;
     const int64_t nl = kl * max_part;                   // left half holds kl full parts
-    // pseudo-peripheral start: the last node reached from the smallest member
+    // pseudo-peripheral start: the farthest node from the smallest member (largest id among ties)
     for (int32_t v : set) lab[v] = label;
     std::vector<int32_t> order;
-    bfs_order(g, set, set.front(), label, lab, order);
-    const int32_t start = order.back();
+    bfs_order(g, set, set.front(), label, lab, dist, order);
+    const int32_t start = order.back();  // farthest, then largest id
     for (int32_t v : set) lab[v] = label;
-    bfs_order(g, set, start, label, lab, order);
+    bfs_order(g, set, start, label, lab, dist, order);
     // halves in ascending order: mark the first nl nodes, then one scan of `set`
     const int32_t lbl_l = 2 * label + 1, lbl_r = 2 * label + 2;
     for (int64_t q = 0; q < nl; ++q) lab[order[q]] = lbl_l;
@@ -66,14 +79,14 @@ void bisect(const Pattern &g, std::vector<int32_t> set, int max_part, int32_t pa
     std::vector<int32_t>().swap(order);
     std::vector<int32_t>().swap(set);
     if (depth < 12) {
-#pragma omp task default(none) firstprivate(L, max_part, part_base, lbl_l, depth) shared(g, lab, part_of)
-        bisect(g, std::move(L), max_part, part_base, lbl_l, lab, part_of, depth + 1);
-#pragma omp task default(none) firstprivate(R, max_part, part_base, kl, lbl_r, depth) shared(g, lab, part_of)
-        bisect(g, std::move(R), max_part, int32_t(part_base + kl), lbl_r, lab, part_of, depth + 1);
+#pragma omp task default(none) firstprivate(L, max_part, part_base, lbl_l, depth) shared(g, lab, dist, part_of)
+        bisect(g, std::move(L), max_part, part_base, lbl_l, lab, dist, part_of, depth + 1);
+#pragma omp task default(none) firstprivate(R, max_part, part_base, kl, lbl_r, depth) shared(g, lab, dist, part_of)
+        bisect(g, std::move(R), max_part, int32_t(part_base + kl), lbl_r, lab, dist, part_of, depth + 1);
 #pragma omp taskwait
     } else {
-        bisect(g, std::move(L), max_part, part_base, lbl_l, lab, part_of, depth + 1);
-        bisect(g, std::move(R), max_part, int32_t(part_base + kl), lbl_r, lab, part_of, depth + 1);
+        bisect(g, std::move(L), max_part, part_base, lbl_l, lab, dist, part_of, depth + 1);
+        bisect(g, std::move(R), max_part, int32_t(part_base + kl), lbl_r, lab, dist, part_of, depth + 1);
     }
 }
 
@@ -85,8 +98,8 @@ void partition_graph(const Pattern &g, int max_part, std::vector<int32_t> &part_
     // (e.g. eliminated Dirichlet rows) are set aside into trailing parts: left
     // in place they become singleton components that never merge and fill the
     // coarse levels.  Each subset is split by a search from a pseudo-peripheral
-    // node (last node reached from its smallest member): the first
-    // ceil(k/2)*max_part nodes in search order go left.  Part ids follow the
+    // node (farthest from its smallest member, then largest id): the first
+    // ceil(k/2)*max_part nodes in (distance, id) order go left.  Part ids follow the
     // recursion from left to right, so every run of consecutive parts is one
     // subtree, compact at every scale, as the bank-local coarse levels need.
     const int32_t n = g.n;
@@ -102,11 +115,11 @@ void partition_graph(const Pattern &g, int max_part, std::vector<int32_t> &part_
     for (size_t q = 0; q < iso.size(); ++q) part_of[iso[q]] = real_parts + int32_t(q / max_part);
     if (real.empty()) return;
     // labels: subproblem ids (heap numbering) >= 0, visited markers < -1, isolated -1
-    std::vector<int32_t> lab(n, 0);
+    std::vector<int32_t> lab(n, 0), dist(n, 0);
     for (int32_t v : iso) lab[v] = -1;
 #pragma omp parallel
 #pragma omp single
-    bisect(g, std::move(real), max_part, 0, 0, lab, part_of, 0);
+    bisect(g, std::move(real), max_part, 0, 0, lab, dist, part_of, 0);
 }
 
 void partition_permutation(const std::vector<int32_t> &part_of, int32_t nparts, std::vector<int32_t> &perm,

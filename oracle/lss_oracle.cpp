@@ -363,38 +363,48 @@ extern "C" int orc_decompose(int32_t n, const int32_t *ptr, const int32_t *idx, 
 // ===================================================================== partition
 namespace {
 
-// breadth-first order of `set` inside its induced subgraph, from `start`, then
-// from the smallest unvisited member for every further component
+// members of `set` ordered by (breadth-first distance from `start` inside the
+// induced subgraph, node id); when the search runs out, it restarts from the
+// smallest unreached member, one level beyond the deepest level so far
 std::vector<int32_t> bfs_in(const int64_t *bptr, const int32_t *bidx, const std::vector<int32_t> &set, int32_t start,
-                            std::vector<char> &in, std::vector<char> &seen)
+                            std::vector<char> &in, std::vector<int32_t> &dist)
 {
-    for (int32_t v : set) in[v] = 1, seen[v] = 0;
-    std::vector<int32_t> order;
-    order.reserve(set.size());
-    auto from = [&](int32_t s0) {
-        size_t head = order.size();
-        seen[s0] = 1;
-        order.push_back(s0);
-        while (head < order.size()) {
-            const int32_t u = order[head++];
+    for (int32_t v : set) in[v] = 1, dist[v] = -1;
+    std::vector<int32_t> queue;
+    queue.reserve(set.size());
+    int32_t deepest = 0;
+    auto from = [&](int32_t s0, int32_t d0) {
+        size_t head = queue.size();
+        dist[s0] = d0;
+        deepest = std::max(deepest, d0);
+        queue.push_back(s0);
+        while (head < queue.size()) {
+            const int32_t u = queue[head++];
             for (int64_t e = bptr[u]; e < bptr[u + 1]; ++e) {
                 const int32_t w = bidx[e];
-                if (in[w] && !seen[w]) {
-                    seen[w] = 1;
-                    order.push_back(w);
+                if (in[w] && dist[w] < 0) {
+                    dist[w] = dist[u] + 1;
+                    deepest = std::max(deepest, dist[w]);
+                    queue.push_back(w);
                 }
             }
         }
     };
-    from(start);
+    from(start, 0);
     for (int32_t v : set)
-        if (!seen[v]) from(v);
+        if (dist[v] < 0) from(v, deepest + 1);
+    // stable counting sort of the ascending set by distance
+    std::vector<int64_t> cnt(size_t(deepest) + 2, 0);
+    for (int32_t v : set) cnt[dist[v] + 1]++;
+    for (int32_t d = 0; d <= deepest; ++d) cnt[d + 1] += cnt[d];
+    std::vector<int32_t> order(set.size());
+    for (int32_t v : set) order[cnt[dist[v]]++] = v;
     for (int32_t v : set) in[v] = 0;
     return order;
 }
 
 void split(const int64_t *bptr, const int32_t *bidx, std::vector<int32_t> set, int max_part, int32_t base,
-           int32_t *part_of, std::vector<char> &in, std::vector<char> &seen)
+           int32_t *part_of, std::vector<char> &in, std::vector<int32_t> &seen)
 {
     if (int64_t(set.size()) <= max_part) {
         for (int32_t v : set) part_of[v] = base;
@@ -417,9 +427,9 @@ extern "C" int orc_partition(int32_t nb, const int64_t *bptr, const int32_t *bid
 {
     // Recursive breadth-first bisection.  Nodes without off-diagonal entries go
     // to trailing parts (32 per part, ascending).  A subset of k parts is
-    // searched from the last node reached by a search from its smallest member;
-    // its first ceil(k/2)*max_part nodes in that order go left; parts are
-    // numbered left to right.
+    // ordered by (distance, id) from the last node in that order from its
+    // smallest member (a pseudo-peripheral node: farthest, then largest id);
+    // its first ceil(k/2)*max_part nodes go left; parts are numbered left to right.
     std::vector<int32_t> real, iso;
     for (int32_t v = 0; v < nb; ++v) {
         bool coupled = false;
@@ -430,8 +440,9 @@ extern "C" int orc_partition(int32_t nb, const int64_t *bptr, const int32_t *bid
     *nparts = real_parts + int32_t((int64_t(iso.size()) + max_part - 1) / max_part);
     for (size_t q = 0; q < iso.size(); ++q) part_of[iso[q]] = real_parts + int32_t(q / max_part);
     if (real.empty()) return 0;
-    std::vector<char> in(nb, 0), seen(nb, 0);
-    split(bptr, bidx, std::move(real), max_part, 0, part_of, in, seen);
+    std::vector<char> in(nb, 0);
+    std::vector<int32_t> dist(nb, -1);
+    split(bptr, bidx, std::move(real), max_part, 0, part_of, in, dist);
     return 0;
 }
 

@@ -113,12 +113,14 @@ def test_solve_matches_oracle(gpu, name, N):
     assert np.linalg.norm(x - xr) <= 1e-5 * np.linalg.norm(xr)
 
 
+@pytest.mark.parametrize("variant", [{}, {"partitioner": "host"}])
 @pytest.mark.parametrize("name,N", CASES)
-def test_partition_matches_oracle(gpu, name, N):
+def test_partition_matches_oracle(gpu, name, N, variant):
+    # default: the device bisection; host: the OpenMP implementation
     prob = problems.make(name, N=N)
     A = prob.csc()
     bs = prob.block_size
-    s = Solver.create(_cfg(bs))
+    s = Solver.create(_cfg(bs, **variant))
     s.factorize(A)
     nb = A.n // bs
     perm, off = s.partition(nb)
