@@ -321,6 +321,8 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
                                   hipStream_t s)
 {
     const int bb = bs_ * bs_;
+    const bool mas = cfg_.base_preconditioner == "MAS";
+    const bool dev_topo = mas && cfg_.partitioner == "device";
     // scalar rows -> block rows (original numbering)
     DBuf<int32_t> cnt(nb_), flag(1);
     flag.zero(s);
@@ -345,14 +347,16 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     d_rp0.upload(g.rowptr.data(), nb_ + 1, s);
     d_v0.zero(s);
     k::bsr_fill(n, bs_, ptr, idx, vals, d_rp0.get(), d_c0.get(), d_v0.get(), s);
-    g.cols.resize(nnzb_);
-    d_c0.download(g.cols.data(), nnzb_, s);
+    if (!dev_topo) {  // the host partition and coarse spaces read the pattern
+        g.cols.resize(nnzb_);
+        d_c0.download(g.cols.data(), nnzb_, s);
+    }
     sync(s);
 
     // partition (MAS banks) and permutation
     auto tp = clk::now();
-    if (cfg_.base_preconditioner == "MAS" && cfg_.partitioner == "device") {
-        DBuf<int32_t> d_off;
+    DBuf<int32_t> d_off;
+    if (dev_topo) {
         d_perm_.resize(nb_);
         const int32_t nparts = k::device_partition(nb_, d_rp0.get(), d_c0.get(), 32, d_perm_.get(), d_off, s);
         perm_.resize(nb_);
@@ -360,7 +364,7 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
         d_perm_.download(perm_.data(), nb_, s);
         d_off.download(part_off_.data(), part_off_.size(), s);
         sync(s);
-    } else if (cfg_.base_preconditioner == "MAS") {
+    } else if (mas) {
         std::vector<int32_t> part_of;
         int32_t nparts = 0;
         partition_graph(g, 32, part_of, nparts);
@@ -370,10 +374,13 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
         std::iota(perm_.begin(), perm_.end(), 0);
         part_off_.clear();
     }
-    Pattern gp = permute_pattern(g, perm_);
+    // row lengths of the permuted pattern
+    std::vector<int32_t> prow(size_t(nb_) + 1, 0);
+    for (int32_t I = 0; I < nb_; ++I) prow[size_t(perm_[I]) + 1] = hc[I];
+    for (int32_t I = 0; I < nb_; ++I) prow[size_t(I) + 1] += prow[I];
     t_partition_ = since(tp);
-    if (!(cfg_.base_preconditioner == "MAS" && cfg_.partitioner == "device")) d_perm_.upload(perm_.data(), nb_, s);
-    d_rowptr_.upload(gp.rowptr.data(), nb_ + 1, s);
+    if (!dev_topo) d_perm_.upload(perm_.data(), nb_, s);
+    d_rowptr_.upload(prow.data(), nb_ + 1, s);
     d_cols_.resize(std::max<int64_t>(nnzb_, 1));
     d_bvals_.resize(std::max<int64_t>(nnzb_ * bb, 1));
     k::bsr_permute(nb_, bs_, d_rp0.get(), d_c0.get(), d_v0.get(), d_perm_.get(), d_rowptr_.get(), d_cols_.get(),
@@ -385,7 +392,7 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     std::vector<int64_t> sp(nsl + 1, 0);
     for (int32_t sl = 0; sl < nsl; ++sl) {
         int32_t w = 0;
-        for (int32_t r = sl * H; r < std::min(nb_, (sl + 1) * H); ++r) w = std::max(w, gp.rowptr[r + 1] - gp.rowptr[r]);
+        for (int32_t r = sl * H; r < std::min(nb_, (sl + 1) * H); ++r) w = std::max(w, prow[r + 1] - prow[r]);
         sp[sl + 1] = sp[sl] + int64_t(w) * H;
     }
     sell_slots_ = sp[nsl];
@@ -409,39 +416,106 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     sell_.vals = d_svals_.get();
     sell_.vals32 = cfg_.fp32_inner() ? d_svals32_.get() : nullptr;
     sync(s);
-    // the permuted pattern is kept for the MAS hierarchy
     level_nodes_.clear();
     mas_.clear();
-    if (cfg_.base_preconditioner == "MAS") {
-        std::vector<MasLevel> levels;
-        build_mas_levels(gp, part_off_, kMaxCoarseLevels, levels);
-        mas_.resize(levels.size());
-        for (size_t l = 0; l < levels.size(); ++l) {
-            const MasLevel &h = levels[l];
-            DevLevel &d = mas_[l];
-            d.nodes = h.nodes;
-            d.nbanks = int32_t(h.bank_off.size()) - 1;
-            d.bank_off.upload(h.bank_off.data(), h.bank_off.size(), s);
-            if (l >= 1) {
-                d.rowptr.upload(h.op.rowptr.data(), h.op.rowptr.size(), s);
-                d.cols.upload(h.op.cols.data(), std::max<size_t>(h.op.cols.size(), 1), s);
-                const int64_t nslots = int64_t(h.op.cols.size());
-                d.vals.resize(std::max<int64_t>(nslots * bb, 1));
-                DBuf<int64_t> sptr;
-                DBuf<int32_t> sblk;
-                sptr.upload(h.src_ptr.data(), h.src_ptr.size(), s);
-                sblk.upload(h.src_blk.data(), std::max<size_t>(h.src_blk.size(), 1), s);
-                const double *fine_vals = (l == 1) ? d_bvals_.get() : mas_[l - 1].vals.get();
-                k::mas_galerkin(nslots, bs_, sptr.get(), sblk.get(), fine_vals, d.vals.get(), s);
-                d.child_ptr.upload(h.child_ptr.data(), h.child_ptr.size(), s);
-                d.child.upload(h.child.data(), std::max<size_t>(h.child.size(), 1), s);
-                d.r.resize(size_t(h.nodes) * bs_);
-                d.w.resize(size_t(h.nodes) * bs_);
-                sync(s);  // sptr / sblk are released at scope end
-            }
-            if (!h.parent.empty()) d.parent.upload(h.parent.data(), h.parent.size(), s);
-            level_nodes_.push_back(h.nodes);
+    if (!mas) return;
+    if (dev_topo && build_levels_device(std::move(d_off), int32_t(part_off_.size()) - 1, s)) return;
+    // host coarse spaces (host partition, or a bank too dense for the device sort)
+    Pattern gp;
+    if (dev_topo) {
+        gp.n = nb_;
+        gp.rowptr = prow;
+        gp.cols.resize(nnzb_);
+        d_cols_.download(gp.cols.data(), nnzb_, s);
+        sync(s);
+    } else {
+        gp = permute_pattern(g, perm_);
+    }
+    build_levels_host(gp, s);
+}
+
+bool HybridSolver::build_levels_device(DBuf<int32_t> &&part_off, int32_t nparts, hipStream_t s)
+{
+    const int bb = bs_ * bs_;
+    mas_.clear();
+    level_nodes_.clear();
+    mas_.emplace_back();
+    mas_[0].nodes = nb_;
+    mas_[0].nbanks = nparts;
+    mas_[0].bank_off = std::move(part_off);
+    level_nodes_.push_back(nb_);
+    for (int lv = 0; lv < kMaxCoarseLevels; ++lv) {
+        const size_t l = mas_.size() - 1;
+        const int32_t *rp = l == 0 ? d_rowptr_.get() : mas_[l].rowptr.get();
+        const int32_t *cl = l == 0 ? d_cols_.get() : mas_[l].cols.get();
+        const int64_t fine_nnz = l == 0 ? nnzb_ : int64_t(mas_[l].cols.size());
+        DBuf<int32_t> parent, child, base, count;
+        DBuf<int64_t> child_ptr;
+        const int32_t coarse_n = k::device_components(mas_[l].nodes, rp, cl, mas_[l].nbanks, mas_[l].bank_off.get(),
+                                                      parent, child_ptr, child, base, count, s);
+        // stop when nothing merges (MASPreconditioner.cu:449-454); level 0 always coarsens
+        if (lv > 0 && coarse_n == mas_[l].nodes) break;
+        DevLevel nxt;
+        nxt.nodes = coarse_n;
+        DBuf<int64_t> sptr;
+        DBuf<int32_t> sblk;
+        int64_t nslots = 0;
+        if (!k::device_galerkin_pattern(fine_nnz, rp, cl, mas_[l].nbanks, mas_[l].bank_off.get(), base.get(),
+                                        count.get(), parent.get(), coarse_n, nxt.rowptr, nxt.cols, sptr, sblk, nslots,
+                                        s)) {
+            sync(s);
+            mas_.clear();
+            level_nodes_.clear();
+            return false;
         }
+        nxt.vals.resize(std::max<int64_t>(nslots * bb, 1));
+        const double *fine_vals = l == 0 ? d_bvals_.get() : mas_[l].vals.get();
+        k::mas_galerkin(nslots, bs_, sptr.get(), sblk.get(), fine_vals, nxt.vals.get(), s);
+        nxt.child_ptr = std::move(child_ptr);
+        nxt.child = std::move(child);
+        nxt.r.resize(size_t(coarse_n) * bs_);
+        nxt.w.resize(size_t(coarse_n) * bs_);
+        nxt.nbanks = k::uniform_banks(coarse_n, nxt.bank_off, s);
+        mas_[l].parent = std::move(parent);
+        sync(s);  // the source lists are released at scope end
+        mas_.push_back(std::move(nxt));
+        level_nodes_.push_back(coarse_n);
+    }
+    sync(s);
+    return true;
+}
+
+void HybridSolver::build_levels_host(const Pattern &gp, hipStream_t s)
+{
+    const int bb = bs_ * bs_;
+    std::vector<MasLevel> levels;
+    build_mas_levels(gp, part_off_, kMaxCoarseLevels, levels);
+    mas_.resize(levels.size());
+    for (size_t l = 0; l < levels.size(); ++l) {
+        const MasLevel &h = levels[l];
+        DevLevel &d = mas_[l];
+        d.nodes = h.nodes;
+        d.nbanks = int32_t(h.bank_off.size()) - 1;
+        d.bank_off.upload(h.bank_off.data(), h.bank_off.size(), s);
+        if (l >= 1) {
+            d.rowptr.upload(h.op.rowptr.data(), h.op.rowptr.size(), s);
+            d.cols.upload(h.op.cols.data(), std::max<size_t>(h.op.cols.size(), 1), s);
+            const int64_t nslots = int64_t(h.op.cols.size());
+            d.vals.resize(std::max<int64_t>(nslots * bb, 1));
+            DBuf<int64_t> sptr;
+            DBuf<int32_t> sblk;
+            sptr.upload(h.src_ptr.data(), h.src_ptr.size(), s);
+            sblk.upload(h.src_blk.data(), std::max<size_t>(h.src_blk.size(), 1), s);
+            const double *fine_vals = (l == 1) ? d_bvals_.get() : mas_[l - 1].vals.get();
+            k::mas_galerkin(nslots, bs_, sptr.get(), sblk.get(), fine_vals, d.vals.get(), s);
+            d.child_ptr.upload(h.child_ptr.data(), h.child_ptr.size(), s);
+            d.child.upload(h.child.data(), std::max<size_t>(h.child.size(), 1), s);
+            d.r.resize(size_t(h.nodes) * bs_);
+            d.w.resize(size_t(h.nodes) * bs_);
+            sync(s);  // sptr / sblk are released at scope end
+        }
+        if (!h.parent.empty()) d.parent.upload(h.parent.data(), h.parent.size(), s);
+        level_nodes_.push_back(h.nodes);
     }
     sync(s);
 }

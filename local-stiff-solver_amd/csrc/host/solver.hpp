@@ -13,6 +13,7 @@
 #include "../kernels/mas_kernels.hpp"
 #include "config.hpp"
 #include "stiff_select.hpp"
+#include "topology.hpp"
 
 namespace lss {
 
@@ -75,6 +76,9 @@ private:
                       bool with_vals, hipStream_t s);
     void build_operator(int32_t n, int64_t nnz, const int32_t *ptr, const int32_t *idx, const double *vals,
                         hipStream_t s);
+    // MAS coarse spaces: device kernels (false if a bank is too dense for them) or host
+    bool build_levels_device(DBuf<int32_t> &&part_off, int32_t nparts, hipStream_t s);
+    void build_levels_host(const Pattern &gp, hipStream_t s);
     void build_mas(hipStream_t s);
     void build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s);
     void estimate_omega(hipStream_t s);

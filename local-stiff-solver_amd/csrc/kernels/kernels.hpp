@@ -90,6 +90,25 @@ void gmm_flags(int64_t n, const double *g0, const double *g1, uint8_t *flags, hi
 int32_t device_partition(int32_t nb, const int32_t *rowptr, const int32_t *cols, int max_part, int32_t *perm,
                          DBuf<int32_t> &part_off, hipStream_t s);
 
+// Exclusive scan of n int32 values (out may not alias a); tsum holds
+// ceil(n / 1024) values; the total goes to *total when it is not null.
+void excl_scan_i32(int32_t n, const int32_t *a, int32_t *out, int32_t *tsum, int32_t *total, hipStream_t s);
+
+// ---------------------------------------------------------------- MAS coarse spaces
+// Components of each bank (at most 32 nodes): parent[fine_n], child_ptr[coarse_n + 1],
+// child[fine_n] (children ascending), base/count per bank; returns coarse_n.  Synchronizes s.
+int32_t device_components(int32_t fine_n, const int32_t *rowptr, const int32_t *cols, int32_t nbanks,
+                          const int32_t *bank_off, DBuf<int32_t> &parent, DBuf<int64_t> &child_ptr,
+                          DBuf<int32_t> &child, DBuf<int32_t> &base, DBuf<int32_t> &count, hipStream_t s);
+// Pattern of R A R^T and the fine entries each coarse entry sums (ascending).
+// Returns false (nothing built) when a bank holds more entries than one workgroup sorts.
+bool device_galerkin_pattern(int64_t fine_nnz, const int32_t *rowptr, const int32_t *cols, int32_t nbanks,
+                             const int32_t *bank_off, const int32_t *base, const int32_t *count,
+                             const int32_t *parent, int32_t coarse_n, DBuf<int32_t> &crowptr, DBuf<int32_t> &ccols,
+                             DBuf<int64_t> &src_ptr, DBuf<int32_t> &src_blk, int64_t &nslots, hipStream_t s);
+// bank_off = {0, 32, 64, ..., n}; returns the bank count.
+int32_t uniform_banks(int32_t n, DBuf<int32_t> &bank_off, hipStream_t s);
+
 // ---------------------------------------------------------------- adjoint
 // grad[e] = -lam[j] * x[rowidx[e]] for every entry e of CSC column j (operator entry (j, rowidx[e])).
 void pattern_grad(int32_t n, const int32_t *colptr, const int32_t *rowidx, const double *lam, const double *x,

@@ -118,15 +118,6 @@ __global__ __launch_bounds__(kPT) void k_tile_apply(int32_t n, const int32_t *__
     }
 }
 
-void excl_scan(int32_t n, const int32_t *a, int32_t *out, int32_t *tsum, int32_t *total, hipStream_t s)
-{
-    const int32_t ntiles = div_up(n, kTile);
-    if (ntiles == 0) return;
-    k_tile_sum<<<ntiles, kPT, 0, s>>>(n, a, tsum);
-    k_tile_scan<<<1, kPT, 0, s>>>(ntiles, tsum, total);
-    k_tile_apply<<<ntiles, kPT, 0, s>>>(n, a, tsum, out);
-}
-
 // ---------------------------------------------------------------- set-up
 __global__ __launch_bounds__(kPT) void k_coupled(int32_t nb, const int32_t *__restrict__ rowptr,
                                                   const int32_t *__restrict__ cols, int32_t *__restrict__ flag)
@@ -430,6 +421,15 @@ __global__ void k_tail_part_off(int32_t real_parts, int32_t iso_parts, int32_t r
 
 }  // namespace
 
+void excl_scan_i32(int32_t n, const int32_t *a, int32_t *out, int32_t *tsum, int32_t *total, hipStream_t s)
+{
+    const int32_t ntiles = div_up(n, kTile);
+    if (ntiles == 0) return;
+    k_tile_sum<<<ntiles, kPT, 0, s>>>(n, a, tsum);
+    k_tile_scan<<<1, kPT, 0, s>>>(ntiles, tsum, total);
+    k_tile_apply<<<ntiles, kPT, 0, s>>>(n, a, tsum, out);
+}
+
 int32_t device_partition(int32_t nb, const int32_t *rowptr, const int32_t *cols, int max_part, int32_t *perm,
                          DBuf<int32_t> &part_off, hipStream_t s)
 {
@@ -437,7 +437,7 @@ int32_t device_partition(int32_t nb, const int32_t *rowptr, const int32_t *cols,
     DBuf<int32_t> tsum(std::max(div_up(nb, kTile), 1)), dflags(8);
     dflags.zero(s);
     k_coupled<<<div_up(nb, kPT), kPT, 0, s>>>(nb, rowptr, cols, flag.get());
-    excl_scan(nb, flag.get(), rank.get(), tsum.get(), dflags.get() + 2, s);
+    excl_scan_i32(nb, flag.get(), rank.get(), tsum.get(), dflags.get() + 2, s);
     k_place<<<div_up(nb, 256), 256, 0, s>>>(nb, flag.get(), rank.get(), dflags.get() + 2, mem.get(), sub.get());
     LSS_LAUNCH_CHECK();
     // the splits rewrite only the coupled range [0, real_n) of the second
@@ -494,10 +494,10 @@ int32_t device_partition(int32_t nb, const int32_t *rowptr, const int32_t *cols,
         // split every searched subset (flag, rank and aux are free here)
         const int gq = div_up(real_n, 256);
         k_cut_flags<<<gq, 256, 0, s>>>(real_n, S.get(), mem.get(), sub.get(), dist.get(), max_part, flag.get());
-        excl_scan(real_n, flag.get(), rank.get(), tsum.get(), nullptr, s);
+        excl_scan_i32(real_n, flag.get(), rank.get(), tsum.get(), nullptr, s);
         k_left_flags<<<gq, 256, 0, s>>>(real_n, S.get(), mem.get(), sub.get(), dist.get(), rank.get(), max_part,
                                         aux.get());
-        excl_scan(real_n, aux.get(), rank.get(), tsum.get(), nullptr, s);
+        excl_scan_i32(real_n, aux.get(), rank.get(), tsum.get(), nullptr, s);
         k_scatter<<<gq, 256, 0, s>>>(real_n, S.get(), mem.get(), mem2.get(), sub.get(), aux.get(), rank.get(),
                                      max_part);
         N.resize(size_t(nsub) * 2);
