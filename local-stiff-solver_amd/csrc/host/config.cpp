@@ -92,6 +92,7 @@ Config parse_config(const char *text, const Config &base)
         else if (k == "profile_kernels") c.profile_kernels = as_bool(v, k);
         else if (k == "true_residual_check") c.true_residual_check = as_bool(v, k);
         else if (k == "partitioner") c.partitioner = as_str(v, k);
+        else if (k == "max_coarse_levels") c.max_coarse_levels = int(as_int(v, k));
         else bad("unknown key GPUHybrid/" + k);
     }
     if (c.block_size < 1 || c.block_size > 3) bad("block_size must be 1, 2 or 3");
@@ -101,6 +102,7 @@ Config parse_config(const char *text, const Config &base)
     if (c.base_preconditioner != "MAS" && c.base_preconditioner != "Jacobi") bad("base_preconditioner must be MAS|Jacobi");
     if (c.check_every < 1) bad("check_every must be >= 1");
     if (c.partitioner != "device" && c.partitioner != "host") bad("partitioner must be device|host");
+    if (c.max_coarse_levels < 0 || c.max_coarse_levels > 4) bad("max_coarse_levels must be in 0..4");
     if (!(c.damping > 0.0) || c.damping >= 2.0) bad("damping must be in (0, 2)");
     if (c.lanczos_steps < 1) bad("lanczos_steps must be >= 1");
     return c;

@@ -37,11 +37,13 @@ struct Config {
     bool profile_kernels = false;       // time SpMV / MAS launches with HIP events
     bool true_residual_check = false;   // on convergence verify ||b - A x|| and restart PCG from it if needed
     std::string partitioner = "device";  // MAS partition: "device" (HIP kernels) | "host" (OpenMP)
+    int max_coarse_levels = 4;           // MAS coarse levels (MAS_MAX_COARSE_LEVEL, MASPreconditioner.hpp:10-12)
 
-    // FP32 matrix values in the preconditioner-internal SpMV only on request: next to
-    // an FP64 stiff-block solve they make the sweep inconsistent (an FP32 rounding of
-    // a 1e8 contact entry exceeds the soft entries)
-    bool fp32_inner() const { return inner_spmv == "fp32"; }
+    // FP32 matrix values in the preconditioner-internal SpMV (by default with reduced-
+    // precision banks).  The stiff rows are then recomputed in FP64: an FP32 rounding
+    // of a 1e8 contact entry exceeds the soft entries and would make the sweep
+    // inconsistent with the FP64 stiff solve.
+    bool fp32_inner() const { return inner_spmv == "fp32" || (inner_spmv == "auto" && precision != "fp64"); }
 };
 
 // Parse a configuration document; throws lss::Error(LSS_ERR_CONFIG) on bad input.

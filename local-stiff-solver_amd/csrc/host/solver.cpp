@@ -18,7 +18,6 @@ using clk = std::chrono::steady_clock;
 double since(clk::time_point t0) { return std::chrono::duration<double>(clk::now() - t0).count(); }
 
 constexpr double kPi = 3.14159265358979323846;
-constexpr int kMaxCoarseLevels = 4;  // MAS_MAX_COARSE_LEVEL, MASPreconditioner.hpp:10-12
 constexpr size_t kPartialsCap = size_t(1) << 20;
 
 template <class T>
@@ -444,7 +443,7 @@ bool HybridSolver::build_levels_device(DBuf<int32_t> &&part_off, int32_t nparts,
     mas_[0].nbanks = nparts;
     mas_[0].bank_off = std::move(part_off);
     level_nodes_.push_back(nb_);
-    for (int lv = 0; lv < kMaxCoarseLevels; ++lv) {
+    for (int lv = 0; lv < cfg_.max_coarse_levels; ++lv) {
         const size_t l = mas_.size() - 1;
         const int32_t *rp = l == 0 ? d_rowptr_.get() : mas_[l].rowptr.get();
         const int32_t *cl = l == 0 ? d_cols_.get() : mas_[l].cols.get();
@@ -489,7 +488,7 @@ void HybridSolver::build_levels_host(const Pattern &gp, hipStream_t s)
 {
     const int bb = bs_ * bs_;
     std::vector<MasLevel> levels;
-    build_mas_levels(gp, part_off_, kMaxCoarseLevels, levels);
+    build_mas_levels(gp, part_off_, cfg_.max_coarse_levels, levels);
     mas_.resize(levels.size());
     for (size_t l = 0; l < levels.size(); ++l) {
         const MasLevel &h = levels[l];
@@ -745,6 +744,9 @@ void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, 
     if (prof) prof_begin(1, s);
     k::spmv_residual(sell_, z, r, u_.get(), stop, s, cfg_.fp32_inner());
     if (prof) prof_end(s);
+    if (cfg_.fp32_inner())
+        k::stiff_rows_exact(ns_, bs_, d_pdofs_.get(), d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z, u_.get(),
+                            stop, s);
     // z = z2 + B u
     base_apply(u_.get(), omega_, z, z, st, stop, s, first);
 }

@@ -50,4 +50,9 @@ void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, c
                    const double *bvals, const double *r, double *z, double *work, double *work2, const int *stop,
                    hipStream_t s);
 
+// u[p] = r[p] - (A z)[p] in FP64 for the stiff rows p: after a preconditioner-internal
+// SpMV with FP32 values, the rows that hold the stiff entries are recomputed exactly.
+void stiff_rows_exact(int64_t ns, int bs, const int32_t *pdofs, const int32_t *rowptr, const int32_t *cols,
+                      const double *bvals, const double *r, const double *z, double *u, const int *stop, hipStream_t s);
+
 }  // namespace lss::k

@@ -138,6 +138,27 @@ __global__ void k_stiff_residual(int64_t ns, int bs, const int32_t *__restrict__
     work[k] = r[p] - acc;
 }
 
+// u[p] = r[p] - (A z)[p] in FP64 for the stiff rows p (after a reduced-precision SpMV)
+__global__ void k_stiff_rows_exact(int64_t ns, int bs, const int32_t *__restrict__ pdofs,
+                                   const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
+                                   const double *__restrict__ bvals, const double *__restrict__ r,
+                                   const double *__restrict__ z, double *__restrict__ u, const int *__restrict__ stop)
+{
+    if (stop && *stop) return;
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    const int32_t p = pdofs[k];
+    const int32_t I = p / bs;
+    const int a = p - I * bs;
+    const int bb = bs * bs;
+    double acc = 0.0;
+    for (int32_t e = rowptr[I]; e < rowptr[I + 1]; ++e) {
+        const int32_t J = cols[e];
+        for (int c = 0; c < bs; ++c) acc += bvals[int64_t(e) * bb + a * bs + c] * z[int64_t(J) * bs + c];
+    }
+    u[p] = r[p] - acc;
+}
+
 // G = L^{-1}: one lane per column j, G(i, j) = -(sum_{k=j}^{i-1} L(i,k) G(k,j)) / L(i,i).
 // Stored as M with M(i, j) = G(i, j) for j <= i and M(j, i) = G(i, j) above the
 // diagonal (G and G^T share one square array).
@@ -235,6 +256,14 @@ void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, c
     k_stiff_rows<false><<<div_up(ns, 4), 256, 0, s>>>(ns, row_comp, comp_off, mat_off, pdofs, G, work, work2, z, stop);
     LSS_LAUNCH_CHECK();
     k_stiff_rows<true><<<div_up(ns, 4), 256, 0, s>>>(ns, row_comp, comp_off, mat_off, pdofs, G, work2, nullptr, z, stop);
+    LSS_LAUNCH_CHECK();
+}
+
+void stiff_rows_exact(int64_t ns, int bs, const int32_t *pdofs, const int32_t *rowptr, const int32_t *cols,
+                      const double *bvals, const double *r, const double *z, double *u, const int *stop, hipStream_t s)
+{
+    if (ns == 0) return;
+    k_stiff_rows_exact<<<div_up(ns, 256), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r, z, u, stop);
     LSS_LAUNCH_CHECK();
 }
 
