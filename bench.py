@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-iter", type=int, default=20000,
                     help="PCG iteration cap (profiling runs cut it; such a line is marked invalid)")
+    ap.add_argument("--mixed-precision", default="fp32", choices=["fp32", "bf16", "fp16"],
+                    help="bank storage of the mixed path")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=JSON",
                     help="extra GPUHybrid configuration entry for both paths (experiments)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -117,7 +119,7 @@ def main():
     for kv in args.set:
         k, v = kv.split("=", 1)
         base[k] = json.loads(v)
-    paths = {"fp64": dict(base, precision="fp64"), "mixed": dict(base, precision="fp32")}
+    paths = {"fp64": dict(base, precision="fp64"), "mixed": dict(base, precision=args.mixed_precision)}
     res = {}
     for name, g in paths.items():
         s = Solver.create({"solver": "GPUHybrid", "GPUHybrid": g})

@@ -97,13 +97,14 @@ Config parse_config(const char *text, const Config &base)
     }
     if (c.block_size < 1 || c.block_size > 3) bad("block_size must be 1, 2 or 3");
     if (c.max_iter < 0) bad("max_iter must be >= 0");
-    if (c.precision != "fp64" && c.precision != "fp32" && c.precision != "bf16") bad("precision must be fp64|fp32|bf16");
+    if (c.precision != "fp64" && c.precision != "fp32" && c.precision != "bf16" && c.precision != "fp16")
+        bad("precision must be fp64|fp32|bf16|fp16");
     if (c.inner_spmv != "auto" && c.inner_spmv != "fp64" && c.inner_spmv != "fp32") bad("inner_spmv must be auto|fp64|fp32");
     if (c.base_preconditioner != "MAS" && c.base_preconditioner != "Jacobi") bad("base_preconditioner must be MAS|Jacobi");
     if (c.check_every < 1) bad("check_every must be >= 1");
     if (c.partitioner != "device" && c.partitioner != "host") bad("partitioner must be device|host");
     if (c.max_coarse_levels < 0 || c.max_coarse_levels > 4) bad("max_coarse_levels must be in 0..4");
-    if (!(c.damping > 0.0) || c.damping >= 2.0) bad("damping must be in (0, 2)");
+    if (!(c.damping > 0.0) || c.damping >= 4.0) bad("damping must be in (0, 4)");
     if (c.lanczos_steps < 1) bad("lanczos_steps must be >= 1");
     return c;
 }

@@ -31,7 +31,7 @@ struct Config {
     std::string inner_spmv = "auto";    // preconditioner-internal SpMV values: "auto" | "fp64" | "fp32"
     std::string base_preconditioner = "MAS";  // "MAS" | "Jacobi"
     int check_every = 8;                // PCG iterations between host polls of the device stop flag
-    double damping = 1.5;               // omega = damping / lambda_max(B A) inside the B-P-B sweep
+    double damping = 1.8;               // omega = damping / lambda_max(B A) inside the B-P-B sweep
     int lanczos_steps = 20;          // CG-Lanczos steps of the lambda_max(B A) estimate
     int64_t max_dense_component = 16384;  // largest stiff component for the dense Cholesky
     bool profile_kernels = false;       // time SpMV / MAS launches with HIP events

@@ -12,6 +12,10 @@
 #include "topology.hpp"
 
 namespace lss {
+
+// damping of the base preconditioner after a breakdown (omega * lambda_max < 2 holds
+// while the Lanczos estimate is within 25 % of lambda_max)
+constexpr double kSafeDamping = 1.5;
 namespace {
 
 using clk = std::chrono::steady_clock;
@@ -74,6 +78,7 @@ k::Precision HybridSolver::prec() const
 {
     if (cfg_.precision == "fp32") return k::Precision::FP32;
     if (cfg_.precision == "bf16") return k::Precision::BF16;
+    if (cfg_.precision == "fp16") return k::Precision::FP16;
     return k::Precision::FP64;
 }
 
@@ -882,7 +887,27 @@ void HybridSolver::solve_device(const double *b, double *x, hipStream_t s)
     }
     k::permute_in(nb_, bs_, n_, d_perm_.get(), b, bp_.get(), s);
     k::permute_in(nb_, bs_, n_, d_perm_.get(), x, xp_.get(), s);
+    // omega = damping / theta keeps omega * lambda_max(B A) < 2 while theta is within
+    // damping / 2 of lambda_max; should the sweep still lose definiteness (a PCG
+    // breakdown), solve again from the same start with the safe damping
+    const bool damped = cfg_.do_hybrid_precond && ncomp_ > 0 && cfg_.damping > kSafeDamping;
+    if (damped) {
+        x0p_.resize(size_t(np));
+        k::copy(np, xp_.get(), x0p_.get(), s);
+    }
+    damping_retry_ = false;
     run_pcg(s);
+    if (damped && stop_code_ >= dev::kZeroSdotp) {
+        damping_retry_ = true;
+        const double omega = omega_;
+        omega_ = kSafeDamping / lambda_max_;
+        k::copy(np, x0p_.get(), xp_.get(), s);
+        run_pcg(s);
+        omega_used_ = omega_;
+        omega_ = omega;
+    } else {
+        omega_used_ = omega_;
+    }
     k::permute_out(nb_, bs_, n_, d_perm_.get(), xp_.get(), x, s);
     sync(s);
     t_solve_ = since(t0);
@@ -959,7 +984,8 @@ std::string HybridSolver::info_json() const
     std::ostringstream o;
     o.precision(17);
     o << "{\"num_iterations\":" << num_iterations_ << ",\"final_res_norm\":" << final_res_norm_
-      << ",\"restarts\":" << restarts_ << ",\"solver\":\""
+      << ",\"restarts\":" << restarts_ << ",\"damping_retry\":" << (damping_retry_ ? "true" : "false")
+      << ",\"omega_used\":" << omega_used_ << ",\"solver\":\""
       << name() << "\",\"stop_reason\":\"" << stops[(stop_code_ >= 0 && stop_code_ < 6) ? stop_code_ : 0]
       << "\",\"n\":" << n_ << ",\"nnz\":" << nnz_ << ",\"block_size\":" << bs_ << ",\"nnz_blocks\":" << nnzb_
       << ",\"sell_slots\":" << sell_slots_ << ",\"precision\":\"" << cfg_.precision << "\",\"inner_spmv_fp32\":"

@@ -136,6 +136,9 @@ private:
     DBuf<int32_t> d_pdofs_, d_row_comp_;
     DBuf<double> d_L_, d_swork_;
     double omega_ = 1.0;
+    double omega_used_ = 1.0;  // omega of the last solve (after a damping retry)
+    bool damping_retry_ = false;
+    DBuf<double> x0p_;         // start vector kept for a damping retry
     double lambda_max_ = 0.0;
 
     // PCG

@@ -47,6 +47,12 @@ __device__ __forceinline__ __hip_bfloat16 to_store<__hip_bfloat16>(double v)
     return __float2bfloat16(static_cast<float>(v));
 }
 
+template <>
+__device__ __forceinline__ _Float16 to_store<_Float16>(double v)
+{
+    return static_cast<_Float16>(static_cast<float>(v));
+}
+
 template <class T>
 __device__ __forceinline__ double to_lds_any(T v) { return double(v); }
 template <>
@@ -58,6 +64,8 @@ template <>
 __device__ __forceinline__ float to_lds<float>(float v) { return v; }
 template <>
 __device__ __forceinline__ float to_lds<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
+template <>
+__device__ __forceinline__ float to_lds<_Float16>(_Float16 v) { return static_cast<float>(v); }
 
 constexpr int kInvThreads = 256;
 
@@ -250,6 +258,10 @@ template <>
 struct LdsType<__hip_bfloat16> {
     using type = float;
 };
+template <>
+struct LdsType<_Float16> {
+    using type = float;
+};
 
 // Stage one bank's packed factor into LDS: every thread issues all of its
 // 16-byte global loads (non-temporal: each factor is read once per apply) before
@@ -389,6 +401,8 @@ void mas_set_lds_limits()
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
     LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, __hip_bfloat16>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
+    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, _Float16>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
     done = true;
 }
 
@@ -406,6 +420,10 @@ void mas_bank_invert(Precision prec, int bs, int32_t nbanks, const int32_t *bank
         case Precision::BF16:
             launch_bank_invert<__hip_bfloat16>(bs, nbanks, bank_off, rowptr, cols, vals,
                                                static_cast<__hip_bfloat16 *>(out), status, s);
+            break;
+        case Precision::FP16:
+            launch_bank_invert<_Float16>(bs, nbanks, bank_off, rowptr, cols, vals, static_cast<_Float16 *>(out),
+                                         status, s);
             break;
     }
 }
@@ -447,6 +465,7 @@ void mas_coarse_apply(Precision prec, int bs, int32_t nodes, const void *inv, co
         case Precision::FP64: LSS_CA_BS(double); break;
         case Precision::FP32: LSS_CA_BS(float); break;
         case Precision::BF16: LSS_CA_BS(__hip_bfloat16); break;
+        case Precision::FP16: LSS_CA_BS(_Float16); break;
     }
 #undef LSS_CA_BS
 #undef LSS_CA
@@ -474,6 +493,7 @@ void mas_level0_apply(Precision prec, int bs, int32_t nbanks, const int32_t *par
         case Precision::FP64: LSS_L0_BS(double); break;
         case Precision::FP32: LSS_L0_BS(float); break;
         case Precision::BF16: LSS_L0_BS(__hip_bfloat16); break;
+        case Precision::FP16: LSS_L0_BS(_Float16); break;
     }
 #undef LSS_L0_BS
 #undef LSS_L0

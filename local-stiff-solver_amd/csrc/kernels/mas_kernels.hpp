@@ -11,7 +11,7 @@
 namespace lss::k {
 
 // Storage precision of the MAS bank inverses.
-enum class Precision : int { FP64 = 0, FP32 = 1, BF16 = 2 };
+enum class Precision : int { FP64 = 0, FP32 = 1, BF16 = 2, FP16 = 3 };
 
 inline size_t precision_bytes(Precision p) { return p == Precision::FP64 ? 8 : (p == Precision::FP32 ? 4 : 2); }
 

@@ -759,6 +759,9 @@ extern "C" void orc_mas_apply(const orc_mas *m, const double *r, double *z)
 }
 
 // ===================================================================== hybrid
+// damping after a breakdown of the damped sweep
+constexpr double kSafeDamping = 1.5;
+
 struct orc_hybrid {
     orc_config cfg{};
     Csr A;
@@ -773,6 +776,8 @@ struct orc_hybrid {
     std::vector<int32_t> perm, part_off;
     std::vector<int32_t> given_perm, given_part_off;  // optional partition supplied by the caller
     double omega = 1.0;  // damping of B inside the multiplicative sweep
+    double theta = 0.0;  // Lanczos estimate of lambda_max(B A)
+    bool damping_retry = false;
     int iters = 0;
     double final_res = 0.0, t_setup = 0.0, t_solve = 0.0;
     ~orc_hybrid() { orc_mas_destroy(mas); }
@@ -1072,6 +1077,7 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
     h->omega = 1.0;
     if (c.do_hybrid_precond && !h->comps.empty() && c.composition == 0) {
         const double theta = lanczos_lambda_max(*h, n);
+        h->theta = theta;
         h->omega = (c.damping > 0 ? c.damping : 1.0) / theta;
         phase("lanczos", tp);
     }
@@ -1098,34 +1104,49 @@ extern "C" int orc_hybrid_solve(orc_hybrid *h, const double *b, double *xio)
         h->t_solve = since(t0);
         return 0;
     }
-    spmv(A, x.data(), buffer.data());
-    for (int32_t i = 0; i < n; ++i) r[i] = b[i] - buffer[i];
-    h->precond(r.data(), z.data(), buffer, z2);
-    p = z;
-    double gamma = dot(n, r.data(), z.data()), old_gamma = gamma;
-    int iters = 0;
-    for (int k = 0; k < h->cfg.max_iter; ++k) {
-        iters = k + 1;
-        spmv(A, p.data(), buffer.data());
-        const double sdotp = dot(n, p.data(), buffer.data());
-        if (sdotp == 0.0) break;
-        const double alpha = gamma / sdotp;
-        if (alpha <= 0.0) break;
-        if (alpha < std::numeric_limits<double>::min()) break;
-#pragma omp parallel for schedule(static)
-        for (int32_t i = 0; i < n; ++i) {
-            x[i] += alpha * p[i];
-            r[i] += -alpha * buffer[i];
-        }
-        const double i_prod = dot(n, r.data(), r.data());
-        if (rel_eps > 0 && (i_prod / bi_prod) < rel_eps) break;
-        if (abs_eps > 0 && i_prod < abs_eps) break;
+    // returns true on a breakdown (GPUHybrid.cu:1463-1480)
+    auto pcg = [&](int &iters) {
+        spmv(A, x.data(), buffer.data());
+        for (int32_t i = 0; i < n; ++i) r[i] = b[i] - buffer[i];
         h->precond(r.data(), z.data(), buffer, z2);
-        gamma = dot(n, r.data(), z.data());
-        const double beta = gamma / old_gamma;
-        old_gamma = gamma;
+        p = z;
+        double gamma = dot(n, r.data(), z.data()), old_gamma = gamma;
+        iters = 0;
+        for (int k = 0; k < h->cfg.max_iter; ++k) {
+            iters = k + 1;
+            spmv(A, p.data(), buffer.data());
+            const double sdotp = dot(n, p.data(), buffer.data());
+            if (sdotp == 0.0) return true;
+            const double alpha = gamma / sdotp;
+            if (alpha <= 0.0) return true;
+            if (alpha < std::numeric_limits<double>::min()) return true;
 #pragma omp parallel for schedule(static)
-        for (int32_t i = 0; i < n; ++i) p[i] = beta * p[i] + z[i];
+            for (int32_t i = 0; i < n; ++i) {
+                x[i] += alpha * p[i];
+                r[i] += -alpha * buffer[i];
+            }
+            const double i_prod = dot(n, r.data(), r.data());
+            if (rel_eps > 0 && (i_prod / bi_prod) < rel_eps) break;
+            if (abs_eps > 0 && i_prod < abs_eps) break;
+            h->precond(r.data(), z.data(), buffer, z2);
+            gamma = dot(n, r.data(), z.data());
+            const double beta = gamma / old_gamma;
+            old_gamma = gamma;
+#pragma omp parallel for schedule(static)
+            for (int32_t i = 0; i < n; ++i) p[i] = beta * p[i] + z[i];
+        }
+        return false;
+    };
+    int iters = 0;
+    const double omega = h->omega;
+    h->damping_retry = false;
+    if (pcg(iters) && h->theta > 0 && h->cfg.damping > kSafeDamping) {
+        // the damped sweep lost definiteness: again from the same start, safe damping
+        h->damping_retry = true;
+        h->omega = kSafeDamping / h->theta;
+        std::copy(xio, xio + n, x.begin());
+        pcg(iters);
+        h->omega = omega;
     }
     // true residual, GPUHybrid.cu:383-386
     spmv(A, x.data(), buffer.data());
@@ -1138,6 +1159,8 @@ extern "C" int orc_hybrid_solve(orc_hybrid *h, const double *b, double *xio)
 }
 
 extern "C" double orc_hybrid_omega(const orc_hybrid *h) { return h->omega; }
+
+extern "C" int orc_hybrid_damping_retry(const orc_hybrid *h) { return h->damping_retry ? 1 : 0; }
 
 extern "C" void orc_hybrid_set_partition(orc_hybrid *h, const int32_t *perm, int32_t nb, const int32_t *part_off,
                                          int32_t nparts)

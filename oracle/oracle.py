@@ -80,6 +80,7 @@ def lib():
         L.orc_hybrid_set_partition.argtypes = [P, P, ctypes.c_int32, P, ctypes.c_int32]
         L.orc_hybrid_omega.restype = ctypes.c_double
         L.orc_hybrid_omega.argtypes = [P]
+        L.orc_hybrid_damping_retry.argtypes = [P]
         L.orc_hybrid_partition.restype = ctypes.c_int32
         L.orc_hybrid_partition.argtypes = [P, P, P]
         _lib = L
@@ -227,7 +228,7 @@ def config(block_size=1, rel_tol=1e-8, abs_tol=0.0, max_iter=10000, base="mas", 
     c.conditioning_threshold = kw.get("conditioning_threshold", 100.0)
     c.base_precond = {"mas": 0, "jacobi": 1, "identity": 2}[base]
     c.threads = threads
-    c.damping = kw.get("damping", 1.5)
+    c.damping = kw.get("damping", 1.8)
     c.composition = {"BPB": 0, "PBP": 1, "additive": 2}[kw.get("composition", "BPB")]
     return c
 
@@ -280,7 +281,8 @@ class Hybrid:
                               ctypes.byref(ns), ctypes.byref(nc))
         return {"num_iterations": it.value, "final_res_norm": res.value, "t_setup": ts.value,
                 "t_solve": tv.value, "n_stiff": ns.value, "n_components": nc.value,
-                "omega": lib().orc_hybrid_omega(self._h)}
+                "omega": lib().orc_hybrid_omega(self._h),
+                "damping_retry": bool(lib().orc_hybrid_damping_retry(self._h))}
 
     def stiff_components(self):
         inf = self.info()

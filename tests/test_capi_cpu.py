@@ -42,7 +42,7 @@ def _create(doc):
     ('{"solver": "Eigen::SimplicialLDLT"}', "GPUHybrid"),
     ('{"solver": "GPUHybrid", "GPUHybrid": {"no_such_key": 1}}', "no_such_key"),
     ('{"solver": "GPUHybrid", "GPUHybrid": {"block_size": 4}}', "block_size"),
-    ('{"solver": "GPUHybrid", "GPUHybrid": {"precision": "fp16"}}', "precision"),
+    ('{"solver": "GPUHybrid", "GPUHybrid": {"precision": "int8"}}', "precision"),
     ('{"solver": "GPUHybrid", "GPUHybrid": {"max_iter": "many"}}', "max_iter"),
     ('{"solver": "GPUHybrid", ', "JSON"),
 ])

@@ -222,3 +222,24 @@ def test_reduced_precision_banks_converge(gpu, name, N, precision):
     # the FP64 outer PCG recovers the accuracy; FP32 bank rounding costs few
     # iterations, BF16 rounding of contact banks (entries spanning 1e8) up to ~2.2x
     assert info["num_iterations"] <= (1.25 if precision == "fp32" else 2.5) * it64 + 3
+
+
+def test_damping_retry_after_breakdown(gpu):
+    # omega * lambda_max(BA) near 3.5 makes the damped sweep indefinite: the PCG
+    # breaks down and the solve restarts from the same start with damping 1.5
+    prob = problems.make("elastic_contact", N=10)
+    A = prob.csc()
+    b = prob.rhs(0)
+    s = Solver.create(_cfg(3, damping=3.5))
+    s.factorize(A)
+    x = s.solve(b)
+    info = s.get_info()
+    assert info["damping_retry"] and info["stop_reason"] == "converged_rel"
+    assert info["omega_used"] == pytest.approx(1.5 / info["lambda_max_BA"], rel=1e-12)
+    assert _res_ok(A.to_scipy(), x, b)
+    H = O.Hybrid(_ocfg(3, damping=3.5))
+    H.factorize(A.colptr, A.rowidx, A.vals)
+    xr = H.solve(b)
+    oi = H.info()
+    assert abs(info["num_iterations"] - oi["num_iterations"]) <= max(2, 0.05 * oi["num_iterations"])
+    assert np.linalg.norm(x - xr) <= 1e-5 * np.linalg.norm(xr)

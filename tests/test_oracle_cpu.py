@@ -263,3 +263,16 @@ def test_zero_rhs_gives_zero_solution():
     H.factorize(A.colptr, A.rowidx, A.vals)
     x = H.solve(np.zeros(A.n), np.ones(A.n))
     assert np.all(x == 0) and H.info()["num_iterations"] == 0
+
+
+def test_damping_retry_recovers_from_breakdown():
+    # damping 3.5 puts omega * lambda_max(BA) far above 2: the damped sweep is
+    # indefinite, the first PCG breaks down, the retry with damping 1.5 converges
+    prob, A = _small("elastic_contact", 8)
+    b = prob.rhs(0)
+    H = O.Hybrid(O.config(block_size=3, rel_tol=1e-10, gmm_jump_threshold=10.0, min_subdomain_size=4, damping=3.5))
+    H.factorize(A.colptr, A.rowidx, A.vals)
+    x = H.solve(b)
+    assert H.info()["damping_retry"]
+    xd = spla.spsolve(A.to_scipy().tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
