@@ -233,7 +233,7 @@ void HybridSolver::select_stiff(int32_t n, const int32_t *ptr, const int32_t *id
         DBuf<double> g0(n), g1(n), part(4 * k::kCsumChunks), sums(8);
         const double *x = d_norms_.get();
         k::canonical_sum(n, x, part.get(), sums.get(), s);
-        k::minmax(n, x, sums.get() + 1, s);
+        k::minmax(n, x, part.get(), sums.get() + 1, s);
         double h[8];
         sums.download(h, 3, s);
         sync(s);

@@ -39,15 +39,28 @@ __device__ __forceinline__ double lane_tree(double v, double *sh)
     return r;
 }
 
+// GPUHybrid.cu:753-765: each row summed sequentially in storage order.  A
+// block of L rows stages the contiguous value range of its rows through LDS
+// in coalesced tiles; every thread adds the values of its own row in order.
+constexpr int kL1Tile = 2048;
 __global__ __launch_bounds__(L) void k_row_l1(int32_t n, const int32_t *__restrict__ ptr,
                                               const double *__restrict__ vals, double *__restrict__ out)
 {
-    // GPUHybrid.cu:753-765: one row per thread, sequential in storage order
-    const int32_t i = blockIdx.x * L + threadIdx.x;
-    if (i >= n) return;
+    __shared__ double tile[kL1Tile];
+    const int32_t r0 = int32_t(blockIdx.x) * L;
+    const int32_t i = r0 + threadIdx.x;
+    const int32_t lo = ptr[r0], hi = ptr[min(r0 + L, n)];
+    const int32_t a = i < n ? ptr[i] : 0, e = i < n ? ptr[i + 1] : 0;
     double s = 0.0;
-    for (int32_t j = ptr[i]; j < ptr[i + 1]; ++j) s += fabs(vals[j]);
-    out[i] = s;
+    for (int32_t c0 = lo; c0 < hi; c0 += kL1Tile) {  // uniform over the block
+        for (int q = threadIdx.x; q < kL1Tile; q += L)
+            if (c0 + q < hi) tile[q] = fabs(vals[c0 + q]);
+        __syncthreads();
+        const int32_t from = max(a, c0), to = min(e, c0 + kL1Tile);
+        for (int32_t j = from; j < to; ++j) s += tile[j - c0];
+        __syncthreads();
+    }
+    if (i < n) out[i] = s;
 }
 
 __global__ __launch_bounds__(L) void k_csum(int64_t n, int64_t clen, const double *__restrict__ v,
@@ -143,14 +156,13 @@ __global__ __launch_bounds__(C / 2) void k_csum_finish(const double *__restrict_
     if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
 }
 
-__global__ __launch_bounds__(256) void k_minmax(int64_t n, const double *__restrict__ v, double *__restrict__ out)
+__global__ __launch_bounds__(256) void k_minmax_finish(const double *__restrict__ part, double *__restrict__ out)
 {
-    // single workgroup: exact regardless of order
     __shared__ double smin[256], smax[256];
     double lo = INFINITY, hi = -INFINITY;
-    for (int64_t i = threadIdx.x; i < n; i += 256) {
-        lo = fmin(lo, v[i]);
-        hi = fmax(hi, v[i]);
+    for (int b = threadIdx.x; b < kMinmaxBlocks; b += 256) {
+        lo = fmin(lo, part[2 * b]);
+        hi = fmax(hi, part[2 * b + 1]);
     }
     smin[threadIdx.x] = lo;
     smax[threadIdx.x] = hi;
@@ -165,6 +177,31 @@ __global__ __launch_bounds__(256) void k_minmax(int64_t n, const double *__restr
     if (threadIdx.x == 0) {
         out[0] = smin[0];
         out[1] = smax[0];
+    }
+}
+
+// min / max are exact in any order: per-block partials, then one block (k_minmax_finish)
+__global__ __launch_bounds__(256) void k_minmax(int64_t n, const double *__restrict__ v, double *__restrict__ part)
+{
+    __shared__ double smin[256], smax[256];
+    double lo = INFINITY, hi = -INFINITY;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        lo = fmin(lo, v[i]);
+        hi = fmax(hi, v[i]);
+    }
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = smin[0];
+        part[2 * blockIdx.x + 1] = smax[0];
     }
 }
 
@@ -198,9 +235,10 @@ void gmm_var_global(int64_t n, const double *x, double mean, double *partials1, 
     LSS_LAUNCH_CHECK();
 }
 
-void minmax(int64_t n, const double *v, double *out2, hipStream_t s)
+void minmax(int64_t n, const double *v, double *part, double *out2, hipStream_t s)
 {
-    k_minmax<<<1, 256, 0, s>>>(n, v, out2);
+    k_minmax<<<kMinmaxBlocks, 256, 0, s>>>(n, v, part);
+    k_minmax_finish<<<1, 256, 0, s>>>(part, out2);
     LSS_LAUNCH_CHECK();
 }
 

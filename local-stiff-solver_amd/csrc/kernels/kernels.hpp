@@ -70,7 +70,9 @@ void permute_out(int32_t nb, int bs, int64_t n, const int32_t *perm, const doubl
 
 // ---------------------------------------------------------------- stiff DOF selection
 void row_l1(int32_t n, const int32_t *ptr, const double *vals, double *out, hipStream_t s);
-void minmax(int64_t n, const double *v, double *out2, hipStream_t s);
+// min and max of v into out2[0..1]; part holds 2 * kMinmaxBlocks partials
+constexpr int kMinmaxBlocks = 512;
+void minmax(int64_t n, const double *v, double *part, double *out2, hipStream_t s);
 struct GmmParams {
     double mean0, mean1, var0, var1, log_w0, log_w1, c0, c1;
 };

@@ -21,61 +21,146 @@ namespace {
 
 constexpr int kSpmvThreads = 256;  // 4 waves = 4 slices per workgroup
 
-__global__ void k_bsr_count(int32_t n, int bs, const int32_t *__restrict__ ptr, const int32_t *__restrict__ idx,
-                            int32_t *__restrict__ cnt, int32_t *__restrict__ unsorted)
-{
-    const int32_t nb = (n + bs - 1) / bs;
-    const int32_t I = blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= nb) return;
+// CSR -> BSR, one wave per block row.  The bs scalar rows of block row I are
+// the contiguous entry range [ptr[I*bs], ptr[I*bs + rows]).  An entry starts
+// a block column of the row's union when it is the first of its block column
+// in its scalar row and no earlier scalar row of the block row holds that
+// block column (binary search; rows are ascending).  The union's block
+// columns are collected in LDS and ranked by counting; a block row with more
+// than kUnionCap distinct block columns is done by lane 0 alone.
+constexpr int kUnionCap = 256;
+
+struct RowRanges {
     int32_t p[3], e[3];
-    const int rows = min(bs, n - I * bs);
-    for (int r = 0; r < bs; ++r) {
-        if (r < rows) {
-            p[r] = ptr[I * bs + r];
-            e[r] = ptr[I * bs + r + 1];
-            for (int32_t k = p[r] + 1; k < e[r]; ++k)
-                if (idx[k] <= idx[k - 1]) *unsorted = 1;
-        } else {
-            p[r] = e[r] = 0;
-        }
+    int rows;
+};
+
+__device__ __forceinline__ RowRanges block_rows(int32_t n, int bs, const int32_t *__restrict__ ptr, int32_t I)
+{
+    RowRanges R;
+    R.rows = min(bs, n - I * bs);
+    for (int r = 0; r < 3; ++r) {
+        R.p[r] = r < R.rows ? ptr[I * bs + r] : 0;
+        R.e[r] = r < R.rows ? ptr[I * bs + r + 1] : 0;
     }
-    int32_t c = 0;
-    bool has_diag = false;
-    while (true) {
-        int32_t cur = 0x7fffffff;
-        for (int r = 0; r < rows; ++r)
-            if (p[r] < e[r]) cur = min(cur, idx[p[r]] / bs);
-        if (cur == 0x7fffffff) break;
-        ++c;
-        has_diag |= (cur == I);
-        for (int r = 0; r < rows; ++r)
-            while (p[r] < e[r] && idx[p[r]] / bs == cur) ++p[r];
-    }
-    if (rows < bs && !has_diag) ++c;  // padded tail diagonal block
-    cnt[I] = c;
+    return R;
 }
 
-__global__ void k_bsr_fill(int32_t n, int bs, const int32_t *__restrict__ ptr, const int32_t *__restrict__ idx,
-                           const double *__restrict__ vals, const int32_t *__restrict__ rowptr,
-                           int32_t *__restrict__ cols, double *__restrict__ bvals)
+// does scalar row [p, e) (ascending) hold an entry of block column c?
+__device__ __forceinline__ bool row_has_block(const int32_t *__restrict__ idx, int32_t p, int32_t e, int bs, int32_t c)
 {
+    int32_t lo = p, hi = e;  // first entry >= c * bs
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (idx[mid] < c * bs) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < e && idx[lo] < (c + 1) * bs;
+}
+
+// union-first test of entry k (row r of the block row)
+__device__ __forceinline__ bool union_first(const int32_t *__restrict__ idx, const RowRanges &R, int r, int32_t k,
+                                            int bs)
+{
+    const int32_t c = idx[k] / bs;
+    if (k > R.p[r] && idx[k - 1] / bs == c) return false;
+    for (int q = 0; q < r; ++q)
+        if (row_has_block(idx, R.p[q], R.e[q], bs, c)) return false;
+    return true;
+}
+
+__device__ __forceinline__ int row_of(const RowRanges &R, int32_t k)
+{
+    return k < R.e[0] ? 0 : (k < R.e[1] ? 1 : 2);
+}
+
+__global__ __launch_bounds__(256) void k_bsr_count(int32_t n, int bs, const int32_t *__restrict__ ptr,
+                                                    const int32_t *__restrict__ idx, int32_t *__restrict__ cnt,
+                                                    int32_t *__restrict__ unsorted)
+{
+    const int lane = threadIdx.x & 63;
     const int32_t nb = (n + bs - 1) / bs;
-    const int32_t I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t I = int32_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (I >= nb) return;
+    const RowRanges R = block_rows(n, bs, ptr, I);
+    const int32_t lo = R.p[0], hi = R.e[R.rows - 1];
+    int32_t c = 0, bad = 0, diag = 0;
+    for (int32_t k = lo + lane; k < hi; k += 64) {
+        const int r = row_of(R, k);
+        if (k > R.p[r] && idx[k] <= idx[k - 1]) bad = 1;
+        const bool first = union_first(idx, R, r, k, bs);
+        c += first;
+        diag |= first && idx[k] / bs == I;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        c += __shfl_xor(c, o, 64);
+        bad |= __shfl_xor(bad, o, 64);
+        diag |= __shfl_xor(diag, o, 64);
+    }
+    if (lane == 0) {
+        if (bad) *unsorted = 1;
+        cnt[I] = c + ((R.rows < bs && !diag) ? 1 : 0);  // padded tail diagonal block
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bsr_fill(int32_t n, int bs, const int32_t *__restrict__ ptr,
+                                                   const int32_t *__restrict__ idx, const double *__restrict__ vals,
+                                                   const int32_t *__restrict__ rowptr, int32_t *__restrict__ cols,
+                                                   double *__restrict__ bvals)
+{
+    __shared__ int32_t ucol[4][kUnionCap];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int32_t nb = (n + bs - 1) / bs;
+    const int32_t I = int32_t(blockIdx.x) * 4 + wv;
     if (I >= nb) return;
     const int bb = bs * bs;
-    int32_t p[3], e[3];
-    const int rows = min(bs, n - I * bs);
-    for (int r = 0; r < bs; ++r) {
-        p[r] = r < rows ? ptr[I * bs + r] : 0;
-        e[r] = r < rows ? ptr[I * bs + r + 1] : 0;
+    const RowRanges R = block_rows(n, bs, ptr, I);
+    const int32_t lo = R.p[0], hi = R.e[R.rows - 1];
+    const int32_t base = rowptr[I];
+    const int32_t m = rowptr[I + 1] - base;  // distinct block columns (with a padded tail diagonal)
+    int32_t *u = ucol[wv];
+    if (m <= kUnionCap) {
+        // collect the union's block columns in LDS (any order)
+        int32_t at = 0;
+        for (int32_t k0 = lo; k0 < hi; k0 += 64) {
+            const int32_t k = k0 + lane;
+            const bool first = k < hi && union_first(idx, R, row_of(R, k), k, bs);
+            const unsigned long long mask = __ballot(first);
+            if (first) u[at + __popcll(mask & ((1ull << lane) - 1ull))] = idx[k] / bs;
+            at += __popcll(mask);
+        }
+        if (at < m && lane == 0) u[at] = I;  // padded tail diagonal block
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // slot of block column c = number of union columns below c
+        for (int32_t q = lane; q < m; q += 64) {
+            const int32_t c = u[q];
+            int32_t rank = 0;
+            for (int32_t t = 0; t < m; ++t) rank += u[t] < c;
+            cols[base + rank] = c;
+            if (c == I && R.rows < bs)
+                for (int r = R.rows; r < bs; ++r) bvals[int64_t(base + rank) * bb + r * bs + r] = 1.0;
+        }
+        for (int32_t k = lo + lane; k < hi; k += 64) {
+            const int r = row_of(R, k);
+            const int32_t c = idx[k] / bs;
+            int32_t rank = 0;
+            for (int32_t t = 0; t < m; ++t) rank += u[t] < c;
+            bvals[int64_t(base + rank) * bb + r * bs + (idx[k] - c * bs)] = vals[k];
+        }
+        return;
     }
-    int32_t w = rowptr[I];
+    if (lane != 0) return;
+    // wide block row: sequential merge of the scalar rows
+    int32_t p[3] = {R.p[0], R.p[1], R.p[2]};
+    int32_t w = base;
     int32_t diag_slot = -1;
-    bool tail_pending = rows < bs;
+    const bool tail_pending = R.rows < bs;
     while (true) {
         int32_t cur = 0x7fffffff;
-        for (int r = 0; r < rows; ++r)
-            if (p[r] < e[r]) cur = min(cur, idx[p[r]] / bs);
+        for (int r = 0; r < R.rows; ++r)
+            if (p[r] < R.e[r]) cur = min(cur, idx[p[r]] / bs);
         if (tail_pending && I < cur && diag_slot < 0) {  // insert missing diagonal in order
             cols[w] = I;
             diag_slot = w++;
@@ -84,33 +169,40 @@ __global__ void k_bsr_fill(int32_t n, int bs, const int32_t *__restrict__ ptr, c
         if (cur == 0x7fffffff) break;
         if (cur == I) diag_slot = w;
         cols[w] = cur;
-        for (int r = 0; r < rows; ++r)
-            while (p[r] < e[r] && idx[p[r]] / bs == cur) {
+        for (int r = 0; r < R.rows; ++r)
+            while (p[r] < R.e[r] && idx[p[r]] / bs == cur) {
                 bvals[int64_t(w) * bb + r * bs + (idx[p[r]] - cur * bs)] = vals[p[r]];
                 ++p[r];
             }
         ++w;
     }
     if (tail_pending && diag_slot >= 0)
-        for (int r = rows; r < bs; ++r) bvals[int64_t(diag_slot) * bb + r * bs + r] = 1.0;
+        for (int r = R.rows; r < bs; ++r) bvals[int64_t(diag_slot) * bb + r * bs + r] = 1.0;
 }
 
-__global__ void k_bsr_permute(int32_t nb, int bs, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
-                              const double *__restrict__ vals, const int32_t *__restrict__ perm,
-                              const int32_t *__restrict__ new_rowptr, int32_t *__restrict__ new_cols,
-                              double *__restrict__ new_vals)
+// One wave per block row: each lane takes entries of the row, ranks them by
+// permuted column among all entries of the row and moves its block.
+__global__ __launch_bounds__(256) void k_bsr_permute(int32_t nb, int bs, const int32_t *__restrict__ rowptr,
+                                                      const int32_t *__restrict__ cols,
+                                                      const double *__restrict__ vals,
+                                                      const int32_t *__restrict__ perm,
+                                                      const int32_t *__restrict__ new_rowptr,
+                                                      int32_t *__restrict__ new_cols, double *__restrict__ new_vals)
 {
-    const int32_t I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int32_t I = int32_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
     if (I >= nb) return;
     const int bb = bs * bs;
     const int32_t b = rowptr[I], e = rowptr[I + 1];
     const int32_t dst = new_rowptr[perm[I]];
-    for (int32_t k = b; k < e; ++k) {
+    for (int32_t k = b + lane; k < e; k += 64) {
         const int32_t ck = perm[cols[k]];
         int32_t rank = 0;
         for (int32_t q = b; q < e; ++q) rank += perm[cols[q]] < ck;
         new_cols[dst + rank] = ck;
-        for (int t = 0; t < bb; ++t) new_vals[int64_t(dst + rank) * bb + t] = vals[int64_t(k) * bb + t];
+        const double *src = vals + int64_t(k) * bb;
+        double *out = new_vals + int64_t(dst + rank) * bb;
+        for (int t = 0; t < bb; ++t) out[t] = src[t];
     }
 }
 
@@ -260,7 +352,7 @@ void bsr_count(int32_t n, int bs, const int32_t *ptr, const int32_t *idx, int32_
 {
     const int32_t nb = (n + bs - 1) / bs;
     if (nb == 0) return;
-    k_bsr_count<<<div_up(nb, 256), 256, 0, s>>>(n, bs, ptr, idx, cnt, unsorted_flag);
+    k_bsr_count<<<div_up(nb, 4), 256, 0, s>>>(n, bs, ptr, idx, cnt, unsorted_flag);
     LSS_LAUNCH_CHECK();
 }
 
@@ -269,7 +361,7 @@ void bsr_fill(int32_t n, int bs, const int32_t *ptr, const int32_t *idx, const d
 {
     const int32_t nb = (n + bs - 1) / bs;
     if (nb == 0) return;
-    k_bsr_fill<<<div_up(nb, 256), 256, 0, s>>>(n, bs, ptr, idx, vals, rowptr, cols, bvals);
+    k_bsr_fill<<<div_up(nb, 4), 256, 0, s>>>(n, bs, ptr, idx, vals, rowptr, cols, bvals);
     LSS_LAUNCH_CHECK();
 }
 
@@ -277,7 +369,7 @@ void bsr_permute(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols,
                  const int32_t *perm, const int32_t *new_rowptr, int32_t *new_cols, double *new_vals, hipStream_t s)
 {
     if (nb == 0) return;
-    k_bsr_permute<<<div_up(nb, 256), 256, 0, s>>>(nb, bs, rowptr, cols, vals, perm, new_rowptr, new_cols, new_vals);
+    k_bsr_permute<<<div_up(nb, 4), 256, 0, s>>>(nb, bs, rowptr, cols, vals, perm, new_rowptr, new_cols, new_vals);
     LSS_LAUNCH_CHECK();
 }
 
