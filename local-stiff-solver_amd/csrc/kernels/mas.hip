@@ -76,7 +76,10 @@ __device__ __forceinline__ int packed_lower(int i, int j)
 }
 
 // Assemble the bank matrices of one level from its block operator, factor
-// A_b = L L^T and invert L in LDS; store G = L^{-1} packed lower.
+// A_b = L L^T and invert L in LDS; store G = L^{-1} packed lower.  Rows of the
+// LDS matrix are padded to an odd number of doubles (DP = D + 1) so that the
+// lanes of a wave reading one column hit different LDS banks; the trailing
+// updates assign rows to waves and columns to lanes.
 template <int BS, class TS>
 __global__ __launch_bounds__(kInvThreads) void k_bank_factor(int32_t nbanks, const int32_t *__restrict__ bank_off,
                                                              const int32_t *__restrict__ rowptr,
@@ -85,21 +88,23 @@ __global__ __launch_bounds__(kInvThreads) void k_bank_factor(int32_t nbanks, con
                                                              int *__restrict__ status)
 {
     constexpr int D = 32 * BS;
+    constexpr int DP = D + 1;
     constexpr int S = D * (D + 1) / 2;
     constexpr int BB = BS * BS;
+    constexpr int NW = kInvThreads / 64;
     extern __shared__ double lds[];
-    double *M = lds;         // D x D row-major, lower triangle used
-    double *tmp = lds + D * D;
+    double *M = lds;         // D x DP row-major, lower triangle used
+    double *tmp = lds + D * DP;
     __shared__ int bad;
     const int32_t bank = blockIdx.x;
     if (bank >= nbanks) return;
     const int32_t lo = bank_off[bank], hi = bank_off[bank + 1];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (threadIdx.x == 0) bad = 0;
-    for (int t = threadIdx.x; t < D * D; t += kInvThreads) M[t] = 0.0;
+    for (int t = threadIdx.x; t < D * DP; t += kInvThreads) M[t] = 0.0;
     __syncthreads();
     // assemble: block (i, J) with both nodes in the bank
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int32_t i = lo + wid; i < hi; i += kInvThreads / 64) {
+    for (int32_t i = lo + wid; i < hi; i += NW) {
         for (int32_t k = rowptr[i] + lane; k < rowptr[i + 1]; k += 64) {
             const int32_t J = cols[k];
             if (J < lo || J >= hi) continue;
@@ -107,58 +112,55 @@ __global__ __launch_bounds__(kInvThreads) void k_bank_factor(int32_t nbanks, con
 #pragma unroll
             for (int a = 0; a < BS; ++a)
 #pragma unroll
-                for (int b = 0; b < BS; ++b) M[(r0 + a) * D + c0 + b] = vals[int64_t(k) * BB + a * BS + b];
+                for (int b = 0; b < BS; ++b) M[(r0 + a) * DP + c0 + b] = vals[int64_t(k) * BB + a * BS + b];
         }
     }
     __syncthreads();
     // zero diagonal -> 1 (MASPreconditioner.cu:779-796): padding nodes and empty rows
     for (int d = threadIdx.x; d < D; d += kInvThreads)
-        if (M[d * D + d] == 0.0) M[d * D + d] = 1.0;
+        if (M[d * DP + d] == 0.0) M[d * DP + d] = 1.0;
     __syncthreads();
     // Cholesky, right-looking, lower triangle in place
     for (int k = 0; k < D; ++k) {
-        if (threadIdx.x == 0) {
-            const double d = M[k * D + k];
-            if (!(d > 0.0) || !isfinite(d)) bad = 1;
-            M[k * D + k] = sqrt(d);
+        const double d = M[k * DP + k];
+        if (threadIdx.x == 0 && (!(d > 0.0) || !isfinite(d))) bad = 1;
+        const double rdk = 1.0 / sqrt(d);
+        // scaled column k below the diagonal, kept in tmp as well
+        for (int i = k + 1 + threadIdx.x; i < D; i += kInvThreads) {
+            const double v = M[i * DP + k] * rdk;
+            M[i * DP + k] = v;
+            tmp[i] = v;
         }
         __syncthreads();
-        const double dk = M[k * D + k];
-        for (int i = k + 1 + threadIdx.x; i < D; i += kInvThreads) M[i * D + k] /= dk;
-        __syncthreads();
-        const int T = D - k - 1;
-        for (int t = threadIdx.x; t < T * T; t += kInvThreads) {
-            const int i = k + 1 + t / T, j = k + 1 + t % T;
-            if (j <= i) M[i * D + j] -= M[i * D + k] * M[j * D + k];
+        if (threadIdx.x == 0) M[k * DP + k] = sqrt(d);
+        for (int i = k + 1 + wid; i < D; i += NW) {
+            const double li = tmp[i];
+            for (int j = k + 1 + lane; j <= i; j += 64) M[i * DP + j] -= li * tmp[j];
         }
         __syncthreads();
     }
     // G = L^{-1} in place, columns from the last to the first (LAPACK trti2, lower)
     for (int j = D - 1; j >= 0; --j) {
-        if (threadIdx.x == 0) M[j * D + j] = 1.0 / M[j * D + j];
-        __syncthreads();
-        const double ajj = -M[j * D + j];
-        // x(j+1:D) := G(j+1:D, j+1:D) * L(j+1:D, j), then scale by ajj
+        const double gjj = 1.0 / M[j * DP + j];
+        // x(j+1:D) := G(j+1:D, j+1:D) * L(j+1:D, j), then scale by -G(j, j)
         for (int i = j + 1 + threadIdx.x; i < D; i += kInvThreads) {
             double acc = 0.0;
-            for (int q = j + 1; q <= i; ++q) acc += M[i * D + q] * M[q * D + j];
-            tmp[i] = acc * ajj;
+            for (int q = j + 1; q <= i; ++q) acc += M[i * DP + q] * M[q * DP + j];
+            tmp[i] = -acc * gjj;
         }
         __syncthreads();
-        for (int i = j + 1 + threadIdx.x; i < D; i += kInvThreads) M[i * D + j] = tmp[i];
+        for (int i = j + 1 + threadIdx.x; i < D; i += kInvThreads) M[i * DP + j] = tmp[i];
+        if (threadIdx.x == 0) M[j * DP + j] = gjj;
         __syncthreads();
     }
     TS *dst = out + int64_t(bank) * S;
     bool ok = true;
-    for (int t = threadIdx.x; t < S; t += kInvThreads) {
-        int i = int((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-        while (i > 0 && packed_lower(i, 0) > t) --i;
-        while (i + 1 < D && packed_lower(i + 1, 0) <= t) ++i;
-        const int j = t - packed_lower(i, 0);
-        const double v = M[i * D + j];
-        if (!isfinite(v)) ok = false;
-        dst[t] = to_store<TS>(v);
-    }
+    for (int i = wid; i < D; i += NW)
+        for (int j = lane; j <= i; j += 64) {
+            const double v = M[i * DP + j];
+            if (!isfinite(v)) ok = false;
+            dst[packed_lower(i, j)] = to_store<TS>(v);
+        }
     if (!ok || (threadIdx.x == 0 && bad)) atomicOr(status, 1);
 }
 
@@ -372,7 +374,7 @@ void launch_bank_invert(int bs, int32_t nbanks, const int32_t *bank_off, const i
 {
     if (nbanks == 0) return;
     const int D = 32 * bs;
-    const size_t lds = size_t(D * D + D) * sizeof(double);
+    const size_t lds = size_t(D * (D + 1) + D) * sizeof(double);
     switch (bs) {
         case 1:
             k_bank_factor<1, TS><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out, status);
@@ -394,7 +396,7 @@ void mas_set_lds_limits()
 {
     static bool done = false;
     if (done) return;
-    const size_t lds3 = size_t(96 * 96 + 96) * sizeof(double);
+    const size_t lds3 = size_t(96 * 97 + 96) * sizeof(double);
     LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, double>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
     LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, float>),

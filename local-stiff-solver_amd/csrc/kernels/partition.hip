@@ -472,7 +472,7 @@ int32_t device_partition(int32_t nb, const int32_t *rowptr, const int32_t *cols,
         LSS_HIP(hipMemcpyAsync(hflags, dflags.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
         LSS_HIP(hipStreamSynchronize(s));
         if (!hflags[0]) break;  // only leaves at this depth: the member array is final
-        const int32_t bx = std::max(1, 1024 >> depth);
+        const int32_t bx = std::max(1, 8192 >> depth);  // about 8192 blocks: one pass of 16 rows per block
         const int64_t grid = int64_t(nsub) * bx;
         // each step reaches a node, restarts or switches searches: 4 nb bounds the steps
         const int64_t max_batches = (4 * int64_t(nb)) / 8 + 16;
