@@ -77,6 +77,8 @@ def main():
                     help="extra GPUHybrid configuration entry for both paths (experiments)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (HBM counter calibration)")
+    ap.add_argument("--multi-rhs", type=int, default=0, metavar="K",
+                    help="also time one fused K-right-hand-side solve of the mixed path against K single solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solver has no multi-GPU path.",
@@ -143,6 +145,37 @@ def main():
                      "x": x.cpu().numpy().copy(), "stop": info["stop_reason"]}
         log(name, f"{dt:.3f} s", info["num_iterations"], "iterations", info["stop_reason"])
         del s
+    multi = None
+    if args.multi_rhs > 1:
+        # solve phase only: one factorization, K right-hand sides fused vs one by one
+        K = args.multi_rhs
+        s = Solver.create({"solver": "GPUHybrid", "GPUHybrid": paths["mixed"]})
+        s.factorize_device(n, d_ptr, d_idx, d_val, stream=stream)
+        dB = torch.stack([torch.from_numpy(prob.rhs(c)).to(dev) for c in range(K)])
+        dX = torch.zeros_like(dB)
+        s.solve_multi_device(dB, dX, stream=stream)  # warm-up
+        dX.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.solve_multi_device(dB, dX, stream=stream)
+        torch.cuda.synchronize()
+        t_fused = time.perf_counter() - t0
+        minfo = s.get_info()
+        xs = torch.zeros_like(dB)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for c in range(K):
+            s.solve_device(dB[c], xs[c], stream=stream)
+        torch.cuda.synchronize()
+        t_seq = time.perf_counter() - t0
+        bn = torch.linalg.norm(dB, dim=1).cpu().numpy()
+        multi = {"k": K, "fused_solve_s": round(t_fused, 4), "single_solves_s": round(t_seq, 4),
+                 "speedup": round(t_seq / t_fused, 3),
+                 "bit_identical_to_single": bool(torch.equal(dX, xs)),
+                 "iterations": [c["num_iterations"] for c in minfo["columns"]],
+                 "rel_residual": [c["final_res_norm"] / float(bn[i]) for i, c in enumerate(minfo["columns"])]}
+        log("multi-rhs", K, f"fused {t_fused:.3f} s", f"single {t_seq:.3f} s")
+        del s, dB, dX, xs
     # roofline of the outer FP64 SpMV, timed with HIP events on the solver's stream
     g = dict(paths["mixed"], profile_kernels=True)
     s = Solver.create({"solver": "GPUHybrid", "GPUHybrid": g})
@@ -204,6 +237,8 @@ def main():
         "kernel_ms": pinfo["kernel_ms"],
         "cpu_baseline": cpu,
     }
+    if multi is not None:
+        line["multi_rhs"] = multi
     print(json.dumps(line))
 
 

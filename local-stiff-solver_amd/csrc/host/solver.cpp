@@ -49,10 +49,10 @@ HybridSolver::HybridSolver(const Config &cfg) : cfg_(cfg)
     // callers that pass NULL (e.g. torch's default stream) need no extra sync
     LSS_HIP(hipStreamCreate(&stream_));
     void *p = nullptr;
-    LSS_HIP(hipHostMalloc(&p, sizeof(dev::PcgState), hipHostMallocDefault));
+    LSS_HIP(hipHostMalloc(&p, sizeof(dev::PcgState) * kMaxCols, hipHostMallocDefault));
     h_st_ = static_cast<dev::PcgState *>(p);
-    std::memset(h_st_, 0, sizeof(dev::PcgState));
-    st_.resize(1);
+    std::memset(h_st_, 0, sizeof(dev::PcgState) * kMaxCols);
+    st_.resize(kMaxCols);
     st_.zero(stream_);
     partials_.resize(kPartialsCap);
     d_status_.resize(1);
@@ -477,8 +477,8 @@ bool HybridSolver::build_levels_device(DBuf<int32_t> &&part_off, int32_t nparts,
         k::mas_galerkin(nslots, bs_, sptr.get(), sblk.get(), fine_vals, nxt.vals.get(), s);
         nxt.child_ptr = std::move(child_ptr);
         nxt.child = std::move(child);
-        nxt.r.resize(size_t(coarse_n) * bs_);
-        nxt.w.resize(size_t(coarse_n) * bs_);
+        nxt.r.resize(size_t(coarse_n) * bs_ * kMaxCols);
+        nxt.w.resize(size_t(coarse_n) * bs_ * kMaxCols);
         nxt.nbanks = k::uniform_banks(coarse_n, nxt.bank_off, s);
         mas_[l].parent = std::move(parent);
         sync(s);  // the source lists are released at scope end
@@ -514,8 +514,8 @@ void HybridSolver::build_levels_host(const Pattern &gp, hipStream_t s)
             k::mas_galerkin(nslots, bs_, sptr.get(), sblk.get(), fine_vals, d.vals.get(), s);
             d.child_ptr.upload(h.child_ptr.data(), h.child_ptr.size(), s);
             d.child.upload(h.child.data(), std::max<size_t>(h.child.size(), 1), s);
-            d.r.resize(size_t(h.nodes) * bs_);
-            d.w.resize(size_t(h.nodes) * bs_);
+            d.r.resize(size_t(h.nodes) * bs_ * kMaxCols);
+            d.w.resize(size_t(h.nodes) * bs_ * kMaxCols);
             sync(s);  // sptr / sblk are released at scope end
         }
         if (!h.parent.empty()) d.parent.upload(h.parent.data(), h.parent.size(), s);
@@ -603,13 +603,26 @@ void HybridSolver::build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx
     k::stiff_inverse_factor(ncomp_, int32_t(max_comp_), d_comp_off_.get(), d_mat_off_.get(), d_L_.get(), G.get(), s);
     sync(s);
     d_L_ = std::move(G);  // only the inverse factor is needed by the apply
-    d_swork_.resize(std::max<int64_t>(2 * ns_, 1));
+    d_swork_.resize(std::max<int64_t>(2 * kMaxCols * ns_, 1));
 }
 
 void HybridSolver::alloc_vectors()
 {
     const size_t np = size_t(nb_) * bs_;
     xp_.resize(np), bp_.resize(np), r_.resize(np), p_.resize(np), q_.resize(np), z_.resize(np), u_.resize(np);
+    kcap_ = 1;
+    // partials of one reduction for kMaxCols columns
+    int64_t per = std::max<int64_t>(k::partials_per_column(int64_t(np)), k::spmv_partials_per_column(sell_));
+    if (!mas_.empty()) per = std::max<int64_t>(per, mas_[0].nbanks);
+    partials_.resize(std::max<size_t>(kPartialsCap, size_t(per) * kMaxCols));
+}
+
+void HybridSolver::ensure_columns(int k)
+{
+    if (k <= kcap_) return;
+    const size_t m = size_t(nb_) * bs_ * k;
+    xp_.resize(m), bp_.resize(m), r_.resize(m), p_.resize(m), q_.resize(m), z_.resize(m), u_.resize(m);
+    kcap_ = k;
 }
 
 namespace {
@@ -698,62 +711,63 @@ void HybridSolver::estimate_omega(hipStream_t s)
 
 // ------------------------------------------------------------------ preconditioner
 void HybridSolver::base_apply(const double *r, double scale, const double *add, double *z, dev::PcgState *st,
-                              const int *stop, hipStream_t s, bool first)
+                              const int *stop, hipStream_t s, bool first, int k)
 {
     const int64_t np = int64_t(nb_) * bs_;
     const int op = first ? dev::kFinishRzFirst : dev::kFinishRz;
     if (cfg_.base_preconditioner != "MAS") {
-        const int g =
-            k::jacobi_apply(np, d_dinv_.get(), r, scale, add, z, st ? r_.get() : nullptr, st, partials_.get(), stop, s);
-        if (st) k::finish(g, partials_.get(), op, st, nullptr, 0, s);
+        const int g = k::jacobi_apply(np, d_dinv_.get(), r, scale, add, z, st ? r_.get() : nullptr, st,
+                                      partials_.get(), stop, s, k, np);
+        if (st) k::finish(g, partials_.get(), op, st, nullptr, 0, s, k);
         return;
     }
     const int L = int(mas_.size()) - 1;
+    auto ld = [&](int l) { return l == 0 ? np : int64_t(mas_[l].nodes) * bs_; };
     const double *src = r;
     for (int l = 1; l <= L; ++l) {
         k::mas_restrict(mas_[l].nodes, bs_, mas_[l].child_ptr.get(), mas_[l].child.get(), src, mas_[l].r.get(), stop,
-                        s);
+                        s, k, ld(l - 1), ld(l));
         src = mas_[l].r.get();
     }
     for (int l = L; l >= 1; --l) {
         const bool top = (l == L);
         k::mas_coarse_apply(prec(), bs_, mas_[l].nodes, mas_[l].inv.get(), mas_[l].r.get(),
                             top ? nullptr : mas_[l].parent.get(), top ? nullptr : mas_[l + 1].w.get(),
-                            mas_[l].w.get(), stop, s);
+                            mas_[l].w.get(), stop, s, k, ld(l), top ? 0 : ld(l + 1));
     }
-    const bool prof = cfg_.profile_kernels && st;
+    const bool prof = cfg_.profile_kernels && st && k == 1;
     if (prof) prof_begin(2, s);
     k::mas_level0_apply(prec(), bs_, mas_[0].nbanks, mas_[0].bank_off.get(), mas_[0].inv.get(), r,
                         L >= 1 ? mas_[0].parent.get() : nullptr, L >= 1 ? mas_[1].w.get() : nullptr, scale, add, z,
-                        st ? r_.get() : nullptr, st, partials_.get(), stop, s);
+                        st ? r_.get() : nullptr, st, partials_.get(), stop, s, k, np, L >= 1 ? ld(1) : 0);
     if (prof) prof_end(s);
-    if (st) k::finish(mas_[0].nbanks, partials_.get(), op, st, nullptr, 0, s);
+    if (st) k::finish(mas_[0].nbanks, partials_.get(), op, st, nullptr, 0, s, k);
 }
 
 void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, const int *stop, hipStream_t s,
-                                 bool first)
+                                 bool first, int k)
 {
     // custom_mixed_precond_iter, GPUHybrid.cu:521-542
     if (!cfg_.do_hybrid_precond || ncomp_ == 0) {
-        base_apply(r, 1.0, nullptr, z, st, stop, s, first);
+        base_apply(r, 1.0, nullptr, z, st, stop, s, first, k);
         return;
     }
-    base_apply(r, omega_, nullptr, z, nullptr, stop, s);  // z1 = B r
+    const int64_t np = int64_t(nb_) * bs_;
+    base_apply(r, omega_, nullptr, z, nullptr, stop, s, false, k);  // z1 = B r
     // z2 = z1 + P (r - A z1)  (dss_precond_iter, GPUHybrid.cu:544-675)
     k::stiff_correct(ncomp_, ns_, bs_, d_comp_off_.get(), d_mat_off_.get(), d_row_comp_.get(), d_pdofs_.get(),
-                     d_L_.get(),
-                     d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z, d_swork_.get(), d_swork_.get() + ns_, stop,
-                     s);
+                     d_L_.get(), d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z, d_swork_.get(),
+                     d_swork_.get() + kMaxCols * ns_, stop, s, k, np);
     // u = r - A z2
-    const bool prof = cfg_.profile_kernels && st;
+    const bool prof = cfg_.profile_kernels && st && k == 1;
     if (prof) prof_begin(1, s);
-    k::spmv_residual(sell_, z, r, u_.get(), stop, s, cfg_.fp32_inner());
+    k::spmv_residual(sell_, z, r, u_.get(), stop, s, cfg_.fp32_inner(), k, np);
     if (prof) prof_end(s);
     if (cfg_.fp32_inner())
         k::stiff_rows_exact(ns_, bs_, d_pdofs_.get(), d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z, u_.get(),
-                            stop, s);
+                            stop, s, k, np);
     // z = z2 + B u
-    base_apply(u_.get(), omega_, z, z, st, stop, s, first);
+    base_apply(u_.get(), omega_, z, z, st, stop, s, first, k);
 }
 
 // ------------------------------------------------------------------ PCG
@@ -803,76 +817,111 @@ void HybridSolver::prof_collect()
     prof_mas0_n_ = cnt[2];
 }
 
-void HybridSolver::run_pcg(hipStream_t s)
+void HybridSolver::run_pcg(hipStream_t s, int k)
 {
-    // GPUHybrid::pcg_solve, GPUHybrid.cu:1413-1512, scalars on the device
+    // GPUHybrid::pcg_solve, GPUHybrid.cu:1413-1512, scalars on the device.  k > 1:
+    // k independent PCG recurrences in lockstep; every launch serves all columns,
+    // a column that stops keeps its x and r, and the loop ends when all stopped.
     const int64_t np = int64_t(nb_) * bs_;
     dev::PcgState *st = st_.get();
-    DBuf<double> sc(1);
-    k::dot(np, bp_.get(), bp_.get(), partials_.get(), sc.get(), s);
-    const double bb = d2h_scalar(sc.get(), s);
-    if (!(bb > 0.0)) {
-        k::fill(np, xp_.get(), 0.0, s);
+    DBuf<double> sc(k);
+    double bb[kMaxCols];
+    k::dot(np, bp_.get(), bp_.get(), partials_.get(), sc.get(), s, k, np);
+    sc.download(bb, k, s);
+    sync(s);
+    std::memset(h_st_, 0, sizeof(dev::PcgState) * k);
+    bool any = false;
+    for (int c = 0; c < k; ++c) {
+        h_st_[c].bb = bb[c];
+        h_st_[c].rel_eps = cfg_.relative_tolerance * cfg_.relative_tolerance;
+        h_st_[c].abs_eps = cfg_.absolute_tolerance * cfg_.absolute_tolerance;
+        if (!(bb[c] > 0.0)) {  // zero right-hand side: x = 0 without iterating
+            h_st_[c].stop = dev::kZeroRhs;
+            k::fill(np, xp_.get() + c * np, 0.0, s);
+        } else {
+            any = true;
+        }
+    }
+    col_iters_.assign(k, 0);
+    col_res_.assign(k, 0.0);
+    col_stop_.assign(k, dev::kZeroRhs);
+    restarts_ = 0;
+    if (!any) {
         num_iterations_ = 0;
         final_res_norm_ = 0.0;
-        stop_code_ = 0;
+        stop_code_ = dev::kZeroRhs;
         return;
     }
-    std::memset(h_st_, 0, sizeof(dev::PcgState));
-    h_st_->bb = bb;
-    h_st_->rel_eps = cfg_.relative_tolerance * cfg_.relative_tolerance;
-    h_st_->abs_eps = cfg_.absolute_tolerance * cfg_.absolute_tolerance;
-    LSS_HIP(hipMemcpyAsync(st, h_st_, sizeof(dev::PcgState), hipMemcpyHostToDevice, s));
+    LSS_HIP(hipMemcpyAsync(st, h_st_, sizeof(dev::PcgState) * k, hipMemcpyHostToDevice, s));
     ev_marks_.clear();
     ev_used_ = 0;
     prof_spmv_outer_n_ = prof_spmv_inner_n_ = prof_mas0_n_ = 0;
 
-    k::spmv_residual(sell_, xp_.get(), bp_.get(), r_.get(), nullptr, s);  // r = b - A x
-    precond_apply(r_.get(), z_.get(), st, nullptr, s, true);               // z = M r, gamma = r.z
-    k::copy(np, z_.get(), p_.get(), s);
-    int k = 0;
-    restarts_ = 0;
+    auto poll = [&]() {
+        LSS_HIP(hipMemcpyAsync(h_st_, st, sizeof(dev::PcgState) * k, hipMemcpyDeviceToHost, s));
+        sync(s);
+        for (int c = 0; c < k; ++c)
+            if (!h_st_[c].stop) return false;
+        return true;
+    };
+    k::spmv_residual(sell_, xp_.get(), bp_.get(), r_.get(), nullptr, s, false, k, np);  // r = b - A x
+    precond_apply(r_.get(), z_.get(), st, nullptr, s, true, k);                        // z = M r, gamma = r.z
+    k::copy(np * k, z_.get(), p_.get(), s);
+    int it = 0;
     while (true) {
         bool stopped = false;
-        while (k < cfg_.max_iter && !stopped) {
-            const int batch = std::min(cfg_.check_every, cfg_.max_iter - k);
-            for (int b = 0; b < batch; ++b, ++k) {
-                const bool prof = cfg_.profile_kernels;
+        while (it < cfg_.max_iter && !stopped) {
+            const int batch = std::min(cfg_.check_every, cfg_.max_iter - it);
+            for (int b = 0; b < batch; ++b, ++it) {
+                const bool prof = cfg_.profile_kernels && k == 1;
                 if (prof) prof_begin(0, s);
-                k::spmv_pq(sell_, p_.get(), q_.get(), st, partials_.get(), k, s);
+                k::spmv_pq(sell_, p_.get(), q_.get(), st, partials_.get(), it, s, k, np);
                 if (prof) prof_end(s);
-                k::pcg_update_xr(np, xp_.get(), r_.get(), p_.get(), q_.get(), st, partials_.get(), s);
-                precond_apply(r_.get(), z_.get(), st, &st->stop, s);
-                k::pcg_update_p(np, p_.get(), z_.get(), st, s);
+                k::pcg_update_xr(np, xp_.get(), r_.get(), p_.get(), q_.get(), st, partials_.get(), s, k, np);
+                precond_apply(r_.get(), z_.get(), st, &st->stop, s, false, k);
+                k::pcg_update_p(np, p_.get(), z_.get(), st, s, k, np);
             }
-            LSS_HIP(hipMemcpyAsync(h_st_, st, sizeof(dev::PcgState), hipMemcpyDeviceToHost, s));
-            sync(s);
-            stopped = h_st_->stop != 0;
+            stopped = poll();
         }
-        const int code = h_st_->stop;
-        if (!cfg_.true_residual_check || (code != dev::kConvergedRel && code != dev::kConvergedAbs)) break;
+        const int code = h_st_[0].stop;
+        if (k > 1 || !cfg_.true_residual_check || (code != dev::kConvergedRel && code != dev::kConvergedAbs)) break;
         // The recurrence residual drifts from b - A x on badly conditioned systems;
         // accept only a true residual inside the tolerance, else restart from it.
         k::spmv_residual(sell_, xp_.get(), bp_.get(), r_.get(), nullptr, s);
         k::dot(np, r_.get(), r_.get(), partials_.get(), sc.get(), s);
         const double rr = d2h_scalar(sc.get(), s);
-        if ((h_st_->rel_eps > 0 && rr / bb < h_st_->rel_eps) || (h_st_->abs_eps > 0 && rr < h_st_->abs_eps)) break;
-        if (k >= cfg_.max_iter || restarts_ >= 50) break;
+        if ((h_st_->rel_eps > 0 && rr / bb[0] < h_st_->rel_eps) || (h_st_->abs_eps > 0 && rr < h_st_->abs_eps))
+            break;
+        if (it >= cfg_.max_iter || restarts_ >= 50) break;
         ++restarts_;
         h_st_->stop = 0;
         LSS_HIP(hipMemcpyAsync(&st->stop, &h_st_->stop, sizeof(int), hipMemcpyHostToDevice, s));
         precond_apply(r_.get(), z_.get(), st, nullptr, s, true);
         k::copy(np, z_.get(), p_.get(), s);
     }
-    LSS_HIP(hipMemcpyAsync(h_st_, st, sizeof(dev::PcgState), hipMemcpyDeviceToHost, s));
+    poll();
+    // true residual ||A x - b|| per column (GPUHybrid.cu:383-386)
+    k::spmv_residual(sell_, xp_.get(), bp_.get(), q_.get(), nullptr, s, false, k, np);
+    k::dot(np, q_.get(), q_.get(), partials_.get(), sc.get(), s, k, np);
+    double rr[kMaxCols];
+    sc.download(rr, k, s);
     sync(s);
-    num_iterations_ = h_st_->iters;
-    stop_code_ = h_st_->stop;
-    // true residual ||A x - b|| (GPUHybrid.cu:383-386)
-    k::spmv_residual(sell_, xp_.get(), bp_.get(), q_.get(), nullptr, s);
-    k::dot(np, q_.get(), q_.get(), partials_.get(), sc.get(), s);
-    final_res_norm_ = std::sqrt(d2h_scalar(sc.get(), s));
-    if (cfg_.profile_kernels) prof_collect();
+    num_iterations_ = 0;
+    final_res_norm_ = 0.0;
+    stop_code_ = -1;
+    for (int c = 0; c < k; ++c) {
+        col_iters_[c] = h_st_[c].iters;
+        col_stop_[c] = h_st_[c].stop;
+        col_res_[c] = std::sqrt(rr[c]);
+        num_iterations_ = std::max(num_iterations_, col_iters_[c]);
+        final_res_norm_ = std::max(final_res_norm_, col_res_[c]);
+        // the solve reports the first column that did not converge, else column 0
+        const bool conv = col_stop_[c] == dev::kConvergedRel || col_stop_[c] == dev::kConvergedAbs ||
+                          col_stop_[c] == dev::kZeroRhs;
+        if (stop_code_ < 0 && !conv) stop_code_ = col_stop_[c];
+    }
+    if (stop_code_ < 0) stop_code_ = col_stop_[0];
+    if (cfg_.profile_kernels && k == 1) prof_collect();
 }
 
 void HybridSolver::solve_device(const double *b, double *x, hipStream_t s)
@@ -897,7 +946,7 @@ void HybridSolver::solve_device(const double *b, double *x, hipStream_t s)
     }
     damping_retry_ = false;
     run_pcg(s);
-    if (damped && stop_code_ >= dev::kZeroSdotp) {
+    if (damped && dev::is_breakdown(stop_code_)) {
         damping_retry_ = true;
         const double omega = omega_;
         omega_ = kSafeDamping / lambda_max_;
@@ -910,6 +959,7 @@ void HybridSolver::solve_device(const double *b, double *x, hipStream_t s)
     }
     k::permute_out(nb_, bs_, n_, d_perm_.get(), xp_.get(), x, s);
     sync(s);
+    last_k_ = 1;
     t_solve_ = since(t0);
 }
 
@@ -923,9 +973,79 @@ void HybridSolver::solve_host(const double *b, double *x)
     sync(stream_);
 }
 
-void HybridSolver::solve_multi_device(int32_t k, const double *B, double *X, hipStream_t s)
+void HybridSolver::solve_multi_device(int32_t nrhs, const double *B, double *X, hipStream_t s)
 {
-    for (int32_t c = 0; c < k; ++c) solve_device(B + int64_t(c) * n_, X + int64_t(c) * n_, s);
+    if (!factorized_) fail(LSS_ERR_NOT_FACTORIZED, "solve called before factorize");
+    if (nrhs <= 0) fail(LSS_ERR_ARG, "number of right-hand sides must be positive");
+    if (!s) s = stream_;
+    // the restart logic of true_residual_check is per solve: keep those one by one
+    if (nrhs == 1 || cfg_.true_residual_check) {
+        for (int32_t c = 0; c < nrhs; ++c) solve_device(B + int64_t(c) * n_, X + int64_t(c) * n_, s);
+        return;
+    }
+    auto t0 = clk::now();
+    const int64_t np = int64_t(nb_) * bs_;
+    const bool damped = cfg_.do_hybrid_precond && ncomp_ > 0 && cfg_.damping > kSafeDamping;
+    std::vector<int> iters(nrhs), stops(nrhs);
+    std::vector<double> res(nrhs);
+    bool retried = false;
+    for (int32_t c0 = 0; c0 < nrhs;) {
+        const int rem = nrhs - c0;
+        const int k = rem >= kMaxCols ? kMaxCols : (rem >= 2 ? 2 : 1);
+        ensure_columns(k);
+        if (np != n_) {
+            k::fill(np * k, bp_.get(), 0.0, s);
+            k::fill(np * k, xp_.get(), 0.0, s);
+        }
+        for (int c = 0; c < k; ++c) {
+            k::permute_in(nb_, bs_, n_, d_perm_.get(), B + int64_t(c0 + c) * n_, bp_.get() + c * np, s);
+            k::permute_in(nb_, bs_, n_, d_perm_.get(), X + int64_t(c0 + c) * n_, xp_.get() + c * np, s);
+        }
+        if (damped) {
+            x0p_.resize(size_t(np) * k);
+            k::copy(np * k, xp_.get(), x0p_.get(), s);
+        }
+        run_pcg(s, k);
+        std::vector<int> redo;
+        for (int c = 0; c < k; ++c) {
+            iters[c0 + c] = col_iters_[c];
+            stops[c0 + c] = col_stop_[c];
+            res[c0 + c] = col_res_[c];
+            if (damped && dev::is_breakdown(col_stop_[c])) redo.push_back(c);
+            else k::permute_out(nb_, bs_, n_, d_perm_.get(), xp_.get() + c * np, X + int64_t(c0 + c) * n_, s);
+        }
+        // a column whose sweep lost definiteness is solved again alone with the safe damping
+        for (int c : redo) {
+            retried = true;
+            k::copy(np, bp_.get() + c * np, bp_.get(), s);
+            k::copy(np, x0p_.get() + c * np, xp_.get(), s);
+            const double omega = omega_;
+            omega_ = kSafeDamping / lambda_max_;
+            run_pcg(s, 1);
+            omega_ = omega;
+            iters[c0 + c] = num_iterations_;
+            stops[c0 + c] = stop_code_;
+            res[c0 + c] = final_res_norm_;
+            k::permute_out(nb_, bs_, n_, d_perm_.get(), xp_.get(), X + int64_t(c0 + c) * n_, s);
+        }
+        c0 += k;
+    }
+    sync(s);
+    col_iters_ = iters;
+    col_stop_ = stops;
+    col_res_ = res;
+    num_iterations_ = *std::max_element(iters.begin(), iters.end());
+    final_res_norm_ = *std::max_element(res.begin(), res.end());
+    stop_code_ = stops[0];
+    for (int32_t c = 0; c < nrhs; ++c)
+        if (stops[c] != dev::kConvergedRel && stops[c] != dev::kConvergedAbs && stops[c] != dev::kZeroRhs) {
+            stop_code_ = stops[c];
+            break;
+        }
+    damping_retry_ = retried;
+    omega_used_ = omega_;
+    last_k_ = nrhs;
+    t_solve_ = since(t0);
 }
 
 // ------------------------------------------------------------------ inspection
@@ -977,8 +1097,9 @@ void HybridSolver::debug_precond(const double *r, double *z)
 
 std::string HybridSolver::info_json() const
 {
-    static const char *stops[] = {"max_iter", "converged_rel", "converged_abs", "err_zero_sdotp",
-                                  "err_negative_alpha", "err_subnormal_alpha"};
+    static const char *stops[] = {"max_iter",           "converged_rel",       "converged_abs", "err_zero_sdotp",
+                                  "err_negative_alpha", "err_subnormal_alpha", "zero_rhs"};
+    auto stop_name = [&](int c) { return stops[(c >= 0 && c < 7) ? c : 0]; };
     int64_t nst = 0;
     for (auto &c : comps_) nst += int64_t(c.size());
     std::ostringstream o;
@@ -986,7 +1107,7 @@ std::string HybridSolver::info_json() const
     o << "{\"num_iterations\":" << num_iterations_ << ",\"final_res_norm\":" << final_res_norm_
       << ",\"restarts\":" << restarts_ << ",\"damping_retry\":" << (damping_retry_ ? "true" : "false")
       << ",\"omega_used\":" << omega_used_ << ",\"solver\":\""
-      << name() << "\",\"stop_reason\":\"" << stops[(stop_code_ >= 0 && stop_code_ < 6) ? stop_code_ : 0]
+      << name() << "\",\"stop_reason\":\"" << stop_name(stop_code_)
       << "\",\"n\":" << n_ << ",\"nnz\":" << nnz_ << ",\"block_size\":" << bs_ << ",\"nnz_blocks\":" << nnzb_
       << ",\"sell_slots\":" << sell_slots_ << ",\"precision\":\"" << cfg_.precision << "\",\"inner_spmv_fp32\":"
       << (cfg_.fp32_inner() ? "true" : "false") << ",\"omega\":" << omega_ << ",\"lambda_max_BA\":" << lambda_max_ << ",\"timings\":{\"factorize\":"
@@ -1005,7 +1126,16 @@ std::string HybridSolver::info_json() const
     for (size_t l = 0; l < level_nodes_.size(); ++l) o << (l ? "," : "") << level_nodes_[l];
     o << "]},\"kernel_ms\":{\"spmv_outer\":" << prof_spmv_outer_ms_ << ",\"spmv_outer_n\":" << prof_spmv_outer_n_
       << ",\"spmv_inner\":" << prof_spmv_inner_ms_ << ",\"spmv_inner_n\":" << prof_spmv_inner_n_
-      << ",\"mas_level0\":" << prof_mas0_ms_ << ",\"mas_level0_n\":" << prof_mas0_n_ << "}}";
+      << ",\"mas_level0\":" << prof_mas0_ms_ << ",\"mas_level0_n\":" << prof_mas0_n_ << "}";
+    if (last_k_ > 1) {
+        // per right-hand side of the last multi solve
+        o << ",\"columns\":[";
+        for (size_t c = 0; c < col_iters_.size(); ++c)
+            o << (c ? "," : "") << "{\"num_iterations\":" << col_iters_[c] << ",\"final_res_norm\":" << col_res_[c]
+              << ",\"stop_reason\":\"" << stop_name(col_stop_[c]) << "\"}";
+        o << "]";
+    }
+    o << "}";
     return o.str();
 }
 

@@ -40,7 +40,10 @@ public:
                           hipStream_t s);
     void solve_host(const double *b, double *x);
     void solve_device(const double *b, double *x, hipStream_t s);
-    void solve_multi_device(int32_t k, const double *B, double *X, hipStream_t s);
+    // nrhs right-hand sides (column c at B + c * n), solved in fused groups of up
+    // to kMaxCols columns that share every matrix and bank read
+    void solve_multi_device(int32_t nrhs, const double *B, double *X, hipStream_t s);
+    static constexpr int kMaxCols = 4;
 
     std::string info_json() const;
     const char *name() const;
@@ -83,14 +86,16 @@ private:
     void build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s);
     void estimate_omega(hipStream_t s);
     void alloc_vectors();
+    void ensure_columns(int k);  // PCG vectors for k columns (stride nb * bs)
 
     // preconditioner pieces on permuted vectors
     // st != nullptr: gamma = r . z is reduced into the PCG state (first: before the loop)
+    // k columns at stride nb * bs, states st[0..k), stop = &st[0].stop
     void base_apply(const double *r, double scale, const double *add, double *z, dev::PcgState *st, const int *stop,
-                    hipStream_t s, bool first = false);
+                    hipStream_t s, bool first = false, int k = 1);
     void precond_apply(const double *r, double *z, dev::PcgState *st, const int *stop, hipStream_t s,
-                       bool first = false);
-    void run_pcg(hipStream_t s);
+                       bool first = false, int k = 1);
+    void run_pcg(hipStream_t s, int k = 1);
 
     k::Precision prec() const;
     void sync(hipStream_t s) const { LSS_HIP(hipStreamSynchronize(s)); }
@@ -144,8 +149,9 @@ private:
     // PCG
     DBuf<double> xp_, bp_, r_, p_, q_, z_, u_;
     DBuf<double> partials_;
-    DBuf<dev::PcgState> st_;
+    DBuf<dev::PcgState> st_;         // kMaxCols states
     dev::PcgState *h_st_ = nullptr;  // pinned mirror
+    int kcap_ = 1;                   // columns the PCG vectors hold
     DBuf<int> d_status_;
 
     // info
@@ -153,6 +159,9 @@ private:
     double final_res_norm_ = 0.0;
     int stop_code_ = 0;
     int restarts_ = 0;
+    int last_k_ = 1;  // right-hand sides of the last solve
+    std::vector<int> col_iters_, col_stop_;
+    std::vector<double> col_res_;
     double t_factorize_ = 0, t_select_ = 0, t_operator_ = 0, t_partition_ = 0, t_mas_ = 0, t_stiff_ = 0,
            t_omega_ = 0, t_solve_ = 0;
     int mas_levels_ = 0;

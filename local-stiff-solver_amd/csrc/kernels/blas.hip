@@ -5,6 +5,11 @@
 // one-workgroup finisher kernel that adds the per-workgroup partials in index
 // order (deterministic) and applies the scalar update of GPUHybrid.cu:1461-1507
 // on the device, so the PCG loop issues no host sync.
+//
+// Several right-hand sides: column c of a vector starts at v + c * ld, the
+// partials of column c at partials + c * count, its state at st + c.  Vector
+// kernels run one grid row (blockIdx.y) per column, finishers one workgroup
+// per column, so every column sees exactly the arithmetic of a single solve.
 #include "kernels.hpp"
 #include "../common/runtime.hpp"
 
@@ -15,13 +20,15 @@ constexpr int kT = 256;
 
 inline int red_grid(int64_t n) { return int(std::min<int64_t>(std::max<int64_t>(div_up(n, kT * 4), 1), 4096)); }
 
-__global__ __launch_bounds__(kT) void k_dot(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
-                                            double *partials)
+__global__ __launch_bounds__(kT) void k_dot(int64_t n, int64_t ld, const double *__restrict__ a,
+                                            const double *__restrict__ b, double *partials)
 {
     __shared__ double sh[kT / 64];
+    a += blockIdx.y * ld;
+    b += blockIdx.y * ld;
     double v = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT) v += a[i] * b[i];
-    dev::store_partial(dev::block_sum<kT>(v, sh), partials);
+    dev::store_partial(dev::block_sum<kT>(v, sh), partials + int64_t(blockIdx.y) * gridDim.x);
 }
 
 constexpr int kFinT = 1024;
@@ -30,6 +37,9 @@ constexpr int kFinT = 1024;
 __global__ __launch_bounds__(kFinT) void k_finish(int count, const double *__restrict__ partials, int op,
                                                   dev::PcgState *st, double *out, int iter)
 {
+    partials += int64_t(blockIdx.x) * count;
+    if (st) st += blockIdx.x;
+    if (out) out += blockIdx.x;
     if (op != dev::kFinishStore && st->stop) return;
     __shared__ double sh[kFinT / 64];
     double v = 0.0;
@@ -76,10 +86,14 @@ __global__ __launch_bounds__(kFinT) void k_finish(int count, const double *__res
     }
 }
 
-__global__ __launch_bounds__(kT) void k_update_xr(int64_t n, double *__restrict__ x, double *__restrict__ r,
-                                                  const double *__restrict__ p, const double *__restrict__ q,
-                                                  dev::PcgState *st, double *partials)
+__global__ __launch_bounds__(kT) void k_update_xr(int64_t n, int64_t ld, double *__restrict__ x,
+                                                  double *__restrict__ r, const double *__restrict__ p,
+                                                  const double *__restrict__ q, dev::PcgState *st, double *partials)
 {
+    const int64_t o = blockIdx.y * ld;
+    x += o, r += o, p += o, q += o;
+    st += blockIdx.y;
+    partials += int64_t(blockIdx.y) * gridDim.x;
     if (st->stop) return;
     __shared__ double sh[kT / 64];
     const double alpha = st->alpha;
@@ -93,9 +107,12 @@ __global__ __launch_bounds__(kT) void k_update_xr(int64_t n, double *__restrict_
     dev::store_partial(dev::block_sum<kT>(v, sh), partials);
 }
 
-__global__ __launch_bounds__(kT) void k_update_p(int64_t n, double *__restrict__ p, const double *__restrict__ z,
-                                                 const dev::PcgState *st)
+__global__ __launch_bounds__(kT) void k_update_p(int64_t n, int64_t ld, double *__restrict__ p,
+                                                 const double *__restrict__ z, const dev::PcgState *st)
 {
+    p += blockIdx.y * ld;
+    z += blockIdx.y * ld;
+    st += blockIdx.y;
     if (st->stop) return;
     const double beta = st->beta;
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT)
@@ -149,33 +166,37 @@ inline int ew_grid(int64_t n) { return int(std::min<int64_t>(std::max<int64_t>(d
 
 }  // namespace
 
-void finish(int count, const double *partials, int op, dev::PcgState *st, double *out, int iter, hipStream_t s)
+void finish(int count, const double *partials, int op, dev::PcgState *st, double *out, int iter, hipStream_t s,
+            int k)
 {
-    if (count <= 0) return;
-    k_finish<<<1, kFinT, 0, s>>>(count, partials, op, st, out, iter);
+    if (count <= 0 || k <= 0) return;
+    k_finish<<<k, kFinT, 0, s>>>(count, partials, op, st, out, iter);
     LSS_LAUNCH_CHECK();
 }
 
-void dot(int64_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s)
+int partials_per_column(int64_t n) { return red_grid(n); }
+
+void dot(int64_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s, int k,
+         int64_t ld)
 {
     const int g = red_grid(n);
-    k_dot<<<g, kT, 0, s>>>(n, a, b, partials);
+    k_dot<<<dim3(g, k), kT, 0, s>>>(n, ld, a, b, partials);
     LSS_LAUNCH_CHECK();
-    finish(g, partials, dev::kFinishStore, nullptr, out, 0, s);
+    finish(g, partials, dev::kFinishStore, nullptr, out, 0, s, k);
 }
 
 void pcg_update_xr(int64_t n, double *x, double *r, const double *p, const double *q, dev::PcgState *st,
-                   double *partials, hipStream_t s)
+                   double *partials, hipStream_t s, int k, int64_t ld)
 {
     const int g = red_grid(n);
-    k_update_xr<<<g, kT, 0, s>>>(n, x, r, p, q, st, partials);
+    k_update_xr<<<dim3(g, k), kT, 0, s>>>(n, ld, x, r, p, q, st, partials);
     LSS_LAUNCH_CHECK();
-    finish(g, partials, dev::kFinishRr, st, nullptr, 0, s);
+    finish(g, partials, dev::kFinishRr, st, nullptr, 0, s, k);
 }
 
-void pcg_update_p(int64_t n, double *p, const double *z, const dev::PcgState *st, hipStream_t s)
+void pcg_update_p(int64_t n, double *p, const double *z, const dev::PcgState *st, hipStream_t s, int k, int64_t ld)
 {
-    k_update_p<<<ew_grid(n), kT, 0, s>>>(n, p, z, st);
+    k_update_p<<<dim3(ew_grid(n), k), kT, 0, s>>>(n, ld, p, z, st);
     LSS_LAUNCH_CHECK();
 }
 

@@ -19,7 +19,11 @@ enum StopCode : int {
     kZeroSdotp = 3,
     kNegativeAlpha = 4,
     kSubnormalAlpha = 5,
+    kZeroRhs = 6,  // b = 0: x = 0 without iterating
 };
+
+// the PCG stopped on a breakdown of the recurrence (not a convergence)
+inline __host__ __device__ bool is_breakdown(int code) { return code >= kZeroSdotp && code <= kSubnormalAlpha; }
 
 // Scalars of one PCG instance; lives in device memory, mirrored to pinned host
 // memory when the host polls.
@@ -38,6 +42,20 @@ struct PcgState {
     unsigned ticket[4];
     double scratch[4];
 };
+
+// A solve over k right-hand sides keeps k PcgStates back to back; the stop word of
+// column c sits kStateInts ints after that of column c - 1.
+constexpr int kStateInts = int(sizeof(PcgState) / sizeof(int));
+static_assert(sizeof(PcgState) % sizeof(int) == 0, "PcgState must be a whole number of ints");
+
+// Work shared by all columns is skipped once every column has stopped.
+__device__ __forceinline__ bool all_stopped(const int *stop, int k)
+{
+    if (!stop) return false;
+    for (int c = 0; c < k; ++c)
+        if (!stop[c * kStateInts]) return false;
+    return true;
+}
 
 __device__ __forceinline__ double wave_sum(double v)
 {

@@ -37,13 +37,16 @@ struct SellView {
 };
 
 // y = A x (sub == nullptr) or y = sub - A x.  spmv_pq also reduces p . A p and
-// updates alpha / the breakdown flags of the PCG state.  With stop != nullptr and
-// *stop != 0 the kernels do nothing.
+// updates alpha / the breakdown flags of the PCG state.  With stop != nullptr the
+// kernels do nothing once the stop words of all k columns are set.  k = 1, 2 or 4
+// right-hand sides share one pass over the matrix (column c at v + c * ld).
 void spmv(const SellView &A, const double *x, double *y, const double *sub, hipStream_t s, bool fp32 = false);
 void spmv_pq(const SellView &A, const double *p, double *q, dev::PcgState *st, double *partials, int iter,
-             hipStream_t s);
+             hipStream_t s, int k = 1, int64_t ld = 0);
 void spmv_residual(const SellView &A, const double *x, const double *b, double *r, const int *stop, hipStream_t s,
-                   bool fp32 = false);
+                   bool fp32 = false, int k = 1, int64_t ld = 0);
+// Partials one column of spmv_pq writes.
+int spmv_partials_per_column(const SellView &A);
 
 // ---------------------------------------------------------------- BLAS-1 / PCG
 constexpr int kCsumChunks = 1024;  // canonical reduction tree (shared with the oracle)
@@ -51,14 +54,24 @@ constexpr int kCsumLanes = 256;
 // Canonical-order sums used by the stiff DOF selection.
 void canonical_sum(int64_t n, const double *v, double *partials, double *out, hipStream_t s);
 
-// Sum count partials in index order and apply the scalar update `op` (dev::FinishOp).
-void finish(int count, const double *partials, int op, dev::PcgState *st, double *out, int iter, hipStream_t s);
-void dot(int64_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s);
-// x += alpha p; r -= alpha q; rr = r.r; convergence test.
+// Several right-hand sides (k > 1): column c of a vector v is v + c * ld, its
+// partials start at partials + c * count and its PCG state is st[c].
+
+// Sum count partials in index order and apply the scalar update `op` (dev::FinishOp),
+// one workgroup per column.
+void finish(int count, const double *partials, int op, dev::PcgState *st, double *out, int iter, hipStream_t s,
+            int k = 1);
+// Partials one column of dot / pcg_update_xr writes for vectors of length n.
+int partials_per_column(int64_t n);
+// out[c] = a_c . b_c
+void dot(int64_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s, int k = 1,
+         int64_t ld = 0);
+// x += alpha p; r -= alpha q; rr = r.r; convergence test (columns that stopped are left alone).
 void pcg_update_xr(int64_t n, double *x, double *r, const double *p, const double *q, dev::PcgState *st,
-                   double *partials, hipStream_t s);
+                   double *partials, hipStream_t s, int k = 1, int64_t ld = 0);
 // p = beta p + z.
-void pcg_update_p(int64_t n, double *p, const double *z, const dev::PcgState *st, hipStream_t s);
+void pcg_update_p(int64_t n, double *p, const double *z, const dev::PcgState *st, hipStream_t s, int k = 1,
+                  int64_t ld = 0);
 void copy(int64_t n, const double *a, double *b, hipStream_t s);
 void fill(int64_t n, double *a, double v, hipStream_t s);
 void scale(int64_t n, double *a, double v, hipStream_t s);

@@ -178,14 +178,16 @@ __global__ void k_galerkin(int64_t nslots, int bb, const int64_t *__restrict__ s
     coarse_vals[g] = s;
 }
 
-// r_l[a] = sum of children (level l-1 nodes) in index order.
+// r_l[a] = sum of children (level l-1 nodes) in index order; grid row y = column.
 __global__ void k_restrict(int32_t nodes, int bs, const int64_t *__restrict__ child_ptr,
-                           const int32_t *__restrict__ child, const double *__restrict__ fine,
-                           double *__restrict__ coarse, const int *__restrict__ stop)
+                           const int32_t *__restrict__ child, const double *__restrict__ fine, int64_t ld_fine,
+                           double *__restrict__ coarse, int64_t ld_coarse, const int *__restrict__ stop)
 {
-    if (stop && *stop) return;
+    if (dev::all_stopped(stop, gridDim.y)) return;
     const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= int64_t(nodes) * bs) return;
+    fine += blockIdx.y * ld_fine;
+    coarse += blockIdx.y * ld_coarse;
     const int32_t a = int32_t(g / bs);
     const int c = int(g - int64_t(a) * bs);
     double s = 0.0;
@@ -194,57 +196,79 @@ __global__ void k_restrict(int32_t nodes, int bs, const int64_t *__restrict__ ch
 }
 
 // y = G^T (G x) with G packed lower in LDS (rows >= `rows` are identity
-// padding and skipped).  Phase 1: thread i forms t_i = G(i, 0:i) x (row i,
+// padding and skipped), for K columns x[k * D ..] at once: every LDS load of G
+// serves all columns.  Phase 1: thread i forms t_i = G(i, 0:i) x (row i,
 // contiguous); phase 2: thread j forms y_j = G(j:rows, j)^T t (column j:
 // consecutive addresses across threads).  The dot loops are unrolled by 8 with
 // all LDS loads of a step issued before the multiply-adds and four independent
-// accumulators: the bank apply is bound by LDS latency, not by its flops.
-template <int D, int NT, class TL>
+// accumulators per column: the bank apply is bound by LDS latency, not by its
+// flops.  Each column is summed in the same order for every K.
+template <int D, int NT, class TL, int K>
 __device__ __forceinline__ void bank_apply(const TL *__restrict__ G, const double *__restrict__ x,
                                            double *__restrict__ t, double *__restrict__ y, int rows)
 {
     for (int i = threadIdx.x; i < rows; i += NT) {
         const TL *g = G + packed_lower(i, 0);
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        double a[K][4];
+#pragma unroll
+        for (int k = 0; k < K; ++k) a[k][0] = a[k][1] = a[k][2] = a[k][3] = 0.0;
         int j = 0;
         for (; j + 8 <= i + 1; j += 8) {
-            double gv[8], xv[8];
+            double gv[8], xv[K][8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 gv[u] = double(g[j + u]);
-                xv[u] = x[j + u];
+#pragma unroll
+                for (int k = 0; k < K; ++k) xv[k][u] = x[k * D + j + u];
             }
-            a0 += gv[0] * xv[0] + gv[4] * xv[4];
-            a1 += gv[1] * xv[1] + gv[5] * xv[5];
-            a2 += gv[2] * xv[2] + gv[6] * xv[6];
-            a3 += gv[3] * xv[3] + gv[7] * xv[7];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                a[k][0] += gv[0] * xv[k][0] + gv[4] * xv[k][4];
+                a[k][1] += gv[1] * xv[k][1] + gv[5] * xv[k][5];
+                a[k][2] += gv[2] * xv[k][2] + gv[6] * xv[k][6];
+                a[k][3] += gv[3] * xv[k][3] + gv[7] * xv[k][7];
+            }
         }
-        for (; j <= i; ++j) a0 += double(g[j]) * x[j];
-        t[i] = (a0 + a1) + (a2 + a3);
+        for (; j <= i; ++j) {
+            const double gj = double(g[j]);
+#pragma unroll
+            for (int k = 0; k < K; ++k) a[k][0] += gj * x[k * D + j];
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) t[k * D + i] = (a[k][0] + a[k][1]) + (a[k][2] + a[k][3]);
     }
     __syncthreads();
     for (int j = threadIdx.x; j < rows; j += NT) {
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        double a[K][4];
+#pragma unroll
+        for (int k = 0; k < K; ++k) a[k][0] = a[k][1] = a[k][2] = a[k][3] = 0.0;
         int i = j;
         int idx = packed_lower(j, j);  // advances by i + 1 from row i to row i + 1
         for (; i + 8 <= rows; i += 8) {
-            double gv[8], tv[8];
+            double gv[8], tv[K][8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 gv[u] = double(G[idx]);
-                tv[u] = t[i + u];
+#pragma unroll
+                for (int k = 0; k < K; ++k) tv[k][u] = t[k * D + i + u];
                 idx += i + u + 1;
             }
-            a0 += gv[0] * tv[0] + gv[4] * tv[4];
-            a1 += gv[1] * tv[1] + gv[5] * tv[5];
-            a2 += gv[2] * tv[2] + gv[6] * tv[6];
-            a3 += gv[3] * tv[3] + gv[7] * tv[7];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                a[k][0] += gv[0] * tv[k][0] + gv[4] * tv[k][4];
+                a[k][1] += gv[1] * tv[k][1] + gv[5] * tv[k][5];
+                a[k][2] += gv[2] * tv[k][2] + gv[6] * tv[k][6];
+                a[k][3] += gv[3] * tv[k][3] + gv[7] * tv[k][7];
+            }
         }
         for (; i < rows; ++i) {
-            a0 += double(G[idx]) * t[i];
+            const double gi = double(G[idx]);
+#pragma unroll
+            for (int k = 0; k < K; ++k) a[k][0] += gi * t[k * D + i];
             idx += i + 1;
         }
-        y[j] = (a0 + a1) + (a2 + a3);
+#pragma unroll
+        for (int k = 0; k < K; ++k) y[k * D + j] = (a[k][0] + a[k][1]) + (a[k][2] + a[k][3]);
     }
 }
 
@@ -297,74 +321,94 @@ __device__ __forceinline__ void stage_matrix(const TS *__restrict__ src, TL *__r
 constexpr int kApplyThreads = 256;
 
 // Coarse level: y = A^{-1} r per bank of 32 nodes; w = y + w_parent[parent].
-template <int BS, class TS>
+// K columns: r and w at stride ld, w_parent at stride ld_parent.
+template <int BS, class TS, int K>
 __global__ __launch_bounds__(kApplyThreads) void k_coarse_apply(int32_t nodes, const TS *__restrict__ inv,
-                                                                const double *__restrict__ r,
+                                                                const double *__restrict__ r, int64_t ld,
                                                                 const int32_t *__restrict__ parent,
                                                                 const double *__restrict__ w_parent,
-                                                                double *__restrict__ w, const int *__restrict__ stop)
+                                                                int64_t ld_parent, double *__restrict__ w,
+                                                                const int *__restrict__ stop)
 {
     constexpr int D = 32 * BS;
     constexpr int S = D * (D + 1) / 2;
     using TL = typename LdsType<TS>::type;
-    if (stop && *stop) return;
+    if (dev::all_stopped(stop, K)) return;
     __shared__ __attribute__((aligned(16))) TL A[S + 4];
-    __shared__ double x[D], tv[D], yv[D];
+    __shared__ double x[K * D], tv[K * D], yv[K * D];
     const int32_t bank = blockIdx.x;
     const int32_t lo = bank * 32;
     const int32_t cnt = min(32, nodes - lo);
     stage_matrix<TS, TL, S, kApplyThreads>(inv + int64_t(bank) * S, A);
-    for (int t = threadIdx.x; t < D; t += kApplyThreads) x[t] = (t < cnt * BS) ? r[int64_t(lo) * BS + t] : 0.0;
+    for (int t = threadIdx.x; t < K * D; t += kApplyThreads) {
+        const int k = t / D, e = t - k * D;
+        x[t] = (e < cnt * BS) ? r[k * ld + int64_t(lo) * BS + e] : 0.0;
+    }
     __syncthreads();
     // rows beyond the bank's nodes are identity padding: they do not couple to real rows
-    bank_apply<D, kApplyThreads, TL>(A, x, tv, yv, cnt * BS);
+    bank_apply<D, kApplyThreads, TL, K>(A, x, tv, yv, cnt * BS);
     for (int i = threadIdx.x; i < cnt * BS; i += kApplyThreads) {
-        double y = yv[i];
-        if (w_parent) {
-            const int32_t node = lo + i / BS;
-            y += w_parent[int64_t(parent[node]) * BS + (i % BS)];
+        const int64_t pi = w_parent ? int64_t(parent[lo + i / BS]) * BS + (i % BS) : 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double y = yv[k * D + i];
+            if (w_parent) y += w_parent[k * ld_parent + pi];
+            w[k * ld + int64_t(lo) * BS + i] = y;
         }
-        w[int64_t(lo) * BS + i] = y;
     }
 }
 
 // Level 0: z = scale * (A_bank^{-1} r_bank + w1[parent]) (+ add); optional r.z reduction.
-template <int BS, class TS, bool DOT>
+// K columns: r, add, z, dotv at stride ld, w1 at stride ld1; the partials of
+// column k start at partials + k * nbanks.
+template <int BS, class TS, bool DOT, int K>
 __global__ __launch_bounds__(kApplyThreads) void k_level0_apply(int32_t nbanks, const int32_t *__restrict__ part_off,
                                                                 const TS *__restrict__ inv,
-                                                                const double *__restrict__ r,
+                                                                const double *__restrict__ r, int64_t ld,
                                                                 const int32_t *__restrict__ parent,
-                                                                const double *__restrict__ w1, double scale,
-                                                                const double *__restrict__ add, double *__restrict__ z,
-                                                                const double *__restrict__ dotv, dev::PcgState *st,
-                                                                double *partials, const int *__restrict__ stop)
+                                                                const double *__restrict__ w1, int64_t ld1,
+                                                                double scale, const double *__restrict__ add,
+                                                                double *__restrict__ z,
+                                                                const double *__restrict__ dotv, double *partials,
+                                                                const int *__restrict__ stop)
 {
     constexpr int D = 32 * BS;
     constexpr int S = D * (D + 1) / 2;
     using TL = typename LdsType<TS>::type;
-    if (stop && *stop) return;
+    if (dev::all_stopped(stop, K)) return;
     __shared__ __attribute__((aligned(16))) TL A[S + 4];
-    __shared__ double x[D], tv[D], yv[D];
+    __shared__ double x[K * D], tv[K * D], yv[K * D];
     const int32_t bank = blockIdx.x;
     const int32_t lo = part_off[bank];
     const int32_t cnt = part_off[bank + 1] - lo;
     stage_matrix<TS, TL, S, kApplyThreads>(inv + int64_t(bank) * S, A);
-    for (int t = threadIdx.x; t < D; t += kApplyThreads) x[t] = (t < cnt * BS) ? r[int64_t(lo) * BS + t] : 0.0;
+    for (int t = threadIdx.x; t < K * D; t += kApplyThreads) {
+        const int k = t / D, e = t - k * D;
+        x[t] = (e < cnt * BS) ? r[k * ld + int64_t(lo) * BS + e] : 0.0;
+    }
     __syncthreads();
-    bank_apply<D, kApplyThreads, TL>(A, x, tv, yv, cnt * BS);
-    double part = 0.0;
+    bank_apply<D, kApplyThreads, TL, K>(A, x, tv, yv, cnt * BS);
+    double part[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) part[k] = 0.0;
     for (int i = threadIdx.x; i < cnt * BS; i += kApplyThreads) {
-        double y = yv[i];
         const int64_t gi = int64_t(lo) * BS + i;
-        if (w1) y += w1[int64_t(parent[lo + i / BS]) * BS + (i % BS)];
-        y *= scale;
-        if (add) y += add[gi];
-        z[gi] = y;
-        if constexpr (DOT) part += dotv[gi] * y;
+        const int64_t pi = w1 ? int64_t(parent[lo + i / BS]) * BS + (i % BS) : 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double y = yv[k * D + i];
+            if (w1) y += w1[k * ld1 + pi];
+            y *= scale;
+            if (add) y += add[k * ld + gi];
+            z[k * ld + gi] = y;
+            if constexpr (DOT) part[k] += dotv[k * ld + gi] * y;
+        }
     }
     if constexpr (DOT) {
         __shared__ double sh[kApplyThreads / 64];
-        dev::store_partial(dev::block_sum<kApplyThreads>(part, sh), partials);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            dev::store_partial(dev::block_sum<kApplyThreads>(part[k], sh), partials + int64_t(k) * nbanks);
     }
 }
 
@@ -440,22 +484,26 @@ void mas_galerkin(int64_t nslots, int bs, const int64_t *src_ptr, const int32_t 
 }
 
 void mas_restrict(int32_t nodes, int bs, const int64_t *child_ptr, const int32_t *child, const double *fine,
-                  double *coarse, const int *stop, hipStream_t s)
+                  double *coarse, const int *stop, hipStream_t s, int k, int64_t ld_fine, int64_t ld_coarse)
 {
     const int64_t work = int64_t(nodes) * bs;
     if (work == 0) return;
-    k_restrict<<<div_up(work, 256), 256, 0, s>>>(nodes, bs, child_ptr, child, fine, coarse, stop);
+    k_restrict<<<dim3(div_up(work, 256), k), 256, 0, s>>>(nodes, bs, child_ptr, child, fine, ld_fine, coarse,
+                                                          ld_coarse, stop);
     LSS_LAUNCH_CHECK();
 }
 
-void mas_coarse_apply(Precision prec, int bs, int32_t nodes, const void *inv, const double *r, const int32_t *parent,
-                      const double *w_parent, double *w, const int *stop, hipStream_t s)
+namespace {
+
+template <int K>
+void coarse_apply_k(Precision prec, int bs, int32_t nodes, const void *inv, const double *r, int64_t ld,
+                    const int32_t *parent, const double *w_parent, int64_t ld_parent, double *w, const int *stop,
+                    hipStream_t s)
 {
     const int32_t nbanks = div_up(nodes, 32);
-    if (nbanks == 0) return;
 #define LSS_CA(BS, TS)                                                                                         \
-    k_coarse_apply<BS, TS><<<nbanks, kApplyThreads, 0, s>>>(nodes, static_cast<const TS *>(inv), r, parent,    \
-                                                            w_parent, w, stop)
+    k_coarse_apply<BS, TS, K><<<nbanks, kApplyThreads, 0, s>>>(nodes, static_cast<const TS *>(inv), r, ld,      \
+                                                               parent, w_parent, ld_parent, w, stop)
 #define LSS_CA_BS(TS)                                                                                          \
     switch (bs) {                                                                                              \
         case 1: LSS_CA(1, TS); break;                                                                          \
@@ -474,16 +522,16 @@ void mas_coarse_apply(Precision prec, int bs, int32_t nodes, const void *inv, co
     LSS_LAUNCH_CHECK();
 }
 
-void mas_level0_apply(Precision prec, int bs, int32_t nbanks, const int32_t *part_off, const void *inv,
-                      const double *r, const int32_t *parent, const double *w1, double scale, const double *add,
-                      double *z, const double *dotv, dev::PcgState *st, double *partials, const int *stop,
-                      hipStream_t s)
+template <int K>
+void level0_apply_k(Precision prec, int bs, int32_t nbanks, const int32_t *part_off, const void *inv,
+                    const double *r, int64_t ld, const int32_t *parent, const double *w1, int64_t ld1, double scale,
+                    const double *add, double *z, const double *dotv, bool dot, double *partials, const int *stop,
+                    hipStream_t s)
 {
-    if (nbanks == 0) return;
-    const bool dot = st != nullptr;
 #define LSS_L0(BS, TS, DOT)                                                                                    \
-    k_level0_apply<BS, TS, DOT><<<nbanks, kApplyThreads, 0, s>>>(nbanks, part_off, static_cast<const TS *>(inv), \
-                                                                 r, parent, w1, scale, add, z, dotv, st, partials, stop)
+    k_level0_apply<BS, TS, DOT, K><<<nbanks, kApplyThreads, 0, s>>>(nbanks, part_off,                          \
+                                                                    static_cast<const TS *>(inv), r, ld, parent, \
+                                                                    w1, ld1, scale, add, z, dotv, partials, stop)
 #define LSS_L0_BS(TS)                                                                                          \
     switch (bs) {                                                                                              \
         case 1: if (dot) LSS_L0(1, TS, true); else LSS_L0(1, TS, false); break;                                \
@@ -500,6 +548,45 @@ void mas_level0_apply(Precision prec, int bs, int32_t nbanks, const int32_t *par
 #undef LSS_L0_BS
 #undef LSS_L0
     LSS_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void mas_coarse_apply(Precision prec, int bs, int32_t nodes, const void *inv, const double *r, const int32_t *parent,
+                      const double *w_parent, double *w, const int *stop, hipStream_t s, int k, int64_t ld,
+                      int64_t ld_parent)
+{
+    if (nodes == 0) return;
+    switch (k) {
+        case 1: coarse_apply_k<1>(prec, bs, nodes, inv, r, ld, parent, w_parent, ld_parent, w, stop, s); break;
+        case 2: coarse_apply_k<2>(prec, bs, nodes, inv, r, ld, parent, w_parent, ld_parent, w, stop, s); break;
+        case 4: coarse_apply_k<4>(prec, bs, nodes, inv, r, ld, parent, w_parent, ld_parent, w, stop, s); break;
+        default: fail(LSS_ERR_ARG, "a fused MAS apply takes 1, 2 or 4 right-hand sides");
+    }
+}
+
+void mas_level0_apply(Precision prec, int bs, int32_t nbanks, const int32_t *part_off, const void *inv,
+                      const double *r, const int32_t *parent, const double *w1, double scale, const double *add,
+                      double *z, const double *dotv, dev::PcgState *st, double *partials, const int *stop,
+                      hipStream_t s, int k, int64_t ld, int64_t ld1)
+{
+    if (nbanks == 0) return;
+    const bool dot = st != nullptr;
+    switch (k) {
+        case 1:
+            level0_apply_k<1>(prec, bs, nbanks, part_off, inv, r, ld, parent, w1, ld1, scale, add, z, dotv, dot,
+                              partials, stop, s);
+            break;
+        case 2:
+            level0_apply_k<2>(prec, bs, nbanks, part_off, inv, r, ld, parent, w1, ld1, scale, add, z, dotv, dot,
+                              partials, stop, s);
+            break;
+        case 4:
+            level0_apply_k<4>(prec, bs, nbanks, part_off, inv, r, ld, parent, w1, ld1, scale, add, z, dotv, dot,
+                              partials, stop, s);
+            break;
+        default: fail(LSS_ERR_ARG, "a fused MAS apply takes 1, 2 or 4 right-hand sides");
+    }
 }
 
 }  // namespace lss::k

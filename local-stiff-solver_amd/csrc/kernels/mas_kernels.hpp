@@ -21,15 +21,20 @@ void mas_bank_invert(Precision prec, int bs, int32_t nbanks, const int32_t *bank
 // Galerkin values: coarse slot t = sum of fine blocks src_blk[src_ptr[t] .. src_ptr[t+1]).
 void mas_galerkin(int64_t nslots, int bs, const int64_t *src_ptr, const int32_t *src_blk, const double *fine_vals,
                   double *coarse_vals, hipStream_t s);
+// Apply kernels take k = 1, 2 or 4 right-hand sides: column c of a vector v is
+// v + c * ld (level vectors: ld = nodes * bs of their level); with stop != nullptr
+// they do nothing once the stop words of all k columns are set.
 void mas_restrict(int32_t nodes, int bs, const int64_t *child_ptr, const int32_t *child, const double *fine,
-                  double *coarse, const int *stop, hipStream_t s);
+                  double *coarse, const int *stop, hipStream_t s, int k = 1, int64_t ld_fine = 0,
+                  int64_t ld_coarse = 0);
 void mas_coarse_apply(Precision prec, int bs, int32_t nodes, const void *inv, const double *r, const int32_t *parent,
-                      const double *w_parent, double *w, const int *stop, hipStream_t s);
-// st != nullptr: per-bank partials of dotv . z go to partials[bank] (reduced by k::finish).
+                      const double *w_parent, double *w, const int *stop, hipStream_t s, int k = 1, int64_t ld = 0,
+                      int64_t ld_parent = 0);
+// st != nullptr: per-bank partials of dotv . z go to partials[c * nbanks + bank] (reduced by k::finish).
 void mas_level0_apply(Precision prec, int bs, int32_t nbanks, const int32_t *part_off, const void *inv,
                       const double *r, const int32_t *parent, const double *w1, double scale, const double *add,
                       double *z, const double *dotv, dev::PcgState *st, double *partials, const int *stop,
-                      hipStream_t s);
+                      hipStream_t s, int k = 1, int64_t ld = 0, int64_t ld1 = 0);
 
 // ---------------------------------------------------------------- stiff blocks
 // Dense S_c A S_c^T per component from operator rows (original numbering).
@@ -44,15 +49,17 @@ void stiff_inverse_factor(int32_t ncomp, int32_t max_m, const int64_t *comp_off,
                           const double *L, double *G, hipStream_t s);
 // z[p] += (S_c A S_c^T)^{-1} (r - A z)[p] = G^T G (r - A z)[p] over the stiff rows p of
 // each component, (A z) taken from the block operator in the permuted numbering.
+// k right-hand sides: r, z at stride ld; work, work2 hold k * ns values each.
 void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, const int64_t *mat_off,
                    const int32_t *row_comp, const int32_t *pdofs, const double *G, const int32_t *rowptr,
                    const int32_t *cols,
                    const double *bvals, const double *r, double *z, double *work, double *work2, const int *stop,
-                   hipStream_t s);
+                   hipStream_t s, int k = 1, int64_t ld = 0);
 
 // u[p] = r[p] - (A z)[p] in FP64 for the stiff rows p: after a preconditioner-internal
 // SpMV with FP32 values, the rows that hold the stiff entries are recomputed exactly.
 void stiff_rows_exact(int64_t ns, int bs, const int32_t *pdofs, const int32_t *rowptr, const int32_t *cols,
-                      const double *bvals, const double *r, const double *z, double *u, const int *stop, hipStream_t s);
+                      const double *bvals, const double *r, const double *z, double *u, const int *stop, hipStream_t s,
+                      int k = 1, int64_t ld = 0);
 
 }  // namespace lss::k

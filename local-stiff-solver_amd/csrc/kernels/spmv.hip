@@ -249,23 +249,27 @@ __global__ void k_sell_fill(int32_t nb, int bs, int H, const int32_t *__restrict
 
 // One wave per slice.  MODE 0: y = A x; MODE 1: y = sub - A x.
 // DOT: per-lane partial of x_i * y_i (p . A p) reduced deterministically.
-template <int BS, int MODE, bool DOT, bool F32>
+// K right-hand sides (column c of a vector at v + c * ld) share every column
+// index and matrix value load; each column accumulates in the order of K = 1.
+template <int BS, int MODE, bool DOT, bool F32, int K>
 __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslices, const int64_t *__restrict__ slice_ptr,
                                                         const int32_t *__restrict__ cols, const double *__restrict__ vals,
                                                         const float *__restrict__ vals32, const double *__restrict__ x,
                                                         double *__restrict__ y, const double *__restrict__ sub,
-                                                        const int *__restrict__ stop, dev::PcgState *st,
-                                                        double *__restrict__ partials, int iter)
+                                                        const int *__restrict__ stop, int64_t ld,
+                                                        double *__restrict__ partials)
 {
     constexpr int H = 64 / BS;
     constexpr int BB = BS * BS;
-    if (stop && *stop) return;
+    if (dev::all_stopped(stop, K)) return;
     const int lane = threadIdx.x & 63;
     const int32_t slice = blockIdx.x * (kSpmvThreads / 64) + (threadIdx.x >> 6);
     const int h = lane / BS, r = lane - h * BS;
     const int32_t row = slice * H + h;
     const bool active = (h < H) && (slice < nslices) && (row < nb);
-    double acc = 0.0;
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
     if (slice < nslices) {
         const int64_t base = slice_ptr[slice];
         const int32_t W = int32_t((slice_ptr[slice + 1] - base) / H);
@@ -275,13 +279,13 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslic
         // loads back to back.  The matrix is streamed with non-temporal loads.
         const int hh = h < H ? h : H - 1;
         const int ll = hh * BS + r;  // lane within the slot's value layout
-        constexpr int C = 4;
+        constexpr int C = K >= 4 ? 2 : 4;
         int32_t j0 = 0;
         for (; j0 + C <= W; j0 += C) {
             int32_t c[C];
 #pragma unroll
             for (int u = 0; u < C; ++u) c[u] = __builtin_nontemporal_load(&cols[base + int64_t(j0 + u) * H + hh]);
-            double a[C][BS], xv[C][BS];
+            double a[C][BS], xv[C][K][BS];
 #pragma unroll
             for (int u = 0; u < C; ++u) {
                 const int64_t vb = (base + int64_t(j0 + u) * H) * BB + ll;
@@ -292,13 +296,16 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslic
                         a[u][q] = double(__builtin_nontemporal_load(&vals32[vb + q * (H * BS)]));
                     else
                         a[u][q] = __builtin_nontemporal_load(&vals[vb + q * (H * BS)]);
-                    xv[u][q] = x[xc + q];
+#pragma unroll
+                    for (int k = 0; k < K; ++k) xv[u][k][q] = x[k * ld + xc + q];
                 }
             }
 #pragma unroll
             for (int u = 0; u < C; ++u)
 #pragma unroll
-                for (int q = 0; q < BS; ++q) acc += a[u][q] * xv[u][q];
+                for (int q = 0; q < BS; ++q)
+#pragma unroll
+                    for (int k = 0; k < K; ++k) acc[k] += a[u][q] * xv[u][k][q];
         }
         for (; j0 < W; ++j0) {
             const int64_t slot = base + int64_t(j0) * H + hh;
@@ -309,32 +316,39 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslic
             for (int q = 0; q < BS; ++q) {
                 const double a = F32 ? double(__builtin_nontemporal_load(&vals32[vb + q * (H * BS)]))
                                      : __builtin_nontemporal_load(&vals[vb + q * (H * BS)]);
-                acc += a * x[xc + q];
+#pragma unroll
+                for (int k = 0; k < K; ++k) acc[k] += a * x[k * ld + xc + q];
             }
         }
     }
-    double out = 0.0;
-    if (active) {
-        const int64_t i = int64_t(row) * BS + r;
-        out = (MODE == 1) ? sub[i] - acc : acc;
-        y[i] = out;
+    const int64_t i = int64_t(row) * BS + r;
+    double out[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        out[k] = 0.0;
+        if (active) {
+            out[k] = (MODE == 1) ? sub[k * ld + i] - acc[k] : acc[k];
+            y[k * ld + i] = out[k];
+        }
     }
     if constexpr (DOT) {
         __shared__ double sh[kSpmvThreads / 64];
-        dev::store_partial(dev::block_sum<kSpmvThreads>(active ? x[int64_t(row) * BS + r] * out : 0.0, sh),
-                           partials);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            dev::store_partial(dev::block_sum<kSpmvThreads>(active ? x[k * ld + i] * out[k] : 0.0, sh),
+                               partials + int64_t(k) * gridDim.x);
     }
 }
 
-template <int MODE, bool DOT, bool F32>
-void launch_spmv(const SellView &A, const double *x, double *y, const double *sub, const int *stop,
-                 dev::PcgState *st, double *partials, int iter, hipStream_t s)
+template <int MODE, bool DOT, bool F32, int K>
+void launch_spmv_k(const SellView &A, const double *x, double *y, const double *sub, const int *stop, int64_t ld,
+                   double *partials, hipStream_t s)
 {
     const int grid = div_up(A.nslices, kSpmvThreads / 64);
     if (grid == 0) return;
 #define LSS_SPMV_CASE(BS)                                                                                      \
-    k_spmv<BS, MODE, DOT, F32><<<grid, kSpmvThreads, 0, s>>>(A.nb, A.nslices, A.slice_ptr, A.cols, A.vals,     \
-                                                             A.vals32, x, y, sub, stop, st, partials, iter)
+    k_spmv<BS, MODE, DOT, F32, K><<<grid, kSpmvThreads, 0, s>>>(A.nb, A.nslices, A.slice_ptr, A.cols, A.vals,  \
+                                                                A.vals32, x, y, sub, stop, ld, partials)
     switch (A.bs) {
         case 1: LSS_SPMV_CASE(1); break;
         case 2: LSS_SPMV_CASE(2); break;
@@ -343,6 +357,18 @@ void launch_spmv(const SellView &A, const double *x, double *y, const double *su
     }
 #undef LSS_SPMV_CASE
     LSS_LAUNCH_CHECK();
+}
+
+template <int MODE, bool DOT, bool F32>
+void launch_spmv(const SellView &A, const double *x, double *y, const double *sub, const int *stop, int k,
+                 int64_t ld, double *partials, hipStream_t s)
+{
+    switch (k) {
+        case 1: launch_spmv_k<MODE, DOT, F32, 1>(A, x, y, sub, stop, ld, partials, s); break;
+        case 2: launch_spmv_k<MODE, DOT, F32, 2>(A, x, y, sub, stop, ld, partials, s); break;
+        case 4: launch_spmv_k<MODE, DOT, F32, 4>(A, x, y, sub, stop, ld, partials, s); break;
+        default: fail(LSS_ERR_ARG, "a fused SpMV takes 1, 2 or 4 right-hand sides");
+    }
 }
 
 }  // namespace
@@ -394,26 +420,28 @@ void sell_fill(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols, c
 void spmv(const SellView &A, const double *x, double *y, const double *sub, hipStream_t s, bool fp32)
 {
     if (fp32) {
-        if (sub) launch_spmv<1, false, true>(A, x, y, sub, nullptr, nullptr, nullptr, 0, s);
-        else launch_spmv<0, false, true>(A, x, y, nullptr, nullptr, nullptr, nullptr, 0, s);
+        if (sub) launch_spmv<1, false, true>(A, x, y, sub, nullptr, 1, 0, nullptr, s);
+        else launch_spmv<0, false, true>(A, x, y, nullptr, nullptr, 1, 0, nullptr, s);
     } else {
-        if (sub) launch_spmv<1, false, false>(A, x, y, sub, nullptr, nullptr, nullptr, 0, s);
-        else launch_spmv<0, false, false>(A, x, y, nullptr, nullptr, nullptr, nullptr, 0, s);
+        if (sub) launch_spmv<1, false, false>(A, x, y, sub, nullptr, 1, 0, nullptr, s);
+        else launch_spmv<0, false, false>(A, x, y, nullptr, nullptr, 1, 0, nullptr, s);
     }
 }
 
+int spmv_partials_per_column(const SellView &A) { return div_up(A.nslices, kSpmvThreads / 64); }
+
 void spmv_pq(const SellView &A, const double *p, double *q, dev::PcgState *st, double *partials, int iter,
-             hipStream_t s)
+             hipStream_t s, int k, int64_t ld)
 {
-    launch_spmv<0, true, false>(A, p, q, nullptr, &st->stop, st, partials, iter, s);
-    finish(div_up(A.nslices, kSpmvThreads / 64), partials, dev::kFinishPq, st, nullptr, iter, s);
+    launch_spmv<0, true, false>(A, p, q, nullptr, &st->stop, k, ld, partials, s);
+    finish(spmv_partials_per_column(A), partials, dev::kFinishPq, st, nullptr, iter, s, k);
 }
 
 void spmv_residual(const SellView &A, const double *x, const double *b, double *r, const int *stop, hipStream_t s,
-                   bool fp32)
+                   bool fp32, int k, int64_t ld)
 {
-    if (fp32) launch_spmv<1, false, true>(A, x, r, b, stop, nullptr, nullptr, 0, s);
-    else launch_spmv<1, false, false>(A, x, r, b, stop, nullptr, nullptr, 0, s);
+    if (fp32) launch_spmv<1, false, true>(A, x, r, b, stop, k, ld, nullptr, s);
+    else launch_spmv<1, false, false>(A, x, r, b, stop, k, ld, nullptr, s);
 }
 
 }  // namespace lss::k

@@ -118,14 +118,20 @@ __global__ __launch_bounds__(kT) void k_cholesky(const int64_t *__restrict__ com
     if (threadIdx.x == 0 && bad) atomicOr(status, 2);
 }
 
+// Several right-hand sides: grid row y is column y (vectors at stride ld, stiff
+// work vectors at stride ns).
 __global__ void k_stiff_residual(int64_t ns, int bs, const int32_t *__restrict__ pdofs,
                                  const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
                                  const double *__restrict__ bvals, const double *__restrict__ r,
-                                 const double *__restrict__ z, double *__restrict__ work, const int *__restrict__ stop)
+                                 const double *__restrict__ z, int64_t ld, double *__restrict__ work,
+                                 const int *__restrict__ stop)
 {
-    if (stop && *stop) return;
+    if (dev::all_stopped(stop, gridDim.y)) return;
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= ns) return;
+    r += blockIdx.y * ld;
+    z += blockIdx.y * ld;
+    work += blockIdx.y * ns;
     const int32_t p = pdofs[k];
     const int32_t I = p / bs;
     const int a = p - I * bs;
@@ -142,11 +148,15 @@ __global__ void k_stiff_residual(int64_t ns, int bs, const int32_t *__restrict__
 __global__ void k_stiff_rows_exact(int64_t ns, int bs, const int32_t *__restrict__ pdofs,
                                    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
                                    const double *__restrict__ bvals, const double *__restrict__ r,
-                                   const double *__restrict__ z, double *__restrict__ u, const int *__restrict__ stop)
+                                   const double *__restrict__ z, double *__restrict__ u, int64_t ld,
+                                   const int *__restrict__ stop)
 {
-    if (stop && *stop) return;
+    if (dev::all_stopped(stop, gridDim.y)) return;
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= ns) return;
+    r += blockIdx.y * ld;
+    z += blockIdx.y * ld;
+    u += blockIdx.y * ld;
     const int32_t p = pdofs[k];
     const int32_t I = p / bs;
     const int a = p - I * bs;
@@ -191,13 +201,16 @@ __global__ __launch_bounds__(256) void k_stiff_rows(int64_t ns, const int32_t *_
                                                     const int64_t *__restrict__ comp_off,
                                                     const int64_t *__restrict__ mat_off, const int32_t *__restrict__ pdofs,
                                                     const double *__restrict__ M, const double *__restrict__ in,
-                                                    double *__restrict__ out, double *__restrict__ z,
+                                                    double *__restrict__ out, double *__restrict__ z, int64_t ld,
                                                     const int *__restrict__ stop)
 {
-    if (stop && *stop) return;
+    if (dev::all_stopped(stop, gridDim.y)) return;
     const int64_t k = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (k >= ns) return;
+    in += blockIdx.y * ns;
+    if (out) out += blockIdx.y * ns;
+    z += blockIdx.y * ld;
     const int32_t c = row_comp[k];
     const int64_t off = comp_off[c];
     const int32_t m = int32_t(comp_off[c + 1] - off);
@@ -248,22 +261,27 @@ void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, c
                    const int32_t *row_comp, const int32_t *pdofs, const double *G, const int32_t *rowptr,
                    const int32_t *cols,
                    const double *bvals, const double *r, double *z, double *work, double *work2, const int *stop,
-                   hipStream_t s)
+                   hipStream_t s, int k, int64_t ld)
 {
     if (ncomp == 0 || ns == 0) return;
-    k_stiff_residual<<<div_up(ns, 256), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r, z, work, stop);
+    k_stiff_residual<<<dim3(div_up(ns, 256), k), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r, z, ld, work,
+                                                             stop);
     LSS_LAUNCH_CHECK();
-    k_stiff_rows<false><<<div_up(ns, 4), 256, 0, s>>>(ns, row_comp, comp_off, mat_off, pdofs, G, work, work2, z, stop);
+    k_stiff_rows<false><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, row_comp, comp_off, mat_off, pdofs, G, work, work2,
+                                                              z, ld, stop);
     LSS_LAUNCH_CHECK();
-    k_stiff_rows<true><<<div_up(ns, 4), 256, 0, s>>>(ns, row_comp, comp_off, mat_off, pdofs, G, work2, nullptr, z, stop);
+    k_stiff_rows<true><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, row_comp, comp_off, mat_off, pdofs, G, work2,
+                                                             nullptr, z, ld, stop);
     LSS_LAUNCH_CHECK();
 }
 
 void stiff_rows_exact(int64_t ns, int bs, const int32_t *pdofs, const int32_t *rowptr, const int32_t *cols,
-                      const double *bvals, const double *r, const double *z, double *u, const int *stop, hipStream_t s)
+                      const double *bvals, const double *r, const double *z, double *u, const int *stop, hipStream_t s,
+                      int k, int64_t ld)
 {
     if (ns == 0) return;
-    k_stiff_rows_exact<<<div_up(ns, 256), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r, z, u, stop);
+    k_stiff_rows_exact<<<dim3(div_up(ns, 256), k), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r, z, u, ld,
+                                                               stop);
     LSS_LAUNCH_CHECK();
 }
 

@@ -60,13 +60,19 @@ __global__ void k_diag_inverse(int32_t nb, int bs, const int32_t *__restrict__ r
 
 constexpr int kT = 256;
 
+// grid row y = column y (vectors at stride ld, partials at stride gridDim.x)
 template <bool DOT>
 __global__ __launch_bounds__(kT) void k_jacobi(int64_t n, const double *__restrict__ dinv, const double *__restrict__ r,
                                                double scale, const double *__restrict__ add, double *__restrict__ z,
-                                               const double *__restrict__ dotv, dev::PcgState *st, double *partials,
+                                               const double *__restrict__ dotv, int64_t ld, double *partials,
                                                const int *__restrict__ stop)
 {
-    if (stop && *stop) return;
+    if (dev::all_stopped(stop, gridDim.y)) return;
+    const int64_t o = blockIdx.y * ld;
+    r += o, z += o;
+    if (add) add += o;
+    if (dotv) dotv += o;
+    partials += int64_t(blockIdx.y) * gridDim.x;
     double v = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT) {
         double y = scale * (dinv[i] * r[i]);
@@ -119,13 +125,14 @@ void diag_inverse(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols
 }
 
 int jacobi_apply(int64_t n, const double *dinv, const double *r, double scale, const double *add, double *z,
-                 const double *dotv, dev::PcgState *st, double *partials, const int *stop, hipStream_t s)
+                 const double *dotv, dev::PcgState *st, double *partials, const int *stop, hipStream_t s, int k,
+                 int64_t ld)
 {
     const int grid = int(std::min<int64_t>(std::max<int64_t>(div_up(n, kT * 4), 1), 4096));
     if (st)
-        k_jacobi<true><<<grid, kT, 0, s>>>(n, dinv, r, scale, add, z, dotv, st, partials, stop);
+        k_jacobi<true><<<dim3(grid, k), kT, 0, s>>>(n, dinv, r, scale, add, z, dotv, ld, partials, stop);
     else
-        k_jacobi<false><<<grid, kT, 0, s>>>(n, dinv, r, scale, add, z, dotv, st, partials, stop);
+        k_jacobi<false><<<dim3(grid, k), kT, 0, s>>>(n, dinv, r, scale, add, z, dotv, ld, partials, stop);
     LSS_LAUNCH_CHECK();
     return grid;
 }

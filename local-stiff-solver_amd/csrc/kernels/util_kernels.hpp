@@ -19,8 +19,9 @@ void hash_vector(int64_t n, double *out, hipStream_t s);
 void diag_inverse(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols, const double *vals, double *dinv,
                   hipStream_t s);
 // z = scale * D^{-1} r (+ add); with st: per-workgroup partials of dotv . z.
-// Returns the number of partials written.
+// k right-hand sides at stride ld.  Returns the number of partials per column.
 int jacobi_apply(int64_t n, const double *dinv, const double *r, double scale, const double *add, double *z,
-                  const double *dotv, dev::PcgState *st, double *partials, const int *stop, hipStream_t s);
+                 const double *dotv, dev::PcgState *st, double *partials, const int *stop, hipStream_t s, int k = 1,
+                 int64_t ld = 0);
 
 }  // namespace lss::k
