@@ -475,8 +475,6 @@ bool HybridSolver::build_levels_device(DBuf<int32_t> &&part_off, int32_t nparts,
         nxt.vals.resize(std::max<int64_t>(nslots * bb, 1));
         const double *fine_vals = l == 0 ? d_bvals_.get() : mas_[l].vals.get();
         k::mas_galerkin(nslots, bs_, sptr.get(), sblk.get(), fine_vals, nxt.vals.get(), s);
-        nxt.child_ptr = std::move(child_ptr);
-        nxt.child = std::move(child);
         nxt.r.resize(size_t(coarse_n) * bs_ * kMaxCols);
         nxt.w.resize(size_t(coarse_n) * bs_ * kMaxCols);
         nxt.nbanks = k::uniform_banks(coarse_n, nxt.bank_off, s);
@@ -512,8 +510,6 @@ void HybridSolver::build_levels_host(const Pattern &gp, hipStream_t s)
             sblk.upload(h.src_blk.data(), std::max<size_t>(h.src_blk.size(), 1), s);
             const double *fine_vals = (l == 1) ? d_bvals_.get() : mas_[l - 1].vals.get();
             k::mas_galerkin(nslots, bs_, sptr.get(), sblk.get(), fine_vals, d.vals.get(), s);
-            d.child_ptr.upload(h.child_ptr.data(), h.child_ptr.size(), s);
-            d.child.upload(h.child.data(), std::max<size_t>(h.child.size(), 1), s);
             d.r.resize(size_t(h.nodes) * bs_ * kMaxCols);
             d.w.resize(size_t(h.nodes) * bs_ * kMaxCols);
             sync(s);  // sptr / sblk are released at scope end
@@ -725,8 +721,8 @@ void HybridSolver::base_apply(const double *r, double scale, const double *add, 
     auto ld = [&](int l) { return l == 0 ? np : int64_t(mas_[l].nodes) * bs_; };
     const double *src = r;
     for (int l = 1; l <= L; ++l) {
-        k::mas_restrict(mas_[l].nodes, bs_, mas_[l].child_ptr.get(), mas_[l].child.get(), src, mas_[l].r.get(), stop,
-                        s, k, ld(l - 1), ld(l));
+        k::mas_restrict(bs_, mas_[l - 1].nbanks, mas_[l - 1].bank_off.get(), mas_[l - 1].parent.get(), src,
+                        mas_[l].r.get(), stop, s, k, ld(l - 1), ld(l));
         src = mas_[l].r.get();
     }
     for (int l = L; l >= 1; --l) {

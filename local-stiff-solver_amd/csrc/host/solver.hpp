@@ -67,9 +67,7 @@ private:
         DBuf<uint8_t> inv;         // packed inverses, precision-dependent element type
         DBuf<int32_t> rowptr, cols;  // level operator (levels >= 1)
         DBuf<double> vals;
-        DBuf<int64_t> child_ptr;
-        DBuf<int32_t> child;
-        DBuf<int32_t> parent;      // to the next level
+        DBuf<int32_t> parent;      // to the next level (restriction: children of a coarse node lie in one bank)
         DBuf<double> r, w;         // restricted residual / prolongated correction
     };
 

@@ -22,7 +22,10 @@
 //  * the apply runs restriction level by level, the coarse banks from the
 //    coarsest level down (each adding its parent's prolongated correction), and
 //    finally the level-0 banks, whose epilogue adds the coarse correction,
-//    scales, optionally adds a vector and reduces r.z.
+//    scales, optionally adds a vector and reduces r.z;
+//  * a bank apply splits both triangular products into 32 x 32 tiles so that
+//    every lane of three waves runs 32 steps; 1, 2 or 4 right-hand sides share
+//    each staged factor.
 #include "mas_kernels.hpp"
 #include "../common/runtime.hpp"
 
@@ -178,98 +181,217 @@ __global__ void k_galerkin(int64_t nslots, int bb, const int64_t *__restrict__ s
     coarse_vals[g] = s;
 }
 
-// r_l[a] = sum of children (level l-1 nodes) in index order; grid row y = column.
-__global__ void k_restrict(int32_t nodes, int bs, const int64_t *__restrict__ child_ptr,
-                           const int32_t *__restrict__ child, const double *__restrict__ fine, int64_t ld_fine,
-                           double *__restrict__ coarse, int64_t ld_coarse, const int *__restrict__ stop)
+// Restriction r_l = R r_{l-1}: a coarse node sums its children, which are the
+// nodes of one component of one fine bank (at most 32 nodes).  32 lanes per
+// fine bank (lane j = node lo + j).  A bank whose nodes share one parent (the
+// usual case at level 1) is summed by a 32-lane butterfly; otherwise the bank's
+// parents and values are staged in LDS and the first child of each parent sums
+// that parent's children in index order.  Both orders are fixed.
+template <int BS, int K>
+__global__ __launch_bounds__(256) void k_restrict(int32_t nbanks, const int32_t *__restrict__ bank_off,
+                                                  const int32_t *__restrict__ parent, const double *__restrict__ fine,
+                                                  int64_t ld_fine, double *__restrict__ coarse, int64_t ld_coarse,
+                                                  const int *__restrict__ stop)
 {
-    if (dev::all_stopped(stop, gridDim.y)) return;
-    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g >= int64_t(nodes) * bs) return;
-    fine += blockIdx.y * ld_fine;
-    coarse += blockIdx.y * ld_coarse;
-    const int32_t a = int32_t(g / bs);
-    const int c = int(g - int64_t(a) * bs);
-    double s = 0.0;
-    for (int64_t q = child_ptr[a]; q < child_ptr[a + 1]; ++q) s += fine[int64_t(child[q]) * bs + c];
-    coarse[g] = s;
+    if (dev::all_stopped(stop, K)) return;
+    __shared__ int32_t spar[8][32];
+    __shared__ double sval[8][32 * BS * K];
+    const int g = threadIdx.x >> 5, j = threadIdx.x & 31;  // bank group, lane in the group
+    const int32_t bank = int32_t(blockIdx.x) * 8 + g;
+    const int32_t lo = bank < nbanks ? bank_off[bank] : 0;
+    const bool own = bank < nbanks && j < bank_off[bank + 1] - lo;
+    const int32_t node = lo + j;
+    const int32_t p = own ? parent[node] : -1 - j;  // never matches a real parent
+    double v[K][BS];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int c = 0; c < BS; ++c) v[k][c] = own ? fine[k * ld_fine + int64_t(node) * BS + c] : 0.0;
+    const int32_t p0 = __shfl(p, 0, 32);
+    const unsigned long long same = __ballot(!own || p == p0);
+    const bool single = ((same >> (threadIdx.x & 32)) & 0xffffffffull) == 0xffffffffull;  // per half-wave
+    if (single) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                double a = v[k][c];
+#pragma unroll
+                for (int off = 16; off > 0; off >>= 1) a += __shfl_xor(a, off, 32);
+                v[k][c] = a;
+            }
+        if (j == 0 && bank < nbanks)
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int c = 0; c < BS; ++c) coarse[k * ld_coarse + int64_t(p0) * BS + c] = v[k][c];
+        return;
+    }
+    // several parents: both halves of a wave may come here; LDS is per half-wave
+    spar[g][j] = p;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int c = 0; c < BS; ++c) sval[g][(j * BS + c) * K + k] = v[k][c];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (!own) return;
+    bool first = true;
+    for (int i = 0; i < 32; ++i) first &= !(i < j && spar[g][i] == p);
+    if (!first) return;
+    double acc[K][BS];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int c = 0; c < BS; ++c) acc[k][c] = 0.0;
+    for (int i = j; i < 32; ++i) {
+        if (spar[g][i] != p) continue;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int c = 0; c < BS; ++c) acc[k][c] += sval[g][(i * BS + c) * K + k];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int c = 0; c < BS; ++c) coarse[k * ld_coarse + int64_t(p) * BS + c] = acc[k][c];
 }
 
-// y = G^T (G x) with G packed lower in LDS (rows >= `rows` are identity
-// padding and skipped), for K columns x[k * D ..] at once: every LDS load of G
-// serves all columns.  Phase 1: thread i forms t_i = G(i, 0:i) x (row i,
-// contiguous); phase 2: thread j forms y_j = G(j:rows, j)^T t (column j:
-// consecutive addresses across threads).  The dot loops are unrolled by 8 with
-// all LDS loads of a step issued before the multiply-adds and four independent
-// accumulators per column: the bank apply is bound by LDS latency, not by its
-// flops.  Each column is summed in the same order for every K.
-template <int D, int NT, class TL, int K>
-__device__ __forceinline__ void bank_apply(const TL *__restrict__ G, const double *__restrict__ x,
-                                           double *__restrict__ t, double *__restrict__ y, int rows)
+// y = G^T (G x) per bank, G packed lower in LDS, for K columns.
+//
+// The matrix is cut into 32 x 32 tiles.  Phase 1 (t = G x): a half-wave task
+// (b, s) multiplies rows 32b..32b+31 of G by entries 32s..32s+31 of x (s <= b,
+// the diagonal tile masked to its lower triangle); phase 2 (y = G^T t): task
+// (b, s) multiplies columns 32b..32b+31 by entries 32s..32s+31 of t (s >= b).
+// Each lane owns one row (phase 1) or column (phase 2) of its tile and walks
+// the 32 entries of the segment, so every lane runs the same 32 steps: for
+// bs = 3 the six tasks of a phase fill three waves (96 wave-steps per phase,
+// against 160 for one thread per row, whose longest row sets the pace).  The
+// 32 lanes of a half-wave read the same vector entry (an LDS broadcast); the
+// vectors are stored with one pad entry per 32 (entry e at e + e / 32) so the
+// two halves of a wave read different LDS banks.  Each LDS load of G serves
+// all K columns.  Partial sums of a row (column) over its tiles are added in
+// tile order.
+struct ApplyTask {
+    int b, s;
+};
+
+template <int BS>
+__device__ __forceinline__ ApplyTask apply_task(int phase, int w, int h)
 {
-    for (int i = threadIdx.x; i < rows; i += NT) {
-        const TL *g = G + packed_lower(i, 0);
-        double a[K][4];
+    if constexpr (BS == 1) {
+        return h == 0 ? ApplyTask{0, 0} : ApplyTask{-1, -1};
+    } else if constexpr (BS == 2) {
+        if (phase == 1) return w == 0 ? (h == 0 ? ApplyTask{0, 0} : ApplyTask{1, 0}) : (h == 0 ? ApplyTask{1, 1} : ApplyTask{-1, -1});
+        return w == 0 ? (h == 0 ? ApplyTask{0, 1} : ApplyTask{1, 1}) : (h == 0 ? ApplyTask{0, 0} : ApplyTask{-1, -1});
+    } else {
+        if (phase == 1) {
+            if (w == 0) return h == 0 ? ApplyTask{0, 0} : ApplyTask{1, 0};
+            if (w == 1) return h == 0 ? ApplyTask{1, 1} : ApplyTask{2, 1};
+            return h == 0 ? ApplyTask{2, 0} : ApplyTask{2, 2};
+        }
+        if (w == 0) return h == 0 ? ApplyTask{0, 1} : ApplyTask{1, 1};
+        if (w == 1) return h == 0 ? ApplyTask{0, 2} : ApplyTask{1, 2};
+        return h == 0 ? ApplyTask{0, 0} : ApplyTask{2, 2};
+    }
+}
+
+// padded position of vector entry e (one pad entry per 32)
+__device__ __forceinline__ int vpad(int e) { return e + (e >> 5); }
+
+// xs: K padded bank vectors of stride XS = D + BS (zero past the bank's rows).
+// ts, P, Q: LDS scratch (K * XS, K * BS * D, K * BS * D doubles).  After the
+// final barrier, y_k(j) = sum over s = j / 32 .. BS - 1 of Q[(k * BS + s) * D + j].
+template <int BS, class TL, int K>
+__device__ __forceinline__ void bank_solve(const TL *__restrict__ G, const double *__restrict__ xs,
+                                           double *__restrict__ ts, double *__restrict__ P, double *__restrict__ Q)
+{
+    constexpr int D = 32 * BS;
+    constexpr int XS = D + BS;
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, h = lane >> 5, l = lane & 31;
+    if (w < BS) {  // phase 1: P(s, row) = G(row, 32s..32s+31) x(32s..32s+31)
+        const ApplyTask tB0 = apply_task<BS>(1, w, 1);
+        const bool hasB = tB0.b >= 0;
+        const ApplyTask t = (h && hasB) ? tB0 : apply_task<BS>(1, w, 0);  // a missing half repeats half A
+        const int row = t.b * 32 + l;
+        const int rbase = packed_lower(row, 0);
+        const int xbase = vpad(t.s * 32);
+        double a0[K], a1[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) a[k][0] = a[k][1] = a[k][2] = a[k][3] = 0.0;
-        int j = 0;
-        for (; j + 8 <= i + 1; j += 8) {
-            double gv[8], xv[K][8];
+        for (int k = 0; k < K; ++k) a0[k] = a1[k] = 0.0;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                gv[u] = double(g[j + u]);
+        for (int u0 = 0; u0 < 32; u0 += 8) {
+            double g[8], xv[K][8];
 #pragma unroll
-                for (int k = 0; k < K; ++k) xv[k][u] = x[k * D + j + u];
+            for (int v = 0; v < 8; ++v) {
+                const int j = t.s * 32 + u0 + v;  // in bounds for every lane; masked above the diagonal
+                const double gv = double(G[rbase + j]);
+                g[v] = j <= row ? gv : 0.0;
+#pragma unroll
+                for (int k = 0; k < K; ++k) xv[k][v] = xs[k * XS + xbase + u0 + v];
             }
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                a[k][0] += gv[0] * xv[k][0] + gv[4] * xv[k][4];
-                a[k][1] += gv[1] * xv[k][1] + gv[5] * xv[k][5];
-                a[k][2] += gv[2] * xv[k][2] + gv[6] * xv[k][6];
-                a[k][3] += gv[3] * xv[k][3] + gv[7] * xv[k][7];
+                a0[k] += g[0] * xv[k][0] + g[2] * xv[k][2] + g[4] * xv[k][4] + g[6] * xv[k][6];
+                a1[k] += g[1] * xv[k][1] + g[3] * xv[k][3] + g[5] * xv[k][5] + g[7] * xv[k][7];
             }
         }
-        for (; j <= i; ++j) {
-            const double gj = double(g[j]);
+        if (h == 0 || hasB)
 #pragma unroll
-            for (int k = 0; k < K; ++k) a[k][0] += gj * x[k * D + j];
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) t[k * D + i] = (a[k][0] + a[k][1]) + (a[k][2] + a[k][3]);
+            for (int k = 0; k < K; ++k) P[(k * BS + t.s) * D + row] = a0[k] + a1[k];
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < rows; j += NT) {
-        double a[K][4];
+    // t = G x, padded like x
+    for (int e = threadIdx.x; e < K * D; e += blockDim.x) {
+        const int k = e / D, i = e - k * D;
+        double v = P[(k * BS) * D + i];
+        for (int q = 1; q <= i / 32; ++q) v += P[(k * BS + q) * D + i];
+        ts[k * XS + vpad(i)] = v;
+    }
+    __syncthreads();
+    if (w < BS) {  // phase 2: Q(s, col) = G(32s..32s+31, col)^T t(32s..32s+31)
+        const ApplyTask tB0 = apply_task<BS>(2, w, 1);
+        const bool hasB = tB0.b >= 0;
+        const ApplyTask t = (h && hasB) ? tB0 : apply_task<BS>(2, w, 0);
+        const int col = t.b * 32 + l;
+        const int tbase = vpad(t.s * 32);
+        double a0[K], a1[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) a[k][0] = a[k][1] = a[k][2] = a[k][3] = 0.0;
-        int i = j;
-        int idx = packed_lower(j, j);  // advances by i + 1 from row i to row i + 1
-        for (; i + 8 <= rows; i += 8) {
-            double gv[8], tv[K][8];
+        for (int k = 0; k < K; ++k) a0[k] = a1[k] = 0.0;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                gv[u] = double(G[idx]);
+        for (int u0 = 0; u0 < 32; u0 += 8) {
+            double g[8], tv[K][8];
 #pragma unroll
-                for (int k = 0; k < K; ++k) tv[k][u] = t[k * D + i + u];
-                idx += i + u + 1;
+            for (int v = 0; v < 8; ++v) {
+                const int i = t.s * 32 + u0 + v;
+                const double gv = double(G[packed_lower(i, 0) + col]);  // in bounds; masked below
+                g[v] = i >= col ? gv : 0.0;
+#pragma unroll
+                for (int k = 0; k < K; ++k) tv[k][v] = ts[k * XS + tbase + u0 + v];
             }
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                a[k][0] += gv[0] * tv[k][0] + gv[4] * tv[k][4];
-                a[k][1] += gv[1] * tv[k][1] + gv[5] * tv[k][5];
-                a[k][2] += gv[2] * tv[k][2] + gv[6] * tv[k][6];
-                a[k][3] += gv[3] * tv[k][3] + gv[7] * tv[k][7];
+                a0[k] += g[0] * tv[k][0] + g[2] * tv[k][2] + g[4] * tv[k][4] + g[6] * tv[k][6];
+                a1[k] += g[1] * tv[k][1] + g[3] * tv[k][3] + g[5] * tv[k][5] + g[7] * tv[k][7];
             }
         }
-        for (; i < rows; ++i) {
-            const double gi = double(G[idx]);
+        if (h == 0 || hasB)
 #pragma unroll
-            for (int k = 0; k < K; ++k) a[k][0] += gi * t[k * D + i];
-            idx += i + 1;
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) y[k * D + j] = (a[k][0] + a[k][1]) + (a[k][2] + a[k][3]);
+            for (int k = 0; k < K; ++k) Q[(k * BS + t.s) * D + col] = a0[k] + a1[k];
     }
+    __syncthreads();
+}
+
+// y_k(j) from the phase-2 partials
+template <int BS>
+__device__ __forceinline__ double bank_result(const double *__restrict__ Q, int k, int j)
+{
+    constexpr int D = 32 * BS;
+    double v = Q[(k * BS + j / 32) * D + j];
+    for (int q = j / 32 + 1; q < BS; ++q) v += Q[(k * BS + q) * D + j];
+    return v;
 }
 
 template <class TS>
@@ -319,9 +441,24 @@ __device__ __forceinline__ void stage_matrix(const TS *__restrict__ src, TL *__r
 }
 
 constexpr int kApplyThreads = 256;
+static_assert(kApplyThreads >= 96, "one thread per bank row in the epilogue");
+
+// bank vector of each column into LDS (padded positions, zero past n entries)
+template <int BS, int K>
+__device__ __forceinline__ void load_bank_vector(const double *__restrict__ src, int64_t ld, int n,
+                                                 double *__restrict__ xs)
+{
+    constexpr int D = 32 * BS;
+    constexpr int XS = D + BS;
+    for (int e = threadIdx.x; e < K * D; e += blockDim.x) {
+        const int k = e / D, i = e - k * D;
+        xs[k * XS + vpad(i)] = i < n ? src[k * ld + i] : 0.0;
+    }
+}
 
 // Coarse level: y = A^{-1} r per bank of 32 nodes; w = y + w_parent[parent].
-// K columns: r and w at stride ld, w_parent at stride ld_parent.
+// K columns: r and w at stride ld, w_parent at stride ld_parent.  The epilogue's
+// gathers are issued before the bank is staged, so their latency overlaps it.
 template <int BS, class TS, int K>
 __global__ __launch_bounds__(kApplyThreads) void k_coarse_apply(int32_t nodes, const TS *__restrict__ inv,
                                                                 const double *__restrict__ r, int64_t ld,
@@ -335,24 +472,27 @@ __global__ __launch_bounds__(kApplyThreads) void k_coarse_apply(int32_t nodes, c
     using TL = typename LdsType<TS>::type;
     if (dev::all_stopped(stop, K)) return;
     __shared__ __attribute__((aligned(16))) TL A[S + 4];
-    __shared__ double x[K * D], tv[K * D], yv[K * D];
+    __shared__ double xs[K * (D + BS)], ts[K * (D + BS)], P[K * BS * D], Q[K * BS * D];
     const int32_t bank = blockIdx.x;
     const int32_t lo = bank * 32;
-    const int32_t cnt = min(32, nodes - lo);
-    stage_matrix<TS, TL, S, kApplyThreads>(inv + int64_t(bank) * S, A);
-    for (int t = threadIdx.x; t < K * D; t += kApplyThreads) {
-        const int k = t / D, e = t - k * D;
-        x[t] = (e < cnt * BS) ? r[k * ld + int64_t(lo) * BS + e] : 0.0;
+    const int n = min(32, nodes - lo) * BS;
+    const int i = threadIdx.x;
+    double wp[K];
+    if (i < n && w_parent) {
+        const int64_t pi = int64_t(parent[lo + i / BS]) * BS + (i % BS);
+#pragma unroll
+        for (int k = 0; k < K; ++k) wp[k] = w_parent[k * ld_parent + pi];
     }
+    load_bank_vector<BS, K>(r + int64_t(lo) * BS, ld, n, xs);
+    stage_matrix<TS, TL, S, kApplyThreads>(inv + int64_t(bank) * S, A);
     __syncthreads();
-    // rows beyond the bank's nodes are identity padding: they do not couple to real rows
-    bank_apply<D, kApplyThreads, TL, K>(A, x, tv, yv, cnt * BS);
-    for (int i = threadIdx.x; i < cnt * BS; i += kApplyThreads) {
-        const int64_t pi = w_parent ? int64_t(parent[lo + i / BS]) * BS + (i % BS) : 0;
+    // rows beyond the bank's nodes are identity padding with zero right-hand side
+    bank_solve<BS, TL, K>(A, xs, ts, P, Q);
+    if (i < n) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            double y = yv[k * D + i];
-            if (w_parent) y += w_parent[k * ld_parent + pi];
+            double y = bank_result<BS>(Q, k, i);
+            if (w_parent) y += wp[k];
             w[k * ld + int64_t(lo) * BS + i] = y;
         }
     }
@@ -377,31 +517,39 @@ __global__ __launch_bounds__(kApplyThreads) void k_level0_apply(int32_t nbanks, 
     using TL = typename LdsType<TS>::type;
     if (dev::all_stopped(stop, K)) return;
     __shared__ __attribute__((aligned(16))) TL A[S + 4];
-    __shared__ double x[K * D], tv[K * D], yv[K * D];
+    __shared__ double xs[K * (D + BS)], ts[K * (D + BS)], P[K * BS * D], Q[K * BS * D];
     const int32_t bank = blockIdx.x;
     const int32_t lo = part_off[bank];
-    const int32_t cnt = part_off[bank + 1] - lo;
-    stage_matrix<TS, TL, S, kApplyThreads>(inv + int64_t(bank) * S, A);
-    for (int t = threadIdx.x; t < K * D; t += kApplyThreads) {
-        const int k = t / D, e = t - k * D;
-        x[t] = (e < cnt * BS) ? r[k * ld + int64_t(lo) * BS + e] : 0.0;
-    }
-    __syncthreads();
-    bank_apply<D, kApplyThreads, TL, K>(A, x, tv, yv, cnt * BS);
-    double part[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) part[k] = 0.0;
-    for (int i = threadIdx.x; i < cnt * BS; i += kApplyThreads) {
-        const int64_t gi = int64_t(lo) * BS + i;
+    const int n = (part_off[bank + 1] - lo) * BS;
+    const int i = threadIdx.x;
+    const int64_t gi = int64_t(lo) * BS + i;
+    // epilogue operands first: their latency overlaps the staging of the bank
+    double cw[K], ca[K], cd[K];
+    if (i < n) {
         const int64_t pi = w1 ? int64_t(parent[lo + i / BS]) * BS + (i % BS) : 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            double y = yv[k * D + i];
-            if (w1) y += w1[k * ld1 + pi];
+            cw[k] = w1 ? w1[k * ld1 + pi] : 0.0;
+            ca[k] = add ? add[k * ld + gi] : 0.0;
+            if constexpr (DOT) cd[k] = dotv[k * ld + gi];
+        }
+    }
+    load_bank_vector<BS, K>(r + int64_t(lo) * BS, ld, n, xs);
+    stage_matrix<TS, TL, S, kApplyThreads>(inv + int64_t(bank) * S, A);
+    __syncthreads();
+    bank_solve<BS, TL, K>(A, xs, ts, P, Q);
+    double part[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) part[k] = 0.0;
+    if (i < n) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double y = bank_result<BS>(Q, k, i);
+            if (w1) y += cw[k];
             y *= scale;
-            if (add) y += add[k * ld + gi];
+            if (add) y += ca[k];
             z[k * ld + gi] = y;
-            if constexpr (DOT) part[k] += dotv[k * ld + gi] * y;
+            if constexpr (DOT) part[k] = cd[k] * y;
         }
     }
     if constexpr (DOT) {
@@ -483,13 +631,30 @@ void mas_galerkin(int64_t nslots, int bs, const int64_t *src_ptr, const int32_t 
     LSS_LAUNCH_CHECK();
 }
 
-void mas_restrict(int32_t nodes, int bs, const int64_t *child_ptr, const int32_t *child, const double *fine,
-                  double *coarse, const int *stop, hipStream_t s, int k, int64_t ld_fine, int64_t ld_coarse)
+void mas_restrict(int bs, int32_t fine_nbanks, const int32_t *fine_bank_off, const int32_t *fine_parent,
+                  const double *fine, double *coarse, const int *stop, hipStream_t s, int k, int64_t ld_fine,
+                  int64_t ld_coarse)
 {
-    const int64_t work = int64_t(nodes) * bs;
-    if (work == 0) return;
-    k_restrict<<<dim3(div_up(work, 256), k), 256, 0, s>>>(nodes, bs, child_ptr, child, fine, ld_fine, coarse,
-                                                          ld_coarse, stop);
+    if (fine_nbanks == 0) return;
+    const int grid = div_up(fine_nbanks, 8);
+#define LSS_RS(BS, K)                                                                                         \
+    k_restrict<BS, K><<<grid, 256, 0, s>>>(fine_nbanks, fine_bank_off, fine_parent, fine, ld_fine, coarse,    \
+                                           ld_coarse, stop)
+#define LSS_RS_K(BS)                                                                                          \
+    switch (k) {                                                                                              \
+        case 1: LSS_RS(BS, 1); break;                                                                         \
+        case 2: LSS_RS(BS, 2); break;                                                                         \
+        case 4: LSS_RS(BS, 4); break;                                                                         \
+        default: fail(LSS_ERR_ARG, "a fused MAS apply takes 1, 2 or 4 right-hand sides");                     \
+    }
+    switch (bs) {
+        case 1: LSS_RS_K(1); break;
+        case 2: LSS_RS_K(2); break;
+        case 3: LSS_RS_K(3); break;
+        default: fail(LSS_ERR_UNSUPPORTED, "block_size must be 1, 2 or 3");
+    }
+#undef LSS_RS_K
+#undef LSS_RS
     LSS_LAUNCH_CHECK();
 }
 

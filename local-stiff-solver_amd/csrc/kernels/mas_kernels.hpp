@@ -24,9 +24,10 @@ void mas_galerkin(int64_t nslots, int bs, const int64_t *src_ptr, const int32_t 
 // Apply kernels take k = 1, 2 or 4 right-hand sides: column c of a vector v is
 // v + c * ld (level vectors: ld = nodes * bs of their level); with stop != nullptr
 // they do nothing once the stop words of all k columns are set.
-void mas_restrict(int32_t nodes, int bs, const int64_t *child_ptr, const int32_t *child, const double *fine,
-                  double *coarse, const int *stop, hipStream_t s, int k = 1, int64_t ld_fine = 0,
-                  int64_t ld_coarse = 0);
+// coarse[parent[i]] = sum of fine[i] over the nodes i of each fine bank (banks of at most 32 nodes)
+void mas_restrict(int bs, int32_t fine_nbanks, const int32_t *fine_bank_off, const int32_t *fine_parent,
+                  const double *fine, double *coarse, const int *stop, hipStream_t s, int k = 1,
+                  int64_t ld_fine = 0, int64_t ld_coarse = 0);
 void mas_coarse_apply(Precision prec, int bs, int32_t nodes, const void *inv, const double *r, const int32_t *parent,
                       const double *w_parent, double *w, const int *stop, hipStream_t s, int k = 1, int64_t ld = 0,
                       int64_t ld_parent = 0);
