@@ -118,55 +118,45 @@ __global__ __launch_bounds__(kT) void k_cholesky(const int64_t *__restrict__ com
     if (threadIdx.x == 0 && bad) atomicOr(status, 2);
 }
 
-// Several right-hand sides: grid row y is column y (vectors at stride ld, stiff
-// work vectors at stride ns).
-__global__ void k_stiff_residual(int64_t ns, int bs, const int32_t *__restrict__ pdofs,
-                                 const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
-                                 const double *__restrict__ bvals, const double *__restrict__ r,
-                                 const double *__restrict__ z, int64_t ld, double *__restrict__ work,
-                                 const int *__restrict__ stop)
+// (r - A z)[p] for the stiff rows p, in FP64 from the block operator: one wave
+// per stiff row, lanes over the row's (block, component) entries, a wave
+// reduction.  OUT_U: write u[p] (recomputing the stiff rows after a
+// reduced-precision SpMV); else work[k] for the stiff correction.  Several
+// right-hand sides: grid row y is column y (vectors at stride ld, stiff work
+// vectors at stride ns).
+template <bool OUT_U>
+__global__ __launch_bounds__(256) void k_stiff_residual(int64_t ns, int bs, const int32_t *__restrict__ pdofs,
+                                                        const int32_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ cols,
+                                                        const double *__restrict__ bvals, const double *__restrict__ r,
+                                                        const double *__restrict__ z, int64_t ld,
+                                                        double *__restrict__ out, const int *__restrict__ stop)
 {
     if (dev::all_stopped(stop, gridDim.y)) return;
-    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t k = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (k >= ns) return;
     r += blockIdx.y * ld;
     z += blockIdx.y * ld;
-    work += blockIdx.y * ns;
     const int32_t p = pdofs[k];
     const int32_t I = p / bs;
     const int a = p - I * bs;
     const int bb = bs * bs;
+    const int32_t e0 = rowptr[I];
+    const int32_t m = (rowptr[I + 1] - e0) * bs;
     double acc = 0.0;
-    for (int32_t e = rowptr[I]; e < rowptr[I + 1]; ++e) {
-        const int32_t J = cols[e];
-        for (int c = 0; c < bs; ++c) acc += bvals[int64_t(e) * bb + a * bs + c] * z[int64_t(J) * bs + c];
+    for (int32_t q = lane; q < m; q += 64) {
+        const int32_t e = e0 + q / bs;
+        const int c = q - (q / bs) * bs;
+        acc += bvals[int64_t(e) * bb + a * bs + c] * z[int64_t(cols[e]) * bs + c];
     }
-    work[k] = r[p] - acc;
-}
-
-// u[p] = r[p] - (A z)[p] in FP64 for the stiff rows p (after a reduced-precision SpMV)
-__global__ void k_stiff_rows_exact(int64_t ns, int bs, const int32_t *__restrict__ pdofs,
-                                   const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
-                                   const double *__restrict__ bvals, const double *__restrict__ r,
-                                   const double *__restrict__ z, double *__restrict__ u, int64_t ld,
-                                   const int *__restrict__ stop)
-{
-    if (dev::all_stopped(stop, gridDim.y)) return;
-    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= ns) return;
-    r += blockIdx.y * ld;
-    z += blockIdx.y * ld;
-    u += blockIdx.y * ld;
-    const int32_t p = pdofs[k];
-    const int32_t I = p / bs;
-    const int a = p - I * bs;
-    const int bb = bs * bs;
-    double acc = 0.0;
-    for (int32_t e = rowptr[I]; e < rowptr[I + 1]; ++e) {
-        const int32_t J = cols[e];
-        for (int c = 0; c < bs; ++c) acc += bvals[int64_t(e) * bb + a * bs + c] * z[int64_t(J) * bs + c];
+    acc = dev::wave_sum(acc);
+    if (lane == 0) {
+        if (OUT_U)
+            out[blockIdx.y * ld + p] = r[p] - acc;
+        else
+            out[blockIdx.y * ns + k] = r[p] - acc;
     }
-    u[p] = r[p] - acc;
 }
 
 // G = L^{-1}: one lane per column j, G(i, j) = -(sum_{k=j}^{i-1} L(i,k) G(k,j)) / L(i,i).
@@ -190,6 +180,44 @@ __global__ __launch_bounds__(64) void k_tri_inverse(const int64_t *__restrict__ 
         const double g = -s / Li[i];
         G[int64_t(i) * m + j] = g;  // lower triangle: G
         G[int64_t(j) * m + i] = g;  // strict upper triangle: G^T, so both products read rows
+    }
+}
+
+// Same inverse for components of at most kPanelMaxM DOFs: one wave per panel
+// of 32 columns, the panel G(j0:m, j0:j0+32) kept in LDS.  Rows go in order;
+// lane (h, c) forms the dot product of column j0 + c over the entries k of
+// parity h, the two halves are added with one cross-half shuffle.
+constexpr int kPanelMaxM = 512;
+__global__ __launch_bounds__(64) void k_tri_inverse_panel(const int64_t *__restrict__ comp_off,
+                                                          const int64_t *__restrict__ mat_off,
+                                                          const double *__restrict__ Lall, double *__restrict__ Gall)
+{
+    extern __shared__ double P[];  // P[(i - j0) * 32 + c] = G(i, j0 + c)
+    const int32_t c = blockIdx.x;
+    const int32_t m = int32_t(comp_off[c + 1] - comp_off[c]);
+    const int32_t j0 = blockIdx.y * 32;
+    if (j0 >= m) return;
+    const int lane = threadIdx.x, h = lane >> 5, cl = lane & 31;
+    const int32_t col = j0 + cl;
+    const double *L = Lall + mat_off[c];
+    double *G = Gall + mat_off[c];
+    for (int32_t i = j0; i < m; ++i) {
+        const double *Li = L + int64_t(i) * m;
+        double sacc = 0.0;
+        if (col < i)
+            for (int32_t k = col + h; k < i; k += 2) sacc += Li[k] * P[(k - j0) * 32 + cl];
+        sacc += __shfl_xor(sacc, 32, 64);
+        double g = 0.0;
+        if (col == i) g = 1.0 / Li[i];
+        else if (col < i) g = -sacc / Li[i];
+        if (h == 0 && col < m) P[(i - j0) * 32 + cl] = g;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (h == 0 && col < m && col <= i) {
+            G[int64_t(i) * m + col] = g;  // lower triangle: G
+            G[int64_t(col) * m + i] = g;  // upper triangle: G^T (the diagonal twice, same value)
+        }
     }
 }
 
@@ -252,8 +280,19 @@ void stiff_inverse_factor(int32_t ncomp, int32_t max_m, const int64_t *comp_off,
                           const double *L, double *G, hipStream_t s)
 {
     if (ncomp == 0) return;
-    dim3 grid(ncomp, div_up(max_m, 64));
-    k_tri_inverse<<<grid, 64, 0, s>>>(comp_off, mat_off, L, G);
+    if (max_m <= kPanelMaxM) {
+        const size_t lds = size_t(max_m) * 32 * sizeof(double);
+        static bool attr = false;
+        if (!attr) {
+            LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tri_inverse_panel),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        int(size_t(kPanelMaxM) * 32 * sizeof(double))));
+            attr = true;
+        }
+        k_tri_inverse_panel<<<dim3(ncomp, div_up(max_m, 32)), 64, lds, s>>>(comp_off, mat_off, L, G);
+    } else {
+        k_tri_inverse<<<dim3(ncomp, div_up(max_m, 64)), 64, 0, s>>>(comp_off, mat_off, L, G);
+    }
     LSS_LAUNCH_CHECK();
 }
 
@@ -264,8 +303,8 @@ void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, c
                    hipStream_t s, int k, int64_t ld)
 {
     if (ncomp == 0 || ns == 0) return;
-    k_stiff_residual<<<dim3(div_up(ns, 256), k), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r, z, ld, work,
-                                                             stop);
+    k_stiff_residual<false><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r, z, ld,
+                                                                  work, stop);
     LSS_LAUNCH_CHECK();
     k_stiff_rows<false><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, row_comp, comp_off, mat_off, pdofs, G, work, work2,
                                                               z, ld, stop);
@@ -280,8 +319,8 @@ void stiff_rows_exact(int64_t ns, int bs, const int32_t *pdofs, const int32_t *r
                       int k, int64_t ld)
 {
     if (ns == 0) return;
-    k_stiff_rows_exact<<<dim3(div_up(ns, 256), k), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r, z, u, ld,
-                                                               stop);
+    k_stiff_residual<true><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r, z, ld, u,
+                                                                 stop);
     LSS_LAUNCH_CHECK();
 }
 

@@ -243,3 +243,37 @@ def test_damping_retry_after_breakdown(gpu):
     oi = H.info()
     assert abs(info["num_iterations"] - oi["num_iterations"]) <= max(2, 0.05 * oi["num_iterations"])
     assert np.linalg.norm(x - xr) <= 1e-5 * np.linalg.norm(xr)
+
+
+def test_user_stiff_set_matches_oracle(gpu):
+    # GPUAMGF: the caller names the stiff DOFs (set_problematic_dofs with
+    # select_bad_dofs_from_l1_norm false, GPUHybrid.cu:739-741), here by a threshold
+    # on a per-DOF indicator as linear_solve -i/-t does (tests/linear_solve.cpp:396-440)
+    prob = problems.make("elastic_contact", N=10)
+    A = prob.csc()
+    ind = O.row_l1(A.colptr, A.vals)
+    dofs = np.nonzero(ind > np.quantile(ind, 0.99))[0].astype(np.int32)
+    s = Solver.create(_cfg(3, select_bad_dofs_from_l1_norm=False))
+    s.set_problematic_dofs(dofs)
+    s.factorize(A)
+    assert s.name() == "GPUAMGF"
+    H = O.Hybrid(_ocfg(3, select_bad_dofs_from_l1_norm=False))
+    H.set_problematic_dofs(dofs)
+    H.factorize(A.colptr, A.rowidx, A.vals)
+    dc, oc = s.stiff_components(), H.stiff_components()
+    assert len(dc) == len(oc) > 0
+    for a, b in zip(dc, oc):
+        assert np.array_equal(a, b)
+    b = prob.rhs(4)
+    x = s.solve(b)
+    info = s.get_info()
+    xr = H.solve(b)
+    oi = H.info()
+    assert info["stop_reason"] == "converged_rel"
+    assert _res_ok(A.to_scipy(), x, b)
+    assert abs(info["num_iterations"] - oi["num_iterations"]) <= max(2, 0.05 * oi["num_iterations"])
+    assert np.linalg.norm(x - xr) <= 1e-5 * np.linalg.norm(xr)
+    # without a user set nothing is stiff: the base preconditioner alone
+    s2 = Solver.create(_cfg(3, select_bad_dofs_from_l1_norm=False))
+    s2.factorize(A)
+    assert s2.stiff_components() == []
