@@ -114,6 +114,7 @@ def main():
     d_b = torch.from_numpy(b).to(dev)
     x = torch.zeros(n, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
+    log("system on the device:", n, "DOF,", A.nnz, "stored entries")
     bnorm = float(np.linalg.norm(b))
     stream = torch.cuda.current_stream()
     base = {"block_size": prob.block_size, "relative_tolerance": 1e-8, "gmm_jump_threshold": 10.0,
@@ -133,6 +134,7 @@ def main():
 
         for _ in range(args.warmup):
             step()
+            log(name, "warm-up step done")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):

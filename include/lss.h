@@ -48,8 +48,12 @@ typedef struct lss_solver lss_solver;
  *   {"solver": "GPUHybrid" | ["GPUHybrid", ...], "GPUHybrid": {key: value, ...}}
  * Keys and defaults follow linear-solver-spec.json:646-747 (the defaults
  * Solver::create injects, Solver.cpp:143-173); keys parsed as in
- * GPUHybrid::set_parameters (GPUHybrid.cu:106-179).  Additional keys of this
- * build are listed in INTEGRATION.md.  json may be NULL or "" for all defaults.
+ * GPUHybrid::set_parameters (GPUHybrid.cu:106-179).  Additional keys:
+ * "precision" (fp64 | fp32 | bf16 | fp16 bank storage), "inner_spmv" (auto | fp32 |
+ * fp64), "base_preconditioner" (MAS | Jacobi), "partitioner" (device | host),
+ * "max_coarse_levels" (0-4), "damping", "lanczos_steps", "check_every",
+ * "max_dense_component", "true_residual_check", "profile_kernels"; unknown keys
+ * are rejected.  json may be NULL or "" for all defaults.
  * Replaces: Solver::create(json, logger) + GPUHybrid::GPUHybrid() (GPUHybrid.cu:79-104).
  */
 int lss_create(const char *json, lss_solver **out);

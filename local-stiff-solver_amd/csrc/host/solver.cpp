@@ -590,13 +590,29 @@ void HybridSolver::build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx
     k::stiff_extract(ncomp_, d_comp_off_.get(), d_mat_off_.get(), d_dofs.get(), g2l.get(), comp_of.get(), ptr, idx,
                      vals, d_L_.get(), s);
     d_status_.zero(s);
+    DBuf<double> G(moff[ncomp_]);
+    // up to stiff_small_max() DOFs: one workgroup per component; larger: blocked path
     k::stiff_cholesky(ncomp_, d_comp_off_.get(), d_mat_off_.get(), d_L_.get(), d_status_.get(), s);
+    int64_t max_small = 0, work = 0;
+    for (int32_t c = 0; c < ncomp_; ++c) {
+        const int64_t m = off[c + 1] - off[c];
+        if (m <= k::stiff_small_max()) max_small = std::max(max_small, m);
+        else work = std::max(work, DenseBlas::work_size(m));
+    }
+    if (work > 0) {
+        DBuf<double> w(work);
+        for (int32_t c = 0; c < ncomp_; ++c) {
+            const int64_t m = off[c + 1] - off[c];
+            if (m > k::stiff_small_max())
+                dense_.factor_inverse(m, d_L_.get() + moff[c], G.get() + moff[c], w.get(), d_status_.get(), s);
+        }
+        sync(s);  // w is released at scope end
+    }
     int st = 0;
     d_status_.download(&st, 1, s);
     sync(s);
     if (st) fail(LSS_ERR_NOT_SPD, "stiff block Cholesky failed: a selected sub-block is not SPD");
-    DBuf<double> G(moff[ncomp_]);
-    k::stiff_inverse_factor(ncomp_, int32_t(max_comp_), d_comp_off_.get(), d_mat_off_.get(), d_L_.get(), G.get(), s);
+    k::stiff_inverse_factor(ncomp_, int32_t(max_small), d_comp_off_.get(), d_mat_off_.get(), d_L_.get(), G.get(), s);
     sync(s);
     d_L_ = std::move(G);  // only the inverse factor is needed by the apply
     d_swork_.resize(std::max<int64_t>(2 * kMaxCols * ns_, 1));

@@ -12,6 +12,7 @@
 #include "../kernels/kernels.hpp"
 #include "../kernels/mas_kernels.hpp"
 #include "config.hpp"
+#include "dense.hpp"
 #include "stiff_select.hpp"
 #include "topology.hpp"
 
@@ -138,6 +139,7 @@ private:
     DBuf<int64_t> d_comp_off_, d_mat_off_;
     DBuf<int32_t> d_pdofs_, d_row_comp_;
     DBuf<double> d_L_, d_swork_;
+    DenseBlas dense_;  // blocked factorization of large components
     double omega_ = 1.0;
     double omega_used_ = 1.0;  // omega of the last solve (after a damping retry)
     bool damping_retry_ = false;

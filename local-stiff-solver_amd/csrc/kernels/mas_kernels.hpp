@@ -42,12 +42,24 @@ void mas_level0_apply(Precision prec, int bs, int32_t nbanks, const int32_t *par
 void stiff_extract(int32_t ncomp, const int64_t *comp_off, const int64_t *mat_off, const int32_t *dofs,
                    const int32_t *g2l, const int32_t *comp_of, const int32_t *ptr, const int32_t *idx,
                    const double *vals, double *mats, hipStream_t s);
-// In-place dense Cholesky (lower, row-major) of every component; status |= 2 on failure.
+// In-place dense Cholesky (lower, row-major) of every component of at most
+// stiff_small_max() DOFs; status |= 2 on failure.
 void stiff_cholesky(int32_t ncomp, const int64_t *comp_off, const int64_t *mat_off, double *mats, int *status,
                     hipStream_t s);
-// G = L^{-1} per component from the Cholesky factors; stored with G^T in the strict upper triangle.
-void stiff_inverse_factor(int32_t ncomp, int32_t max_m, const int64_t *comp_off, const int64_t *mat_off,
+// G = L^{-1} per component of at most stiff_small_max() DOFs (larger ones are skipped);
+// stored with G^T in the strict upper triangle.  max_small_m: largest such component.
+void stiff_inverse_factor(int32_t ncomp, int32_t max_small_m, const int64_t *comp_off, const int64_t *mat_off,
                           const double *L, double *G, hipStream_t s);
+// stiff_cholesky and stiff_inverse_factor handle components up to this size.
+int stiff_small_max();
+
+// Blocked path of one large component (row-major m x m, leading dimension m):
+// factor the diagonal block at k0 (kb <= dense_block_size()) in place and write its
+// inverse into the same block of G; status |= 2 when it is not positive definite.
+int dense_block_size();
+void dense_diag_block(int64_t m, int k0, int kb, double *A, double *G, int *status, hipStream_t s);
+// G(i, j) := G(j, i) for j > i.
+void dense_mirror(int64_t m, double *G, hipStream_t s);
 // z[p] += (S_c A S_c^T)^{-1} (r - A z)[p] = G^T G (r - A z)[p] over the stiff rows p of
 // each component, (A z) taken from the block operator in the permuted numbering.
 // k right-hand sides: r, z at stride ld; work, work2 hold k * ns values each.

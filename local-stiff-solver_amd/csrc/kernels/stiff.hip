@@ -19,6 +19,10 @@ namespace {
 
 constexpr int NB = 32;
 constexpr int kT = 256;
+// Components of at most kSmallMax DOFs are factored and inverted by one
+// workgroup each; larger ones take the blocked path (dense_diag_block +
+// library GEMMs, dense.cpp) and are skipped here.
+constexpr int kSmallMax = 512;
 
 __global__ __launch_bounds__(kT) void k_extract(const int64_t *__restrict__ comp_off, const int64_t *__restrict__ mat_off,
                                                 const int32_t *__restrict__ dofs, const int32_t *__restrict__ g2l,
@@ -47,6 +51,7 @@ __global__ __launch_bounds__(kT) void k_cholesky(const int64_t *__restrict__ com
     __shared__ int bad;
     const int32_t c = blockIdx.x;
     const int32_t m = int32_t(comp_off[c + 1] - comp_off[c]);
+    if (m > kSmallMax) return;
     double *A = mats + mat_off[c];
     if (threadIdx.x == 0) bad = 0;
     for (int32_t k0 = 0; k0 < m; k0 += NB) {
@@ -159,35 +164,13 @@ __global__ __launch_bounds__(256) void k_stiff_residual(int64_t ns, int bs, cons
     }
 }
 
-// G = L^{-1}: one lane per column j, G(i, j) = -(sum_{k=j}^{i-1} L(i,k) G(k,j)) / L(i,i).
-// Stored as M with M(i, j) = G(i, j) for j <= i and M(j, i) = G(i, j) above the
-// diagonal (G and G^T share one square array).
-__global__ __launch_bounds__(64) void k_tri_inverse(const int64_t *__restrict__ comp_off,
-                                                    const int64_t *__restrict__ mat_off, const double *__restrict__ Lall,
-                                                    double *__restrict__ Gall)
-{
-    const int32_t c = blockIdx.x;
-    const int32_t m = int32_t(comp_off[c + 1] - comp_off[c]);
-    const int32_t j = blockIdx.y * 64 + threadIdx.x;
-    if (j >= m) return;
-    const double *L = Lall + mat_off[c];
-    double *G = Gall + mat_off[c];
-    G[int64_t(j) * m + j] = 1.0 / L[int64_t(j) * m + j];
-    for (int32_t i = j + 1; i < m; ++i) {
-        const double *Li = L + int64_t(i) * m;
-        double s = 0.0;
-        for (int32_t k = j; k < i; ++k) s += Li[k] * G[int64_t(k) * m + j];
-        const double g = -s / Li[i];
-        G[int64_t(i) * m + j] = g;  // lower triangle: G
-        G[int64_t(j) * m + i] = g;  // strict upper triangle: G^T, so both products read rows
-    }
-}
-
-// Same inverse for components of at most kPanelMaxM DOFs: one wave per panel
-// of 32 columns, the panel G(j0:m, j0:j0+32) kept in LDS.  Rows go in order;
-// lane (h, c) forms the dot product of column j0 + c over the entries k of
-// parity h, the two halves are added with one cross-half shuffle.
-constexpr int kPanelMaxM = 512;
+// G = L^{-1} of the components of at most kSmallMax DOFs, stored as M with
+// M(i, j) = G(i, j) for j <= i and M(j, i) = G(i, j) above the diagonal (G and
+// G^T share one square array, so both triangular products read rows).  One wave
+// per panel of 32 columns, the panel G(j0:m, j0:j0+32) kept in LDS.  Rows go in
+// order, G(i, j) = -(sum_{k=j}^{i-1} L(i,k) G(k,j)) / L(i,i); lane (h, c) forms
+// the dot product of column j0 + c over the entries k of parity h, the two
+// halves are added with one cross-half shuffle.
 __global__ __launch_bounds__(64) void k_tri_inverse_panel(const int64_t *__restrict__ comp_off,
                                                           const int64_t *__restrict__ mat_off,
                                                           const double *__restrict__ Lall, double *__restrict__ Gall)
@@ -196,7 +179,7 @@ __global__ __launch_bounds__(64) void k_tri_inverse_panel(const int64_t *__restr
     const int32_t c = blockIdx.x;
     const int32_t m = int32_t(comp_off[c + 1] - comp_off[c]);
     const int32_t j0 = blockIdx.y * 32;
-    if (j0 >= m) return;
+    if (m > kSmallMax || j0 >= m) return;
     const int lane = threadIdx.x, h = lane >> 5, cl = lane & 31;
     const int32_t col = j0 + cl;
     const double *L = Lall + mat_off[c];
@@ -257,6 +240,88 @@ __global__ __launch_bounds__(256) void k_stiff_rows(int64_t ns, const int32_t *_
     }
 }
 
+// Diagonal block of the blocked factorization of one large component: the
+// kb x kb block A(k0:k0+kb, k0:k0+kb) (row-major, leading dimension m, lower
+// triangle read) is factored A = L L^T in LDS; L goes back in place (strict
+// upper triangle of the block zeroed) and L^{-1} into the same block of G.
+// Rows of the LDS matrix are padded to DP = kb + 1 doubles.
+constexpr int kDenseNB = 128;
+__global__ __launch_bounds__(256) void k_dense_diag(int64_t m, int k0, int kb, double *__restrict__ A,
+                                                    double *__restrict__ G, int *__restrict__ status)
+{
+    extern __shared__ double lds[];
+    const int DP = kb + 1;
+    double *M = lds;
+    double *tmp = lds + int64_t(kb) * DP;
+    __shared__ int bad;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    constexpr int NW = 4;
+    if (threadIdx.x == 0) bad = 0;
+    for (int t = threadIdx.x; t < kb * kb; t += 256) {
+        const int i = t / kb, j = t - i * kb;
+        M[i * DP + j] = j <= i ? A[(k0 + i) * m + k0 + j] : 0.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < kb; ++k) {
+        const double d = M[k * DP + k];
+        if (threadIdx.x == 0 && (!(d > 0.0) || !isfinite(d))) bad = 1;
+        const double rdk = 1.0 / sqrt(d);
+        for (int i = k + 1 + threadIdx.x; i < kb; i += 256) {
+            const double v = M[i * DP + k] * rdk;
+            M[i * DP + k] = v;
+            tmp[i] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) M[k * DP + k] = sqrt(d);
+        for (int i = k + 1 + wid; i < kb; i += NW) {
+            const double li = tmp[i];
+            for (int j = k + 1 + lane; j <= i; j += 64) M[i * DP + j] -= li * tmp[j];
+        }
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < kb * kb; t += 256) {
+        const int i = t / kb, j = t - i * kb;
+        A[(k0 + i) * m + k0 + j] = j <= i ? M[i * DP + j] : 0.0;
+    }
+    __syncthreads();
+    // L^{-1} in place, columns from the last to the first
+    for (int j = kb - 1; j >= 0; --j) {
+        const double gjj = 1.0 / M[j * DP + j];
+        for (int i = j + 1 + threadIdx.x; i < kb; i += 256) {
+            double acc = 0.0;
+            for (int q = j + 1; q <= i; ++q) acc += M[i * DP + q] * M[q * DP + j];
+            tmp[i] = -acc * gjj;
+        }
+        __syncthreads();
+        for (int i = j + 1 + threadIdx.x; i < kb; i += 256) M[i * DP + j] = tmp[i];
+        if (threadIdx.x == 0) M[j * DP + j] = gjj;
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < kb * kb; t += 256) {
+        const int i = t / kb, j = t - i * kb;
+        if (j <= i) G[(k0 + i) * m + k0 + j] = M[i * DP + j];
+    }
+    if (threadIdx.x == 0 && bad) atomicOr(status, 2);
+}
+
+// Upper triangle of G := transpose of its lower triangle (32 x 32 tiles via LDS).
+__global__ __launch_bounds__(256) void k_dense_mirror(int64_t m, double *__restrict__ G)
+{
+    __shared__ double t[32][33];
+    const int bx = blockIdx.x, by = blockIdx.y;  // tile column, tile row of the source
+    if (bx > by) return;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t i = int64_t(by) * 32 + r, j = int64_t(bx) * 32 + tx;
+        t[r][tx] = (i < m && j < m) ? G[i * m + j] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t i = int64_t(bx) * 32 + r, j = int64_t(by) * 32 + tx;  // destination (i, j) = source (j, i)
+        if (i < m && j < m && j > i) G[i * m + j] = t[tx][r];
+    }
+}
+
 }  // namespace
 
 void stiff_extract(int32_t ncomp, const int64_t *comp_off, const int64_t *mat_off, const int32_t *dofs,
@@ -276,23 +341,46 @@ void stiff_cholesky(int32_t ncomp, const int64_t *comp_off, const int64_t *mat_o
     LSS_LAUNCH_CHECK();
 }
 
-void stiff_inverse_factor(int32_t ncomp, int32_t max_m, const int64_t *comp_off, const int64_t *mat_off,
+void stiff_inverse_factor(int32_t ncomp, int32_t max_small_m, const int64_t *comp_off, const int64_t *mat_off,
                           const double *L, double *G, hipStream_t s)
 {
-    if (ncomp == 0) return;
-    if (max_m <= kPanelMaxM) {
-        const size_t lds = size_t(max_m) * 32 * sizeof(double);
-        static bool attr = false;
-        if (!attr) {
-            LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tri_inverse_panel),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        int(size_t(kPanelMaxM) * 32 * sizeof(double))));
-            attr = true;
-        }
-        k_tri_inverse_panel<<<dim3(ncomp, div_up(max_m, 32)), 64, lds, s>>>(comp_off, mat_off, L, G);
-    } else {
-        k_tri_inverse<<<dim3(ncomp, div_up(max_m, 64)), 64, 0, s>>>(comp_off, mat_off, L, G);
+    if (ncomp == 0 || max_small_m == 0) return;
+    if (max_small_m > kSmallMax) fail(LSS_ERR_ARG, "stiff_inverse_factor: component above the one-workgroup size");
+    const size_t lds = size_t(max_small_m) * 32 * sizeof(double);
+    static bool attr = false;
+    if (!attr) {
+        LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tri_inverse_panel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(size_t(kSmallMax) * 32 * sizeof(double))));
+        attr = true;
     }
+    k_tri_inverse_panel<<<dim3(ncomp, div_up(max_small_m, 32)), 64, lds, s>>>(comp_off, mat_off, L, G);
+    LSS_LAUNCH_CHECK();
+}
+
+int stiff_small_max() { return kSmallMax; }
+int dense_block_size() { return kDenseNB; }
+
+void dense_diag_block(int64_t m, int k0, int kb, double *A, double *G, int *status, hipStream_t s)
+{
+    if (kb <= 0 || kb > kDenseNB) fail(LSS_ERR_ARG, "dense_diag_block: block size out of range");
+    const size_t lds = (size_t(kb) * (kb + 1) + kb) * sizeof(double);
+    static bool attr = false;
+    if (!attr) {
+        LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_dense_diag),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int((size_t(kDenseNB) * (kDenseNB + 1) + kDenseNB) * sizeof(double))));
+        attr = true;
+    }
+    k_dense_diag<<<1, 256, lds, s>>>(m, k0, kb, A, G, status);
+    LSS_LAUNCH_CHECK();
+}
+
+void dense_mirror(int64_t m, double *G, hipStream_t s)
+{
+    if (m == 0) return;
+    const int t = int(div_up(m, 32));
+    k_dense_mirror<<<dim3(t, t), 256, 0, s>>>(m, G);
     LSS_LAUNCH_CHECK();
 }
 
