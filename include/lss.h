@@ -110,7 +110,8 @@ int lss_get_stiff_dofs(const lss_solver *s, int32_t *dofs, int64_t cap, int64_t 
                        int64_t comp_cap, int64_t *n_comp);
 
 /* Last error message of this handle (empty string when none).  With s == NULL:
- * the message of the last failed handle-free call (lss_adjoint_*) on this thread. */
+ * the message of the last failed lss_create or handle-free call (lss_adjoint_*)
+ * on this thread. */
 const char *lss_last_error(const lss_solver *s);
 
 /* Release all device memory.  Replaces GPUHybrid::~GPUHybrid (GPUHybrid.cu:1514-1530). */

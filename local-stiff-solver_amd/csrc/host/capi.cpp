@@ -71,7 +71,9 @@ int lss_create(const char *json, lss_solver **out)
         s->impl = std::make_unique<lss::HybridSolver>(cfg);
     });
     if (rc != LSS_OK) {
-        // keep the handle so the caller can read the message, but mark it unusable
+        // keep the handle so the caller can read the message, but mark it unusable;
+        // the message is also the thread's handle-free error (lss_last_error(NULL))
+        g_error = s->last_error;
         *out = s;
         return rc;
     }
