@@ -34,6 +34,8 @@ std::string as_str(const json::Value &v, const std::string &k)
 
 }  // namespace
 
+Config validate(Config c);
+
 Config parse_config(const char *text, const Config &base)
 {
     Config c = base;
@@ -50,17 +52,66 @@ Config parse_config(const char *text, const Config &base)
         bad(e.what());
     }
     if (doc.kind != json::Value::Object) bad("document must be an object");
-    // solver selection (Solver.cpp:99-141): a name or a priority list that must contain GPUHybrid
+    // solver selection (Solver.cpp:99-141): a name, or a priority list whose first
+    // entry this library implements (GPUHybrid or MAS)
     if (const json::Value *sv = doc.find("solver")) {
-        bool ok = false;
-        if (sv->kind == json::Value::String) ok = sv->str == "GPUHybrid";
+        std::string sel;
+        auto known = [](const std::string &n) { return n == "GPUHybrid" || n == "MAS"; };
+        if (sv->kind == json::Value::String && known(sv->str)) sel = sv->str;
         if (sv->kind == json::Value::Array)
             for (auto &e : sv->arr)
-                if (e.kind == json::Value::String && e.str == "GPUHybrid") ok = true;
-        if (!ok) bad("\"solver\" must select GPUHybrid");
+                if (sel.empty() && e.kind == json::Value::String && known(e.str)) sel = e.str;
+        if (sel.empty()) bad("\"solver\" must select GPUHybrid or MAS");
+        if (sel != c.solver) {
+            // a different solver starts from its own defaults
+            c = Config{};
+            c.solver = sel;
+            if (sel == "MAS") {
+                c.max_iter = 100000;
+                c.relative_tolerance = 1e-6;
+                c.absolute_tolerance = 1e-20;
+                c.block_size = 3;
+            }
+        }
     }
-    const json::Value *g = doc.find("GPUHybrid");
-    if (!g) return c;
+    if (c.mas_mode()) {
+        // MASSolver::set_parameters reads the "MAS" object (MASSolver.cu:226-279, 760-768)
+        if (const json::Value *m = doc.find("MAS")) {
+            if (m->kind != json::Value::Object) bad("\"MAS\" must be an object");
+            for (auto &kv : m->obj) {
+                const std::string &k = kv.first;
+                const json::Value &v = kv.second;
+                if (k == "block_dim") c.block_size = int(as_int(v, k));
+                else if (k == "max_iter") c.max_iter = int(as_int(v, k));
+                else if (k == "relative_tolerance") c.relative_tolerance = as_float(v, k);
+                else if (k == "absolute_tolerance") c.absolute_tolerance = as_float(v, k);
+                else if (k == "lazy_partitioning") c.lazy_partitioning = as_bool(v, k);
+                else if (k == "use_preconditioned_residual_norm") c.use_preconditioned_residual_norm = as_bool(v, k);
+                else if (k == "use_bad_dof_precond") c.use_bad_dof_precond = as_bool(v, k);
+                else if (k == "bad_dof_strategy") {
+                    const std::string st = as_str(v, k);
+                    c.bad_dof_strategy = (st == "kmean" || st == "gmm") ? st : "elbow";
+                } else if (k == "bad_dof_kmeans_search_fraction") c.bad_dof_kmeans_search_fraction = as_float(v, k);
+                else if (k == "bad_dof_kmeans_jump_threshold") c.bad_dof_kmeans_jump_threshold = as_float(v, k);
+                else if (k == "bad_dof_kmeans_max_iterations") c.bad_dof_kmeans_max_iterations = int(as_int(v, k));
+                else if (k == "bad_dof_kmeans_expand_neighbors") c.bad_dof_kmeans_expand_neighbors = int(as_int(v, k));
+                else if (k == "bad_dof_gmm_jump_threshold") c.bad_dof_gmm_jump_threshold = as_float(v, k);
+                else if (k == "bad_dof_gmm_tol") c.bad_dof_gmm_tol = as_float(v, k);
+                else if (k == "bad_dof_gmm_max_iterations") c.bad_dof_gmm_max_iterations = int(as_int(v, k));
+                else if (k == "bad_dof_gmm_expand_neighbors") c.bad_dof_gmm_expand_neighbors = int(as_int(v, k));
+                else if (k == "precision") c.precision = as_str(v, k);
+                else if (k == "inner_spmv") c.inner_spmv = as_str(v, k);
+                else if (k == "check_every") c.check_every = int(as_int(v, k));
+                else if (k == "max_dense_component") c.max_dense_component = as_int(v, k);
+                else if (k == "profile_kernels") c.profile_kernels = as_bool(v, k);
+                else if (k == "partitioner") c.partitioner = as_str(v, k);
+                else if (k == "max_coarse_levels") c.max_coarse_levels = int(as_int(v, k));
+                else bad("unknown key MAS/" + k);
+            }
+        }
+    }
+    const json::Value *g = c.mas_mode() ? nullptr : doc.find("GPUHybrid");
+    if (!g) return validate(c);
     if (g->kind != json::Value::Object) bad("\"GPUHybrid\" must be an object");
     for (auto &kv : g->obj) {
         const std::string &k = kv.first;
@@ -95,6 +146,11 @@ Config parse_config(const char *text, const Config &base)
         else if (k == "max_coarse_levels") c.max_coarse_levels = int(as_int(v, k));
         else bad("unknown key GPUHybrid/" + k);
     }
+    return validate(c);
+}
+
+Config validate(Config c)
+{
     if (c.block_size < 1 || c.block_size > 3) bad("block_size must be 1, 2 or 3");
     if (c.max_iter < 0) bad("max_iter must be >= 0");
     if (c.precision != "fp64" && c.precision != "fp32" && c.precision != "bf16" && c.precision != "fp16")
@@ -106,6 +162,9 @@ Config parse_config(const char *text, const Config &base)
     if (c.max_coarse_levels < 0 || c.max_coarse_levels > 4) bad("max_coarse_levels must be in 0..4");
     if (!(c.damping > 0.0) || c.damping >= 4.0) bad("damping must be in (0, 4)");
     if (c.lanczos_steps < 1) bad("lanczos_steps must be >= 1");
+    if (c.bad_dof_gmm_max_iterations < 0 || c.bad_dof_gmm_expand_neighbors < 0 ||
+        c.bad_dof_kmeans_expand_neighbors < 0 || c.bad_dof_kmeans_max_iterations < 0)
+        bad("bad_dof iteration counts and expand_neighbors must be >= 0");
     return c;
 }
 

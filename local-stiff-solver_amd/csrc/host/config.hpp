@@ -8,6 +8,8 @@
 namespace lss {
 
 struct Config {
+    // "GPUHybrid" (SRC/GPUHybrid.hpp) or "MAS" (SRC/MASSolver.cu:140-788)
+    std::string solver = "GPUHybrid";
     // reference keys (linear-solver-spec.json defaults)
     int max_iter = 10000;
     double relative_tolerance = 1e-10;
@@ -26,6 +28,23 @@ struct Config {
     bool expand_subdomains = true;
     int64_t max_dense_size = 0;           // accepted; every component uses the dense device Cholesky
     bool use_float_on_subdomains = false;  // accepted; the stiff factor stays FP64
+    // keys of the "MAS" solver section (MASSolver.cu:148-172, 226-279); with solver
+    // "MAS", max_iter / relative_tolerance / absolute_tolerance / block_size (its
+    // "block_dim") take that solver's defaults 100000 / 1e-6 / 1e-20 / 3
+    bool lazy_partitioning = false;  // accepted; the partition is rebuilt by every factorize
+    bool use_preconditioned_residual_norm = false;
+    bool use_bad_dof_precond = false;
+    std::string bad_dof_strategy = "elbow";  // "elbow" | "kmean" | "gmm" (other strings: elbow)
+    double bad_dof_kmeans_search_fraction = 0.02;
+    double bad_dof_kmeans_jump_threshold = 10.0;
+    int bad_dof_kmeans_max_iterations = 32;
+    int bad_dof_kmeans_expand_neighbors = 0;
+    double bad_dof_gmm_jump_threshold = 10.0;
+    double bad_dof_gmm_tol = 1e-3;
+    int bad_dof_gmm_max_iterations = 5;
+    int bad_dof_gmm_expand_neighbors = 32;
+    int true_residual_period = 4;  // MASSolver.cu:150 (a constant there)
+    bool mas_mode() const { return solver == "MAS"; }
     // keys of this build
     std::string precision = "fp64";     // MAS bank inverses: "fp64" | "fp32" | "bf16"
     std::string inner_spmv = "auto";    // preconditioner-internal SpMV values: "auto" | "fp64" | "fp32"

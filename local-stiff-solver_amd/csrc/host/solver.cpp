@@ -69,6 +69,7 @@ HybridSolver::~HybridSolver()
 
 const char *HybridSolver::name() const
 {
+    if (cfg_.mas_mode()) return "MAS";  // MASSolver.cu:785-788
     // GPUHybrid.hpp:57-67
     if (cfg_.do_hybrid_precond) return cfg_.select_bad_dofs_from_l1_norm ? "GPUHybrid" : "GPUAMGF";
     return "GPUAMG";
@@ -151,7 +152,7 @@ void HybridSolver::factorize_device(int32_t n, int64_t nnz, const int32_t *ptr, 
     n_selected_ = n_filtered_ = n_expanded_ = 0;
 
     auto t = clk::now();
-    if (cfg_.do_hybrid_precond) select_stiff(n, ptr, idx, vals, s);
+    if (!cfg_.mas_mode() && cfg_.do_hybrid_precond) select_stiff(n, ptr, idx, vals, s);
     t_select_ = since(t);
 
     t = clk::now();
@@ -169,6 +170,12 @@ void HybridSolver::factorize_device(int32_t n, int64_t nnz, const int32_t *ptr, 
     sync(s);
     t_mas_ = since(t);
 
+    if (cfg_.mas_mode() && cfg_.use_bad_dof_precond) {
+        // the MAS solver selects on the partition-permuted operator (MASSolver.cu:398-432)
+        t = clk::now();
+        select_mas_bad(n, ptr, idx, vals, s);
+        t_select_ += since(t);
+    }
     t = clk::now();
     if (!comps_.empty()) build_stiff(n, ptr, idx, vals, s);
     sync(s);
@@ -673,7 +680,7 @@ double tridiag_max_eig(const std::vector<double> &d, const std::vector<double> &
 void HybridSolver::estimate_omega(hipStream_t s)
 {
     omega_ = 1.0;
-    if (!cfg_.do_hybrid_precond || ncomp_ == 0) return;
+    if (cfg_.mas_mode() || !cfg_.do_hybrid_precond || ncomp_ == 0) return;
     // The additive MAS can have lambda_max(B A) > 2, which makes the multiplicative
     // sweep (I - BA)(I - PA)(I - BA) non-contractive and the composed preconditioner
     // indefinite.  omega = damping / theta, theta the largest Ritz value of B A after
@@ -759,12 +766,21 @@ void HybridSolver::base_apply(const double *r, double scale, const double *add, 
 void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, const int *stop, hipStream_t s,
                                  bool first, int k)
 {
+    const int64_t np = int64_t(nb_) * bs_;
+    if (cfg_.mas_mode()) {
+        // MASSolverImpl::apply_preconditioner, MASSolver.cu:583-615: z = B r (+ S^T D^{-1} S r)
+        base_apply(r, 1.0, nullptr, z, st, stop, s, first, k);
+        if (ncomp_)
+            k::stiff_correct(ncomp_, ns_, bs_, d_comp_off_.get(), d_mat_off_.get(), d_row_comp_.get(),
+                             d_pdofs_.get(), d_L_.get(), d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z,
+                             d_swork_.get(), d_swork_.get() + kMaxCols * ns_, stop, s, k, np, true);
+        return;
+    }
     // custom_mixed_precond_iter, GPUHybrid.cu:521-542
     if (!cfg_.do_hybrid_precond || ncomp_ == 0) {
         base_apply(r, 1.0, nullptr, z, st, stop, s, first, k);
         return;
     }
-    const int64_t np = int64_t(nb_) * bs_;
     base_apply(r, omega_, nullptr, z, nullptr, stop, s, false, k);  // z1 = B r
     // z2 = z1 + P (r - A z1)  (dss_precond_iter, GPUHybrid.cu:544-675)
     k::stiff_correct(ncomp_, ns_, bs_, d_comp_off_.get(), d_mat_off_.get(), d_row_comp_.get(), d_pdofs_.get(),
@@ -951,13 +967,14 @@ void HybridSolver::solve_device(const double *b, double *x, hipStream_t s)
     // omega = damping / theta keeps omega * lambda_max(B A) < 2 while theta is within
     // damping / 2 of lambda_max; should the sweep still lose definiteness (a PCG
     // breakdown), solve again from the same start with the safe damping
-    const bool damped = cfg_.do_hybrid_precond && ncomp_ > 0 && cfg_.damping > kSafeDamping;
+    const bool damped = !cfg_.mas_mode() && cfg_.do_hybrid_precond && ncomp_ > 0 && cfg_.damping > kSafeDamping;
     if (damped) {
         x0p_.resize(size_t(np));
         k::copy(np, xp_.get(), x0p_.get(), s);
     }
     damping_retry_ = false;
-    run_pcg(s);
+    if (cfg_.mas_mode()) run_pcg_mas(s);
+    else run_pcg(s);
     if (damped && dev::is_breakdown(stop_code_)) {
         damping_retry_ = true;
         const double omega = omega_;
@@ -990,8 +1007,9 @@ void HybridSolver::solve_multi_device(int32_t nrhs, const double *B, double *X, 
     if (!factorized_) fail(LSS_ERR_NOT_FACTORIZED, "solve called before factorize");
     if (nrhs <= 0) fail(LSS_ERR_ARG, "number of right-hand sides must be positive");
     if (!s) s = stream_;
-    // the restart logic of true_residual_check is per solve: keep those one by one
-    if (nrhs == 1 || cfg_.true_residual_check) {
+    // the restart logic of true_residual_check is per solve, and the MAS solver's loop
+    // polls the host every 10 iterations: keep those one by one
+    if (nrhs == 1 || cfg_.true_residual_check || cfg_.mas_mode()) {
         for (int32_t c = 0; c < nrhs; ++c) solve_device(B + int64_t(c) * n_, X + int64_t(c) * n_, s);
         return;
     }
@@ -1139,6 +1157,20 @@ std::string HybridSolver::info_json() const
     o << "]},\"kernel_ms\":{\"spmv_outer\":" << prof_spmv_outer_ms_ << ",\"spmv_outer_n\":" << prof_spmv_outer_n_
       << ",\"spmv_inner\":" << prof_spmv_inner_ms_ << ",\"spmv_inner_n\":" << prof_spmv_inner_n_
       << ",\"mas_level0\":" << prof_mas0_ms_ << ",\"mas_level0_n\":" << prof_mas0_n_ << "}";
+    if (cfg_.mas_mode()) {
+        // MASSolver::get_info, MASSolver.cu:311-316
+        static const char *status[] = {"Running", "Reach relative tolerance", "Reach absolute tolerance",
+                                       "Reach max iterations"};
+        const MasSelection &m = mas_sel_;
+        o << ",\"solver_iter\":" << num_iterations_ << ",\"solver_error\":" << mas_residual_
+          << ",\"solver_status\":\"" << status[(mas_status_ >= 0 && mas_status_ < 4) ? mas_status_ : 0]
+          << "\",\"bad_dof\":{\"enabled\":" << (cfg_.use_bad_dof_precond ? "true" : "false") << ",\"strategy\":\""
+          << cfg_.bad_dof_strategy << "\",\"selected\":" << m.selected << ",\"expanded\":" << m.expanded
+          << ",\"mean0\":" << m.mean0 << ",\"mean1\":" << m.mean1 << ",\"var0\":" << m.var0 << ",\"var1\":" << m.var1
+          << ",\"global_mean\":" << m.global_mean << ",\"global_var\":" << m.global_var << ",\"iters\":" << m.iters
+          << ",\"ratio\":" << m.ratio << ",\"c0\":" << m.c0 << ",\"c1\":" << m.c1 << ",\"cutoff\":" << m.cutoff
+          << "}";
+    }
     if (last_k_ > 1) {
         // per right-hand side of the last multi solve
         o << ",\"columns\":[";

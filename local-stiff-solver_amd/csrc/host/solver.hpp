@@ -74,6 +74,10 @@ private:
 
     void release();
     void select_stiff(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s);
+    // MAS solver: stiff set of the elbow / k-means / GMM strategy on the permuted operator
+    void select_mas_bad(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s);
+    std::vector<int64_t> mas_gmm_positions(int64_t n, const double *x, hipStream_t s);
+    void run_pcg_mas(hipStream_t s);
     RowSet fetch_rows(const std::vector<int32_t> &rows, const int32_t *ptr, const int32_t *idx, const double *vals,
                       bool with_vals, hipStream_t s);
     void build_operator(int32_t n, int64_t nnz, const int32_t *ptr, const int32_t *idx, const double *vals,
@@ -160,6 +164,17 @@ private:
     int stop_code_ = 0;
     int restarts_ = 0;
     int last_k_ = 1;  // right-hand sides of the last solve
+    // MAS solver (MASSolver.cu): status, selection statistics
+    int mas_status_ = 0;  // 0 running, 1 relative tolerance, 2 absolute tolerance, 3 max iterations
+    double mas_residual_ = 0.0;
+    struct MasSelection {
+        std::string strategy;
+        double mean0 = 0, mean1 = 0, var0 = 0, var1 = 0, global_mean = 0, global_var = 0, ratio = 0;
+        double c0 = 0, c1 = 0;  // k-means centres
+        int iters = 0;
+        int64_t cutoff = -1;  // elbow position
+        int64_t selected = 0, expanded = 0;
+    } mas_sel_;
     std::vector<int> col_iters_, col_stop_;
     std::vector<double> col_res_;
     double t_factorize_ = 0, t_select_ = 0, t_operator_ = 0, t_partition_ = 0, t_mas_ = 0, t_stiff_ = 0,

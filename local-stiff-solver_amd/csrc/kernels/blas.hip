@@ -83,7 +83,25 @@ __global__ __launch_bounds__(kFinT) void k_finish(int count, const double *__res
             st->beta = total / st->gamma_old;
             st->gamma_old = total;
             break;
+        case dev::kFinishPqMas:
+            st->iters = iter + 1;
+            st->pq = total;
+            st->alpha = fabs(total) < 1e-20 ? 0.0 : st->gamma / total;
+            break;
+        case dev::kFinishRzMas:
+            st->gamma_old = st->gamma;
+            st->gamma = total;
+            st->beta = fabs(st->gamma_old) < 1e-20 ? 0.0 : total / st->gamma_old;
+            break;
     }
+}
+
+__global__ void k_axpy_alpha(int64_t n, double sign, const double *__restrict__ x, double *__restrict__ y,
+                             const dev::PcgState *st)
+{
+    const double a = sign * st->alpha;
+    for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kT)
+        y[i] = a * x[i] + 1.0 * y[i];
 }
 
 __global__ __launch_bounds__(kT) void k_update_xr(int64_t n, int64_t ld, double *__restrict__ x,
@@ -176,6 +194,14 @@ void finish(int count, const double *partials, int op, dev::PcgState *st, double
 
 int partials_per_column(int64_t n) { return red_grid(n); }
 
+int dot_partials(int64_t n, const double *a, const double *b, double *partials, hipStream_t s)
+{
+    const int g = red_grid(n);
+    k_dot<<<dim3(g, 1), kT, 0, s>>>(n, 0, a, b, partials);
+    LSS_LAUNCH_CHECK();
+    return g;
+}
+
 void dot(int64_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s, int k,
          int64_t ld)
 {
@@ -197,6 +223,13 @@ void pcg_update_xr(int64_t n, double *x, double *r, const double *p, const doubl
 void pcg_update_p(int64_t n, double *p, const double *z, const dev::PcgState *st, hipStream_t s, int k, int64_t ld)
 {
     k_update_p<<<dim3(ew_grid(n), k), kT, 0, s>>>(n, ld, p, z, st);
+    LSS_LAUNCH_CHECK();
+}
+
+void axpy_alpha(int64_t n, double sign, const double *x, double *y, const dev::PcgState *st, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_axpy_alpha<<<ew_grid(n), kT, 0, s>>>(n, sign, x, y, st);
     LSS_LAUNCH_CHECK();
 }
 

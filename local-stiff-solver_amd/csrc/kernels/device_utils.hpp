@@ -100,6 +100,10 @@ enum FinishOp : int {
     kFinishRr = 2,     // r.r: convergence test
     kFinishRzFirst = 3,  // r.z before the loop: gamma = gamma_old
     kFinishRz = 4,     // r.z in the loop: beta
+    // MAS solver PCG (MASSolver.cu:618-744): no breakdown stops, quotients of tiny
+    // denominators are 0 (scalar_division, MASSolver.cu:39-50)
+    kFinishPqMas = 5,  // alpha = |p.Ap| < 1e-20 ? 0 : rz / p.Ap
+    kFinishRzMas = 6,  // rz_old = rz; rz = r.z; beta = |rz_old| < 1e-20 ? 0 : rz / rz_old
 };
 
 __device__ __forceinline__ void store_partial(double block_total, double *partials)

@@ -206,10 +206,12 @@ __global__ __launch_bounds__(256) void k_minmax(int64_t n, const double *__restr
 }
 
 __global__ void k_flags(int64_t n, const double *__restrict__ g0, const double *__restrict__ g1,
-                        uint8_t *__restrict__ flags)
+                        uint8_t *__restrict__ flags, int high)
 {
+    // GPUHybrid.cu:862-868 (high = 1); BadDofGMMPreconditioner.cu:224-235 picks the cluster
+    // with the larger |mean|
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i < n) flags[i] = g0[i] < g1[i] ? 1 : 0;  // GPUHybrid.cu:862-868
+    if (i < n) flags[i] = (high ? g0[i] < g1[i] : g1[i] < g0[i]) ? 1 : 0;
 }
 
 }  // namespace
@@ -261,10 +263,10 @@ void csum_finish(int nsums, const double *partials, double *out, hipStream_t s)
     LSS_LAUNCH_CHECK();
 }
 
-void gmm_flags(int64_t n, const double *g0, const double *g1, uint8_t *flags, hipStream_t s)
+void gmm_flags(int64_t n, const double *g0, const double *g1, uint8_t *flags, hipStream_t s, int high)
 {
     if (n == 0) return;
-    k_flags<<<div_up(n, 256), 256, 0, s>>>(n, g0, g1, flags);
+    k_flags<<<div_up(n, 256), 256, 0, s>>>(n, g0, g1, flags, high);
     LSS_LAUNCH_CHECK();
 }
 

@@ -42,7 +42,7 @@ struct SellView {
 // right-hand sides share one pass over the matrix (column c at v + c * ld).
 void spmv(const SellView &A, const double *x, double *y, const double *sub, hipStream_t s, bool fp32 = false);
 void spmv_pq(const SellView &A, const double *p, double *q, dev::PcgState *st, double *partials, int iter,
-             hipStream_t s, int k = 1, int64_t ld = 0);
+             hipStream_t s, int k = 1, int64_t ld = 0, int finish_op = dev::kFinishPq);
 void spmv_residual(const SellView &A, const double *x, const double *b, double *r, const int *stop, hipStream_t s,
                    bool fp32 = false, int k = 1, int64_t ld = 0);
 // Partials one column of spmv_pq writes.
@@ -63,6 +63,8 @@ void finish(int count, const double *partials, int op, dev::PcgState *st, double
             int k = 1);
 // Partials one column of dot / pcg_update_xr writes for vectors of length n.
 int partials_per_column(int64_t n);
+// per-workgroup partials of a . b (one column); returns their count (reduce with finish)
+int dot_partials(int64_t n, const double *a, const double *b, double *partials, hipStream_t s);
 // out[c] = a_c . b_c
 void dot(int64_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s, int k = 1,
          int64_t ld = 0);
@@ -72,6 +74,8 @@ void pcg_update_xr(int64_t n, double *x, double *r, const double *p, const doubl
 // p = beta p + z.
 void pcg_update_p(int64_t n, double *p, const double *z, const dev::PcgState *st, hipStream_t s, int k = 1,
                   int64_t ld = 0);
+// y = sign * alpha * x + y with alpha from the PCG state (MASSolver.cu:651-666 axpby)
+void axpy_alpha(int64_t n, double sign, const double *x, double *y, const dev::PcgState *st, hipStream_t s);
 void copy(int64_t n, const double *a, double *b, hipStream_t s);
 void fill(int64_t n, double *a, double v, hipStream_t s);
 void scale(int64_t n, double *a, double v, hipStream_t s);
@@ -96,7 +100,17 @@ void gmm_var(int64_t n, const double *x, const double *g0, const double *g1, dou
              double *partials2, hipStream_t s);
 void gmm_var_global(int64_t n, const double *x, double mean, double *partials1, hipStream_t s);
 void csum_finish(int nsums, const double *partials, double *out, hipStream_t s);
-void gmm_flags(int64_t n, const double *g0, const double *g1, uint8_t *flags, hipStream_t s);
+// flags[i] = g1[i] > g0[i] (high = 1) or g0[i] > g1[i] (high = 0)
+void gmm_flags(int64_t n, const double *g0, const double *g1, uint8_t *flags, hipStream_t s, int high = 1);
+
+// ---------------------------------------------------------------- MAS solver selection
+// Per permuted scalar row of the block operator: mode 0 sum |a_ij|, 1 |a_ii|,
+// 2 max |a_ij| (j != i), each in the reference's loop order; padding rows get 0.
+void mas_indicator(int mode, int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols, const double *vals,
+                   int32_t pblock, int npad, double *out, hipStream_t s);
+// Stable sort of n keys (ascending or descending) with their positions 0..n-1; synchronizes s.
+void sort_positions(int32_t n, const double *keys, bool descending, double *keys_out, int32_t *pos_out,
+                    hipStream_t s);
 
 // ---------------------------------------------------------------- partition
 // Recursive breadth-first bisection of the block graph (rowptr/cols, rows

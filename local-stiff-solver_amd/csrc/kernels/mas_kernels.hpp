@@ -63,11 +63,13 @@ void dense_mirror(int64_t m, double *G, hipStream_t s);
 // z[p] += (S_c A S_c^T)^{-1} (r - A z)[p] = G^T G (r - A z)[p] over the stiff rows p of
 // each component, (A z) taken from the block operator in the permuted numbering.
 // k right-hand sides: r, z at stride ld; work, work2 hold k * ns values each.
+// With additive = true the correction is z[p] += G^T G r[p] (the MAS solver's
+// additive stiff preconditioner, MASSolver.cu:583-615).
 void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, const int64_t *mat_off,
                    const int32_t *row_comp, const int32_t *pdofs, const double *G, const int32_t *rowptr,
                    const int32_t *cols,
                    const double *bvals, const double *r, double *z, double *work, double *work2, const int *stop,
-                   hipStream_t s, int k = 1, int64_t ld = 0);
+                   hipStream_t s, int k = 1, int64_t ld = 0, bool additive = false);
 
 // u[p] = r[p] - (A z)[p] in FP64 for the stiff rows p: after a preconditioner-internal
 // SpMV with FP32 values, the rows that hold the stiff entries are recomputed exactly.

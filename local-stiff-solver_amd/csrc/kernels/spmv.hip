@@ -431,10 +431,10 @@ void spmv(const SellView &A, const double *x, double *y, const double *sub, hipS
 int spmv_partials_per_column(const SellView &A) { return div_up(A.nslices, kSpmvThreads / 64); }
 
 void spmv_pq(const SellView &A, const double *p, double *q, dev::PcgState *st, double *partials, int iter,
-             hipStream_t s, int k, int64_t ld)
+             hipStream_t s, int k, int64_t ld, int finish_op)
 {
     launch_spmv<0, true, false>(A, p, q, nullptr, &st->stop, k, ld, partials, s);
-    finish(spmv_partials_per_column(A), partials, dev::kFinishPq, st, nullptr, iter, s, k);
+    finish(spmv_partials_per_column(A), partials, finish_op, st, nullptr, iter, s, k);
 }
 
 void spmv_residual(const SellView &A, const double *x, const double *b, double *r, const int *stop, hipStream_t s,

@@ -142,13 +142,13 @@ __global__ __launch_bounds__(256) void k_stiff_residual(int64_t ns, int bs, cons
     const int lane = threadIdx.x & 63;
     if (k >= ns) return;
     r += blockIdx.y * ld;
-    z += blockIdx.y * ld;
     const int32_t p = pdofs[k];
     const int32_t I = p / bs;
     const int a = p - I * bs;
     const int bb = bs * bs;
     const int32_t e0 = rowptr[I];
-    const int32_t m = (rowptr[I + 1] - e0) * bs;
+    const int32_t m = z ? (rowptr[I + 1] - e0) * bs : 0;  // z == nullptr: gather r alone
+    if (z) z += blockIdx.y * ld;
     double acc = 0.0;
     for (int32_t q = lane; q < m; q += 64) {
         const int32_t e = e0 + q / bs;
@@ -388,11 +388,11 @@ void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, c
                    const int32_t *row_comp, const int32_t *pdofs, const double *G, const int32_t *rowptr,
                    const int32_t *cols,
                    const double *bvals, const double *r, double *z, double *work, double *work2, const int *stop,
-                   hipStream_t s, int k, int64_t ld)
+                   hipStream_t s, int k, int64_t ld, bool additive)
 {
     if (ncomp == 0 || ns == 0) return;
-    k_stiff_residual<false><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r, z, ld,
-                                                                  work, stop);
+    k_stiff_residual<false><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r,
+                                                                  additive ? nullptr : z, ld, work, stop);
     LSS_LAUNCH_CHECK();
     k_stiff_rows<false><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, row_comp, comp_off, mat_off, pdofs, G, work, work2,
                                                               z, ld, stop);

@@ -758,6 +758,202 @@ extern "C" void orc_mas_apply(const orc_mas *m, const double *r, double *z)
     (void)B;
 }
 
+// ===================================================================== MAS solver selection
+// MAS/BadDOFPreconditioner.cu, BadDofKMeanPreconditioner.cu, BadDofGMMPreconditioner.cu:
+// indicators over the partition-permuted block operator and the three selections.
+extern "C" void orc_mas_indicator(int mode, int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals,
+                                  int bs, const int32_t *perm, double *out)
+{
+    const int32_t nb = (n + bs - 1) / bs;
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int32_t I = 0; I < nb; ++I) {
+        for (int a = 0; a < bs; ++a) {
+            const int64_t pr = int64_t(perm[I]) * bs + a;
+            const int64_t i = int64_t(I) * bs + a;
+            if (i >= n) {  // padding row
+                out[pr] = 0.0;
+                continue;
+            }
+            // the row's entries in (permuted block column, component) order
+            std::vector<std::pair<int64_t, double>> e;
+            for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k)
+                e.emplace_back(int64_t(perm[idx[k] / bs]) * bs + idx[k] % bs, vals[k]);
+            std::sort(e.begin(), e.end(), [](const auto &l, const auto &r) { return l.first < r.first; });
+            double v = 0.0;
+            for (auto &[pc, val] : e) {
+                if (mode == 0) v += std::fabs(val);
+                else if (mode == 1) {
+                    if (pc == pr) v = std::fabs(val);
+                } else if (pc != pr) v = std::max(v, std::fabs(val));
+            }
+            out[pr] = v;
+        }
+    }
+}
+
+namespace {
+
+double rel_change(double now, double before) { return std::fabs(now - before) / std::max(std::fabs(before), 1e-300); }
+
+// union of the windows [p - e, p + e], ascending (BadDofGMMPreconditioner.cu:236-255)
+int64_t windows(const std::vector<int64_t> &pos, int64_t n, int e, int64_t *out)
+{
+    if (e <= 0) {
+        std::vector<int64_t> s(pos);
+        std::sort(s.begin(), s.end());
+        std::copy(s.begin(), s.end(), out);
+        return int64_t(s.size());
+    }
+    std::vector<uint8_t> mark(size_t(n), 0);
+    for (int64_t p : pos)
+        for (int64_t j = std::max<int64_t>(0, p - e); j <= std::min<int64_t>(n - 1, p + e); ++j) mark[j] = 1;
+    int64_t w = 0;
+    for (int64_t j = 0; j < n; ++j)
+        if (mark[j]) out[w++] = j;
+    return w;
+}
+
+}  // namespace
+
+extern "C" int64_t orc_mas_gmm_positions(int64_t n, const double *x, double tol, int max_iter, double jump, int expand,
+                                         int64_t *out, double *stats)
+{
+#pragma STDC FP_CONTRACT OFF
+    // select_bad_dofs_from_l1_row_norm_gmm, BadDofGMMPreconditioner.cu:86-258, sums in
+    // the canonical tree order of the device
+    for (int k = 0; k < 8; ++k) stats[k] = 0.0;
+    if (n < 2) return 0;
+    const double gmean = canonical_sum(n, [x](int64_t i) { return x[i]; }) / n;
+    const double gvar = canonical_sum(n, [x, gmean](int64_t i) {
+                            const double d = x[i] - gmean;
+                            return d * d;
+                        }) / n;
+    double mean_0 = *std::min_element(x, x + n), mean_1 = *std::max_element(x, x + n);
+    const double var_reg = 1e-6;
+    double var_0 = std::max(gvar, var_reg), var_1 = std::max(gvar, var_reg), w0 = 0.5, w1 = 0.5;
+    auto put = [&](double ratio, int it) {
+        stats[0] = mean_0, stats[1] = mean_1, stats[2] = var_0, stats[3] = var_1;
+        stats[4] = gmean, stats[5] = gvar, stats[6] = ratio, stats[7] = it;
+    };
+    put(0.0, 0);
+    if (mean_0 == mean_1 || !std::isfinite(gvar)) return 0;
+    std::vector<double> g0(n), g1(n);
+    const int iterations = std::max(1, max_iter);
+    int it = 0;
+    for (; it < iterations; ++it) {
+        const double lw0 = std::log(std::max(w0, std::numeric_limits<double>::min()));
+        const double lw1 = std::log(std::max(w1, std::numeric_limits<double>::min()));
+        const double c0 = -0.5 * std::log(2.0 * M_PI * var_0);
+        const double c1 = -0.5 * std::log(2.0 * M_PI * var_1);
+#pragma omp parallel for schedule(static, 4096)
+        for (int64_t i = 0; i < n; ++i) {
+            const double xi = x[i];
+            const double lg0 = lw0 + (c0 - 0.5 * (xi - mean_0) * (xi - mean_0) / var_0);
+            const double lg1 = lw1 + (c1 - 0.5 * (xi - mean_1) * (xi - mean_1) / var_1);
+            const double mx = std::max(lg0, lg1);
+            const double lt = mx + std::log(std::exp(lg0 - mx) + std::exp(lg1 - mx));
+            g0[i] = std::exp(lg0 - lt);
+            g1[i] = std::exp(lg1 - lt);
+        }
+        const double s0 = canonical_sum(n, [&](int64_t i) { return g0[i]; });
+        const double s1 = canonical_sum(n, [&](int64_t i) { return g1[i]; });
+        if (s0 <= 0.0 || s1 <= 0.0) break;
+        w0 = s0 / n;
+        w1 = s1 / n;
+        const double om0 = mean_0, om1 = mean_1, ov0 = var_0, ov1 = var_1;
+        mean_0 = canonical_sum(n, [&](int64_t i) { return g0[i] * x[i]; }) / s0;
+        mean_1 = canonical_sum(n, [&](int64_t i) { return g1[i] * x[i]; }) / s1;
+        const double m0 = mean_0, m1 = mean_1;
+        var_0 = canonical_sum(n, [&](int64_t i) { return g0[i] * (x[i] - m0) * (x[i] - m0); }) / s0 + var_reg;
+        var_1 = canonical_sum(n, [&](int64_t i) { return g1[i] * (x[i] - m1) * (x[i] - m1); }) / s1 + var_reg;
+        if (rel_change(mean_0, om0) < tol && rel_change(mean_1, om1) < tol && rel_change(var_0, ov0) < tol &&
+            rel_change(var_1, ov1) < tol)
+            break;
+    }
+    const int high = std::fabs(mean_0) > std::fabs(mean_1) ? 0 : 1;
+    const double hm = high == 0 ? mean_0 : mean_1, lm = high == 0 ? mean_1 : mean_0;
+    const double ratio = std::fabs(hm) / std::fabs(lm);
+    put(ratio, it);
+    if (std::isnan(ratio) || ratio < jump) return 0;
+    std::vector<int64_t> pos;
+    for (int64_t i = 0; i < n; ++i)
+        if (high == 0 ? g0[i] > g1[i] : g1[i] > g0[i]) pos.push_back(i);
+    return windows(pos, n, expand, out);
+}
+
+// positions sorted by value (stable: equal values keep ascending positions)
+static std::vector<int64_t> sorted_positions(int64_t n, const double *x, bool descending)
+{
+    std::vector<int64_t> o(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) o[i] = i;
+    std::stable_sort(o.begin(), o.end(),
+                     [&](int64_t a, int64_t b) { return descending ? x[a] > x[b] : x[a] < x[b]; });
+    return o;
+}
+
+extern "C" int64_t orc_mas_elbow_positions(int64_t n, const double *x, int64_t *out, int64_t *cutoff_out)
+{
+#pragma STDC FP_CONTRACT OFF
+    // select_bad_dofs_from_diagonal, BadDOFPreconditioner.cu:61-100
+    if (n == 0) return 0;
+    const auto o = sorted_positions(n, x, false);
+    int64_t cutoff = 0;
+    if (n > 1) {
+        std::vector<double> lg(static_cast<size_t>(n));
+        for (int64_t i = 0; i < n; ++i) lg[i] = std::log(x[o[i]]);
+        const double mn = lg.front(), mx = lg.back();
+        double best = -1.0;
+        for (int64_t i = 0; i < n; ++i) {
+            const double expected = (mx - mn) / (double(n) - 1.0) * double(i) + mn;
+            const double dev = std::fabs(expected - lg[i]);
+            if (dev > best) {
+                best = dev;
+                cutoff = i;
+            }
+        }
+    }
+    if (cutoff_out) *cutoff_out = cutoff;
+    std::vector<int64_t> pos(o.begin() + cutoff, o.end());
+    return windows(pos, n, 0, out);
+}
+
+extern "C" int64_t orc_mas_kmeans_positions(int64_t n, const double *x, double fraction, double jump, int max_iter,
+                                            int expand, int64_t *out, double *stats)
+{
+#pragma STDC FP_CONTRACT OFF
+    // select_bad_dofs_from_diagonal_kmeans, BadDofKMeanPreconditioner.cu:63-172
+    for (int k = 0; k < 4; ++k) stats[k] = 0.0;
+    if (n < 2) return 0;
+    const auto o = sorted_positions(n, x, true);
+    const int64_t sub = std::clamp<int64_t>(int64_t(double(n) * fraction), 2, n);
+    double c[2] = {x[o[0]], x[o[sub - 1]]};
+    int it = 0;
+    for (; it < max_iter; ++it) {
+        double sum[2] = {0.0, 0.0};
+        int64_t cnt[2] = {0, 0};
+        for (int64_t i = 0; i < sub; ++i) {
+            const double v = x[o[i]];
+            const int cid = std::fabs(v - c[0]) < std::fabs(v - c[1]) ? 1 : 0;
+            cnt[cid]++;
+            sum[cid] += v;
+        }
+        const double n0 = cnt[0] > 0 ? sum[0] / double(cnt[0]) : c[0];
+        const double n1 = cnt[1] > 1 ? sum[1] / double(cnt[1]) : c[1];
+        if (n0 == c[0] && n1 == c[1]) break;
+        c[0] = n0;
+        c[1] = n1;
+    }
+    if (c[0] < c[1]) std::swap(c[0], c[1]);
+    const double ratio = c[0] / c[1];
+    stats[0] = c[0], stats[1] = c[1], stats[2] = ratio, stats[3] = it;
+    if (std::isnan(ratio) || ratio < jump) return 0;
+    const double sep = 0.5 * (c[0] + c[1]);
+    std::vector<int64_t> pos;
+    for (int64_t i = 0; i < sub; ++i)
+        if (x[o[i]] > sep) pos.push_back(o[i]);
+    return windows(pos, n, expand, out);
+}
+
 // ===================================================================== hybrid
 // damping after a breakdown of the damped sweep
 constexpr double kSafeDamping = 1.5;
@@ -780,6 +976,8 @@ struct orc_hybrid {
     bool damping_retry = false;
     int iters = 0;
     double final_res = 0.0, t_setup = 0.0, t_solve = 0.0;
+    int mas_status = 0;  // MAS mode: 1 relative, 2 absolute tolerance, 3 max iterations
+    double mas_residual = 0.0;
     ~orc_hybrid() { orc_mas_destroy(mas); }
 
     void base_apply_raw(const double *r, double *z) const
@@ -814,10 +1012,43 @@ struct orc_hybrid {
             for (size_t r = 0; r < cc.size(); ++r) out[cc[r]] = x[r];
         }
     }
+    // dense sub-blocks S_c A S_c^T (GPUHybrid.cu:1351-1383 semantics) + Cholesky
+    bool factor_comps()
+    {
+        chol.resize(comps.size());
+        bool ok = true;
+#pragma omp parallel for schedule(dynamic, 1) reduction(&& : ok)
+        for (size_t k = 0; k < comps.size(); ++k) {
+            const auto &cc = comps[k];
+            const int msz = int(cc.size());
+            std::unordered_map<int32_t, int> loc;
+            for (int r = 0; r < msz; ++r) loc[cc[r]] = r;
+            std::vector<double> D(size_t(msz) * msz, 0.0);
+            for (int r = 0; r < msz; ++r)
+                for (int32_t e = A.ptr[cc[r]]; e < A.ptr[cc[r] + 1]; ++e) {
+                    auto it = loc.find(A.idx[e]);
+                    if (it != loc.end()) D[size_t(r) * msz + it->second] = A.vals[e];
+                }
+            if (!cholesky(msz, D)) ok = false;
+            chol[k] = std::move(D);
+        }
+        return ok;
+    }
     // custom_mixed_precond_iter, GPUHybrid.cu:521-542
     void precond(const double *r, double *z, std::vector<double> &buffer, std::vector<double> &z2) const
     {
         const int32_t n = A.n;
+        if (cfg.solver_mode == 1) {
+            // MASSolverImpl::apply_preconditioner, MASSolver.cu:583-615: z = B r + S^T D^{-1} S r
+            base_apply_raw(r, z);
+            if (!comps.empty()) {
+                std::vector<double> pr(n, 0.0);
+                stiff_apply(r, pr.data());
+#pragma omp parallel for schedule(static)
+                for (int32_t i = 0; i < n; ++i) z[i] += pr[i];
+            }
+            return;
+        }
         if (!cfg.do_hybrid_precond || comps.empty()) {
             base_apply(r, z);
             return;
@@ -963,7 +1194,7 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
     h->comps.clear();
     h->chol.clear();
     auto tp = clk::now();
-    if (c.do_hybrid_precond) {
+    if (c.solver_mode == 0 && c.do_hybrid_precond) {
         // GPUHybrid.cu:214-264
         std::vector<int32_t> S;
         if (c.select_bad_dofs_from_l1_norm) {
@@ -1001,24 +1232,7 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
         }
         phase("select", tp);
         // dense sub-blocks S_c A S_c^T (GPUHybrid.cu:1351-1383 semantics) + Cholesky
-        h->chol.resize(h->comps.size());
-        bool ok = true;
-#pragma omp parallel for schedule(dynamic, 1) reduction(&& : ok)
-        for (size_t k = 0; k < h->comps.size(); ++k) {
-            const auto &cc = h->comps[k];
-            const int msz = int(cc.size());
-            std::unordered_map<int32_t, int> loc;
-            for (int r = 0; r < msz; ++r) loc[cc[r]] = r;
-            std::vector<double> D(size_t(msz) * msz, 0.0);
-            for (int r = 0; r < msz; ++r)
-                for (int32_t e = A.ptr[cc[r]]; e < A.ptr[cc[r] + 1]; ++e) {
-                    auto it = loc.find(A.idx[e]);
-                    if (it != loc.end()) D[size_t(r) * msz + it->second] = A.vals[e];
-                }
-            if (!cholesky(msz, D)) ok = false;
-            h->chol[k] = std::move(D);
-        }
-        if (!ok) return -5;
+        if (!h->factor_comps()) return -5;
         phase("stiff", tp);
     }
     // base preconditioner
@@ -1063,6 +1277,54 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
         h->mas = orc_mas_create(n, A.ptr, A.idx, A.vals, bs, h->perm.data(), h->part_off.data(), np);
         if (!h->mas) return -5;
         phase("mas", tp);
+        if (c.solver_mode == 1 && c.use_bad_dof) {
+            // MAS solver: the selected positions over the permuted operator (MASSolver.cu:398-432)
+            const int64_t npos = int64_t(nb) * bs;
+            std::vector<double> ind(npos);
+            const int mode = c.bad_dof_strategy == 2 ? 0 : (c.bad_dof_strategy == 1 ? 2 : 1);
+            orc_mas_indicator(mode, n, A.ptr, A.idx, A.vals, bs, h->perm.data(), ind.data());
+            // positions: non-padding permuted scalars in order
+            std::vector<int64_t> pos2p;
+            std::vector<double> x;
+            const int32_t pad_block = h->perm[nb - 1];
+            const int npad = int(npos - n);
+            for (int64_t p = 0; p < npos; ++p) {
+                if (npad && p / bs == pad_block && p % bs >= bs - npad) continue;
+                pos2p.push_back(p);
+                x.push_back(ind[p]);
+            }
+            const int64_t m = int64_t(x.size());
+            std::vector<int64_t> sel(m);
+            double st[8];
+            int64_t cnt = 0;
+            if (c.bad_dof_strategy == 2)
+                cnt = orc_mas_gmm_positions(m, x.data(), c.mgmm_tol, c.mgmm_max_iter, c.mgmm_jump, c.mgmm_expand,
+                                            sel.data(), st);
+            else if (c.bad_dof_strategy == 1)
+                cnt = orc_mas_kmeans_positions(m, x.data(), c.km_fraction, c.km_jump, c.km_max_iter, c.km_expand,
+                                               sel.data(), st);
+            else
+                cnt = orc_mas_elbow_positions(m, x.data(), sel.data(), nullptr);
+            std::vector<int32_t> inv(nb);
+            for (int32_t I = 0; I < nb; ++I) inv[h->perm[I]] = I;
+            std::vector<int32_t> S;
+            for (int64_t k = 0; k < cnt; ++k) {
+                const int64_t p = pos2p[sel[k]];
+                S.push_back(int32_t(int64_t(inv[p / bs]) * bs + p % bs));
+            }
+            std::sort(S.begin(), S.end());
+            if (!S.empty()) {
+                std::vector<int32_t> dofs(S.size());
+                std::vector<int64_t> off(S.size() + 1);
+                int64_t nc = 0;
+                orc_decompose(n, A.ptr, A.idx, S.data(), int64_t(S.size()), std::numeric_limits<int64_t>::max(),
+                              dofs.data(), off.data(), &nc);
+                for (int64_t k = 0; k < nc; ++k)
+                    h->comps.emplace_back(dofs.begin() + off[k], dofs.begin() + off[k + 1]);
+            }
+            if (!h->factor_comps()) return -5;
+            phase("mas_bad_dof", tp);
+        }
     } else if (c.base_precond == 1) {
         h->jacobi.assign(n, 1.0);
         for (int32_t i = 0; i < n; ++i)
@@ -1075,7 +1337,7 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
     // Ritz value of B A after kLanczosSteps steps of B-preconditioned CG on A x = v
     // (v a hashed vector), which never exceeds lambda_max(BA) and approaches it fast.
     h->omega = 1.0;
-    if (c.do_hybrid_precond && !h->comps.empty() && c.composition == 0) {
+    if (c.solver_mode == 0 && c.do_hybrid_precond && !h->comps.empty() && c.composition == 0) {
         const double theta = lanczos_lambda_max(*h, n);
         h->theta = theta;
         h->omega = (c.damping > 0 ? c.damping : 1.0) / theta;
@@ -1085,8 +1347,72 @@ extern "C" int orc_hybrid_factorize(orc_hybrid *h, int32_t n, const int32_t *ptr
     return 0;
 }
 
+// MASSolverImpl::pcg_solve, MASSolver.cu:618-744 (x0 = 0, MASSolver.cu:500-504)
+static int mas_solve(orc_hybrid *h, const double *b, double *xio)
+{
+    auto t0 = clk::now();
+    const Csr &A = h->A;
+    const int32_t n = A.n;
+    const orc_config &c = h->cfg;
+    std::vector<double> x(n, 0.0), r(n), p(n), z(n), Ap(n), buffer(n), z2(n);
+    auto div = [](double num, double den) { return std::fabs(den) < 1e-20 ? 0.0 : num / den; };
+    spmv(A, x.data(), r.data());
+    for (int32_t i = 0; i < n; ++i) r[i] = 1.0 * b[i] + -1.0 * r[i];
+    h->precond(r.data(), z.data(), buffer, z2);
+    p = z;
+    double rz = dot(n, r.data(), z.data());
+    if (std::isnan(rz) || !std::isfinite(rz)) return -5;
+    const double rr0 = c.precond_norm ? rz : dot(n, r.data(), r.data());
+    const double rtol2 = c.rel_tol * c.rel_tol, atol2 = c.abs_tol * c.abs_tol;
+    h->mas_status = 0;
+    h->mas_residual = 0.0;
+    int iters = 0;
+    for (int k = 1; k <= c.max_iter; ++k) {
+        spmv(A, p.data(), Ap.data());
+        const double alpha = div(rz, dot(n, p.data(), Ap.data()));
+        for (int32_t i = 0; i < n; ++i) x[i] = alpha * p[i] + 1.0 * x[i];
+        if (k % c.true_residual_period == 0) {
+            spmv(A, x.data(), r.data());
+            for (int32_t i = 0; i < n; ++i) r[i] = 1.0 * b[i] + -1.0 * r[i];
+        } else {
+            for (int32_t i = 0; i < n; ++i) r[i] = -alpha * Ap[i] + 1.0 * r[i];
+        }
+        h->precond(r.data(), z.data(), buffer, z2);
+        const double rz_old = rz;
+        rz = dot(n, r.data(), z.data());
+        iters = k;
+        bool converged = false;
+        if (k % 10 == 0) {
+            const double rr = c.precond_norm ? rz : dot(n, r.data(), r.data());
+            h->mas_residual = std::sqrt(rr);
+            if (rr <= rtol2 * rr0 || rr <= atol2) {
+                h->mas_status = rr <= atol2 ? 2 : 1;
+                converged = true;
+            }
+        }
+        if (converged) break;
+        const double beta = div(rz, rz_old);
+        for (int32_t i = 0; i < n; ++i) p[i] = 1.0 * z[i] + beta * p[i];
+    }
+    if (iters == c.max_iter) h->mas_status = 3;
+    spmv(A, x.data(), buffer.data());
+    for (int32_t i = 0; i < n; ++i) buffer[i] -= b[i];
+    h->final_res = std::sqrt(dot(n, buffer.data(), buffer.data()));
+    h->iters = iters;
+    std::copy(x.begin(), x.end(), xio);
+    h->t_solve = since(t0);
+    return 0;
+}
+
+extern "C" int orc_hybrid_mas_status(const orc_hybrid *h, double *residual)
+{
+    if (residual) *residual = h->mas_residual;
+    return h->mas_status;
+}
+
 extern "C" int orc_hybrid_solve(orc_hybrid *h, const double *b, double *xio)
 {
+    if (h->cfg.solver_mode == 1) return mas_solve(h, b, xio);
     // GPUHybrid::pcg_solve, GPUHybrid.cu:1413-1512
     auto t0 = clk::now();
     const Csr &A = h->A;

@@ -105,6 +105,24 @@ int orc_mas_inverse(const orc_mas *m, int level, int32_t bank, double *out);
 /* z = M_MAS^{-1} r in the original numbering. */
 void orc_mas_apply(const orc_mas *m, const double *r, double *z);
 
+typedef struct orc_hybrid orc_hybrid;
+/* ---- MAS solver stiff-DOF selection (MAS/BadDOF*Preconditioner.cu) ------- */
+/* Indicator per permuted scalar row (perm[old block] = new block), in the
+ * reference's loop order over the permuted block row: mode 0 sum |a_ij|
+ * (BadDofGMMPreconditioner.cu:273-306), 1 |a_ii| (BadDOFPreconditioner.cu:117-143),
+ * 2 max |a_ij|, j != i (BadDofKMeanPreconditioner.cu:187-222).  out: nb*bs values,
+ * padding rows 0. */
+void orc_mas_indicator(int mode, int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, int bs,
+                       const int32_t *perm, double *out);
+/* Selected positions (ascending) of each strategy over n indicator values. */
+int64_t orc_mas_gmm_positions(int64_t n, const double *x, double tol, int max_iter, double jump, int expand,
+                              int64_t *out, double *stats /* mean0 mean1 var0 var1 gmean gvar ratio iters */);
+int64_t orc_mas_elbow_positions(int64_t n, const double *x, int64_t *out, int64_t *cutoff);
+int64_t orc_mas_kmeans_positions(int64_t n, const double *x, double fraction, double jump, int max_iter, int expand,
+                                 int64_t *out, double *stats /* c0 c1 ratio iters */);
+/* Solver status of the last MAS-mode solve: 1 relative, 2 absolute tolerance, 3 max iterations. */
+int orc_hybrid_mas_status(const orc_hybrid *h, double *residual);
+
 /* ---- hybrid solver (factorize + PCG) ------------------------------------- */
 typedef struct orc_config {
     int max_iter;
@@ -122,6 +140,16 @@ typedef struct orc_config {
     int threads;        /* OpenMP threads (0 = runtime default) */
     double damping;     /* omega = damping / lambda_max(BA) in the B-P-B sweep (0 -> 1) */
     int composition;    /* 0 = B-P-B (GPUHybrid.cu:521-542), 1 = P + (I-PA) B (I-AP), 2 = B + P */
+    /* the standalone MAS solver (MASSolver.cu): solver_mode 1 */
+    int solver_mode;            /* 0 = GPUHybrid, 1 = MAS */
+    int use_bad_dof;            /* MAS: additive stiff correction */
+    int bad_dof_strategy;       /* 0 = elbow, 1 = k-means, 2 = GMM */
+    double mgmm_jump, mgmm_tol; /* GMM strategy (defaults 10, 1e-3) */
+    int mgmm_max_iter, mgmm_expand;  /* (5, 32) */
+    double km_fraction, km_jump;     /* k-means strategy (0.02, 10) */
+    int km_max_iter, km_expand;      /* (32, 0) */
+    int true_residual_period;        /* 4 */
+    int precond_norm;                /* use_preconditioned_residual_norm */
 } orc_config;
 
 typedef struct orc_hybrid orc_hybrid;

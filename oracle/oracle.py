@@ -37,6 +37,11 @@ class Config(ctypes.Structure):
         ("gmm_tol", ctypes.c_double), ("max_gmm_iterations", ctypes.c_int),
         ("conditioning_threshold", ctypes.c_double), ("base_precond", ctypes.c_int),
         ("threads", ctypes.c_int), ("damping", ctypes.c_double), ("composition", ctypes.c_int),
+        ("solver_mode", ctypes.c_int), ("use_bad_dof", ctypes.c_int), ("bad_dof_strategy", ctypes.c_int),
+        ("mgmm_jump", ctypes.c_double), ("mgmm_tol", ctypes.c_double), ("mgmm_max_iter", ctypes.c_int),
+        ("mgmm_expand", ctypes.c_int), ("km_fraction", ctypes.c_double), ("km_jump", ctypes.c_double),
+        ("km_max_iter", ctypes.c_int), ("km_expand", ctypes.c_int), ("true_residual_period", ctypes.c_int),
+        ("precond_norm", ctypes.c_int),
     ]
 
 
@@ -78,6 +83,17 @@ def lib():
         L.orc_hybrid_stiff.argtypes = [P, P, ctypes.c_int64, P, ctypes.c_int64]
         L.orc_hybrid_precond.argtypes = [P, P, P]
         L.orc_hybrid_set_partition.argtypes = [P, P, ctypes.c_int32, P, ctypes.c_int32]
+        L.orc_mas_indicator.argtypes = [ctypes.c_int, ctypes.c_int32, P, P, P, ctypes.c_int, P, P]
+        L.orc_mas_gmm_positions.restype = ctypes.c_int64
+        L.orc_mas_gmm_positions.argtypes = [ctypes.c_int64, P, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_int, P, P]
+        L.orc_mas_elbow_positions.restype = ctypes.c_int64
+        L.orc_mas_elbow_positions.argtypes = [ctypes.c_int64, P, P, P]
+        L.orc_mas_kmeans_positions.restype = ctypes.c_int64
+        L.orc_mas_kmeans_positions.argtypes = [ctypes.c_int64, P, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_int, P, P]
+        L.orc_hybrid_mas_status.restype = ctypes.c_int
+        L.orc_hybrid_mas_status.argtypes = [P, P]
         L.orc_hybrid_omega.restype = ctypes.c_double
         L.orc_hybrid_omega.argtypes = [P]
         L.orc_hybrid_damping_retry.argtypes = [P]
@@ -230,7 +246,55 @@ def config(block_size=1, rel_tol=1e-8, abs_tol=0.0, max_iter=10000, base="mas", 
     c.threads = threads
     c.damping = kw.get("damping", 1.8)
     c.composition = {"BPB": 0, "PBP": 1, "additive": 2}[kw.get("composition", "BPB")]
+    # the standalone MAS solver (MASSolver.cu) and its stiff-DOF strategies
+    c.solver_mode = {"GPUHybrid": 0, "MAS": 1}[kw.get("solver", "GPUHybrid")]
+    c.use_bad_dof = int(kw.get("use_bad_dof_precond", False))
+    c.bad_dof_strategy = {"elbow": 0, "kmean": 1, "gmm": 2}[kw.get("bad_dof_strategy", "elbow")]
+    c.mgmm_jump = kw.get("bad_dof_gmm_jump_threshold", 10.0)
+    c.mgmm_tol = kw.get("bad_dof_gmm_tol", 1e-3)
+    c.mgmm_max_iter = kw.get("bad_dof_gmm_max_iterations", 5)
+    c.mgmm_expand = kw.get("bad_dof_gmm_expand_neighbors", 32)
+    c.km_fraction = kw.get("bad_dof_kmeans_search_fraction", 0.02)
+    c.km_jump = kw.get("bad_dof_kmeans_jump_threshold", 10.0)
+    c.km_max_iter = kw.get("bad_dof_kmeans_max_iterations", 32)
+    c.km_expand = kw.get("bad_dof_kmeans_expand_neighbors", 0)
+    c.true_residual_period = kw.get("true_residual_period", 4)
+    c.precond_norm = int(kw.get("use_preconditioned_residual_norm", False))
     return c
+
+
+def mas_indicator(mode, A_colptr, A_rowidx, A_vals, bs, perm):
+    """Per permuted scalar row: mode 0 sum |a_ij|, 1 |a_ii|, 2 max |a_ij| (j != i)."""
+    ptr, idx, vals, perm = _i32(A_colptr), _i32(A_rowidx), _f64(A_vals), _i32(perm)
+    n = ptr.size - 1
+    out = np.empty(perm.size * bs, np.float64)
+    lib().orc_mas_indicator(mode, n, _p(ptr), _p(idx), _p(vals), bs, _p(perm), _p(out))
+    return out
+
+
+def mas_gmm_positions(x, tol=1e-3, max_iter=5, jump=10.0, expand=32):
+    x = _f64(x)
+    out = np.empty(max(x.size, 1), np.int64)
+    st = np.zeros(8)
+    k = lib().orc_mas_gmm_positions(x.size, _p(x), tol, max_iter, jump, expand, _p(out), _p(st))
+    keys = ("mean0", "mean1", "var0", "var1", "global_mean", "global_var", "ratio", "iters")
+    return out[:k].copy(), dict(zip(keys, st))
+
+
+def mas_elbow_positions(x):
+    x = _f64(x)
+    out = np.empty(max(x.size, 1), np.int64)
+    cut = np.zeros(1, np.int64)
+    k = lib().orc_mas_elbow_positions(x.size, _p(x), _p(out), _p(cut))
+    return out[:k].copy(), int(cut[0])
+
+
+def mas_kmeans_positions(x, fraction=0.02, jump=10.0, max_iter=32, expand=0):
+    x = _f64(x)
+    out = np.empty(max(x.size, 1), np.int64)
+    st = np.zeros(4)
+    k = lib().orc_mas_kmeans_positions(x.size, _p(x), fraction, jump, max_iter, expand, _p(out), _p(st))
+    return out[:k].copy(), dict(zip(("c0", "c1", "ratio", "iters"), st))
 
 
 class Hybrid:
@@ -282,6 +346,7 @@ class Hybrid:
         return {"num_iterations": it.value, "final_res_norm": res.value, "t_setup": ts.value,
                 "t_solve": tv.value, "n_stiff": ns.value, "n_components": nc.value,
                 "omega": lib().orc_hybrid_omega(self._h),
+                "mas_status": lib().orc_hybrid_mas_status(self._h, None),
                 "damping_retry": bool(lib().orc_hybrid_damping_retry(self._h))}
 
     def stiff_components(self):

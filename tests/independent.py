@@ -155,3 +155,105 @@ def fsum_bound(values) -> tuple[float, float]:
     """Exact sum and a first-order bound for any summation order."""
     v = np.asarray(values, np.float64)
     return math.fsum(v.tolist()), float(np.abs(v).sum()) * v.size * np.finfo(np.float64).eps
+
+
+# ---------------------------------------------------------------- MAS solver selection
+def permuted_rows(R: sp.csr_matrix, bs: int, perm):
+    """Operator R renumbered so that node I becomes perm[I] (scalar p = perm[I]*bs + a)."""
+    n = R.shape[0]
+    perm = np.asarray(perm, np.int64)
+    sp_perm = (perm[np.arange(n) // bs] * bs + np.arange(n) % bs)
+    P = sp.csr_matrix((np.ones(n), (sp_perm, np.arange(n))), shape=(n, n))
+    return (P @ R @ P.T).tocsr(), sp_perm
+
+
+def gmm_mas_numpy(x, tol=1e-3, max_iter=5, jump=10.0, expand=32):
+    """select_bad_dofs_from_l1_row_norm_gmm (BadDofGMMPreconditioner.cu:86-258), vectorized numpy."""
+    x = np.asarray(x, np.float64)
+    n = x.size
+    if n < 2:
+        return np.zeros(0, np.int64), {}
+    gm = x.sum() / n
+    gv = ((x - gm) ** 2).sum() / n
+    m = np.array([x.min(), x.max()])
+    v = np.array([max(gv, 1e-6)] * 2)
+    w = np.array([0.5, 0.5])
+    if m[0] == m[1] or not np.isfinite(gv):
+        return np.zeros(0, np.int64), {}
+    it, g = 0, None
+    while it < max(1, max_iter):
+        logp = (np.log(np.maximum(w, np.finfo(float).tiny))[:, None] - 0.5 * np.log(2 * np.pi * v)[:, None]
+                - 0.5 * (x[None, :] - m[:, None]) ** 2 / v[:, None])
+        mx = logp.max(0)
+        g = np.exp(logp - (mx + np.log(np.exp(logp - mx).sum(0))))
+        sg = g.sum(1)
+        if sg[0] <= 0 or sg[1] <= 0:
+            break
+        w = sg / n
+        om, ov = m.copy(), v.copy()
+        m = (g * x).sum(1) / sg
+        v = (g * (x[None, :] - m[:, None]) ** 2).sum(1) / sg + 1e-6
+        rc = lambda a, b: np.abs(a - b) / np.maximum(np.abs(b), 1e-300)  # noqa: E731
+        if np.all(rc(m, om) < tol) and np.all(rc(v, ov) < tol):
+            break
+        it += 1
+    hi = 0 if abs(m[0]) > abs(m[1]) else 1
+    ratio = abs(m[hi]) / abs(m[1 - hi])
+    st = {"mean0": m[0], "mean1": m[1], "var0": v[0], "var1": v[1], "ratio": ratio, "iters": it}
+    if np.isnan(ratio) or ratio < jump:
+        return np.zeros(0, np.int64), st
+    sel = np.flatnonzero(g[hi] > g[1 - hi])
+    return window_union(sel, n, expand), st
+
+
+def window_union(sel, n, e):
+    """Positions within e of a selected position (BadDofGMMPreconditioner.cu:236-255)."""
+    sel = np.asarray(sel, np.int64)
+    if e <= 0 or sel.size == 0:
+        return np.sort(sel)
+    mark = np.zeros(n + 1, np.int64)
+    np.add.at(mark, np.maximum(sel - e, 0), 1)
+    np.add.at(mark, np.minimum(sel + e, n - 1) + 1, -1)
+    return np.flatnonzero(np.cumsum(mark)[:n] > 0)
+
+
+def elbow_numpy(x):
+    """select_bad_dofs_from_diagonal (BadDOFPreconditioner.cu:61-100); ties by position."""
+    x = np.asarray(x, np.float64)
+    n = x.size
+    o = np.argsort(x, kind="stable")
+    if n < 2:
+        return np.sort(o), 0
+    lg = np.log(x[o])
+    expected = (lg[-1] - lg[0]) / (n - 1.0) * np.arange(n) + lg[0]
+    cut = int(np.argmax(np.abs(expected - lg)))  # first maximum
+    return np.sort(o[cut:]), cut
+
+
+def kmeans_numpy(x, fraction=0.02, jump=10.0, max_iter=32, expand=0):
+    """select_bad_dofs_from_diagonal_kmeans (BadDofKMeanPreconditioner.cu:63-172)."""
+    x = np.asarray(x, np.float64)
+    n = x.size
+    if n < 2:
+        return np.zeros(0, np.int64), {}
+    o = np.argsort(-x, kind="stable")
+    sub = int(np.clip(int(n * fraction), 2, n))
+    v = x[o[:sub]]
+    c = np.array([v[0], v[-1]])
+    it = 0
+    while it < max_iter:
+        cid = (np.abs(v - c[0]) < np.abs(v - c[1])).astype(int)
+        cnt = np.bincount(cid, minlength=2)
+        s = np.array([v[cid == 0].sum(), v[cid == 1].sum()])
+        nxt = np.array([s[0] / cnt[0] if cnt[0] > 0 else c[0], s[1] / cnt[1] if cnt[1] > 1 else c[1]])
+        if np.array_equal(nxt, c):
+            break
+        c = nxt
+        it += 1
+    c = np.sort(c)[::-1]
+    ratio = c[0] / c[1]
+    st = {"c0": c[0], "c1": c[1], "ratio": ratio, "iters": it}
+    if np.isnan(ratio) or ratio < jump:
+        return np.zeros(0, np.int64), st
+    sel = o[:sub][v > 0.5 * (c[0] + c[1])]
+    return window_union(sel, n, expand), st

@@ -1,0 +1,93 @@
+"""The standalone MAS solver (SRC/MASSolver.cu) on the GPU against the oracle:
+the stiff set of each strategy, the additive preconditioner and the PCG loop."""
+import numpy as np
+import pytest
+
+from lss_amd import problems
+from lss_amd.solver import Solver
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev_cfg(bs, **kw):
+    m = {"block_dim": bs}
+    m.update(kw)
+    return {"solver": "MAS", "MAS": m}
+
+
+def _orc_cfg(bs, **kw):
+    base = dict(block_size=bs, solver="MAS", rel_tol=kw.pop("relative_tolerance", 1e-6), abs_tol=1e-20,
+                max_iter=kw.pop("max_iter", 100000))
+    base.update(kw)
+    return O.config(**base)
+
+
+CASES = [("elastic_contact", 10), ("laplace_sphere", 12), ("elastic_logE", 8)]
+STRATEGIES = [{}, {"use_bad_dof_precond": True, "bad_dof_strategy": "gmm"},
+              {"use_bad_dof_precond": True, "bad_dof_strategy": "elbow"},
+              {"use_bad_dof_precond": True, "bad_dof_strategy": "kmean", "bad_dof_kmeans_expand_neighbors": 2}]
+
+
+@pytest.mark.parametrize("kw", STRATEGIES)
+@pytest.mark.parametrize("name,N", CASES)
+def test_mas_solver_matches_oracle(gpu, name, N, kw):
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    bs = prob.block_size
+    s = Solver.create(_dev_cfg(bs, **kw))
+    assert s.name() == "MAS"
+    s.factorize(A)
+    H = O.Hybrid(_orc_cfg(bs, **kw))
+    H.factorize(A.colptr, A.rowidx, A.vals)
+    dc, oc = s.stiff_components(), H.stiff_components()
+    assert len(dc) == len(oc)
+    for a, b in zip(dc, oc):
+        assert np.array_equal(a, b)
+    b = prob.rhs(2)
+    x = s.solve(b, np.ones(A.n))  # the initial guess is ignored (x0 = 0)
+    info = s.get_info()
+    xr = H.solve(b)
+    oi = H.info()
+    assert info["solver_status"] == "Reach relative tolerance" and oi["mas_status"] == 1
+    assert info["solver_iter"] == info["num_iterations"]
+    assert abs(info["solver_iter"] - oi["num_iterations"]) <= 10  # both test every 10 iterations
+    S = A.to_scipy()
+    assert np.linalg.norm(S @ x - b) <= 2e-6 * np.linalg.norm(b)
+    assert np.linalg.norm(x - xr) <= 1e-4 * np.linalg.norm(xr)
+    if kw.get("bad_dof_strategy") == "gmm":
+        st = info["bad_dof"]
+        perm, _ = H.partition()
+        g = O.mas_indicator(0, A.colptr, A.rowidx, A.vals, bs, perm)
+        _, ost = O.mas_gmm_positions(g)
+        for k in ("mean0", "mean1", "var0", "var1"):
+            assert st[k] == pytest.approx(ost[k], rel=1e-12), k
+
+
+def test_mas_solver_status_and_selection(gpu):
+    prob = problems.make("elastic_contact", N=8)
+    A = prob.csc()
+    b = prob.rhs(0)
+    # priority list: the first solver this library implements
+    s = Solver.create({"solver": ["Eigen::SimplicialLDLT", "MAS", "GPUHybrid"], "MAS": {"max_iter": 7}})
+    s.factorize(A)
+    s.solve(b)
+    info = s.get_info()
+    assert s.name() == "MAS" and info["solver_status"] == "Reach max iterations" and info["solver_iter"] == 7
+    # the contact rows are the GMM strategy's high cluster; windows of +-32 positions
+    s2 = Solver.create(_dev_cfg(3, use_bad_dof_precond=True, bad_dof_strategy="gmm"))
+    s2.factorize(A)
+    st = s2.get_info()["bad_dof"]
+    assert st["selected"] > 0 and st["expanded"] >= st["selected"]
+    assert sum(c.size for c in s2.stiff_components()) == st["expanded"]
+    # multi-RHS on the MAS solver: columns one at a time, equal to single solves
+    import torch
+
+    B = np.stack([prob.rhs(c) for c in range(2)])
+    dB = torch.from_numpy(B).cuda()
+    dX = torch.zeros_like(dB)
+    s2.solve_multi_device(dB, dX, stream=torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    X = dX.cpu().numpy()
+    for c in range(2):
+        assert np.array_equal(X[c], s2.solve(B[c]))
