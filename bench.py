@@ -79,6 +79,9 @@ def main():
                     help="first copy this many MiB device-to-device (HBM counter calibration)")
     ap.add_argument("--multi-rhs", type=int, default=0, metavar="K",
                     help="also time one fused K-right-hand-side solve of the mixed path against K single solves")
+    ap.add_argument("--mas-solver", action="store_true",
+                    help="also time the standalone MAS solver (its defaults, rtol 1e-6, at most 3000 iterations) "
+                         "without and with each stiff-DOF strategy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solver has no multi-GPU path.",
@@ -178,6 +181,41 @@ def main():
                  "rel_residual": [c["final_res_norm"] / float(bn[i]) for i, c in enumerate(minfo["columns"])]}
         log("multi-rhs", K, f"fused {t_fused:.3f} s", f"single {t_seq:.3f} s")
         del s, dB, dX, xs
+    mas_runs = None
+    if args.mas_solver:
+        mas_runs = {}
+        variants = {"mas": {}, "mas+gmm": {"use_bad_dof_precond": True, "bad_dof_strategy": "gmm"},
+                    "mas+elbow": {"use_bad_dof_precond": True, "bad_dof_strategy": "elbow"},
+                    "mas+kmean": {"use_bad_dof_precond": True, "bad_dof_strategy": "kmean"}}
+        for vname, extra in variants.items():
+            # MAS alone converges slowly on contact stiffness: cap the iterations
+            m = {"block_dim": prob.block_size, "precision": args.mixed_precision, "max_iter": 3000}
+            m.update(extra)
+            log(vname, "start")
+            s = Solver.create({"solver": "MAS", "MAS": m})
+
+            def mstep():
+                s.factorize_device(n, d_ptr, d_idx, d_val, stream=stream)
+                x.zero_()
+                s.solve_device(d_b, x, stream=stream)
+
+            try:
+                mstep()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                mstep()
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                mi = s.get_info()
+                mas_runs[vname] = {"s": round(dt, 4), "factorize_s": round(mi["timings"]["factorize"], 4),
+                                   "iterations": mi["solver_iter"], "status": mi["solver_status"],
+                                   "rel_residual": mi["final_res_norm"] / bnorm,
+                                   "stiff_dofs": mi["stiff"]["n_stiff_dofs"],
+                                   "components": mi["stiff"]["n_components"]}
+            except Exception as e:  # a strategy can select a component too large for the dense factor
+                mas_runs[vname] = {"error": str(e)}
+            log(vname, mas_runs[vname])
+            del s
     # roofline of the outer FP64 SpMV, timed with HIP events on the solver's stream
     g = dict(paths["mixed"], profile_kernels=True)
     s = Solver.create({"solver": "GPUHybrid", "GPUHybrid": g})
@@ -241,6 +279,8 @@ def main():
     }
     if multi is not None:
         line["multi_rhs"] = multi
+    if mas_runs is not None:
+        line["mas_solver"] = mas_runs
     print(json.dumps(line))
 
 
