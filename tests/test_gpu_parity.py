@@ -277,3 +277,47 @@ def test_user_stiff_set_matches_oracle(gpu):
     s2 = Solver.create(_cfg(3, select_bad_dofs_from_l1_norm=False))
     s2.factorize(A)
     assert s2.stiff_components() == []
+
+
+def _truncated(A, m):
+    # leading principal m x m block: still SPD, and m need not be a multiple of the block size
+    S = A.to_scipy().tocsc()[:m, :m].tocsc()
+    S.sort_indices()
+    return problems.CSC(m, S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.copy())
+
+
+# block size 2, and DOF counts that leave 1 or 2 padding scalars in the last block
+# (BSR tail padding with a unit diagonal, MAS/BSRMatrix.cu:123-134, 189-191)
+@pytest.mark.parametrize("name,N,bs,m", [("laplace_sphere", 12, 2, None), ("laplace_sphere", 11, 3, None),
+                                         ("laplace_sphere", 11, 2, None), ("elastic_contact", 6, 3, 647),
+                                         ("elastic_contact", 6, 3, 646)])
+@pytest.mark.parametrize("mode", ["GPUHybrid", "MAS"])
+def test_block_size_and_tail_padding(gpu, name, N, bs, m, mode):
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    if m:
+        A = _truncated(A, m)
+    b = problems.random_vector(A.n, 3)
+    if mode == "GPUHybrid":
+        s = Solver.create(_cfg(bs))
+        H = O.Hybrid(_ocfg(bs))
+    else:
+        s = Solver.create({"solver": "MAS", "MAS": {"block_dim": bs, "use_bad_dof_precond": True,
+                                                    "bad_dof_strategy": "gmm"}})
+        H = O.Hybrid(O.config(block_size=bs, solver="MAS", rel_tol=1e-6, abs_tol=1e-20, max_iter=100000,
+                              use_bad_dof_precond=True, bad_dof_strategy="gmm"))
+    s.factorize(A)
+    H.factorize(A.colptr, A.rowidx, A.vals)
+    dc, oc = s.stiff_components(), H.stiff_components()
+    assert len(dc) == len(oc)
+    for a, c in zip(dc, oc):
+        assert np.array_equal(a, c)
+    x = s.solve(b)
+    xr = H.solve(b)
+    info, oi = s.get_info(), H.info()
+    tol = 1e-8 if mode == "GPUHybrid" else 1e-6
+    assert _res_ok(A.to_scipy(), x, b, rtol=tol * 2)
+    assert abs(info["num_iterations"] - oi["num_iterations"]) <= max(10 if mode == "MAS" else 2,
+                                                                    0.05 * oi["num_iterations"])
+    assert np.linalg.norm(x - xr) <= (1e-5 if mode == "GPUHybrid" else 1e-3) * np.linalg.norm(xr)
+    assert np.max(np.abs(s.spmv(b) - O.spmv(A.colptr, A.rowidx, A.vals, b))) <= 1e-13 * np.max(np.abs(A.vals))
