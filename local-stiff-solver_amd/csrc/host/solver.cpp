@@ -48,6 +48,10 @@ HybridSolver::HybridSolver(const Config &cfg) : cfg_(cfg)
     // a blocking stream: work on it is ordered with the legacy default stream, so
     // callers that pass NULL (e.g. torch's default stream) need no extra sync
     LSS_HIP(hipStreamCreate(&stream_));
+    // second stream (default priority) for the MAS coarse levels, ordered by events
+    LSS_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    LSS_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+    LSS_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
     void *p = nullptr;
     LSS_HIP(hipHostMalloc(&p, sizeof(dev::PcgState) * kMaxCols, hipHostMallocDefault));
     h_st_ = static_cast<dev::PcgState *>(p);
@@ -65,6 +69,9 @@ HybridSolver::~HybridSolver()
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
     if (h_st_) (void)hipHostFree(h_st_);
     if (stream_) (void)hipStreamDestroy(stream_);
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (side_) (void)hipStreamDestroy(side_);
 }
 
 const char *HybridSolver::name() const
@@ -742,25 +749,41 @@ void HybridSolver::base_apply(const double *r, double scale, const double *add, 
     }
     const int L = int(mas_.size()) - 1;
     auto ld = [&](int l) { return l == 0 ? np : int64_t(mas_[l].nodes) * bs_; };
+    const bool prof = cfg_.profile_kernels && st && k == 1;
+    if (L == 0) {
+        if (prof) prof_begin(2, s);
+        k::mas_level0_apply(prec(), bs_, mas_[0].nbanks, mas_[0].bank_off.get(), mas_[0].inv.get(), r, nullptr,
+                            nullptr, scale, add, z, st ? r_.get() : nullptr, st, partials_.get(), stop, s, k, np, 0);
+        if (prof) prof_end(s);
+        if (st) k::finish(mas_[0].nbanks, partials_.get(), op, st, nullptr, 0, s, k);
+        return;
+    }
+    // The coarse levels (restriction chain, then the coarse banks from the top down) are a
+    // chain of small latency-bound launches; they run on a second stream while the level-0
+    // banks solve on s, and the prolongated level-1 correction is added after the join.
+    LSS_HIP(hipEventRecord(ev_fork_, s));
+    LSS_HIP(hipStreamWaitEvent(side_, ev_fork_, 0));
     const double *src = r;
     for (int l = 1; l <= L; ++l) {
         k::mas_restrict(bs_, mas_[l - 1].nbanks, mas_[l - 1].bank_off.get(), mas_[l - 1].parent.get(), src,
-                        mas_[l].r.get(), stop, s, k, ld(l - 1), ld(l));
+                        mas_[l].r.get(), stop, side_, k, ld(l - 1), ld(l));
         src = mas_[l].r.get();
     }
     for (int l = L; l >= 1; --l) {
         const bool top = (l == L);
         k::mas_coarse_apply(prec(), bs_, mas_[l].nodes, mas_[l].inv.get(), mas_[l].r.get(),
                             top ? nullptr : mas_[l].parent.get(), top ? nullptr : mas_[l + 1].w.get(),
-                            mas_[l].w.get(), stop, s, k, ld(l), top ? 0 : ld(l + 1));
+                            mas_[l].w.get(), stop, side_, k, ld(l), top ? 0 : ld(l + 1));
     }
-    const bool prof = cfg_.profile_kernels && st && k == 1;
+    LSS_HIP(hipEventRecord(ev_join_, side_));
     if (prof) prof_begin(2, s);
-    k::mas_level0_apply(prec(), bs_, mas_[0].nbanks, mas_[0].bank_off.get(), mas_[0].inv.get(), r,
-                        L >= 1 ? mas_[0].parent.get() : nullptr, L >= 1 ? mas_[1].w.get() : nullptr, scale, add, z,
-                        st ? r_.get() : nullptr, st, partials_.get(), stop, s, k, np, L >= 1 ? ld(1) : 0);
+    k::mas_level0_apply(prec(), bs_, mas_[0].nbanks, mas_[0].bank_off.get(), mas_[0].inv.get(), r, nullptr, nullptr,
+                        scale, add, z, nullptr, nullptr, partials_.get(), stop, s, k, np, 0);
     if (prof) prof_end(s);
-    if (st) k::finish(mas_[0].nbanks, partials_.get(), op, st, nullptr, 0, s, k);
+    LSS_HIP(hipStreamWaitEvent(s, ev_join_, 0));
+    const int g = k::mas_prolong_add(np, bs_, mas_[0].parent.get(), mas_[1].w.get(), scale, z,
+                                     st ? r_.get() : nullptr, partials_.get(), stop, s, k, np, ld(1));
+    if (st) k::finish(g, partials_.get(), op, st, nullptr, 0, s, k);
 }
 
 void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, const int *stop, hipStream_t s,

@@ -105,6 +105,8 @@ private:
 
     Config cfg_;
     hipStream_t stream_ = nullptr;
+    hipStream_t side_ = nullptr;  // MAS coarse levels, concurrent with the level-0 banks
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
     bool factorized_ = false;
     int32_t n_ = 0, nb_ = 0;
     int bs_ = 1;

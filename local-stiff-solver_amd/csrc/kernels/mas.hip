@@ -560,6 +560,32 @@ __global__ __launch_bounds__(kApplyThreads) void k_level0_apply(int32_t nbanks, 
     }
 }
 
+// z += scale * w1[parent] (the level-1 correction prolongated to level 0) with an
+// optional per-workgroup partial of dotv . z; grid row y = column y.
+template <bool DOT>
+__global__ __launch_bounds__(256) void k_prolong_add(int64_t np, int bs, const int32_t *__restrict__ parent,
+                                                     const double *__restrict__ w1, int64_t ld1, double scale,
+                                                     double *__restrict__ z, int64_t ld,
+                                                     const double *__restrict__ dotv, double *partials,
+                                                     const int *__restrict__ stop)
+{
+    if (dev::all_stopped(stop, gridDim.y)) return;
+    const int64_t o = blockIdx.y * ld;
+    w1 += blockIdx.y * ld1;
+    z += o;
+    double part = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < np; i += int64_t(gridDim.x) * 256) {
+        const int64_t node = i / bs;
+        const double v = z[i] + scale * w1[int64_t(parent[node]) * bs + (i - node * bs)];
+        z[i] = v;
+        if constexpr (DOT) part += dotv[o + i] * v;
+    }
+    if constexpr (DOT) {
+        __shared__ double sh[4];
+        dev::store_partial(dev::block_sum<256>(part, sh), partials + int64_t(blockIdx.y) * gridDim.x);
+    }
+}
+
 template <class TS>
 void launch_bank_invert(int bs, int32_t nbanks, const int32_t *bank_off, const int32_t *rowptr, const int32_t *cols,
                         const double *vals, TS *out, int *status, hipStream_t s)
@@ -716,6 +742,22 @@ void level0_apply_k(Precision prec, int bs, int32_t nbanks, const int32_t *part_
 }
 
 }  // namespace
+
+int mas_prolong_add(int64_t np, int bs, const int32_t *parent, const double *w1, double scale, double *z,
+                    const double *dotv, double *partials, const int *stop, hipStream_t s, int k, int64_t ld,
+                    int64_t ld1)
+{
+    if (np == 0) return 0;
+    const int grid = int(std::min<int64_t>(std::max<int64_t>(div_up(np, 256 * 4), 1), 4096));
+    if (dotv)
+        k_prolong_add<true><<<dim3(grid, k), 256, 0, s>>>(np, bs, parent, w1, ld1, scale, z, ld, dotv, partials,
+                                                          stop);
+    else
+        k_prolong_add<false><<<dim3(grid, k), 256, 0, s>>>(np, bs, parent, w1, ld1, scale, z, ld, nullptr,
+                                                           nullptr, stop);
+    LSS_LAUNCH_CHECK();
+    return grid;
+}
 
 void mas_coarse_apply(Precision prec, int bs, int32_t nodes, const void *inv, const double *r, const int32_t *parent,
                       const double *w_parent, double *w, const int *stop, hipStream_t s, int k, int64_t ld,

@@ -31,6 +31,11 @@ void mas_restrict(int bs, int32_t fine_nbanks, const int32_t *fine_bank_off, con
 void mas_coarse_apply(Precision prec, int bs, int32_t nodes, const void *inv, const double *r, const int32_t *parent,
                       const double *w_parent, double *w, const int *stop, hipStream_t s, int k = 1, int64_t ld = 0,
                       int64_t ld_parent = 0);
+// z += scale * w1[parent] over the level-0 scalars (parent: level-0 node -> level-1 node);
+// dotv != nullptr: per-workgroup partials of dotv . z (returns their count per column).
+int mas_prolong_add(int64_t np, int bs, const int32_t *parent, const double *w1, double scale, double *z,
+                    const double *dotv, double *partials, const int *stop, hipStream_t s, int k = 1,
+                    int64_t ld = 0, int64_t ld1 = 0);
 // st != nullptr: per-bank partials of dotv . z go to partials[c * nbanks + bank] (reduced by k::finish).
 void mas_level0_apply(Precision prec, int bs, int32_t nbanks, const int32_t *part_off, const void *inv,
                       const double *r, const int32_t *parent, const double *w1, double scale, const double *add,
