@@ -106,6 +106,7 @@ Config parse_config(const char *text, const Config &base)
                 else if (k == "profile_kernels") c.profile_kernels = as_bool(v, k);
                 else if (k == "partitioner") c.partitioner = as_str(v, k);
                 else if (k == "max_coarse_levels") c.max_coarse_levels = int(as_int(v, k));
+                else if (k == "partition_local_max") c.partition_local_max = int(as_int(v, k));
                 else bad("unknown key MAS/" + k);
             }
         }
@@ -144,6 +145,7 @@ Config parse_config(const char *text, const Config &base)
         else if (k == "true_residual_check") c.true_residual_check = as_bool(v, k);
         else if (k == "partitioner") c.partitioner = as_str(v, k);
         else if (k == "max_coarse_levels") c.max_coarse_levels = int(as_int(v, k));
+        else if (k == "partition_local_max") c.partition_local_max = int(as_int(v, k));
         else bad("unknown key GPUHybrid/" + k);
     }
     return validate(c);
@@ -160,6 +162,7 @@ Config validate(Config c)
     if (c.check_every < 1) bad("check_every must be >= 1");
     if (c.partitioner != "device" && c.partitioner != "host") bad("partitioner must be device|host");
     if (c.max_coarse_levels < 0 || c.max_coarse_levels > 4) bad("max_coarse_levels must be in 0..4");
+    if (c.partition_local_max < 0 || c.partition_local_max > 8192) bad("partition_local_max must be in 0..8192");
     if (!(c.damping > 0.0) || c.damping >= 4.0) bad("damping must be in (0, 4)");
     if (c.lanczos_steps < 1) bad("lanczos_steps must be >= 1");
     if (c.bad_dof_gmm_max_iterations < 0 || c.bad_dof_gmm_expand_neighbors < 0 ||

@@ -57,6 +57,7 @@ struct Config {
     bool true_residual_check = false;   // on convergence verify ||b - A x|| and restart PCG from it if needed
     std::string partitioner = "device";  // MAS partition: "device" (HIP kernels) | "host" (OpenMP)
     int max_coarse_levels = 4;           // MAS coarse levels (MAS_MAX_COARSE_LEVEL, MASPreconditioner.hpp:10-12)
+    int partition_local_max = 8192;      // device partition: subsets this small finish in one workgroup (0: never)
 
     // FP32 matrix values in the preconditioner-internal SpMV (by default with reduced-
     // precision banks).  The stiff rows are then recomputed in FP64: an FP32 rounding

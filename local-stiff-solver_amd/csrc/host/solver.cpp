@@ -376,7 +376,8 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     DBuf<int32_t> d_off;
     if (dev_topo) {
         d_perm_.resize(nb_);
-        const int32_t nparts = k::device_partition(nb_, d_rp0.get(), d_c0.get(), 32, d_perm_.get(), d_off, s);
+        const int32_t nparts = k::device_partition(nb_, d_rp0.get(), d_c0.get(), 32, d_perm_.get(), d_off, s,
+                                                    cfg_.partition_local_max);
         perm_.resize(nb_);
         part_off_.resize(size_t(nparts) + 1);
         d_perm_.download(perm_.data(), nb_, s);

@@ -116,8 +116,10 @@ void sort_positions(int32_t n, const double *keys, bool descending, double *keys
 // Recursive breadth-first bisection of the block graph (rowptr/cols, rows
 // ascending) into parts of at most max_part nodes; writes perm[old] = new and
 // part_off (nparts + 1 entries, resized); returns nparts.  Synchronizes s.
+// Subsets of at most local_max nodes (capped at 8192; 0: none) finish their
+// recursion in one workgroup each, with the same result.
 int32_t device_partition(int32_t nb, const int32_t *rowptr, const int32_t *cols, int max_part, int32_t *perm,
-                         DBuf<int32_t> &part_off, hipStream_t s);
+                         DBuf<int32_t> &part_off, hipStream_t s, int32_t local_max = 8192);
 
 // Exclusive scan of n int32 values (out may not alias a); tsum holds
 // ceil(n / 1024) values; the total goes to *total when it is not null.

@@ -186,13 +186,19 @@ __global__ void k_begin(int32_t nsub, Sub *__restrict__ S, const int32_t *__rest
 }
 
 // One level of every search: 16 lanes per frontier row; a node is taken by
-// the lane whose compare-and-swap marks it, and each wave appends its new
-// nodes to the subset's queue with one atomic.
+// the lane whose compare-and-swap marks it and collected in LDS, and the
+// workgroup appends its new nodes to the subset's queue with one atomic per
+// flush (a device-scope atomic on the subset's queue end is the contended
+// resource when one subset has a frontier of ~10^4 rows).
+constexpr int kExpBuf = 2048;
+
 __global__ __launch_bounds__(kPT) void k_expand(int32_t bx, Sub *__restrict__ S, const int32_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ cols, int32_t *__restrict__ order,
                                                  int32_t *__restrict__ vis, int32_t *__restrict__ dist,
                                                  const int32_t *__restrict__ sub, int32_t pid0)
 {
+    __shared__ int32_t buf[kExpBuf];
+    __shared__ int32_t cnt, base, wlen[kPT / dev::kWave];
     const int32_t s = blockIdx.x / bx, b = blockIdx.x % bx;
     const Sub st = S[s];
     if (st.phase == 2) return;
@@ -202,10 +208,20 @@ __global__ __launch_bounds__(kPT) void k_expand(int32_t bx, Sub *__restrict__ S,
     int32_t chunk = (f + bx - 1) / bx;
     chunk = ((chunk + kRowsPerPass - 1) / kRowsPerPass) * kRowsPerPass;
     const int32_t lo = min(f, b * chunk), hi = min(f, lo + chunk);
-    const int lane = threadIdx.x & (dev::kWave - 1);
-    const int g = threadIdx.x / kRowLanes, gl = threadIdx.x % kRowLanes;
+    if (lo >= hi) return;  // uniform over the block
+    const int t = threadIdx.x, lane = t & (dev::kWave - 1);
+    const int g = t / kRowLanes, gl = t % kRowLanes;
     const int32_t dnew = st.level + 1;
-    for (int32_t i0 = lo; i0 < hi; i0 += kRowsPerPass) {  // uniform over the block
+    auto flush = [&]() {  // every thread; cnt was last changed before a barrier
+        if (t == 0) base = atomicAdd(&S[s].qend, cnt);
+        __syncthreads();
+        for (int32_t i = t; i < cnt; i += kPT) order[base + i] = buf[i];
+        __syncthreads();
+        if (t == 0) cnt = 0;
+        __syncthreads();
+    };
+    if (t == 0) cnt = 0;
+    for (int32_t i0 = lo; i0 < hi; i0 += kRowsPerPass) {
         const int32_t i = i0 + g;
         int32_t r0 = 0, r1 = 0;
         if (i < hi) {
@@ -213,32 +229,31 @@ __global__ __launch_bounds__(kPT) void k_expand(int32_t bx, Sub *__restrict__ S,
             r0 = rowptr[u];
             r1 = rowptr[u + 1];
         }
-        // longest row of the wave: every lane runs the same number of rounds
+        // longest row of the block: every lane runs the same number of rounds
         int32_t len = r1 - r0;
         for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o, dev::kWave));
+        __syncthreads();
+        if (lane == 0) wlen[t / dev::kWave] = len;
+        __syncthreads();
+        for (int w = 0; w < kPT / dev::kWave; ++w) len = max(len, wlen[w]);
         for (int32_t k = 0; k < len; k += kRowLanes) {
             const int32_t e = r0 + k + gl;
-            int32_t w = -1;
             if (e < r1) {
                 const int32_t c = cols[e];
                 if (sub[c] == s) {
                     const int32_t old = vis[c];
-                    if (old != pid && atomicCAS(&vis[c], old, pid) == old) w = c;
+                    if (old != pid && atomicCAS(&vis[c], old, pid) == old) {
+                        dist[c] = dnew;
+                        buf[atomicAdd(&cnt, 1)] = c;
+                    }
                 }
             }
-            const unsigned long long mask = __ballot(w >= 0);
-            if (mask == 0) continue;
-            const int leader = __builtin_ctzll(mask);
-            int32_t base = 0;
-            if (lane == leader) base = atomicAdd(&S[s].qend, __popcll(mask));
-            base = __shfl(base, leader, dev::kWave);
-            if (w >= 0) {
-                const int32_t at = base + __popcll(mask & ((1ull << lane) - 1ull));
-                order[at] = w;
-                dist[w] = dnew;
-            }
+            __syncthreads();
+            if (cnt > kExpBuf - kPT) flush();  // room for one more round
         }
     }
+    __syncthreads();
+    if (cnt > 0) flush();
 }
 
 // Close the level of every search: new frontier, cut level of the second
@@ -405,6 +420,386 @@ __global__ void k_children(int32_t nsub, const Sub *__restrict__ S, Sub *__restr
     N[2 * s + 1] = c;
 }
 
+// ---------------------------------------------------------------- small subsets in one workgroup
+// Every subset of at most T nodes finishes its whole recursion in one
+// workgroup: the searches, cuts and splits above, over all parts of one
+// recursion depth at once, with the per-node state in LDS.  A node is named
+// by its rank u in the subset (rank order = id order, as the member array is
+// ascending); key[v] = position of v in the member array, so a neighbour c
+// belongs to the subset iff key[c] - off is in [0, m).  vd[u] packs the
+// search stamp (high 16 bits) and the distance; a level is expanded by
+// marking nodes, not by a queue: nodes at the same distance are only ever
+// ordered by id.  The subset's adjacency is first rewritten as ranks (int16,
+// -1 outside the subset) into adj, rows in member order from roff[off], so
+// a search step waits on one global load instead of the row / column / key
+// chain.
+constexpr int kLT = 1024;      // threads per workgroup
+constexpr int kLocalCap = 8192;  // most nodes per subset (8 positions per thread)
+
+struct LocalLds {
+    int32_t *vd, *aoff, *desc[2], *ph, *lv, *rc, *ct, *nd, *cn, *cm, *cf, *rq, *misc;
+    uint16_t *mem[2], *lsub, *scan;
+};
+
+__host__ __device__ inline size_t local_lds_bytes(int32_t T, int32_t idcap)
+{
+    return size_t(2 * T + 1) * sizeof(int32_t) + size_t(idcap) * 17 * sizeof(int32_t) + 32 * sizeof(int32_t) +
+           size_t(T) * 4 * sizeof(uint16_t);
+}
+
+__device__ inline LocalLds local_layout(unsigned char *lds, int32_t T, int32_t idcap)
+{
+    LocalLds L;
+    L.vd = reinterpret_cast<int32_t *>(lds);
+    L.aoff = L.vd + T;            // T + 1 row offsets into adj
+    L.desc[0] = L.aoff + T + 1;  // off, m, base, carried (idcap each)
+    L.desc[1] = L.desc[0] + 4 * idcap;
+    L.ph = L.desc[1] + 4 * idcap;  // search state: phase (2 = none), level, reached, cut, need
+    L.lv = L.ph + idcap;
+    L.rc = L.lv + idcap;
+    L.ct = L.rc + idcap;
+    L.nd = L.ct + idcap;
+    L.cn = L.nd + idcap;  // per step: new nodes, largest frontier rank, restart rank, restart request
+    L.cm = L.cn + idcap;
+    L.cf = L.cm + idcap;
+    L.rq = L.cf + idcap;
+    L.misc = L.rq + idcap;  // [0,16) wave sums, [16,18) active, [18] error, [19] any search, [20,22) restart
+    L.mem[0] = reinterpret_cast<uint16_t *>(L.misc + 32);
+    L.mem[1] = L.mem[0] + T;
+    L.lsub = L.mem[1] + T;
+    L.scan = L.lsub + T;
+    return L;
+}
+
+// exclusive scan over the kLT threads; wsum holds 16 ints
+__device__ __forceinline__ int32_t wg_excl_scan(int32_t v, int32_t *wsum, int32_t &total)
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int32_t x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (t < 64) {
+        int32_t a = t < kLT / 64 ? wsum[t] : 0;
+        for (int d = 1; d < kLT / 64; d <<= 1) {
+            const int32_t y = __shfl_up(a, d, 64);
+            if (lane >= d) a += y;
+        }
+        if (t < kLT / 64) wsum[t] = a;
+    }
+    __syncthreads();
+    total = wsum[kLT / 64 - 1];
+    const int32_t r = (w ? wsum[w - 1] : 0) + x - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kLT) void k_local_partition(const Sub *__restrict__ S, const int32_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ cols, int32_t *__restrict__ mem,
+                                                        const int32_t *__restrict__ key,
+                                                        const int32_t *__restrict__ roff, int16_t *__restrict__ adj,
+                                                        int32_t *__restrict__ part_off, int32_t max_part, int32_t T,
+                                                        int32_t idcap, int32_t *__restrict__ flags)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const Sub top = S[blockIdx.x];
+    const int t = threadIdx.x;
+    if (top.m == 0 || top.carried) return;
+    if (top.m <= max_part) {  // a leaf seen for the first time
+        if (t == 0) part_off[top.base] = top.off;
+        return;
+    }
+    if (top.m > T) {
+        if (t == 0) flags[3] = 1;
+        return;
+    }
+    const int32_t O = top.off, M = top.m;
+    LocalLds L = local_layout(lds, T, idcap);
+    for (int32_t u = t; u < M; u += kLT) {
+        L.vd[u] = 0;
+        L.mem[0][u] = uint16_t(u);
+        L.lsub[u] = 0;
+    }
+    for (int32_t j = t; j < idcap; j += kLT) {
+        L.ph[j] = 2;
+        L.cn[j] = 0;
+        L.cm[j] = -1;
+        L.cf[j] = INT_MAX;
+        L.rq[j] = 0;
+    }
+    // ranks of the neighbours inside the subset, row by row in member order
+    const int32_t A0 = roff[O];
+    for (int32_t u = t; u < M; u += kLT) {
+        const int32_t g = mem[O + u];
+        const int32_t r0 = rowptr[g], r1 = rowptr[g + 1], a = roff[O + u];
+        L.aoff[u] = a - A0;
+        if (u == M - 1) L.aoff[M] = a - A0 + (r1 - r0);
+        for (int32_t e0 = r0; e0 < r1; e0 += 16) {
+            int32_t c[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) c[i] = e0 + i < r1 ? cols[e0 + i] : -1;
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                if (c[i] >= 0) {
+                    const int32_t q = key[c[i]] - O;
+                    adj[a + (e0 - r0) + i] = int16_t(uint32_t(q) < uint32_t(M) ? q : -1);
+                }
+        }
+    }
+    if (t == 0) {
+        L.desc[0][0] = O;
+        L.desc[0][idcap] = M;
+        L.desc[0][2 * idcap] = top.base;
+        L.desc[0][3 * idcap] = 0;
+        L.misc[18] = 0;
+    }
+    __syncthreads();
+    const int32_t P = (M + kLT - 1) / kLT;  // member positions per thread in the splits
+    const int32_t q0 = min(M, t * P), q1 = min(M, q0 + P);
+    int cb = 0;  // current descriptor / member buffer
+    int32_t ncur = 1;
+    for (int depth = 0;; ++depth) {
+        int32_t *D = cb ? L.desc[1] : L.desc[0];
+        const uint16_t *mc = cb ? L.mem[1] : L.mem[0];
+        const int32_t st0 = 2 * depth + 1;  // stamp of the first search; the second uses st0 + 1
+        if (t == 0) {
+            L.misc[16] = L.misc[17] = L.misc[20] = L.misc[21] = 0;
+            L.misc[19] = 0;
+        }
+        __syncthreads();
+        // leaves take their part id once; larger parts start the first search from their smallest member
+        for (int32_t j = t; j < ncur; j += kLT) {
+            const int32_t m = D[idcap + j];
+            L.ph[j] = 2;
+            if (m > 0 && !D[3 * idcap + j]) {
+                if (m <= max_part) {
+                    part_off[D[2 * idcap + j]] = D[j];
+                    D[3 * idcap + j] = 1;
+                } else {
+                    L.vd[mc[D[j] - O]] = st0 << 16;
+                    L.ph[j] = 0;
+                    L.lv[j] = 0;
+                    L.rc[j] = 1;
+                    L.ct[j] = -1;
+                    L.nd[j] = 0;
+                    L.misc[19] = 1;
+                }
+            }
+        }
+        __syncthreads();
+        if (!L.misc[19]) break;
+        for (int32_t step = 0;; ++step) {
+            if (step > 4 * M + 16) {  // each step reaches a node, restarts or switches searches
+                if (t == 0) L.misc[18] = 1;
+                break;
+            }
+            // expand: mark the unvisited neighbours of every frontier node
+            for (int32_t u = t; u < M; u += kLT) {
+                const int j = L.lsub[u];
+                const int p = L.ph[j];
+                if (p == 2) continue;
+                const int32_t cv = ((st0 + p) << 16) | L.lv[j];
+                if (L.vd[u] != cv) continue;
+                const int32_t a1 = L.aoff[u + 1];
+                for (int32_t a0 = L.aoff[u]; a0 < a1; a0 += 16) {
+                    int32_t q[16];
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) q[i] = a0 + i < a1 ? int32_t(adj[A0 + a0 + i]) : -1;
+#pragma unroll
+                    for (int i = 0; i < 16; ++i)
+                        if (q[i] >= 0 && L.lsub[q[i]] == j && (L.vd[q[i]] >> 16) != st0 + p) L.vd[q[i]] = cv + 1;
+                }
+            }
+            __syncthreads();
+            // count the new level; largest rank of the current level
+            for (int32_t u = t; u < M; u += kLT) {
+                const int j = L.lsub[u];
+                const int p = L.ph[j];
+                if (p == 2) continue;
+                const int32_t cv = ((st0 + p) << 16) | L.lv[j];
+                const int32_t v = L.vd[u];
+                if (v == cv + 1) atomicAdd(&L.cn[j], 1);
+                else if (v == cv) atomicMax(&L.cm[j], u);
+            }
+            __syncthreads();
+            const int slot = step & 1;
+            if (t == 0) {
+                L.misc[16 + (slot ^ 1)] = 0;
+                L.misc[20 + (slot ^ 1)] = 0;
+            }
+            for (int32_t j = t; j < ncur; j += kLT) {
+                const int p = L.ph[j];
+                if (p == 2) continue;
+                const int32_t m = D[idcap + j];
+                const int32_t before = L.rc[j], lvl = L.lv[j] + 1, nw = L.cn[j], far = L.cm[j];
+                L.cn[j] = 0;
+                L.cm[j] = -1;
+                if (nw > 0) {
+                    L.rc[j] = before + nw;
+                    L.lv[j] = lvl;
+                    const int32_t nl = left_count(m, max_part);
+                    if (p == 1 && L.ct[j] < 0 && before + nw >= nl) {
+                        L.ct[j] = lvl;
+                        L.nd[j] = nl - before;
+                    }
+                    L.misc[16 + slot] = 1;
+                } else if (before < m) {  // restart from the smallest unreached member
+                    L.rq[j] = 1;
+                    L.misc[20 + slot] = 1;
+                    L.misc[16 + slot] = 1;
+                } else if (p == 0) {  // farthest level's largest id starts the second search
+                    if (far < 0) {
+                        L.misc[18] = 1;
+                        L.ph[j] = 2;
+                        continue;
+                    }
+                    L.vd[far] = (st0 + 1) << 16;
+                    L.ph[j] = 1;
+                    L.lv[j] = 0;
+                    L.rc[j] = 1;
+                    L.misc[16 + slot] = 1;
+                } else {
+                    L.ph[j] = 2;
+                }
+            }
+            __syncthreads();
+            if (L.misc[20 + slot]) {
+                for (int32_t u = t; u < M; u += kLT) {
+                    const int j = L.lsub[u];
+                    if (L.rq[j] && (L.vd[u] >> 16) != st0 + L.ph[j]) atomicMin(&L.cf[j], u);
+                }
+                __syncthreads();
+                for (int32_t j = t; j < ncur; j += kLT) {
+                    if (!L.rq[j]) continue;
+                    L.rq[j] = 0;
+                    const int32_t v = L.cf[j];
+                    L.cf[j] = INT_MAX;
+                    const int p = L.ph[j];
+                    if (v == INT_MAX) {
+                        L.misc[18] = 1;
+                        L.ph[j] = 2;
+                        continue;
+                    }
+                    const int32_t lvl = L.lv[j] + 1, before = L.rc[j];
+                    L.vd[v] = ((st0 + p) << 16) | lvl;
+                    L.lv[j] = lvl;
+                    L.rc[j] = before + 1;
+                    const int32_t nl = left_count(D[idcap + j], max_part);
+                    if (p == 1 && L.ct[j] < 0 && before + 1 >= nl) {
+                        L.ct[j] = lvl;
+                        L.nd[j] = nl - before;
+                    }
+                }
+                __syncthreads();
+            }
+            if (!L.misc[16 + slot]) break;
+        }
+        __syncthreads();
+        if (L.misc[18]) break;
+        if (2 * ncur > idcap) {
+            if (t == 0) L.misc[18] = 1;
+            break;
+        }
+        // split every searched part: rank inside the cut level, then left rank, in member order
+        auto searched = [&](int j) { return D[idcap + j] > max_part && !D[3 * idcap + j]; };
+        int32_t c = 0;
+        for (int32_t q = q0; q < q1; ++q) {
+            const int u = mc[q], j = L.lsub[u];
+            c += searched(j) && (L.vd[u] & 0xFFFF) == L.ct[j];
+        }
+        int32_t tot;
+        int32_t run = wg_excl_scan(c, L.misc, tot);
+        for (int32_t q = q0; q < q1; ++q) {
+            const int u = mc[q], j = L.lsub[u];
+            L.scan[q] = uint16_t(run);
+            run += searched(j) && (L.vd[u] & 0xFFFF) == L.ct[j];
+        }
+        __syncthreads();
+        uint32_t bits = 0;
+        c = 0;
+        for (int32_t q = q0; q < q1; ++q) {
+            const int u = mc[q], j = L.lsub[u];
+            if (!searched(j)) continue;
+            const int32_t d = L.vd[u] & 0xFFFF, cut = L.ct[j];
+            const bool left = d < cut || (d == cut && int32_t(L.scan[q]) - int32_t(L.scan[D[j] - O]) < L.nd[j]);
+            bits |= uint32_t(left) << (q - q0);
+            c += left;
+        }
+        __syncthreads();
+        run = wg_excl_scan(c, L.misc, tot);
+        for (int32_t q = q0; q < q1; ++q) {
+            L.scan[q] = uint16_t(run);
+            run += (bits >> (q - q0)) & 1u;
+        }
+        __syncthreads();
+        uint16_t *mn = cb ? L.mem[0] : L.mem[1];
+        for (int32_t q = q0; q < q1; ++q) {
+            const int u = mc[q], j = L.lsub[u];
+            if (!searched(j)) {
+                mn[q] = uint16_t(u);
+                L.lsub[u] = uint16_t(2 * j);
+                continue;
+            }
+            const int32_t o = D[j] - O, lr = int32_t(L.scan[q]) - int32_t(L.scan[o]);
+            if ((bits >> (q - q0)) & 1u) {
+                mn[o + lr] = uint16_t(u);
+                L.lsub[u] = uint16_t(2 * j);
+            } else {
+                mn[o + left_count(D[idcap + j], max_part) + (q - o - lr)] = uint16_t(u);
+                L.lsub[u] = uint16_t(2 * j + 1);
+            }
+        }
+        int32_t *N = cb ? L.desc[0] : L.desc[1];
+        for (int32_t j = t; j < ncur; j += kLT) {
+            const int32_t off = D[j], m = D[idcap + j], base = D[2 * idcap + j], car = D[3 * idcap + j];
+            int32_t am = m, acar = car, coff = off + m, cm = 0, cbase = base;
+            if (!car && m > max_part) {
+                const int32_t nl = left_count(m, max_part);
+                am = nl;
+                coff = off + nl;
+                cm = m - nl;
+                cbase = base + nl / max_part;
+            }
+            N[2 * j] = off;
+            N[idcap + 2 * j] = am;
+            N[2 * idcap + 2 * j] = base;
+            N[3 * idcap + 2 * j] = acar;
+            N[2 * j + 1] = coff;
+            N[idcap + 2 * j + 1] = cm;
+            N[2 * idcap + 2 * j + 1] = cbase;
+            N[3 * idcap + 2 * j + 1] = 0;
+        }
+        __syncthreads();
+        cb ^= 1;
+        ncur *= 2;
+    }
+    __syncthreads();
+    if (L.misc[18]) {
+        if (t == 0) flags[3] = 1;
+        return;
+    }
+    // the member array of the subset, part by part
+    int32_t g[kLocalCap / kLT];
+    const uint16_t *mc = cb ? L.mem[1] : L.mem[0];
+#pragma unroll
+    for (int i = 0; i < kLocalCap / kLT; ++i)
+        if (q0 + i < q1) g[i] = mem[O + mc[q0 + i]];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kLocalCap / kLT; ++i)
+        if (q0 + i < q1) mem[O + q0 + i] = g[i];
+}
+
+__global__ void k_member_row_len(int32_t n, const int32_t *__restrict__ mem, const int32_t *__restrict__ rowptr,
+                                 int32_t *__restrict__ len)
+{
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q < n) len[q] = rowptr[mem[q] + 1] - rowptr[mem[q]];
+}
+
 __global__ void k_inverse_perm(int32_t nb, const int32_t *__restrict__ mem, int32_t *__restrict__ perm)
 {
     const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -430,8 +825,16 @@ void excl_scan_i32(int32_t n, const int32_t *a, int32_t *out, int32_t *tsum, int
     k_tile_apply<<<ntiles, kPT, 0, s>>>(n, a, tsum, out);
 }
 
+// recursion depths until every part of a subset of m nodes has at most max_part
+static int split_depths(int32_t m, int32_t max_part)
+{
+    int d = 0;
+    for (; m > max_part; ++d) m = left_count(m, max_part);  // the left part is the larger one
+    return d;
+}
+
 int32_t device_partition(int32_t nb, const int32_t *rowptr, const int32_t *cols, int max_part, int32_t *perm,
-                         DBuf<int32_t> &part_off, hipStream_t s)
+                         DBuf<int32_t> &part_off, hipStream_t s, int32_t local_max)
 {
     DBuf<int32_t> flag(nb), rank(nb), mem(nb), mem2(nb), sub(nb), vis(nb), dist(nb), order(nb), aux(nb);
     DBuf<int32_t> tsum(std::max(div_up(nb, kTile), 1)), dflags(8);
@@ -463,7 +866,36 @@ int32_t device_partition(int32_t nb, const int32_t *rowptr, const int32_t *cols,
     S.upload(h0.data(), 1, s);
     int32_t nsub = 1;
     int32_t hflags[2] = {0, 0};
+    int32_t maxm = real_n;  // largest subset of the current depth
+    local_max = std::min<int32_t>(local_max, kLocalCap);
     for (int depth = 0; real_n > 0; ++depth) {
+        if (maxm <= local_max && maxm > max_part) {
+            // every subset of this depth finishes in one workgroup
+            const int32_t T = maxm;
+            const int32_t idcap = std::max(2, 1 << split_depths(T, max_part));
+            const size_t lds = local_lds_bytes(T, idcap);
+            int dev = 0, cap = 0;
+            LSS_HIP(hipGetDevice(&dev));
+            LSS_HIP(hipDeviceGetAttribute(&cap, hipDeviceAttributeSharedMemPerBlockOptin, dev));
+            if (lds <= size_t(cap)) {
+                LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_local_partition),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+                k_inverse_perm<<<div_up(nb, 256), 256, 0, s>>>(nb, mem.get(), aux.get());
+                // row offsets of the rank adjacency: scan of the row lengths in member order
+                k_member_row_len<<<div_up(real_n, 256), 256, 0, s>>>(real_n, mem.get(), rowptr, flag.get());
+                excl_scan_i32(real_n, flag.get(), rank.get(), tsum.get(), dflags.get() + 4, s);
+                int32_t nadj = 0;
+                LSS_HIP(hipMemcpyAsync(&nadj, dflags.get() + 4, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+                LSS_HIP(hipStreamSynchronize(s));
+                DBuf<int16_t> adj(std::max(nadj, 1));
+                k_local_partition<<<nsub, kLT, lds, s>>>(S.get(), rowptr, cols, mem.get(), aux.get(), rank.get(),
+                                                         adj.get(), part_off.get(), max_part, T, idcap, dflags.get());
+                LSS_LAUNCH_CHECK();
+                LSS_HIP(hipStreamSynchronize(s));  // adj is freed at scope end
+                break;
+            }
+        }
+        maxm = maxm > max_part ? left_count(maxm, max_part) : maxm;
         const int32_t pid0 = 2 * depth + 1;
         LSS_HIP(hipMemsetAsync(dflags.get(), 0, 2 * sizeof(int32_t), s));
         k_begin<<<div_up(nsub, 256), 256, 0, s>>>(nsub, S.get(), mem.get(), order.get(), vis.get(), dist.get(),

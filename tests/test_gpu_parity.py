@@ -113,10 +113,13 @@ def test_solve_matches_oracle(gpu, name, N):
     assert np.linalg.norm(x - xr) <= 1e-5 * np.linalg.norm(xr)
 
 
-@pytest.mark.parametrize("variant", [{}, {"partitioner": "host"}])
-@pytest.mark.parametrize("name,N", CASES)
+@pytest.mark.parametrize("variant", [{}, {"partitioner": "host"}, {"partition_local_max": 0},
+                                     {"partition_local_max": 200}])
+@pytest.mark.parametrize("name,N", CASES + [("elastic_contact", 24)])
 def test_partition_matches_oracle(gpu, name, N, variant):
-    # default: the device bisection; host: the OpenMP implementation
+    # default: the device bisection, subsets of <= 8192 nodes finished in one workgroup
+    # each; partition_local_max 0: level-synchronous searches only; 200: both kinds;
+    # host: the OpenMP implementation
     prob = problems.make(name, N=N)
     A = prob.csc()
     bs = prob.block_size
@@ -136,6 +139,10 @@ def test_partition_matches_oracle(gpu, name, N, variant):
     # product numbering: new = perm[old], parts are ranges of new ids
     part_dev = np.searchsorted(off, perm, side="right") - 1
     assert np.array_equal(part_dev, part)
+    # inside a part the nodes keep ascending id order
+    want = np.empty(nb, dtype=np.int64)
+    want[np.argsort(part, kind="stable")] = np.arange(nb)
+    assert np.array_equal(perm, want)
 
 
 # ---------------------------------------------------------------- reference example system
