@@ -79,10 +79,14 @@ __device__ __forceinline__ int packed_lower(int i, int j)
 }
 
 // Assemble the bank matrices of one level from its block operator, factor
-// A_b = L L^T and invert L in LDS; store G = L^{-1} packed lower.  Rows of the
-// LDS matrix are padded to an odd number of doubles (DP = D + 1) so that the
-// lanes of a wave reading one column hit different LDS banks; the trailing
-// updates assign rows to waves and columns to lanes.
+// A_b = L L^T and invert L; store G = L^{-1} packed lower.  The matrix lives
+// in registers: thread (a, b) of a 16 x 16 grid owns the entries
+// (a + 16u, b + 16v), u, v < D / 16, so a rank-1 step of the right-looking
+// Cholesky (and of the forward elimination L X = I that gives G) costs each
+// thread 2 D / 16 LDS reads of the broadcast column / row and (D / 16)^2
+// FMAs, with two barriers per step.  The packed lower triangle in LDS holds
+// the assembled matrix, then L (read by the elimination), then G for the
+// coalesced store.
 template <int BS, class TS>
 __global__ __launch_bounds__(kInvThreads) void k_bank_factor(int32_t nbanks, const int32_t *__restrict__ bank_off,
                                                              const int32_t *__restrict__ rowptr,
@@ -91,80 +95,157 @@ __global__ __launch_bounds__(kInvThreads) void k_bank_factor(int32_t nbanks, con
                                                              int *__restrict__ status)
 {
     constexpr int D = 32 * BS;
-    constexpr int DP = D + 1;
+    constexpr int R = D / 16;
     constexpr int S = D * (D + 1) / 2;
     constexpr int BB = BS * BS;
     constexpr int NW = kInvThreads / 64;
+    static_assert(kInvThreads == 256, "16 x 16 thread grid");
     extern __shared__ double lds[];
-    double *M = lds;         // D x DP row-major, lower triangle used
-    double *tmp = lds + D * DP;
+    double *P = lds;       // packed lower triangle, S doubles
+    double *vec = P + S;   // D doubles: column k of L, or row k of X
+    double *dg = vec + D;  // the next diagonal entry
     __shared__ int bad;
     const int32_t bank = blockIdx.x;
     if (bank >= nbanks) return;
     const int32_t lo = bank_off[bank], hi = bank_off[bank + 1];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (threadIdx.x == 0) bad = 0;
-    for (int t = threadIdx.x; t < D * DP; t += kInvThreads) M[t] = 0.0;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int ta = t >> 4, tb = t & 15;
+    if (t == 0) bad = 0;
+    for (int q = t; q < S + D; q += kInvThreads) P[q] = 0.0;  // and vec
     __syncthreads();
-    // assemble: block (i, J) with both nodes in the bank
+    // assemble the lower triangle: block (i, J), J <= i, both nodes in the bank
     for (int32_t i = lo + wid; i < hi; i += NW) {
         for (int32_t k = rowptr[i] + lane; k < rowptr[i + 1]; k += 64) {
             const int32_t J = cols[k];
-            if (J < lo || J >= hi) continue;
+            if (J < lo || J > i) continue;
             const int r0 = (i - lo) * BS, c0 = (J - lo) * BS;
 #pragma unroll
             for (int a = 0; a < BS; ++a)
 #pragma unroll
-                for (int b = 0; b < BS; ++b) M[(r0 + a) * DP + c0 + b] = vals[int64_t(k) * BB + a * BS + b];
+                for (int b = 0; b < BS; ++b)
+                    if (c0 + b <= r0 + a) P[packed_lower(r0 + a, c0 + b)] = vals[int64_t(k) * BB + a * BS + b];
         }
     }
     __syncthreads();
-    // zero diagonal -> 1 (MASPreconditioner.cu:779-796): padding nodes and empty rows
-    for (int d = threadIdx.x; d < D; d += kInvThreads)
-        if (M[d * DP + d] == 0.0) M[d * DP + d] = 1.0;
+    // slots v <= u only: (a + 16u, b + 16v) with v > u lies above the diagonal
+    double m[R][R];
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+#pragma unroll
+        for (int v = 0; v <= u; ++v) {
+            const int i = ta + 16 * u, j = tb + 16 * v;
+            double x = j <= i ? P[packed_lower(i, j)] : 0.0;
+            if (i == j && x == 0.0) x = 1.0;  // zero diagonal -> 1 (MASPreconditioner.cu:779-796): padding, empty rows
+            m[u][v] = x;
+        }
+    if (t == 0) *dg = m[0][0];
     __syncthreads();
-    // Cholesky, right-looking, lower triangle in place
+    // Cholesky, right-looking: scale column k, then the rank-1 update of the trailing lower triangle
     for (int k = 0; k < D; ++k) {
-        const double d = M[k * DP + k];
-        if (threadIdx.x == 0 && (!(d > 0.0) || !isfinite(d))) bad = 1;
+        const double d = *dg;
+        if (t == 0 && (!(d > 0.0) || !isfinite(d))) bad = 1;
         const double rdk = 1.0 / sqrt(d);
-        // scaled column k below the diagonal, kept in tmp as well
-        for (int i = k + 1 + threadIdx.x; i < D; i += kInvThreads) {
-            const double v = M[i * DP + k] * rdk;
-            M[i * DP + k] = v;
-            tmp[i] = v;
+        if (tb == (k & 15)) {
+#pragma unroll
+            for (int u = 0; u < R; ++u)
+#pragma unroll
+                for (int v = 0; v <= u; ++v) {
+                    if (v != (k >> 4)) continue;
+                    const int i = ta + 16 * u;
+                    if (i > k) {
+                        const double x = m[u][v] * rdk;
+                        m[u][v] = x;
+                        vec[i] = x;
+                    } else if (i == k) {
+                        m[u][v] = sqrt(d);
+                    }
+                }
         }
         __syncthreads();
-        if (threadIdx.x == 0) M[k * DP + k] = sqrt(d);
-        for (int i = k + 1 + wid; i < D; i += NW) {
-            const double li = tmp[i];
-            for (int j = k + 1 + lane; j <= i; j += 64) M[i * DP + j] -= li * tmp[j];
+        // slot columns with every column <= k are finished (a uniform test); inside the
+        // others a zero factor leaves finished rows / columns unchanged
+        double ci[R], cj[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const double a = vec[ta + 16 * u], c = vec[tb + 16 * u];
+            ci[u] = ta + 16 * u > k ? a : 0.0;
+            cj[u] = tb + 16 * u > k ? c : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < R; ++u)
+#pragma unroll
+            for (int v = 0; v <= u; ++v)
+                if (16 * v + 15 > k) m[u][v] -= ci[u] * cj[v];
+        if (k + 1 < D && ta == ((k + 1) & 15) && tb == ta) {
+#pragma unroll
+            for (int u = 0; u < R; ++u)
+                if (u == ((k + 1) >> 4)) *dg = m[u][u];
         }
         __syncthreads();
     }
-    // G = L^{-1} in place, columns from the last to the first (LAPACK trti2, lower)
-    for (int j = D - 1; j >= 0; --j) {
-        const double gjj = 1.0 / M[j * DP + j];
-        // x(j+1:D) := G(j+1:D, j+1:D) * L(j+1:D, j), then scale by -G(j, j)
-        for (int i = j + 1 + threadIdx.x; i < D; i += kInvThreads) {
-            double acc = 0.0;
-            for (int q = j + 1; q <= i; ++q) acc += M[i * DP + q] * M[q * DP + j];
-            tmp[i] = -acc * gjj;
+    // L to LDS; X = L^{-1} by forward elimination of the identity, row k final at step k
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+#pragma unroll
+        for (int v = 0; v <= u; ++v) {
+            const int i = ta + 16 * u, j = tb + 16 * v;
+            if (j <= i) P[packed_lower(i, j)] = m[u][v];
+        }
+    double x[R][R];
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+#pragma unroll
+        for (int v = 0; v <= u; ++v) x[u][v] = (u == v && ta == tb) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int k = 0; k < D; ++k) {
+        if (ta == (k & 15)) {
+            const double lkk = P[packed_lower(k, k)];
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                if (u != (k >> 4)) continue;
+#pragma unroll
+                for (int v = 0; v <= u; ++v) {
+                    const int c = tb + 16 * v;
+                    if (c <= k) {
+                        const double y = x[u][v] / lkk;
+                        x[u][v] = y;
+                        vec[c] = y;
+                    }
+                }
+            }
         }
         __syncthreads();
-        for (int i = j + 1 + threadIdx.x; i < D; i += kInvThreads) M[i * DP + j] = tmp[i];
-        if (threadIdx.x == 0) M[j * DP + j] = gjj;
+        double li[R], rc[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int i = ta + 16 * u;
+            const double a = P[packed_lower(i > k ? i : k, k)], c = vec[tb + 16 * u];
+            li[u] = i > k ? a : 0.0;
+            rc[u] = tb + 16 * u <= k ? c : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < R; ++u)
+#pragma unroll
+            for (int v = 0; v <= u; ++v)
+                if (16 * u + 15 > k && 16 * v <= k) x[u][v] -= li[u] * rc[v];
         __syncthreads();
     }
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+#pragma unroll
+        for (int v = 0; v <= u; ++v) {
+            const int i = ta + 16 * u, j = tb + 16 * v;
+            if (j <= i) P[packed_lower(i, j)] = x[u][v];
+        }
+    __syncthreads();
     TS *dst = out + int64_t(bank) * S;
     bool ok = true;
-    for (int i = wid; i < D; i += NW)
-        for (int j = lane; j <= i; j += 64) {
-            const double v = M[i * DP + j];
-            if (!isfinite(v)) ok = false;
-            dst[packed_lower(i, j)] = to_store<TS>(v);
-        }
-    if (!ok || (threadIdx.x == 0 && bad)) atomicOr(status, 1);
+    for (int q = t; q < S; q += kInvThreads) {
+        const double v = P[q];
+        if (!isfinite(v)) ok = false;
+        dst[q] = to_store<TS>(v);
+    }
+    if (!ok || (t == 0 && bad)) atomicOr(status, 1);
 }
 
 // Galerkin coarse operator values: target slot t sums its source blocks in order.
@@ -592,7 +673,7 @@ void launch_bank_invert(int bs, int32_t nbanks, const int32_t *bank_off, const i
 {
     if (nbanks == 0) return;
     const int D = 32 * bs;
-    const size_t lds = size_t(D * (D + 1) + D) * sizeof(double);
+    const size_t lds = size_t(D * (D + 1) / 2 + D + 2) * sizeof(double);
     switch (bs) {
         case 1:
             k_bank_factor<1, TS><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out, status);
@@ -614,7 +695,7 @@ void mas_set_lds_limits()
 {
     static bool done = false;
     if (done) return;
-    const size_t lds3 = size_t(96 * 97 + 96) * sizeof(double);
+    const size_t lds3 = size_t(96 * 97 / 2 + 96 + 2) * sizeof(double);
     LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, double>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
     LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, float>),

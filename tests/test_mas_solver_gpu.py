@@ -51,7 +51,16 @@ def test_mas_solver_matches_oracle(gpu, name, N, kw):
     oi = H.info()
     assert info["solver_status"] == "Reach relative tolerance" and oi["mas_status"] == 1
     assert info["solver_iter"] == info["num_iterations"]
-    assert abs(info["solver_iter"] - oi["num_iterations"]) <= 10  # both test every 10 iterations
+    # Both test every 10 iterations.  On the contact system the count is not a smooth
+    # function of the rounding: the oracle alone moves between 80 and 100 (elbow) or 100
+    # and 120 (k-means) under 1e-13 relative perturbations of b, so the device count must
+    # lie in the oracle's range over such perturbations, give or take one test period.
+    its = [oi["num_iterations"]]
+    rng = np.random.default_rng(0)
+    for _ in range(4):
+        H.solve(b * (1.0 + 1e-13 * rng.standard_normal(b.size)))
+        its.append(H.info()["num_iterations"])
+    assert min(its) - 10 <= info["solver_iter"] <= max(its) + 10, its
     S = A.to_scipy()
     assert np.linalg.norm(S @ x - b) <= 2e-6 * np.linalg.norm(b)
     assert np.linalg.norm(x - xr) <= 1e-4 * np.linalg.norm(xr)
