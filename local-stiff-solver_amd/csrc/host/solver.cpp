@@ -416,15 +416,10 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     }
     sell_slots_ = sp[nsl];
     d_slice_ptr_.upload(sp.data(), nsl + 1, s);
+    // sell_fill writes every slot (padding: column -1, zero values)
     d_scols_.resize(std::max<int64_t>(sell_slots_, 1));
-    // -1 everywhere first: slots of rows past the last block row stay padding
-    LSS_HIP(hipMemsetAsync(d_scols_.get(), 0xFF, d_scols_.bytes(), s));
     d_svals_.resize(std::max<int64_t>(sell_slots_ * bb, 1));
-    d_svals_.zero(s);
-    if (cfg_.fp32_inner()) {
-        d_svals32_.resize(std::max<int64_t>(sell_slots_ * bb, 1));
-        d_svals32_.zero(s);
-    }
+    if (cfg_.fp32_inner()) d_svals32_.resize(std::max<int64_t>(sell_slots_ * bb, 1));
     k::sell_fill(nb_, bs_, d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), d_slice_ptr_.get(), d_scols_.get(),
                  d_svals_.get(), cfg_.fp32_inner() ? d_svals32_.get() : nullptr, s);
     sell_.nb = nb_;

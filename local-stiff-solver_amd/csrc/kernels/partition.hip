@@ -261,7 +261,8 @@ __global__ __launch_bounds__(kPT) void k_expand(int32_t bx, Sub *__restrict__ S,
 __global__ __launch_bounds__(kPT) void k_advance(Sub *__restrict__ S, const int32_t *__restrict__ mem,
                                                   int32_t *__restrict__ order, int32_t *__restrict__ vis,
                                                   int32_t *__restrict__ dist, int32_t pid0, int32_t max_part,
-                                                  int32_t count_active, int32_t *__restrict__ flags)
+                                                  int32_t count_active, int32_t *__restrict__ active,
+                                                  int32_t *__restrict__ flags)
 {
     __shared__ int32_t sh[kPT];
     __shared__ int32_t found;
@@ -339,7 +340,7 @@ __global__ __launch_bounds__(kPT) void k_advance(Sub *__restrict__ S, const int3
     }
     if (threadIdx.x == 0) {
         S[s] = st;
-        if (count_active && st.phase != 2) atomicAdd(&flags[1], 1);
+        if (count_active && st.phase != 2) atomicAdd(active, 1);
     }
 }
 
@@ -866,6 +867,19 @@ int32_t device_partition(int32_t nb, const int32_t *rowptr, const int32_t *cols,
     S.upload(h0.data(), 1, s);
     int32_t nsub = 1;
     int32_t hflags[2] = {0, 0};
+    // two pinned active counts and their events
+    struct Pinned {
+        int32_t *p = nullptr;
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        ~Pinned()
+        {
+            if (p) (void)hipHostFree(p);
+            for (hipEvent_t e : ev)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } hact;
+    LSS_HIP(hipHostMalloc(reinterpret_cast<void **>(&hact.p), 2 * sizeof(int32_t), hipHostMallocDefault));
+    for (hipEvent_t &e : hact.ev) LSS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     int32_t maxm = real_n;  // largest subset of the current depth
     local_max = std::min<int32_t>(local_max, kLocalCap);
     for (int depth = 0; real_n > 0; ++depth) {
@@ -907,21 +921,33 @@ int32_t device_partition(int32_t nb, const int32_t *rowptr, const int32_t *cols,
         const int32_t bx = std::max(1, 8192 >> depth);  // about 8192 blocks: one pass of 16 rows per block
         const int64_t grid = int64_t(nsub) * bx;
         // each step reaches a node, restarts or switches searches: 4 nb bounds the steps
-        const int64_t max_batches = (4 * int64_t(nb)) / 8 + 16;
-        for (int64_t batch = 0;; ++batch) {
-            if (batch >= max_batches) fail(LSS_ERR_HIP, "device partition did not terminate");
-            constexpr int kBatch = 8;
-            LSS_HIP(hipMemsetAsync(dflags.get() + 1, 0, sizeof(int32_t), s));
+        constexpr int kBatch = 8;
+        const int64_t max_batches = (4 * int64_t(nb)) / kBatch + 16;
+        // batches of level steps, each ending with a count of the searches still active
+        // (device slot 5 + parity, copied to pinned host memory); the next batch is queued
+        // before the host waits for the count of the current one
+        auto enqueue = [&](int slot) {
+            int32_t *act = dflags.get() + 5 + slot;
+            LSS_HIP(hipMemsetAsync(act, 0, sizeof(int32_t), s));
             for (int j = 0; j < kBatch; ++j) {
                 k_expand<<<grid, kPT, 0, s>>>(bx, S.get(), rowptr, cols, order.get(), vis.get(), dist.get(),
                                               sub.get(), pid0);
                 k_advance<<<nsub, kPT, 0, s>>>(S.get(), mem.get(), order.get(), vis.get(), dist.get(), pid0, max_part,
-                                               j == kBatch - 1, dflags.get());
+                                               j == kBatch - 1, act, dflags.get());
             }
             LSS_LAUNCH_CHECK();
-            LSS_HIP(hipMemcpyAsync(hflags + 1, dflags.get() + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-            LSS_HIP(hipStreamSynchronize(s));
-            if (hflags[1] == 0) break;
+            LSS_HIP(hipMemcpyAsync(hact.p + slot, act, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            LSS_HIP(hipEventRecord(hact.ev[slot], s));
+        };
+        enqueue(0);
+        for (int64_t batch = 0;; ++batch) {
+            if (batch >= max_batches) {
+                LSS_HIP(hipStreamSynchronize(s));
+                fail(LSS_ERR_HIP, "device partition did not terminate");
+            }
+            enqueue(int((batch + 1) & 1));
+            LSS_HIP(hipEventSynchronize(hact.ev[batch & 1]));
+            if (hact.p[batch & 1] == 0) break;
         }
         // split every searched subset (flag, rank and aux are free here)
         const int gq = div_up(real_n, 256);

@@ -180,8 +180,12 @@ __global__ __launch_bounds__(256) void k_bsr_fill(int32_t n, int bs, const int32
         for (int r = R.rows; r < bs; ++r) bvals[int64_t(diag_slot) * bb + r * bs + r] = 1.0;
 }
 
-// One wave per block row: each lane takes entries of the row, ranks them by
-// permuted column among all entries of the row and moves its block.
+// One wave per block row: the permuted columns of the row are ranked in LDS,
+// then the row's blocks move element by element, so that the wave reads the
+// source row and writes the destination row contiguously.  Rows longer than
+// kPermCap blocks rank and move one block per lane.
+constexpr int kPermCap = 256;
+
 __global__ __launch_bounds__(256) void k_bsr_permute(int32_t nb, int bs, const int32_t *__restrict__ rowptr,
                                                       const int32_t *__restrict__ cols,
                                                       const double *__restrict__ vals,
@@ -189,12 +193,35 @@ __global__ __launch_bounds__(256) void k_bsr_permute(int32_t nb, int bs, const i
                                                       const int32_t *__restrict__ new_rowptr,
                                                       int32_t *__restrict__ new_cols, double *__restrict__ new_vals)
 {
-    const int lane = threadIdx.x & 63;
-    const int32_t I = int32_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    __shared__ int32_t pc[4][kPermCap], rk[4][kPermCap];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int32_t I = int32_t(blockIdx.x) * 4 + wv;
     if (I >= nb) return;
     const int bb = bs * bs;
-    const int32_t b = rowptr[I], e = rowptr[I + 1];
+    const int32_t b = rowptr[I], e = rowptr[I + 1], m = e - b;
     const int32_t dst = new_rowptr[perm[I]];
+    if (m <= kPermCap) {
+        for (int32_t k = lane; k < m; k += 64) pc[wv][k] = perm[cols[b + k]];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int32_t k = lane; k < m; k += 64) {
+            const int32_t ck = pc[wv][k];
+            int32_t rank = 0;
+            for (int32_t q = 0; q < m; ++q) rank += pc[wv][q] < ck;
+            rk[wv][k] = rank;
+            new_cols[dst + rank] = ck;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const double *src = vals + int64_t(b) * bb;
+        for (int32_t q = lane; q < m * bb; q += 64) {
+            const int32_t k = q / bb;
+            new_vals[int64_t(dst + rk[wv][k]) * bb + (q - k * bb)] = src[q];
+        }
+        return;
+    }
     for (int32_t k = b + lane; k < e; k += 64) {
         const int32_t ck = perm[cols[k]];
         int32_t rank = 0;
@@ -216,33 +243,33 @@ __global__ void k_sell_width(int32_t nb, int H, const int32_t *__restrict__ rowp
     width[s] = w;
 }
 
-__global__ void k_sell_fill(int32_t nb, int bs, int H, const int32_t *__restrict__ rowptr,
-                            const int32_t *__restrict__ cols, const double *__restrict__ vals,
-                            const int64_t *__restrict__ slice_ptr, int32_t *__restrict__ scols,
-                            double *__restrict__ svals, float *__restrict__ svals32)
+// One wave per slice, lane l = h * bs + rr (scalar row rr of the slice's block
+// row h): every slot of the slice is written, padding included (column -1,
+// zero values), and a slot's values are written by consecutive lanes.
+__global__ __launch_bounds__(256) void k_sell_fill(int32_t nb, int bs, int H, int32_t nslices,
+                                                    const int32_t *__restrict__ rowptr,
+                                                    const int32_t *__restrict__ cols, const double *__restrict__ vals,
+                                                    const int64_t *__restrict__ slice_ptr, int32_t *__restrict__ scols,
+                                                    double *__restrict__ svals, float *__restrict__ svals32)
 {
-    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nb) return;
+    const int lane = threadIdx.x & 63;
+    const int32_t sl = int32_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (sl >= nslices || lane >= H * bs) return;
     const int bb = bs * bs;
-    const int32_t s = r / H, h = r % H;
-    const int64_t base = slice_ptr[s];
-    const int32_t W = int32_t((slice_ptr[s + 1] - base) / H);
-    const int32_t b = rowptr[r], len = rowptr[r + 1] - b;
+    const int h = lane / bs, rr = lane - h * bs;
+    const int32_t row = sl * H + h;
+    const int32_t b = row < nb ? rowptr[row] : 0, len = row < nb ? rowptr[row + 1] - b : 0;
+    const int64_t base = slice_ptr[sl];
+    const int32_t W = int32_t((slice_ptr[sl + 1] - base) / H);
     for (int32_t j = 0; j < W; ++j) {
-        const int64_t slot = base + int64_t(j) * H + h;
-        if (j < len) {
-            scols[slot] = cols[b + j];
-            // value (rr, c) of the block -> lane l = h*bs + rr, component c
-            const int64_t vbase = (base + int64_t(j) * H) * bb;
-            for (int rr = 0; rr < bs; ++rr)
-                for (int c = 0; c < bs; ++c) {
-                    const double v = vals[int64_t(b + j) * bb + rr * bs + c];
-                    const int64_t at = vbase + int64_t(c) * (H * bs) + h * bs + rr;
-                    svals[at] = v;
-                    if (svals32) svals32[at] = static_cast<float>(v);
-                }
-        } else {
-            scols[slot] = -1;
+        const bool have = j < len;
+        if (rr == 0) scols[base + int64_t(j) * H + h] = have ? cols[b + j] : -1;
+        const int64_t vbase = (base + int64_t(j) * H) * bb;
+        for (int c = 0; c < bs; ++c) {
+            const double v = have ? vals[int64_t(b + j) * bb + rr * bs + c] : 0.0;
+            const int64_t at = vbase + int64_t(c) * (H * bs) + lane;
+            svals[at] = v;
+            if (svals32) svals32[at] = static_cast<float>(v);
         }
     }
 }
@@ -412,8 +439,9 @@ void sell_fill(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols, c
                const int64_t *slice_ptr, int32_t *scols, double *svals, float *svals32, hipStream_t s)
 {
     if (nb == 0) return;
-    k_sell_fill<<<div_up(nb, 256), 256, 0, s>>>(nb, bs, 64 / bs, rowptr, cols, vals, slice_ptr, scols, svals,
-                                                  svals32);
+    const int H = 64 / bs;
+    const int32_t ns = div_up(nb, H);
+    k_sell_fill<<<div_up(ns, 4), 256, 0, s>>>(nb, bs, H, ns, rowptr, cols, vals, slice_ptr, scols, svals, svals32);
     LSS_LAUNCH_CHECK();
 }
 
