@@ -29,6 +29,9 @@ constexpr int kSpmvThreads = 256;  // 4 waves = 4 slices per workgroup
 // columns are collected in LDS and ranked by counting; a block row with more
 // than kUnionCap distinct block columns is done by lane 0 alone.
 constexpr int kUnionCap = 256;
+// A block row of at most kStageCap entries is first copied to LDS, so that the
+// union tests search LDS instead of global memory.
+constexpr int kStageCap = 512;
 
 struct RowRanges {
     int32_t p[3], e[3];
@@ -82,15 +85,35 @@ __global__ __launch_bounds__(256) void k_bsr_count(int32_t n, int bs, const int3
     const int32_t nb = (n + bs - 1) / bs;
     const int32_t I = int32_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
     if (I >= nb) return;
-    const RowRanges R = block_rows(n, bs, ptr, I);
+    __shared__ int32_t stage[4][kStageCap];
+    RowRanges R = block_rows(n, bs, ptr, I);
     const int32_t lo = R.p[0], hi = R.e[R.rows - 1];
     int32_t c = 0, bad = 0, diag = 0;
-    for (int32_t k = lo + lane; k < hi; k += 64) {
-        const int r = row_of(R, k);
-        if (k > R.p[r] && idx[k] <= idx[k - 1]) bad = 1;
-        const bool first = union_first(idx, R, r, k, bs);
-        c += first;
-        diag |= first && idx[k] / bs == I;
+    if (hi - lo <= kStageCap) {
+        int32_t *ix = stage[threadIdx.x >> 6];
+        for (int32_t k = lane; k < hi - lo; k += 64) ix[k] = idx[lo + k];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int r = 0; r < R.rows; ++r) {
+            R.p[r] -= lo;
+            R.e[r] -= lo;
+        }
+        for (int32_t k = lane; k < hi - lo; k += 64) {
+            const int r = row_of(R, k);
+            if (k > R.p[r] && ix[k] <= ix[k - 1]) bad = 1;
+            const bool first = union_first(ix, R, r, k, bs);
+            c += first;
+            diag |= first && ix[k] / bs == I;
+        }
+    } else {
+        for (int32_t k = lo + lane; k < hi; k += 64) {
+            const int r = row_of(R, k);
+            if (k > R.p[r] && idx[k] <= idx[k - 1]) bad = 1;
+            const bool first = union_first(idx, R, r, k, bs);
+            c += first;
+            diag |= first && idx[k] / bs == I;
+        }
     }
     for (int o = 32; o > 0; o >>= 1) {
         c += __shfl_xor(c, o, 64);
@@ -109,6 +132,7 @@ __global__ __launch_bounds__(256) void k_bsr_fill(int32_t n, int bs, const int32
                                                    double *__restrict__ bvals)
 {
     __shared__ int32_t ucol[4][kUnionCap];
+    __shared__ int32_t stage[4][kStageCap];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int32_t nb = (n + bs - 1) / bs;
     const int32_t I = int32_t(blockIdx.x) * 4 + wv;
@@ -119,6 +143,47 @@ __global__ __launch_bounds__(256) void k_bsr_fill(int32_t n, int bs, const int32
     const int32_t base = rowptr[I];
     const int32_t m = rowptr[I + 1] - base;  // distinct block columns (with a padded tail diagonal)
     int32_t *u = ucol[wv];
+    if (m <= kUnionCap && hi - lo <= kStageCap) {
+        // the row's entries in LDS, positions relative to lo
+        int32_t *ix = stage[wv];
+        for (int32_t k = lane; k < hi - lo; k += 64) ix[k] = idx[lo + k];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        RowRanges L = R;
+        for (int r = 0; r < R.rows; ++r) {
+            L.p[r] -= lo;
+            L.e[r] -= lo;
+        }
+        int32_t at = 0;
+        for (int32_t k0 = 0; k0 < hi - lo; k0 += 64) {
+            const int32_t k = k0 + lane;
+            const bool first = k < hi - lo && union_first(ix, L, row_of(L, k), k, bs);
+            const unsigned long long mask = __ballot(first);
+            if (first) u[at + __popcll(mask & ((1ull << lane) - 1ull))] = ix[k] / bs;
+            at += __popcll(mask);
+        }
+        if (at < m && lane == 0) u[at] = I;  // padded tail diagonal block
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int32_t q = lane; q < m; q += 64) {
+            const int32_t c = u[q];
+            int32_t rank = 0;
+            for (int32_t t = 0; t < m; ++t) rank += u[t] < c;
+            cols[base + rank] = c;
+            if (c == I && R.rows < bs)
+                for (int r = R.rows; r < bs; ++r) bvals[int64_t(base + rank) * bb + r * bs + r] = 1.0;
+        }
+        for (int32_t k = lane; k < hi - lo; k += 64) {
+            const int r = row_of(L, k);
+            const int32_t c = ix[k] / bs;
+            int32_t rank = 0;
+            for (int32_t t = 0; t < m; ++t) rank += u[t] < c;
+            bvals[int64_t(base + rank) * bb + r * bs + (ix[k] - c * bs)] = vals[lo + k];
+        }
+        return;
+    }
     if (m <= kUnionCap) {
         // collect the union's block columns in LDS (any order)
         int32_t at = 0;
