@@ -524,6 +524,41 @@ __device__ __forceinline__ void stage_matrix(const TS *__restrict__ src, TL *__r
 constexpr int kApplyThreads = 256;
 static_assert(kApplyThreads >= 96, "one thread per bank row in the epilogue");
 
+// stage_matrix in two steps: the 16-byte loads of a bank into registers, and
+// (after they land) the conversion into LDS, so that a workgroup can issue the
+// loads of its next bank before it solves the current one
+template <class TS, int S, int NT>
+struct MatrixRegs {
+    static constexpr int VEC = 16 / sizeof(TS);
+    static constexpr int NV = S / VEC;
+    static constexpr int PER = (NV + NT - 1) / NT;
+    typedef unsigned int V __attribute__((ext_vector_type(4)));
+    V reg[PER];
+    __device__ __forceinline__ void issue(const TS *__restrict__ src)
+    {
+        const V *sv = reinterpret_cast<const V *>(src);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int t = threadIdx.x + q * NT;
+            if (t < NV) reg[q] = __builtin_nontemporal_load(&sv[t]);
+        }
+    }
+    template <class TL>
+    __device__ __forceinline__ void store(const TS *__restrict__ src, TL *__restrict__ dst) const
+    {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int t = threadIdx.x + q * NT;
+            if (t < NV) {
+                const TS *e = reinterpret_cast<const TS *>(&reg[q]);
+#pragma unroll
+                for (int u = 0; u < VEC; ++u) dst[t * VEC + u] = static_cast<TL>(to_lds_any<TS>(e[u]));
+            }
+        }
+        for (int t = NV * VEC + threadIdx.x; t < S; t += NT) dst[t] = static_cast<TL>(to_lds_any<TS>(src[t]));
+    }
+};
+
 // bank vector of each column into LDS (padded positions, zero past n entries)
 template <int BS, int K>
 __device__ __forceinline__ void load_bank_vector(const double *__restrict__ src, int64_t ld, int n,
@@ -638,6 +673,78 @@ __global__ __launch_bounds__(kApplyThreads) void k_level0_apply(int32_t nbanks, 
 #pragma unroll
         for (int k = 0; k < K; ++k)
             dev::store_partial(dev::block_sum<kApplyThreads>(part[k], sh), partials + int64_t(k) * nbanks);
+    }
+}
+
+// Level 0 for one right-hand side, two banks per workgroup (k_level0_apply's
+// steps per bank); the FP64 banks (37 KB each) gain most from the overlap.
+template <int BS, class TS, bool DOT, int K>
+__global__ __launch_bounds__(kApplyThreads) void k_level0_apply2(int32_t nbanks, const int32_t *__restrict__ part_off,
+                                                                const TS *__restrict__ inv,
+                                                                const double *__restrict__ r, int64_t ld,
+                                                                const int32_t *__restrict__ parent,
+                                                                const double *__restrict__ w1, int64_t ld1,
+                                                                double scale, const double *__restrict__ add,
+                                                                double *__restrict__ z,
+                                                                const double *__restrict__ dotv, double *partials,
+                                                                const int *__restrict__ stop)
+{
+    constexpr int D = 32 * BS;
+    constexpr int S = D * (D + 1) / 2;
+    using TL = typename LdsType<TS>::type;
+    if (dev::all_stopped(stop, K)) return;
+    __shared__ __attribute__((aligned(16))) TL A[S + 4];
+    __shared__ double xs[K * (D + BS)], ts[K * (D + BS)], P[K * BS * D], Q[K * BS * D];
+    const int i = threadIdx.x;
+    // two consecutive banks per workgroup; the matrix loads of the second are in
+    // flight while the first is solved
+    const int32_t first = int32_t(blockIdx.x) * 2;
+    const int32_t last = min(nbanks, first + 2);
+    MatrixRegs<TS, S, kApplyThreads> mr;
+    mr.issue(inv + int64_t(first) * S);
+    for (int32_t bank = first; bank < last; ++bank) {
+        const int32_t lo = part_off[bank];
+        const int n = (part_off[bank + 1] - lo) * BS;
+        const int64_t gi = int64_t(lo) * BS + i;
+        // epilogue operands first: their latency overlaps the staging of the bank
+        double cw[K], ca[K], cd[K];
+        if (i < n) {
+            const int64_t pi = w1 ? int64_t(parent[lo + i / BS]) * BS + (i % BS) : 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                cw[k] = w1 ? w1[k * ld1 + pi] : 0.0;
+                ca[k] = add ? add[k * ld + gi] : 0.0;
+                if constexpr (DOT) cd[k] = dotv[k * ld + gi];
+            }
+        }
+        load_bank_vector<BS, K>(r + int64_t(lo) * BS, ld, n, xs);
+        mr.store(inv + int64_t(bank) * S, A);
+        __syncthreads();
+        if (bank + 1 < last) mr.issue(inv + int64_t(bank + 1) * S);
+        bank_solve<BS, TL, K>(A, xs, ts, P, Q);
+        double part[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) part[k] = 0.0;
+        if (i < n) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                double y = bank_result<BS>(Q, k, i);
+                if (w1) y += cw[k];
+                y *= scale;
+                if (add) y += ca[k];
+                z[k * ld + gi] = y;
+                if constexpr (DOT) part[k] = cd[k] * y;
+            }
+        }
+        if constexpr (DOT) {
+            __shared__ double sh[kApplyThreads / 64];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const double tot = dev::block_sum<kApplyThreads>(part[k], sh);
+                if (threadIdx.x == 0) partials[int64_t(k) * nbanks + bank] = tot;
+            }
+        }
+        __syncthreads();  // A, xs and Q are rewritten for the next bank
     }
 }
 
@@ -801,9 +908,15 @@ void level0_apply_k(Precision prec, int bs, int32_t nbanks, const int32_t *part_
                     hipStream_t s)
 {
 #define LSS_L0(BS, TS, DOT)                                                                                    \
-    k_level0_apply<BS, TS, DOT, K><<<nbanks, kApplyThreads, 0, s>>>(nbanks, part_off,                          \
-                                                                    static_cast<const TS *>(inv), r, ld, parent, \
-                                                                    w1, ld1, scale, add, z, dotv, partials, stop)
+    if constexpr (K == 1)                                                                                      \
+        k_level0_apply2<BS, TS, DOT, K><<<div_up(nbanks, 2), kApplyThreads, 0, s>>>(                           \
+            nbanks, part_off, static_cast<const TS *>(inv), r, ld, parent, w1, ld1, scale, add, z, dotv,        \
+            partials, stop);                                                                                   \
+    else                                                                                                       \
+        k_level0_apply<BS, TS, DOT, K><<<nbanks, kApplyThreads, 0, s>>>(nbanks, part_off,                      \
+                                                                        static_cast<const TS *>(inv), r, ld,   \
+                                                                        parent, w1, ld1, scale, add, z, dotv,  \
+                                                                        partials, stop)
 #define LSS_L0_BS(TS)                                                                                          \
     switch (bs) {                                                                                              \
         case 1: if (dot) LSS_L0(1, TS, true); else LSS_L0(1, TS, false); break;                                \
