@@ -51,7 +51,8 @@ typedef struct lss_solver lss_solver;
  * GPUHybrid::set_parameters (GPUHybrid.cu:106-179).  Additional keys:
  * "precision" (fp64 | fp32 | bf16 | fp16 bank storage), "inner_spmv" (auto | fp32 |
  * fp64), "base_preconditioner" (MAS | Jacobi), "partitioner" (device | host),
- * "max_coarse_levels" (0-4), "damping", "lanczos_steps", "check_every",
+ * "partition_local_max" (0-8192), "max_coarse_levels" (0-4), "damping",
+ * "lanczos_steps", "check_every",
  * "max_dense_component", "true_residual_check", "profile_kernels"; unknown keys
  * are rejected.  json may be NULL or "" for all defaults.
  * Replaces: Solver::create(json, logger) + GPUHybrid::GPUHybrid() (GPUHybrid.cu:79-104).
