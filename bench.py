@@ -53,10 +53,17 @@ def cpu_baseline(prob, A, b, gpu_iters: int, sample_iters: int, threads: int):
     H.solve(b)
     t_iter = (time.perf_counter() - t0) / max(1, H.info()["num_iterations"])
     value = t_setup + t_iter * gpu_iters
+    # the same PCG iteration on one thread (a few iterations, same factorization)
+    H.set_run(1, 3)
+    t0 = time.perf_counter()
+    H.solve(b)
+    t_iter1 = (time.perf_counter() - t0) / max(1, H.info()["num_iterations"])
     return {"value": value, "unit": "s", "cores": threads, "kind": "port",
+            "pcg_iteration_s": {str(threads): round(t_iter, 4), "1": round(t_iter1, 4)},
             "sample": f"oracle factorize ({t_setup:.2f} s) + {sample_iters} PCG iterations "
-                      f"({t_iter * 1e3:.1f} ms each) on the same system, extrapolated to the "
-                      f"{gpu_iters} iterations of the FP64 GPU path"}
+                      f"({t_iter * 1e3:.1f} ms each) on {threads} threads on the same system, "
+                      f"extrapolated to the {gpu_iters} iterations of the FP64 GPU path; "
+                      f"3 iterations on 1 thread: {t_iter1 * 1e3:.0f} ms each"}
 
 
 def main():

@@ -1410,8 +1410,17 @@ extern "C" int orc_hybrid_mas_status(const orc_hybrid *h, double *residual)
     return h->mas_status;
 }
 
+extern "C" void orc_hybrid_set_run(orc_hybrid *h, int threads, int max_iter)
+{
+    h->cfg.threads = threads;
+    h->cfg.max_iter = max_iter;
+}
+
 extern "C" int orc_hybrid_solve(orc_hybrid *h, const double *b, double *xio)
 {
+#ifdef _OPENMP
+    if (h->cfg.threads > 0) omp_set_num_threads(h->cfg.threads);
+#endif
     if (h->cfg.solver_mode == 1) return mas_solve(h, b, xio);
     // GPUHybrid::pcg_solve, GPUHybrid.cu:1413-1512
     auto t0 = clk::now();

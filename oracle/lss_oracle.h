@@ -169,6 +169,9 @@ void orc_hybrid_set_partition(orc_hybrid *h, const int32_t *perm, int32_t nb, co
 double orc_hybrid_omega(const orc_hybrid *h);
 /* 1 when the last solve broke down with the configured damping and was redone with damping 1.5 */
 int orc_hybrid_damping_retry(const orc_hybrid *h);
+/* Thread count (0: OpenMP default) and iteration cap of the following solves (timing of the
+   CPU baseline at another thread count without a new factorization). */
+void orc_hybrid_set_run(orc_hybrid *h, int threads, int max_iter);
 /* Export the stiff components (sorted by smallest DOF). Returns count written. */
 int64_t orc_hybrid_stiff(const orc_hybrid *h, int32_t *dofs, int64_t cap, int64_t *comp_off, int64_t ccap);
 /* Apply the full hybrid preconditioner once: z = M^{-1} r (after factorize). */

@@ -97,6 +97,8 @@ def lib():
         L.orc_hybrid_omega.restype = ctypes.c_double
         L.orc_hybrid_omega.argtypes = [P]
         L.orc_hybrid_damping_retry.argtypes = [P]
+        L.orc_hybrid_set_run.restype = None
+        L.orc_hybrid_set_run.argtypes = [P, ctypes.c_int, ctypes.c_int]
         L.orc_hybrid_partition.restype = ctypes.c_int32
         L.orc_hybrid_partition.argtypes = [P, P, P]
         _lib = L
@@ -336,6 +338,10 @@ class Hybrid:
         z = np.empty_like(r)
         lib().orc_hybrid_precond(self._h, _p(r), _p(z))
         return z
+
+    def set_run(self, threads, max_iter):
+        """Thread count and iteration cap of the following solves."""
+        lib().orc_hybrid_set_run(self._h, int(threads), int(max_iter))
 
     def info(self):
         it = ctypes.c_int()
