@@ -129,10 +129,8 @@ std::vector<int64_t> HybridSolver::mas_gmm_positions(int64_t n, const double *x,
     if (std::isnan(st.ratio) || st.ratio < cfg_.bad_dof_gmm_jump_threshold) return {};
     DBuf<uint8_t> fl(n);
     k::gmm_flags(n, g0.get(), g1.get(), fl.get(), s, high);
-    const auto hf = to_host(fl.get(), size_t(n), s);
-    std::vector<int64_t> pos;
-    for (int64_t i = 0; i < n; ++i)
-        if (hf[i]) pos.push_back(i);
+    const auto sel = k::flagged_indices(int32_t(n), fl.get(), s);
+    const std::vector<int64_t> pos(sel.begin(), sel.end());
     st.selected = int64_t(pos.size());
     return expand_windows(pos, n, cfg_.bad_dof_gmm_expand_neighbors);
 }

@@ -145,7 +145,6 @@ void HybridSolver::factorize_device(int32_t n, int64_t nnz, const int32_t *ptr, 
     if (n <= 0) fail(LSS_ERR_SIZE, "matrix must have at least one row");
     auto t0 = clk::now();
     release();
-    h_ptr_.clear();
     n_ = n;
     bs_ = cfg_.block_size;
     nb_ = (n + bs_ - 1) / bs_;
@@ -193,8 +192,6 @@ void HybridSolver::factorize_device(int32_t n, int64_t nnz, const int32_t *ptr, 
     estimate_omega(s);
     t_omega_ = since(t);
     factorized_ = true;
-    h_ptr_.clear();
-    h_ptr_.shrink_to_fit();
     t_factorize_ = since(t0);
 }
 
@@ -206,19 +203,12 @@ RowSet HybridSolver::fetch_rows(const std::vector<int32_t> &rows, const int32_t 
     const int64_t m = int64_t(rows.size());
     rs.ptr.assign(m + 1, 0);
     if (m == 0) return rs;
-    DBuf<int32_t> d_rows;
+    DBuf<int32_t> d_rows, d_len(m);
     d_rows.upload(rows.data(), m, s);
-    if (h_ptr_.size() != size_t(n_) + 1) {
-        h_ptr_.resize(size_t(n_) + 1);
-        LSS_HIP(hipMemcpyAsync(h_ptr_.data(), ptr, sizeof(int32_t) * (size_t(n_) + 1), hipMemcpyDeviceToHost, s));
-        sync(s);
-    }
-    std::vector<int32_t> lo(m), hi(m);
-    for (int64_t k = 0; k < m; ++k) {
-        lo[k] = h_ptr_[rows[k]];
-        hi[k] = h_ptr_[rows[k] + 1];
-    }
-    for (int64_t k = 0; k < m; ++k) rs.ptr[k + 1] = rs.ptr[k] + (hi[k] - lo[k]);
+    // row lengths on the device: only m values cross to the host, not the column pointers
+    k::row_lengths(m, d_rows.get(), ptr, d_len.get(), s);
+    const auto len = to_host(d_len, m, s);
+    for (int64_t k = 0; k < m; ++k) rs.ptr[k + 1] = rs.ptr[k] + len[k];
     const int64_t tot = rs.ptr[m];
     DBuf<int64_t> d_optr;
     d_optr.upload(rs.ptr.data(), m + 1, s);
@@ -303,9 +293,7 @@ void HybridSolver::select_stiff(int32_t n, const int32_t *ptr, const int32_t *id
         if (gmm_.jumped) {
             DBuf<uint8_t> fl(n);
             k::gmm_flags(n, g0.get(), g1.get(), fl.get(), s);
-            auto hf = to_host(fl, n, s);
-            for (int32_t i = 0; i < n; ++i)
-                if (hf[i]) S.push_back(i);
+            S = k::flagged_indices(n, fl.get(), s);
         }
         gmm_.selected = int64_t(S.size());
     } else {

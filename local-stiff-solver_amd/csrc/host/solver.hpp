@@ -187,7 +187,6 @@ private:
     // kernel timing (profile_kernels)
     double prof_spmv_outer_ms_ = 0, prof_spmv_inner_ms_ = 0, prof_mas0_ms_ = 0;
     int prof_spmv_outer_n_ = 0, prof_spmv_inner_n_ = 0, prof_mas0_n_ = 0;
-    std::vector<int32_t> h_ptr_;  // host copy of the row pointer during factorize
     bool pending_skip_ = false;
     std::vector<hipEvent_t> ev_pool_;
     std::vector<std::pair<int, int>> ev_marks_;  // (kind, event index)

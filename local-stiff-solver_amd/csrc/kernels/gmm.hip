@@ -7,6 +7,8 @@
 // the CPU restatement's canonical sum.  The reference reduces with thrust in an
 // unspecified order.  Expressions keep the reference's operation order and this
 // file is compiled without FP contraction.
+#include <vector>
+
 #include "kernels.hpp"
 #include "../common/runtime.hpp"
 
@@ -267,6 +269,57 @@ void gmm_flags(int64_t n, const double *g0, const double *g1, uint8_t *flags, hi
 {
     if (n == 0) return;
     k_flags<<<div_up(n, 256), 256, 0, s>>>(n, g0, g1, flags, high);
+    LSS_LAUNCH_CHECK();
+}
+
+namespace {
+
+__global__ void k_flags_i32(int32_t n, const uint8_t *__restrict__ f, int32_t *__restrict__ out)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = f[i] ? 1 : 0;
+}
+
+__global__ void k_scatter_flagged(int32_t n, const uint8_t *__restrict__ f, const int32_t *__restrict__ rank,
+                                  int32_t *__restrict__ out)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && f[i]) out[rank[i]] = int32_t(i);
+}
+
+__global__ void k_row_lengths(int64_t m, const int32_t *__restrict__ rows, const int32_t *__restrict__ ptr,
+                              int32_t *__restrict__ len)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < m) len[k] = ptr[rows[k] + 1] - ptr[rows[k]];
+}
+
+}  // namespace
+
+std::vector<int32_t> flagged_indices(int32_t n, const uint8_t *flags, hipStream_t s)
+{
+    std::vector<int32_t> h;
+    if (n <= 0) return h;
+    DBuf<int32_t> f(n), rank(n), tsum(std::max(div_up(n, 1024), 1)), total(1);
+    k_flags_i32<<<div_up(n, 256), 256, 0, s>>>(n, flags, f.get());
+    excl_scan_i32(n, f.get(), rank.get(), tsum.get(), total.get(), s);
+    int32_t cnt = 0;
+    LSS_HIP(hipMemcpyAsync(&cnt, total.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    LSS_HIP(hipStreamSynchronize(s));
+    if (cnt == 0) return h;
+    DBuf<int32_t> out(cnt);
+    k_scatter_flagged<<<div_up(n, 256), 256, 0, s>>>(n, flags, rank.get(), out.get());
+    LSS_LAUNCH_CHECK();
+    h.resize(size_t(cnt));
+    LSS_HIP(hipMemcpyAsync(h.data(), out.get(), sizeof(int32_t) * size_t(cnt), hipMemcpyDeviceToHost, s));
+    LSS_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+void row_lengths(int64_t m, const int32_t *rows, const int32_t *ptr, int32_t *len, hipStream_t s)
+{
+    if (m == 0) return;
+    k_row_lengths<<<div_up(m, 256), 256, 0, s>>>(m, rows, ptr, len);
     LSS_LAUNCH_CHECK();
 }
 

@@ -1,5 +1,6 @@
 // Launchers of the hand-written gfx950 kernels.  All take an explicit stream.
 #pragma once
+#include <vector>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -102,6 +103,10 @@ void gmm_var_global(int64_t n, const double *x, double mean, double *partials1, 
 void csum_finish(int nsums, const double *partials, double *out, hipStream_t s);
 // flags[i] = g1[i] > g0[i] (high = 1) or g0[i] > g1[i] (high = 0)
 void gmm_flags(int64_t n, const double *g0, const double *g1, uint8_t *flags, hipStream_t s, int high = 1);
+// Ascending indices of the nonzero flags (compacted on the device; synchronizes s).
+std::vector<int32_t> flagged_indices(int32_t n, const uint8_t *flags, hipStream_t s);
+// len[k] = ptr[rows[k] + 1] - ptr[rows[k]] for k < m.
+void row_lengths(int64_t m, const int32_t *rows, const int32_t *ptr, int32_t *len, hipStream_t s);
 
 // ---------------------------------------------------------------- MAS solver selection
 // Per permuted scalar row of the block operator: mode 0 sum |a_ij|, 1 |a_ii|,
