@@ -121,8 +121,10 @@ def test_partition_matches_oracle(gpu, name, N, variant):
     # each; partition_local_max 0: level-synchronous searches only; 200: both kinds;
     # host: the OpenMP implementation
     prob = problems.make(name, N=N)
-    A = prob.csc()
-    bs = prob.block_size
+    _check_partition(prob.csc(), prob.block_size, variant)
+
+
+def _check_partition(A, bs, variant):
     s = Solver.create(_cfg(bs, **variant))
     s.factorize(A)
     nb = A.n // bs
@@ -143,6 +145,48 @@ def test_partition_matches_oracle(gpu, name, N, variant):
     want = np.empty(nb, dtype=np.int64)
     want[np.argsort(part, kind="stable")] = np.arange(nb)
     assert np.array_equal(perm, want)
+
+
+def _forest_matrix(seed):
+    """SPD M-matrix D - W on a forest: random recursive trees of 3 to 9000 nodes (a BFS cut
+    of a tree leaves disconnected branches, so both searches restart often), a few chords,
+    isolated nodes, ids shuffled."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(seed)
+    sizes = [3, 31, 32, 33, 700, 2900, 9000]
+    rows, cols = [], []
+    base = 0
+    for m in sizes:
+        par = np.array([rng.integers(0, v) for v in range(1, m)], dtype=np.int64)
+        rows.append(base + np.arange(1, m))
+        cols.append(base + par)
+        k = m // 50
+        if k:
+            a, b = rng.integers(0, m, k), rng.integers(0, m, k)
+            keep = a != b
+            rows.append(base + a[keep])
+            cols.append(base + b[keep])
+        base += m
+    n = base + 57  # isolated nodes
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    w = rng.uniform(0.1, 1.0, r.size)
+    W = sp.coo_matrix((np.r_[w, w], (np.r_[r, c], np.r_[c, r])), shape=(n, n)).tocsr()
+    W.sum_duplicates()
+    p = rng.permutation(n)
+    W = W[p][:, p]
+    A = (sp.diags(np.asarray(W.sum(axis=1)).ravel() + 1.0) - W).tocsc()
+    A.sort_indices()
+    return problems.CSC(n, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64))
+
+
+@pytest.mark.parametrize("variant", [{}, {"partition_local_max": 0}, {"partition_local_max": 33},
+                                     {"partition_local_max": 200}])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_partition_forests_match_oracle(gpu, seed, variant):
+    # trees of 3 ... 9000 nodes: restarts in both searches of both device paths, parts of
+    # exactly 32 and 33 nodes, a subset above the single-workgroup limit, isolated nodes
+    _check_partition(_forest_matrix(seed), 1, variant)
 
 
 # ---------------------------------------------------------------- reference example system
