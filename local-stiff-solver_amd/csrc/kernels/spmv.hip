@@ -373,24 +373,38 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslic
         const int ll = hh * BS + r;  // lane within the slot's value layout
         constexpr int C = K >= 4 ? 2 : 4;
         int32_t j0 = 0;
-        for (; j0 + C <= W; j0 += C) {
-            int32_t c[C];
+        // the column indices of a chunk are loaded during the previous chunk, so a chunk
+        // waits on one memory latency (values and x together), not on cols then x
+        int32_t cn[C];
+        if (C <= W) {
 #pragma unroll
-            for (int u = 0; u < C; ++u) c[u] = __builtin_nontemporal_load(&cols[base + int64_t(j0 + u) * H + hh]);
+            for (int u = 0; u < C; ++u) cn[u] = __builtin_nontemporal_load(&cols[base + int64_t(u) * H + hh]);
+        }
+        for (; j0 + C <= W; j0 += C) {
             double a[C][BS], xv[C][K][BS];
 #pragma unroll
             for (int u = 0; u < C; ++u) {
                 const int64_t vb = (base + int64_t(j0 + u) * H) * BB + ll;
-                const int64_t xc = int64_t(c[u] < 0 ? 0 : c[u]) * BS;
 #pragma unroll
                 for (int q = 0; q < BS; ++q) {
                     if constexpr (F32)
                         a[u][q] = double(__builtin_nontemporal_load(&vals32[vb + q * (H * BS)]));
                     else
                         a[u][q] = __builtin_nontemporal_load(&vals[vb + q * (H * BS)]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < C; ++u) {
+                const int64_t xc = int64_t(max(cn[u], 0)) * BS;  // padding: column -1, zero values
+#pragma unroll
+                for (int q = 0; q < BS; ++q)
 #pragma unroll
                     for (int k = 0; k < K; ++k) xv[u][k][q] = x[k * ld + xc + q];
-                }
+            }
+            if (j0 + 2 * C <= W) {
+#pragma unroll
+                for (int u = 0; u < C; ++u)
+                    cn[u] = __builtin_nontemporal_load(&cols[base + int64_t(j0 + C + u) * H + hh]);
             }
 #pragma unroll
             for (int u = 0; u < C; ++u)
@@ -402,7 +416,7 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslic
         for (; j0 < W; ++j0) {
             const int64_t slot = base + int64_t(j0) * H + hh;
             const int32_t c0 = __builtin_nontemporal_load(&cols[slot]);
-            const int64_t xc = int64_t(c0 < 0 ? 0 : c0) * BS;
+            const int64_t xc = int64_t(max(c0, 0)) * BS;
             const int64_t vb = (base + int64_t(j0) * H) * BB + ll;
 #pragma unroll
             for (int q = 0; q < BS; ++q) {
