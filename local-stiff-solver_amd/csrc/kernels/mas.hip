@@ -385,8 +385,10 @@ __device__ __forceinline__ int vpad(int e) { return e + (e >> 5); }
 // ts, P, Q: LDS scratch (K * XS, K * BS * D, K * BS * D doubles).  After the
 // final barrier, y_k(j) = sum over s = j / 32 .. BS - 1 of Q[(k * BS + s) * D + j].
 template <int BS, class TL, int K>
+// P (phase-1 partials) and Q (phase-2 partials) may be one buffer: P is last read
+// before the barrier that precedes the first write of Q.
 __device__ __forceinline__ void bank_solve(const TL *__restrict__ G, const double *__restrict__ xs,
-                                           double *__restrict__ ts, double *__restrict__ P, double *__restrict__ Q)
+                                           double *__restrict__ ts, double *P, double *Q)
 {
     constexpr int D = 32 * BS;
     constexpr int XS = D + BS;
@@ -588,7 +590,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_coarse_apply(int32_t nodes, c
     using TL = typename LdsType<TS>::type;
     if (dev::all_stopped(stop, K)) return;
     __shared__ __attribute__((aligned(16))) TL A[S + 4];
-    __shared__ double xs[K * (D + BS)], ts[K * (D + BS)], P[K * BS * D], Q[K * BS * D];
+    __shared__ double xs[K * (D + BS)], ts[K * (D + BS)], PQ[K * BS * D];  // P, then Q (bank_solve)
     const int32_t bank = blockIdx.x;
     const int32_t lo = bank * 32;
     const int n = min(32, nodes - lo) * BS;
@@ -603,11 +605,11 @@ __global__ __launch_bounds__(kApplyThreads) void k_coarse_apply(int32_t nodes, c
     stage_matrix<TS, TL, S, kApplyThreads>(inv + int64_t(bank) * S, A);
     __syncthreads();
     // rows beyond the bank's nodes are identity padding with zero right-hand side
-    bank_solve<BS, TL, K>(A, xs, ts, P, Q);
+    bank_solve<BS, TL, K>(A, xs, ts, PQ, PQ);
     if (i < n) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            double y = bank_result<BS>(Q, k, i);
+            double y = bank_result<BS>(PQ, k, i);
             if (w_parent) y += wp[k];
             w[k * ld + int64_t(lo) * BS + i] = y;
         }
@@ -633,7 +635,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_level0_apply(int32_t nbanks, 
     using TL = typename LdsType<TS>::type;
     if (dev::all_stopped(stop, K)) return;
     __shared__ __attribute__((aligned(16))) TL A[S + 4];
-    __shared__ double xs[K * (D + BS)], ts[K * (D + BS)], P[K * BS * D], Q[K * BS * D];
+    __shared__ double xs[K * (D + BS)], ts[K * (D + BS)], PQ[K * BS * D];  // P, then Q (bank_solve)
     const int32_t bank = blockIdx.x;
     const int32_t lo = part_off[bank];
     const int n = (part_off[bank + 1] - lo) * BS;
@@ -653,14 +655,14 @@ __global__ __launch_bounds__(kApplyThreads) void k_level0_apply(int32_t nbanks, 
     load_bank_vector<BS, K>(r + int64_t(lo) * BS, ld, n, xs);
     stage_matrix<TS, TL, S, kApplyThreads>(inv + int64_t(bank) * S, A);
     __syncthreads();
-    bank_solve<BS, TL, K>(A, xs, ts, P, Q);
+    bank_solve<BS, TL, K>(A, xs, ts, PQ, PQ);
     double part[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) part[k] = 0.0;
     if (i < n) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            double y = bank_result<BS>(Q, k, i);
+            double y = bank_result<BS>(PQ, k, i);
             if (w1) y += cw[k];
             y *= scale;
             if (add) y += ca[k];
@@ -694,7 +696,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_level0_apply2(int32_t nbanks,
     using TL = typename LdsType<TS>::type;
     if (dev::all_stopped(stop, K)) return;
     __shared__ __attribute__((aligned(16))) TL A[S + 4];
-    __shared__ double xs[K * (D + BS)], ts[K * (D + BS)], P[K * BS * D], Q[K * BS * D];
+    __shared__ double xs[K * (D + BS)], ts[K * (D + BS)], PQ[K * BS * D];  // P, then Q (bank_solve)
     const int i = threadIdx.x;
     // two consecutive banks per workgroup; the matrix loads of the second are in
     // flight while the first is solved
@@ -721,14 +723,14 @@ __global__ __launch_bounds__(kApplyThreads) void k_level0_apply2(int32_t nbanks,
         mr.store(inv + int64_t(bank) * S, A);
         __syncthreads();
         if (bank + 1 < last) mr.issue(inv + int64_t(bank + 1) * S);
-        bank_solve<BS, TL, K>(A, xs, ts, P, Q);
+        bank_solve<BS, TL, K>(A, xs, ts, PQ, PQ);
         double part[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) part[k] = 0.0;
         if (i < n) {
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                double y = bank_result<BS>(Q, k, i);
+                double y = bank_result<BS>(PQ, k, i);
                 if (w1) y += cw[k];
                 y *= scale;
                 if (add) y += ca[k];
