@@ -333,24 +333,39 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     DBuf<int32_t> cnt(nb_), flag(1);
     flag.zero(s);
     k::bsr_count(n, bs_, ptr, idx, cnt.get(), flag.get(), s);
-    auto hc = to_host(cnt, nb_, s);
-    int32_t hflag = 0;
-    flag.download(&hflag, 1, s);
-    sync(s);
-    if (hflag) fail(LSS_ERR_ARG, "row indices must be strictly ascending within each column");
+    // block row pointer: scanned on the device when no int32 prefix can overflow (every
+    // block holds a stored entry or is a padded tail diagonal: nnzb <= nnz + nb)
+    const bool dev_scan = dev_topo && nnz + nb_ < (int64_t(1) << 31);
     Pattern g;
     g.n = nb_;
-    g.rowptr.assign(nb_ + 1, 0);
-    int64_t acc = 0;
-    for (int32_t I = 0; I < nb_; ++I) {
-        acc += hc[I];
-        if (acc >= (int64_t(1) << 31)) fail(LSS_ERR_UNSUPPORTED, "more than 2^31 nonzero blocks");
-        g.rowptr[I + 1] = int32_t(acc);
+    DBuf<int32_t> d_rp0(size_t(nb_) + 1), tsum(std::max(div_up(nb_, 1024), 1));
+    std::vector<int32_t> hc;
+    if (dev_scan) {
+        k::excl_scan_i32(nb_, cnt.get(), d_rp0.get(), tsum.get(), d_rp0.get() + nb_, s);
+        int32_t h[2] = {0, 0};
+        flag.download(&h[0], 1, s);
+        LSS_HIP(hipMemcpyAsync(&h[1], d_rp0.get() + nb_, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        sync(s);
+        if (h[0]) fail(LSS_ERR_ARG, "row indices must be strictly ascending within each column");
+        nnzb_ = h[1];
+    } else {
+        hc = to_host(cnt, nb_, s);
+        int32_t hflag = 0;
+        flag.download(&hflag, 1, s);
+        sync(s);
+        if (hflag) fail(LSS_ERR_ARG, "row indices must be strictly ascending within each column");
+        g.rowptr.assign(nb_ + 1, 0);
+        int64_t acc = 0;
+        for (int32_t I = 0; I < nb_; ++I) {
+            acc += hc[I];
+            if (acc >= (int64_t(1) << 31)) fail(LSS_ERR_UNSUPPORTED, "more than 2^31 nonzero blocks");
+            g.rowptr[I + 1] = int32_t(acc);
+        }
+        nnzb_ = acc;
+        d_rp0.upload(g.rowptr.data(), nb_ + 1, s);
     }
-    nnzb_ = acc;
-    DBuf<int32_t> d_rp0, d_c0(std::max<int64_t>(nnzb_, 1));
+    DBuf<int32_t> d_c0(std::max<int64_t>(nnzb_, 1));
     DBuf<double> d_v0(std::max<int64_t>(nnzb_ * bb, 1));
-    d_rp0.upload(g.rowptr.data(), nb_ + 1, s);
     d_v0.zero(s);
     k::bsr_fill(n, bs_, ptr, idx, vals, d_rp0.get(), d_c0.get(), d_v0.get(), s);
     if (!dev_topo) {  // the host partition and coarse spaces read the pattern
@@ -381,13 +396,22 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
         std::iota(perm_.begin(), perm_.end(), 0);
         part_off_.clear();
     }
-    // row lengths of the permuted pattern
-    std::vector<int32_t> prow(size_t(nb_) + 1, 0);
-    for (int32_t I = 0; I < nb_; ++I) prow[size_t(perm_[I]) + 1] = hc[I];
-    for (int32_t I = 0; I < nb_; ++I) prow[size_t(I) + 1] += prow[I];
+    // row pointer of the permuted pattern
     t_partition_ = since(tp);
     if (!dev_topo) d_perm_.upload(perm_.data(), nb_, s);
-    d_rowptr_.upload(prow.data(), nb_ + 1, s);
+    std::vector<int32_t> prow;
+    if (dev_scan) {
+        d_rowptr_.resize(size_t(nb_) + 1);
+        DBuf<int32_t> plen(nb_);
+        k::permute_lengths(nb_, d_perm_.get(), cnt.get(), plen.get(), s);
+        k::excl_scan_i32(nb_, plen.get(), d_rowptr_.get(), tsum.get(), d_rowptr_.get() + nb_, s);
+        sync(s);  // plen is released at scope end
+    } else {
+        prow.assign(size_t(nb_) + 1, 0);
+        for (int32_t I = 0; I < nb_; ++I) prow[size_t(perm_[I]) + 1] = hc[I];
+        for (int32_t I = 0; I < nb_; ++I) prow[size_t(I) + 1] += prow[I];
+        d_rowptr_.upload(prow.data(), nb_ + 1, s);
+    }
     d_cols_.resize(std::max<int64_t>(nnzb_, 1));
     d_bvals_.resize(std::max<int64_t>(nnzb_ * bb, 1));
     k::bsr_permute(nb_, bs_, d_rp0.get(), d_c0.get(), d_v0.get(), d_perm_.get(), d_rowptr_.get(), d_cols_.get(),
@@ -397,10 +421,11 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     const int H = 64 / bs_;
     const int32_t nsl = (nb_ + H - 1) / H;
     std::vector<int64_t> sp(nsl + 1, 0);
-    for (int32_t sl = 0; sl < nsl; ++sl) {
-        int32_t w = 0;
-        for (int32_t r = sl * H; r < std::min(nb_, (sl + 1) * H); ++r) w = std::max(w, prow[r + 1] - prow[r]);
-        sp[sl + 1] = sp[sl] + int64_t(w) * H;
+    {
+        DBuf<int32_t> wd(std::max(nsl, 1));
+        k::sell_width(nb_, bs_, d_rowptr_.get(), wd.get(), s);
+        const auto w = to_host(wd, size_t(nsl), s);
+        for (int32_t sl = 0; sl < nsl; ++sl) sp[sl + 1] = sp[sl] + int64_t(w[sl]) * H;
     }
     sell_slots_ = sp[nsl];
     d_slice_ptr_.upload(sp.data(), nsl + 1, s);
@@ -426,6 +451,7 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     Pattern gp;
     if (dev_topo) {
         gp.n = nb_;
+        if (prow.empty()) prow = to_host(d_rowptr_, size_t(nb_) + 1, s);
         gp.rowptr = prow;
         gp.cols.resize(nnzb_);
         d_cols_.download(gp.cols.data(), nnzb_, s);

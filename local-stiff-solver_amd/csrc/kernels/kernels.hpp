@@ -21,6 +21,8 @@ void bsr_fill(int32_t n, int bs, const int32_t *ptr, const int32_t *idx, const d
 // Relabel block rows/cols: new row perm[I] <- old row I, columns mapped and sorted.
 void bsr_permute(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols, const double *vals,
                  const int32_t *perm, const int32_t *new_rowptr, int32_t *new_cols, double *new_vals, hipStream_t s);
+// plen[perm[I]] = len[I]: row lengths in the permuted numbering.
+void permute_lengths(int32_t nb, const int32_t *perm, const int32_t *len, int32_t *plen, hipStream_t s);
 // Sliced block-ELL layout (one wave per slice of 64/bs block rows).
 void sell_width(int32_t nb, int bs, const int32_t *rowptr, int32_t *slice_width, hipStream_t s);
 void sell_fill(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols, const double *vals,

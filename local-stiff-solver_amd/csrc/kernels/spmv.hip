@@ -505,6 +505,22 @@ void bsr_permute(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols,
     LSS_LAUNCH_CHECK();
 }
 
+namespace {
+__global__ void k_permute_lengths(int32_t nb, const int32_t *__restrict__ perm, const int32_t *__restrict__ len,
+                                  int32_t *__restrict__ plen)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < nb) plen[perm[i]] = len[i];
+}
+}  // namespace
+
+void permute_lengths(int32_t nb, const int32_t *perm, const int32_t *len, int32_t *plen, hipStream_t s)
+{
+    if (nb == 0) return;
+    k_permute_lengths<<<div_up(nb, 256), 256, 0, s>>>(nb, perm, len, plen);
+    LSS_LAUNCH_CHECK();
+}
+
 void sell_width(int32_t nb, int bs, const int32_t *rowptr, int32_t *slice_width, hipStream_t s)
 {
     const int H = 64 / bs;
