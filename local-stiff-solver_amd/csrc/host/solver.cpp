@@ -546,19 +546,19 @@ void HybridSolver::build_levels_host(const Pattern &gp, hipStream_t s)
 
 void HybridSolver::build_mas(hipStream_t s)
 {
-    const int D = 32 * bs_;
-    const int64_t S = int64_t(D) * (D + 1) / 2;
     const size_t eb = k::precision_bytes(prec());
     d_status_.zero(s);
     mas_banks_ = 0;
     mas_bytes_ = 0;
     for (size_t l = 0; l < mas_.size(); ++l) {
         DevLevel &d = mas_[l];
+        const int64_t S = k::mas_bank_entries(prec(), bs_, l == 0);
         d.inv.resize(std::max<size_t>(size_t(d.nbanks) * S * eb, 16));
         const int32_t *rp = l == 0 ? d_rowptr_.get() : d.rowptr.get();
         const int32_t *cl = l == 0 ? d_cols_.get() : d.cols.get();
         const double *vl = l == 0 ? d_bvals_.get() : d.vals.get();
-        k::mas_bank_invert(prec(), bs_, d.nbanks, d.bank_off.get(), rp, cl, vl, d.inv.get(), d_status_.get(), s);
+        k::mas_bank_invert(prec(), bs_, d.nbanks, d.bank_off.get(), rp, cl, vl, d.inv.get(), d_status_.get(), s,
+                           l == 0);
         mas_banks_ += d.nbanks;
         mas_bytes_ += double(d.nbanks) * S * eb;
     }

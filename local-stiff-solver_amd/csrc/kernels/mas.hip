@@ -72,10 +72,69 @@ __device__ __forceinline__ float to_lds<_Float16>(_Float16 v) { return static_ca
 
 constexpr int kInvThreads = 256;
 
-__device__ __forceinline__ int packed_lower(int i, int j)
+__host__ __device__ constexpr int packed_lower(int i, int j)
 {
     // row-major packed lower triangle, j <= i
     return (i * (i + 1)) / 2 + j;
+}
+
+// Tiled layout of the FP32 level-0 bank factors, in HBM and in LDS.  The lower
+// triangle of a D x D factor (D = 32 BS) is cut into BS (BS + 1) / 2 tiles of
+// 32 x 32, tile (b, s), s <= b, holding rows 32b.. and columns 32s.., numbered
+// tile_index(b, s).
+//  * HBM: the tiles in that order; an off-diagonal tile as 32 rows of 32
+//    floats, row r of a diagonal tile as its r + 1 entries followed by zeros up
+//    to a multiple of 4 (576 floats per diagonal tile; 4 800 per bank for bs = 3
+//    against 4 656 packed).  Every 16-byte vector then lies in one tile row.
+//  * LDS: every tile as 32 rows of kTileLd floats (16-byte aligned rows, so a
+//    row segment is one 16-byte read), tiles 32 LDS banks apart; the entries
+//    above the diagonal of a diagonal tile are zeroed once per workgroup and
+//    never written again, so neither triangular product masks its loads.
+constexpr int kTileLd = 36;
+constexpr int kTileStride = 32 * kTileLd + 32;
+constexpr int kDiagTileFloats = 576;
+
+__host__ __device__ constexpr int tile_index(int b, int s) { return b * (b + 1) / 2 + s; }
+__host__ __device__ constexpr int diag_row_len(int r) { return 4 * ((r + 4) / 4); }
+__host__ __device__ constexpr int tiled_floats(int bs) { return bs * kDiagTileFloats + bs * (bs - 1) / 2 * 1024; }
+
+// src[q]: packed-lower index of tiled HBM entry q (-1: zero padding);
+// stage[v]: LDS byte offset of the 16-byte vector v of a tiled factor
+template <int BS>
+struct TiledLayout {
+    static constexpr int ST = tiled_floats(BS);
+    int16_t src[ST];
+    uint16_t stage[ST / 4];
+    constexpr TiledLayout() : src{}, stage{}
+    {
+        int q = 0;
+        for (int b = 0; b < BS; ++b)
+            for (int s = 0; s <= b; ++s)
+                for (int r = 0; r < 32; ++r) {
+                    const int len = s < b ? 32 : diag_row_len(r);
+                    for (int c = 0; c < len; ++c, ++q) {
+                        const int i = 32 * b + r, j = 32 * s + c;
+                        src[q] = j <= i ? int16_t(packed_lower(i, j)) : int16_t(-1);
+                        if (c % 4 == 0)
+                            stage[q / 4] = uint16_t(4 * (tile_index(b, s) * kTileStride + r * kTileLd + c));
+                    }
+                }
+    }
+};
+static_assert(diag_row_len(0) == 4 && diag_row_len(3) == 4 && diag_row_len(4) == 8 && diag_row_len(31) == 32,
+              "rows padded to whole 16-byte vectors");
+static_assert(tiled_floats(3) == 4800 && 4 * (tile_index(2, 2) * kTileStride + 31 * kTileLd + 31) < 65536,
+              "tiled sizes and 16-bit LDS offsets");
+__constant__ TiledLayout<1> kTiled1 = TiledLayout<1>();
+__constant__ TiledLayout<2> kTiled2 = TiledLayout<2>();
+__constant__ TiledLayout<3> kTiled3 = TiledLayout<3>();
+
+template <int BS>
+__device__ __forceinline__ const TiledLayout<BS> &tiled_layout()
+{
+    if constexpr (BS == 1) return kTiled1;
+    else if constexpr (BS == 2) return kTiled2;
+    else return kTiled3;
 }
 
 // Assemble the bank matrices of one level from its block operator, factor
@@ -87,7 +146,7 @@ __device__ __forceinline__ int packed_lower(int i, int j)
 // FMAs, with two barriers per step.  The packed lower triangle in LDS holds
 // the assembled matrix, then L (read by the elimination), then G for the
 // coalesced store.
-template <int BS, class TS>
+template <int BS, class TS, bool TILED>
 __global__ __launch_bounds__(kInvThreads) void k_bank_factor(int32_t nbanks, const int32_t *__restrict__ bank_off,
                                                              const int32_t *__restrict__ rowptr,
                                                              const int32_t *__restrict__ cols,
@@ -238,12 +297,24 @@ __global__ __launch_bounds__(kInvThreads) void k_bank_factor(int32_t nbanks, con
             if (j <= i) P[packed_lower(i, j)] = x[u][v];
         }
     __syncthreads();
-    TS *dst = out + int64_t(bank) * S;
     bool ok = true;
-    for (int q = t; q < S; q += kInvThreads) {
-        const double v = P[q];
-        if (!isfinite(v)) ok = false;
-        dst[q] = to_store<TS>(v);
+    if constexpr (TILED) {
+        constexpr int ST = tiled_floats(BS);
+        const int16_t *src = tiled_layout<BS>().src;
+        TS *dst = out + int64_t(bank) * ST;
+        for (int q = t; q < ST; q += kInvThreads) {
+            const int p = src[q];
+            const double v = p >= 0 ? P[p] : 0.0;
+            if (!isfinite(v)) ok = false;
+            dst[q] = to_store<TS>(v);
+        }
+    } else {
+        TS *dst = out + int64_t(bank) * S;
+        for (int q = t; q < S; q += kInvThreads) {
+            const double v = P[q];
+            if (!isfinite(v)) ok = false;
+            dst[q] = to_store<TS>(v);
+        }
     }
     if (!ok || (t == 0 && bad)) atomicOr(status, 1);
 }
@@ -750,6 +821,193 @@ __global__ __launch_bounds__(kApplyThreads) void k_level0_apply2(int32_t nbanks,
     }
 }
 
+// bank_solve on the tiled LDS layout: the same tasks and the same products in
+// the same order (identical results), with the diagonal tiles' zero upper
+// halves in place of the masks and 16-byte row reads in phase 1.
+template <int BS, int K>
+__device__ __forceinline__ void tile_solve(const float *__restrict__ Gt, const double *__restrict__ xs,
+                                           double *__restrict__ ts, double *P, double *Q)
+{
+    constexpr int D = 32 * BS;
+    constexpr int XS = D + BS;
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, h = lane >> 5, l = lane & 31;
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    if (w < BS) {  // phase 1: P(s, row) = G(row, 32s..32s+31) x(32s..32s+31)
+        const ApplyTask tB0 = apply_task<BS>(1, w, 1);
+        const bool hasB = tB0.b >= 0;
+        const ApplyTask t = (h && hasB) ? tB0 : apply_task<BS>(1, w, 0);
+        const int row = t.b * 32 + l;
+        const float *G = Gt + tile_index(t.b, t.s) * kTileStride + l * kTileLd;
+        const int xbase = vpad(t.s * 32);
+        double a0[K], a1[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) a0[k] = a1[k] = 0.0;
+#pragma unroll
+        for (int u0 = 0; u0 < 32; u0 += 8) {
+            const F4 ga = *reinterpret_cast<const F4 *>(G + u0);
+            const F4 gb = *reinterpret_cast<const F4 *>(G + u0 + 4);
+            const double g[8] = {double(ga.x), double(ga.y), double(ga.z), double(ga.w),
+                                 double(gb.x), double(gb.y), double(gb.z), double(gb.w)};
+            double xv[K][8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v)
+#pragma unroll
+                for (int k = 0; k < K; ++k) xv[k][v] = xs[k * XS + xbase + u0 + v];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                a0[k] += g[0] * xv[k][0] + g[2] * xv[k][2] + g[4] * xv[k][4] + g[6] * xv[k][6];
+                a1[k] += g[1] * xv[k][1] + g[3] * xv[k][3] + g[5] * xv[k][5] + g[7] * xv[k][7];
+            }
+        }
+        if (h == 0 || hasB)
+#pragma unroll
+            for (int k = 0; k < K; ++k) P[(k * BS + t.s) * D + row] = a0[k] + a1[k];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < K * D; e += blockDim.x) {
+        const int k = e / D, i = e - k * D;
+        double v = P[(k * BS) * D + i];
+        for (int q = 1; q <= i / 32; ++q) v += P[(k * BS + q) * D + i];
+        ts[k * XS + vpad(i)] = v;
+    }
+    __syncthreads();
+    if (w < BS) {  // phase 2: Q(s, col) = G(32s..32s+31, col)^T t(32s..32s+31)
+        const ApplyTask tB0 = apply_task<BS>(2, w, 1);
+        const bool hasB = tB0.b >= 0;
+        const ApplyTask t = (h && hasB) ? tB0 : apply_task<BS>(2, w, 0);
+        const int col = t.b * 32 + l;
+        const float *G = Gt + tile_index(t.s, t.b) * kTileStride + l;
+        const int tbase = vpad(t.s * 32);
+        double a0[K], a1[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) a0[k] = a1[k] = 0.0;
+#pragma unroll
+        for (int u0 = 0; u0 < 32; u0 += 8) {
+            double g[8], tv[K][8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                g[v] = double(G[(u0 + v) * kTileLd]);
+#pragma unroll
+                for (int k = 0; k < K; ++k) tv[k][v] = ts[k * XS + tbase + u0 + v];
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                a0[k] += g[0] * tv[k][0] + g[2] * tv[k][2] + g[4] * tv[k][4] + g[6] * tv[k][6];
+                a1[k] += g[1] * tv[k][1] + g[3] * tv[k][3] + g[5] * tv[k][5] + g[7] * tv[k][7];
+            }
+        }
+        if (h == 0 || hasB)
+#pragma unroll
+            for (int k = 0; k < K; ++k) Q[(k * BS + t.s) * D + col] = a0[k] + a1[k];
+    }
+    __syncthreads();
+}
+
+// Waves per SIMD asked of the compiler, as many workgroups as the LDS of a CU
+// holds for bs = 3: 5 of 32 KB for one column (the variant without the dot
+// product fits the 96 registers that takes; with it, or for bs = 2, the kernel
+// would spill and keeps the compiler's own choice), 4 of 36 KB for two columns,
+// 3 of 44 KB for four (again not with the dot product, which would spill).
+// banks per workgroup of the tiled level-0 apply (the grid is nbanks / this)
+__host__ __device__ constexpr int tiled_banks_per_group(int k) { return k <= 2 ? 2 : 1; }
+
+template <int BS, bool DOT, int K>
+constexpr int kTiledWaves = BS != 3 ? 1 : (K == 1 ? (DOT ? 1 : 5) : (K == 2 ? 4 : (DOT ? 1 : 3)));
+
+// Level 0 with FP32 factors in the tiled layout: one workgroup solves two
+// consecutive banks for 1 or 2 right-hand sides (the second bank's factor loads
+// in flight while the first solves), one bank for 4 columns.  Each 16-byte
+// vector of a factor goes to LDS with one 16-byte store at its offset from the
+// constant layout table.  Epilogue as in k_level0_apply.
+template <int BS, bool DOT, int K>
+__global__ __launch_bounds__(kApplyThreads, (kTiledWaves<BS, DOT, K>)) void k_level0_apply_tiled(int32_t nbanks,
+                                                                     const int32_t *__restrict__ part_off,
+                                                                     const float *__restrict__ inv,
+                                                                     const double *__restrict__ r, int64_t ld,
+                                                                     const int32_t *__restrict__ parent,
+                                                                     const double *__restrict__ w1, int64_t ld1,
+                                                                     double scale, const double *__restrict__ add,
+                                                                     double *__restrict__ z,
+                                                                     const double *__restrict__ dotv,
+                                                                     double *partials, const int *__restrict__ stop)
+{
+    constexpr int D = 32 * BS;
+    constexpr int ST = tiled_floats(BS);
+    constexpr int NTILE = BS * (BS + 1) / 2;
+    constexpr int NB = tiled_banks_per_group(K);
+    using Regs = MatrixRegs<float, ST, kApplyThreads>;
+    constexpr int NV = Regs::NV, PER = Regs::PER;
+    static_assert(ST % 4 == 0, "whole 16-byte vectors");
+    if (dev::all_stopped(stop, K)) return;
+    __shared__ __attribute__((aligned(16))) float Gt[NTILE * kTileStride];
+    __shared__ double xs[K * (D + BS)], ts[K * (D + BS)], PQ[K * BS * D];  // P, then Q (tile_solve)
+    const int i = threadIdx.x;
+    const int32_t first = int32_t(blockIdx.x) * NB;
+    if (first >= nbanks) return;
+    const int32_t last = min(nbanks, first + NB);
+    Regs mr;
+    mr.issue(inv + int64_t(first) * ST);
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    for (int e = i; e < NTILE * kTileStride / 4; e += kApplyThreads)
+        reinterpret_cast<F4 *>(Gt)[e] = F4{0.0f, 0.0f, 0.0f, 0.0f};
+    __syncthreads();  // the zeros land before any factor entry
+    const uint16_t *stage = tiled_layout<BS>().stage;
+    char *Gb = reinterpret_cast<char *>(Gt);
+    for (int32_t bank = first; bank < last; ++bank) {
+        const int32_t lo = part_off[bank];
+        const int n = (part_off[bank + 1] - lo) * BS;
+        const int64_t gi = int64_t(lo) * BS + i;
+        int off[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int t = i + q * kApplyThreads;
+            off[q] = t < NV ? stage[t] : 0;
+        }
+        // epilogue operands first: their latency overlaps the staging of the bank
+        double cw[K], ca[K], cd[K];
+        if (i < n) {
+            const int64_t pi = w1 ? int64_t(parent[lo + i / BS]) * BS + (i % BS) : 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                cw[k] = w1 ? w1[k * ld1 + pi] : 0.0;
+                ca[k] = add ? add[k * ld + gi] : 0.0;
+                if constexpr (DOT) cd[k] = dotv[k * ld + gi];
+            }
+        }
+        load_bank_vector<BS, K>(r + int64_t(lo) * BS, ld, n, xs);
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            if (i + q * kApplyThreads < NV) *reinterpret_cast<typename Regs::V *>(Gb + off[q]) = mr.reg[q];
+        __syncthreads();
+        if (bank + 1 < last) mr.issue(inv + int64_t(bank + 1) * ST);
+        tile_solve<BS, K>(Gt, xs, ts, PQ, PQ);
+        double part[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) part[k] = 0.0;
+        if (i < n) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                double y = bank_result<BS>(PQ, k, i);
+                if (w1) y += cw[k];
+                y *= scale;
+                if (add) y += ca[k];
+                z[k * ld + gi] = y;
+                if constexpr (DOT) part[k] = cd[k] * y;
+            }
+        }
+        if constexpr (DOT) {
+            __shared__ double sh[kApplyThreads / 64];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const double tot = dev::block_sum<kApplyThreads>(part[k], sh);
+                if (threadIdx.x == 0) partials[int64_t(k) * nbanks + bank] = tot;
+            }
+        }
+        __syncthreads();  // Gt (tile rows), xs and Q are rewritten for the next bank
+    }
+}
+
 // z += scale * w1[parent] (the level-1 correction prolongated to level 0) with an
 // optional per-workgroup partial of dotv . z; grid row y = column y.
 template <bool DOT>
@@ -776,7 +1034,7 @@ __global__ __launch_bounds__(256) void k_prolong_add(int64_t np, int bs, const i
     }
 }
 
-template <class TS>
+template <class TS, bool TILED = false>
 void launch_bank_invert(int bs, int32_t nbanks, const int32_t *bank_off, const int32_t *rowptr, const int32_t *cols,
                         const double *vals, TS *out, int *status, hipStream_t s)
 {
@@ -785,13 +1043,16 @@ void launch_bank_invert(int bs, int32_t nbanks, const int32_t *bank_off, const i
     const size_t lds = size_t(D * (D + 1) / 2 + D + 2) * sizeof(double);
     switch (bs) {
         case 1:
-            k_bank_factor<1, TS><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out, status);
+            k_bank_factor<1, TS, TILED><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out,
+                                                                         status);
             break;
         case 2:
-            k_bank_factor<2, TS><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out, status);
+            k_bank_factor<2, TS, TILED><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out,
+                                                                         status);
             break;
         case 3:
-            k_bank_factor<3, TS><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out, status);
+            k_bank_factor<3, TS, TILED><<<nbanks, kInvThreads, lds, s>>>(nbanks, bank_off, rowptr, cols, vals, out,
+                                                                         status);
             break;
         default: fail(LSS_ERR_UNSUPPORTED, "block_size must be 1, 2 or 3");
     }
@@ -805,19 +1066,28 @@ void mas_set_lds_limits()
     static bool done = false;
     if (done) return;
     const size_t lds3 = size_t(96 * 97 / 2 + 96 + 2) * sizeof(double);
-    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, double>),
+    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, double, false>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
-    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, float>),
+    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, float, false>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
-    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, __hip_bfloat16>),
+    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, float, true>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
-    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, _Float16>),
+    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, __hip_bfloat16, false>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
+    LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bank_factor<3, _Float16, false>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds3)));
     done = true;
 }
 
+int64_t mas_bank_entries(Precision prec, int bs, bool level0)
+{
+    if (level0 && prec == Precision::FP32) return tiled_floats(bs);
+    const int64_t D = 32 * bs;
+    return D * (D + 1) / 2;
+}
+
 void mas_bank_invert(Precision prec, int bs, int32_t nbanks, const int32_t *bank_off, const int32_t *rowptr,
-                     const int32_t *cols, const double *vals, void *out, int *status, hipStream_t s)
+                     const int32_t *cols, const double *vals, void *out, int *status, hipStream_t s, bool level0)
 {
     mas_set_lds_limits();
     switch (prec) {
@@ -825,7 +1095,12 @@ void mas_bank_invert(Precision prec, int bs, int32_t nbanks, const int32_t *bank
             launch_bank_invert<double>(bs, nbanks, bank_off, rowptr, cols, vals, static_cast<double *>(out), status, s);
             break;
         case Precision::FP32:
-            launch_bank_invert<float>(bs, nbanks, bank_off, rowptr, cols, vals, static_cast<float *>(out), status, s);
+            if (level0)
+                launch_bank_invert<float, true>(bs, nbanks, bank_off, rowptr, cols, vals, static_cast<float *>(out),
+                                                status, s);
+            else
+                launch_bank_invert<float>(bs, nbanks, bank_off, rowptr, cols, vals, static_cast<float *>(out), status,
+                                          s);
             break;
         case Precision::BF16:
             launch_bank_invert<__hip_bfloat16>(bs, nbanks, bank_off, rowptr, cols, vals,
@@ -910,7 +1185,11 @@ void level0_apply_k(Precision prec, int bs, int32_t nbanks, const int32_t *part_
                     hipStream_t s)
 {
 #define LSS_L0(BS, TS, DOT)                                                                                    \
-    if constexpr (K == 1)                                                                                      \
+    if constexpr (std::is_same_v<TS, float>)                                                                   \
+        k_level0_apply_tiled<BS, DOT, K><<<div_up(nbanks, tiled_banks_per_group(K)), kApplyThreads, 0, s>>>(   \
+            nbanks, part_off, static_cast<const float *>(inv), r, ld, parent, w1, ld1, scale, add, z, dotv,     \
+            partials, stop);                                                                                   \
+    else if constexpr (K == 1)                                                                                 \
         k_level0_apply2<BS, TS, DOT, K><<<div_up(nbanks, 2), kApplyThreads, 0, s>>>(                           \
             nbanks, part_off, static_cast<const TS *>(inv), r, ld, parent, w1, ld1, scale, add, z, dotv,        \
             partials, stop);                                                                                   \

@@ -15,9 +15,15 @@ enum class Precision : int { FP64 = 0, FP32 = 1, BF16 = 2, FP16 = 3 };
 
 inline size_t precision_bytes(Precision p) { return p == Precision::FP64 ? 8 : (p == Precision::FP32 ? 4 : 2); }
 
-// Assemble + invert the banks of one level.  bank_off: nbanks + 1 node offsets.
+// Entries stored per bank of a level: the FP32 level-0 factors use a tiled layout
+// (whole tiles of the lower triangle, diagonal-tile rows padded to 16 bytes) that
+// the level-0 apply stages into LDS with 16-byte stores; every other level and
+// precision stores the packed lower triangle, D (D + 1) / 2 entries, D = 32 bs.
+int64_t mas_bank_entries(Precision prec, int bs, bool level0);
+// Assemble + invert the banks of one level into mas_bank_entries(prec, bs, level0)
+// entries per bank.  bank_off: nbanks + 1 node offsets.
 void mas_bank_invert(Precision prec, int bs, int32_t nbanks, const int32_t *bank_off, const int32_t *rowptr,
-                     const int32_t *cols, const double *vals, void *out, int *status, hipStream_t s);
+                     const int32_t *cols, const double *vals, void *out, int *status, hipStream_t s, bool level0);
 // Galerkin values: coarse slot t = sum of fine blocks src_blk[src_ptr[t] .. src_ptr[t+1]).
 void mas_galerkin(int64_t nslots, int bs, const int64_t *src_ptr, const int32_t *src_blk, const double *fine_vals,
                   double *coarse_vals, hipStream_t s);
