@@ -910,7 +910,7 @@ __device__ __forceinline__ void tile_solve(const float *__restrict__ Gt, const d
 // would spill and keeps the compiler's own choice), 4 of 36 KB for two columns,
 // 3 of 44 KB for four (again not with the dot product, which would spill).
 // banks per workgroup of the tiled level-0 apply (the grid is nbanks / this)
-__host__ __device__ constexpr int tiled_banks_per_group(int k) { return k <= 2 ? 2 : 1; }
+__host__ __device__ constexpr int tiled_banks_per_group(int k) { return k == 1 ? 4 : (k == 2 ? 2 : 1); }
 
 template <int BS, bool DOT, int K>
 constexpr int kTiledWaves = BS != 3 ? 1 : (K == 1 ? (DOT ? 1 : 5) : (K == 2 ? 4 : (DOT ? 1 : 3)));

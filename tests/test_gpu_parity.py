@@ -93,6 +93,29 @@ def test_mas_apply_matches(gpu, name, N):
     assert info["mas"]["levels"] == mas.levels
 
 
+# FP32 level-0 factors are stored and staged in a tiled layout of their own (whole
+# 32 x 32 tiles, diagonal-tile rows padded to 16 bytes).  The apply must agree with
+# the oracle's FP64 MAS to within the rounding of the factors to FP32; an entry in
+# the wrong place would be an O(1) error.  Systems with well-conditioned banks, for
+# block sizes 1, 2 and 3 (one kernel instantiation each).
+@pytest.mark.parametrize("name,N,bs", [("laplace_sphere", 12, 1), ("laplace_sphere", 12, 2),
+                                       ("laplace_sphere", 12, 3), ("elastic_logE", 8, 3)])
+def test_mas_apply_fp32_banks(gpu, name, N, bs):
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    assert A.n % bs == 0
+    s = Solver.create(_cfg(bs, do_hybrid_precond=False, precision="fp32"))
+    s.factorize(A)
+    perm, off = s.partition(A.n // bs)
+    mas = O.Mas(A.colptr, A.rowidx, A.vals, bs, perm, off)
+    for seed in (3, 4):
+        r = problems.random_vector(A.n, seed)
+        zr = mas.apply(r)
+        z = s.base_apply(r)
+        assert np.linalg.norm(z - zr) <= 1e-4 * np.linalg.norm(zr), seed
+        assert np.linalg.norm(z - zr) > 0.0  # FP32 factors: not the FP64 path
+
+
 @pytest.mark.parametrize("name,N", CASES)
 def test_solve_matches_oracle(gpu, name, N):
     prob = problems.make(name, N=N)

@@ -56,6 +56,8 @@ def _singles(s, B, X0):
         ("elastic_contact", 10, 4, {}),
         ("elastic_contact", 10, 5, {}),
         ("elastic_contact", 10, 3, {"precision": "fp32"}),
+        ("elastic_contact", 10, 2, {"precision": "fp32"}),
+        ("laplace_sphere", 12, 4, {"precision": "fp32"}),
         ("laplace_sphere", 12, 2, {}),
         ("elastic_logE", 8, 4, {"base_preconditioner": "Jacobi"}),
         ("elastic_contact", 8, 2, {"do_hybrid_precond": False}),
