@@ -1,12 +1,16 @@
 #include "solver.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <future>
 #include <numeric>
 #include <sstream>
+#include <thread>
 
 #include "../kernels/util_kernels.hpp"
 #include "topology.hpp"
@@ -52,6 +56,9 @@ HybridSolver::HybridSolver(const Config &cfg) : cfg_(cfg)
     LSS_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
     LSS_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
     LSS_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
+    // third stream (default priority) for the stiff selection and factorization
+    LSS_HIP(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
+    LSS_HIP(hipEventCreateWithFlags(&ev_aux_, hipEventDisableTiming));
     void *p = nullptr;
     LSS_HIP(hipHostMalloc(&p, sizeof(dev::PcgState) * kMaxCols, hipHostMallocDefault));
     h_st_ = static_cast<dev::PcgState *>(p);
@@ -72,6 +79,8 @@ HybridSolver::~HybridSolver()
     if (ev_fork_) (void)hipEventDestroy(ev_fork_);
     if (ev_join_) (void)hipEventDestroy(ev_join_);
     if (side_) (void)hipStreamDestroy(side_);
+    if (ev_aux_) (void)hipEventDestroy(ev_aux_);
+    if (aux_) (void)hipStreamDestroy(aux_);
 }
 
 const char *HybridSolver::name() const
@@ -156,41 +165,105 @@ void HybridSolver::factorize_device(int32_t n, int64_t nnz, const int32_t *ptr, 
     gmm_ = GmmResult{};
     filter_ = FilterStats{};
     n_selected_ = n_filtered_ = n_expanded_ = 0;
+    t_select_ = t_stiff_ = 0.0;
 
-    auto t = clk::now();
-    if (!cfg_.mas_mode() && cfg_.do_hybrid_precond) select_stiff(n, ptr, idx, vals, s);
-    t_select_ = since(t);
-
-    t = clk::now();
-    build_operator(n, nnz, ptr, idx, vals, s);
-    sync(s);
-    t_operator_ = since(t) - t_partition_;
-
-    t = clk::now();
-    if (cfg_.base_preconditioner == "MAS") {
-        build_mas(s);
-    } else {
-        d_dinv_.resize(size_t(nb_) * bs_);
-        k::diag_inverse(nb_, bs_, d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), d_dinv_.get(), s);
+    // The hybrid preconditioner's stiff selection (row norms, GMM, components: host
+    // round trips and host graph work) needs neither the operator nor the MAS, and
+    // its stiff factorization is not needed by the omega estimate.  Both run on a
+    // worker thread and stream: the selection while this thread builds the operator,
+    // partition and MAS, the factorization (after the MAS, which shares d_status_)
+    // while it estimates omega.
+    const bool overlap = !cfg_.mas_mode() && cfg_.do_hybrid_precond;
+    std::thread worker;
+    std::exception_ptr select_err, stiff_err;
+    std::promise<void> selected_p, mas_done_p;
+    std::future<void> selected = selected_p.get_future();
+    std::shared_future<void> mas_done = mas_done_p.get_future().share();
+    std::atomic<bool> abandon{false};
+    bool mas_signalled = false;
+    auto join = [&] {
+        if (!worker.joinable()) return;
+        if (!mas_signalled) {
+            abandon = true;
+            mas_done_p.set_value();
+            mas_signalled = true;
+        }
+        worker.join();
+    };
+    if (overlap) {
+        int dev = 0;
+        LSS_HIP(hipGetDevice(&dev));
+        LSS_HIP(hipEventRecord(ev_aux_, s));  // the input arrays are ready on s
+        LSS_HIP(hipStreamWaitEvent(aux_, ev_aux_, 0));
+        worker = std::thread([&, dev] {
+            try {
+                LSS_HIP(hipSetDevice(dev));
+                const auto tw = clk::now();
+                select_stiff(n, ptr, idx, vals, aux_);
+                t_select_ = since(tw);
+            } catch (...) {
+                select_err = std::current_exception();
+            }
+            selected_p.set_value();
+            if (select_err) return;
+            mas_done.wait();
+            if (abandon) return;
+            try {
+                const auto tw = clk::now();
+                if (!comps_.empty()) build_stiff(n, ptr, idx, vals, aux_);
+                sync(aux_);
+                t_stiff_ = since(tw);
+            } catch (...) {
+                stiff_err = std::current_exception();
+            }
+        });
     }
-    sync(s);
-    t_mas_ = since(t);
+    try {
+        auto t = clk::now();
+        build_operator(n, nnz, ptr, idx, vals, s);
+        sync(s);
+        t_operator_ = since(t) - t_partition_;
 
-    if (cfg_.mas_mode() && cfg_.use_bad_dof_precond) {
-        // the MAS solver selects on the partition-permuted operator (MASSolver.cu:398-432)
         t = clk::now();
-        select_mas_bad(n, ptr, idx, vals, s);
-        t_select_ += since(t);
-    }
-    t = clk::now();
-    if (!comps_.empty()) build_stiff(n, ptr, idx, vals, s);
-    sync(s);
-    t_stiff_ = since(t);
+        if (cfg_.base_preconditioner == "MAS") {
+            build_mas(s);
+        } else {
+            d_dinv_.resize(size_t(nb_) * bs_);
+            k::diag_inverse(nb_, bs_, d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), d_dinv_.get(), s);
+        }
+        sync(s);
+        t_mas_ = since(t);
 
-    alloc_vectors();
-    t = clk::now();
-    estimate_omega(s);
-    t_omega_ = since(t);
+        if (overlap) {
+            mas_done_p.set_value();
+            mas_signalled = true;
+        } else {
+            if (cfg_.mas_mode() && cfg_.use_bad_dof_precond) {
+                // the MAS solver selects on the partition-permuted operator (MASSolver.cu:398-432)
+                t = clk::now();
+                select_mas_bad(n, ptr, idx, vals, s);
+                t_select_ += since(t);
+            }
+            t = clk::now();
+            if (!comps_.empty()) build_stiff(n, ptr, idx, vals, s);
+            sync(s);
+            t_stiff_ = since(t);
+        }
+
+        alloc_vectors();
+        if (overlap) {
+            selected.wait();  // estimate_omega needs to know whether the stiff set is empty
+            if (select_err) std::rethrow_exception(select_err);
+        }
+        t = clk::now();
+        estimate_omega(s);
+        t_omega_ = since(t);
+    } catch (...) {
+        join();
+        throw;
+    }
+    join();
+    if (stiff_err) std::rethrow_exception(stiff_err);
     factorized_ = true;
     t_factorize_ = since(t0);
 }
@@ -697,7 +770,9 @@ double tridiag_max_eig(const std::vector<double> &d, const std::vector<double> &
 void HybridSolver::estimate_omega(hipStream_t s)
 {
     omega_ = 1.0;
-    if (cfg_.mas_mode() || !cfg_.do_hybrid_precond || ncomp_ == 0) return;
+    // comps_ (not ncomp_, which the stiff factorization may be writing) tells whether
+    // there is a stiff set
+    if (cfg_.mas_mode() || !cfg_.do_hybrid_precond || comps_.empty()) return;
     // The additive MAS can have lambda_max(B A) > 2, which makes the multiplicative
     // sweep (I - BA)(I - PA)(I - BA) non-contractive and the composed preconditioner
     // indefinite.  omega = damping / theta, theta the largest Ritz value of B A after

@@ -107,6 +107,9 @@ private:
     hipStream_t stream_ = nullptr;
     hipStream_t side_ = nullptr;  // MAS coarse levels, concurrent with the level-0 banks
     hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+    // stiff selection and factorization on a worker thread during factorize
+    hipStream_t aux_ = nullptr;
+    hipEvent_t ev_aux_ = nullptr;
     bool factorized_ = false;
     int32_t n_ = 0, nb_ = 0;
     int bs_ = 1;
