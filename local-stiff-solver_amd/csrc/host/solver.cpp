@@ -107,6 +107,19 @@ void HybridSolver::set_problematic_dofs(const int32_t *dofs, int64_t n)
     user_set_given_ = true;
 }
 
+void HybridSolver::reset_factorization()
+{
+    // Device buffers of the operator, stiff blocks and vectors keep their allocations:
+    // DBuf::resize keeps one whose size is unchanged, so a factorization of a matrix of
+    // the same shape allocates nothing for them.  Each is rewritten before it is read.
+    factorized_ = false;
+    mas_.clear();
+    comps_.clear();
+    ncomp_ = 0;
+    ns_ = 0;
+    omega_ = 1.0;
+}
+
 void HybridSolver::release()
 {
     factorized_ = false;
@@ -153,7 +166,7 @@ void HybridSolver::factorize_device(int32_t n, int64_t nnz, const int32_t *ptr, 
     if (!s) s = stream_;
     if (n <= 0) fail(LSS_ERR_SIZE, "matrix must have at least one row");
     auto t0 = clk::now();
-    release();
+    reset_factorization();
     n_ = n;
     bs_ = cfg_.block_size;
     nb_ = (n + bs_ - 1) / bs_;

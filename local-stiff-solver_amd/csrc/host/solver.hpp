@@ -72,7 +72,8 @@ private:
         DBuf<double> r, w;         // restricted residual / prolongated correction
     };
 
-    void release();
+    void release();              // frees every device buffer
+    void reset_factorization();  // clears the factorization; buffers stay allocated for reuse
     void select_stiff(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s);
     // MAS solver: stiff set of the elbow / k-means / GMM strategy on the permuted operator
     void select_mas_bad(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s);
