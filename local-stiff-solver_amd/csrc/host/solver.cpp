@@ -112,7 +112,9 @@ void HybridSolver::reset_factorization()
     // Device buffers of the operator, stiff blocks and vectors keep their allocations:
     // DBuf::resize keeps one whose size is unchanged, so a factorization of a matrix of
     // the same shape allocates nothing for them.  Each is rewritten before it is read.
+    // The level-0 bank factors (most of the MAS bytes) are kept aside the same way.
     factorized_ = false;
+    if (!mas_.empty()) spare_inv0_ = std::move(mas_[0].inv);
     mas_.clear();
     comps_.clear();
     ncomp_ = 0;
@@ -124,6 +126,7 @@ void HybridSolver::release()
 {
     factorized_ = false;
     mas_.clear();
+    spare_inv0_.release();
     comps_.clear();
     d_perm_.release();
     d_rowptr_.release();
@@ -450,13 +453,19 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
         nnzb_ = acc;
         d_rp0.upload(g.rowptr.data(), nb_ + 1, s);
     }
-    DBuf<int32_t> d_c0(std::max<int64_t>(nnzb_, 1));
-    DBuf<double> d_v0(std::max<int64_t>(nnzb_ * bb, 1));
-    d_v0.zero(s);
-    k::bsr_fill(n, bs_, ptr, idx, vals, d_rp0.get(), d_c0.get(), d_v0.get(), s);
+    // The unpermuted BSR (columns c0, values v0) is scratch until bsr_permute.  It lives
+    // in the sliced arrays, which are written only after it is consumed and hold at
+    // least nnzb slots (slices add padding): a refactorization of the same shape
+    // allocates nothing for it.
+    if (d_scols_.size() < size_t(std::max<int64_t>(nnzb_, 1))) d_scols_.resize(std::max<int64_t>(nnzb_, 1));
+    if (d_svals_.size() < size_t(std::max<int64_t>(nnzb_ * bb, 1))) d_svals_.resize(std::max<int64_t>(nnzb_ * bb, 1));
+    int32_t *c0 = d_scols_.get();
+    double *v0 = d_svals_.get();
+    LSS_HIP(hipMemsetAsync(v0, 0, size_t(nnzb_) * bb * sizeof(double), s));
+    k::bsr_fill(n, bs_, ptr, idx, vals, d_rp0.get(), c0, v0, s);
     if (!dev_topo) {  // the host partition and coarse spaces read the pattern
         g.cols.resize(nnzb_);
-        d_c0.download(g.cols.data(), nnzb_, s);
+        LSS_HIP(hipMemcpyAsync(g.cols.data(), c0, size_t(nnzb_) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     }
     sync(s);
 
@@ -465,7 +474,7 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     DBuf<int32_t> d_off;
     if (dev_topo) {
         d_perm_.resize(nb_);
-        const int32_t nparts = k::device_partition(nb_, d_rp0.get(), d_c0.get(), 32, d_perm_.get(), d_off, s,
+        const int32_t nparts = k::device_partition(nb_, d_rp0.get(), c0, 32, d_perm_.get(), d_off, s,
                                                     cfg_.partition_local_max);
         perm_.resize(nb_);
         part_off_.resize(size_t(nparts) + 1);
@@ -500,7 +509,7 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     }
     d_cols_.resize(std::max<int64_t>(nnzb_, 1));
     d_bvals_.resize(std::max<int64_t>(nnzb_ * bb, 1));
-    k::bsr_permute(nb_, bs_, d_rp0.get(), d_c0.get(), d_v0.get(), d_perm_.get(), d_rowptr_.get(), d_cols_.get(),
+    k::bsr_permute(nb_, bs_, d_rp0.get(), c0, v0, d_perm_.get(), d_rowptr_.get(), d_cols_.get(),
                    d_bvals_.get(), s);
 
     // sliced block-ELL copy for SpMV
@@ -515,6 +524,7 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     }
     sell_slots_ = sp[nsl];
     d_slice_ptr_.upload(sp.data(), nsl + 1, s);
+    sync(s);  // bsr_permute has read c0 and v0 before the sliced arrays are resized
     // sell_fill writes every slot (padding: column -1, zero values)
     d_scols_.resize(std::max<int64_t>(sell_slots_, 1));
     d_svals_.resize(std::max<int64_t>(sell_slots_ * bb, 1));
@@ -639,7 +649,12 @@ void HybridSolver::build_mas(hipStream_t s)
     for (size_t l = 0; l < mas_.size(); ++l) {
         DevLevel &d = mas_[l];
         const int64_t S = k::mas_bank_entries(prec(), bs_, l == 0);
-        d.inv.resize(std::max<size_t>(size_t(d.nbanks) * S * eb, 16));
+        const size_t bytes = std::max<size_t>(size_t(d.nbanks) * S * eb, 16);
+        if (l == 0) {
+            if (spare_inv0_.size() == bytes) d.inv = std::move(spare_inv0_);
+            spare_inv0_.release();
+        }
+        d.inv.resize(bytes);
         const int32_t *rp = l == 0 ? d_rowptr_.get() : d.rowptr.get();
         const int32_t *cl = l == 0 ? d_cols_.get() : d.cols.get();
         const double *vl = l == 0 ? d_bvals_.get() : d.vals.get();

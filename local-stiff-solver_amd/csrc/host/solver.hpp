@@ -65,7 +65,7 @@ private:
     struct DevLevel {
         int32_t nodes = 0, nbanks = 0;
         DBuf<int32_t> bank_off;
-        DBuf<uint8_t> inv;         // packed inverses, precision-dependent element type
+        DBuf<uint8_t> inv;         // bank factors, precision-dependent element type and layout
         DBuf<int32_t> rowptr, cols;  // level operator (levels >= 1)
         DBuf<double> vals;
         DBuf<int32_t> parent;      // to the next level (restriction: children of a coarse node lie in one bank)
@@ -139,6 +139,7 @@ private:
 
     // base preconditioner
     std::vector<DevLevel> mas_;
+    DBuf<uint8_t> spare_inv0_;  // the previous factorization's level-0 factor buffer, for reuse
     std::vector<int32_t> level_nodes_;
     DBuf<double> d_dinv_;
 
