@@ -918,7 +918,10 @@ int32_t device_partition(int32_t nb, const int32_t *rowptr, const int32_t *cols,
         LSS_HIP(hipMemcpyAsync(hflags, dflags.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
         LSS_HIP(hipStreamSynchronize(s));
         if (!hflags[0]) break;  // only leaves at this depth: the member array is final
-        const int32_t bx = std::max(1, 8192 >> depth);  // about 8192 blocks: one pass of 16 rows per block
+        // about 2048 blocks per level step: a frontier of up to 32 768 rows takes one pass
+        // of 16 rows per block, a larger one several; more blocks would only add
+        // dispatch rounds of blocks that find no rows (most frontiers are far smaller)
+        const int32_t bx = std::max(1, 2048 >> depth);
         const int64_t grid = int64_t(nsub) * bx;
         // each step reaches a node, restarts or switches searches: 4 nb bounds the steps
         constexpr int kBatch = 8;
