@@ -298,6 +298,29 @@ def test_reduced_precision_banks_converge(gpu, name, N, precision):
     assert info["num_iterations"] <= (1.25 if precision == "fp32" else 2.5) * it64 + 3
 
 
+# Without coarse MAS levels the level-0 apply itself reduces r.z for the PCG (the
+# dot-product variants of the FP32 tiled and FP64 packed kernels).  The oracle always
+# builds the reference's levels, so the check is convergence: both precisions reach
+# the tolerance, FP32 banks within the iterations the reduced-precision test allows,
+# and the two solutions agree.
+def test_single_level_mas_solve(gpu):
+    prob = problems.make("elastic_contact", N=10)
+    A = prob.csc()
+    S = A.to_scipy()
+    b = prob.rhs(0)
+    xs, its = {}, {}
+    for precision in ("fp64", "fp32"):
+        s = Solver.create(_cfg(3, precision=precision, max_coarse_levels=0))
+        s.factorize(A)
+        xs[precision] = s.solve(b)
+        info = s.get_info()
+        assert info["mas"]["levels"] == 1
+        assert _res_ok(S, xs[precision], b), precision
+        its[precision] = info["num_iterations"]
+    assert its["fp32"] <= 1.25 * its["fp64"] + 3, its
+    assert np.linalg.norm(xs["fp32"] - xs["fp64"]) <= 1e-5 * np.linalg.norm(xs["fp64"])
+
+
 def test_damping_retry_after_breakdown(gpu):
     # omega * lambda_max(BA) near 3.5 makes the damped sweep indefinite: the PCG
     # breaks down and the solve restarts from the same start with damping 1.5

@@ -58,6 +58,10 @@ def _singles(s, B, X0):
         ("elastic_contact", 10, 3, {"precision": "fp32"}),
         ("elastic_contact", 10, 2, {"precision": "fp32"}),
         ("laplace_sphere", 12, 4, {"precision": "fp32"}),
+        # no coarse MAS levels: the level-0 apply reduces r.z itself (its dot variants)
+        ("elastic_contact", 10, 2, {"precision": "fp32", "max_coarse_levels": 0}),
+        ("laplace_sphere", 12, 4, {"precision": "fp32", "max_coarse_levels": 0}),
+        ("elastic_contact", 10, 2, {"max_coarse_levels": 0}),
         ("laplace_sphere", 12, 2, {}),
         ("elastic_logE", 8, 4, {"base_preconditioner": "Jacobi"}),
         ("elastic_contact", 8, 2, {"do_hybrid_precond": False}),
