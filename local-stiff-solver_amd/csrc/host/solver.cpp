@@ -144,6 +144,7 @@ void HybridSolver::release()
     d_L_.release();
     d_swork_.release();
     xp_.release(), bp_.release(), r_.release(), p_.release(), q_.release(), z_.release(), u_.release();
+    z32_.release();
     ncomp_ = 0;
     ns_ = 0;
     omega_ = 1.0;
@@ -747,6 +748,7 @@ void HybridSolver::alloc_vectors()
 {
     const size_t np = size_t(nb_) * bs_;
     xp_.resize(np), bp_.resize(np), r_.resize(np), p_.resize(np), q_.resize(np), z_.resize(np), u_.resize(np);
+    if (cfg_.fp32_inner()) z32_.resize(np);
     kcap_ = 1;
     // partials of one reduction for kMaxCols columns
     int64_t per = std::max<int64_t>(k::partials_per_column(int64_t(np)), k::spmv_partials_per_column(sell_));
@@ -759,6 +761,7 @@ void HybridSolver::ensure_columns(int k)
     if (k <= kcap_) return;
     const size_t m = size_t(nb_) * bs_ * k;
     xp_.resize(m), bp_.resize(m), r_.resize(m), p_.resize(m), q_.resize(m), z_.resize(m), u_.resize(m);
+    if (cfg_.fp32_inner()) z32_.resize(m);
     kcap_ = k;
 }
 
@@ -850,7 +853,7 @@ void HybridSolver::estimate_omega(hipStream_t s)
 
 // ------------------------------------------------------------------ preconditioner
 void HybridSolver::base_apply(const double *r, double scale, const double *add, double *z, dev::PcgState *st,
-                              const int *stop, hipStream_t s, bool first, int k)
+                              const int *stop, hipStream_t s, bool first, int k, float *z32)
 {
     const int64_t np = int64_t(nb_) * bs_;
     const int op = first ? dev::kFinishRzFirst : dev::kFinishRz;
@@ -858,6 +861,7 @@ void HybridSolver::base_apply(const double *r, double scale, const double *add, 
         const int g = k::jacobi_apply(np, d_dinv_.get(), r, scale, add, z, st ? r_.get() : nullptr, st,
                                       partials_.get(), stop, s, k, np);
         if (st) k::finish(g, partials_.get(), op, st, nullptr, 0, s, k);
+        if (z32) k::to_f32(int64_t(k) * np, z, z32, s);
         return;
     }
     const int L = int(mas_.size()) - 1;
@@ -869,6 +873,7 @@ void HybridSolver::base_apply(const double *r, double scale, const double *add, 
                             nullptr, scale, add, z, st ? r_.get() : nullptr, st, partials_.get(), stop, s, k, np, 0);
         if (prof) prof_end(s);
         if (st) k::finish(mas_[0].nbanks, partials_.get(), op, st, nullptr, 0, s, k);
+        if (z32) k::to_f32(int64_t(k) * np, z, z32, s);
         return;
     }
     // The coarse levels (restriction chain, then the coarse banks from the top down) are a
@@ -895,7 +900,7 @@ void HybridSolver::base_apply(const double *r, double scale, const double *add, 
     if (prof) prof_end(s);
     LSS_HIP(hipStreamWaitEvent(s, ev_join_, 0));
     const int g = k::mas_prolong_add(np, bs_, mas_[0].parent.get(), mas_[1].w.get(), scale, z,
-                                     st ? r_.get() : nullptr, partials_.get(), stop, s, k, np, ld(1));
+                                     st ? r_.get() : nullptr, partials_.get(), stop, s, k, np, ld(1), z32);
     if (st) k::finish(g, partials_.get(), op, st, nullptr, 0, s, k);
 }
 
@@ -917,15 +922,19 @@ void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, 
         base_apply(r, 1.0, nullptr, z, st, stop, s, first, k);
         return;
     }
-    base_apply(r, omega_, nullptr, z, nullptr, stop, s, false, k);  // z1 = B r
+    // with FP32 values the inner SpMV also gathers an FP32 copy of z2 (half the gathered
+    // bytes), written by the producers of z1 and of its stiff entries
+    const bool f32 = cfg_.fp32_inner();
+    float *z32 = f32 ? z32_.get() : nullptr;
+    base_apply(r, omega_, nullptr, z, nullptr, stop, s, false, k, z32);  // z1 = B r
     // z2 = z1 + P (r - A z1)  (dss_precond_iter, GPUHybrid.cu:544-675)
     k::stiff_correct(ncomp_, ns_, bs_, d_comp_off_.get(), d_mat_off_.get(), d_row_comp_.get(), d_pdofs_.get(),
                      d_L_.get(), d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z, d_swork_.get(),
-                     d_swork_.get() + kMaxCols * ns_, stop, s, k, np);
+                     d_swork_.get() + kMaxCols * ns_, stop, s, k, np, false, z32);
     // u = r - A z2
     const bool prof = cfg_.profile_kernels && st && k == 1;
     if (prof) prof_begin(1, s);
-    k::spmv_residual(sell_, z, r, u_.get(), stop, s, cfg_.fp32_inner(), k, np);
+    k::spmv_residual(sell_, z, r, u_.get(), stop, s, f32, k, np, z32);
     if (prof) prof_end(s);
     if (cfg_.fp32_inner())
         k::stiff_rows_exact(ns_, bs_, d_pdofs_.get(), d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z, u_.get(),

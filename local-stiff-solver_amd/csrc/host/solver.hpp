@@ -95,8 +95,9 @@ private:
     // preconditioner pieces on permuted vectors
     // st != nullptr: gamma = r . z is reduced into the PCG state (first: before the loop)
     // k columns at stride nb * bs, states st[0..k), stop = &st[0].stop
+    // z32 != nullptr: also an FP32 copy of z (k columns at stride n)
     void base_apply(const double *r, double scale, const double *add, double *z, dev::PcgState *st, const int *stop,
-                    hipStream_t s, bool first = false, int k = 1);
+                    hipStream_t s, bool first = false, int k = 1, float *z32 = nullptr);
     void precond_apply(const double *r, double *z, dev::PcgState *st, const int *stop, hipStream_t s,
                        bool first = false, int k = 1);
     void run_pcg(hipStream_t s, int k = 1);
@@ -159,6 +160,7 @@ private:
 
     // PCG
     DBuf<double> xp_, bp_, r_, p_, q_, z_, u_;
+    DBuf<float> z32_;  // FP32 copy of z2 for the FP32 preconditioner-internal SpMV
     DBuf<double> partials_;
     DBuf<dev::PcgState> st_;         // kMaxCols states
     dev::PcgState *h_st_ = nullptr;  // pinned mirror

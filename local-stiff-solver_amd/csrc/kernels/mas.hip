@@ -1009,23 +1009,26 @@ __global__ __launch_bounds__(kApplyThreads, (kTiledWaves<BS, DOT, K>)) void k_le
 }
 
 // z += scale * w1[parent] (the level-1 correction prolongated to level 0) with an
-// optional per-workgroup partial of dotv . z; grid row y = column y.
+// optional per-workgroup partial of dotv . z and an optional FP32 copy z32 of the
+// result (same stride); grid row y = column y.
 template <bool DOT>
 __global__ __launch_bounds__(256) void k_prolong_add(int64_t np, int bs, const int32_t *__restrict__ parent,
                                                      const double *__restrict__ w1, int64_t ld1, double scale,
                                                      double *__restrict__ z, int64_t ld,
                                                      const double *__restrict__ dotv, double *partials,
-                                                     const int *__restrict__ stop)
+                                                     const int *__restrict__ stop, float *__restrict__ z32)
 {
     if (dev::all_stopped(stop, gridDim.y)) return;
     const int64_t o = blockIdx.y * ld;
     w1 += blockIdx.y * ld1;
     z += o;
+    if (z32) z32 += o;
     double part = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < np; i += int64_t(gridDim.x) * 256) {
         const int64_t node = i / bs;
         const double v = z[i] + scale * w1[int64_t(parent[node]) * bs + (i - node * bs)];
         z[i] = v;
+        if (z32) z32[i] = float(v);
         if constexpr (DOT) part += dotv[o + i] * v;
     }
     if constexpr (DOT) {
@@ -1220,16 +1223,16 @@ void level0_apply_k(Precision prec, int bs, int32_t nbanks, const int32_t *part_
 
 int mas_prolong_add(int64_t np, int bs, const int32_t *parent, const double *w1, double scale, double *z,
                     const double *dotv, double *partials, const int *stop, hipStream_t s, int k, int64_t ld,
-                    int64_t ld1)
+                    int64_t ld1, float *z32)
 {
     if (np == 0) return 0;
     const int grid = int(std::min<int64_t>(std::max<int64_t>(div_up(np, 256 * 4), 1), 4096));
     if (dotv)
         k_prolong_add<true><<<dim3(grid, k), 256, 0, s>>>(np, bs, parent, w1, ld1, scale, z, ld, dotv, partials,
-                                                          stop);
+                                                          stop, z32);
     else
         k_prolong_add<false><<<dim3(grid, k), 256, 0, s>>>(np, bs, parent, w1, ld1, scale, z, ld, nullptr,
-                                                           nullptr, stop);
+                                                           nullptr, stop, z32);
     LSS_LAUNCH_CHECK();
     return grid;
 }

@@ -39,9 +39,10 @@ void mas_coarse_apply(Precision prec, int bs, int32_t nodes, const void *inv, co
                       int64_t ld_parent = 0);
 // z += scale * w1[parent] over the level-0 scalars (parent: level-0 node -> level-1 node);
 // dotv != nullptr: per-workgroup partials of dotv . z (returns their count per column).
+// z32 != nullptr: also an FP32 copy of the updated z (same stride ld).
 int mas_prolong_add(int64_t np, int bs, const int32_t *parent, const double *w1, double scale, double *z,
                     const double *dotv, double *partials, const int *stop, hipStream_t s, int k = 1,
-                    int64_t ld = 0, int64_t ld1 = 0);
+                    int64_t ld = 0, int64_t ld1 = 0, float *z32 = nullptr);
 // st != nullptr: per-bank partials of dotv . z go to partials[c * nbanks + bank] (reduced by k::finish).
 void mas_level0_apply(Precision prec, int bs, int32_t nbanks, const int32_t *part_off, const void *inv,
                       const double *r, const int32_t *parent, const double *w1, double scale, const double *add,
@@ -75,12 +76,13 @@ void dense_mirror(int64_t m, double *G, hipStream_t s);
 // each component, (A z) taken from the block operator in the permuted numbering.
 // k right-hand sides: r, z at stride ld; work, work2 hold k * ns values each.
 // With additive = true the correction is z[p] += G^T G r[p] (the MAS solver's
-// additive stiff preconditioner, MASSolver.cu:583-615).
+// additive stiff preconditioner, MASSolver.cu:583-615).  z32 != nullptr: an FP32 copy
+// of z (same stride) whose stiff entries are updated with z.
 void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, const int64_t *mat_off,
                    const int32_t *row_comp, const int32_t *pdofs, const double *G, const int32_t *rowptr,
                    const int32_t *cols,
                    const double *bvals, const double *r, double *z, double *work, double *work2, const int *stop,
-                   hipStream_t s, int k = 1, int64_t ld = 0, bool additive = false);
+                   hipStream_t s, int k = 1, int64_t ld = 0, bool additive = false, float *z32 = nullptr);
 
 // u[p] = r[p] - (A z)[p] in FP64 for the stiff rows p: after a preconditioner-internal
 // SpMV with FP32 values, the rows that hold the stiff entries are recomputed exactly.

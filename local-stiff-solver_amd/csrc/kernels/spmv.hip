@@ -347,12 +347,16 @@ template <int BS, int MODE, bool DOT, bool F32, int K>
 __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslices, const int64_t *__restrict__ slice_ptr,
                                                         const int32_t *__restrict__ cols, const double *__restrict__ vals,
                                                         const float *__restrict__ vals32, const double *__restrict__ x,
+                                                        const float *__restrict__ x32,
                                                         double *__restrict__ y, const double *__restrict__ sub,
                                                         const int *__restrict__ stop, int64_t ld,
                                                         double *__restrict__ partials)
 {
     constexpr int H = 64 / BS;
     constexpr int BB = BS * BS;
+    // FP32 values go with the FP32 copy of x (x32): the preconditioner-internal
+    // residual, never the dot-product variant
+    static_assert(!(F32 && DOT), "the FP32 SpMV has no dot product");
     if (dev::all_stopped(stop, K)) return;
     const int lane = threadIdx.x & 63;
     const int32_t slice = blockIdx.x * (kSpmvThreads / 64) + (threadIdx.x >> 6);
@@ -399,7 +403,12 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslic
 #pragma unroll
                 for (int q = 0; q < BS; ++q)
 #pragma unroll
-                    for (int k = 0; k < K; ++k) xv[u][k][q] = x[k * ld + xc + q];
+                    for (int k = 0; k < K; ++k) {
+                        if constexpr (F32)
+                            xv[u][k][q] = double(x32[k * ld + xc + q]);
+                        else
+                            xv[u][k][q] = x[k * ld + xc + q];
+                    }
             }
             if (j0 + 2 * C <= W) {
 #pragma unroll
@@ -423,7 +432,12 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslic
                 const double a = F32 ? double(__builtin_nontemporal_load(&vals32[vb + q * (H * BS)]))
                                      : __builtin_nontemporal_load(&vals[vb + q * (H * BS)]);
 #pragma unroll
-                for (int k = 0; k < K; ++k) acc[k] += a * x[k * ld + xc + q];
+                for (int k = 0; k < K; ++k) {
+                    if constexpr (F32)
+                        acc[k] += a * double(x32[k * ld + xc + q]);
+                    else
+                        acc[k] += a * x[k * ld + xc + q];
+                }
             }
         }
     }
@@ -447,14 +461,14 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslic
 }
 
 template <int MODE, bool DOT, bool F32, int K>
-void launch_spmv_k(const SellView &A, const double *x, double *y, const double *sub, const int *stop, int64_t ld,
-                   double *partials, hipStream_t s)
+void launch_spmv_k(const SellView &A, const double *x, const float *x32, double *y, const double *sub,
+                   const int *stop, int64_t ld, double *partials, hipStream_t s)
 {
     const int grid = div_up(A.nslices, kSpmvThreads / 64);
     if (grid == 0) return;
 #define LSS_SPMV_CASE(BS)                                                                                      \
     k_spmv<BS, MODE, DOT, F32, K><<<grid, kSpmvThreads, 0, s>>>(A.nb, A.nslices, A.slice_ptr, A.cols, A.vals,  \
-                                                                A.vals32, x, y, sub, stop, ld, partials)
+                                                                A.vals32, x, x32, y, sub, stop, ld, partials)
     switch (A.bs) {
         case 1: LSS_SPMV_CASE(1); break;
         case 2: LSS_SPMV_CASE(2); break;
@@ -466,13 +480,13 @@ void launch_spmv_k(const SellView &A, const double *x, double *y, const double *
 }
 
 template <int MODE, bool DOT, bool F32>
-void launch_spmv(const SellView &A, const double *x, double *y, const double *sub, const int *stop, int k,
-                 int64_t ld, double *partials, hipStream_t s)
+void launch_spmv(const SellView &A, const double *x, const float *x32, double *y, const double *sub, const int *stop,
+                 int k, int64_t ld, double *partials, hipStream_t s)
 {
     switch (k) {
-        case 1: launch_spmv_k<MODE, DOT, F32, 1>(A, x, y, sub, stop, ld, partials, s); break;
-        case 2: launch_spmv_k<MODE, DOT, F32, 2>(A, x, y, sub, stop, ld, partials, s); break;
-        case 4: launch_spmv_k<MODE, DOT, F32, 4>(A, x, y, sub, stop, ld, partials, s); break;
+        case 1: launch_spmv_k<MODE, DOT, F32, 1>(A, x, x32, y, sub, stop, ld, partials, s); break;
+        case 2: launch_spmv_k<MODE, DOT, F32, 2>(A, x, x32, y, sub, stop, ld, partials, s); break;
+        case 4: launch_spmv_k<MODE, DOT, F32, 4>(A, x, x32, y, sub, stop, ld, partials, s); break;
         default: fail(LSS_ERR_ARG, "a fused SpMV takes 1, 2 or 4 right-hand sides");
     }
 }
@@ -540,15 +554,10 @@ void sell_fill(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols, c
     LSS_LAUNCH_CHECK();
 }
 
-void spmv(const SellView &A, const double *x, double *y, const double *sub, hipStream_t s, bool fp32)
+void spmv(const SellView &A, const double *x, double *y, const double *sub, hipStream_t s)
 {
-    if (fp32) {
-        if (sub) launch_spmv<1, false, true>(A, x, y, sub, nullptr, 1, 0, nullptr, s);
-        else launch_spmv<0, false, true>(A, x, y, nullptr, nullptr, 1, 0, nullptr, s);
-    } else {
-        if (sub) launch_spmv<1, false, false>(A, x, y, sub, nullptr, 1, 0, nullptr, s);
-        else launch_spmv<0, false, false>(A, x, y, nullptr, nullptr, 1, 0, nullptr, s);
-    }
+    if (sub) launch_spmv<1, false, false>(A, x, nullptr, y, sub, nullptr, 1, 0, nullptr, s);
+    else launch_spmv<0, false, false>(A, x, nullptr, y, nullptr, nullptr, 1, 0, nullptr, s);
 }
 
 int spmv_partials_per_column(const SellView &A) { return div_up(A.nslices, kSpmvThreads / 64); }
@@ -556,15 +565,34 @@ int spmv_partials_per_column(const SellView &A) { return div_up(A.nslices, kSpmv
 void spmv_pq(const SellView &A, const double *p, double *q, dev::PcgState *st, double *partials, int iter,
              hipStream_t s, int k, int64_t ld, int finish_op)
 {
-    launch_spmv<0, true, false>(A, p, q, nullptr, &st->stop, k, ld, partials, s);
+    launch_spmv<0, true, false>(A, p, nullptr, q, nullptr, &st->stop, k, ld, partials, s);
     finish(spmv_partials_per_column(A), partials, finish_op, st, nullptr, iter, s, k);
 }
 
 void spmv_residual(const SellView &A, const double *x, const double *b, double *r, const int *stop, hipStream_t s,
-                   bool fp32, int k, int64_t ld)
+                   bool fp32, int k, int64_t ld, const float *x32)
 {
-    if (fp32) launch_spmv<1, false, true>(A, x, r, b, stop, k, ld, nullptr, s);
-    else launch_spmv<1, false, false>(A, x, r, b, stop, k, ld, nullptr, s);
+    if (fp32) {
+        if (!x32 || !A.vals32) fail(LSS_ERR_ARG, "the FP32 residual needs the FP32 values and x32");
+        launch_spmv<1, false, true>(A, x, x32, r, b, stop, k, ld, nullptr, s);
+    } else {
+        launch_spmv<1, false, false>(A, x, nullptr, r, b, stop, k, ld, nullptr, s);
+    }
+}
+
+namespace {
+__global__ __launch_bounds__(256) void k_to_f32(int64_t n, const double *__restrict__ x, float *__restrict__ y)
+{
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+        y[i] = float(x[i]);
+}
+}  // namespace
+
+void to_f32(int64_t n, const double *x, float *y, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_to_f32<<<int(std::min<int64_t>(div_up(n, 256), 8192)), 256, 0, s>>>(n, x, y);
+    LSS_LAUNCH_CHECK();
 }
 
 }  // namespace lss::k

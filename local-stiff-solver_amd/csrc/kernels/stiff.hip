@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void k_stiff_rows(int64_t ns, const int32_t *_
                                                     const int64_t *__restrict__ mat_off, const int32_t *__restrict__ pdofs,
                                                     const double *__restrict__ M, const double *__restrict__ in,
                                                     double *__restrict__ out, double *__restrict__ z, int64_t ld,
-                                                    const int *__restrict__ stop)
+                                                    const int *__restrict__ stop, float *__restrict__ z32)
 {
     if (dev::all_stopped(stop, gridDim.y)) return;
     const int64_t k = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
@@ -222,6 +222,7 @@ __global__ __launch_bounds__(256) void k_stiff_rows(int64_t ns, const int32_t *_
     in += blockIdx.y * ns;
     if (out) out += blockIdx.y * ns;
     z += blockIdx.y * ld;
+    if (z32) z32 += blockIdx.y * ld;
     const int32_t c = row_comp[k];
     const int64_t off = comp_off[c];
     const int32_t m = int32_t(comp_off[c + 1] - off);
@@ -233,9 +234,12 @@ __global__ __launch_bounds__(256) void k_stiff_rows(int64_t ns, const int32_t *_
     for (int32_t j = lo + lane; j < hi; j += 64) acc += row[j] * v[j];
     acc = dev::wave_sum(acc);
     if (lane == 0) {
-        if (TRANSPOSED)
-            z[pdofs[k]] += acc;  // z2 = z1 + S^T x (GPUHybrid.cu:621-671)
-        else
+        if (TRANSPOSED) {
+            const int32_t d = pdofs[k];
+            const double v = z[d] + acc;  // z2 = z1 + S^T x (GPUHybrid.cu:621-671)
+            z[d] = v;
+            if (z32) z32[d] = float(v);
+        } else
             out[k] = acc;
     }
 }
@@ -388,17 +392,17 @@ void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, c
                    const int32_t *row_comp, const int32_t *pdofs, const double *G, const int32_t *rowptr,
                    const int32_t *cols,
                    const double *bvals, const double *r, double *z, double *work, double *work2, const int *stop,
-                   hipStream_t s, int k, int64_t ld, bool additive)
+                   hipStream_t s, int k, int64_t ld, bool additive, float *z32)
 {
     if (ncomp == 0 || ns == 0) return;
     k_stiff_residual<false><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r,
                                                                   additive ? nullptr : z, ld, work, stop);
     LSS_LAUNCH_CHECK();
     k_stiff_rows<false><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, row_comp, comp_off, mat_off, pdofs, G, work, work2,
-                                                              z, ld, stop);
+                                                              z, ld, stop, nullptr);
     LSS_LAUNCH_CHECK();
     k_stiff_rows<true><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, row_comp, comp_off, mat_off, pdofs, G, work2,
-                                                             nullptr, z, ld, stop);
+                                                             nullptr, z, ld, stop, z32);
     LSS_LAUNCH_CHECK();
 }
 
