@@ -111,8 +111,8 @@ int lss_get_stiff_dofs(const lss_solver *s, int32_t *dofs, int64_t cap, int64_t 
                        int64_t comp_cap, int64_t *n_comp);
 
 /* Last error message of this handle (empty string when none).  With s == NULL:
- * the message of the last failed lss_create or handle-free call (lss_adjoint_*)
- * on this thread. */
+ * the message of the last failed lss_create or handle-free call (lss_adjoint_*,
+ * lss_residual_norm) on this thread. */
 const char *lss_last_error(const lss_solver *s);
 
 /* Release all device memory.  Replaces GPUHybrid::~GPUHybrid (GPUHybrid.cu:1514-1530). */
@@ -137,6 +137,19 @@ int lss_adjoint_pattern_grad(int32_t n, int64_t nnz, const int32_t *d_colptr, co
  * e.g. per-element coefficients with (slot, unit-coefficient value) pairs per element. */
 int lss_adjoint_chain(int64_t nseg, const int64_t *d_seg_ptr, const int32_t *d_slot, const double *d_coef,
                       const double *d_grad_vals, double *d_out, void *stream);
+
+/* ---- true residual --------------------------------------------------------
+ * d_out[0] = ||b - A x||_2 and d_out[1] = ||b||_2 (two doubles in device memory)
+ * for the CSC arrays read as operator rows, computed in double-double arithmetic
+ * (exact products by fused multiply-add, compensated sums), so the norm is
+ * accurate even where the FP64 evaluation of b - A x has an error floor of
+ * eps * || |A| |x| || above the residual itself.  The reference reports the FP64
+ * residual norm (final_res_norm, GPUHybrid.cu:383-386); this is its
+ * extended-precision counterpart for checking a solution.  Handle-free and
+ * asynchronous on `stream` (NULL = the null stream); allocates and frees its
+ * partial sums (32 KiB at most) in stream order. */
+int lss_residual_norm(int32_t n, int64_t nnz, const int32_t *d_colptr, const int32_t *d_rowidx, const double *d_vals,
+                      const double *d_b, const double *d_x, double *d_out, void *stream);
 
 /* ---- inspection entry points (parity tests and tools) --------------------
  * Each runs the named device stage of the factorized solver on host data.  No

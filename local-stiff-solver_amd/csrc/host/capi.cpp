@@ -222,6 +222,26 @@ int lss_adjoint_chain(int64_t nseg, const int64_t *d_seg_ptr, const int32_t *d_s
     });
 }
 
+int lss_residual_norm(int32_t n, int64_t nnz, const int32_t *d_colptr, const int32_t *d_rowidx, const double *d_vals,
+                      const double *d_b, const double *d_x, double *d_out, void *stream)
+{
+    return guarded_free([&] {
+        if (n < 0 || nnz < 0 || !d_out || (n > 0 && (!d_colptr || !d_b || !d_x)) ||
+            (nnz > 0 && (!d_rowidx || !d_vals)))
+            lss::fail(LSS_ERR_ARG, "invalid arguments");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        void *part = nullptr;
+        LSS_HIP(hipMallocAsync(&part, size_t(4) * lss::k::residual_groups(n) * sizeof(double), s));
+        try {
+            lss::k::residual_norm_dd(n, d_colptr, d_rowidx, d_vals, d_b, d_x, static_cast<double *>(part), d_out, s);
+        } catch (...) {
+            (void)hipFreeAsync(part, s);
+            throw;
+        }
+        LSS_HIP(hipFreeAsync(part, s));
+    });
+}
+
 void lss_destroy(lss_solver *s) { delete s; }
 
 int lss_debug_row_norms(const lss_solver *s, double *out, int64_t n)

@@ -4,6 +4,7 @@
 #pragma once
 #include <cstdint>
 #include <string>
+#include <tuple>
 
 namespace lss {
 
@@ -64,6 +65,23 @@ struct Config {
     // of a 1e8 contact entry exceeds the soft entries and would make the sweep
     // inconsistent with the FP64 stiff solve.
     bool fp32_inner() const { return inner_spmv == "fp32" || (inner_spmv == "auto" && precision != "fp64"); }
+
+    // Every key that shapes what factorize builds (buffers, layouts, selections, omega).
+    // The others (tolerances, iteration caps, polling, timing, residual checks) only
+    // steer a solve and may change between factorize and solve.
+    auto factor_keys() const
+    {
+        return std::tie(solver, block_size, theta, do_hybrid_precond, select_bad_dofs_from_l1_norm,
+                        conditioning_threshold, gmm_jump_threshold, gmm_tol, max_gmm_iterations,
+                        decompose_subdomains, min_subdomain_size, max_subdomain_size, expand_subdomains,
+                        max_dense_size, use_float_on_subdomains, lazy_partitioning, use_bad_dof_precond,
+                        bad_dof_strategy, bad_dof_kmeans_search_fraction, bad_dof_kmeans_jump_threshold,
+                        bad_dof_kmeans_max_iterations, bad_dof_kmeans_expand_neighbors, bad_dof_gmm_jump_threshold,
+                        bad_dof_gmm_tol, bad_dof_gmm_max_iterations, bad_dof_gmm_expand_neighbors, precision,
+                        inner_spmv, base_preconditioner, damping, lanczos_steps, max_dense_component, partitioner,
+                        max_coarse_levels, partition_local_max);
+    }
+    bool same_factorization(const Config &o) const { return factor_keys() == o.factor_keys(); }
 };
 
 // Parse a configuration document; throws lss::Error(LSS_ERR_CONFIG) on bad input.

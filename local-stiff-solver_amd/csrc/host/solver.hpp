@@ -32,7 +32,13 @@ public:
     HybridSolver(const HybridSolver &) = delete;
     HybridSolver &operator=(const HybridSolver &) = delete;
 
-    void set_config(const Config &cfg) { cfg_ = cfg; }
+    // A change of any key that shapes the factorization drops it (buffers and layouts
+    // were built for the old keys): solve then reports LSS_ERR_NOT_FACTORIZED.
+    void set_config(const Config &cfg)
+    {
+        if (factorized_ && !cfg.same_factorization(cfg_)) reset_factorization();
+        cfg_ = cfg;
+    }
     const Config &config() const { return cfg_; }
     void set_problematic_dofs(const int32_t *dofs, int64_t n);
 

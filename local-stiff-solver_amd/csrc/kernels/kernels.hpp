@@ -150,6 +150,15 @@ bool device_galerkin_pattern(int64_t fine_nnz, const int32_t *rowptr, const int3
 // bank_off = {0, 32, 64, ..., n}; returns the bank count.
 int32_t uniform_banks(int32_t n, DBuf<int32_t> &bank_off, hipStream_t s);
 
+// ---------------------------------------------------------------- true residual
+// out[0] = ||b - A x||_2, out[1] = ||b||_2 (device, two doubles) in double-double
+// arithmetic; the CSC arrays are read as operator rows.  partials holds
+// 4 * residual_groups(n) doubles (at most 4 * kResidualGroups).
+constexpr int kResidualGroups = 2048;
+int residual_groups(int32_t n);
+void residual_norm_dd(int32_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals, const double *b,
+                      const double *x, double *partials, double *out, hipStream_t s);
+
 // ---------------------------------------------------------------- adjoint
 // grad[e] = -lam[j] * x[rowidx[e]] for every entry e of CSC column j (operator entry (j, rowidx[e])).
 void pattern_grad(int32_t n, const int32_t *colptr, const int32_t *rowidx, const double *lam, const double *x,

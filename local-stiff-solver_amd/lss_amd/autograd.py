@@ -22,7 +22,7 @@ import ctypes
 import torch
 
 from ._lib import LssError, load
-from .solver import Solver
+from .solver import Solver, require_device
 
 _P = ctypes.c_void_p
 
@@ -42,7 +42,11 @@ def _cur_stream():
 
 def pattern_grad(colptr: torch.Tensor, rowidx: torch.Tensor, lam: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
     """-lam_j x_i on every stored entry (CSC column j, row rowidx[e])."""
+    require_device(colptr, torch.int32, "colptr")
     n = int(colptr.numel()) - 1
+    require_device(rowidx, torch.int32, "rowidx")
+    require_device(lam, torch.float64, "lam", numel=n)
+    require_device(x, torch.float64, "x", numel=n)
     grad = torch.empty(rowidx.numel(), dtype=torch.float64, device=x.device)
     _check_free(load().lss_adjoint_pattern_grad(n, int(rowidx.numel()), _dp(colptr), _dp(rowidx), _dp(lam), _dp(x),
                                                 _dp(grad), _cur_stream()))
@@ -51,9 +55,14 @@ def pattern_grad(colptr: torch.Tensor, rowidx: torch.Tensor, lam: torch.Tensor, 
 
 def element_grad(seg_ptr: torch.Tensor, slot: torch.Tensor, coef: torch.Tensor, grad_vals: torch.Tensor) -> torch.Tensor:
     """out[e] = sum_k coef[k] grad_vals[slot[k]] over element e's Jacobian entries."""
+    require_device(seg_ptr, torch.int64, "seg_ptr")
     nseg = int(seg_ptr.numel()) - 1
+    require_device(slot, torch.int32, "slot")
+    require_device(coef, torch.float64, "coef", numel=slot.numel())
+    grad_vals = grad_vals.contiguous()
+    require_device(grad_vals, torch.float64, "grad_vals")
     out = torch.empty(nseg, dtype=torch.float64, device=grad_vals.device)
-    _check_free(load().lss_adjoint_chain(nseg, _dp(seg_ptr), _dp(slot), _dp(coef), _dp(grad_vals.contiguous()),
+    _check_free(load().lss_adjoint_chain(nseg, _dp(seg_ptr), _dp(slot), _dp(coef), _dp(grad_vals),
                                          _dp(out), _cur_stream()))
     return out
 
@@ -71,8 +80,9 @@ class HybridSolve(torch.autograd.Function):
         n = int(b.numel())
         stream = torch.cuda.current_stream()
         solver.factorize_device(n, colptr, rowidx, vals.detach().contiguous(), stream=stream)
+        b = b.detach().contiguous()
         x = torch.zeros_like(b)
-        solver.solve_device(b.detach().contiguous(), x, stream=stream)
+        solver.solve_device(b, x, stream=stream)
         ctx.solver = solver
         ctx.forward_info = solver.get_info()
         ctx.save_for_backward(x, colptr, rowidx)

@@ -16,10 +16,39 @@ import numpy as np
 from ._lib import LssError, load
 
 _P = ctypes.c_void_p
+LSS_ERR_SIZE, LSS_ERR_NOT_FACTORIZED, LSS_ERR_ARG = 2, 3, 8
 
 
 def _ptr(a: np.ndarray):
     return _P(a.ctypes.data)
+
+
+def require_device(t, dtype, name: str, numel: int | None = None, shape: tuple | None = None):
+    """Check a tensor handed to the C ABI as a raw device pointer: a contiguous GPU
+    tensor of `dtype` with `numel` elements (or `shape`).  The C side reads the
+    pointer as that type and length, so anything else would be read out of bounds."""
+    import torch
+
+    if not isinstance(t, torch.Tensor):
+        raise LssError(LSS_ERR_ARG, f"{name} must be a torch.Tensor")
+    if not t.is_cuda:
+        raise LssError(LSS_ERR_ARG, f"{name} must be a GPU tensor (got device {t.device})")
+    if t.dtype != dtype:
+        raise LssError(LSS_ERR_ARG, f"{name} must be {dtype} (got {t.dtype})")
+    if not t.is_contiguous():
+        raise LssError(LSS_ERR_ARG, f"{name} must be contiguous")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise LssError(LSS_ERR_SIZE, f"{name} must have shape {tuple(shape)} (got {tuple(t.shape)})")
+    if numel is not None and t.numel() != numel:
+        raise LssError(LSS_ERR_SIZE, f"{name} must have {numel} elements (got {t.numel()})")
+    return t
+
+
+def _host_vector(a, n: int, name: str) -> np.ndarray:
+    v = np.ascontiguousarray(a, dtype=np.float64)
+    if v.ndim != 1 or v.size != n:
+        raise LssError(LSS_ERR_SIZE, f"{name} must be a vector of {n} values (got shape {v.shape})")
+    return v
 
 
 class Solver:
@@ -80,12 +109,22 @@ class Solver:
     def factorize(self, A):
         """A: scipy.sparse matrix or (n, colptr, rowidx, vals) CSC arrays (host)."""
         n, colptr, rowidx, vals = _csc_arrays(A)
+        if colptr.size != n + 1:
+            raise LssError(LSS_ERR_SIZE, f"colptr must have n + 1 = {n + 1} entries (got {colptr.size})")
+        nnz = int(colptr[-1])
+        if rowidx.size < nnz or vals.size < nnz:
+            raise LssError(LSS_ERR_SIZE, f"rowidx and vals must hold colptr[n] = {nnz} entries")
         self._check(self._lib.lss_factorize_csc(self._h, n, int(colptr[-1]), _ptr(colptr), _ptr(rowidx), _ptr(vals)))
         self._n = n
 
     def factorize_device(self, n: int, colptr, rowidx, vals, stream=None):
-        """Device CSC tensors (torch, on the GPU)."""
+        """Device CSC tensors (torch, on the GPU): colptr, rowidx int32; vals float64."""
+        import torch
+
+        require_device(colptr, torch.int32, "colptr", numel=n + 1)
+        require_device(rowidx, torch.int32, "rowidx")
         nnz = int(rowidx.numel())
+        require_device(vals, torch.float64, "vals", numel=nnz)
         self._check(self._lib.lss_factorize_csc_device(self._h, n, nnz, _P(colptr.data_ptr()),
                                                        _P(rowidx.data_ptr()), _P(vals.data_ptr()),
                                                        _stream(stream)))
@@ -93,22 +132,41 @@ class Solver:
 
     def solve(self, b, x=None) -> np.ndarray:
         """Solve A x = b; x (optional) is the initial guess (GPUHybrid.cu:342-348)."""
-        b = np.ascontiguousarray(b, dtype=np.float64)
-        x = np.zeros_like(b) if x is None else np.array(x, dtype=np.float64, copy=True)
+        n = self._factorized_size()
+        b = _host_vector(b, n, "b")
+        x = np.zeros_like(b) if x is None else _host_vector(x, n, "x").copy()
         self._check(self._lib.lss_solve(self._h, _ptr(b), _ptr(x)))
         return x
 
     def solve_device(self, b, x, stream=None):
         """b, x: contiguous float64 torch tensors on the GPU; x is updated in place."""
+        import torch
+
+        n = self._factorized_size()
+        require_device(b, torch.float64, "b", numel=n)
+        require_device(x, torch.float64, "x", numel=n)
         self._check(self._lib.lss_solve_device(self._h, _P(b.data_ptr()), _P(x.data_ptr()), _stream(stream)))
         return x
 
     def solve_multi_device(self, B, X, stream=None):
         """B, X: (k, n) contiguous float64 device tensors (one right-hand side per row)."""
+        import torch
+
+        n = self._factorized_size()
+        require_device(B, torch.float64, "B")
+        if B.dim() != 2:
+            raise LssError(LSS_ERR_SIZE, f"B must be (k, n) (got shape {tuple(B.shape)})")
         k = int(B.shape[0])
+        require_device(B, torch.float64, "B", shape=(k, n))
+        require_device(X, torch.float64, "X", shape=(k, n))
         self._check(self._lib.lss_solve_multi_device(self._h, k, _P(B.data_ptr()), _P(X.data_ptr()),
                                                      _stream(stream)))
         return X
+
+    def _factorized_size(self) -> int:
+        if self._n is None:
+            raise LssError(LSS_ERR_NOT_FACTORIZED, "solve called before factorize")
+        return self._n
 
     # ---------------------------------------------------------------- inspection
     def stiff_components(self) -> list[np.ndarray]:
@@ -121,7 +179,7 @@ class Solver:
         return [dofs[off[c]:off[c + 1]].copy() for c in range(nc.value)]
 
     def row_norms(self) -> np.ndarray:
-        out = np.empty(self._n, np.float64)
+        out = np.empty(self._factorized_size(), np.float64)
         self._check(self._lib.lss_debug_row_norms(self._h, _ptr(out), self._n))
         return out
 
@@ -145,22 +203,45 @@ class Solver:
         return perm, off
 
     def spmv(self, x) -> np.ndarray:
-        x = np.ascontiguousarray(x, np.float64)
+        x = _host_vector(x, self._factorized_size(), "x")
         y = np.empty_like(x)
         self._check(self._lib.lss_debug_spmv(self._h, _ptr(x), _ptr(y)))
         return y
 
     def base_apply(self, r) -> np.ndarray:
-        r = np.ascontiguousarray(r, np.float64)
+        r = _host_vector(r, self._factorized_size(), "r")
         z = np.empty_like(r)
         self._check(self._lib.lss_debug_base_apply(self._h, _ptr(r), _ptr(z)))
         return z
 
     def precond_apply(self, r) -> np.ndarray:
-        r = np.ascontiguousarray(r, np.float64)
+        r = _host_vector(r, self._factorized_size(), "r")
         z = np.empty_like(r)
         self._check(self._lib.lss_debug_precond_apply(self._h, _ptr(r), _ptr(z)))
         return z
+
+
+def residual_norm(colptr, rowidx, vals, b, x, stream=None) -> tuple[float, float]:
+    """(||b - A x||_2, ||b||_2) in double-double arithmetic on the GPU (lss_residual_norm);
+    device CSC tensors (int32, int32, float64) read as operator rows, float64 b and x."""
+    import torch
+
+    require_device(colptr, torch.int32, "colptr")
+    n = int(colptr.numel()) - 1
+    require_device(rowidx, torch.int32, "rowidx")
+    nnz = int(rowidx.numel())
+    require_device(vals, torch.float64, "vals", numel=nnz)
+    require_device(b, torch.float64, "b", numel=n)
+    require_device(x, torch.float64, "x", numel=n)
+    out = torch.empty(2, dtype=torch.float64, device=b.device)
+    lib = load()
+    st = _stream(stream) if stream is not None else _P(int(torch.cuda.current_stream().cuda_stream))
+    rc = lib.lss_residual_norm(n, nnz, _P(colptr.data_ptr()), _P(rowidx.data_ptr()), _P(vals.data_ptr()),
+                               _P(b.data_ptr()), _P(x.data_ptr()), _P(out.data_ptr()), st)
+    if rc != 0:
+        raise LssError(rc, lib.lss_last_error(None).decode())
+    h = out.cpu()
+    return float(h[0]), float(h[1])
 
 
 def _stream(stream):

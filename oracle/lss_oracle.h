@@ -167,6 +167,7 @@ void orc_hybrid_set_partition(orc_hybrid *h, const int32_t *perm, int32_t nb, co
                               int32_t nparts);
 /* Damping omega = 1 / lambda_max(BA) of the base preconditioner in the B-P-B sweep. */
 double orc_hybrid_omega(const orc_hybrid *h);
+void orc_hybrid_set_omega(orc_hybrid *h, double omega);
 /* 1 when the last solve broke down with the configured damping and was redone with damping 1.5 */
 int orc_hybrid_damping_retry(const orc_hybrid *h);
 /* Thread count (0: OpenMP default) and iteration cap of the following solves (timing of the

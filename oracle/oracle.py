@@ -96,6 +96,8 @@ def lib():
         L.orc_hybrid_mas_status.argtypes = [P, P]
         L.orc_hybrid_omega.restype = ctypes.c_double
         L.orc_hybrid_omega.argtypes = [P]
+        L.orc_hybrid_set_omega.restype = None
+        L.orc_hybrid_set_omega.argtypes = [P, ctypes.c_double]
         L.orc_hybrid_damping_retry.argtypes = [P]
         L.orc_hybrid_set_run.restype = None
         L.orc_hybrid_set_run.argtypes = [P, ctypes.c_int, ctypes.c_int]
@@ -338,6 +340,10 @@ class Hybrid:
         z = np.empty_like(r)
         lib().orc_hybrid_precond(self._h, _p(r), _p(z))
         return z
+
+    def set_omega(self, omega):
+        """Damping of B inside the multiplicative sweep (replaces factorize's estimate)."""
+        lib().orc_hybrid_set_omega(self._h, float(omega))
 
     def set_run(self, threads, max_iter):
         """Thread count and iteration cap of the following solves."""

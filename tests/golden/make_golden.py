@@ -12,6 +12,7 @@ Outputs (committed):
                          (tests/linear_solve.cpp:112-132, seed 0)
   A_contact_golden.npz   row l1 norms, the stiff set after each selection stage,
                          the component offsets and a direct solution
+  A_pq.npz, A_pq_golden.npz   the same for A_pq
   golden.json            scalar goldens for both matrices (GMM parameters,
                          stage sizes, component sizes, set digests)
 
@@ -99,13 +100,12 @@ def main(mats_dir):
         x = spla.spsolve(A.T.tocsc(), b)
         rec["direct_rel_residual"] = float(np.linalg.norm(A.T @ x - b) / np.linalg.norm(b))
         out[name] = rec
-        if name == "A_contact":
-            np.savez_compressed(os.path.join(HERE, "A_contact.npz"), n=np.int64(n), colptr=A.indptr.astype(np.int32),
-                                rowidx=A.indices.astype(np.int32), vals=A.data, rhs=b)
-            off = np.cumsum([0] + [c.size for c in comps]).astype(np.int64)
-            np.savez_compressed(os.path.join(HERE, "A_contact_golden.npz"), row_l1=l1, selected=s0, filtered=s1,
-                                expanded=s2, comp_off=off, comp_dofs=np.concatenate(comps) if comps else s2,
-                                x_direct=x)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), n=np.int64(n), colptr=A.indptr.astype(np.int32),
+                            rowidx=A.indices.astype(np.int32), vals=A.data, rhs=b)
+        off = np.cumsum([0] + [c.size for c in comps]).astype(np.int64)
+        np.savez_compressed(os.path.join(HERE, name + "_golden.npz"), row_l1=l1, selected=s0, filtered=s1,
+                            expanded=s2, comp_off=off, comp_dofs=np.concatenate(comps) if comps else s2,
+                            x_direct=x)
         print(name, json.dumps({k: v for k, v in rec.items() if k != "component_sizes"}), flush=True)
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1)

@@ -257,3 +257,47 @@ def kmeans_numpy(x, fraction=0.02, jump=10.0, max_iter=32, expand=0):
         return np.zeros(0, np.int64), st
     sel = o[:sub][v > 0.5 * (c[0] + c[1])]
     return window_union(sel, n, expand), st
+
+
+def mas_coarse_maps(Bg: sp.csr_matrix, part_off, max_coarse=4, bank=32):
+    """Coarse-space maps of the MAS, from the rule of MASPreconditioner.cu:128-213, 376-481.
+
+    Bg: block graph in partition order (node v of part p lies in
+    [part_off[p], part_off[p+1])).  Level 0: the coarse nodes are the connected
+    components of each part's induced subgraph.  Level l >= 1: the level l-1
+    coarse nodes form banks of `bank` consecutive ids, the graph is the quotient
+    of the block graph, and the coarse nodes are the components of each bank's
+    induced subgraph.  Inside a bank (and so overall, banks being ascending id
+    ranges) components are numbered by their smallest member.  A level that
+    merges nothing ends the construction; at most `max_coarse` levels.
+    Returns (maps, counts): maps[l][v] = level-l coarse id of block node v.
+    Uses scipy's connected_components per level, not the lane masks."""
+    nb = Bg.shape[0]
+    part_off = np.asarray(part_off, np.int64)
+    G = sp.coo_matrix(Bg)
+    u, w = G.row.astype(np.int64), G.col.astype(np.int64)
+    maps, counts = [], []
+    cur = np.arange(nb, dtype=np.int64)  # level l-1 id of every block node (identity before level 0)
+    ncur = nb
+    bank_of_fine = np.searchsorted(part_off, np.arange(nb), side="right") - 1
+    for lv in range(max_coarse):
+        if lv == 0:
+            bk = bank_of_fine  # parts are the level-0 banks
+        else:
+            bk = np.arange(ncur) // bank
+        a, c = cur[u], cur[w]
+        keep = bk[a] == bk[c]
+        E = sp.coo_matrix((np.ones(int(keep.sum())), (a[keep], c[keep])), shape=(ncur, ncur))
+        ncomp, lab = connected_components(E, directed=False)
+        mins = np.full(ncomp, ncur, np.int64)
+        np.minimum.at(mins, lab, np.arange(ncur))
+        rank = np.empty(ncomp, np.int64)
+        rank[np.argsort(mins, kind="stable")] = np.arange(ncomp)
+        newid = rank[lab]
+        if lv > 0 and ncomp == ncur:
+            break
+        cur = newid[cur]
+        maps.append(cur.copy())
+        counts.append(ncomp)
+        ncur = ncomp
+    return maps, counts

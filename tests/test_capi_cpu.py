@@ -67,3 +67,71 @@ def test_no_device_fails_loudly():
     with pytest.raises(_lib.LssError) as e:
         Solver.create({"solver": "GPUHybrid"})
     assert e.value.code == 4  # LSS_ERR_HIP: no silent CPU path
+
+
+# ---------------------------------------------------------------- wrapper argument checks
+# The Python wrappers hand raw device pointers to the C ABI, which reads them as the
+# documented type and length; every mismatch must raise before a native call.  A Solver
+# without a native handle shows that the checks run first (no GPU is needed).
+def _unbound(n=None):
+    s = Solver.__new__(Solver)
+    s._h, s._lib, s._n = None, None, n
+    return s
+
+
+def test_solve_before_factorize_raises():
+    import numpy as np
+
+    with pytest.raises(_lib.LssError) as e:
+        _unbound().solve(np.zeros(3))
+    assert e.value.code == 3  # LSS_ERR_NOT_FACTORIZED
+
+
+@pytest.mark.parametrize("b,code", [([1.0, 2.0], 2), (list(range(5)), 2), ([[1.0, 2.0, 3.0]], 2)])
+def test_host_vector_size_checked(b, code):
+    with pytest.raises(_lib.LssError) as e:
+        _unbound(3).solve(b)
+    assert e.value.code == code
+
+
+def test_host_matrix_arrays_checked():
+    import numpy as np
+
+    s = _unbound()
+    with pytest.raises(_lib.LssError) as e:  # colptr of n entries instead of n + 1
+        s.factorize((3, np.array([0, 1, 2], np.int32), np.array([0, 1, 2], np.int32), np.ones(3)))
+    assert e.value.code == 2
+    with pytest.raises(_lib.LssError) as e:  # colptr[n] beyond the value array
+        s.factorize((2, np.array([0, 1, 4], np.int32), np.array([0, 1], np.int32), np.ones(2)))
+    assert e.value.code == 2
+
+
+def test_device_tensor_checks():
+    import torch
+
+    from lss_amd.solver import require_device
+
+    cpu = torch.zeros(4, dtype=torch.float64)
+    with pytest.raises(_lib.LssError) as e:
+        require_device(cpu, torch.float64, "b", numel=4)
+    assert e.value.code == 8 and "GPU" in str(e.value)
+    with pytest.raises(_lib.LssError) as e:
+        require_device([0.0] * 4, torch.float64, "b")
+    assert e.value.code == 8
+    # the device wrappers check their tensors before any native call: CPU tensors,
+    # int64 indices and float32 values are all rejected with LSS_ERR_ARG
+    s = _unbound(4)
+    with pytest.raises(_lib.LssError) as e:
+        s.solve_device(cpu, cpu)
+    assert e.value.code == 8
+    with pytest.raises(_lib.LssError) as e:
+        s.factorize_device(4, torch.zeros(5, dtype=torch.int64), torch.zeros(4, dtype=torch.int64), cpu)
+    assert e.value.code == 8
+    with pytest.raises(_lib.LssError) as e:
+        s.solve_multi_device(torch.zeros(2, 4, dtype=torch.float32), cpu)
+    assert e.value.code == 8
+    from lss_amd import autograd
+
+    with pytest.raises(_lib.LssError) as e:
+        autograd.pattern_grad(torch.zeros(5, dtype=torch.int64), torch.zeros(4, dtype=torch.int32), cpu, cpu)
+    assert e.value.code == 8

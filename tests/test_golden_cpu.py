@@ -88,16 +88,39 @@ def test_contact_hybrid_solution(contact):
     assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
 
 
+@pytest.fixture(scope="module")
+def pq():
+    d = np.load(os.path.join(GOLD, "A_pq.npz"))
+    g = np.load(os.path.join(GOLD, "A_pq_golden.npz"))
+    return d, g
+
+
+def test_pq_fixture_is_the_mtx(pq):
+    d, _ = pq
+    gold = _gold()["A_pq"]
+    assert int(d["n"]) == gold["n"] and d["vals"].size == gold["nnz"]
+    assert hashlib.sha256(d["rhs"].tobytes()).hexdigest()[:16] == gold["rhs_digest"]
+    assert np.array_equal(d["rhs"], problems.random_vector(int(d["n"]), 0))
+
+
 @pytest.mark.skipif(not os.path.exists(os.path.join(MATS, "A_pq.mtx")), reason="reference data files not mounted")
-def test_pq_selection_stages():
+def test_pq_fixture_arrays_equal_the_mtx(pq):
     import scipy.io
 
-    gold = _gold()["A_pq"]
+    d, _ = pq
     A = scipy.io.mmread(os.path.join(MATS, "A_pq.mtx")).tocsc()
     A.sum_duplicates()
     A.sort_indices()
-    cp, ri, v = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+    assert np.array_equal(A.indptr, d["colptr"]) and np.array_equal(A.indices, d["rowidx"])
+    assert np.array_equal(A.data, d["vals"])
+
+
+def test_pq_selection_stages(pq):
+    d, g = pq
+    gold = _gold()["A_pq"]
+    cp, ri, v = d["colptr"], d["rowidx"], d["vals"]
     l1 = O.row_l1(cp, v)
+    assert np.array_equal(l1, g["row_l1"])
     assert hashlib.sha256(l1.tobytes()).hexdigest()[:16] == gold["row_l1_digest"]
     gm = O.gmm_select(l1, 1e-3, 20, 10.0)
     for k, val in gold["gmm"].items():
@@ -107,4 +130,7 @@ def test_pq_selection_stages():
     assert _digest(s1) == gold["filtered_digest"]
     s2 = O.expand_set(cp, ri, s1)
     assert _digest(s2) == gold["expanded_digest"]
-    assert [c.size for c in O.decompose_set(cp, ri, s2)] == gold["component_sizes"]
+    comps = O.decompose_set(cp, ri, s2)
+    assert [c.size for c in comps] == gold["component_sizes"]
+    assert np.array_equal(np.concatenate(comps), g["comp_dofs"])
+

@@ -124,3 +124,37 @@ def test_fused_damping_retry_column(gpu):
         assert np.array_equal(X[c], Xs[c])
         assert info["columns"][c]["num_iterations"] == its[c]
         assert info["columns"][c]["stop_reason"] == "converged_rel"
+
+
+# Each fused column against the oracle's single solve of the same column (FP64 banks in
+# the oracle): the same stiff set and damping, so FP64 columns take the oracle's
+# iteration count (within the partition-independent slack of the single-solve parity
+# tests) and every column, FP32 and BF16 bank storage included, reaches the oracle's
+# solution to within the tolerance the FP64 outer PCG guarantees.
+@pytest.mark.parametrize("precision,k", [("fp64", 2), ("fp32", 2), ("bf16", 2), ("fp32", 4), ("bf16", 4)])
+def test_fused_columns_match_oracle(gpu, precision, k):
+    from oracle import oracle as O
+
+    prob = problems.make("elastic_contact", N=10)
+    A = prob.csc()
+    s = Solver.create(_cfg(3, precision=precision))
+    s.factorize(A)
+    B = _rhs(A.n, k, 31)
+    X, info = _fused(s, B, np.zeros_like(B))
+    H = O.Hybrid(O.config(block_size=3, rel_tol=1e-8, gmm_jump_threshold=10.0, min_subdomain_size=4))
+    H.factorize(A.colptr, A.rowidx, A.vals)
+    assert [c.size for c in s.stiff_components()] == [c.size for c in H.stiff_components()]
+    S = A.to_scipy()
+    for c in range(k):
+        xr = H.solve(B[c])
+        oi = H.info()
+        col = info["columns"][c]
+        assert col["stop_reason"] == "converged_rel"
+        assert np.linalg.norm(X[c] - xr) <= 1e-5 * np.linalg.norm(xr), c
+        floor = 50 * np.finfo(np.float64).eps * np.linalg.norm(abs(S) @ np.abs(X[c]))
+        assert np.linalg.norm(S @ X[c] - B[c]) <= max(1.01e-8 * np.linalg.norm(B[c]), floor)
+        if precision == "fp64":
+            assert abs(col["num_iterations"] - oi["num_iterations"]) <= max(2, 0.05 * oi["num_iterations"])
+        else:
+            # reduced-precision bank factors cost iterations, never accuracy
+            assert col["num_iterations"] <= 2.5 * oi["num_iterations"] + 3

@@ -194,6 +194,83 @@ def test_mas_matches_dense_construction(name, N):
     assert r @ z > 0
 
 
+def _block_graph(A, bs):
+    nb = -(-A.n // bs)
+    G = sp.csr_matrix((np.ones(A.nnz), A.rowidx, A.colptr), shape=(A.n, A.n))
+    Rb = sp.csr_matrix((np.ones(A.n), (np.arange(A.n) // bs, np.arange(A.n))), shape=(nb, A.n))
+    B = (Rb @ G @ Rb.T).tocsr()
+    B.sort_indices()
+    return B
+
+
+def _random_forest(seed, n=2500):
+    """SPD M-matrix on random trees plus isolated nodes: parts whose induced graphs fall
+    apart into several components, coarse banks that merge partly, and a hierarchy that
+    stops before the level cap."""
+    rng = np.random.default_rng(seed)
+    rows, cols, base = [], [], 0
+    for m in (2, 5, 40, 300, 1200, n - 1547 - 60):
+        par = np.array([rng.integers(0, v) for v in range(1, m)], dtype=np.int64)
+        rows.append(base + np.arange(1, m))
+        cols.append(base + par)
+        base += m
+    nn = base + 60
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    w = rng.uniform(0.1, 1.0, r.size)
+    W = sp.coo_matrix((np.r_[w, w], (np.r_[r, c], np.r_[c, r])), shape=(nn, nn)).tocsr()
+    p = rng.permutation(nn)
+    W = W[p][:, p]
+    S = (sp.diags(np.asarray(W.sum(axis=1)).ravel() + 1.0) - W).tocsc()
+    S.sort_indices()
+    return problems.CSC(nn, S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.astype(np.float64))
+
+
+# The coarse-space map of the oracle's MAS against an independent construction (scipy
+# connected components per bank of each level, components ranked by smallest member,
+# stop when a level merges nothing, at most four coarse levels:
+# MASPreconditioner.cu:128-213, 376-481), for every block node and level.
+@pytest.mark.parametrize("case", [("laplace_sphere", 10, 1), ("laplace_sphere", 12, 3), ("elastic_contact", 10, 3),
+                                  ("elastic_slivers", 8, 3), ("elastic_logE", 9, 3), ("forest", 0, 1),
+                                  ("forest", 1, 1), ("elastic_contact", 16, 3)])
+def test_mas_coarse_map_matches_components(case):
+    name, N, bs = case
+    A = _random_forest(N) if name == "forest" else problems.make(name, N=N).csc()
+    H = O.Hybrid(O.config(block_size=bs, do_hybrid_precond=False))
+    H.factorize(A.colptr, A.rowidx, A.vals)
+    perm, off = H.partition()
+    mas = O.Mas(A.colptr, A.rowidx, A.vals, bs, perm, off)
+    B = _block_graph(A, bs).tocoo()
+    nb = B.shape[0]
+    Bp = sp.csr_matrix((B.data, (perm[B.row], perm[B.col])), shape=(nb, nb))
+    maps, counts = ind.mas_coarse_maps(Bp, off)
+    assert mas.levels == len(maps) + 1
+    assert list(mas.cco) == counts
+    for lv, m in enumerate(maps):
+        got = np.array([mas.map(v, lv) for v in range(nb)])
+        assert np.array_equal(got, m), lv
+    # parts are the level-0 banks: no coarse node spans two parts, and every level
+    # maps each bank of 32 coarse nodes into one coarse node range
+    part = np.searchsorted(off, np.arange(nb), side="right") - 1
+    for c in range(counts[0]):
+        assert np.unique(part[maps[0] == c]).size == 1
+
+
+def test_mas_coarse_map_catches_a_wrong_rule():
+    # the independent construction tells a rule that ignores bank boundaries from the
+    # right one (a plausible mistake: components of the whole graph at level 1)
+    A = problems.make("laplace_sphere", N=12).csc()
+    H = O.Hybrid(O.config(block_size=1, do_hybrid_precond=False))
+    H.factorize(A.colptr, A.rowidx, A.vals)
+    perm, off = H.partition()
+    B = _block_graph(A, 1).tocoo()
+    nb = B.shape[0]
+    Bp = sp.csr_matrix((B.data, (perm[B.row], perm[B.col])), shape=(nb, nb))
+    maps, counts = ind.mas_coarse_maps(Bp, off)
+    wrong, _ = ind.mas_coarse_maps(Bp, off, bank=10 ** 9)
+    assert len(maps) >= 2
+    assert not np.array_equal(wrong[1], maps[1])
+
+
 # ---------------------------------------------------------------- hybrid preconditioner and PCG
 def _dense_precond(H, n):
     return np.column_stack([H.precond(np.eye(n)[:, j]) for j in range(n)])
