@@ -39,31 +39,49 @@ def load_traffic(workload: str):
         return json.load(f).get("traffic_bytes_per_launch")
 
 
-def cpu_baseline(prob, A, b, gpu_iters: int, sample_iters: int, threads: int):
-    """Oracle (CPU FP64 restatement) factorize + a bounded PCG sample, extrapolated."""
+def host_cores():
+    """(threads this process may run on, logical CPUs of the host)."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return aff, os.cpu_count() or aff
+
+
+def cpu_baseline(prob, A, b, gpu_iters: int, sample_iters: int, threads: int, full: bool):
+    """Oracle (CPU FP64 restatement) factorize + a bounded PCG sample, extrapolated to the
+    GPU's iteration count; with full=True the oracle solves to convergence instead."""
     from oracle import oracle as O
 
-    cfg = O.config(block_size=prob.block_size, rel_tol=1e-8, max_iter=sample_iters, threads=threads,
-                   gmm_jump_threshold=10.0, min_subdomain_size=4)
+    aff, ncpu = host_cores()
+    cfg = O.config(block_size=prob.block_size, rel_tol=1e-8, max_iter=(10000 if full else sample_iters),
+                   threads=threads, gmm_jump_threshold=10.0, min_subdomain_size=4)
     H = O.Hybrid(cfg)
     t0 = time.perf_counter()
     H.factorize(A.colptr, A.rowidx, A.vals)
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
     H.solve(b)
-    t_iter = (time.perf_counter() - t0) / max(1, H.info()["num_iterations"])
-    value = t_setup + t_iter * gpu_iters
+    t_loop = time.perf_counter() - t0
+    its = H.info()["num_iterations"]
+    t_iter = t_loop / max(1, its)
+    value = t_setup + (t_loop if full else t_iter * gpu_iters)
     # the same PCG iteration on one thread (a few iterations, same factorization)
     H.set_run(1, 3)
     t0 = time.perf_counter()
     H.solve(b)
     t_iter1 = (time.perf_counter() - t0) / max(1, H.info()["num_iterations"])
-    return {"value": value, "unit": "s", "cores": threads, "kind": "port",
-            "pcg_iteration_s": {str(threads): round(t_iter, 4), "1": round(t_iter1, 4)},
-            "sample": f"oracle factorize ({t_setup:.2f} s) + {sample_iters} PCG iterations "
-                      f"({t_iter * 1e3:.1f} ms each) on {threads} threads on the same system, "
-                      f"extrapolated to the {gpu_iters} iterations of the FP64 GPU path; "
-                      f"3 iterations on 1 thread: {t_iter1 * 1e3:.0f} ms each"}
+    if full:
+        sample = (f"oracle factorize ({t_setup:.2f} s) + PCG to convergence ({its} iterations, "
+                  f"{t_iter * 1e3:.1f} ms each) on {threads} threads, measured")
+    else:
+        sample = (f"oracle factorize ({t_setup:.2f} s) + {its} PCG iterations ({t_iter * 1e3:.1f} ms each) "
+                  f"on {threads} threads on the same system, extrapolated to the {gpu_iters} iterations of "
+                  f"the FP64 GPU path; 3 iterations on 1 thread: {t_iter1 * 1e3:.0f} ms each")
+    return {"value": round(value, 3), "unit": "s", "cores": threads, "kind": "port",
+            "host_threads_available": aff, "host_logical_cpus": ncpu,
+            "factorize_s": round(t_setup, 3), "iterations_run": its, "measured_to_convergence": full,
+            "pcg_iteration_s": {str(threads): round(t_iter, 4), "1": round(t_iter1, 4)}, "sample": sample}
 
 
 def main():
@@ -74,7 +92,10 @@ def main():
     ap.add_argument("--workload", default="elastic_contact")
     ap.add_argument("--N", type=int, default=None, help="grid nodes per axis (default: the workload's)")
     ap.add_argument("--cpu-sample-iters", type=int, default=10)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: all this process may use)")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="run the CPU baseline to convergence instead of a bounded sample")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint timing")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-iter", type=int, default=20000,
                     help="PCG iteration cap (profiling runs cut it; such a line is marked invalid)")
@@ -99,7 +120,8 @@ def main():
     import torch
 
     from lss_amd import problems
-    from lss_amd.solver import Solver
+    from lss_amd.autograd import pattern_grad
+    from lss_amd.solver import Solver, residual_norm
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
@@ -134,6 +156,11 @@ def main():
         base[k] = json.loads(v)
     paths = {"fp64": dict(base, precision="fp64"), "mixed": dict(base, precision=args.mixed_precision)}
     res = {}
+    cp_d, ri_d, va_d = d_ptr, d_idx, d_val
+    # forward+adjoint: L = 0.5 ||x - x_t||^2, dL/dx = x - x_t, A^T lam = dL/dx (A symmetric:
+    # the same factorization), dL/dA = -lam x^T on the stored pattern
+    x_t = torch.from_numpy(problems.random_vector(n, 1)).to(dev)
+    lam = torch.zeros_like(x)
     for name, g in paths.items():
         s = Solver.create({"solver": "GPUHybrid", "GPUHybrid": g})
 
@@ -142,21 +169,69 @@ def main():
             x.zero_()
             s.solve_device(d_b, x, stream=stream)
 
+        def adj_step():
+            step()
+            gx = x - x_t
+            lam.zero_()
+            s.solve_device(gx, lam, stream=stream)
+            return pattern_grad(cp_d, ri_d, lam, x)
+
         for _ in range(args.warmup):
             step()
             log(name, "warm-up step done")
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
+        times = []
         for _ in range(args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
             step()
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / args.steps
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
         info = s.get_info()
         rel = info["final_res_norm"] / bnorm
-        res[name] = {"s": dt, "iters": info["num_iterations"], "rel_res": rel, "info": info,
-                     "x": x.cpu().numpy().copy(), "stop": info["stop_reason"]}
-        log(name, f"{dt:.3f} s", info["num_iterations"], "iterations", info["stop_reason"])
+        r_dd, _ = residual_norm(cp_d, ri_d, va_d, d_b, x)
+        res[name] = {"s": float(np.median(times)), "all": times, "iters": info["num_iterations"], "rel_res": rel,
+                     "rel_res_dd": r_dd / bnorm, "info": info, "x": x.cpu().numpy().copy(),
+                     "stop": info["stop_reason"]}
+        log(name, f"{res[name]['s']:.3f} s (median of {args.steps})", info["num_iterations"], "iterations",
+            info["stop_reason"], f"true rel residual (double-double) {r_dd / bnorm:.3e}")
+        if not args.no_adjoint:
+            for _ in range(args.warmup):
+                adj_step()
+            at = []
+            for _ in range(args.steps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                grad = adj_step()
+                torch.cuda.synchronize()
+                at.append(time.perf_counter() - t0)
+            ainfo = s.get_info()
+            res[name]["adjoint"] = {"s": float(np.median(at)), "all": at, "adjoint_iters": ainfo["num_iterations"],
+                                    "adjoint_stop": ainfo["stop_reason"],
+                                    "grad_norm": float(torch.linalg.norm(grad))}
+            log(name, f"forward+adjoint {res[name]['adjoint']['s']:.3f} s")
         del s
+    # accuracy: the FP64 GPU path solved far past the benchmark tolerance is the
+    # reference solution (a sparse-direct FP64 solve of this size is out of reach of the
+    # host's direct solvers); for systems of up to 40 000 DOF, scipy's sparse LU
+    ref_kind = "FP64 GPU path, recurrence residual 1e-13"
+    s = Solver.create({"solver": "GPUHybrid", "GPUHybrid": dict(paths["fp64"], relative_tolerance=1e-13)})
+    s.factorize_device(n, d_ptr, d_idx, d_val, stream=stream)
+    x.zero_()
+    s.solve_device(d_b, x, stream=stream)
+    ref_info = s.get_info()
+    x_ref = x.cpu().numpy().copy()
+    r_ref_dd, _ = residual_norm(cp_d, ri_d, va_d, d_b, x)
+    del s
+    if n <= 40_000:
+        import scipy.sparse.linalg as spla
+
+        x_ref = spla.spsolve(A.to_scipy().tocsc(), b)
+        ref_kind = "scipy sparse LU (SuperLU), FP64"
+        r_ref_dd, _ = residual_norm(cp_d, ri_d, va_d, d_b, torch.from_numpy(x_ref).to(dev))
+    accuracy = {"reference": ref_kind, "reference_rel_residual_dd": r_ref_dd / bnorm,
+                "reference_iterations": ref_info["num_iterations"]}
+    for name in ("mixed", "fp64"):
+        accuracy[f"x_{name}_vs_ref"] = float(np.linalg.norm(res[name]["x"] - x_ref) / np.linalg.norm(x_ref))
     multi = None
     if args.multi_rhs > 1:
         # solve phase only: one factorization, K right-hand sides fused vs one by one
@@ -237,8 +312,9 @@ def main():
     workload = f"{args.workload}_{prob.N}"
     traffic = load_traffic(workload)
     mixed, fp64 = res["mixed"], res["fp64"]
-    # the reference stops on the recurrence residual (GPUHybrid.cu:1484-1496); the true
-    # FP64 residual of this system cannot be measured below eps * || |A| |x| ||
+    # the reference stops on the recurrence residual (GPUHybrid.cu:1484-1496), as both
+    # paths here do; beside it the true residual of each solution in double-double
+    # arithmetic and eps || |A| |x| ||, below which no vector of doubles gets on stiff rows
     valid = mixed["stop"] == "converged_rel" and fp64["stop"] == "converged_rel"
     S = A.to_scipy().tocsr()
     absx = np.abs(fp64["x"])
@@ -246,8 +322,9 @@ def main():
     x_diff = float(np.linalg.norm(mixed["x"] - fp64["x"]) / np.linalg.norm(fp64["x"]))
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline sample")
-        cpu = cpu_baseline(prob, A, b, fp64["iters"], args.cpu_sample_iters, args.cpu_threads)
+        log("cpu baseline", "to convergence" if args.cpu_full else "sample")
+        threads = args.cpu_threads or host_cores()[0]
+        cpu = cpu_baseline(prob, A, b, fp64["iters"], args.cpu_sample_iters, threads, args.cpu_full)
     line = {
         "metric": "time-to-1e-8 residual (s), mixed vs FP64 GPU, 6.3M-DOF; SpMV HBM GB/s",
         "value": round(mixed["s"], 4),
@@ -267,10 +344,13 @@ def main():
         "speedup_mixed_vs_fp64": round(fp64["s"] / mixed["s"], 3),
         "iterations": {"mixed": mixed["iters"], "fp64": fp64["iters"]},
         "rel_residual": {"mixed": mixed["rel_res"], "fp64": fp64["rel_res"]},
+        "true_rel_residual_dd": {"mixed": mixed["rel_res_dd"], "fp64": fp64["rel_res_dd"]},
         "stop_reason": {"mixed": mixed["stop"], "fp64": fp64["stop"]},
         "rel_residual_fp64_floor": floor,
         "x_mixed_vs_fp64": x_diff,
+        "accuracy": accuracy,
         "valid": valid,
+        "step_s": {"mixed": [round(t, 4) for t in mixed["all"]], "fp64": [round(t, 4) for t in fp64["all"]]},
         "phases_mixed_s": mixed["info"]["timings"],
         "stiff": {"n_stiff_dofs": mixed["info"]["stiff"]["n_stiff_dofs"],
                   "n_components": mixed["info"]["stiff"]["n_components"],
@@ -284,6 +364,13 @@ def main():
         "kernel_ms": pinfo["kernel_ms"],
         "cpu_baseline": cpu,
     }
+    if "adjoint" in mixed:
+        ma, fa = mixed["adjoint"], fp64["adjoint"]
+        line["forward_adjoint"] = {
+            "mixed_s": round(ma["s"], 4), "fp64_s": round(fa["s"], 4), "speedup_mixed_vs_fp64": round(fa["s"] / ma["s"], 3),
+            "adjoint_iterations": {"mixed": ma["adjoint_iters"], "fp64": fa["adjoint_iters"]},
+            "adjoint_stop_reason": {"mixed": ma["adjoint_stop"], "fp64": fa["adjoint_stop"]},
+            "what": "factorize + forward solve + adjoint solve A^T lam = x - x_t + dL/dA on the pattern (median)"}
     if multi is not None:
         line["multi_rhs"] = multi
     if mas_runs is not None:

@@ -418,3 +418,35 @@ def test_block_size_and_tail_padding(gpu, name, N, bs, m, mode):
                                                                     0.05 * oi["num_iterations"])
     assert np.linalg.norm(x - xr) <= (1e-5 if mode == "GPUHybrid" else 1e-3) * np.linalg.norm(xr)
     assert np.max(np.abs(s.spmv(b) - O.spmv(A.colptr, A.rowidx, A.vals, b))) <= 1e-13 * np.max(np.abs(A.vals))
+
+
+# ---------------------------------------------------------------- the reference's second example system
+def _pq_fixture():
+    import os
+
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    d = np.load(os.path.join(gold, "A_pq.npz"))
+    g = np.load(os.path.join(gold, "A_pq_golden.npz"))
+    A = problems.CSC(int(d["n"]), d["colptr"], d["rowidx"], d["vals"])
+    return A, d["rhs"], g
+
+
+def test_pq_fixture_selection_and_solve(gpu):
+    # mats/A_pq.mtx (130 569 DOF, block size 1, identity Dirichlet rows): 30 stiff
+    # components of 37 to 6595 DOFs (the large ones through the blocked dense factor)
+    A, b, g = _pq_fixture()
+    s = Solver.create(_cfg(1))
+    s.factorize(A)
+    assert np.array_equal(s.row_norms(), g["row_l1"])
+    comps = s.stiff_components()
+    assert len(comps) == 30
+    assert np.array_equal(np.cumsum([0] + [c.size for c in comps]), g["comp_off"])
+    assert np.array_equal(np.concatenate(comps), g["comp_dofs"])
+    x = s.solve(b)
+    info = s.get_info()
+    assert info["stop_reason"] == "converged_rel"
+    R = A.to_scipy().T
+    floor = 50 * np.finfo(np.float64).eps * np.linalg.norm(abs(R) @ np.abs(x))
+    assert np.linalg.norm(R @ x - b) <= max(1.01e-8 * np.linalg.norm(b), floor)
+    xd = g["x_direct"]
+    assert np.linalg.norm(x - xd) <= 1e-5 * np.linalg.norm(xd)
