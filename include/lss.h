@@ -112,7 +112,7 @@ int lss_get_stiff_dofs(const lss_solver *s, int32_t *dofs, int64_t cap, int64_t 
 
 /* Last error message of this handle (empty string when none).  With s == NULL:
  * the message of the last failed lss_create or handle-free call (lss_adjoint_*,
- * lss_residual_norm) on this thread. */
+ * lss_residual_norm*) on this thread. */
 const char *lss_last_error(const lss_solver *s);
 
 /* Release all device memory.  Replaces GPUHybrid::~GPUHybrid (GPUHybrid.cu:1514-1530). */
@@ -150,6 +150,18 @@ int lss_adjoint_chain(int64_t nseg, const int64_t *d_seg_ptr, const int32_t *d_s
  * partial sums (32 KiB at most) in stream order. */
 int lss_residual_norm(int32_t n, int64_t nnz, const int32_t *d_colptr, const int32_t *d_rowidx, const double *d_vals,
                       const double *d_b, const double *d_x, double *d_out, void *stream);
+
+/* lss_residual_norm without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The partial sums live in
+ * a static device array of the library (64 KiB; each GPU the library runs on holds
+ * its own copy until the process ends); every n fits it (the grid is capped at 2048
+ * workgroups), and each call overwrites what it reads, so nothing needs clearing.
+ * All calls on one GPU share that array: calls that overlap on one GPU (other
+ * streams, other host threads) race and must be serialized by the caller.  Same
+ * arguments, results and status codes as lss_residual_norm. */
+int lss_residual_norm_noalloc(int32_t n, int64_t nnz, const int32_t *d_colptr, const int32_t *d_rowidx,
+                              const double *d_vals, const double *d_b, const double *d_x, double *d_out, void *stream);
 
 /* ---- inspection entry points (parity tests and tools) --------------------
  * Each runs the named device stage of the factorized solver on host data.  No

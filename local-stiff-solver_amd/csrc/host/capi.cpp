@@ -242,6 +242,22 @@ int lss_residual_norm(int32_t n, int64_t nnz, const int32_t *d_colptr, const int
     });
 }
 
+int lss_residual_norm_noalloc(int32_t n, int64_t nnz, const int32_t *d_colptr, const int32_t *d_rowidx,
+                              const double *d_vals, const double *d_b, const double *d_x, double *d_out, void *stream)
+{
+    return guarded_free([&] {
+        if (n < 0 || nnz < 0 || !d_out || (n > 0 && (!d_colptr || !d_b || !d_x)) ||
+            (nnz > 0 && (!d_rowidx || !d_vals)))
+            lss::fail(LSS_ERR_ARG, "invalid arguments");
+        // the grid is capped, so every size fits the static partials array; the check
+        // guards the invariant should the cap and the array ever disagree
+        if (lss::k::residual_groups(n) > lss::k::kResidualGroups)
+            lss::fail(LSS_ERR_SIZE, "residual needs more partial sums than the static array holds");
+        lss::k::residual_norm_dd(n, d_colptr, d_rowidx, d_vals, d_b, d_x, lss::k::residual_scratch(), d_out,
+                                 static_cast<hipStream_t>(stream));
+    });
+}
+
 void lss_destroy(lss_solver *s) { delete s; }
 
 int lss_debug_row_norms(const lss_solver *s, double *out, int64_t n)

@@ -158,6 +158,9 @@ constexpr int kResidualGroups = 2048;
 int residual_groups(int32_t n);
 void residual_norm_dd(int32_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals, const double *b,
                       const double *x, double *partials, double *out, hipStream_t s);
+// Device address of the current GPU's static partials array (4 * kResidualGroups doubles),
+// looked up once per device; nothing is allocated.
+double *residual_scratch();
 
 // ---------------------------------------------------------------- adjoint
 // grad[e] = -lam[j] * x[rowidx[e]] for every entry e of CSC column j (operator entry (j, rowidx[e])).

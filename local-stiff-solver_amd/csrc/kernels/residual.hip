@@ -14,6 +14,7 @@
 // stores four doubles (r.r and b.b as hi/lo pairs) into a partials buffer that a
 // one-workgroup finisher sums in index order, so the result is deterministic.
 #include <algorithm>
+#include <mutex>
 
 #include "kernels.hpp"
 #include "../common/runtime.hpp"
@@ -161,7 +162,31 @@ __global__ __launch_bounds__(kResThreads) void k_residual_finish(int ngroups, co
     }
 }
 
+// Partial sums of the allocation-free entry point: 4 doubles per workgroup of the
+// capped grid (64 KiB).  Each GPU the library runs on holds its own copy until the
+// process ends; host code reaches it only through the address hipGetSymbolAddress
+// returns for the current device (the array's name is a host-side stand-in).
+static __device__ __align__(256) double scratch[4 * kResidualGroups];
+
 }  // namespace
+
+double *residual_scratch()
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double *addr[kMaxDevices] = {};
+    int dev = 0;
+    LSS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) fail(LSS_ERR_UNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        LSS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) fail(LSS_ERR_HIP, "no device address for the residual scratch array");
+        addr[dev] = static_cast<double *>(p);
+    }
+    return addr[dev];
+}
 
 int residual_groups(int32_t n)
 {
