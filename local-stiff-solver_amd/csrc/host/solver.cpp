@@ -528,8 +528,9 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     sync(s);  // bsr_permute has read c0 and v0 before the sliced arrays are resized
     // sell_fill writes every slot (padding: column -1, zero values)
     d_scols_.resize(std::max<int64_t>(sell_slots_, 1));
-    d_svals_.resize(std::max<int64_t>(sell_slots_ * bb, 1));
-    if (cfg_.fp32_inner()) d_svals32_.resize(std::max<int64_t>(sell_slots_ * bb, 1));
+    const int64_t nsv = k::sell_value_count(sell_slots_, bs_);
+    d_svals_.resize(std::max<int64_t>(nsv, 1));
+    if (cfg_.fp32_inner()) d_svals32_.resize(std::max<int64_t>(nsv, 1));
     k::sell_fill(nb_, bs_, d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), d_slice_ptr_.get(), d_scols_.get(),
                  d_svals_.get(), cfg_.fp32_inner() ? d_svals32_.get() : nullptr, s);
     sell_.nb = nb_;

@@ -53,6 +53,8 @@ void spmv_residual(const SellView &A, const double *x, const double *b, double *
 void to_f32(int64_t n, const double *x, float *y, hipStream_t s);
 // Partials one column of spmv_pq writes.
 int spmv_partials_per_column(const SellView &A);
+// Values of the sliced layout for `slots` block slots (counted per block row).
+int64_t sell_value_count(int64_t slots, int bs);
 
 // ---------------------------------------------------------------- BLAS-1 / PCG
 constexpr int kCsumChunks = 1024;  // canonical reduction tree (shared with the oracle)

@@ -1,13 +1,14 @@
 // Block sparse formats and SpMV for gfx950.
 //
 // Matrix layout in HBM (sliced block-ELL, "SELL-H/bs"): block rows are grouped
-// into slices of H = 64/bs rows, one wave64 per slice.  Lane l of the wave owns
-// scalar row r = l % bs of block row h = l / bs, so a wave covers 64 (bs = 1),
-// 64 (bs = 2) or 63 (bs = 3) scalar rows and every lane accumulates its own row:
-// no cross-lane reduction, no atomics.  Slot j of slice s holds block j of each
-// of the H rows; its H*bs*bs values are stored lane-major: value c of lane l
-// (column c of scalar row l) sits at c*H*bs + l, so each of the bs value loads
-// of a wave reads one contiguous span of 8*H*bs bytes (504 B for bs = 3).
+// into slices of H = 64/bs rows, one wave64 per slice.  Lane l of the
+// wave owns column r = l % bs of the blocks of block row h = l / bs, so a wave
+// covers 64 (bs = 1), 64 (bs = 2) or 63 (bs = 3) scalar columns.  Slot j of slice
+// s holds block j of each of the H rows; its H*bs*bs values are stored
+// lane-major: entry (c, r) of the block of lane l = h*bs + r sits at c*H*bs + l,
+// so each of the bs value loads of a wave reads one contiguous span of 8*H*bs
+// bytes (504 B for bs = 3), and a lane gathers one x entry per slot.  The bs
+// lanes of a block row add their partial rows once per slice (lane shuffles).
 // Rows shorter than the slice width are padded with column -1 and zero values.
 //
 // Reference counterparts: BSR layout MAS/BSRMatrix.hpp:10-57 (row-major blocks,
@@ -308,9 +309,11 @@ __global__ void k_sell_width(int32_t nb, int H, const int32_t *__restrict__ rowp
     width[s] = w;
 }
 
-// One wave per slice, lane l = h * bs + rr (scalar row rr of the slice's block
-// row h): every slot of the slice is written, padding included (column -1,
-// zero values), and a slot's values are written by consecutive lanes.
+// One wave per slice, lane l = h * bs + rr (column rr of the blocks of the slice's
+// block row h): every slot of the slice is written, padding included (column -1,
+// zero values), and a slot's values are written by consecutive lanes.  Position
+// (c, l) of a slot holds entry (c, rr) of the block, so a lane holds one block
+// column and the SpMV gathers one x entry per lane and slot.
 __global__ __launch_bounds__(256) void k_sell_fill(int32_t nb, int bs, int H, int32_t nslices,
                                                     const int32_t *__restrict__ rowptr,
                                                     const int32_t *__restrict__ cols, const double *__restrict__ vals,
@@ -331,7 +334,7 @@ __global__ __launch_bounds__(256) void k_sell_fill(int32_t nb, int bs, int H, in
         if (rr == 0) scols[base + int64_t(j) * H + h] = have ? cols[b + j] : -1;
         const int64_t vbase = (base + int64_t(j) * H) * bb;
         for (int c = 0; c < bs; ++c) {
-            const double v = have ? vals[int64_t(b + j) * bb + rr * bs + c] : 0.0;
+            const double v = have ? vals[int64_t(b + j) * bb + c * bs + rr] : 0.0;
             const int64_t at = vbase + int64_t(c) * (H * bs) + lane;
             svals[at] = v;
             if (svals32) svals32[at] = static_cast<float>(v);
@@ -343,8 +346,14 @@ __global__ __launch_bounds__(256) void k_sell_fill(int32_t nb, int bs, int H, in
 // DOT: per-lane partial of x_i * y_i (p . A p) reduced deterministically.
 // K right-hand sides (column c of a vector at v + c * ld) share every column
 // index and matrix value load; each column accumulates in the order of K = 1.
+//
+// Lane (h, r) owns column r of every block of block row h: per slot it gathers the
+// single entry x[col * BS + r] and accumulates A(q, r) x_r into BS partial rows q.
+// The BS lanes of a block row add their partial rows once, after the last slot (a
+// fixed rotation of lane shuffles), so lane (h, q) ends with row q of y.  One x
+// load per lane and slot (instead of BS), and fewer registers for the gathered x.
 template <int BS, int MODE, bool DOT, bool F32, int K>
-__global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslices, const int64_t *__restrict__ slice_ptr,
+__global__ __launch_bounds__(kSpmvThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_spmv(int32_t nb, int32_t nslices, const int64_t *__restrict__ slice_ptr,
                                                         const int32_t *__restrict__ cols, const double *__restrict__ vals,
                                                         const float *__restrict__ vals32, const double *__restrict__ x,
                                                         const float *__restrict__ x32,
@@ -354,91 +363,124 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslic
 {
     constexpr int H = 64 / BS;
     constexpr int BB = BS * BS;
+    // 5 slots per chunk for one right-hand side: three round trips for the usual
+    // width of 15 (14 neighbours and the diagonal block of a tetrahedral mesh node)
+    constexpr int C = K >= 4 ? 2 : (K == 2 ? 4 : 5);
     // FP32 values go with the FP32 copy of x (x32): the preconditioner-internal
     // residual, never the dot-product variant
     static_assert(!(F32 && DOT), "the FP32 SpMV has no dot product");
     if (dev::all_stopped(stop, K)) return;
     const int lane = threadIdx.x & 63;
-    const int32_t slice = blockIdx.x * (kSpmvThreads / 64) + (threadIdx.x >> 6);
+    // wave-uniform: the slice pointer is a scalar load and the chunk loop has scalar control
+    const int32_t slice = __builtin_amdgcn_readfirstlane(int(blockIdx.x * (kSpmvThreads / 64) + (threadIdx.x >> 6)));
     const int h = lane / BS, r = lane - h * BS;
     const int32_t row = slice * H + h;
     const bool active = (h < H) && (slice < nslices) && (row < nb);
-    double acc[K];
+    // lanes past the last block row (lane 63 for BS = 3) shadow row H-1: their loads
+    // and shuffles stay in the slice, their results are discarded
+    const int hh = h < H ? h : H - 1;
+    double acc[K][BS];
 #pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int q = 0; q < BS; ++q) acc[k][q] = 0.0;
+    // DOT: x_i of the lane's own row is the x entry it gathered for the diagonal block
+    // (column == row), so p . A p needs no second read of p (up to two columns: four
+    // would need registers beyond six waves per SIMD)
+    constexpr bool DIAGX = DOT && K <= 2;
+    double xd[K];
+    bool found = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) xd[k] = 0.0;
     if (slice < nslices) {
         const int64_t base = slice_ptr[slice];
         const int32_t W = int32_t((slice_ptr[slice + 1] - base) / H);
         // Every load is unconditional: padding slots hold column -1 and zero
-        // values, lanes past the last row read the slot of row H-1 (results
-        // discarded), so the C-slot chunk issues all its column, value and x
-        // loads back to back.  The matrix is streamed with non-temporal loads.
-        const int hh = h < H ? h : H - 1;
+        // values, so a C-slot chunk issues all its column, value and x loads back
+        // to back.  The last chunk of a slice is a full chunk too: its slots past
+        // the slice width W re-read slot W-1 (valid addresses) and are masked to
+        // zero, so a slice costs ceil(W / C) memory round trips (a slot-by-slot
+        // tail would wait on the column index and then on x for every slot).
+        // Values use ordinary loads (measured faster than non-temporal ones).
         const int ll = hh * BS + r;  // lane within the slot's value layout
-        constexpr int C = K >= 4 ? 2 : 4;
-        int32_t j0 = 0;
         // the column indices of a chunk are loaded during the previous chunk, so a chunk
         // waits on one memory latency (values and x together), not on cols then x
         int32_t cn[C];
-        if (C <= W) {
+        if (W > 0) {
 #pragma unroll
-            for (int u = 0; u < C; ++u) cn[u] = __builtin_nontemporal_load(&cols[base + int64_t(u) * H + hh]);
+            for (int u = 0; u < C; ++u)
+                cn[u] = __builtin_nontemporal_load(&cols[base + int64_t(min(u, W - 1)) * H + hh]);
         }
-        for (; j0 + C <= W; j0 += C) {
-            double a[C][BS], xv[C][K][BS];
+        for (int32_t j0 = 0; j0 < W; j0 += C) {
+            double a[C][BS], xv[C][K];
 #pragma unroll
             for (int u = 0; u < C; ++u) {
-                const int64_t vb = (base + int64_t(j0 + u) * H) * BB + ll;
+                const int64_t vb = (base + int64_t(min(j0 + u, W - 1)) * H) * BB + ll;
 #pragma unroll
                 for (int q = 0; q < BS; ++q) {
                     if constexpr (F32)
-                        a[u][q] = double(__builtin_nontemporal_load(&vals32[vb + q * (H * BS)]));
+                        a[u][q] = double(vals32[vb + q * (H * BS)]);
                     else
-                        a[u][q] = __builtin_nontemporal_load(&vals[vb + q * (H * BS)]);
+                        a[u][q] = vals[vb + q * (H * BS)];
                 }
             }
+            bool diag[C];
 #pragma unroll
             for (int u = 0; u < C; ++u) {
-                const int64_t xc = int64_t(max(cn[u], 0)) * BS;  // padding: column -1, zero values
-#pragma unroll
-                for (int q = 0; q < BS; ++q)
-#pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        if constexpr (F32)
-                            xv[u][k][q] = double(x32[k * ld + xc + q]);
-                        else
-                            xv[u][k][q] = x[k * ld + xc + q];
-                    }
-            }
-            if (j0 + 2 * C <= W) {
-#pragma unroll
-                for (int u = 0; u < C; ++u)
-                    cn[u] = __builtin_nontemporal_load(&cols[base + int64_t(j0 + C + u) * H + hh]);
-            }
-#pragma unroll
-            for (int u = 0; u < C; ++u)
-#pragma unroll
-                for (int q = 0; q < BS; ++q)
-#pragma unroll
-                    for (int k = 0; k < K; ++k) acc[k] += a[u][q] * xv[u][k][q];
-        }
-        for (; j0 < W; ++j0) {
-            const int64_t slot = base + int64_t(j0) * H + hh;
-            const int32_t c0 = __builtin_nontemporal_load(&cols[slot]);
-            const int64_t xc = int64_t(max(c0, 0)) * BS;
-            const int64_t vb = (base + int64_t(j0) * H) * BB + ll;
-#pragma unroll
-            for (int q = 0; q < BS; ++q) {
-                const double a = F32 ? double(__builtin_nontemporal_load(&vals32[vb + q * (H * BS)]))
-                                     : __builtin_nontemporal_load(&vals[vb + q * (H * BS)]);
+                const int64_t xc = int64_t(max(cn[u], 0)) * BS + r;  // padding: column -1, zero values
+                diag[u] = DIAGX && cn[u] == row && j0 + u < W;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     if constexpr (F32)
-                        acc[k] += a * double(x32[k * ld + xc + q]);
+                        xv[u][k] = double(x32[k * ld + xc]);
                     else
-                        acc[k] += a * x[k * ld + xc + q];
+                        xv[u][k] = x[k * ld + xc];
                 }
             }
+            if (j0 + C < W) {
+#pragma unroll
+                for (int u = 0; u < C; ++u)
+                    cn[u] = __builtin_nontemporal_load(&cols[base + int64_t(min(j0 + C + u, W - 1)) * H + hh]);
+            }
+#pragma unroll
+            for (int u = 0; u < C; ++u) {
+                if (j0 + u < W) {  // uniform across the wave
+#pragma unroll
+                    for (int q = 0; q < BS; ++q)
+#pragma unroll
+                        for (int k = 0; k < K; ++k) acc[k][q] += a[u][q] * xv[u][k];
+                }
+                if constexpr (DIAGX) {
+                    if (diag[u]) {
+                        found = true;
+#pragma unroll
+                        for (int k = 0; k < K; ++k) xd[k] = xv[u][k];
+                    }
+                }
+            }
+        }
+    }
+    // row r of the block row: sum over the BS lanes (h, r') of their partial row r, in the
+    // order r' = r, r + 1, ... (mod BS); in round d lane (h, r') hands out its partial row
+    // (r' - d) mod BS to lane (h, (r' - d) mod BS)
+    double rowsum[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if constexpr (BS == 1) {
+            rowsum[k] = acc[k][0];
+        } else {
+            double t = acc[k][0];
+#pragma unroll
+            for (int q = 1; q < BS; ++q) t = (r == q) ? acc[k][q] : t;
+#pragma unroll
+            for (int d = 1; d < BS; ++d) {
+                const int give = (r - d + BS) % BS;
+                double v = acc[k][0];
+#pragma unroll
+                for (int q = 1; q < BS; ++q) v = (give == q) ? acc[k][q] : v;
+                t += __shfl(v, hh * BS + (r + d) % BS, 64);
+            }
+            rowsum[k] = t;
         }
     }
     const int64_t i = int64_t(row) * BS + r;
@@ -447,25 +489,29 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int32_t nb, int32_t nslic
     for (int k = 0; k < K; ++k) {
         out[k] = 0.0;
         if (active) {
-            out[k] = (MODE == 1) ? sub[k * ld + i] - acc[k] : acc[k];
+            out[k] = (MODE == 1) ? sub[k * ld + i] - rowsum[k] : rowsum[k];
             y[k * ld + i] = out[k];
         }
     }
     if constexpr (DOT) {
+        if (active && !found) {  // a row without a stored diagonal block
+#pragma unroll
+            for (int k = 0; k < K; ++k) xd[k] = x[k * ld + i];
+        }
         __shared__ double sh[kSpmvThreads / 64];
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            dev::store_partial(dev::block_sum<kSpmvThreads>(active ? x[k * ld + i] * out[k] : 0.0, sh),
+            dev::store_partial(dev::block_sum<kSpmvThreads>(active ? xd[k] * out[k] : 0.0, sh),
                                partials + int64_t(k) * gridDim.x);
     }
 }
 
 template <int MODE, bool DOT, bool F32, int K>
-void launch_spmv_k(const SellView &A, const double *x, const float *x32, double *y, const double *sub,
-                   const int *stop, int64_t ld, double *partials, hipStream_t s)
+int launch_spmv_k(const SellView &A, const double *x, const float *x32, double *y, const double *sub,
+                  const int *stop, int64_t ld, double *partials, hipStream_t s)
 {
     const int grid = div_up(A.nslices, kSpmvThreads / 64);
-    if (grid == 0) return;
+    if (grid == 0) return 0;
 #define LSS_SPMV_CASE(BS)                                                                                      \
     k_spmv<BS, MODE, DOT, F32, K><<<grid, kSpmvThreads, 0, s>>>(A.nb, A.nslices, A.slice_ptr, A.cols, A.vals,  \
                                                                 A.vals32, x, x32, y, sub, stop, ld, partials)
@@ -477,16 +523,17 @@ void launch_spmv_k(const SellView &A, const double *x, const float *x32, double 
     }
 #undef LSS_SPMV_CASE
     LSS_LAUNCH_CHECK();
+    return grid;
 }
 
 template <int MODE, bool DOT, bool F32>
-void launch_spmv(const SellView &A, const double *x, const float *x32, double *y, const double *sub, const int *stop,
-                 int k, int64_t ld, double *partials, hipStream_t s)
+int launch_spmv(const SellView &A, const double *x, const float *x32, double *y, const double *sub, const int *stop,
+                int k, int64_t ld, double *partials, hipStream_t s)
 {
     switch (k) {
-        case 1: launch_spmv_k<MODE, DOT, F32, 1>(A, x, x32, y, sub, stop, ld, partials, s); break;
-        case 2: launch_spmv_k<MODE, DOT, F32, 2>(A, x, x32, y, sub, stop, ld, partials, s); break;
-        case 4: launch_spmv_k<MODE, DOT, F32, 4>(A, x, x32, y, sub, stop, ld, partials, s); break;
+        case 1: return launch_spmv_k<MODE, DOT, F32, 1>(A, x, x32, y, sub, stop, ld, partials, s);
+        case 2: return launch_spmv_k<MODE, DOT, F32, 2>(A, x, x32, y, sub, stop, ld, partials, s);
+        case 4: return launch_spmv_k<MODE, DOT, F32, 4>(A, x, x32, y, sub, stop, ld, partials, s);
         default: fail(LSS_ERR_ARG, "a fused SpMV takes 1, 2 or 4 right-hand sides");
     }
 }
@@ -562,11 +609,13 @@ void spmv(const SellView &A, const double *x, double *y, const double *sub, hipS
 
 int spmv_partials_per_column(const SellView &A) { return div_up(A.nslices, kSpmvThreads / 64); }
 
+int64_t sell_value_count(int64_t slots, int bs) { return slots * bs * bs; }
+
 void spmv_pq(const SellView &A, const double *p, double *q, dev::PcgState *st, double *partials, int iter,
              hipStream_t s, int k, int64_t ld, int finish_op)
 {
-    launch_spmv<0, true, false>(A, p, nullptr, q, nullptr, &st->stop, k, ld, partials, s);
-    finish(spmv_partials_per_column(A), partials, finish_op, st, nullptr, iter, s, k);
+    const int g = launch_spmv<0, true, false>(A, p, nullptr, q, nullptr, &st->stop, k, ld, partials, s);
+    if (g > 0) finish(g, partials, finish_op, st, nullptr, iter, s, k);
 }
 
 void spmv_residual(const SellView &A, const double *x, const double *b, double *r, const int *stop, hipStream_t s,
