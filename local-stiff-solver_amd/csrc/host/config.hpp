@@ -27,7 +27,7 @@ struct Config {
     int64_t min_subdomain_size = 1;
     int64_t max_subdomain_size = 10000000;
     bool expand_subdomains = true;
-    int64_t max_dense_size = 0;           // accepted; every component uses the dense device Cholesky
+    int64_t max_dense_size = 0;           // dense branch: components of at most this many DOFs go to bins
     bool use_float_on_subdomains = false;  // accepted; the stiff factor stays FP64
     // keys of the "MAS" solver section (MASSolver.cu:148-172, 226-279); with solver
     // "MAS", max_iter / relative_tolerance / absolute_tolerance / block_size (its

@@ -120,6 +120,9 @@ void HybridSolver::reset_factorization()
     ncomp_ = 0;
     ns_ = 0;
     omega_ = 1.0;
+    d_unit_dense_.release();
+    dense_bins_ = dense_dim_ = dense_dofs_ = 0;
+    bin_sizes_.clear();
 }
 
 void HybridSolver::release()
@@ -143,6 +146,7 @@ void HybridSolver::release()
     d_row_comp_.release();
     d_L_.release();
     d_swork_.release();
+    d_unit_dense_.release();
     xp_.release(), bp_.release(), r_.release(), p_.release(), q_.release(), z_.release(), u_.release();
     z32_.release();
     ncomp_ = 0;
@@ -680,12 +684,57 @@ void HybridSolver::build_mas(hipStream_t s)
 
 void HybridSolver::build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s)
 {
-    // factorize_submatrix, GPUHybrid.cu:993-1411, with a dense FP64 Cholesky per component
-    ncomp_ = int32_t(comps_.size());
+    // factorize_submatrix, GPUHybrid.cu:993-1411, with a dense FP64 Cholesky per unit.
+    // Units: the components, except that with max_dense_size > 0 the components of at
+    // most max_dense_size DOFs go to the dense branch (GPUHybrid.cu:1253-1406): first-fit
+    // packed, in component order, into bins of the largest such size, each bin one unit
+    // (its DOFs the concatenation of its components) whose explicit inverse is applied
+    // as one product.  The larger components come first, then the bins (the order of
+    // d_all_bad_dofs, GPUHybrid.cu:244-259).  Components are decoupled (no stored entry
+    // joins two of them), so a bin's S A S^T is block diagonal and its identity padding
+    // in the reference (a bin of fewer DOFs than the largest) changes nothing.
+    std::vector<std::vector<int32_t>> units;
+    std::vector<uint8_t> dense;
+    dense_bins_ = 0;
+    dense_dim_ = 0;
+    dense_dofs_ = 0;
+    if (cfg_.max_dense_size > 0) {
+        for (auto &c : comps_)
+            if (int64_t(c.size()) <= cfg_.max_dense_size) dense_dim_ = std::max<int64_t>(dense_dim_, c.size());
+        std::vector<std::vector<int32_t>> bins;
+        for (auto &c : comps_) {
+            if (int64_t(c.size()) > cfg_.max_dense_size) {
+                units.push_back(c);
+                dense.push_back(0);
+                continue;
+            }
+            dense_dofs_ += int64_t(c.size());
+            bool placed = false;
+            for (auto &b : bins)
+                if (int64_t(b.size() + c.size()) <= dense_dim_) {
+                    b.insert(b.end(), c.begin(), c.end());
+                    placed = true;
+                    break;
+                }
+            if (!placed) bins.push_back(c);
+        }
+        dense_bins_ = int64_t(bins.size());
+        bin_sizes_.clear();
+        for (auto &b : bins) {
+            bin_sizes_.push_back(int64_t(b.size()));
+            units.push_back(std::move(b));
+            dense.push_back(1);
+        }
+    } else {
+        units = comps_;
+        dense.assign(units.size(), 0);
+        bin_sizes_.clear();
+    }
+    ncomp_ = int32_t(units.size());
     std::vector<int64_t> off(ncomp_ + 1, 0), moff(ncomp_ + 1, 0);
     max_comp_ = 0;
     for (int32_t c = 0; c < ncomp_; ++c) {
-        const int64_t m = int64_t(comps_[c].size());
+        const int64_t m = int64_t(units[c].size());
         if (m > cfg_.max_dense_component)
             fail(LSS_ERR_UNSUPPORTED, "stiff component of " + std::to_string(m) +
                                           " DOFs exceeds max_dense_component (" +
@@ -697,8 +746,8 @@ void HybridSolver::build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx
     ns_ = off[ncomp_];
     std::vector<int32_t> dofs(ns_), pd(ns_);
     for (int32_t c = 0; c < ncomp_; ++c)
-        for (size_t r = 0; r < comps_[c].size(); ++r) {
-            const int32_t d = comps_[c][r];
+        for (size_t r = 0; r < units[c].size(); ++r) {
+            const int32_t d = units[c][r];
             dofs[off[c] + r] = d;
             pd[off[c] + r] = perm_[d / bs_] * bs_ + d % bs_;
         }
@@ -742,6 +791,28 @@ void HybridSolver::build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx
     k::stiff_inverse_factor(ncomp_, int32_t(max_small), d_comp_off_.get(), d_mat_off_.get(), d_L_.get(), G.get(), s);
     sync(s);
     d_L_ = std::move(G);  // only the inverse factor is needed by the apply
+    // dense branch: the bins' storage takes their explicit inverses M = G^T G
+    std::vector<int32_t> dl;
+    int64_t dmax = 0;
+    for (int32_t c = 0; c < ncomp_; ++c)
+        if (dense[c]) {
+            dl.push_back(c);
+            dmax = std::max(dmax, off[c + 1] - off[c]);
+        }
+    if (!dl.empty()) {
+        DBuf<int32_t> d_dl;
+        d_dl.upload(dl.data(), dl.size(), s);
+        DBuf<double> M(moff[ncomp_]);
+        k::explicit_inverse(int32_t(dl.size()), d_dl.get(), int32_t(dmax), d_comp_off_.get(), d_mat_off_.get(),
+                            d_L_.get(), M.get(), s);
+        for (int32_t c : dl)
+            LSS_HIP(hipMemcpyAsync(d_L_.get() + moff[c], M.get() + moff[c], size_t(moff[c + 1] - moff[c]) * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+        sync(s);  // M and the list are released at scope end
+        d_unit_dense_.upload(dense.data(), dense.size(), s);
+    } else {
+        d_unit_dense_.release();
+    }
     d_swork_.resize(std::max<int64_t>(2 * kMaxCols * ns_, 1));
 }
 
@@ -915,7 +986,8 @@ void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, 
         if (ncomp_)
             k::stiff_correct(ncomp_, ns_, bs_, d_comp_off_.get(), d_mat_off_.get(), d_row_comp_.get(),
                              d_pdofs_.get(), d_L_.get(), d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z,
-                             d_swork_.get(), d_swork_.get() + kMaxCols * ns_, stop, s, k, np, true);
+                             d_swork_.get(), d_swork_.get() + kMaxCols * ns_, stop, s, k, np, true, nullptr,
+                             d_unit_dense_.get());
         return;
     }
     // custom_mixed_precond_iter, GPUHybrid.cu:521-542
@@ -931,7 +1003,7 @@ void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, 
     // z2 = z1 + P (r - A z1)  (dss_precond_iter, GPUHybrid.cu:544-675)
     k::stiff_correct(ncomp_, ns_, bs_, d_comp_off_.get(), d_mat_off_.get(), d_row_comp_.get(), d_pdofs_.get(),
                      d_L_.get(), d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z, d_swork_.get(),
-                     d_swork_.get() + kMaxCols * ns_, stop, s, k, np, false, z32);
+                     d_swork_.get() + kMaxCols * ns_, stop, s, k, np, false, z32, d_unit_dense_.get());
     // u = r - A z2
     const bool prof = cfg_.profile_kernels && st && k == 1;
     if (prof) prof_begin(1, s);
@@ -1296,7 +1368,10 @@ std::string HybridSolver::info_json() const
       << "},\"n_selected\":" << n_selected_ << ",\"n_filtered\":" << n_filtered_ << ",\"n_expanded\":" << n_expanded_
       << ",\"filter\":{\"too_small\":" << filter_.too_small << ",\"too_large\":" << filter_.too_large
       << ",\"not_poorly_conditioned\":" << filter_.not_poorly_conditioned << "},\"n_components\":" << comps_.size()
-      << ",\"n_stiff_dofs\":" << nst << ",\"max_component\":" << max_comp_ << "},\"mas\":{\"levels\":" << mas_levels_
+      << ",\"n_stiff_dofs\":" << nst << ",\"max_component\":" << max_comp_ << ",\"dense_branch\":{\"bins\":"
+      << dense_bins_ << ",\"dim\":" << dense_dim_ << ",\"dofs\":" << dense_dofs_ << ",\"bin_sizes\":[";
+    for (size_t b = 0; b < bin_sizes_.size(); ++b) o << (b ? "," : "") << bin_sizes_[b];
+    o << "]}},\"mas\":{\"levels\":" << mas_levels_
       << ",\"banks\":" << mas_banks_ << ",\"inverse_bytes\":" << mas_bytes_ << ",\"parts\":"
       << (part_off_.empty() ? 0 : part_off_.size() - 1) << ",\"level_nodes\":[";
     for (size_t l = 0; l < level_nodes_.size(); ++l) o << (l ? "," : "") << level_nodes_[l];

@@ -157,6 +157,9 @@ private:
     DBuf<int64_t> d_comp_off_, d_mat_off_;
     DBuf<int32_t> d_pdofs_, d_row_comp_;
     DBuf<double> d_L_, d_swork_;
+    DBuf<uint8_t> d_unit_dense_;  // per unit: 1 = dense-branch bin holding its explicit inverse
+    int64_t dense_bins_ = 0, dense_dim_ = 0, dense_dofs_ = 0;
+    std::vector<int64_t> bin_sizes_;
     DenseBlas dense_;  // blocked factorization of large components
     double omega_ = 1.0;
     double omega_used_ = 1.0;  // omega of the last solve (after a damping retry)

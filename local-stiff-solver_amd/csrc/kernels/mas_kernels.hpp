@@ -82,7 +82,13 @@ void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, c
                    const int32_t *row_comp, const int32_t *pdofs, const double *G, const int32_t *rowptr,
                    const int32_t *cols,
                    const double *bvals, const double *r, double *z, double *work, double *work2, const int *stop,
-                   hipStream_t s, int k = 1, int64_t ld = 0, bool additive = false, float *z32 = nullptr);
+                   hipStream_t s, int k = 1, int64_t ld = 0, bool additive = false, float *z32 = nullptr,
+                   const uint8_t *dense = nullptr);
+// M = G^T G (explicit inverse, both triangles) of the units listed in units[0 .. nunits),
+// each at most max_m DOFs, in the storage of G (comp_off / mat_off).  dense != nullptr
+// in stiff_correct marks the units whose storage holds such an M (one product).
+void explicit_inverse(int32_t nunits, const int32_t *units, int32_t max_m, const int64_t *comp_off,
+                      const int64_t *mat_off, const double *G, double *M, hipStream_t s);
 
 // u[p] = r[p] - (A z)[p] in FP64 for the stiff rows p: after a preconditioner-internal
 // SpMV with FP32 values, the rows that hold the stiff entries are recomputed exactly.

@@ -206,14 +206,17 @@ __global__ __launch_bounds__(64) void k_tri_inverse_panel(const int64_t *__restr
 
 // One wave per stiff row (all components at once).  Phase G: u_i = M(i, 0:i) t;
 // phase GT: x_j = M(j, j) u_j + M(j, j+1:m) u_{j+1:m}.  Lanes read a row with
-// coalesced loads, a wave reduction finishes the dot product.
+// coalesced loads, a wave reduction finishes the dot product.  A unit of the dense
+// branch (dense[c] = 1) holds its explicit inverse: phase G is the full row
+// u_i = M(i, 0:m) t and phase GT adds u unchanged.
 template <bool TRANSPOSED>
 __global__ __launch_bounds__(256) void k_stiff_rows(int64_t ns, const int32_t *__restrict__ row_comp,
                                                     const int64_t *__restrict__ comp_off,
                                                     const int64_t *__restrict__ mat_off, const int32_t *__restrict__ pdofs,
                                                     const double *__restrict__ M, const double *__restrict__ in,
                                                     double *__restrict__ out, double *__restrict__ z, int64_t ld,
-                                                    const int *__restrict__ stop, float *__restrict__ z32)
+                                                    const int *__restrict__ stop, float *__restrict__ z32,
+                                                    const uint8_t *__restrict__ dense)
 {
     if (dev::all_stopped(stop, gridDim.y)) return;
     const int64_t k = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
@@ -229,10 +232,15 @@ __global__ __launch_bounds__(256) void k_stiff_rows(int64_t ns, const int32_t *_
     const int32_t i = int32_t(k - off);
     const double *row = M + mat_off[c] + int64_t(i) * m;
     const double *v = in + off;
-    const int32_t lo = TRANSPOSED ? i : 0, hi = TRANSPOSED ? m : i + 1;
+    const bool full = dense && dense[c];
     double acc = 0.0;
-    for (int32_t j = lo + lane; j < hi; j += 64) acc += row[j] * v[j];
-    acc = dev::wave_sum(acc);
+    if (TRANSPOSED && full) {
+        acc = v[i];
+    } else {
+        const int32_t lo = TRANSPOSED ? i : 0, hi = (TRANSPOSED || full) ? m : i + 1;
+        for (int32_t j = lo + lane; j < hi; j += 64) acc += row[j] * v[j];
+        acc = dev::wave_sum(acc);
+    }
     if (lane == 0) {
         if (TRANSPOSED) {
             const int32_t d = pdofs[k];
@@ -241,6 +249,30 @@ __global__ __launch_bounds__(256) void k_stiff_rows(int64_t ns, const int32_t *_
             if (z32) z32[d] = float(v);
         } else
             out[k] = acc;
+    }
+}
+
+// Explicit inverse M = G^T G = (S A S^T)^{-1} of the listed units (the dense branch,
+// GPUHybrid.cu:1385-1405 inverts by LU instead): G is the lower triangle of the stored
+// factor, M(i, j) = sum over k >= max(i, j) of G(k, i) G(k, j).  One wave per row i,
+// lanes over j <= i; both triangles of M are written.
+__global__ __launch_bounds__(256) void k_explicit_inverse(const int32_t *__restrict__ units,
+                                                          const int64_t *__restrict__ comp_off,
+                                                          const int64_t *__restrict__ mat_off,
+                                                          const double *__restrict__ G, double *__restrict__ M)
+{
+    const int32_t c = units[blockIdx.x];
+    const int32_t m = int32_t(comp_off[c + 1] - comp_off[c]);
+    const int32_t i = int32_t(blockIdx.y) * 4 + int32_t(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= m) return;
+    const double *Gc = G + mat_off[c];
+    double *Mc = M + mat_off[c];
+    for (int32_t j = lane; j <= i; j += 64) {
+        double acc = 0.0;
+        for (int32_t k = i; k < m; ++k) acc += Gc[int64_t(k) * m + i] * Gc[int64_t(k) * m + j];
+        Mc[int64_t(i) * m + j] = acc;
+        Mc[int64_t(j) * m + i] = acc;
     }
 }
 
@@ -392,17 +424,25 @@ void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, c
                    const int32_t *row_comp, const int32_t *pdofs, const double *G, const int32_t *rowptr,
                    const int32_t *cols,
                    const double *bvals, const double *r, double *z, double *work, double *work2, const int *stop,
-                   hipStream_t s, int k, int64_t ld, bool additive, float *z32)
+                   hipStream_t s, int k, int64_t ld, bool additive, float *z32, const uint8_t *dense)
 {
     if (ncomp == 0 || ns == 0) return;
     k_stiff_residual<false><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r,
                                                                   additive ? nullptr : z, ld, work, stop);
     LSS_LAUNCH_CHECK();
     k_stiff_rows<false><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, row_comp, comp_off, mat_off, pdofs, G, work, work2,
-                                                              z, ld, stop, nullptr);
+                                                              z, ld, stop, nullptr, dense);
     LSS_LAUNCH_CHECK();
     k_stiff_rows<true><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, row_comp, comp_off, mat_off, pdofs, G, work2,
-                                                             nullptr, z, ld, stop, z32);
+                                                             nullptr, z, ld, stop, z32, dense);
+    LSS_LAUNCH_CHECK();
+}
+
+void explicit_inverse(int32_t nunits, const int32_t *units, int32_t max_m, const int64_t *comp_off,
+                      const int64_t *mat_off, const double *G, double *M, hipStream_t s)
+{
+    if (nunits == 0 || max_m == 0) return;
+    k_explicit_inverse<<<dim3(nunits, div_up(max_m, 4)), 256, 0, s>>>(units, comp_off, mat_off, G, M);
     LSS_LAUNCH_CHECK();
 }
 

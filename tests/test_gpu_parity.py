@@ -450,3 +450,85 @@ def test_pq_fixture_selection_and_solve(gpu):
     assert np.linalg.norm(R @ x - b) <= max(1.01e-8 * np.linalg.norm(b), floor)
     xd = g["x_direct"]
     assert np.linalg.norm(x - xd) <= 1e-5 * np.linalg.norm(xd)
+
+
+# ---------------------------------------------------------------- dense branch (max_dense_size > 0)
+def _first_fit(sizes, cap):
+    """First-fit packing of the component sizes, in order, into bins of capacity cap
+    (GPUHybrid.cu:1262-1285), written from the rule itself."""
+    bins = []
+    for sz in sizes:
+        for j, b in enumerate(bins):
+            if b + sz <= cap:
+                bins[j] = b + sz
+                break
+        else:
+            bins.append(sz)
+    return bins
+
+
+def _check_dense_bins(info, sizes, max_dense):
+    dense_sizes = [z for z in sizes if z <= max_dense]
+    db = info["stiff"]["dense_branch"]
+    if not dense_sizes:
+        assert db["bins"] == 0
+        return db
+    cap = max(dense_sizes)
+    assert db["dim"] == cap
+    assert db["bin_sizes"] == _first_fit(dense_sizes, cap)
+    assert db["dofs"] == sum(dense_sizes)
+    return db
+
+
+# components of 63 to 321 DOFs: 150 sends three to the dense branch (no two share a
+# bin), 400 sends all four and packs 129 + 135 into one bin
+@pytest.mark.parametrize("max_dense", [150, 400, 100000])
+def test_dense_branch_bins_and_solve(gpu, max_dense):
+    # components of at most max_dense_size DOFs are packed first-fit into bins and get
+    # explicit inverses; the preconditioner is the same operator (components are
+    # decoupled), so the solve takes the oracle's iterations and reaches its solution
+    prob = problems.make("elastic_contact", N=24)
+    A = prob.csc()
+    b = prob.rhs(0)
+    s = Solver.create(_cfg(3, max_dense_size=max_dense))
+    s.factorize(A)
+    sizes = [c.size for c in s.stiff_components()]
+    assert len(sizes) == 4
+    info = s.get_info()
+    db = _check_dense_bins(info, sizes, max_dense)
+    if max_dense == 400:
+        assert db["bins"] == 3  # one bin holds two components
+    x = s.solve(b)
+    info = s.get_info()
+    H = O.Hybrid(_ocfg(3))
+    H.factorize(A.colptr, A.rowidx, A.vals)
+    xr = H.solve(b)
+    oi = H.info()
+    assert info["stop_reason"] == "converged_rel"
+    assert _res_ok(A.to_scipy(), x, b)
+    assert abs(info["num_iterations"] - oi["num_iterations"]) <= max(2, 0.05 * oi["num_iterations"])
+    assert np.linalg.norm(x - xr) <= 1e-5 * np.linalg.norm(xr)
+    # M^{-1} with explicit bin inverses equals the oracle's per-component solves
+    perm, off = s.partition(A.n // 3)
+    H2 = O.Hybrid(_ocfg(3))
+    H2.set_partition(perm, off)
+    H2.factorize(A.colptr, A.rowidx, A.vals)
+    H2.set_omega(info["omega"])
+    r = problems.random_vector(A.n, 8)
+    zr = H2.precond(r)
+    assert np.linalg.norm(s.precond_apply(r) - zr) <= 1e-8 * np.linalg.norm(zr)
+
+
+def test_dense_branch_pq_fixture(gpu):
+    # the reference's A_pq system: 30 components of 37 to 6595 DOFs; 500 packs the
+    # 26 small ones into bins of 489 DOFs next to the four larger direct factors
+    A, b, g = _pq_fixture()
+    s = Solver.create(_cfg(1, max_dense_size=500))
+    s.factorize(A)
+    sizes = [c.size for c in s.stiff_components()]
+    db = _check_dense_bins(s.get_info(), sizes, 500)
+    assert db["bins"] < sum(1 for z in sizes if z <= 500)
+    x = s.solve(b)
+    assert s.get_info()["stop_reason"] == "converged_rel"
+    xd = g["x_direct"]
+    assert np.linalg.norm(x - xd) <= 1e-5 * np.linalg.norm(xd)
