@@ -178,6 +178,15 @@ int lss_debug_partition(const lss_solver *s, int32_t *perm, int64_t nb, int32_t 
                         int32_t *nparts);
 /* y = A x with the device SpMV (FP64). */
 int lss_debug_spmv(lss_solver *s, const double *x, double *y);
+/* AMG hierarchy of base_preconditioner "AMG" (BoomerAMG setup, GPUHybrid.cu:357-376):
+ * *nlevels levels; level `level` has *n rows, operator A (levels >= 1; level 0 is the
+ * input) and interpolation P (n x rows of the next level, none on the coarsest) with
+ * *nnz_a / *nnz_p entries, and the C/F splitting cf (1 C, -1 F, -3 no strong
+ * connection; n entries, none on the coarsest).  Arrays (CSR, int64 row pointers)
+ * are copied when non-NULL. */
+int lss_debug_amg(const lss_solver *s, int32_t level, int32_t *nlevels, int32_t *n, int64_t *nnz_a, int64_t *nnz_p,
+                  int8_t *cf, int64_t *a_ptr, int32_t *a_col, double *a_val, int64_t *p_ptr, int32_t *p_col,
+                  double *p_val);
 /* z = B r (base preconditioner, undamped) and z = M^{-1} r (full hybrid preconditioner). */
 int lss_debug_base_apply(lss_solver *s, const double *r, double *z);
 int lss_debug_precond_apply(lss_solver *s, const double *r, double *z);

@@ -300,6 +300,16 @@ int lss_debug_partition(const lss_solver *s, int32_t *perm, int64_t nb, int32_t 
     });
 }
 
+int lss_debug_amg(const lss_solver *s, int32_t level, int32_t *nlevels, int32_t *n, int64_t *nnz_a, int64_t *nnz_p,
+                  int8_t *cf, int64_t *a_ptr, int32_t *a_col, double *a_val, int64_t *p_ptr, int32_t *p_col,
+                  double *p_val)
+{
+    return guarded(const_cast<lss_solver *>(s), [&] {
+        if (!s->impl) lss::fail(LSS_ERR_CONFIG, "solver was not created");
+        s->impl->debug_amg(level, nlevels, n, nnz_a, nnz_p, cf, a_ptr, a_col, a_val, p_ptr, p_col, p_val);
+    });
+}
+
 int lss_debug_spmv(lss_solver *s, const double *x, double *y)
 {
     return guarded(s, [&] {

@@ -16,7 +16,7 @@ struct Config {
     double relative_tolerance = 1e-10;
     double absolute_tolerance = 0.0;
     int block_size = 1;
-    double theta = 0.5;  // AMG strength threshold; accepted, unused (MAS has no strength graph)
+    double theta = 0.5;  // AMG strength threshold for block_size > 1 (0.25 for scalar problems, GPUHybrid.cu:285-339)
     bool do_hybrid_precond = true;
     bool select_bad_dofs_from_l1_norm = true;
     double conditioning_threshold = 100.0;
@@ -49,7 +49,7 @@ struct Config {
     // keys of this build
     std::string precision = "fp64";     // MAS bank inverses: "fp64" | "fp32" | "bf16"
     std::string inner_spmv = "auto";    // preconditioner-internal SpMV values: "auto" | "fp64" | "fp32"
-    std::string base_preconditioner = "MAS";  // "MAS" | "Jacobi"
+    std::string base_preconditioner = "MAS";  // "MAS" | "AMG" | "Jacobi"
     int check_every = 8;                // PCG iterations between host polls of the device stop flag
     double damping = 1.8;               // omega = damping / lambda_max(B A) inside the B-P-B sweep
     int lanczos_steps = 20;          // CG-Lanczos steps of the lambda_max(B A) estimate

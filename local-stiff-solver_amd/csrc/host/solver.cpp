@@ -13,6 +13,7 @@
 #include <thread>
 
 #include "../kernels/util_kernels.hpp"
+#include "amg.hpp"
 #include "topology.hpp"
 
 namespace lss {
@@ -123,6 +124,7 @@ void HybridSolver::reset_factorization()
     d_unit_dense_.release();
     dense_bins_ = dense_dim_ = dense_dofs_ = 0;
     bin_sizes_.clear();
+    amg_.clear();
 }
 
 void HybridSolver::release()
@@ -248,6 +250,8 @@ void HybridSolver::factorize_device(int32_t n, int64_t nnz, const int32_t *ptr, 
         t = clk::now();
         if (cfg_.base_preconditioner == "MAS") {
             build_mas(s);
+        } else if (cfg_.base_preconditioner == "AMG") {
+            build_amg(n, ptr, idx, vals, s);
         } else {
             d_dinv_.resize(size_t(nb_) * bs_);
             k::diag_inverse(nb_, bs_, d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), d_dinv_.get(), s);
@@ -682,6 +686,106 @@ void HybridSolver::build_mas(hipStream_t s)
     if (st) fail(LSS_ERR_NOT_SPD, "MAS bank inverse failed (zero pivot or non-finite value)");
 }
 
+void HybridSolver::build_amg(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s)
+{
+    // BoomerAMG setup (GPUHybrid.cu:357-376 with the options of 285-339): the hierarchy
+    // is built on the host from the operator rows (CSC column j = row j), then every
+    // level's interpolation, restriction, operator and l1 norms go to the device
+    CsrHost A0;
+    A0.n = A0.m = n;
+    {
+        std::vector<int32_t> p32(size_t(n) + 1);
+        LSS_HIP(hipMemcpyAsync(p32.data(), ptr, (size_t(n) + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        sync(s);
+        A0.ptr.assign(p32.begin(), p32.end());
+        A0.col.resize(size_t(nnz_));
+        A0.val.resize(size_t(nnz_));
+        LSS_HIP(hipMemcpyAsync(A0.col.data(), idx, size_t(nnz_) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        LSS_HIP(hipMemcpyAsync(A0.val.data(), vals, size_t(nnz_) * sizeof(double), hipMemcpyDeviceToHost, s));
+        sync(s);
+    }
+    AmgOptions opt;
+    opt.num_functions = bs_;
+    opt.theta = bs_ > 1 ? cfg_.theta : 0.25;
+    std::vector<AmgLevelHost> lv;
+    amg_setup_host(A0, opt, lv);
+    A0 = CsrHost{};
+    auto up = [&](const CsrHost &h, DevCsr &d) {
+        d.n = h.n;
+        d.ptr.upload(h.ptr.data(), h.ptr.size(), s);
+        d.col.upload(h.col.data(), std::max<size_t>(h.col.size(), 1), s);
+        d.val.upload(h.val.data(), std::max<size_t>(h.val.size(), 1), s);
+    };
+    const int64_t np = int64_t(nb_) * bs_;
+    amg_.clear();
+    amg_.resize(lv.size());
+    amg_cf_.clear();
+    amg_sizes_.clear();
+    amg_nnz_.clear();
+    for (size_t l = 0; l < lv.size(); ++l) {
+        AmgLevel &L = amg_[l];
+        const AmgLevelHost &H = lv[l];
+        L.n = H.A.n;
+        amg_sizes_.push_back(H.A.n);
+        amg_nnz_.push_back(int64_t(H.A.col.size()));
+        if (l > 0) up(H.A, L.A);
+        amg_cf_.push_back(H.cf);
+        if (!H.P.ptr.empty()) {
+            up(H.P, L.P);
+            up(H.R, L.R);
+        }
+        // level 0 works on the padded vectors of the sliced operator: its tail
+        // rows have a unit diagonal, l1 norm 1
+        std::vector<double> dinv = H.dinv;
+        if (l == 0) dinv.resize(size_t(np), 1.0);
+        L.dinv.upload(dinv.data(), dinv.size(), s);
+        const size_t m = l == 0 ? size_t(np) : size_t(std::max(H.A.n, 1));
+        if (l > 0) {
+            L.b.resize(m);
+            L.x.resize(m);
+        }
+        L.t.resize(m);
+    }
+    amg_z_.resize(size_t(np));
+    sync(s);
+}
+
+void HybridSolver::amg_vcycle(const double *b0, double *x0, hipStream_t s)
+{
+    // one V-cycle from x = 0 (GPUHybrid.cu:677-689: HYPRE_BoomerAMGSolve, one
+    // iteration): l1-Jacobi pre-smoothing, residual, restriction R = P^T down to the
+    // coarsest level (one l1-Jacobi sweep there), then prolongation and l1-Jacobi
+    // post-smoothing up.  Level 0 uses the sliced block operator for its products.
+    const int nl = int(amg_.size());
+    const int64_t np = int64_t(nb_) * bs_;
+    auto bvec = [&](int l) { return l == 0 ? b0 : amg_[l].b.get(); };
+    auto xvec = [&](int l) { return l == 0 ? x0 : amg_[l].x.get(); };
+    for (int l = 0; l + 1 < nl; ++l) {
+        AmgLevel &L = amg_[l];
+        const int64_t m = l == 0 ? np : L.n;
+        k::amg_level_smooth_pre(int32_t(m), L.dinv.get(), bvec(l), xvec(l), s);
+        if (l == 0) k::spmv_residual(sell_, xvec(0), bvec(0), L.t.get(), nullptr, s);
+        else k::amg_residual(L.A.view(), xvec(l), bvec(l), L.t.get(), s);
+        k::amg_spmv(L.R.view(), L.t.get(), amg_[l + 1].b.get(), s);
+    }
+    {
+        AmgLevel &C = amg_[nl - 1];
+        const int64_t m = nl == 1 ? np : C.n;
+        k::amg_level_smooth_pre(int32_t(m), C.dinv.get(), bvec(nl - 1), xvec(nl - 1), s);
+    }
+    for (int l = nl - 2; l >= 0; --l) {
+        AmgLevel &L = amg_[l];
+        k::amg_spmv_add(L.P.view(), amg_[l + 1].x.get(), xvec(l), s);
+        if (l == 0) {
+            k::spmv_residual(sell_, xvec(0), bvec(0), L.t.get(), nullptr, s);
+            k::amg_add_diag_scaled(np, L.dinv.get(), L.t.get(), xvec(0), s);
+        } else {
+            k::amg_smooth_post(L.A.view(), L.dinv.get(), xvec(l), bvec(l), L.t.get(), s);
+            LSS_HIP(hipMemcpyAsync(xvec(l), L.t.get(), size_t(L.n) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
+    }
+}
+
 void HybridSolver::build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s)
 {
     // factorize_submatrix, GPUHybrid.cu:993-1411, with a dense FP64 Cholesky per unit.
@@ -876,6 +980,9 @@ void HybridSolver::estimate_omega(hipStream_t s)
     // comps_ (not ncomp_, which the stiff factorization may be writing) tells whether
     // there is a stiff set
     if (cfg_.mas_mode() || !cfg_.do_hybrid_precond || comps_.empty()) return;
+    // a V-cycle of a convergent multigrid has lambda(B A) in (0, 2): the reference
+    // composes it undamped (GPUHybrid.cu:521-542)
+    if (cfg_.base_preconditioner == "AMG") return;
     // The additive MAS can have lambda_max(B A) > 2, which makes the multiplicative
     // sweep (I - BA)(I - PA)(I - BA) non-contractive and the composed preconditioner
     // indefinite.  omega = damping / theta, theta the largest Ritz value of B A after
@@ -929,6 +1036,29 @@ void HybridSolver::base_apply(const double *r, double scale, const double *add, 
 {
     const int64_t np = int64_t(nb_) * bs_;
     const int op = first ? dev::kFinishRzFirst : dev::kFinishRz;
+    if (cfg_.base_preconditioner == "AMG") {
+        // z = scale * V(r) + add per column, then r . z for the PCG.  The V-cycle has no
+        // stop test: after a column stops, the PCG updates leave its x and r alone.
+        for (int c = 0; c < k; ++c) {
+            const double *rc = r + c * np;
+            double *zc = z + c * np;
+            amg_vcycle(rc, amg_z_.get(), s);
+            if (add) {
+                if (add + c * np != zc) k::copy(np, add + c * np, zc, s);
+                k::axpy(np, scale, amg_z_.get(), zc, s);
+            } else {
+                k::copy(np, amg_z_.get(), zc, s);
+                if (scale != 1.0) k::scale(np, zc, scale, s);
+            }
+        }
+        if (st) {
+            for (int c = 0; c < k; ++c)
+                k::dot_partials(np, r_.get() + c * np, z + c * np, partials_.get() + c * k::partials_per_column(np), s);
+            k::finish(k::partials_per_column(np), partials_.get(), op, st, nullptr, 0, s, k);
+        }
+        if (z32) k::to_f32(int64_t(k) * np, z, z32, s);
+        return;
+    }
     if (cfg_.base_preconditioner != "MAS") {
         const int g = k::jacobi_apply(np, d_dinv_.get(), r, scale, add, z, st ? r_.get() : nullptr, st,
                                       partials_.get(), stop, s, k, np);
@@ -1185,7 +1315,8 @@ void HybridSolver::solve_device(const double *b, double *x, hipStream_t s)
     // omega = damping / theta keeps omega * lambda_max(B A) < 2 while theta is within
     // damping / 2 of lambda_max; should the sweep still lose definiteness (a PCG
     // breakdown), solve again from the same start with the safe damping
-    const bool damped = !cfg_.mas_mode() && cfg_.do_hybrid_precond && ncomp_ > 0 && cfg_.damping > kSafeDamping;
+    const bool damped = !cfg_.mas_mode() && cfg_.do_hybrid_precond && ncomp_ > 0 && cfg_.damping > kSafeDamping &&
+                        cfg_.base_preconditioner != "AMG";
     if (damped) {
         x0p_.resize(size_t(np));
         k::copy(np, xp_.get(), x0p_.get(), s);
@@ -1233,7 +1364,8 @@ void HybridSolver::solve_multi_device(int32_t nrhs, const double *B, double *X, 
     }
     auto t0 = clk::now();
     const int64_t np = int64_t(nb_) * bs_;
-    const bool damped = cfg_.do_hybrid_precond && ncomp_ > 0 && cfg_.damping > kSafeDamping;
+    const bool damped = cfg_.do_hybrid_precond && ncomp_ > 0 && cfg_.damping > kSafeDamping &&
+                        cfg_.base_preconditioner != "AMG";
     std::vector<int> iters(nrhs), stops(nrhs);
     std::vector<double> res(nrhs);
     bool retried = false;
@@ -1302,6 +1434,34 @@ void HybridSolver::debug_row_norms(double *out) const
     if (!d_norms_) fail(LSS_ERR_NOT_FACTORIZED, "row norms are computed by factorize when stiff DOFs are selected");
     d_norms_.download(out, n_, stream_);
     LSS_HIP(hipStreamSynchronize(stream_));
+}
+
+void HybridSolver::debug_amg(int32_t level, int32_t *nlevels, int32_t *n, int64_t *nnz_a, int64_t *nnz_p,
+                             int8_t *cf, int64_t *a_ptr, int32_t *a_col, double *a_val, int64_t *p_ptr,
+                             int32_t *p_col, double *p_val)
+{
+    if (!factorized_ || amg_.empty()) fail(LSS_ERR_NOT_FACTORIZED, "no AMG hierarchy (base_preconditioner AMG)");
+    if (nlevels) *nlevels = int32_t(amg_.size());
+    if (level < 0 || level >= int32_t(amg_.size())) {
+        if (n || nnz_a || nnz_p || cf || a_ptr || p_ptr) fail(LSS_ERR_ARG, "AMG level out of range");
+        return;
+    }
+    const AmgLevel &L = amg_[level];
+    if (n) *n = L.n;
+    if (nnz_a) *nnz_a = level == 0 ? nnz_ : int64_t(L.A.col.size());
+    if (nnz_p) *nnz_p = L.P.ptr ? int64_t(L.P.col.size()) : 0;
+    if (cf && !amg_cf_[level].empty()) std::memcpy(cf, amg_cf_[level].data(), amg_cf_[level].size());
+    if (level > 0 && a_ptr) {
+        L.A.ptr.download(a_ptr, size_t(L.n) + 1, stream_);
+        L.A.col.download(a_col, L.A.col.size(), stream_);
+        L.A.val.download(a_val, L.A.val.size(), stream_);
+    }
+    if (L.P.ptr && p_ptr) {
+        L.P.ptr.download(p_ptr, size_t(L.n) + 1, stream_);
+        L.P.col.download(p_col, L.P.col.size(), stream_);
+        L.P.val.download(p_val, L.P.val.size(), stream_);
+    }
+    sync(stream_);
 }
 
 void HybridSolver::debug_spmv(const double *x, double *y)
@@ -1375,6 +1535,10 @@ std::string HybridSolver::info_json() const
       << ",\"banks\":" << mas_banks_ << ",\"inverse_bytes\":" << mas_bytes_ << ",\"parts\":"
       << (part_off_.empty() ? 0 : part_off_.size() - 1) << ",\"level_nodes\":[";
     for (size_t l = 0; l < level_nodes_.size(); ++l) o << (l ? "," : "") << level_nodes_[l];
+    o << "]},\"amg\":{\"levels\":" << amg_.size() << ",\"rows\":[";
+    for (size_t l = 0; l < amg_sizes_.size(); ++l) o << (l ? "," : "") << amg_sizes_[l];
+    o << "],\"nnz\":[";
+    for (size_t l = 0; l < amg_nnz_.size(); ++l) o << (l ? "," : "") << amg_nnz_[l];
     o << "]},\"kernel_ms\":{\"spmv_outer\":" << prof_spmv_outer_ms_ << ",\"spmv_outer_n\":" << prof_spmv_outer_n_
       << ",\"spmv_inner\":" << prof_spmv_inner_ms_ << ",\"spmv_inner_n\":" << prof_spmv_inner_n_
       << ",\"mas_level0\":" << prof_mas0_ms_ << ",\"mas_level0_n\":" << prof_mas0_n_ << "}";

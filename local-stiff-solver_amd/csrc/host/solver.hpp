@@ -64,6 +64,8 @@ public:
     const std::vector<int32_t> &part_off() const { return part_off_; }
     const std::vector<std::vector<int32_t>> &components() const { return comps_; }
     void debug_spmv(const double *x, double *y);
+    void debug_amg(int32_t level, int32_t *nlevels, int32_t *n, int64_t *nnz_a, int64_t *nnz_p, int8_t *cf,
+                   int64_t *a_ptr, int32_t *a_col, double *a_val, int64_t *p_ptr, int32_t *p_col, double *p_val);
     void debug_base(const double *r, double *z);
     void debug_precond(const double *r, double *z);
 
@@ -93,6 +95,9 @@ private:
     bool build_levels_device(DBuf<int32_t> &&part_off, int32_t nparts, hipStream_t s);
     void build_levels_host(const Pattern &gp, hipStream_t s);
     void build_mas(hipStream_t s);
+    // AMG base preconditioner: hierarchy set up on the host, V-cycle on the device
+    void build_amg(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s);
+    void amg_vcycle(const double *b, double *x, hipStream_t s);
     void build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s);
     void estimate_omega(hipStream_t s);
     void alloc_vectors();
@@ -149,6 +154,23 @@ private:
     DBuf<uint8_t> spare_inv0_;  // the previous factorization's level-0 factor buffer, for reuse
     std::vector<int32_t> level_nodes_;
     DBuf<double> d_dinv_;
+    struct DevCsr {
+        int32_t n = 0;
+        DBuf<int64_t> ptr;
+        DBuf<int32_t> col;
+        DBuf<double> val;
+        k::AmgCsr view() const { return k::AmgCsr{n, ptr.get(), col.get(), val.get()}; }
+    };
+    struct AmgLevel {
+        int32_t n = 0;
+        DevCsr A, P, R;           // A: levels >= 1 (level 0 uses the sliced operator)
+        DBuf<double> dinv, b, x, t;
+    };
+    std::vector<AmgLevel> amg_;
+    std::vector<std::vector<int8_t>> amg_cf_;  // C/F splitting per level (host, inspection)
+    std::vector<int32_t> amg_sizes_;
+    std::vector<int64_t> amg_nnz_;
+    DBuf<double> amg_z_;  // V-cycle output before scaling
 
     // stiff blocks
     int32_t ncomp_ = 0;

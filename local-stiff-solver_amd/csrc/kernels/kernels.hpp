@@ -152,6 +152,27 @@ bool device_galerkin_pattern(int64_t fine_nnz, const int32_t *rowptr, const int3
 // bank_off = {0, 32, 64, ..., n}; returns the bank count.
 int32_t uniform_banks(int32_t n, DBuf<int32_t> &bank_off, hipStream_t s);
 
+// ---------------------------------------------------------------- AMG V-cycle
+// A scalar CSR matrix in device memory (n rows).
+struct AmgCsr {
+    int32_t n = 0;
+    const int64_t *ptr = nullptr;
+    const int32_t *col = nullptr;
+    const double *val = nullptr;
+};
+// x = dinv .* b (one l1-Jacobi sweep from zero)
+void amg_level_smooth_pre(int32_t n, const double *dinv, const double *b, double *x, hipStream_t s);
+// x += dinv .* t
+void amg_add_diag_scaled(int64_t n, const double *dinv, const double *t, double *x, hipStream_t s);
+// r = b - A x
+void amg_residual(const AmgCsr &A, const double *x, const double *b, double *r, hipStream_t s);
+// xout = x + dinv .* (b - A x) (xout must not alias x)
+void amg_smooth_post(const AmgCsr &A, const double *dinv, const double *x, const double *b, double *xout,
+                     hipStream_t s);
+// y = M x; y += M x
+void amg_spmv(const AmgCsr &M, const double *x, double *y, hipStream_t s);
+void amg_spmv_add(const AmgCsr &M, const double *x, double *y, hipStream_t s);
+
 // ---------------------------------------------------------------- true residual
 // out[0] = ||b - A x||_2, out[1] = ||b||_2 (device, two doubles) in double-double
 // arithmetic; the CSC arrays are read as operator rows.  partials holds

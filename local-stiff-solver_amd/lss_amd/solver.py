@@ -202,6 +202,44 @@ class Solver:
                                                   ctypes.byref(nparts)))
         return perm, off
 
+    def amg_levels(self) -> list[dict]:
+        """The AMG hierarchy (base_preconditioner "AMG"): per level n, cf, A (levels >= 1)
+        and P as scipy CSR matrices."""
+        import scipy.sparse as sp
+
+        c_i32, c_i64 = ctypes.c_int32, ctypes.c_int64
+        nl = c_i32()
+        self._check(self._lib.lss_debug_amg(self._h, -1, ctypes.byref(nl), None, None, None, None, None, None, None,
+                                            None, None, None))
+        out = []
+        for lv in range(nl.value):
+            n, na, npp = c_i32(), c_i64(), c_i64()
+            self._check(self._lib.lss_debug_amg(self._h, lv, None, ctypes.byref(n), ctypes.byref(na),
+                                                ctypes.byref(npp), None, None, None, None, None, None, None))
+            last = lv == nl.value - 1
+            cf = np.zeros(n.value if not last else 1, np.int8)
+            ap = np.zeros(n.value + 1, np.int64)
+            ac = np.zeros(max(na.value, 1), np.int32)
+            av = np.zeros(max(na.value, 1), np.float64)
+            pp = np.zeros(n.value + 1, np.int64)
+            pc = np.zeros(max(npp.value, 1), np.int32)
+            pv = np.zeros(max(npp.value, 1), np.float64)
+            self._check(self._lib.lss_debug_amg(self._h, lv, None, None, None, None,
+                                                None if last else _ptr(cf), _ptr(ap) if lv else None,
+                                                _ptr(ac) if lv else None, _ptr(av) if lv else None,
+                                                None if last else _ptr(pp), None if last else _ptr(pc),
+                                                None if last else _ptr(pv)))
+            d = {"n": n.value, "cf": None if last else cf}
+            if lv:
+                d["A"] = sp.csr_matrix((av[:na.value], ac[:na.value], ap), shape=(n.value, n.value))
+            if not last:
+                d["P"] = (pp, pc[:npp.value], pv[:npp.value])
+            out.append(d)
+        for lv in range(len(out) - 1):
+            pp, pc, pv = out[lv]["P"]
+            out[lv]["P"] = sp.csr_matrix((pv, pc, pp), shape=(out[lv]["n"], out[lv + 1]["n"]))
+        return out
+
     def spmv(self, x) -> np.ndarray:
         x = _host_vector(x, self._factorized_size(), "x")
         y = np.empty_like(x)

@@ -1,0 +1,216 @@
+"""GPU AMG base preconditioner (base_preconditioner "AMG") against the AMG restatement.
+
+The reference's GPUHybrid preconditions with hypre BoomerAMG
+(src/polysolve/linear/GPUHybrid.cu:285-339 options, 373 setup, 677-689 one
+V-cycle).  Each level of the device hierarchy is compared with oracle/amg.py run
+on that level's device operator: the C/F splitting exactly, the interpolation to
+1e-14, the coarse operator with the Galerkin product P^T A P of the device P; the
+restatement's own chain from the input matrix gives the same hierarchy bit for
+bit.  The
+V-cycle is compared with the restatement's V-cycle on the device hierarchy, the
+AMG-only PCG ("GPUAMG") with a textbook PCG around that V-cycle, and the hybrid
+preconditioner with base AMG with its composition z2 + B (r - A z2),
+z2 = z1 + S (r - A z1), z1 = B r, S the exact solve on the stiff components
+(GPUHybrid.cu:519-540).
+"""
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from lss_amd import problems
+from lss_amd.solver import Solver
+from oracle import amg
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("laplace_sphere", 10), ("laplace_sphere", 16), ("elastic_contact", 6), ("elastic_logE", 8),
+         ("elastic_slivers", 6)]
+
+
+def _log(*a):
+    print("[amg]", *a, file=sys.stderr, flush=True)
+
+
+def _cfg(bs, **kw):
+    g = {"block_size": bs, "relative_tolerance": 1e-8, "gmm_jump_threshold": 10.0, "min_subdomain_size": 4,
+         "base_preconditioner": "AMG", "theta": 0.5}
+    g.update(kw)
+    return {"solver": "GPUHybrid", "GPUHybrid": g}
+
+
+def _rows(A):
+    """The operator the library builds: CSC column j taken as row j (GPUHybrid.cu:401-442)."""
+    return sp.csr_matrix((A.vals, A.rowidx, A.colptr), shape=(A.n, A.n))
+
+
+def _check_levels(A0, lv, bs, theta):
+    """Level by level: the restatement on the device operator of each level."""
+    func = [i % bs for i in range(A0.shape[0])]
+    As, Ps = [A0], []
+    th = theta if bs > 1 else 0.25
+    for l in range(len(lv) - 1):
+        Al = As[l]
+        S = amg.strength(Al, func, bs, th)
+        cf = amg.pmis(S)
+        assert np.array_equal(lv[l]["cf"], np.array(cf, np.int8)), l
+        P = amg.interp_extpi(Al, S, cf, func, bs)
+        Pd = lv[l]["P"]
+        assert Pd.shape == P.shape
+        assert np.array_equal(Pd.indptr, P.indptr) and np.array_equal(Pd.indices, P.indices), l
+        dp = np.abs(Pd.data - P.data).max(initial=0.0)
+        assert dp <= 1e-14 * np.abs(P.data).max(initial=1.0), (l, dp)
+        Ac = lv[l + 1]["A"]
+        G = sp.csr_matrix(Pd.T @ (Al @ Pd))
+        scale = abs(G).max()
+        assert abs(Ac - G).max() <= 1e-13 * scale, l
+        func = [func[i] for i in range(len(cf)) if cf[i] == 1]
+        As.append(Ac)
+        Ps.append(Pd)
+    # the coarsest level ends the hierarchy by the setup's rules
+    last = As[-1]
+    if last.shape[0] > 5 and len(As) < 25:
+        cf = amg.pmis(amg.strength(last, func, bs, th))
+        assert sum(c == 1 for c in cf) in (0, last.shape[0])
+    return As, Ps
+
+
+@pytest.mark.parametrize("name,N", CASES)
+def test_hierarchy_matches_restatement(gpu, name, N):
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    s = Solver.create(_cfg(prob.block_size))
+    s.factorize(A)
+    lv = s.amg_levels()
+    info = s.get_info()
+    assert info["amg"]["levels"] == len(lv) >= 2
+    assert info["amg"]["rows"] == [d["n"] for d in lv]
+    As, _ = _check_levels(_rows(A), lv, prob.block_size, 0.5)
+    # the restatement's own chain from the input matrix: the same hierarchy, bit for bit
+    H = amg.Hierarchy(_rows(A), block_size=prob.block_size, theta=0.5 if prob.block_size > 1 else 0.25)
+    assert [a.shape[0] for a in H.A] == [d["n"] for d in lv]
+    for l in range(1, len(lv)):
+        assert abs(H.A[l] - lv[l]["A"]).max() == 0, l
+        assert (H.P[l - 1] != lv[l - 1]["P"]).nnz == 0, l
+    _log(name, N, "levels", [a.shape[0] for a in As], "nnz", info["amg"]["nnz"])
+
+
+@pytest.mark.parametrize("name,N", CASES)
+def test_vcycle_matches_restatement(gpu, name, N):
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    s = Solver.create(_cfg(prob.block_size))
+    s.factorize(A)
+    lv = s.amg_levels()
+    As = [_rows(A)] + [d["A"] for d in lv[1:]]
+    Ps = [d["P"] for d in lv[:-1]]
+    dinvs = [amg.l1_dinv(a) for a in As]
+    for seed in (0, 1):
+        r = problems.random_vector(A.n, 70 + seed)
+        z = s.base_apply(r)
+        zr = amg.vcycle(As, Ps, dinvs, r)
+        err = np.linalg.norm(z - zr) / np.linalg.norm(zr)
+        assert err <= 1e-12, (seed, err)
+    # B is symmetric positive definite
+    u, v = problems.random_vector(A.n, 81), problems.random_vector(A.n, 82)
+    Bu, Bv = s.base_apply(u), s.base_apply(v)
+    uBu, vBv, vBu, uBv = float(u @ Bu), float(v @ Bv), float(v @ Bu), float(u @ Bv)
+    assert uBu > 0 and vBv > 0
+    # rounding of the contact workloads' 1e6 stiffness contrast: 1e-10
+    asym = abs(vBu - uBv) / np.sqrt(uBu * vBv)
+    assert asym <= 1e-10, asym
+
+
+def _pcg(A, b, prec, rtol, max_iter=2000):
+    x = np.zeros_like(b)
+    r = b.copy()
+    bn = np.linalg.norm(b)
+    z = prec(r)
+    p = z.copy()
+    rz = r @ z
+    it = 0
+    while np.linalg.norm(r) > rtol * bn and it < max_iter:
+        Ap = A @ p
+        a = rz / (p @ Ap)
+        x += a * p
+        r -= a * Ap
+        z = prec(r)
+        rz, rz_old = r @ z, rz
+        p = z + (rz / rz_old) * p
+        it += 1
+    return x, it
+
+
+@pytest.mark.parametrize("name,N", CASES)
+def test_amg_pcg(gpu, name, N):
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    s = Solver.create(_cfg(prob.block_size, do_hybrid_precond=False))
+    assert s.name() == "GPUAMG"
+    s.factorize(A)
+    b = prob.rhs(0)
+    x = s.solve(b)
+    info = s.get_info()
+    assert info["stop_reason"] == "converged_rel"
+    S = _rows(A)
+    lv = s.amg_levels()
+    As = [S] + [d["A"] for d in lv[1:]]
+    Ps = [d["P"] for d in lv[:-1]]
+    dinvs = [amg.l1_dinv(a) for a in As]
+    xr, it = _pcg(S, b, lambda r: amg.vcycle(As, Ps, dinvs, r), 1e-8)
+    assert abs(info["num_iterations"] - it) <= 1, (info["num_iterations"], it)
+    assert np.linalg.norm(S @ x - b) <= 1.01e-8 * np.linalg.norm(b) + 50 * np.finfo(float).eps * np.linalg.norm(
+        abs(S) @ np.abs(x))
+    assert np.linalg.norm(x - xr) <= 1e-5 * np.linalg.norm(xr)
+    _log(name, N, "GPUAMG iterations", info["num_iterations"], "textbook PCG", it)
+
+
+def _stiff_solve(S, comps):
+    """Exact solve on each stiff component (the reference factorizes the component's
+    principal submatrix, GPUHybrid.cu:993-1411)."""
+    lus = [(c, spla.splu(sp.csc_matrix(S[c][:, c]))) for c in comps]
+
+    def apply(v):
+        out = np.zeros_like(v)
+        for c, lu in lus:
+            out[c] = lu.solve(v[c])
+        return out
+
+    return apply
+
+
+@pytest.mark.parametrize("name,N", [("laplace_sphere", 10), ("elastic_contact", 6), ("elastic_contact", 8)])
+def test_hybrid_amg_composition_and_solve(gpu, name, N):
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    s = Solver.create(_cfg(prob.block_size))
+    s.factorize(A)
+    comps = s.stiff_components()
+    assert comps, "the workload has stiff components"
+    S = _rows(A)
+    stiff = _stiff_solve(S, comps)
+    for seed in (0, 1):
+        r = problems.random_vector(A.n, 90 + seed)
+        z1 = s.base_apply(r)
+        z2 = z1 + stiff(r - S @ z1)
+        want = z2 + s.base_apply(r - S @ z2)
+        got = s.precond_apply(r)
+        err = np.linalg.norm(got - want) / np.linalg.norm(want)
+        # the device applies the stiff solve as an explicit inverse G^T G (G = L^-1):
+        # its rounding grows with the condition number of the contact blocks
+        assert err <= 1e-8, (seed, err)
+    b = prob.rhs(0)
+    x = s.solve(b)
+    info = s.get_info()
+    assert info["stop_reason"] == "converged_rel"
+    assert np.linalg.norm(S @ x - b) <= 1.01e-8 * np.linalg.norm(b) + 50 * np.finfo(float).eps * np.linalg.norm(
+        abs(S) @ np.abs(x))
+    xd = spla.spsolve(sp.csc_matrix(S), b)
+    assert np.linalg.norm(x - xd) <= 1e-5 * np.linalg.norm(xd)
+    s2 = Solver.create(_cfg(prob.block_size, do_hybrid_precond=False))
+    s2.factorize(A)
+    s2.solve(b)
+    _log(name, N, "hybrid(AMG) iterations", info["num_iterations"], "AMG only", s2.get_info()["num_iterations"],
+         "stiff DOFs", sum(c.size for c in comps))
