@@ -107,6 +107,9 @@ def main():
                     help="first copy this many MiB device-to-device (HBM counter calibration)")
     ap.add_argument("--multi-rhs", type=int, default=0, metavar="K",
                     help="also time one fused K-right-hand-side solve of the mixed path against K single solves")
+    ap.add_argument("--amg", action="store_true",
+                    help="also time the AMG base preconditioner (BoomerAMG restated): hybrid with base AMG and "
+                         "AMG alone (GPUAMG), FP64")
     ap.add_argument("--mas-solver", action="store_true",
                     help="also time the standalone MAS solver (its defaults, rtol 1e-6, at most 3000 iterations) "
                          "without and with each stiff-DOF strategy")
@@ -298,6 +301,33 @@ def main():
                 mas_runs[vname] = {"error": str(e)}
             log(vname, mas_runs[vname])
             del s
+    amg_runs = None
+    if args.amg:
+        amg_runs = {}
+        for vname, extra in {"hybrid_amg": {}, "amg_only": {"do_hybrid_precond": False}}.items():
+            g = dict(paths["fp64"], base_preconditioner="AMG", **extra)
+            log(vname, "start")
+            s = Solver.create({"solver": "GPUHybrid", "GPUHybrid": g})
+
+            def astep():
+                s.factorize_device(n, d_ptr, d_idx, d_val, stream=stream)
+                x.zero_()
+                s.solve_device(d_b, x, stream=stream)
+
+            astep()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            astep()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            ai = s.get_info()
+            rdd = residual_norm(cp_d, ri_d, va_d, d_b, x, stream=stream)[0]
+            amg_runs[vname] = {"solver": ai["solver"], "s": round(dt, 4), "phases_s": ai["timings"],
+                               "amg_setup_s": ai["timings"]["mas"], "iterations": ai["num_iterations"],
+                               "stop_reason": ai["stop_reason"], "true_rel_residual_dd": rdd / bnorm,
+                               "levels": ai["amg"]["rows"], "level_nnz": ai["amg"]["nnz"]}
+            log(vname, amg_runs[vname])
+            del s
     # roofline of the outer FP64 SpMV, timed with HIP events on the solver's stream
     g = dict(paths["mixed"], profile_kernels=True)
     s = Solver.create({"solver": "GPUHybrid", "GPUHybrid": g})
@@ -375,6 +405,8 @@ def main():
         line["multi_rhs"] = multi
     if mas_runs is not None:
         line["mas_solver"] = mas_runs
+    if amg_runs is not None:
+        line["amg"] = amg_runs
     print(json.dumps(line))
 
 

@@ -231,6 +231,7 @@ private:
     size_t ev_used_ = 0;
     void prof_begin(int kind, hipStream_t s);
     void prof_end(hipStream_t s);
+    hipEvent_t prof_end_event();  // the closing event of the open pair (nullptr when skipped)
     void prof_collect();
 };
 

@@ -44,8 +44,9 @@ struct SellView {
 // kernels do nothing once the stop words of all k columns are set.  k = 1, 2 or 4
 // right-hand sides share one pass over the matrix (column c at v + c * ld).
 void spmv(const SellView &A, const double *x, double *y, const double *sub, hipStream_t s);
+// `after`, when given, is recorded on s between the SpMV and the finisher (kernel timing).
 void spmv_pq(const SellView &A, const double *p, double *q, dev::PcgState *st, double *partials, int iter,
-             hipStream_t s, int k = 1, int64_t ld = 0, int finish_op = dev::kFinishPq);
+             hipStream_t s, int k = 1, int64_t ld = 0, int finish_op = dev::kFinishPq, hipEvent_t after = nullptr);
 // fp32: the FP32 values with x32, an FP32 copy of x at the same stride (x unused)
 void spmv_residual(const SellView &A, const double *x, const double *b, double *r, const int *stop, hipStream_t s,
                    bool fp32 = false, int k = 1, int64_t ld = 0, const float *x32 = nullptr);

@@ -612,9 +612,10 @@ int spmv_partials_per_column(const SellView &A) { return div_up(A.nslices, kSpmv
 int64_t sell_value_count(int64_t slots, int bs) { return slots * bs * bs; }
 
 void spmv_pq(const SellView &A, const double *p, double *q, dev::PcgState *st, double *partials, int iter,
-             hipStream_t s, int k, int64_t ld, int finish_op)
+             hipStream_t s, int k, int64_t ld, int finish_op, hipEvent_t after)
 {
     const int g = launch_spmv<0, true, false>(A, p, nullptr, q, nullptr, &st->stop, k, ld, partials, s);
+    if (after) LSS_HIP(hipEventRecord(after, s));
     if (g > 0) finish(g, partials, finish_op, st, nullptr, iter, s, k);
 }
 
