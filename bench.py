@@ -271,7 +271,11 @@ def main():
         mas_runs = {}
         variants = {"mas": {}, "mas+gmm": {"use_bad_dof_precond": True, "bad_dof_strategy": "gmm"},
                     "mas+elbow": {"use_bad_dof_precond": True, "bad_dof_strategy": "elbow"},
-                    "mas+kmean": {"use_bad_dof_precond": True, "bad_dof_strategy": "kmean"}}
+                    "mas+kmean": {"use_bad_dof_precond": True, "bad_dof_strategy": "kmean"},
+                    # the recurrence residual kept (no b - A x every 4 iterations)
+                    "mas/keep_r": {"true_residual_period": 0},
+                    "mas+gmm/keep_r": {"use_bad_dof_precond": True, "bad_dof_strategy": "gmm",
+                                       "true_residual_period": 0}}
         for vname, extra in variants.items():
             # MAS alone converges slowly on contact stiffness: cap the iterations
             m = {"block_dim": prob.block_size, "precision": args.mixed_precision, "max_iter": 3000}

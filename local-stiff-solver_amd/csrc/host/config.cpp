@@ -102,6 +102,7 @@ Config parse_config(const char *text, const Config &base)
                 else if (k == "precision") c.precision = as_str(v, k);
                 else if (k == "inner_spmv") c.inner_spmv = as_str(v, k);
                 else if (k == "check_every") c.check_every = int(as_int(v, k));
+                else if (k == "true_residual_period") c.true_residual_period = int(as_int(v, k));
                 else if (k == "max_dense_component") c.max_dense_component = as_int(v, k);
                 else if (k == "profile_kernels") c.profile_kernels = as_bool(v, k);
                 else if (k == "partitioner") c.partitioner = as_str(v, k);
@@ -161,6 +162,7 @@ Config validate(Config c)
     if (c.base_preconditioner != "MAS" && c.base_preconditioner != "Jacobi" && c.base_preconditioner != "AMG")
         bad("base_preconditioner must be MAS|AMG|Jacobi");
     if (c.check_every < 1) bad("check_every must be >= 1");
+    if (c.true_residual_period < 0) bad("true_residual_period must be >= 0 (0: never)");
     if (c.partitioner != "device" && c.partitioner != "host") bad("partitioner must be device|host");
     if (c.max_coarse_levels < 0 || c.max_coarse_levels > 4) bad("max_coarse_levels must be in 0..4");
     if (c.partition_local_max < 0 || c.partition_local_max > 8192) bad("partition_local_max must be in 0..8192");

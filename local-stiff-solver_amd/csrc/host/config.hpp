@@ -44,7 +44,8 @@ struct Config {
     double bad_dof_gmm_tol = 1e-3;
     int bad_dof_gmm_max_iterations = 5;
     int bad_dof_gmm_expand_neighbors = 32;
-    int true_residual_period = 4;  // MASSolver.cu:150 (a constant there)
+    // MASSolver.cu:150 (a constant there); "MAS"/"true_residual_period" sets it, 0 = never
+    int true_residual_period = 4;
     bool mas_mode() const { return solver == "MAS"; }
     // keys of this build
     std::string precision = "fp64";     // MAS bank inverses: "fp64" | "fp32" | "bf16"

@@ -274,7 +274,7 @@ void HybridSolver::run_pcg_mas(hipStream_t s)
     for (int kk = 1; kk <= cfg_.max_iter; ++kk) {
         k::spmv_pq(sell_, p_.get(), q_.get(), st, partials_.get(), kk - 1, s, 1, 0, dev::kFinishPqMas);
         k::axpy_alpha(np, 1.0, p_.get(), xp_.get(), st, s);
-        if (kk % cfg_.true_residual_period == 0)
+        if (cfg_.true_residual_period > 0 && kk % cfg_.true_residual_period == 0)
             k::spmv_residual(sell_, xp_.get(), bp_.get(), r_.get(), nullptr, s);
         else
             k::axpy_alpha(np, -1.0, q_.get(), r_.get(), st, s);

@@ -1371,7 +1371,7 @@ static int mas_solve(orc_hybrid *h, const double *b, double *xio)
         spmv(A, p.data(), Ap.data());
         const double alpha = div(rz, dot(n, p.data(), Ap.data()));
         for (int32_t i = 0; i < n; ++i) x[i] = alpha * p[i] + 1.0 * x[i];
-        if (k % c.true_residual_period == 0) {
+        if (c.true_residual_period > 0 && k % c.true_residual_period == 0) {
             spmv(A, x.data(), r.data());
             for (int32_t i = 0; i < n; ++i) r[i] = 1.0 * b[i] + -1.0 * r[i];
         } else {

@@ -148,7 +148,7 @@ typedef struct orc_config {
     int mgmm_max_iter, mgmm_expand;  /* (5, 32) */
     double km_fraction, km_jump;     /* k-means strategy (0.02, 10) */
     int km_max_iter, km_expand;      /* (32, 0) */
-    int true_residual_period;        /* 4 */
+    int true_residual_period;        /* 4; 0 = never */
     int precond_norm;                /* use_preconditioned_residual_norm */
 } orc_config;
 

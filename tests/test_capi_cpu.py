@@ -45,6 +45,8 @@ def _create(doc):
     ('{"solver": "GPUHybrid", "GPUHybrid": {"precision": "int8"}}', "precision"),
     ('{"solver": "GPUHybrid", "GPUHybrid": {"max_iter": "many"}}', "max_iter"),
     ('{"solver": "GPUHybrid", ', "JSON"),
+    ('{"solver": "MAS", "MAS": {"true_residual_period": -1}}', "true_residual_period"),
+    ('{"solver": "GPUHybrid", "GPUHybrid": {"base_preconditioner": "ILU"}}', "base_preconditioner"),
 ])
 def test_bad_config_is_rejected(doc, frag):
     rc, msg = _create(doc)

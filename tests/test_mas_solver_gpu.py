@@ -73,6 +73,30 @@ def test_mas_solver_matches_oracle(gpu, name, N, kw):
             assert st[k] == pytest.approx(ost[k], rel=1e-12), k
 
 
+@pytest.mark.parametrize("kw", STRATEGIES[:2])
+def test_mas_solver_without_residual_replacement(gpu, kw):
+    # "true_residual_period": 0 keeps the recurrence residual (MASSolver.cu:655-666 replaces
+    # it by b - A x every 4 iterations); the device loop against the oracle's same loop
+    prob = problems.make("elastic_contact", N=10)
+    A = prob.csc()
+    s = Solver.create(_dev_cfg(3, true_residual_period=0, **kw))
+    s.factorize(A)
+    H = O.Hybrid(_orc_cfg(3, true_residual_period=0, **kw))
+    H.factorize(A.colptr, A.rowidx, A.vals)
+    b = prob.rhs(2)
+    x = s.solve(b)
+    info = s.get_info()
+    its = []
+    rng = np.random.default_rng(0)
+    for q in range(5):
+        H.solve(b if q == 0 else b * (1.0 + 1e-13 * rng.standard_normal(b.size)))
+        its.append(H.info()["num_iterations"])
+    assert info["solver_status"] == "Reach relative tolerance"
+    assert min(its) - 10 <= info["solver_iter"] <= max(its) + 10, (info["solver_iter"], its)
+    S = A.to_scipy()
+    assert np.linalg.norm(S @ x - b) <= 2e-6 * np.linalg.norm(b)
+
+
 def test_mas_solver_status_and_selection(gpu):
     prob = problems.make("elastic_contact", N=8)
     A = prob.csc()
