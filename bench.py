@@ -87,7 +87,7 @@ def cpu_baseline(prob, A, b, gpu_iters: int, sample_iters: int, threads: int, fu
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="elastic_contact")
     ap.add_argument("--N", type=int, default=None, help="grid nodes per axis (default: the workload's)")
@@ -308,7 +308,8 @@ def main():
     amg_runs = None
     if args.amg:
         amg_runs = {}
-        for vname, extra in {"hybrid_amg": {}, "amg_only": {"do_hybrid_precond": False}}.items():
+        # AMG alone has no treatment of the contact stiffness: its iterations are capped
+        for vname, extra in {"hybrid_amg": {}, "amg_only": {"do_hybrid_precond": False, "max_iter": 1000}}.items():
             g = dict(paths["fp64"], base_preconditioner="AMG", **extra)
             log(vname, "start")
             s = Solver.create({"solver": "GPUHybrid", "GPUHybrid": g})

@@ -53,7 +53,8 @@ struct AmgLevelHost {
 // Random part of the PMIS measure of point i, in [0, 1).
 double amg_point_random(int64_t i);
 
-void amg_setup_host(const CsrHost &A0, const AmgOptions &opt, std::vector<AmgLevelHost> &levels);
+// A0 is moved into levels[0].A.
+void amg_setup_host(CsrHost A0, const AmgOptions &opt, std::vector<AmgLevelHost> &levels);
 
 // Pieces, exposed for the tests of the setup.
 void amg_strength(const CsrHost &A, const std::vector<int32_t> &func, int num_functions, double theta,

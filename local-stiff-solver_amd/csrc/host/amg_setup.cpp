@@ -283,65 +283,84 @@ void amg_interp_extpi(const CsrHost &A, const CsrHost &S, const std::vector<int8
 void amg_spgemm(const CsrHost &X, const CsrHost &Y, CsrHost &Z)
 {
     // Gustavson, one row at a time: products summed in the order of X's row and
-    // Y's rows, columns of the result sorted
+    // Y's rows, columns of the result sorted.  Rows are taken in chunks; a chunk's
+    // rows go to buffers of its own (no allocation per row), then every chunk is
+    // copied to its place in Z.
     const int32_t n = X.n, m = Y.m;
+    constexpr int32_t kChunk = 256;
+    const int32_t nchunks = (n + kChunk - 1) / kChunk;
     const int nt = omp_get_max_threads();
-    std::vector<std::vector<int32_t>> rc(n);
-    std::vector<std::vector<double>> rv(n);
-    std::vector<std::vector<int32_t>> posv(nt, std::vector<int32_t>(m, -1));
-#pragma omp parallel for schedule(dynamic, 512)
-    for (int32_t i = 0; i < n; ++i) {
+    std::vector<std::vector<int32_t>> ccol(nchunks);
+    std::vector<std::vector<double>> cval(nchunks);
+    Z.n = n;
+    Z.m = m;
+    Z.ptr.assign(size_t(n) + 1, 0);
+    std::vector<std::vector<int32_t>> posv(nt, std::vector<int32_t>(size_t(m), -1));
+#pragma omp parallel
+    {
         std::vector<int32_t> &pos = posv[omp_get_thread_num()];
         std::vector<int32_t> cols;
         std::vector<double> acc;
-        for (int64_t e = X.ptr[i]; e < X.ptr[i + 1]; ++e) {
-            const int32_t k = X.col[e];
-            const double a = X.val[e];
-            for (int64_t f = Y.ptr[k]; f < Y.ptr[k + 1]; ++f) {
-                const int32_t j = Y.col[f];
-                if (pos[j] < 0) {
-                    pos[j] = int32_t(cols.size());
-                    cols.push_back(j);
-                    acc.push_back(0.0);
+#pragma omp for schedule(dynamic, 1)
+        for (int32_t ch = 0; ch < nchunks; ++ch) {
+            const int32_t i0 = ch * kChunk, i1 = std::min(n, i0 + kChunk);
+            std::vector<int32_t> &oc = ccol[ch];
+            std::vector<double> &ov = cval[ch];
+            for (int32_t i = i0; i < i1; ++i) {
+                cols.clear();
+                acc.clear();
+                for (int64_t e = X.ptr[i]; e < X.ptr[i + 1]; ++e) {
+                    const int32_t k = X.col[e];
+                    const double a = X.val[e];
+                    for (int64_t f = Y.ptr[k]; f < Y.ptr[k + 1]; ++f) {
+                        const int32_t j = Y.col[f];
+                        int32_t q = pos[j];
+                        if (q < 0) {
+                            q = pos[j] = int32_t(cols.size());
+                            cols.push_back(j);
+                            acc.push_back(0.0);
+                        }
+                        acc[q] += a * Y.val[f];
+                    }
                 }
-                acc[pos[j]] += a * Y.val[f];
+                const size_t base = oc.size();
+                oc.insert(oc.end(), cols.begin(), cols.end());
+                std::sort(oc.begin() + base, oc.end());
+                for (size_t q = base; q < oc.size(); ++q) {
+                    const int32_t j = oc[q];
+                    ov.push_back(acc[pos[j]]);
+                    pos[j] = -1;
+                }
+                Z.ptr[size_t(i) + 1] = int64_t(cols.size());
             }
         }
-        std::vector<int32_t> order(cols.size());
-        std::iota(order.begin(), order.end(), 0);
-        std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return cols[a] < cols[b]; });
-        rc[i].resize(cols.size());
-        rv[i].resize(cols.size());
-        for (size_t q = 0; q < order.size(); ++q) {
-            rc[i][q] = cols[order[q]];
-            rv[i][q] = acc[order[q]];
-            pos[cols[order[q]]] = -1;
-        }
     }
-    Z.n = n;
-    Z.m = m;
-    Z.ptr.assign(n + 1, 0);
-    for (int32_t i = 0; i < n; ++i) Z.ptr[i + 1] = Z.ptr[i] + int64_t(rc[i].size());
-    Z.col.resize(Z.ptr[n]);
-    Z.val.resize(Z.ptr[n]);
-#pragma omp parallel for schedule(static)
-    for (int32_t i = 0; i < n; ++i) {
-        std::copy(rc[i].begin(), rc[i].end(), Z.col.begin() + Z.ptr[i]);
-        std::copy(rv[i].begin(), rv[i].end(), Z.val.begin() + Z.ptr[i]);
+    for (int32_t i = 0; i < n; ++i) Z.ptr[size_t(i) + 1] += Z.ptr[i];
+    Z.col.resize(size_t(Z.ptr[n]));
+    Z.val.resize(size_t(Z.ptr[n]));
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int32_t ch = 0; ch < nchunks; ++ch) {
+        const int64_t o = Z.ptr[size_t(ch) * kChunk];
+        std::copy(ccol[ch].begin(), ccol[ch].end(), Z.col.begin() + o);
+        std::copy(cval[ch].begin(), cval[ch].end(), Z.val.begin() + o);
+        std::vector<int32_t>().swap(ccol[ch]);
+        std::vector<double>().swap(cval[ch]);
     }
 }
 
-void amg_setup_host(const CsrHost &A0, const AmgOptions &opt, std::vector<AmgLevelHost> &levels)
+void amg_setup_host(CsrHost A0, const AmgOptions &opt, std::vector<AmgLevelHost> &levels)
 {
     levels.clear();
     levels.emplace_back();
-    levels[0].A = A0;
-    levels[0].func.resize(A0.n);
-    for (int32_t i = 0; i < A0.n; ++i) levels[0].func[i] = i % std::max(opt.num_functions, 1);
+    const int32_t n0 = A0.n;
+    levels[0].A = std::move(A0);
+    levels[0].func.resize(n0);
+    for (int32_t i = 0; i < n0; ++i) levels[0].func[i] = i % std::max(opt.num_functions, 1);
     for (int lv = 0;; ++lv) {
         AmgLevelHost &L = levels[lv];
         const int32_t n = L.A.n;
         L.dinv.resize(n);
+#pragma omp parallel for schedule(static)
         for (int32_t i = 0; i < n; ++i) {
             double s = 0.0;
             for (int64_t e = L.A.ptr[i]; e < L.A.ptr[i + 1]; ++e) s += std::fabs(L.A.val[e]);

@@ -708,10 +708,10 @@ void HybridSolver::build_amg(int32_t n, const int32_t *ptr, const int32_t *idx, 
     opt.num_functions = bs_;
     opt.theta = bs_ > 1 ? cfg_.theta : 0.25;
     std::vector<AmgLevelHost> lv;
-    amg_setup_host(A0, opt, lv);
-    A0 = CsrHost{};
+    amg_setup_host(std::move(A0), opt, lv);
     auto up = [&](const CsrHost &h, DevCsr &d) {
         d.n = h.n;
+        d.nnz = int64_t(h.col.size());
         d.ptr.upload(h.ptr.data(), h.ptr.size(), s);
         d.col.upload(h.col.data(), std::max<size_t>(h.col.size(), 1), s);
         d.val.upload(h.val.data(), std::max<size_t>(h.val.size(), 1), s);
@@ -773,15 +773,18 @@ void HybridSolver::amg_vcycle(const double *b0, double *x0, hipStream_t s)
         const int64_t m = nl == 1 ? np : C.n;
         k::amg_level_smooth_pre(int32_t(m), C.dinv.get(), bvec(nl - 1), xvec(nl - 1), s);
     }
+    // the post-smoothed vector of a coarse level is written to its t (the smoother reads
+    // x while it writes), and the prolongation to the next finer level reads it there
+    const double *xc = xvec(nl - 1);
     for (int l = nl - 2; l >= 0; --l) {
         AmgLevel &L = amg_[l];
-        k::amg_spmv_add(L.P.view(), amg_[l + 1].x.get(), xvec(l), s);
+        k::amg_spmv_add(L.P.view(), xc, xvec(l), s);
         if (l == 0) {
             k::spmv_residual(sell_, xvec(0), bvec(0), L.t.get(), nullptr, s);
             k::amg_add_diag_scaled(np, L.dinv.get(), L.t.get(), xvec(0), s);
         } else {
             k::amg_smooth_post(L.A.view(), L.dinv.get(), xvec(l), bvec(l), L.t.get(), s);
-            LSS_HIP(hipMemcpyAsync(xvec(l), L.t.get(), size_t(L.n) * sizeof(double), hipMemcpyDeviceToDevice, s));
+            xc = L.t.get();
         }
     }
 }

@@ -156,10 +156,11 @@ private:
     DBuf<double> d_dinv_;
     struct DevCsr {
         int32_t n = 0;
+        int64_t nnz = 0;
         DBuf<int64_t> ptr;
         DBuf<int32_t> col;
         DBuf<double> val;
-        k::AmgCsr view() const { return k::AmgCsr{n, ptr.get(), col.get(), val.get()}; }
+        k::AmgCsr view() const { return k::AmgCsr{n, ptr.get(), col.get(), val.get(), nnz}; }
     };
     struct AmgLevel {
         int32_t n = 0;

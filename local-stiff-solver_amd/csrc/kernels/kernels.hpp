@@ -160,6 +160,7 @@ struct AmgCsr {
     const int64_t *ptr = nullptr;
     const int32_t *col = nullptr;
     const double *val = nullptr;
+    int64_t nnz = 0;
 };
 // x = dinv .* b (one l1-Jacobi sweep from zero)
 void amg_level_smooth_pre(int32_t n, const double *dinv, const double *b, double *x, hipStream_t s);
