@@ -330,7 +330,8 @@ def main():
             amg_runs[vname] = {"solver": ai["solver"], "s": round(dt, 4), "phases_s": ai["timings"],
                                "amg_setup_s": ai["timings"]["mas"], "iterations": ai["num_iterations"],
                                "stop_reason": ai["stop_reason"], "true_rel_residual_dd": rdd / bnorm,
-                               "levels": ai["amg"]["rows"], "level_nnz": ai["amg"]["nnz"]}
+                               "levels": ai["amg"]["rows"], "level_nnz": ai["amg"]["nnz"],
+                               "setup_phases_s": ai["amg"]["setup_s"]}
             log(vname, amg_runs[vname])
             del s
     # roofline of the outer FP64 SpMV, timed with HIP events on the solver's stream

@@ -13,7 +13,6 @@
 #include <thread>
 
 #include "../kernels/util_kernels.hpp"
-#include "amg.hpp"
 #include "topology.hpp"
 
 namespace lss {
@@ -130,6 +129,8 @@ void HybridSolver::reset_factorization()
 void HybridSolver::release()
 {
     factorized_ = false;
+    amg_.clear();
+    amg_work_ = k::AmgWork{};
     mas_.clear();
     spare_inv0_.release();
     comps_.clear();
@@ -248,6 +249,7 @@ void HybridSolver::factorize_device(int32_t n, int64_t nnz, const int32_t *ptr, 
         t_operator_ = since(t) - t_partition_;
 
         t = clk::now();
+        if (cfg_.base_preconditioner != "AMG") amg_work_ = k::AmgWork{};
         if (cfg_.base_preconditioner == "MAS") {
             build_mas(s);
         } else if (cfg_.base_preconditioner == "AMG") {
@@ -436,7 +438,7 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     const bool dev_scan = dev_topo && nnz + nb_ < (int64_t(1) << 31);
     Pattern g;
     g.n = nb_;
-    DBuf<int32_t> d_rp0(size_t(nb_) + 1), tsum(std::max(div_up(nb_, 1024), 1));
+    DBuf<int32_t> d_rp0(static_cast<size_t>(nb_) + 1), tsum(std::max(div_up(nb_, 1024), 1));
     std::vector<int32_t> hc;
     if (dev_scan) {
         k::excl_scan_i32(nb_, cnt.get(), d_rp0.get(), tsum.get(), d_rp0.get() + nb_, s);
@@ -688,63 +690,80 @@ void HybridSolver::build_mas(hipStream_t s)
 
 void HybridSolver::build_amg(int32_t n, const int32_t *ptr, const int32_t *idx, const double *vals, hipStream_t s)
 {
-    // BoomerAMG setup (GPUHybrid.cu:357-376 with the options of 285-339): the hierarchy
-    // is built on the host from the operator rows (CSC column j = row j), then every
-    // level's interpolation, restriction, operator and l1 norms go to the device
-    CsrHost A0;
-    A0.n = A0.m = n;
-    {
-        std::vector<int32_t> p32(size_t(n) + 1);
-        LSS_HIP(hipMemcpyAsync(p32.data(), ptr, (size_t(n) + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        sync(s);
-        A0.ptr.assign(p32.begin(), p32.end());
-        A0.col.resize(size_t(nnz_));
-        A0.val.resize(size_t(nnz_));
-        LSS_HIP(hipMemcpyAsync(A0.col.data(), idx, size_t(nnz_) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        LSS_HIP(hipMemcpyAsync(A0.val.data(), vals, size_t(nnz_) * sizeof(double), hipMemcpyDeviceToHost, s));
-        sync(s);
-    }
-    AmgOptions opt;
-    opt.num_functions = bs_;
-    opt.theta = bs_ > 1 ? cfg_.theta : 0.25;
-    std::vector<AmgLevelHost> lv;
-    amg_setup_host(std::move(A0), opt, lv);
-    auto up = [&](const CsrHost &h, DevCsr &d) {
-        d.n = h.n;
-        d.nnz = int64_t(h.col.size());
-        d.ptr.upload(h.ptr.data(), h.ptr.size(), s);
-        d.col.upload(h.col.data(), std::max<size_t>(h.col.size(), 1), s);
-        d.val.upload(h.val.data(), std::max<size_t>(h.val.size(), 1), s);
-    };
+    // BoomerAMG setup (GPUHybrid.cu:357-376 with the options of 285-339), on the device:
+    // per level the l1 norms, strength, PMIS splitting, extended+i interpolation
+    // (Pmax 4), R = P^T and the Galerkin operator R (A P), down to at most 5 points or
+    // 25 levels, or until a splitting gives no C point or only C points.  Level 0 is
+    // the input itself (CSC column j = operator row j, as hypre is given it).
+    constexpr int kMaxLevels = 25, kMinCoarse = 5, kPmax = 4;
+    constexpr double kMaxRowSum = 0.9;
+    const double theta = bs_ > 1 ? cfg_.theta : 0.25;
+    const int nf = bs_;
     const int64_t np = int64_t(nb_) * bs_;
+    amg_times_ = AmgSetupTimes{};
+    auto t0 = clk::now();
+    auto lap = [&](double &acc) {
+        sync(s);
+        acc += since(t0);
+        t0 = clk::now();
+    };
+    DBuf<int64_t> p0(static_cast<size_t>(n) + 1);
+    k::amg_widen_ptr(int64_t(n) + 1, ptr, p0.get(), s);
+    const k::AmgCsr A0{n, p0.get(), idx, vals, nnz_};
+    DBuf<int32_t> func;
+    if (nf > 1) {
+        std::vector<int32_t> f0(static_cast<size_t>(n));
+        for (int32_t i = 0; i < n; ++i) f0[i] = i % nf;
+        func.upload(f0.data(), f0.size(), s);
+    }
     amg_.clear();
-    amg_.resize(lv.size());
     amg_cf_.clear();
     amg_sizes_.clear();
     amg_nnz_.clear();
-    for (size_t l = 0; l < lv.size(); ++l) {
+    amg_.reserve(kMaxLevels);
+    amg_.emplace_back();
+    for (int l = 0;; ++l) {
+        const k::AmgCsr A = l == 0 ? A0 : amg_[l].A.view();
         AmgLevel &L = amg_[l];
-        const AmgLevelHost &H = lv[l];
-        L.n = H.A.n;
-        amg_sizes_.push_back(H.A.n);
-        amg_nnz_.push_back(int64_t(H.A.col.size()));
-        if (l > 0) up(H.A, L.A);
-        amg_cf_.push_back(H.cf);
-        if (!H.P.ptr.empty()) {
-            up(H.P, L.P);
-            up(H.R, L.R);
-        }
-        // level 0 works on the padded vectors of the sliced operator: its tail
-        // rows have a unit diagonal, l1 norm 1
-        std::vector<double> dinv = H.dinv;
-        if (l == 0) dinv.resize(size_t(np), 1.0);
-        L.dinv.upload(dinv.data(), dinv.size(), s);
-        const size_t m = l == 0 ? size_t(np) : size_t(std::max(H.A.n, 1));
+        const int32_t m = A.n;
+        L.n = m;
+        amg_sizes_.push_back(m);
+        amg_nnz_.push_back(A.nnz);
+        // level 0 works on the padded vectors of the sliced operator: its tail rows
+        // have a unit diagonal, l1 norm 1
+        const size_t len = l == 0 ? size_t(np) : size_t(std::max(m, 1));
+        L.dinv.resize(len);
+        k::amg_l1_inverse(A, L.dinv.get(), s);
+        if (l == 0 && np > n) k::fill(np - n, L.dinv.get() + n, 1.0, s);
         if (l > 0) {
-            L.b.resize(m);
-            L.x.resize(m);
+            L.b.resize(len);
+            L.x.resize(len);
         }
-        L.t.resize(m);
+        L.t.resize(len);
+        lap(amg_times_.norms);
+        amg_cf_.emplace_back();
+        if (l + 1 >= kMaxLevels || m <= kMinCoarse) break;
+        k::AmgCsrBuf S;
+        k::amg_strength_device(A, nf > 1 ? func.get() : nullptr, nf, theta, kMaxRowSum, S, amg_work_, s);
+        lap(amg_times_.strength);
+        DBuf<int8_t> cf(static_cast<size_t>(std::max(m, 1)));
+        const int32_t nc = k::amg_pmis_device(S.view(), cf.get(), amg_work_, s);
+        lap(amg_times_.pmis);
+        if (nc == 0 || nc == m) break;
+        amg_cf_.back().resize(size_t(m));
+        cf.download(amg_cf_.back().data(), size_t(m), s);
+        DBuf<int32_t> cfunc(static_cast<size_t>(nc));
+        k::amg_interp_device(A, S.view(), cf.get(), nf > 1 ? func.get() : nullptr, nf, kPmax, L.P,
+                             nf > 1 ? cfunc.get() : nullptr, amg_work_, s);
+        lap(amg_times_.interp);
+        k::amg_transpose_device(L.P.view(), nc, L.R, amg_work_, s);
+        lap(amg_times_.transpose);
+        k::AmgCsrBuf AP;
+        k::amg_spgemm_device(A, L.P.view(), nc, AP, amg_work_, s);
+        amg_.emplace_back();  // L is not used past this point (the vector may have moved)
+        k::amg_spgemm_device(amg_[l].R.view(), AP.view(), nc, amg_[l + 1].A, amg_work_, s);
+        lap(amg_times_.rap);
+        func = std::move(cfunc);
     }
     amg_z_.resize(size_t(np));
     sync(s);
@@ -1458,18 +1477,19 @@ void HybridSolver::debug_amg(int32_t level, int32_t *nlevels, int32_t *n, int64_
     }
     const AmgLevel &L = amg_[level];
     if (n) *n = L.n;
-    if (nnz_a) *nnz_a = level == 0 ? nnz_ : int64_t(L.A.col.size());
-    if (nnz_p) *nnz_p = L.P.ptr ? int64_t(L.P.col.size()) : 0;
+    const bool has_p = L.P.ptr.get() != nullptr;
+    if (nnz_a) *nnz_a = level == 0 ? nnz_ : L.A.nnz;
+    if (nnz_p) *nnz_p = has_p ? L.P.nnz : 0;
     if (cf && !amg_cf_[level].empty()) std::memcpy(cf, amg_cf_[level].data(), amg_cf_[level].size());
     if (level > 0 && a_ptr) {
         L.A.ptr.download(a_ptr, size_t(L.n) + 1, stream_);
-        L.A.col.download(a_col, L.A.col.size(), stream_);
-        L.A.val.download(a_val, L.A.val.size(), stream_);
+        L.A.col.download(a_col, size_t(L.A.nnz), stream_);
+        L.A.val.download(a_val, size_t(L.A.nnz), stream_);
     }
-    if (L.P.ptr && p_ptr) {
+    if (has_p && p_ptr) {
         L.P.ptr.download(p_ptr, size_t(L.n) + 1, stream_);
-        L.P.col.download(p_col, L.P.col.size(), stream_);
-        L.P.val.download(p_val, L.P.val.size(), stream_);
+        L.P.col.download(p_col, size_t(L.P.nnz), stream_);
+        L.P.val.download(p_val, size_t(L.P.nnz), stream_);
     }
     sync(stream_);
 }
@@ -1549,7 +1569,9 @@ std::string HybridSolver::info_json() const
     for (size_t l = 0; l < amg_sizes_.size(); ++l) o << (l ? "," : "") << amg_sizes_[l];
     o << "],\"nnz\":[";
     for (size_t l = 0; l < amg_nnz_.size(); ++l) o << (l ? "," : "") << amg_nnz_[l];
-    o << "]},\"kernel_ms\":{\"spmv_outer\":" << prof_spmv_outer_ms_ << ",\"spmv_outer_n\":" << prof_spmv_outer_n_
+    o << "],\"setup_s\":{\"norms\":" << amg_times_.norms << ",\"strength\":" << amg_times_.strength
+      << ",\"pmis\":" << amg_times_.pmis << ",\"interp\":" << amg_times_.interp << ",\"transpose\":"
+      << amg_times_.transpose << ",\"rap\":" << amg_times_.rap << "}},\"kernel_ms\":{\"spmv_outer\":" << prof_spmv_outer_ms_ << ",\"spmv_outer_n\":" << prof_spmv_outer_n_
       << ",\"spmv_inner\":" << prof_spmv_inner_ms_ << ",\"spmv_inner_n\":" << prof_spmv_inner_n_
       << ",\"mas_level0\":" << prof_mas0_ms_ << ",\"mas_level0_n\":" << prof_mas0_n_ << "}";
     if (cfg_.mas_mode()) {

@@ -154,23 +154,19 @@ private:
     DBuf<uint8_t> spare_inv0_;  // the previous factorization's level-0 factor buffer, for reuse
     std::vector<int32_t> level_nodes_;
     DBuf<double> d_dinv_;
-    struct DevCsr {
-        int32_t n = 0;
-        int64_t nnz = 0;
-        DBuf<int64_t> ptr;
-        DBuf<int32_t> col;
-        DBuf<double> val;
-        k::AmgCsr view() const { return k::AmgCsr{n, ptr.get(), col.get(), val.get(), nnz}; }
-    };
     struct AmgLevel {
         int32_t n = 0;
-        DevCsr A, P, R;           // A: levels >= 1 (level 0 uses the sliced operator)
+        k::AmgCsrBuf A, P, R;  // A: levels >= 1 (level 0 uses the sliced operator)
         DBuf<double> dinv, b, x, t;
     };
     std::vector<AmgLevel> amg_;
+    k::AmgWork amg_work_;  // setup scratch, kept for refactorizations
     std::vector<std::vector<int8_t>> amg_cf_;  // C/F splitting per level (host, inspection)
     std::vector<int32_t> amg_sizes_;
     std::vector<int64_t> amg_nnz_;
+    struct AmgSetupTimes {
+        double norms = 0, strength = 0, pmis = 0, interp = 0, transpose = 0, rap = 0;
+    } amg_times_;
     DBuf<double> amg_z_;  // V-cycle output before scaling
 
     // stiff blocks

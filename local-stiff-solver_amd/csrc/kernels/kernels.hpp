@@ -162,6 +162,43 @@ struct AmgCsr {
     const double *val = nullptr;
     int64_t nnz = 0;
 };
+// Owning scalar CSR on the device (n x m, nnz entries, columns ascending per row).
+struct AmgCsrBuf {
+    int32_t n = 0, m = 0;
+    int64_t nnz = 0;
+    DBuf<int64_t> ptr;
+    DBuf<int32_t> col;
+    DBuf<double> val;
+    AmgCsr view() const { return AmgCsr{n, ptr.get(), col.get(), val.get(), nnz}; }
+};
+// Scratch of the AMG setup: grows to the largest product of a setup and is kept
+// (a refactorization reuses it without new allocations).
+struct AmgWork {
+    DBuf<uint64_t> key, key2;
+    DBuf<double> val, val2;
+    DBuf<uint8_t> tmp;
+    DBuf<int32_t> head, uidx, row;
+    DBuf<int64_t> cnt, off;
+};
+// AMG setup on the device (amg_setup.hip).  All of them wait for their work.
+// p64[0..count) = p32[0..count)
+void amg_widen_ptr(int64_t count, const int32_t *p32, int64_t *p64, hipStream_t s);
+// dinv_i = 1 / sum_j |a_ij| (0 for an empty row)
+void amg_l1_inverse(const AmgCsr &A, double *dinv, hipStream_t s);
+// classical strength of connection (pattern only) with the "unknown" approach for nf > 1
+void amg_strength_device(const AmgCsr &A, const int32_t *func, int nf, double theta, double max_row_sum,
+                         AmgCsrBuf &S, AmgWork &w, hipStream_t s);
+// PMIS splitting: cf = 1 (C), -1 (F), -3 (no strong connection); returns the C count
+int32_t amg_pmis_device(const AmgCsr &S, int8_t *cf, AmgWork &w, hipStream_t s);
+// extended+i interpolation truncated to pmax entries (n x C count); cfunc gets the
+// functions of the C points in coarse order
+void amg_interp_device(const AmgCsr &A, const AmgCsr &S, const int8_t *cf, const int32_t *func, int nf, int pmax,
+                       AmgCsrBuf &P, int32_t *cfunc, AmgWork &w, hipStream_t s);
+// R = P^T (P has m columns); a row of R lists the rows of P in ascending order
+void amg_transpose_device(const AmgCsr &P, int32_t m, AmgCsrBuf &R, AmgWork &w, hipStream_t s);
+// Z = X Y (Y has m columns): each entry the sum from 0.0 of its products in the
+// order of X's row and Y's rows
+void amg_spgemm_device(const AmgCsr &X, const AmgCsr &Y, int32_t m, AmgCsrBuf &Z, AmgWork &w, hipStream_t s);
 // x = dinv .* b (one l1-Jacobi sweep from zero)
 void amg_level_smooth_pre(int32_t n, const double *dinv, const double *b, double *x, hipStream_t s);
 // x += dinv .* t

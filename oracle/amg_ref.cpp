@@ -1,4 +1,6 @@
-#include "amg.hpp"
+#include "amg_ref.hpp"
+
+#include <chrono>
 
 #include <omp.h>
 
@@ -10,7 +12,7 @@
 // then reproducible by any IEEE implementation of the same operation order.
 #pragma clang fp contract(off)
 
-namespace lss {
+namespace amgref {
 
 double amg_point_random(int64_t i)
 {
@@ -348,8 +350,16 @@ void amg_spgemm(const CsrHost &X, const CsrHost &Y, CsrHost &Z)
     }
 }
 
-void amg_setup_host(CsrHost A0, const AmgOptions &opt, std::vector<AmgLevelHost> &levels)
+void amg_setup_host(CsrHost A0, const AmgOptions &opt, std::vector<AmgLevelHost> &levels, AmgSetupTimes *times)
 {
+    AmgSetupTimes tm;
+    using clk = std::chrono::steady_clock;
+    auto t = clk::now();
+    auto lap = [&](double &acc) {
+        const auto u = clk::now();
+        acc += std::chrono::duration<double>(u - t).count();
+        t = u;
+    };
     levels.clear();
     levels.emplace_back();
     const int32_t n0 = A0.n;
@@ -366,10 +376,13 @@ void amg_setup_host(CsrHost A0, const AmgOptions &opt, std::vector<AmgLevelHost>
             for (int64_t e = L.A.ptr[i]; e < L.A.ptr[i + 1]; ++e) s += std::fabs(L.A.val[e]);
             L.dinv[i] = s > 0.0 ? 1.0 / s : 0.0;
         }
+        lap(tm.norms);
         if (lv + 1 >= opt.max_levels || n <= opt.min_coarse) break;
         CsrHost S;
         amg_strength(L.A, L.func, opt.num_functions, opt.theta, opt.max_row_sum, S);
+        lap(tm.strength);
         amg_pmis(S, L.cf);
+        lap(tm.pmis);
         int32_t nc = 0;
         for (int8_t c : L.cf) nc += c == 1;
         if (nc == 0 || nc == n) {
@@ -377,16 +390,20 @@ void amg_setup_host(CsrHost A0, const AmgOptions &opt, std::vector<AmgLevelHost>
             break;
         }
         amg_interp_extpi(L.A, S, L.cf, L.func, opt.num_functions, opt.pmax, L.P);
+        lap(tm.interp);
         amg_transpose(L.P, L.R);
+        lap(tm.transpose);
         AmgLevelHost next;
         CsrHost AP;
         amg_spgemm(L.A, L.P, AP);
         amg_spgemm(L.R, AP, next.A);
+        lap(tm.rap);
         next.func.resize(nc);
         for (int32_t i = 0, c = 0; i < n; ++i)
             if (L.cf[i] == 1) next.func[c++] = L.func[i];
         levels.push_back(std::move(next));
     }
+    if (times) *times = tm;
 }
 
-}  // namespace lss
+}  // namespace amgref

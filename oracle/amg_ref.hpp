@@ -1,4 +1,8 @@
-// Algebraic multigrid setup of the GPU AMG base preconditioner.
+// ORACLE — test infrastructure only (built by the tests, never linked into the
+// library).  Sequential C++ restatement of the AMG setup of the GPU AMG base
+// preconditioner, bit-identical to oracle/amg.py (same operation order, no
+// contraction of products into sums); the device setup is compared with it at
+// sizes the numpy restatement would take minutes on.
 //
 // The reference's GPUHybrid preconditions with hypre's BoomerAMG (v3.1.0,
 // cmake/recipes/hypre.cmake; options SRC/GPUHybrid.cu:285-339): PMIS coarsening
@@ -14,8 +18,8 @@
 //  * PMIS: De Sterck, Yang, Heys, SIAM J. Matrix Anal. Appl. 27 (2006) 1019,
 //  * extended+i: De Sterck, Falgout, Nolting, Yang, Numer. Linear Algebra Appl.
 //    15 (2008) 115 (same-function weak entries lumped into the diagonal),
-//  * truncation: the Pmax largest weights, positive and negative kept entries
-//    rescaled to their row's positive and negative sums,
+//  * truncation: the Pmax largest |weights| (ties: the smaller point), scaled by
+//    one factor that keeps the row sum,
 //  * l1-Jacobi: D_ii = sum_j |a_ij| (Baker, Falgout, Kolev, Yang 2011).
 // The random part of PMIS's measure is a hash of the point index (hypre draws it
 // from a generator), so the splitting is reproducible.
@@ -23,7 +27,7 @@
 #include <cstdint>
 #include <vector>
 
-namespace lss {
+namespace amgref {
 
 // Scalar CSR with ascending columns per row.
 struct CsrHost {
@@ -53,8 +57,14 @@ struct AmgLevelHost {
 // Random part of the PMIS measure of point i, in [0, 1).
 double amg_point_random(int64_t i);
 
+// Seconds spent in each step of the setup, summed over the levels.
+struct AmgSetupTimes {
+    double strength = 0, pmis = 0, interp = 0, transpose = 0, rap = 0, norms = 0;
+};
+
 // A0 is moved into levels[0].A.
-void amg_setup_host(CsrHost A0, const AmgOptions &opt, std::vector<AmgLevelHost> &levels);
+void amg_setup_host(CsrHost A0, const AmgOptions &opt, std::vector<AmgLevelHost> &levels,
+                    AmgSetupTimes *times = nullptr);
 
 // Pieces, exposed for the tests of the setup.
 void amg_strength(const CsrHost &A, const std::vector<int32_t> &func, int num_functions, double theta,
@@ -65,4 +75,4 @@ void amg_interp_extpi(const CsrHost &A, const CsrHost &S, const std::vector<int8
 void amg_transpose(const CsrHost &P, CsrHost &R);
 void amg_spgemm(const CsrHost &X, const CsrHost &Y, CsrHost &Z);
 
-}  // namespace lss
+}  // namespace amgref

@@ -1,8 +1,7 @@
-// Test shim: runs the library's host AMG setup (local-stiff-solver_amd/csrc/host/
-// amg_setup.cpp) on a CSR matrix and writes every level (C/F splitting, operator,
-// interpolation) to a binary file, so that the CPU tests compare it with the
-// restatement without a GPU.  Built by tests/test_amg_cpu.py.
-#include "amg.hpp"
+// Test shim: runs the C++ AMG restatement (oracle/amg_ref.cpp) on a CSR matrix and
+// writes every level (C/F splitting, operator, interpolation) to a binary file for
+// the tests (built by tests/amg_ref.py).
+#include "amg_ref.hpp"
 
 #include <cstdio>
 #include <vector>
@@ -19,16 +18,16 @@ template <class T> void put(FILE *fp, const std::vector<T> &v)
 extern "C" int amg_shim_setup(int32_t n, const int64_t *ptr, const int32_t *col, const double *val, int32_t bs,
                               double theta, const char *path)
 {
-    lss::CsrHost A;
+    amgref::CsrHost A;
     A.n = A.m = n;
     A.ptr.assign(ptr, ptr + n + 1);
     A.col.assign(col, col + ptr[n]);
     A.val.assign(val, val + ptr[n]);
-    lss::AmgOptions opt;
+    amgref::AmgOptions opt;
     opt.num_functions = bs;
     opt.theta = theta;
-    std::vector<lss::AmgLevelHost> levels;
-    lss::amg_setup_host(A, opt, levels);
+    std::vector<amgref::AmgLevelHost> levels;
+    amgref::amg_setup_host(A, opt, levels);
     FILE *fp = std::fopen(path, "wb");
     if (!fp) return 1;
     const int64_t nl = int64_t(levels.size());

@@ -229,59 +229,16 @@ def test_vcycle_spd_and_convergent(which):
     assert rates[-1] < 0.7, rates
 
 
-# ---------------------------------------------------------------- library host setup
+# ---------------------------------------------------------------- C++ restatement
 @pytest.fixture(scope="module")
 def host_setup(tmp_path_factory):
-    """The library's host setup (amg_setup.cpp) built with a test shim."""
-    import ctypes
-    import os
-    import shutil
-    import subprocess
+    """oracle/amg_ref.cpp (C++ restatement) built with a test shim."""
+    import amg_ref
 
-    here = os.path.dirname(os.path.abspath(__file__))
-    src = os.path.join(here, "..", "local-stiff-solver_amd", "csrc", "host")
-    cxx = shutil.which("g++")
-    if cxx is None:
+    try:
+        return amg_ref.build(str(tmp_path_factory.mktemp("amgref")))
+    except FileNotFoundError:
         pytest.skip("no host C++ compiler")
-    d = tmp_path_factory.mktemp("amgshim")
-    so = str(d / "amgshim.so")
-    subprocess.run([cxx, "-O2", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-Wno-unknown-pragmas", "-shared",
-                    "-fPIC", "-I", src, os.path.join(here, "amg_setup_shim.cpp"), os.path.join(src, "amg_setup.cpp"),
-                    "-o", so], check=True)
-    lib = ctypes.CDLL(so)
-    P = ctypes.c_void_p
-    lib.amg_shim_setup.argtypes = [ctypes.c_int32, P, P, P, ctypes.c_int32, ctypes.c_double, ctypes.c_char_p]
-    lib.amg_shim_setup.restype = ctypes.c_int
-
-    def run(A, bs, theta):
-        import struct
-
-        A = sp.csr_matrix(A)
-        A.sort_indices()
-        ptr = A.indptr.astype(np.int64)
-        col = A.indices.astype(np.int32)
-        val = np.ascontiguousarray(A.data, dtype=np.float64)
-        out = str(d / "levels.bin")
-        rc = lib.amg_shim_setup(A.shape[0], ptr.ctypes.data, col.ctypes.data, val.ctypes.data, bs, theta,
-                                out.encode())
-        assert rc == 0
-        levels = []
-        with open(out, "rb") as f:
-            def vec(dt):
-                k = struct.unpack("q", f.read(8))[0]
-                return np.frombuffer(f.read(k * np.dtype(dt).itemsize), dt)
-
-            nl = struct.unpack("q", f.read(8))[0]
-            for _ in range(nl):
-                cf, ap, ac, av = vec(np.int8), vec(np.int64), vec(np.int32), vec(np.float64)
-                pp, pc, pv = vec(np.int64), vec(np.int32), vec(np.float64)
-                m = struct.unpack("q", f.read(8))[0]
-                n = ap.size - 1
-                levels.append({"cf": cf, "A": sp.csr_matrix((av, ac, ap), shape=(n, n)),
-                               "P": sp.csr_matrix((pv, pc, pp), shape=(n, m)) if pp.size else None})
-        return levels
-
-    return run
 
 
 def _grid_elasticity(n):
@@ -292,7 +249,7 @@ def _grid_elasticity(n):
 
 
 @pytest.mark.parametrize("which", ["lap2d", "aniso", "lap9", "block3"])
-def test_host_setup_bit_exact_with_restatement(host_setup, which):
+def test_cpp_restatement_bit_exact_with_numpy(host_setup, which):
     bs, theta = 1, 0.25
     if which == "lap2d":
         A = _lap2d(30, 27)

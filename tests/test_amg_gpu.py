@@ -97,6 +97,34 @@ def test_hierarchy_matches_restatement(gpu, name, N):
     _log(name, N, "levels", [a.shape[0] for a in As], "nnz", info["amg"]["nnz"])
 
 
+@pytest.fixture(scope="module")
+def cpp_ref(tmp_path_factory):
+    import amg_ref
+
+    return amg_ref.build(str(tmp_path_factory.mktemp("amgref")))
+
+
+@pytest.mark.parametrize("name,N", [("laplace_sphere", 32), ("elastic_contact", 20), ("elastic_slivers", 24),
+                                    ("elastic_logE", 20)])
+def test_hierarchy_matches_cpp_restatement_at_size(gpu, cpp_ref, name, N):
+    # the C++ restatement (bit-identical to oracle/amg.py) at sizes the numpy one is slow at
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    bs = prob.block_size
+    s = Solver.create(_cfg(bs, do_hybrid_precond=False))  # the hierarchy alone
+    s.factorize(A)
+    lv = s.amg_levels()
+    ref = cpp_ref(_rows(A), bs, 0.5 if bs > 1 else 0.25)
+    assert [d["n"] for d in lv] == [r["A"].shape[0] for r in ref]
+    for l in range(len(lv)):
+        if l:
+            assert abs(lv[l]["A"] - ref[l]["A"]).max() == 0 and lv[l]["A"].nnz == ref[l]["A"].nnz, l
+        if l + 1 < len(lv):
+            assert np.array_equal(lv[l]["cf"], ref[l]["cf"]), l
+            assert (lv[l]["P"] != ref[l]["P"]).nnz == 0 and lv[l]["P"].nnz == ref[l]["P"].nnz, l
+    _log(name, N, "levels", [d["n"] for d in lv], "setup", s.get_info()["amg"]["setup_s"])
+
+
 @pytest.mark.parametrize("name,N", CASES)
 def test_vcycle_matches_restatement(gpu, name, N):
     prob = problems.make(name, N=N)
