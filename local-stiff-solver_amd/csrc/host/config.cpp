@@ -71,6 +71,7 @@ Config parse_config(const char *text, const Config &base)
                 c.relative_tolerance = 1e-6;
                 c.absolute_tolerance = 1e-20;
                 c.block_size = 3;
+                c.lazy_partitioning = true;  // linear-solver-spec.json:776-781
             }
         }
     }
@@ -147,6 +148,7 @@ Config parse_config(const char *text, const Config &base)
         else if (k == "partitioner") c.partitioner = as_str(v, k);
         else if (k == "max_coarse_levels") c.max_coarse_levels = int(as_int(v, k));
         else if (k == "partition_local_max") c.partition_local_max = int(as_int(v, k));
+        else if (k == "lazy_partitioning") c.lazy_partitioning = as_bool(v, k);
         else bad("unknown key GPUHybrid/" + k);
     }
     return validate(c);

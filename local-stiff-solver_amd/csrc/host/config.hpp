@@ -32,7 +32,9 @@ struct Config {
     // keys of the "MAS" solver section (MASSolver.cu:148-172, 226-279); with solver
     // "MAS", max_iter / relative_tolerance / absolute_tolerance / block_size (its
     // "block_dim") take that solver's defaults 100000 / 1e-6 / 1e-20 / 3
-    bool lazy_partitioning = false;  // accepted; the partition is rebuilt by every factorize
+    // reuse the first MAS partition while the block count is unchanged (MASSolver.cu:389-405;
+    // default true for "MAS" as linear-solver-spec.json:776-781, false for GPUHybrid)
+    bool lazy_partitioning = false;
     bool use_preconditioned_residual_norm = false;
     bool use_bad_dof_precond = false;
     std::string bad_dof_strategy = "elbow";  // "elbow" | "kmean" | "gmm" (other strings: elbow)

@@ -480,10 +480,16 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     }
     sync(s);
 
-    // partition (MAS banks) and permutation
+    // partition (MAS banks) and permutation; with lazy_partitioning the first partition is
+    // kept while the block count is unchanged (MASSolver.cu:389-405, which does not compare
+    // patterns either: the banks are a grouping of block rows, valid for any pattern)
     auto tp = clk::now();
     DBuf<int32_t> d_off;
-    if (dev_topo) {
+    const bool reuse = mas && cfg_.lazy_partitioning && perm_.size() == size_t(nb_) && part_off_.size() > 1 &&
+                       d_perm_.size() == size_t(nb_);
+    if (reuse) {
+        if (dev_topo) d_off.upload(part_off_.data(), part_off_.size(), s);
+    } else if (dev_topo) {
         d_perm_.resize(nb_);
         const int32_t nparts = k::device_partition(nb_, d_rp0.get(), c0, 32, d_perm_.get(), d_off, s,
                                                     cfg_.partition_local_max);
@@ -502,6 +508,7 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
         std::iota(perm_.begin(), perm_.end(), 0);
         part_off_.clear();
     }
+    partition_reused_ = reuse;
     // row pointer of the permuted pattern
     t_partition_ = since(tp);
     if (!dev_topo) d_perm_.upload(perm_.data(), nb_, s);
@@ -1547,7 +1554,7 @@ std::string HybridSolver::info_json() const
       << ",\"omega_used\":" << omega_used_ << ",\"solver\":\""
       << name() << "\",\"stop_reason\":\"" << stop_name(stop_code_)
       << "\",\"n\":" << n_ << ",\"nnz\":" << nnz_ << ",\"block_size\":" << bs_ << ",\"nnz_blocks\":" << nnzb_
-      << ",\"sell_slots\":" << sell_slots_ << ",\"precision\":\"" << cfg_.precision << "\",\"inner_spmv_fp32\":"
+      << ",\"sell_slots\":" << sell_slots_ << ",\"partition_reused\":" << (partition_reused_ ? "true" : "false") << ",\"precision\":\"" << cfg_.precision << "\",\"inner_spmv_fp32\":"
       << (cfg_.fp32_inner() ? "true" : "false") << ",\"omega\":" << omega_ << ",\"lambda_max_BA\":" << lambda_max_ << ",\"timings\":{\"factorize\":"
       << t_factorize_ << ",\"select\":" << t_select_ << ",\"operator\":" << t_operator_ << ",\"partition\":"
       << t_partition_ << ",\"mas\":" << t_mas_ << ",\"stiff\":" << t_stiff_ << ",\"omega\":" << t_omega_

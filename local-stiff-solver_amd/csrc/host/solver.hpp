@@ -161,6 +161,7 @@ private:
     };
     std::vector<AmgLevel> amg_;
     k::AmgWork amg_work_;  // setup scratch, kept for refactorizations
+    bool partition_reused_ = false;
     std::vector<std::vector<int8_t>> amg_cf_;  // C/F splitting per level (host, inspection)
     std::vector<int32_t> amg_sizes_;
     std::vector<int64_t> amg_nnz_;

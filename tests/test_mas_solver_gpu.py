@@ -97,6 +97,48 @@ def test_mas_solver_without_residual_replacement(gpu, kw):
     assert np.linalg.norm(S @ x - b) <= 2e-6 * np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("solver", ["MAS", "GPUHybrid"])
+def test_lazy_partitioning_reuses_the_first_partition(gpu, solver):
+    # MASSolver.cu:389-405: with lazy_partitioning (the MAS solver's default) a later
+    # factorization with the same block count keeps the first partition
+    import scipy.sparse as sp
+
+    prob = problems.make("elastic_contact", N=10)
+    a = prob.csc()
+    nb = a.n // 3
+    # b: the same system with its nodes renumbered (same block count, another pattern)
+    node = np.random.default_rng(3).permutation(nb)
+    dof = (3 * node[:, None] + np.arange(3)).ravel()
+    Sb = sp.csc_matrix(a.to_scipy()[dof][:, dof])
+    Sb.sort_indices()
+    b = type(a)(a.n, Sb.indptr.astype(np.int32), Sb.indices.astype(np.int32), Sb.data.copy())
+    if solver == "MAS":
+        cfg, cfg_off = _dev_cfg(3), _dev_cfg(3, lazy_partitioning=False)
+    else:
+        g = {"block_size": 3, "relative_tolerance": 1e-8, "gmm_jump_threshold": 10.0, "min_subdomain_size": 4}
+        cfg = {"solver": "GPUHybrid", "GPUHybrid": dict(g, lazy_partitioning=True)}
+        cfg_off = {"solver": "GPUHybrid", "GPUHybrid": g}
+    s, s_off = Solver.create(cfg), Solver.create(cfg_off)
+    s.factorize(a)
+    assert not s.get_info()["partition_reused"]
+    pa, oa = s.partition(nb)
+    s.factorize(b)
+    assert s.get_info()["partition_reused"]
+    pb, ob = s.partition(nb)
+    assert np.array_equal(pa, pb) and np.array_equal(oa, ob)
+    s_off.factorize(a)
+    s_off.factorize(b)
+    assert not s_off.get_info()["partition_reused"]
+    pc, _ = s_off.partition(nb)
+    assert not np.array_equal(pc, pa)  # b's own partition differs
+    # the kept partition still gives a convergent solve of b
+    rhs = prob.rhs(0)[dof]
+    x = s.solve(rhs)
+    S = b.to_scipy()
+    tol = 2e-6 if solver == "MAS" else 1.01e-8
+    assert np.linalg.norm(S @ x - rhs) <= tol * np.linalg.norm(rhs)
+
+
 def test_mas_solver_status_and_selection(gpu):
     prob = problems.make("elastic_contact", N=8)
     A = prob.csc()
