@@ -456,6 +456,22 @@ int lssgen_block_size(const lssgen_problem *p) { return p->bs; }
 int64_t lssgen_n_contacts(const lssgen_problem *p) { return int64_t(p->contacts.size()); }
 int64_t lssgen_n_slivers(const lssgen_problem *p) { return int64_t(p->slivers.size()); }
 
+int lssgen_sliver_geometry(const lssgen_problem *p, int64_t *elem, double *X, double *X0)
+{
+    for (size_t q = 0; q < p->slivers.size(); ++q) {
+        const Sliver &sl = p->slivers[q];
+        elem[q] = sl.elem;
+        double orig[4][3];
+        p->element_vertices(sl.elem, orig);
+        for (int a = 0; a < 4; ++a)
+            for (int d = 0; d < 3; ++d) {
+                X[q * 12 + a * 3 + d] = sl.X[a][d];
+                X0[q * 12 + a * 3 + d] = orig[a][d];
+            }
+    }
+    return 0;
+}
+
 int lssgen_fill_csc(const lssgen_problem *P, int32_t *colptr, int32_t *rowidx, double *vals)
 {
     const int bs = P->bs, bb = bs * bs;

@@ -60,6 +60,9 @@ int64_t lssgen_n_elements(const lssgen_problem *p);
 int lssgen_block_size(const lssgen_problem *p);
 int64_t lssgen_n_contacts(const lssgen_problem *p);
 int64_t lssgen_n_slivers(const lssgen_problem *p);
+/* the sliver elements (ascending), their vertices after and before the collapse:
+   X, X0 of n_slivers * 4 * 3 doubles (vertex-major) */
+int lssgen_sliver_geometry(const lssgen_problem *p, int64_t *elem, double *X, double *X0);
 
 /* Fill CSC arrays: colptr[n+1], rowidx[nnz], vals[nnz].  Returns 0 on success. */
 int lssgen_fill_csc(const lssgen_problem *p, int32_t *colptr, int32_t *rowidx, double *vals);

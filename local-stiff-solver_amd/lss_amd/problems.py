@@ -73,6 +73,7 @@ def _load():
     lib.lssgen_random_vector.argtypes = [ctypes.c_int64, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, P]
     lib.lssgen_element_coef.argtypes = [P, P]
     lib.lssgen_element_nodes.argtypes = [P, P]
+    lib.lssgen_sliver_geometry.argtypes = [P, P, P, P]
     lib.lssgen_grad_coef.argtypes = [P, P, P, P]
     lib.lssgen_fill_vals_with_coef.argtypes = [P, P, P]
     lib.lssgen_element_jacobian_nnz.restype = ctypes.c_int64
@@ -173,6 +174,15 @@ class Problem:
         out = np.empty(self.n_elements, np.float64)
         _load().lssgen_element_coef(self._h, _ptr(out))
         return out
+
+    def sliver_geometry(self):
+        """(elements, vertices after, vertices before the collapse) of the sliver elements."""
+        k = self.n_slivers
+        elem = np.zeros(max(k, 1), np.int64)
+        X = np.zeros((max(k, 1), 4, 3))
+        X0 = np.zeros((max(k, 1), 4, 3))
+        _load().lssgen_sliver_geometry(self._h, _ptr(elem), _ptr(X), _ptr(X0))
+        return elem[:k], X[:k], X0[:k]
 
     def element_nodes(self) -> np.ndarray:
         out = np.empty((self.n_elements, 4), np.int32)
