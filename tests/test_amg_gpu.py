@@ -242,3 +242,55 @@ def test_hybrid_amg_composition_and_solve(gpu, name, N):
     s2.solve(b)
     _log(name, N, "hybrid(AMG) iterations", info["num_iterations"], "AMG only", s2.get_info()["num_iterations"],
          "stiff DOFs", sum(c.size for c in comps))
+
+
+def _csc_of(S):
+    from lss_amd.problems import CSC
+
+    S = sp.csc_matrix(S)
+    S.sort_indices()
+    return CSC(S.shape[0], S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.astype(np.float64))
+
+
+def test_single_level_hierarchies(gpu):
+    # a diagonal matrix has no strong connection (every point -3, no C point) and a
+    # matrix of at most 5 rows is the coarsest level itself: one level, B = D^-1 (l1)
+    rng = np.random.default_rng(4)
+    for S in (sp.diags(rng.uniform(1.0, 3.0, 50)), sp.csr_matrix(np.array(
+            [[4.0, -1, 0, 0], [-1, 4.0, -1, 0], [0, -1, 4.0, -1], [0, 0, -1, 4.0]]))):
+        A = _csc_of(S)
+        s = Solver.create(_cfg(1, do_hybrid_precond=False))
+        s.factorize(A)
+        lv = s.amg_levels()
+        assert len(lv) == 1 and lv[0]["n"] == A.n
+        r = rng.standard_normal(A.n)
+        z = s.base_apply(r)
+        assert np.allclose(z, amg.l1_dinv(_rows(A)) * r, rtol=1e-15, atol=0)
+        b = rng.standard_normal(A.n)
+        x = s.solve(b)
+        assert np.linalg.norm(S @ x - b) <= 1.01e-8 * np.linalg.norm(b)
+
+
+def test_block_size_two_matches_cpp_restatement(gpu, cpp_ref):
+    # two functions per node: the per-function ("unknown") strength rule with nf = 2
+    n = 24
+    L = sp.diags([-np.ones(n - 1), 2 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1])
+    L2 = sp.kronsum(L, L)
+    C = sp.csr_matrix(np.array([[2.0, 0.4], [0.4, 1.0]]))
+    S = sp.csr_matrix(sp.kron(L2, C))
+    A = _csc_of(S)
+    s = Solver.create(_cfg(2, do_hybrid_precond=False))
+    s.factorize(A)
+    lv = s.amg_levels()
+    ref = cpp_ref(_rows(A), 2, 0.5)
+    assert len(lv) == len(ref) >= 3
+    for l in range(len(lv)):
+        if l:
+            assert abs(lv[l]["A"] - ref[l]["A"]).max() == 0, l
+        if l + 1 < len(lv):
+            assert np.array_equal(lv[l]["cf"], ref[l]["cf"]), l
+            assert (lv[l]["P"] != ref[l]["P"]).nnz == 0, l
+    b = np.random.default_rng(5).standard_normal(A.n)
+    x = s.solve(b)
+    assert s.get_info()["stop_reason"] == "converged_rel"
+    assert np.linalg.norm(S @ x - b) <= 1.01e-8 * np.linalg.norm(b)
