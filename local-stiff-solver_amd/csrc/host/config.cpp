@@ -160,7 +160,8 @@ Config validate(Config c)
     if (c.max_iter < 0) bad("max_iter must be >= 0");
     if (c.precision != "fp64" && c.precision != "fp32" && c.precision != "bf16" && c.precision != "fp16")
         bad("precision must be fp64|fp32|bf16|fp16");
-    if (c.inner_spmv != "auto" && c.inner_spmv != "fp64" && c.inner_spmv != "fp32") bad("inner_spmv must be auto|fp64|fp32");
+    if (c.inner_spmv != "auto" && c.inner_spmv != "fp64" && c.inner_spmv != "fp32" && c.inner_spmv != "bf16")
+        bad("inner_spmv must be auto|fp64|fp32|bf16");
     if (c.base_preconditioner != "MAS" && c.base_preconditioner != "Jacobi" && c.base_preconditioner != "AMG")
         bad("base_preconditioner must be MAS|AMG|Jacobi");
     if (c.check_every < 1) bad("check_every must be >= 1");

@@ -51,7 +51,7 @@ struct Config {
     bool mas_mode() const { return solver == "MAS"; }
     // keys of this build
     std::string precision = "fp64";     // MAS bank inverses: "fp64" | "fp32" | "bf16"
-    std::string inner_spmv = "auto";    // preconditioner-internal SpMV values: "auto" | "fp64" | "fp32"
+    std::string inner_spmv = "auto";    // preconditioner-internal SpMV values: "auto" | "fp64" | "fp32" | "bf16"
     std::string base_preconditioner = "MAS";  // "MAS" | "AMG" | "Jacobi"
     int check_every = 8;                // PCG iterations between host polls of the device stop flag
     double damping = 1.8;               // omega = damping / lambda_max(B A) inside the B-P-B sweep
@@ -67,7 +67,13 @@ struct Config {
     // precision banks).  The stiff rows are then recomputed in FP64: an FP32 rounding
     // of a 1e8 contact entry exceeds the soft entries and would make the sweep
     // inconsistent with the FP64 stiff solve.
-    bool fp32_inner() const { return inner_spmv == "fp32" || (inner_spmv == "auto" && precision != "fp64"); }
+    // reduced-precision inner SpMV (FP32 x copy, stiff rows recomputed in FP64); bf16_inner:
+    // its values in bfloat16 instead of FP32 (opt-in: fewer bytes, workload-dependent iterations)
+    bool fp32_inner() const
+    {
+        return inner_spmv == "fp32" || inner_spmv == "bf16" || (inner_spmv == "auto" && precision != "fp64");
+    }
+    bool bf16_inner() const { return inner_spmv == "bf16"; }
 
     // Every key that shapes what factorize builds (buffers, layouts, selections, omega).
     // The others (tolerances, iteration caps, polling, timing, residual checks) only

@@ -142,6 +142,7 @@ void HybridSolver::release()
     d_scols_.release();
     d_svals_.release();
     d_svals32_.release();
+    d_svals16_.release();
     d_dinv_.release();
     d_comp_off_.release();
     d_mat_off_.release();
@@ -547,16 +548,21 @@ void HybridSolver::build_operator(int32_t n, int64_t nnz, const int32_t *ptr, co
     d_scols_.resize(std::max<int64_t>(sell_slots_, 1));
     const int64_t nsv = k::sell_value_count(sell_slots_, bs_);
     d_svals_.resize(std::max<int64_t>(nsv, 1));
-    if (cfg_.fp32_inner()) d_svals32_.resize(std::max<int64_t>(nsv, 1));
+    const bool v32 = cfg_.fp32_inner() && !cfg_.bf16_inner(), v16 = cfg_.bf16_inner();
+    if (v32) d_svals32_.resize(std::max<int64_t>(nsv, 1));
+    else d_svals32_.release();
+    if (v16) d_svals16_.resize(std::max<int64_t>(nsv, 1));
+    else d_svals16_.release();
     k::sell_fill(nb_, bs_, d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), d_slice_ptr_.get(), d_scols_.get(),
-                 d_svals_.get(), cfg_.fp32_inner() ? d_svals32_.get() : nullptr, s);
+                 d_svals_.get(), v32 ? d_svals32_.get() : nullptr, v16 ? d_svals16_.get() : nullptr, s);
     sell_.nb = nb_;
     sell_.bs = bs_;
     sell_.nslices = nsl;
     sell_.slice_ptr = d_slice_ptr_.get();
     sell_.cols = d_scols_.get();
     sell_.vals = d_svals_.get();
-    sell_.vals32 = cfg_.fp32_inner() ? d_svals32_.get() : nullptr;
+    sell_.vals32 = v32 ? d_svals32_.get() : nullptr;
+    sell_.vals16 = v16 ? d_svals16_.get() : nullptr;
     sync(s);
     level_nodes_.clear();
     mas_.clear();
@@ -1555,7 +1561,8 @@ std::string HybridSolver::info_json() const
       << name() << "\",\"stop_reason\":\"" << stop_name(stop_code_)
       << "\",\"n\":" << n_ << ",\"nnz\":" << nnz_ << ",\"block_size\":" << bs_ << ",\"nnz_blocks\":" << nnzb_
       << ",\"sell_slots\":" << sell_slots_ << ",\"partition_reused\":" << (partition_reused_ ? "true" : "false") << ",\"precision\":\"" << cfg_.precision << "\",\"inner_spmv_fp32\":"
-      << (cfg_.fp32_inner() ? "true" : "false") << ",\"omega\":" << omega_ << ",\"lambda_max_BA\":" << lambda_max_ << ",\"timings\":{\"factorize\":"
+      << (cfg_.fp32_inner() ? "true" : "false") << ",\"inner_spmv_values\":\""
+      << (cfg_.bf16_inner() ? "bf16" : (cfg_.fp32_inner() ? "fp32" : "fp64")) << "\"" << ",\"omega\":" << omega_ << ",\"lambda_max_BA\":" << lambda_max_ << ",\"timings\":{\"factorize\":"
       << t_factorize_ << ",\"select\":" << t_select_ << ",\"operator\":" << t_operator_ << ",\"partition\":"
       << t_partition_ << ",\"mas\":" << t_mas_ << ",\"stiff\":" << t_stiff_ << ",\"omega\":" << t_omega_
       << ",\"solve\":" << t_solve_ << "},\"stiff\":{\"gmm\":{\"global_mean\":" << gmm_.global_mean

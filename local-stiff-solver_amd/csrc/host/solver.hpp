@@ -146,6 +146,7 @@ private:
     DBuf<int32_t> d_scols_;
     DBuf<double> d_svals_;
     DBuf<float> d_svals32_;
+    DBuf<uint16_t> d_svals16_;  // bfloat16 copy (inner_spmv "bf16")
     k::SellView sell_;
     int64_t sell_slots_ = 0;
 

@@ -26,7 +26,8 @@ void permute_lengths(int32_t nb, const int32_t *perm, const int32_t *len, int32_
 // Sliced block-ELL layout (one wave per slice of 64/bs block rows).
 void sell_width(int32_t nb, int bs, const int32_t *rowptr, int32_t *slice_width, hipStream_t s);
 void sell_fill(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols, const double *vals,
-               const int64_t *slice_ptr, int32_t *scols, double *svals, float *svals32, hipStream_t s);
+               const int64_t *slice_ptr, int32_t *scols, double *svals, float *svals32, uint16_t *svals16,
+               hipStream_t s);
 
 // ---------------------------------------------------------------- SpMV
 struct SellView {
@@ -37,6 +38,7 @@ struct SellView {
     const int32_t *cols = nullptr;       // per slot, -1 = padding
     const double *vals = nullptr;        // per slot bs*bs
     const float *vals32 = nullptr;       // optional FP32 copy
+    const uint16_t *vals16 = nullptr;    // optional bfloat16 copy (taken before vals32)
 };
 
 // y = A x (sub == nullptr) or y = sub - A x.  spmv_pq also reduces p . A p and
@@ -47,7 +49,8 @@ void spmv(const SellView &A, const double *x, double *y, const double *sub, hipS
 // `after`, when given, is recorded on s between the SpMV and the finisher (kernel timing).
 void spmv_pq(const SellView &A, const double *p, double *q, dev::PcgState *st, double *partials, int iter,
              hipStream_t s, int k = 1, int64_t ld = 0, int finish_op = dev::kFinishPq, hipEvent_t after = nullptr);
-// fp32: the FP32 values with x32, an FP32 copy of x at the same stride (x unused)
+// fp32: the FP32 (or, when present, bfloat16) values with x32, an FP32 copy of x at the
+// same stride (x unused)
 void spmv_residual(const SellView &A, const double *x, const double *b, double *r, const int *stop, hipStream_t s,
                    bool fp32 = false, int k = 1, int64_t ld = 0, const float *x32 = nullptr);
 // y = float(x) over n entries

@@ -309,6 +309,14 @@ __global__ void k_sell_width(int32_t nb, int H, const int32_t *__restrict__ rowp
     width[s] = w;
 }
 
+// bfloat16 bits of an FP32 value, rounded to nearest even (values here are finite)
+__device__ __forceinline__ uint16_t to_bf16_bits(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float from_bf16_bits(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+
 // One wave per slice, lane l = h * bs + rr (column rr of the blocks of the slice's
 // block row h): every slot of the slice is written, padding included (column -1,
 // zero values), and a slot's values are written by consecutive lanes.  Position
@@ -318,7 +326,8 @@ __global__ __launch_bounds__(256) void k_sell_fill(int32_t nb, int bs, int H, in
                                                     const int32_t *__restrict__ rowptr,
                                                     const int32_t *__restrict__ cols, const double *__restrict__ vals,
                                                     const int64_t *__restrict__ slice_ptr, int32_t *__restrict__ scols,
-                                                    double *__restrict__ svals, float *__restrict__ svals32)
+                                                    double *__restrict__ svals, float *__restrict__ svals32,
+                                                    uint16_t *__restrict__ svals16)
 {
     const int lane = threadIdx.x & 63;
     const int32_t sl = int32_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
@@ -338,6 +347,7 @@ __global__ __launch_bounds__(256) void k_sell_fill(int32_t nb, int bs, int H, in
             const int64_t at = vbase + int64_t(c) * (H * bs) + lane;
             svals[at] = v;
             if (svals32) svals32[at] = static_cast<float>(v);
+            if (svals16) svals16[at] = to_bf16_bits(static_cast<float>(v));
         }
     }
 }
@@ -352,10 +362,12 @@ __global__ __launch_bounds__(256) void k_sell_fill(int32_t nb, int bs, int H, in
 // The BS lanes of a block row add their partial rows once, after the last slot (a
 // fixed rotation of lane shuffles), so lane (h, q) ends with row q of y.  One x
 // load per lane and slot (instead of BS), and fewer registers for the gathered x.
-template <int BS, int MODE, bool DOT, bool F32, int K>
+// VT: 0 FP64 values with x; 1 FP32 values, 2 bfloat16 values, both with x32
+template <int BS, int MODE, bool DOT, int VT, int K>
 __global__ __launch_bounds__(kSpmvThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_spmv(int32_t nb, int32_t nslices, const int64_t *__restrict__ slice_ptr,
                                                         const int32_t *__restrict__ cols, const double *__restrict__ vals,
-                                                        const float *__restrict__ vals32, const double *__restrict__ x,
+                                                        const float *__restrict__ vals32,
+                                                        const uint16_t *__restrict__ vals16, const double *__restrict__ x,
                                                         const float *__restrict__ x32,
                                                         double *__restrict__ y, const double *__restrict__ sub,
                                                         const int *__restrict__ stop, int64_t ld,
@@ -366,9 +378,10 @@ __global__ __launch_bounds__(kSpmvThreads) __attribute__((amdgpu_waves_per_eu(6)
     // 5 slots per chunk for one right-hand side: three round trips for the usual
     // width of 15 (14 neighbours and the diagonal block of a tetrahedral mesh node)
     constexpr int C = K >= 4 ? 2 : (K == 2 ? 4 : 5);
-    // FP32 values go with the FP32 copy of x (x32): the preconditioner-internal
-    // residual, never the dot-product variant
-    static_assert(!(F32 && DOT), "the FP32 SpMV has no dot product");
+    // FP32 and bfloat16 values go with the FP32 copy of x (x32): the
+    // preconditioner-internal residual, never the dot-product variant
+    constexpr bool F32 = VT != 0;
+    static_assert(!(F32 && DOT), "the reduced-precision SpMV has no dot product");
     if (dev::all_stopped(stop, K)) return;
     const int lane = threadIdx.x & 63;
     // wave-uniform: the slice pointer is a scalar load and the chunk loop has scalar control
@@ -418,7 +431,9 @@ __global__ __launch_bounds__(kSpmvThreads) __attribute__((amdgpu_waves_per_eu(6)
                 const int64_t vb = (base + int64_t(min(j0 + u, W - 1)) * H) * BB + ll;
 #pragma unroll
                 for (int q = 0; q < BS; ++q) {
-                    if constexpr (F32)
+                    if constexpr (VT == 2)
+                        a[u][q] = double(from_bf16_bits(vals16[vb + q * (H * BS)]));
+                    else if constexpr (VT == 1)
                         a[u][q] = double(vals32[vb + q * (H * BS)]);
                     else
                         a[u][q] = vals[vb + q * (H * BS)];
@@ -506,15 +521,16 @@ __global__ __launch_bounds__(kSpmvThreads) __attribute__((amdgpu_waves_per_eu(6)
     }
 }
 
-template <int MODE, bool DOT, bool F32, int K>
+template <int MODE, bool DOT, int VT, int K>
 int launch_spmv_k(const SellView &A, const double *x, const float *x32, double *y, const double *sub,
                   const int *stop, int64_t ld, double *partials, hipStream_t s)
 {
     const int grid = div_up(A.nslices, kSpmvThreads / 64);
     if (grid == 0) return 0;
 #define LSS_SPMV_CASE(BS)                                                                                      \
-    k_spmv<BS, MODE, DOT, F32, K><<<grid, kSpmvThreads, 0, s>>>(A.nb, A.nslices, A.slice_ptr, A.cols, A.vals,  \
-                                                                A.vals32, x, x32, y, sub, stop, ld, partials)
+    k_spmv<BS, MODE, DOT, VT, K><<<grid, kSpmvThreads, 0, s>>>(A.nb, A.nslices, A.slice_ptr, A.cols, A.vals,   \
+                                                               A.vals32, A.vals16, x, x32, y, sub, stop, ld,   \
+                                                               partials)
     switch (A.bs) {
         case 1: LSS_SPMV_CASE(1); break;
         case 2: LSS_SPMV_CASE(2); break;
@@ -526,14 +542,14 @@ int launch_spmv_k(const SellView &A, const double *x, const float *x32, double *
     return grid;
 }
 
-template <int MODE, bool DOT, bool F32>
+template <int MODE, bool DOT, int VT>
 int launch_spmv(const SellView &A, const double *x, const float *x32, double *y, const double *sub, const int *stop,
                 int k, int64_t ld, double *partials, hipStream_t s)
 {
     switch (k) {
-        case 1: return launch_spmv_k<MODE, DOT, F32, 1>(A, x, x32, y, sub, stop, ld, partials, s);
-        case 2: return launch_spmv_k<MODE, DOT, F32, 2>(A, x, x32, y, sub, stop, ld, partials, s);
-        case 4: return launch_spmv_k<MODE, DOT, F32, 4>(A, x, x32, y, sub, stop, ld, partials, s);
+        case 1: return launch_spmv_k<MODE, DOT, VT, 1>(A, x, x32, y, sub, stop, ld, partials, s);
+        case 2: return launch_spmv_k<MODE, DOT, VT, 2>(A, x, x32, y, sub, stop, ld, partials, s);
+        case 4: return launch_spmv_k<MODE, DOT, VT, 4>(A, x, x32, y, sub, stop, ld, partials, s);
         default: fail(LSS_ERR_ARG, "a fused SpMV takes 1, 2 or 4 right-hand sides");
     }
 }
@@ -592,19 +608,21 @@ void sell_width(int32_t nb, int bs, const int32_t *rowptr, int32_t *slice_width,
 }
 
 void sell_fill(int32_t nb, int bs, const int32_t *rowptr, const int32_t *cols, const double *vals,
-               const int64_t *slice_ptr, int32_t *scols, double *svals, float *svals32, hipStream_t s)
+               const int64_t *slice_ptr, int32_t *scols, double *svals, float *svals32, uint16_t *svals16,
+               hipStream_t s)
 {
     if (nb == 0) return;
     const int H = 64 / bs;
     const int32_t ns = div_up(nb, H);
-    k_sell_fill<<<div_up(ns, 4), 256, 0, s>>>(nb, bs, H, ns, rowptr, cols, vals, slice_ptr, scols, svals, svals32);
+    k_sell_fill<<<div_up(ns, 4), 256, 0, s>>>(nb, bs, H, ns, rowptr, cols, vals, slice_ptr, scols, svals, svals32,
+                                              svals16);
     LSS_LAUNCH_CHECK();
 }
 
 void spmv(const SellView &A, const double *x, double *y, const double *sub, hipStream_t s)
 {
-    if (sub) launch_spmv<1, false, false>(A, x, nullptr, y, sub, nullptr, 1, 0, nullptr, s);
-    else launch_spmv<0, false, false>(A, x, nullptr, y, nullptr, nullptr, 1, 0, nullptr, s);
+    if (sub) launch_spmv<1, false, 0>(A, x, nullptr, y, sub, nullptr, 1, 0, nullptr, s);
+    else launch_spmv<0, false, 0>(A, x, nullptr, y, nullptr, nullptr, 1, 0, nullptr, s);
 }
 
 int spmv_partials_per_column(const SellView &A) { return div_up(A.nslices, kSpmvThreads / 64); }
@@ -614,7 +632,7 @@ int64_t sell_value_count(int64_t slots, int bs) { return slots * bs * bs; }
 void spmv_pq(const SellView &A, const double *p, double *q, dev::PcgState *st, double *partials, int iter,
              hipStream_t s, int k, int64_t ld, int finish_op, hipEvent_t after)
 {
-    const int g = launch_spmv<0, true, false>(A, p, nullptr, q, nullptr, &st->stop, k, ld, partials, s);
+    const int g = launch_spmv<0, true, 0>(A, p, nullptr, q, nullptr, &st->stop, k, ld, partials, s);
     if (after) LSS_HIP(hipEventRecord(after, s));
     if (g > 0) finish(g, partials, finish_op, st, nullptr, iter, s, k);
 }
@@ -623,10 +641,12 @@ void spmv_residual(const SellView &A, const double *x, const double *b, double *
                    bool fp32, int k, int64_t ld, const float *x32)
 {
     if (fp32) {
-        if (!x32 || !A.vals32) fail(LSS_ERR_ARG, "the FP32 residual needs the FP32 values and x32");
-        launch_spmv<1, false, true>(A, x, x32, r, b, stop, k, ld, nullptr, s);
+        if (!x32 || (!A.vals32 && !A.vals16))
+            fail(LSS_ERR_ARG, "the reduced-precision residual needs FP32 or bfloat16 values and x32");
+        if (A.vals16) launch_spmv<1, false, 2>(A, x, x32, r, b, stop, k, ld, nullptr, s);
+        else launch_spmv<1, false, 1>(A, x, x32, r, b, stop, k, ld, nullptr, s);
     } else {
-        launch_spmv<1, false, false>(A, x, nullptr, r, b, stop, k, ld, nullptr, s);
+        launch_spmv<1, false, 0>(A, x, nullptr, r, b, stop, k, ld, nullptr, s);
     }
 }
 

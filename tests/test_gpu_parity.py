@@ -298,6 +298,28 @@ def test_reduced_precision_banks_converge(gpu, name, N, precision):
     assert info["num_iterations"] <= (1.25 if precision == "fp32" else 2.5) * it64 + 3
 
 
+@pytest.mark.parametrize("name,N", [("elastic_contact", 12), ("laplace_sphere", 14)])
+def test_bf16_inner_spmv_solve(gpu, name, N):
+    # "inner_spmv": "bf16": the preconditioner's residual reads bfloat16 matrix values
+    # (stiff rows still recomputed in FP64); the FP64 outer PCG keeps the accuracy
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    b = prob.rhs(0)
+    s64 = Solver.create(_cfg(prob.block_size))
+    s64.factorize(A)
+    x64 = s64.solve(b)
+    it64 = s64.get_info()["num_iterations"]
+    s = Solver.create(_cfg(prob.block_size, precision="fp32", inner_spmv="bf16"))
+    s.factorize(A)
+    x = s.solve(b)
+    info = s.get_info()
+    assert info["inner_spmv_values"] == "bf16"
+    assert info["stop_reason"] == "converged_rel"
+    assert _res_ok(A.to_scipy(), x, b)
+    assert np.linalg.norm(x - x64) <= 1e-6 * np.linalg.norm(x64)
+    assert info["num_iterations"] <= 1.5 * it64 + 3
+
+
 # Without coarse MAS levels the level-0 apply itself reduces r.z for the PCG (the
 # dot-product variants of the FP32 tiled and FP64 packed kernels).  The oracle always
 # builds the reference's levels, so the check is convergence: both precisions reach
