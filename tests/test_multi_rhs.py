@@ -65,6 +65,9 @@ def _singles(s, B, X0):
         ("laplace_sphere", 12, 2, {}),
         ("elastic_logE", 8, 4, {"base_preconditioner": "Jacobi"}),
         ("elastic_contact", 8, 2, {"do_hybrid_precond": False}),
+        # the AMG base preconditioner: one V-cycle per column
+        ("elastic_contact", 10, 3, {"base_preconditioner": "AMG"}),
+        ("laplace_sphere", 12, 2, {"base_preconditioner": "AMG", "do_hybrid_precond": False}),
     ],
 )
 def test_fused_equals_single_solves(gpu, name, N, k, kw):
