@@ -109,6 +109,8 @@ Config parse_config(const char *text, const Config &base)
                 else if (k == "partitioner") c.partitioner = as_str(v, k);
                 else if (k == "max_coarse_levels") c.max_coarse_levels = int(as_int(v, k));
                 else if (k == "partition_local_max") c.partition_local_max = int(as_int(v, k));
+                else if (k == "bank_apply") c.bank_apply = as_str(v, k);
+                else if (k == "mfma_flush") c.mfma_flush = int(as_int(v, k));
                 else bad("unknown key MAS/" + k);
             }
         }
@@ -149,6 +151,8 @@ Config parse_config(const char *text, const Config &base)
         else if (k == "max_coarse_levels") c.max_coarse_levels = int(as_int(v, k));
         else if (k == "partition_local_max") c.partition_local_max = int(as_int(v, k));
         else if (k == "lazy_partitioning") c.lazy_partitioning = as_bool(v, k);
+        else if (k == "bank_apply") c.bank_apply = as_str(v, k);
+        else if (k == "mfma_flush") c.mfma_flush = int(as_int(v, k));
         else bad("unknown key GPUHybrid/" + k);
     }
     return validate(c);
@@ -164,6 +168,9 @@ Config validate(Config c)
         bad("inner_spmv must be auto|fp64|fp32|bf16");
     if (c.base_preconditioner != "MAS" && c.base_preconditioner != "Jacobi" && c.base_preconditioner != "AMG")
         bad("base_preconditioner must be MAS|AMG|Jacobi");
+    if (c.bank_apply != "vector" && c.bank_apply != "mfma") bad("bank_apply must be vector|mfma");
+    if (c.mfma_flush != 4 && c.mfma_flush != 8 && c.mfma_flush != 16 && c.mfma_flush != 32)
+        bad("mfma_flush must be 4, 8, 16 or 32");
     if (c.check_every < 1) bad("check_every must be >= 1");
     if (c.true_residual_period < 0) bad("true_residual_period must be >= 0 (0: never)");
     if (c.partitioner != "device" && c.partitioner != "host") bad("partitioner must be device|host");

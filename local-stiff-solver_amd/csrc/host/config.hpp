@@ -62,6 +62,11 @@ struct Config {
     std::string partitioner = "device";  // MAS partition: "device" (HIP kernels) | "host" (OpenMP)
     int max_coarse_levels = 4;           // MAS coarse levels (MAS_MAX_COARSE_LEVEL, MASPreconditioner.hpp:10-12)
     int partition_local_max = 8192;      // device partition: subsets this small finish in one workgroup (0: never)
+    // level-0 bank solve of FP32 factors: "vector" (FP64 accumulation on the vector units) or
+    // "mfma" (matrix cores, FP32 accumulators added into FP64 every mfma_flush products)
+    std::string bank_apply = "vector";
+    int mfma_flush = 4;
+    int mfma_flush_steps() const { return bank_apply == "mfma" && precision == "fp32" ? mfma_flush : 0; }
 
     // FP32 matrix values in the preconditioner-internal SpMV (by default with reduced-
     // precision banks).  The stiff rows are then recomputed in FP64: an FP32 rounding

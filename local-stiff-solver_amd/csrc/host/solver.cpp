@@ -1107,7 +1107,8 @@ void HybridSolver::base_apply(const double *r, double scale, const double *add, 
     if (L == 0) {
         if (prof) prof_begin(2, s);
         k::mas_level0_apply(prec(), bs_, mas_[0].nbanks, mas_[0].bank_off.get(), mas_[0].inv.get(), r, nullptr,
-                            nullptr, scale, add, z, st ? r_.get() : nullptr, st, partials_.get(), stop, s, k, np, 0);
+                            nullptr, scale, add, z, st ? r_.get() : nullptr, st, partials_.get(), stop, s, k, np, 0,
+                            cfg_.mfma_flush_steps());
         if (prof) prof_end(s);
         if (st) k::finish(mas_[0].nbanks, partials_.get(), op, st, nullptr, 0, s, k);
         if (z32) k::to_f32(int64_t(k) * np, z, z32, s);
@@ -1133,7 +1134,7 @@ void HybridSolver::base_apply(const double *r, double scale, const double *add, 
     LSS_HIP(hipEventRecord(ev_join_, side_));
     if (prof) prof_begin(2, s);
     k::mas_level0_apply(prec(), bs_, mas_[0].nbanks, mas_[0].bank_off.get(), mas_[0].inv.get(), r, nullptr, nullptr,
-                        scale, add, z, nullptr, nullptr, partials_.get(), stop, s, k, np, 0);
+                        scale, add, z, nullptr, nullptr, partials_.get(), stop, s, k, np, 0, cfg_.mfma_flush_steps());
     if (prof) prof_end(s);
     LSS_HIP(hipStreamWaitEvent(s, ev_join_, 0));
     const int g = k::mas_prolong_add(np, bs_, mas_[0].parent.get(), mas_[1].w.get(), scale, z,
@@ -1560,7 +1561,9 @@ std::string HybridSolver::info_json() const
       << ",\"omega_used\":" << omega_used_ << ",\"solver\":\""
       << name() << "\",\"stop_reason\":\"" << stop_name(stop_code_)
       << "\",\"n\":" << n_ << ",\"nnz\":" << nnz_ << ",\"block_size\":" << bs_ << ",\"nnz_blocks\":" << nnzb_
-      << ",\"sell_slots\":" << sell_slots_ << ",\"partition_reused\":" << (partition_reused_ ? "true" : "false") << ",\"precision\":\"" << cfg_.precision << "\",\"inner_spmv_fp32\":"
+      << ",\"sell_slots\":" << sell_slots_ << ",\"partition_reused\":" << (partition_reused_ ? "true" : "false") << ",\"precision\":\"" << cfg_.precision
+      << "\",\"bank_apply\":\"" << (cfg_.mfma_flush_steps() ? "mfma" : "vector") << "\",\"mfma_flush\":"
+      << cfg_.mfma_flush_steps() << ",\"inner_spmv_fp32\":"
       << (cfg_.fp32_inner() ? "true" : "false") << ",\"inner_spmv_values\":\""
       << (cfg_.bf16_inner() ? "bf16" : (cfg_.fp32_inner() ? "fp32" : "fp64")) << "\"" << ",\"omega\":" << omega_ << ",\"lambda_max_BA\":" << lambda_max_ << ",\"timings\":{\"factorize\":"
       << t_factorize_ << ",\"select\":" << t_select_ << ",\"operator\":" << t_operator_ << ",\"partition\":"

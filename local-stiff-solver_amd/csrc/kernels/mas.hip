@@ -1008,6 +1008,244 @@ __global__ __launch_bounds__(kApplyThreads, (kTiledWaves<BS, DOT, K>)) void k_le
     }
 }
 
+// ------------------------------------------------------------ MFMA bank solve
+// The same G^T (G x) on the tiled LDS factor with v_mfma_f32_4x4x1f32, whose 16
+// blocks each multiply a 4-entry column of A by a 4-entry row of B per step:
+// lane 4b + i holds A_b[i], lane 4b + j holds B_b[j], and lane 4b + j, register
+// v, accumulates D_b[v][j] = RN32(D_b[v][j] + A_b[v] B_b[j]) (exact product,
+// one rounding per step).  An item is 4 rows (phase 1) or 4 columns (phase 2) of
+// one 32 x 32 tile and runs 32 steps; a wave runs 16 items at once, one per block.
+// B carries each right-hand side as two FP32 columns, hi = RN32(x) and
+// lo = RN32(x - hi), so x enters with 48 bits; every FL steps the FP32
+// accumulators are added into FP64 ones, which bounds the FP32 part of the sum to
+// FL products.  Partials of a row (column) over its tiles, hi + lo first, are
+// summed in tile order as in tile_solve.
+template <int BS>
+constexpr int mfma_vs() { return 32 * BS + 4; }  // floats per LDS vector variant (16-byte multiple)
+
+// tile (b, s) of tile_index t
+__device__ __forceinline__ void tile_of(int t, int &b, int &s)
+{
+    b = t >= 3 ? (t >= 6 ? 3 : 2) : (t >= 1 ? 1 : 0);
+    s = t - b * (b + 1) / 2;
+}
+
+// the K bank vectors of src as hi/lo FP32 pairs: variant 2m + h of column m
+template <int BS, int K>
+__device__ __forceinline__ void split_bank_vector(const double *__restrict__ src, int64_t ld, int n,
+                                                  float *__restrict__ xv)
+{
+    constexpr int D = 32 * BS, VS = mfma_vs<BS>();
+    for (int e = threadIdx.x; e < K * D; e += blockDim.x) {
+        const int k = e / D, i = e - k * D;
+        const double v = i < n ? src[k * ld + i] : 0.0;
+        const float hi = float(v);
+        xv[(2 * k) * VS + i] = hi;
+        xv[(2 * k + 1) * VS + i] = float(v - double(hi));
+    }
+}
+
+// one 32-step item on 16 blocks: A from a (strided by astep floats per step),
+// B from the variant vectors at bv[m] (contiguous); acc[m][v] gets the sum
+template <int K, int FL>
+__device__ __forceinline__ void mfma_item(const float *__restrict__ a, int astep, const float *const *bv,
+                                          double (&acc)[K == 4 ? 2 : 1][4])
+{
+    constexpr int NM = K == 4 ? 2 : 1;
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    F4 c[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        c[m] = F4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[m][v] = 0.0;
+    }
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+        float av[4];
+        if (astep == 1) {
+            const F4 t = *reinterpret_cast<const F4 *>(a + 4 * g);
+            av[0] = t.x; av[1] = t.y; av[2] = t.z; av[3] = t.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) av[u] = a[(4 * g + u) * astep];
+        }
+        F4 b[NM];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) b[m] = *reinterpret_cast<const F4 *>(bv[m] + 4 * g);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int m = 0; m < NM; ++m) c[m] = __builtin_amdgcn_mfma_f32_4x4x1f32(av[u], b[m][u], c[m], 0, 0, 0);
+        if ((4 * g + 4) % FL == 0) {
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[m][v] += double(c[m][v]);
+                c[m] = F4{0.0f, 0.0f, 0.0f, 0.0f};
+            }
+        }
+    }
+}
+
+// G^T (G x) for K columns: xv holds x (split_bank_vector), tv receives t = G x the
+// same way; on return (after the final barrier) y_k(j) = bank_result<BS>(Q, k, j).
+// P and Q may be one buffer (K * BS * D doubles).
+template <int BS, int K, int FL>
+__device__ __forceinline__ void mfma_solve(const float *__restrict__ Gt, const float *__restrict__ xv,
+                                           float *__restrict__ tv, double *P, double *Q)
+{
+    constexpr int D = 32 * BS, VS = mfma_vs<BS>();
+    constexpr int NITEM = 4 * BS * (BS + 1);  // 8 row (column) groups per tile
+    constexpr int NM = K == 4 ? 2 : 1;
+    constexpr int NV = 2 * K;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, blk = lane >> 2, q = lane & 3;
+    // variant of B column q in MFMA m (K = 1: columns 2, 3 repeat hi, lo; never stored)
+    const int var0 = q % (NV < 4 ? NV : 4);
+    const bool owner = q < NV;  // lane holds a real variant
+    const int item_raw = w * 16 + blk;
+    const bool live = item_raw < NITEM;
+    const int item = live ? item_raw : item_raw % NITEM;  // dead blocks of a live wave repeat live items
+    const int t = item >> 3, grp = item & 7;
+    int tb, ts;
+    tile_of(t, tb, ts);
+    const float *tile = Gt + t * kTileStride;
+    double acc[NM][4];
+    if (w * 16 < NITEM) {  // phase 1: rows 32 tb + 4 grp + v of G times x over tile column ts
+        const float *bv[NM];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) bv[m] = xv + (var0 + 4 * m) * VS + 32 * ts;
+        mfma_item<K, FL>(tile + (4 * grp + q) * kTileLd, 1, bv, acc);
+#pragma unroll
+        for (int m = 0; m < NM; ++m)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[m][v] += __shfl_xor(acc[m][v], 1);  // hi + lo
+        if (live && owner && (q & 1) == 0)
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    P[((var0 / 2 + 2 * m) * BS + ts) * D + 32 * tb + 4 * grp + v] = acc[m][v];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < K * D; e += blockDim.x) {
+        const int k = e / D, i = e - k * D;
+        double v = P[(k * BS) * D + i];
+        for (int s = 1; s <= i / 32; ++s) v += P[(k * BS + s) * D + i];
+        const float hi = float(v);
+        tv[(2 * k) * VS + i] = hi;
+        tv[(2 * k + 1) * VS + i] = float(v - double(hi));
+    }
+    __syncthreads();
+    if (w * 16 < NITEM) {  // phase 2: columns 32 ts + 4 grp + v of G^T t over tile row tb
+        const float *bv[NM];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) bv[m] = tv + (var0 + 4 * m) * VS + 32 * tb;
+        mfma_item<K, FL>(tile + 4 * grp + q, kTileLd, bv, acc);
+#pragma unroll
+        for (int m = 0; m < NM; ++m)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[m][v] += __shfl_xor(acc[m][v], 1);
+        if (live && owner && (q & 1) == 0)
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    Q[((var0 / 2 + 2 * m) * BS + tb) * D + 32 * ts + 4 * grp + v] = acc[m][v];
+    }
+    __syncthreads();
+}
+
+// k_level0_apply_tiled with the MFMA bank solve (FP32 factors, tiled layout)
+template <int BS, bool DOT, int K, int FL>
+__global__ __launch_bounds__(kApplyThreads, (kTiledWaves<BS, DOT, K>)) void k_level0_apply_mfma(int32_t nbanks,
+                                                                     const int32_t *__restrict__ part_off,
+                                                                     const float *__restrict__ inv,
+                                                                     const double *__restrict__ r, int64_t ld,
+                                                                     const int32_t *__restrict__ parent,
+                                                                     const double *__restrict__ w1, int64_t ld1,
+                                                                     double scale, const double *__restrict__ add,
+                                                                     double *__restrict__ z,
+                                                                     const double *__restrict__ dotv,
+                                                                     double *partials, const int *__restrict__ stop)
+{
+    constexpr int D = 32 * BS, VS = mfma_vs<BS>();
+    constexpr int ST = tiled_floats(BS);
+    constexpr int NTILE = BS * (BS + 1) / 2;
+    constexpr int NB = tiled_banks_per_group(K);
+    using Regs = MatrixRegs<float, ST, kApplyThreads>;
+    constexpr int NV = Regs::NV, PER = Regs::PER;
+    static_assert(ST % 4 == 0 && VS % 4 == 0 && BS <= 3, "whole 16-byte vectors, tile_of covers 6 tiles");
+    static_assert(32 % FL == 0 && FL % 4 == 0, "flush period divides the 32 steps of an item");
+    if (dev::all_stopped(stop, K)) return;
+    __shared__ __attribute__((aligned(16))) float Gt[NTILE * kTileStride];
+    __shared__ __attribute__((aligned(16))) float xv[2 * K * VS], tv[2 * K * VS];
+    __shared__ double PQ[K * BS * D];
+    const int i = threadIdx.x;
+    const int32_t first = int32_t(blockIdx.x) * NB;
+    if (first >= nbanks) return;
+    const int32_t last = min(nbanks, first + NB);
+    Regs mr;
+    mr.issue(inv + int64_t(first) * ST);
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    for (int e = i; e < NTILE * kTileStride / 4; e += kApplyThreads)
+        reinterpret_cast<F4 *>(Gt)[e] = F4{0.0f, 0.0f, 0.0f, 0.0f};
+    __syncthreads();  // the zeros land before any factor entry
+    const uint16_t *stage = tiled_layout<BS>().stage;
+    char *Gb = reinterpret_cast<char *>(Gt);
+    for (int32_t bank = first; bank < last; ++bank) {
+        const int32_t lo = part_off[bank];
+        const int n = (part_off[bank + 1] - lo) * BS;
+        const int64_t gi = int64_t(lo) * BS + i;
+        int off[PER];
+#pragma unroll
+        for (int qq = 0; qq < PER; ++qq) {
+            const int t = i + qq * kApplyThreads;
+            off[qq] = t < NV ? stage[t] : 0;
+        }
+        double cw[K], ca[K], cd[K];
+        if (i < n) {
+            const int64_t pi = w1 ? int64_t(parent[lo + i / BS]) * BS + (i % BS) : 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                cw[k] = w1 ? w1[k * ld1 + pi] : 0.0;
+                ca[k] = add ? add[k * ld + gi] : 0.0;
+                if constexpr (DOT) cd[k] = dotv[k * ld + gi];
+            }
+        }
+        split_bank_vector<BS, K>(r + int64_t(lo) * BS, ld, n, xv);
+#pragma unroll
+        for (int qq = 0; qq < PER; ++qq)
+            if (i + qq * kApplyThreads < NV) *reinterpret_cast<typename Regs::V *>(Gb + off[qq]) = mr.reg[qq];
+        __syncthreads();
+        if (bank + 1 < last) mr.issue(inv + int64_t(bank + 1) * ST);
+        mfma_solve<BS, K, FL>(Gt, xv, tv, PQ, PQ);
+        double part[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) part[k] = 0.0;
+        if (i < n) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                double y = bank_result<BS>(PQ, k, i);
+                if (w1) y += cw[k];
+                y *= scale;
+                if (add) y += ca[k];
+                z[k * ld + gi] = y;
+                if constexpr (DOT) part[k] = cd[k] * y;
+            }
+        }
+        if constexpr (DOT) {
+            __shared__ double sh[kApplyThreads / 64];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const double tot = dev::block_sum<kApplyThreads>(part[k], sh);
+                if (threadIdx.x == 0) partials[int64_t(k) * nbanks + bank] = tot;
+            }
+        }
+        __syncthreads();  // Gt, xv and Q are rewritten for the next bank
+    }
+}
+
 // z += scale * w1[parent] (the level-1 correction prolongated to level 0) with an
 // optional per-workgroup partial of dotv . z and an optional FP32 copy z32 of the
 // result (same stride); grid row y = column y.
@@ -1185,14 +1423,27 @@ template <int K>
 void level0_apply_k(Precision prec, int bs, int32_t nbanks, const int32_t *part_off, const void *inv,
                     const double *r, int64_t ld, const int32_t *parent, const double *w1, int64_t ld1, double scale,
                     const double *add, double *z, const double *dotv, bool dot, double *partials, const int *stop,
-                    hipStream_t s)
+                    hipStream_t s, int mfma_flush)
 {
+#define LSS_L0_MFMA(BS, DOT, FL)                                                                               \
+    k_level0_apply_mfma<BS, DOT, K, FL><<<div_up(nbanks, tiled_banks_per_group(K)), kApplyThreads, 0, s>>>(    \
+        nbanks, part_off, static_cast<const float *>(inv), r, ld, parent, w1, ld1, scale, add, z, dotv, partials, \
+        stop)
 #define LSS_L0(BS, TS, DOT)                                                                                    \
-    if constexpr (std::is_same_v<TS, float>)                                                                   \
-        k_level0_apply_tiled<BS, DOT, K><<<div_up(nbanks, tiled_banks_per_group(K)), kApplyThreads, 0, s>>>(   \
-            nbanks, part_off, static_cast<const float *>(inv), r, ld, parent, w1, ld1, scale, add, z, dotv,     \
-            partials, stop);                                                                                   \
-    else if constexpr (K == 1)                                                                                 \
+    if constexpr (std::is_same_v<TS, float>) {                                                                 \
+        switch (mfma_flush) {                                                                                  \
+            case 0:                                                                                            \
+                k_level0_apply_tiled<BS, DOT, K><<<div_up(nbanks, tiled_banks_per_group(K)), kApplyThreads, 0, \
+                                                   s>>>(nbanks, part_off, static_cast<const float *>(inv), r, ld, \
+                                                        parent, w1, ld1, scale, add, z, dotv, partials, stop); \
+                break;                                                                                         \
+            case 4: LSS_L0_MFMA(BS, DOT, 4); break;                                                            \
+            case 8: LSS_L0_MFMA(BS, DOT, 8); break;                                                            \
+            case 16: LSS_L0_MFMA(BS, DOT, 16); break;                                                          \
+            case 32: LSS_L0_MFMA(BS, DOT, 32); break;                                                          \
+            default: fail(LSS_ERR_ARG, "mfma_flush must be 0, 4, 8, 16 or 32");                                \
+        }                                                                                                      \
+    } else if constexpr (K == 1)                                                                                 \
         k_level0_apply2<BS, TS, DOT, K><<<div_up(nbanks, 2), kApplyThreads, 0, s>>>(                           \
             nbanks, part_off, static_cast<const TS *>(inv), r, ld, parent, w1, ld1, scale, add, z, dotv,        \
             partials, stop);                                                                                   \
@@ -1216,6 +1467,7 @@ void level0_apply_k(Precision prec, int bs, int32_t nbanks, const int32_t *part_
     }
 #undef LSS_L0_BS
 #undef LSS_L0
+#undef LSS_L0_MFMA
     LSS_LAUNCH_CHECK();
 }
 
@@ -1253,22 +1505,22 @@ void mas_coarse_apply(Precision prec, int bs, int32_t nodes, const void *inv, co
 void mas_level0_apply(Precision prec, int bs, int32_t nbanks, const int32_t *part_off, const void *inv,
                       const double *r, const int32_t *parent, const double *w1, double scale, const double *add,
                       double *z, const double *dotv, dev::PcgState *st, double *partials, const int *stop,
-                      hipStream_t s, int k, int64_t ld, int64_t ld1)
+                      hipStream_t s, int k, int64_t ld, int64_t ld1, int mfma_flush)
 {
     if (nbanks == 0) return;
     const bool dot = st != nullptr;
     switch (k) {
         case 1:
             level0_apply_k<1>(prec, bs, nbanks, part_off, inv, r, ld, parent, w1, ld1, scale, add, z, dotv, dot,
-                              partials, stop, s);
+                              partials, stop, s, mfma_flush);
             break;
         case 2:
             level0_apply_k<2>(prec, bs, nbanks, part_off, inv, r, ld, parent, w1, ld1, scale, add, z, dotv, dot,
-                              partials, stop, s);
+                              partials, stop, s, mfma_flush);
             break;
         case 4:
             level0_apply_k<4>(prec, bs, nbanks, part_off, inv, r, ld, parent, w1, ld1, scale, add, z, dotv, dot,
-                              partials, stop, s);
+                              partials, stop, s, mfma_flush);
             break;
         default: fail(LSS_ERR_ARG, "a fused MAS apply takes 1, 2 or 4 right-hand sides");
     }

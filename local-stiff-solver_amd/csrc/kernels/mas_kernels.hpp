@@ -44,10 +44,13 @@ int mas_prolong_add(int64_t np, int bs, const int32_t *parent, const double *w1,
                     const double *dotv, double *partials, const int *stop, hipStream_t s, int k = 1,
                     int64_t ld = 0, int64_t ld1 = 0, float *z32 = nullptr);
 // st != nullptr: per-bank partials of dotv . z go to partials[c * nbanks + bank] (reduced by k::finish).
+// mfma_flush > 0 (FP32 factors only): the bank solve runs on the matrix cores with x and t
+// as hi/lo FP32 pairs, adding its FP32 accumulators into FP64 every mfma_flush (4, 8, 16
+// or 32) products; 0: the vector-unit solve with FP64 accumulation.
 void mas_level0_apply(Precision prec, int bs, int32_t nbanks, const int32_t *part_off, const void *inv,
                       const double *r, const int32_t *parent, const double *w1, double scale, const double *add,
                       double *z, const double *dotv, dev::PcgState *st, double *partials, const int *stop,
-                      hipStream_t s, int k = 1, int64_t ld = 0, int64_t ld1 = 0);
+                      hipStream_t s, int k = 1, int64_t ld = 0, int64_t ld1 = 0, int mfma_flush = 0);
 
 // ---------------------------------------------------------------- stiff blocks
 // Dense S_c A S_c^T per component from operator rows (original numbering).

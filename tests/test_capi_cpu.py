@@ -47,6 +47,8 @@ def _create(doc):
     ('{"solver": "GPUHybrid", ', "JSON"),
     ('{"solver": "MAS", "MAS": {"true_residual_period": -1}}', "true_residual_period"),
     ('{"solver": "GPUHybrid", "GPUHybrid": {"base_preconditioner": "ILU"}}', "base_preconditioner"),
+    ('{"solver": "GPUHybrid", "GPUHybrid": {"bank_apply": "wmma"}}', "bank_apply"),
+    ('{"solver": "GPUHybrid", "GPUHybrid": {"bank_apply": "mfma", "mfma_flush": 6}}', "mfma_flush"),
 ])
 def test_bad_config_is_rejected(doc, frag):
     rc, msg = _create(doc)

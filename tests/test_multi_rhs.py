@@ -68,6 +68,11 @@ def _singles(s, B, X0):
         # the AMG base preconditioner: one V-cycle per column
         ("elastic_contact", 10, 3, {"base_preconditioner": "AMG"}),
         ("laplace_sphere", 12, 2, {"base_preconditioner": "AMG", "do_hybrid_precond": False}),
+        # the matrix-core bank solve: each output column of an MFMA step depends only on
+        # its own B column, so fused columns equal single solves here too
+        ("elastic_contact", 10, 2, {"precision": "fp32", "bank_apply": "mfma"}),
+        ("elastic_contact", 10, 4, {"precision": "fp32", "bank_apply": "mfma", "mfma_flush": 32}),
+        ("laplace_sphere", 12, 3, {"precision": "fp32", "bank_apply": "mfma", "max_coarse_levels": 0}),
     ],
 )
 def test_fused_equals_single_solves(gpu, name, N, k, kw):
