@@ -190,6 +190,17 @@ int lss_debug_amg(const lss_solver *s, int32_t level, int32_t *nlevels, int32_t 
 /* z = B r (base preconditioner, undamped) and z = M^{-1} r (full hybrid preconditioner). */
 int lss_debug_base_apply(lss_solver *s, const double *r, double *z);
 int lss_debug_precond_apply(lss_solver *s, const double *r, double *z);
+/* Nested-dissection ordering and multifrontal structure that the sparse stiff
+ * factorization (units above max_dense_component; the cuDSS analysis of
+ * GPUHybrid.cu:1231-1249) builds for a symmetric graph: CSR adj_ptr[m + 1] /
+ * adj_idx over m vertices, no self loops; subgraphs of at most `leaf` vertices become
+ * one front.  Host only.  Always sets *nfronts and *nstruct; when order is non-NULL
+ * also fills order[m] (position -> vertex), per front (postorder) its first pivot
+ * position, pivot count and parent (-1 for a root), and struct_ptr[nfronts + 1] /
+ * struct_idx[*nstruct]: the update positions of each front, ascending. */
+int lss_debug_nested_dissection(int32_t m, const int32_t *adj_ptr, const int32_t *adj_idx, int32_t leaf,
+                                int32_t *order, int32_t *nfronts, int64_t *nstruct, int32_t *front_p0,
+                                int32_t *front_np, int32_t *front_parent, int64_t *struct_ptr, int32_t *struct_idx);
 
 #ifdef __cplusplus
 }

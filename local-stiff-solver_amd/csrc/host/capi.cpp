@@ -8,6 +8,7 @@
 #include "lss.h"
 #include "config.hpp"
 #include "solver.hpp"
+#include "sparse_stiff.hpp"
 #include "../kernels/kernels.hpp"
 
 struct lss_solver {
@@ -259,6 +260,32 @@ int lss_residual_norm_noalloc(int32_t n, int64_t nnz, const int32_t *d_colptr, c
 }
 
 void lss_destroy(lss_solver *s) { delete s; }
+
+int lss_debug_nested_dissection(int32_t m, const int32_t *adj_ptr, const int32_t *adj_idx, int32_t leaf,
+                                int32_t *order, int32_t *nfronts, int64_t *nstruct, int32_t *front_p0,
+                                int32_t *front_np, int32_t *front_parent, int64_t *struct_ptr, int32_t *struct_idx)
+{
+    return guarded_free([&] {
+        if (m < 0 || !adj_ptr || (m > 0 && adj_ptr[m] > 0 && !adj_idx) || !nfronts || !nstruct)
+            lss::fail(LSS_ERR_ARG, "invalid arguments");
+        for (int32_t v = 0; v < m; ++v)
+            if (adj_ptr[v + 1] < adj_ptr[v]) lss::fail(LSS_ERR_ARG, "adj_ptr must be non-decreasing");
+        for (int32_t e = 0; e < (m > 0 ? adj_ptr[m] : 0); ++e)
+            if (adj_idx[e] < 0 || adj_idx[e] >= m) lss::fail(LSS_ERR_ARG, "adj_idx out of range");
+        const lss::FrontTree T = lss::nested_dissection(m, adj_ptr, adj_idx, leaf);
+        *nfronts = T.fronts();
+        *nstruct = int64_t(T.uidx.size());
+        if (!order) return;
+        if (!front_p0 || !front_np || !front_parent || !struct_ptr || (!T.uidx.empty() && !struct_idx))
+            lss::fail(LSS_ERR_ARG, "output arrays missing");
+        std::memcpy(order, T.order.data(), size_t(m) * sizeof(int32_t));
+        std::memcpy(front_p0, T.p0.data(), T.p0.size() * sizeof(int32_t));
+        std::memcpy(front_np, T.np.data(), T.np.size() * sizeof(int32_t));
+        std::memcpy(front_parent, T.parent.data(), T.parent.size() * sizeof(int32_t));
+        std::memcpy(struct_ptr, T.uptr.data(), T.uptr.size() * sizeof(int64_t));
+        if (!T.uidx.empty()) std::memcpy(struct_idx, T.uidx.data(), T.uidx.size() * sizeof(int32_t));
+    });
+}
 
 int lss_debug_row_norms(const lss_solver *s, double *out, int64_t n)
 {

@@ -105,6 +105,8 @@ Config parse_config(const char *text, const Config &base)
                 else if (k == "check_every") c.check_every = int(as_int(v, k));
                 else if (k == "true_residual_period") c.true_residual_period = int(as_int(v, k));
                 else if (k == "max_dense_component") c.max_dense_component = as_int(v, k);
+                else if (k == "stiff_factorization") c.stiff_factorization = as_str(v, k);
+                else if (k == "sparse_leaf_size") c.sparse_leaf_size = int(as_int(v, k));
                 else if (k == "profile_kernels") c.profile_kernels = as_bool(v, k);
                 else if (k == "partitioner") c.partitioner = as_str(v, k);
                 else if (k == "max_coarse_levels") c.max_coarse_levels = int(as_int(v, k));
@@ -145,6 +147,8 @@ Config parse_config(const char *text, const Config &base)
         else if (k == "damping") c.damping = as_float(v, k);
         else if (k == "lanczos_steps") c.lanczos_steps = int(as_int(v, k));
         else if (k == "max_dense_component") c.max_dense_component = as_int(v, k);
+        else if (k == "stiff_factorization") c.stiff_factorization = as_str(v, k);
+        else if (k == "sparse_leaf_size") c.sparse_leaf_size = int(as_int(v, k));
         else if (k == "profile_kernels") c.profile_kernels = as_bool(v, k);
         else if (k == "true_residual_check") c.true_residual_check = as_bool(v, k);
         else if (k == "partitioner") c.partitioner = as_str(v, k);
@@ -169,6 +173,9 @@ Config validate(Config c)
     if (c.base_preconditioner != "MAS" && c.base_preconditioner != "Jacobi" && c.base_preconditioner != "AMG")
         bad("base_preconditioner must be MAS|AMG|Jacobi");
     if (c.bank_apply != "vector" && c.bank_apply != "mfma") bad("bank_apply must be vector|mfma");
+    if (c.stiff_factorization != "auto" && c.stiff_factorization != "dense" && c.stiff_factorization != "sparse")
+        bad("stiff_factorization must be auto|dense|sparse");
+    if (c.sparse_leaf_size < 1) bad("sparse_leaf_size must be >= 1");
     if (c.mfma_flush != 4 && c.mfma_flush != 8 && c.mfma_flush != 16 && c.mfma_flush != 32)
         bad("mfma_flush must be 4, 8, 16 or 32");
     if (c.check_every < 1) bad("check_every must be >= 1");

@@ -57,6 +57,10 @@ struct Config {
     double damping = 1.8;               // omega = damping / lambda_max(B A) inside the B-P-B sweep
     int lanczos_steps = 20;          // CG-Lanczos steps of the lambda_max(B A) estimate
     int64_t max_dense_component = 16384;  // largest stiff component for the dense Cholesky
+    // units above max_dense_component: "auto" = multifrontal sparse Cholesky, "dense" = rejected;
+    // "sparse" = every unit outside the dense branch takes the sparse path
+    std::string stiff_factorization = "auto";
+    int sparse_leaf_size = 256;  // nested dissection: subgraphs this small are one front
     bool profile_kernels = false;       // time SpMV / MAS launches with HIP events
     bool true_residual_check = false;   // on convergence verify ||b - A x|| and restart PCG from it if needed
     std::string partitioner = "device";  // MAS partition: "device" (HIP kernels) | "host" (OpenMP)
@@ -93,7 +97,7 @@ struct Config {
                         bad_dof_kmeans_max_iterations, bad_dof_kmeans_expand_neighbors, bad_dof_gmm_jump_threshold,
                         bad_dof_gmm_tol, bad_dof_gmm_max_iterations, bad_dof_gmm_expand_neighbors, precision,
                         inner_spmv, base_preconditioner, damping, lanczos_steps, max_dense_component, partitioner,
-                        max_coarse_levels, partition_local_max);
+                        max_coarse_levels, partition_local_max, stiff_factorization, sparse_leaf_size);
     }
     bool same_factorization(const Config &o) const { return factor_keys() == o.factor_keys(); }
 };

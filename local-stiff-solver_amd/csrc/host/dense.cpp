@@ -60,6 +60,13 @@ int64_t DenseBlas::work_size(int64_t m)
 
 void DenseBlas::factor_inverse(int64_t m, double *A, double *G, double *work, int *status, hipStream_t s)
 {
+    partial_factor_inverse(m, m, A, G, work, status, s);
+    k::dense_mirror(m, G, s);
+}
+
+void DenseBlas::partial_factor_inverse(int64_t m, int64_t p, double *A, double *G, double *work, int *status,
+                                       hipStream_t s)
+{
     if (!handle_) {
         rocblas_handle h = nullptr;
         rb_check(rocblas_create_handle(&h), "create_handle");
@@ -68,9 +75,11 @@ void DenseBlas::factor_inverse(int64_t m, double *A, double *G, double *work, in
     rocblas_handle h = static_cast<rocblas_handle>(handle_);
     rb_check(rocblas_set_stream(h, s), "set_stream");
     const int64_t nb = k::dense_block_size();
-    k::fill(m * m, G, 0.0, s);
-    for (int64_t k0 = 0; k0 < m; k0 += nb) {
-        const int64_t kb = std::min(nb, m - k0), r = m - k0 - kb;
+    // the GEMMs read whole blocks of G (the diagonal blocks' and G11's upper parts included):
+    // those entries must be zero, whatever the buffer held before
+    k::fill(p * m, G, 0.0, s);
+    for (int64_t k0 = 0; k0 < p; k0 += nb) {
+        const int64_t kb = std::min(nb, p - k0), r = m - k0 - kb;
         k::dense_diag_block(m, int(k0), int(kb), A, G, status, s);
         if (r == 0) break;
         // panel L_ik = A_ik G_kk^T into work (r x kb), then back into A
@@ -85,8 +94,7 @@ void DenseBlas::factor_inverse(int64_t m, double *A, double *G, double *work, in
                                &alpha, work, rocblas_int(kb), &beta, A + (k0 + kb) * m + k0 + kb, rocblas_int(m)),
                  "dsyrk");
     }
-    inverse(0, m, m, A, G, work, s);
-    k::dense_mirror(m, G, s);
+    inverse(0, p, m, A, G, work, s);
 }
 
 // G(a:b, a:b) = L(a:b, a:b)^{-1}; a is a multiple of the block size, b = a + a

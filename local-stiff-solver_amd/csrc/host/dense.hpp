@@ -21,6 +21,12 @@ public:
     // L^{-1} in the lower triangle and its transpose above, the layout of the
     // one-workgroup path.  status |= 2 when a diagonal block is not positive definite.
     void factor_inverse(int64_t m, double *A, double *G, double *work, int *status, hipStream_t s);
+    // Partial factorization of a front (m x m row-major, lower triangle read): the first p
+    // columns become [L11; L21], the trailing block A22 - L21 L21^T (lower triangle), and
+    // G (leading dimension m) receives L11^{-1} in its leading p x p lower triangle; its
+    // first p rows are overwritten (zero above the diagonal).
+    void partial_factor_inverse(int64_t m, int64_t p, double *A, double *G, double *work, int *status,
+                                hipStream_t s);
 
 private:
     void *handle_ = nullptr;  // rocblas_handle

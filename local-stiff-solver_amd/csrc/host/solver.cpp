@@ -118,7 +118,9 @@ void HybridSolver::reset_factorization()
     mas_.clear();
     comps_.clear();
     ncomp_ = 0;
-    ns_ = 0;
+    ns_ = ns_dense_ = 0;
+    sparse_.clear();
+    sparse_off_.clear();
     omega_ = 1.0;
     d_unit_dense_.release();
     dense_bins_ = dense_dim_ = dense_dofs_ = 0;
@@ -151,10 +153,13 @@ void HybridSolver::release()
     d_L_.release();
     d_swork_.release();
     d_unit_dense_.release();
+    d_sparse_t_.release();
+    sparse_.clear();
+    sparse_off_.clear();
     xp_.release(), bp_.release(), r_.release(), p_.release(), q_.release(), z_.release(), u_.release();
     z32_.release();
     ncomp_ = 0;
-    ns_ = 0;
+    ns_ = ns_dense_ = 0;
     omega_ = 1.0;
 }
 
@@ -869,36 +874,91 @@ void HybridSolver::build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx
         dense.assign(units.size(), 0);
         bin_sizes_.clear();
     }
+    // Units above max_dense_component take the multifrontal path ("stiff_factorization"
+    // "auto"; "sparse": every unit outside the dense branch; "dense": such units are
+    // rejected).  The dense units keep the order of the list, the sparse ones follow.
+    std::vector<std::vector<int32_t>> sparse_units;
+    {
+        std::vector<std::vector<int32_t>> du;
+        std::vector<uint8_t> dd;
+        max_comp_ = 0;
+        for (size_t c = 0; c < units.size(); ++c) {
+            const int64_t m = int64_t(units[c].size());
+            max_comp_ = std::max(max_comp_, m);
+            const bool sp = !dense[c] && (cfg_.stiff_factorization == "sparse" ||
+                                          (cfg_.stiff_factorization == "auto" && m > cfg_.max_dense_component));
+            if (!sp && m > cfg_.max_dense_component)
+                fail(LSS_ERR_UNSUPPORTED, "stiff component of " + std::to_string(m) +
+                                              " DOFs exceeds max_dense_component (" +
+                                              std::to_string(cfg_.max_dense_component) + ")");
+            if (sp) {
+                sparse_units.push_back(std::move(units[c]));
+            } else {
+                du.push_back(std::move(units[c]));
+                dd.push_back(dense[c]);
+            }
+        }
+        units = std::move(du);
+        dense = std::move(dd);
+    }
     ncomp_ = int32_t(units.size());
     std::vector<int64_t> off(ncomp_ + 1, 0), moff(ncomp_ + 1, 0);
-    max_comp_ = 0;
     for (int32_t c = 0; c < ncomp_; ++c) {
         const int64_t m = int64_t(units[c].size());
-        if (m > cfg_.max_dense_component)
-            fail(LSS_ERR_UNSUPPORTED, "stiff component of " + std::to_string(m) +
-                                          " DOFs exceeds max_dense_component (" +
-                                          std::to_string(cfg_.max_dense_component) + ")");
-        max_comp_ = std::max(max_comp_, m);
         off[c + 1] = off[c] + m;
         moff[c + 1] = moff[c] + m * m;
     }
-    ns_ = off[ncomp_];
-    std::vector<int32_t> dofs(ns_), pd(ns_);
+    ns_dense_ = off[ncomp_];
+    sparse_off_.assign(1, 0);
+    for (auto &u : sparse_units) sparse_off_.push_back(sparse_off_.back() + int64_t(u.size()));
+    ns_ = ns_dense_ + sparse_off_.back();
+    std::vector<int32_t> dofs(ns_dense_), pd(ns_);
     for (int32_t c = 0; c < ncomp_; ++c)
         for (size_t r = 0; r < units[c].size(); ++r) {
             const int32_t d = units[c][r];
             dofs[off[c] + r] = d;
             pd[off[c] + r] = perm_[d / bs_] * bs_ + d % bs_;
         }
+    for (size_t u = 0; u < sparse_units.size(); ++u)
+        for (size_t r = 0; r < sparse_units[u].size(); ++r) {
+            const int32_t d = sparse_units[u][r];
+            pd[ns_dense_ + sparse_off_[u] + r] = perm_[d / bs_] * bs_ + d % bs_;
+        }
+    d_pdofs_.upload(pd.data(), ns_, s);
+    sparse_.clear();
+    sparse_.resize(sparse_units.size());
+    if (!sparse_units.empty()) {
+        std::vector<int32_t> h_ptr(size_t(n) + 1);
+        LSS_HIP(hipMemcpyAsync(h_ptr.data(), ptr, (size_t(n) + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        sync(s);
+        int64_t mmax = 0;
+        d_status_.zero(s);
+        for (size_t u = 0; u < sparse_units.size(); ++u) {
+            sparse_[u].factorize(n, sparse_units[u], h_ptr, ptr, idx, vals, cfg_.sparse_leaf_size, kMaxCols, dense_,
+                                 d_status_.get(), s);
+            mmax = std::max<int64_t>(mmax, int64_t(sparse_units[u].size()));
+        }
+        int st = 0;
+        d_status_.download(&st, 1, s);
+        sync(s);
+        if (st) fail(LSS_ERR_NOT_SPD, "stiff block Cholesky failed: a selected sub-block is not SPD");
+        d_sparse_t_.resize(size_t(kMaxCols) * mmax);
+    } else {
+        d_sparse_t_.release();
+    }
+    if (ncomp_ == 0) {
+        d_unit_dense_.release();
+        d_swork_.resize(1);
+        return;
+    }
     d_comp_off_.upload(off.data(), ncomp_ + 1, s);
     d_mat_off_.upload(moff.data(), ncomp_ + 1, s);
-    d_pdofs_.upload(pd.data(), ns_, s);
-    std::vector<int32_t> rc(ns_);
+    std::vector<int32_t> rc(ns_dense_);
     for (int32_t c = 0; c < ncomp_; ++c)
         for (int64_t k = off[c]; k < off[c + 1]; ++k) rc[k] = c;
-    d_row_comp_.upload(rc.data(), ns_, s);
+    d_row_comp_.upload(rc.data(), ns_dense_, s);
     DBuf<int32_t> d_dofs, g2l(n), comp_of(n);
-    d_dofs.upload(dofs.data(), ns_, s);
+    d_dofs.upload(dofs.data(), ns_dense_, s);
     k::stiff_maps(ncomp_, n, d_comp_off_.get(), d_dofs.get(), g2l.get(), comp_of.get(), s);
     d_L_.resize(moff[ncomp_]);
     d_L_.zero(s);
@@ -952,7 +1012,7 @@ void HybridSolver::build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx
     } else {
         d_unit_dense_.release();
     }
-    d_swork_.resize(std::max<int64_t>(2 * kMaxCols * ns_, 1));
+    d_swork_.resize(std::max<int64_t>(2 * kMaxCols * ns_dense_, 1));
 }
 
 void HybridSolver::alloc_vectors()
@@ -1142,6 +1202,18 @@ void HybridSolver::base_apply(const double *r, double scale, const double *add, 
     if (st) k::finish(g, partials_.get(), op, st, nullptr, 0, s, k);
 }
 
+void HybridSolver::sparse_correct(const double *r, double *z, const int *stop, hipStream_t s, int k, int64_t ld,
+                                  bool additive, float *z32)
+{
+    // per sparse unit: t = (r - A z)[unit rows] (additive: r[unit rows]), z[unit] += (S A S^T)^{-1} t
+    for (size_t u = 0; u < sparse_.size(); ++u) {
+        const int32_t *pd = d_pdofs_.get() + ns_dense_ + sparse_off_[u];
+        k::stiff_residual_rows(sparse_[u].dofs(), bs_, pd, d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r,
+                               additive ? nullptr : z, d_sparse_t_.get(), stop, s, k, ld);
+        sparse_[u].solve_add(d_sparse_t_.get(), pd, z, z32, ld, k, stop, s);
+    }
+}
+
 void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, const int *stop, hipStream_t s,
                                  bool first, int k)
 {
@@ -1150,14 +1222,15 @@ void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, 
         // MASSolverImpl::apply_preconditioner, MASSolver.cu:583-615: z = B r (+ S^T D^{-1} S r)
         base_apply(r, 1.0, nullptr, z, st, stop, s, first, k);
         if (ncomp_)
-            k::stiff_correct(ncomp_, ns_, bs_, d_comp_off_.get(), d_mat_off_.get(), d_row_comp_.get(),
+            k::stiff_correct(ncomp_, ns_dense_, bs_, d_comp_off_.get(), d_mat_off_.get(), d_row_comp_.get(),
                              d_pdofs_.get(), d_L_.get(), d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z,
-                             d_swork_.get(), d_swork_.get() + kMaxCols * ns_, stop, s, k, np, true, nullptr,
+                             d_swork_.get(), d_swork_.get() + kMaxCols * ns_dense_, stop, s, k, np, true, nullptr,
                              d_unit_dense_.get());
+        sparse_correct(r, z, stop, s, k, np, true, nullptr);
         return;
     }
     // custom_mixed_precond_iter, GPUHybrid.cu:521-542
-    if (!cfg_.do_hybrid_precond || ncomp_ == 0) {
+    if (!cfg_.do_hybrid_precond || !has_stiff()) {
         base_apply(r, 1.0, nullptr, z, st, stop, s, first, k);
         return;
     }
@@ -1167,9 +1240,12 @@ void HybridSolver::precond_apply(const double *r, double *z, dev::PcgState *st, 
     float *z32 = f32 ? z32_.get() : nullptr;
     base_apply(r, omega_, nullptr, z, nullptr, stop, s, false, k, z32);  // z1 = B r
     // z2 = z1 + P (r - A z1)  (dss_precond_iter, GPUHybrid.cu:544-675)
-    k::stiff_correct(ncomp_, ns_, bs_, d_comp_off_.get(), d_mat_off_.get(), d_row_comp_.get(), d_pdofs_.get(),
-                     d_L_.get(), d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z, d_swork_.get(),
-                     d_swork_.get() + kMaxCols * ns_, stop, s, k, np, false, z32, d_unit_dense_.get());
+    k::stiff_correct(ncomp_, ns_dense_, bs_, d_comp_off_.get(), d_mat_off_.get(), d_row_comp_.get(),
+                     d_pdofs_.get(), d_L_.get(), d_rowptr_.get(), d_cols_.get(), d_bvals_.get(), r, z, d_swork_.get(),
+                     d_swork_.get() + kMaxCols * ns_dense_, stop, s, k, np, false, z32, d_unit_dense_.get());
+    // units are decoupled (no stored entry joins two of them), so the dense units'
+    // corrections leave the sparse units' residual rows unchanged
+    sparse_correct(r, z, stop, s, k, np, false, z32);
     // u = r - A z2
     const bool prof = cfg_.profile_kernels && st && k == 1;
     if (prof) prof_begin(1, s);
@@ -1358,7 +1434,7 @@ void HybridSolver::solve_device(const double *b, double *x, hipStream_t s)
     // omega = damping / theta keeps omega * lambda_max(B A) < 2 while theta is within
     // damping / 2 of lambda_max; should the sweep still lose definiteness (a PCG
     // breakdown), solve again from the same start with the safe damping
-    const bool damped = !cfg_.mas_mode() && cfg_.do_hybrid_precond && ncomp_ > 0 && cfg_.damping > kSafeDamping &&
+    const bool damped = !cfg_.mas_mode() && cfg_.do_hybrid_precond && has_stiff() && cfg_.damping > kSafeDamping &&
                         cfg_.base_preconditioner != "AMG";
     if (damped) {
         x0p_.resize(size_t(np));
@@ -1407,7 +1483,7 @@ void HybridSolver::solve_multi_device(int32_t nrhs, const double *B, double *X, 
     }
     auto t0 = clk::now();
     const int64_t np = int64_t(nb_) * bs_;
-    const bool damped = cfg_.do_hybrid_precond && ncomp_ > 0 && cfg_.damping > kSafeDamping &&
+    const bool damped = cfg_.do_hybrid_precond && has_stiff() && cfg_.damping > kSafeDamping &&
                         cfg_.base_preconditioner != "AMG";
     std::vector<int> iters(nrhs), stops(nrhs);
     std::vector<double> res(nrhs);
@@ -1578,6 +1654,22 @@ std::string HybridSolver::info_json() const
       << ",\"n_stiff_dofs\":" << nst << ",\"max_component\":" << max_comp_ << ",\"dense_branch\":{\"bins\":"
       << dense_bins_ << ",\"dim\":" << dense_dim_ << ",\"dofs\":" << dense_dofs_ << ",\"bin_sizes\":[";
     for (size_t b = 0; b < bin_sizes_.size(); ++b) o << (b ? "," : "") << bin_sizes_[b];
+    o << "]},\"sparse\":{\"units\":" << sparse_.size() << ",\"dofs\":[";
+    for (size_t u = 0; u < sparse_.size(); ++u) o << (u ? "," : "") << sparse_[u].dofs();
+    o << "],\"fronts\":[";
+    for (size_t u = 0; u < sparse_.size(); ++u) o << (u ? "," : "") << sparse_[u].fronts();
+    o << "],\"levels\":[";
+    for (size_t u = 0; u < sparse_.size(); ++u) o << (u ? "," : "") << sparse_[u].levels();
+    o << "],\"max_front\":[";
+    for (size_t u = 0; u < sparse_.size(); ++u) o << (u ? "," : "") << sparse_[u].max_front();
+    o << "],\"factor_entries\":[";
+    for (size_t u = 0; u < sparse_.size(); ++u) o << (u ? "," : "") << sparse_[u].factor_entries();
+    o << "],\"update_stack_bytes\":[";
+    for (size_t u = 0; u < sparse_.size(); ++u) o << (u ? "," : "") << sparse_[u].stack_bytes();
+    o << "],\"t_symbolic\":[";
+    for (size_t u = 0; u < sparse_.size(); ++u) o << (u ? "," : "") << sparse_[u].t_symbolic;
+    o << "],\"t_numeric\":[";
+    for (size_t u = 0; u < sparse_.size(); ++u) o << (u ? "," : "") << sparse_[u].t_numeric;
     o << "]}},\"mas\":{\"levels\":" << mas_levels_
       << ",\"banks\":" << mas_banks_ << ",\"inverse_bytes\":" << mas_bytes_ << ",\"parts\":"
       << (part_off_.empty() ? 0 : part_off_.size() - 1) << ",\"level_nodes\":[";

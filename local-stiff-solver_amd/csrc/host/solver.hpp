@@ -13,6 +13,7 @@
 #include "../kernels/mas_kernels.hpp"
 #include "config.hpp"
 #include "dense.hpp"
+#include "sparse_stiff.hpp"
 #include "stiff_select.hpp"
 #include "topology.hpp"
 
@@ -171,9 +172,16 @@ private:
     } amg_times_;
     DBuf<double> amg_z_;  // V-cycle output before scaling
 
-    // stiff blocks
+    // stiff blocks: ncomp_ dense units (their rows first in d_pdofs_), then the units
+    // factored by the multifrontal path (sparse_, rows from ns_dense_ on); ns_ = all rows
     int32_t ncomp_ = 0;
-    int64_t ns_ = 0;
+    int64_t ns_ = 0, ns_dense_ = 0;
+    std::vector<SparseFactor> sparse_;
+    std::vector<int64_t> sparse_off_;  // first row of each sparse unit after ns_dense_
+    DBuf<double> d_sparse_t_;          // residual of a sparse unit's rows, kMaxCols columns
+    bool has_stiff() const { return ncomp_ > 0 || !sparse_.empty(); }
+    void sparse_correct(const double *r, double *z, const int *stop, hipStream_t s, int k, int64_t ld, bool additive,
+                        float *z32);
     int64_t max_comp_ = 0;
     DBuf<int64_t> d_comp_off_, d_mat_off_;
     DBuf<int32_t> d_pdofs_, d_row_comp_;

@@ -93,6 +93,11 @@ void stiff_correct(int32_t ncomp, int64_t ns, int bs, const int64_t *comp_off, c
 void explicit_inverse(int32_t nunits, const int32_t *units, int32_t max_m, const int64_t *comp_off,
                       const int64_t *mat_off, const double *G, double *M, hipStream_t s);
 
+// out[i] = r[pdofs[i]] - (A z)[pdofs[i]] (z == nullptr: r[pdofs[i]]) for i < ns, k columns:
+// r, z at stride ld, out at stride ns.
+void stiff_residual_rows(int64_t ns, int bs, const int32_t *pdofs, const int32_t *rowptr, const int32_t *cols,
+                         const double *bvals, const double *r, const double *z, double *out, const int *stop,
+                         hipStream_t s, int k = 1, int64_t ld = 0);
 // u[p] = r[p] - (A z)[p] in FP64 for the stiff rows p: after a preconditioner-internal
 // SpMV with FP32 values, the rows that hold the stiff entries are recomputed exactly.
 void stiff_rows_exact(int64_t ns, int bs, const int32_t *pdofs, const int32_t *rowptr, const int32_t *cols,

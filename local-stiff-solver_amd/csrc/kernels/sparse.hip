@@ -1,0 +1,273 @@
+// Multifrontal sparse Cholesky of large stiff units: assembly and solve kernels.
+//
+// The reference factorizes every stiff component with cuDSS (sparse LDL^T,
+// GPUHybrid.cu:1231-1249; MAS/BadDofGMMPreconditioner.cu:493-549) and applies it
+// by triangular solves (GPUHybrid.cu:575).  Units up to max_dense_component DOFs
+// take a dense Cholesky here; larger ones a multifrontal one: a nested-dissection
+// ordering and the front structures on the host, then per front (children first)
+// assembly of the original entries, extend-add of the children's update matrices,
+// and a partial blocked Cholesky (dense kernels + rocBLAS, host/sparse_stiff.cpp).
+// A front keeps G11 = L11^{-1} (with its transpose in the strict upper triangle)
+// and L21, so both triangular solves are products.  The solve walks the tree
+// level by level: forward, bottom-up, each front gathers its right-hand side and
+// its children's update vectors (in child order), y = G11 r, w = r_U - L21 y;
+// backward, top-down, x = G11^T (y - L21^T x_U).  No atomics anywhere: every sum
+// has a fixed order, so solves are reproducible bit for bit.
+#include "sparse_kernels.hpp"
+
+#include "device_utils.hpp"
+#include "../common/runtime.hpp"
+
+namespace lss::k {
+namespace {
+
+constexpr int kT = 256;
+
+__global__ __launch_bounds__(kT) void k_gather_pattern(int32_t m, const int32_t *__restrict__ dofs,
+                                                       const int32_t *__restrict__ ptr,
+                                                       const int64_t *__restrict__ coff,
+                                                       const int32_t *__restrict__ idx, int32_t *__restrict__ out)
+{
+    const int32_t v = int32_t(blockIdx.x) * (kT / 64) + int32_t(threadIdx.x >> 6);
+    if (v >= m) return;
+    const int lane = threadIdx.x & 63;
+    const int32_t d = dofs[v];
+    const int32_t b = ptr[d], e = ptr[d + 1];
+    int32_t *o = out + coff[v];
+    for (int32_t t = b + lane; t < e; t += 64) o[t - b] = idx[t];
+}
+
+__global__ __launch_bounds__(kT) void k_assemble(int64_t ntrip, const int32_t *__restrict__ li,
+                                                 const int32_t *__restrict__ lj, const int64_t *__restrict__ src,
+                                                 const double *__restrict__ vals, double *__restrict__ F, int64_t ld)
+{
+    const int64_t t = int64_t(blockIdx.x) * kT + threadIdx.x;
+    if (t >= ntrip) return;
+    F[int64_t(li[t]) * ld + lj[t]] = vals[src[t]];
+}
+
+// one wave per row a of the child's update matrix
+__global__ __launch_bounds__(kT) void k_extend_add(int32_t u, const int32_t *__restrict__ rel,
+                                                   const double *__restrict__ U, double *__restrict__ F, int64_t ld)
+{
+    const int32_t a = int32_t(blockIdx.x) * (kT / 64) + int32_t(threadIdx.x >> 6);
+    if (a >= u) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t ra = int64_t(rel[a]) * ld;
+    const double *row = U + int64_t(a) * u;
+    for (int32_t b = lane; b <= a; b += 64) F[ra + rel[b]] += row[b];
+}
+
+__global__ __launch_bounds__(kT) void k_permute_in(int32_t m, const int32_t *__restrict__ order,
+                                                   const double *__restrict__ t, int64_t ldt, double *__restrict__ tp,
+                                                   const int *__restrict__ stop)
+{
+    if (dev::all_stopped(stop, gridDim.y)) return;
+    const int32_t q = int32_t(blockIdx.x) * kT + int32_t(threadIdx.x);
+    if (q >= m) return;
+    const int c = blockIdx.y;
+    tp[int64_t(c) * m + q] = t[int64_t(c) * ldt + order[q]];
+}
+
+__global__ __launch_bounds__(kT) void k_fwd_rhs(int64_t ntask, const int2 *__restrict__ tasks, FrontView fv,
+                                                const double *__restrict__ tp, int32_t m,
+                                                const double *__restrict__ w, int64_t wtot,
+                                                double *__restrict__ rhs, int64_t rtot,
+                                                const int *__restrict__ stop)
+{
+    if (dev::all_stopped(stop, gridDim.y)) return;
+    const int64_t t = int64_t(blockIdx.x) * kT + threadIdx.x;
+    if (t >= ntask) return;
+    const int c = blockIdx.y;
+    const int2 task = tasks[t];
+    const int f = task.x, a = task.y;
+    const int64_t r = fv.roff[f] + a;
+    double v = a < fv.np[f] ? tp[int64_t(c) * m + fv.p0[f] + a] : 0.0;
+    const double *wc = w + int64_t(c) * wtot;
+    for (int64_t q = fv.cptr[r]; q < fv.cptr[r + 1]; ++q) v += wc[fv.cidx[q]];
+    rhs[int64_t(c) * rtot + r] = v;
+}
+
+// wave per pivot row i: y_i = sum_{j <= i} G11(i, j) rhs_j
+__global__ __launch_bounds__(kT) void k_fwd_y(int64_t ntask, const int2 *__restrict__ tasks, FrontView fv,
+                                              const double *__restrict__ G, const double *__restrict__ rhs,
+                                              int64_t rtot, double *__restrict__ ypos, int32_t m,
+                                              const int *__restrict__ stop)
+{
+    if (dev::all_stopped(stop, gridDim.y)) return;
+    const int64_t t = int64_t(blockIdx.x) * (kT / 64) + (threadIdx.x >> 6);
+    if (t >= ntask) return;
+    const int lane = threadIdx.x & 63, c = blockIdx.y;
+    const int2 task = tasks[t];
+    const int f = task.x, i = task.y, p = fv.np[f];
+    const double *row = G + fv.goff[f] + int64_t(i) * p;
+    const double *v = rhs + int64_t(c) * rtot + fv.roff[f];
+    double acc = 0.0;
+    for (int j = lane; j <= i; j += 64) acc += row[j] * v[j];
+    acc = dev::wave_sum(acc);
+    if (lane == 0) ypos[int64_t(c) * m + fv.p0[f] + i] = acc;
+}
+
+// wave per update row a: w_a = rhs_{p + a} - sum_j L21(a, j) y_j
+__global__ __launch_bounds__(kT) void k_fwd_w(int64_t ntask, const int2 *__restrict__ tasks, FrontView fv,
+                                              const double *__restrict__ L, const double *__restrict__ rhs,
+                                              int64_t rtot, const double *__restrict__ ypos, int32_t m,
+                                              double *__restrict__ w, int64_t wtot, const int *__restrict__ stop)
+{
+    if (dev::all_stopped(stop, gridDim.y)) return;
+    const int64_t t = int64_t(blockIdx.x) * (kT / 64) + (threadIdx.x >> 6);
+    if (t >= ntask) return;
+    const int lane = threadIdx.x & 63, c = blockIdx.y;
+    const int2 task = tasks[t];
+    const int f = task.x, a = task.y, p = fv.np[f];
+    const double *row = L + fv.loff[f] + int64_t(a) * p;
+    const double *y = ypos + int64_t(c) * m + fv.p0[f];
+    double acc = 0.0;
+    for (int j = lane; j < p; j += 64) acc += row[j] * y[j];
+    acc = dev::wave_sum(acc);
+    if (lane == 0)
+        w[int64_t(c) * wtot + fv.woff[f] + a] = rhs[int64_t(c) * rtot + fv.roff[f] + p + a] - acc;
+}
+
+// wave per 64 pivot columns: s_j = y_j - sum_a L21(a, j) x[U_f(a)], a in row order
+__global__ __launch_bounds__(kT) void k_bwd_s(int64_t ntask, const int2 *__restrict__ tasks, FrontView fv,
+                                              const double *__restrict__ L, const double *__restrict__ xpos,
+                                              const double *__restrict__ ypos, int32_t m,
+                                              double *__restrict__ sbuf, int64_t rtot, const int *__restrict__ stop)
+{
+    if (dev::all_stopped(stop, gridDim.y)) return;
+    const int64_t t = int64_t(blockIdx.x) * (kT / 64) + (threadIdx.x >> 6);
+    if (t >= ntask) return;
+    const int lane = threadIdx.x & 63, c = blockIdx.y;
+    const int2 task = tasks[t];
+    const int f = task.x, p = fv.np[f], u = fv.nu[f];
+    const int j = task.y * 64 + lane;
+    if (j >= p) return;
+    const double *col = L + fv.loff[f] + j;
+    const int32_t *ui = fv.uidx + fv.woff[f];
+    const double *x = xpos + int64_t(c) * m;
+    double acc = 0.0;
+    for (int a = 0; a < u; ++a) acc += col[int64_t(a) * p] * x[ui[a]];
+    sbuf[int64_t(c) * rtot + fv.roff[f] + j] = ypos[int64_t(c) * m + fv.p0[f] + j] - acc;
+}
+
+// wave per pivot row i: x_i = sum_{j >= i} G11(j, i) s_j (G11^T stored in the strict upper triangle)
+__global__ __launch_bounds__(kT) void k_bwd_x(int64_t ntask, const int2 *__restrict__ tasks, FrontView fv,
+                                              const double *__restrict__ G, const double *__restrict__ sbuf,
+                                              int64_t rtot, double *__restrict__ xpos, int32_t m,
+                                              const int *__restrict__ stop)
+{
+    if (dev::all_stopped(stop, gridDim.y)) return;
+    const int64_t t = int64_t(blockIdx.x) * (kT / 64) + (threadIdx.x >> 6);
+    if (t >= ntask) return;
+    const int lane = threadIdx.x & 63, c = blockIdx.y;
+    const int2 task = tasks[t];
+    const int f = task.x, i = task.y, p = fv.np[f];
+    const double *row = G + fv.goff[f] + int64_t(i) * p;
+    const double *v = sbuf + int64_t(c) * rtot + fv.roff[f];
+    double acc = 0.0;
+    for (int j = i + lane; j < p; j += 64) acc += row[j] * v[j];
+    acc = dev::wave_sum(acc);
+    if (lane == 0) xpos[int64_t(c) * m + fv.p0[f] + i] = acc;
+}
+
+__global__ __launch_bounds__(kT) void k_scatter_add(int32_t m, const int32_t *__restrict__ order,
+                                                    const int32_t *__restrict__ pdofs,
+                                                    const double *__restrict__ xpos, double *__restrict__ z,
+                                                    float *__restrict__ z32, int64_t ld, const int *__restrict__ stop)
+{
+    if (dev::all_stopped(stop, gridDim.y)) return;
+    const int32_t q = int32_t(blockIdx.x) * kT + int32_t(threadIdx.x);
+    if (q >= m) return;
+    const int c = blockIdx.y;
+    const int64_t d = int64_t(c) * ld + pdofs[order[q]];
+    const double v = z[d] + xpos[int64_t(c) * m + q];
+    z[d] = v;
+    if (z32) z32[d] = float(v);
+}
+
+int waves_grid(int64_t ntask) { return div_up(ntask, kT / 64); }
+
+}  // namespace
+
+void sparse_gather_pattern(int32_t m, const int32_t *dofs, const int32_t *ptr, const int64_t *coff,
+                           const int32_t *idx, int32_t *out, hipStream_t s)
+{
+    if (m == 0) return;
+    k_gather_pattern<<<div_up(m, kT / 64), kT, 0, s>>>(m, dofs, ptr, coff, idx, out);
+    LSS_LAUNCH_CHECK();
+}
+
+void sparse_assemble(int64_t ntrip, const int32_t *li, const int32_t *lj, const int64_t *src, const double *vals,
+                     double *F, int64_t ld, hipStream_t s)
+{
+    if (ntrip == 0) return;
+    k_assemble<<<div_up(ntrip, kT), kT, 0, s>>>(ntrip, li, lj, src, vals, F, ld);
+    LSS_LAUNCH_CHECK();
+}
+
+void sparse_extend_add(int32_t u, const int32_t *rel, const double *U, double *F, int64_t ld, hipStream_t s)
+{
+    if (u == 0) return;
+    k_extend_add<<<div_up(u, kT / 64), kT, 0, s>>>(u, rel, U, F, ld);
+    LSS_LAUNCH_CHECK();
+}
+
+void sparse_permute_in(int32_t m, const int32_t *order, const double *t, int64_t ldt, double *tp, int k,
+                       const int *stop, hipStream_t s)
+{
+    if (m == 0) return;
+    k_permute_in<<<dim3(div_up(m, kT), k), kT, 0, s>>>(m, order, t, ldt, tp, stop);
+    LSS_LAUNCH_CHECK();
+}
+
+void sparse_fwd_rhs(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *tp, int32_t m,
+                    const double *w, int64_t wtot, double *rhs, int64_t rtot, int k, const int *stop, hipStream_t s)
+{
+    if (ntask == 0) return;
+    k_fwd_rhs<<<dim3(div_up(ntask, kT), k), kT, 0, s>>>(ntask, tasks, fv, tp, m, w, wtot, rhs, rtot, stop);
+    LSS_LAUNCH_CHECK();
+}
+
+void sparse_fwd_y(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *G, const double *rhs,
+                  int64_t rtot, double *ypos, int32_t m, int k, const int *stop, hipStream_t s)
+{
+    if (ntask == 0) return;
+    k_fwd_y<<<dim3(waves_grid(ntask), k), kT, 0, s>>>(ntask, tasks, fv, G, rhs, rtot, ypos, m, stop);
+    LSS_LAUNCH_CHECK();
+}
+
+void sparse_fwd_w(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *L, const double *rhs,
+                  int64_t rtot, const double *ypos, int32_t m, double *w, int64_t wtot, int k, const int *stop,
+                  hipStream_t s)
+{
+    if (ntask == 0) return;
+    k_fwd_w<<<dim3(waves_grid(ntask), k), kT, 0, s>>>(ntask, tasks, fv, L, rhs, rtot, ypos, m, w, wtot, stop);
+    LSS_LAUNCH_CHECK();
+}
+
+void sparse_bwd_s(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *L, const double *xpos,
+                  const double *ypos, int32_t m, double *sbuf, int64_t rtot, int k, const int *stop, hipStream_t s)
+{
+    if (ntask == 0) return;
+    k_bwd_s<<<dim3(waves_grid(ntask), k), kT, 0, s>>>(ntask, tasks, fv, L, xpos, ypos, m, sbuf, rtot, stop);
+    LSS_LAUNCH_CHECK();
+}
+
+void sparse_bwd_x(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *G, const double *sbuf,
+                  int64_t rtot, double *xpos, int32_t m, int k, const int *stop, hipStream_t s)
+{
+    if (ntask == 0) return;
+    k_bwd_x<<<dim3(waves_grid(ntask), k), kT, 0, s>>>(ntask, tasks, fv, G, sbuf, rtot, xpos, m, stop);
+    LSS_LAUNCH_CHECK();
+}
+
+void sparse_scatter_add(int32_t m, const int32_t *order, const int32_t *pdofs, const double *xpos, double *z,
+                        float *z32, int64_t ld, int k, const int *stop, hipStream_t s)
+{
+    if (m == 0) return;
+    k_scatter_add<<<dim3(div_up(m, kT), k), kT, 0, s>>>(m, order, pdofs, xpos, z, z32, ld, stop);
+    LSS_LAUNCH_CHECK();
+}
+
+}  // namespace lss::k

@@ -1,0 +1,55 @@
+// Launchers of the multifrontal sparse Cholesky kernels (stiff units too large for
+// one dense factor).  Front f of a unit holds the dense matrix of its row structure
+// R_f = [pivots p0 .. p0 + p - 1, update positions U_f], m = p + u rows, row-major.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lss::k {
+
+// Per-front tables of the solve, device copies (nfronts entries each).
+struct FrontView {
+    const int32_t *p0 = nullptr;     // first pivot position
+    const int32_t *np = nullptr;     // pivots p
+    const int32_t *nu = nullptr;     // update rows u
+    const int64_t *roff = nullptr;   // offset of the front's m-vector in the rhs buffer
+    const int64_t *woff = nullptr;   // offset of the front's u-vector in the update buffer (and in uidx)
+    const int64_t *goff = nullptr;   // offset of G11 (p x p: G lower, G^T strictly upper)
+    const int64_t *loff = nullptr;   // offset of L21 (u x p, row-major)
+    const int32_t *uidx = nullptr;   // update positions of every front (at woff)
+    const int64_t *cptr = nullptr;   // per front row (at roff): contributions of the children
+    const int64_t *cidx = nullptr;   // ... as indices into the update buffer, children in order
+};
+
+// out[coff[v] + t] = idx[ptr[dofs[v]] + t]: the stored rows of the unit's columns.
+void sparse_gather_pattern(int32_t m, const int32_t *dofs, const int32_t *ptr, const int64_t *coff,
+                           const int32_t *idx, int32_t *out, hipStream_t s);
+// F[li * ld + lj] = vals[src] for the front's original entries (distinct (li, lj)).
+void sparse_assemble(int64_t ntrip, const int32_t *li, const int32_t *lj, const int64_t *src, const double *vals,
+                     double *F, int64_t ld, hipStream_t s);
+// F[rel[a] * ld + rel[b]] += U[a * u + b] for b <= a < u (a child's update matrix).
+void sparse_extend_add(int32_t u, const int32_t *rel, const double *U, double *F, int64_t ld, hipStream_t s);
+
+// Solve (L L^T) x = t of one unit for k columns.  Vectors in elimination (position)
+// order: tp, ypos, xpos (stride m); rhs buffer (stride rtot), update buffer (stride wtot).
+// Level steps take task lists of (front, row) pairs (rows < m, < p, < u; chunks of 64
+// pivot columns for the backward L21^T product).
+void sparse_permute_in(int32_t m, const int32_t *order, const double *t, int64_t ldt, double *tp, int k,
+                       const int *stop, hipStream_t s);
+void sparse_fwd_rhs(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *tp, int32_t m,
+                    const double *w, int64_t wtot, double *rhs, int64_t rtot, int k, const int *stop, hipStream_t s);
+void sparse_fwd_y(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *G, const double *rhs,
+                  int64_t rtot, double *ypos, int32_t m, int k, const int *stop, hipStream_t s);
+void sparse_fwd_w(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *L, const double *rhs,
+                  int64_t rtot, const double *ypos, int32_t m, double *w, int64_t wtot, int k, const int *stop,
+                  hipStream_t s);
+void sparse_bwd_s(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *L, const double *xpos,
+                  const double *ypos, int32_t m, double *sbuf, int64_t rtot, int k, const int *stop, hipStream_t s);
+void sparse_bwd_x(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *G, const double *sbuf,
+                  int64_t rtot, double *xpos, int32_t m, int k, const int *stop, hipStream_t s);
+// z[pdofs[order[q]]] += xpos[q] (and z32 = float of the new z when given), k columns at stride ld.
+void sparse_scatter_add(int32_t m, const int32_t *order, const int32_t *pdofs, const double *xpos, double *z,
+                        float *z32, int64_t ld, int k, const int *stop, hipStream_t s);
+
+}  // namespace lss::k

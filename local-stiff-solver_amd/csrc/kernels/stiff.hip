@@ -446,6 +446,16 @@ void explicit_inverse(int32_t nunits, const int32_t *units, int32_t max_m, const
     LSS_LAUNCH_CHECK();
 }
 
+void stiff_residual_rows(int64_t ns, int bs, const int32_t *pdofs, const int32_t *rowptr, const int32_t *cols,
+                         const double *bvals, const double *r, const double *z, double *out, const int *stop,
+                         hipStream_t s, int k, int64_t ld)
+{
+    if (ns == 0) return;
+    k_stiff_residual<false><<<dim3(div_up(ns, 4), k), 256, 0, s>>>(ns, bs, pdofs, rowptr, cols, bvals, r, z, ld, out,
+                                                                  stop);
+    LSS_LAUNCH_CHECK();
+}
+
 void stiff_rows_exact(int64_t ns, int bs, const int32_t *pdofs, const int32_t *rowptr, const int32_t *cols,
                       const double *bvals, const double *r, const double *z, double *u, const int *stop, hipStream_t s,
                       int k, int64_t ld)

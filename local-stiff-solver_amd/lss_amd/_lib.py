@@ -30,7 +30,7 @@ EXPORTS = [
     "lss_solve_multi_device", "lss_get_info", "lss_name", "lss_get_stiff_dofs", "lss_last_error",
     "lss_destroy", "lss_adjoint_pattern_grad", "lss_adjoint_chain", "lss_debug_row_norms", "lss_debug_gmm",
     "lss_debug_partition", "lss_debug_spmv", "lss_debug_base_apply", "lss_debug_precond_apply",
-    "lss_residual_norm", "lss_residual_norm_noalloc", "lss_debug_amg",
+    "lss_residual_norm", "lss_residual_norm_noalloc", "lss_debug_amg", "lss_debug_nested_dissection",
 ]
 
 _lib = None
@@ -82,6 +82,8 @@ def load():
     L.lss_debug_spmv.argtypes = [P, P, P]
     L.lss_debug_base_apply.argtypes = [P, P, P]
     L.lss_debug_precond_apply.argtypes = [P, P, P]
+    L.lss_debug_nested_dissection.argtypes = [i32, P, P, i32, P, ctypes.POINTER(i32), ctypes.POINTER(i64), P, P, P,
+                                              P, P]
     for name in EXPORTS:
         if name not in ("lss_name", "lss_last_error", "lss_destroy"):
             getattr(L, name).restype = ctypes.c_int

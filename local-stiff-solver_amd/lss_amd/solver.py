@@ -302,3 +302,32 @@ def _csc_arrays(A):
         n, colptr, rowidx, vals = C.shape[0], C.indptr, C.indices, C.data
     return (int(n), np.ascontiguousarray(colptr, np.int32), np.ascontiguousarray(rowidx, np.int32),
             np.ascontiguousarray(vals, np.float64))
+
+
+def nested_dissection(adj_ptr, adj_idx, leaf: int = 256) -> dict:
+    """Ordering and front structure the sparse stiff factorization builds for a
+    symmetric graph (lss_debug_nested_dissection; host only, no GPU needed).
+    adj_ptr/adj_idx: CSR without self loops.  Returns order (position -> vertex) and,
+    per front in postorder, p0 / np / parent and its update positions struct_ptr /
+    struct_idx."""
+    lib = load()
+    ap = np.ascontiguousarray(adj_ptr, dtype=np.int32)
+    ai = np.ascontiguousarray(adj_idx, dtype=np.int32)
+    m = ap.size - 1
+    nf = ctypes.c_int32()
+    ns = ctypes.c_int64()
+
+    def call(*arrays):
+        rc = lib.lss_debug_nested_dissection(m, _ptr(ap), _ptr(ai) if ai.size else None, int(leaf), *arrays[:1],
+                                             ctypes.byref(nf), ctypes.byref(ns), *arrays[1:])
+        if rc:
+            raise LssError(rc, lib.lss_last_error(None).decode())
+
+    call(None, None, None, None, None, None)
+    out = {"order": np.empty(m, np.int32), "p0": np.empty(nf.value, np.int32), "np": np.empty(nf.value, np.int32),
+           "parent": np.empty(nf.value, np.int32), "struct_ptr": np.empty(nf.value + 1, np.int64),
+           "struct_idx": np.empty(max(ns.value, 1), np.int32)}
+    call(_ptr(out["order"]), _ptr(out["p0"]), _ptr(out["np"]), _ptr(out["parent"]), _ptr(out["struct_ptr"]),
+         _ptr(out["struct_idx"]))
+    out["struct_idx"] = out["struct_idx"][: ns.value]
+    return out

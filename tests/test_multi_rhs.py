@@ -73,6 +73,8 @@ def _singles(s, B, X0):
         ("elastic_contact", 10, 2, {"precision": "fp32", "bank_apply": "mfma"}),
         ("elastic_contact", 10, 4, {"precision": "fp32", "bank_apply": "mfma", "mfma_flush": 32}),
         ("laplace_sphere", 12, 3, {"precision": "fp32", "bank_apply": "mfma", "max_coarse_levels": 0}),
+        # stiff units through the multifrontal sparse factor (per-column level solves)
+        ("elastic_contact", 10, 3, {"stiff_factorization": "sparse", "sparse_leaf_size": 16}),
     ],
 )
 def test_fused_equals_single_solves(gpu, name, N, k, kw):
