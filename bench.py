@@ -300,8 +300,13 @@ def main():
                                    "iterations": mi["solver_iter"], "status": mi["solver_status"],
                                    "rel_residual": mi["final_res_norm"] / bnorm,
                                    "stiff_dofs": mi["stiff"]["n_stiff_dofs"],
-                                   "components": mi["stiff"]["n_components"]}
-            except Exception as e:  # a strategy can select a component too large for the dense factor
+                                   "components": mi["stiff"]["n_components"],
+                                   "max_component": mi["stiff"]["max_component"]}
+                sp = mi["stiff"]["sparse"]
+                if sp["units"]:  # units above max_dense_component: the multifrontal factor
+                    mas_runs[vname]["sparse"] = {k: sp[k] for k in ("dofs", "fronts", "levels", "max_front",
+                                                                     "factor_entries", "t_symbolic", "t_numeric")}
+            except Exception as e:  # e.g. a front beyond the sparse factor's dense workspace limit
                 mas_runs[vname] = {"error": str(e)}
             log(vname, mas_runs[vname])
             del s

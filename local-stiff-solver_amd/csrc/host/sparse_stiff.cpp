@@ -219,9 +219,9 @@ void SparseFactor::release()
     rtot_ = wtot_ = 0;
     for (DBuf<int32_t> *b : {&order_, &p0_, &np_, &nu_, &uidx_}) b->release();
     for (DBuf<int64_t> *b : {&roff_, &woff_, &goff_, &loff_, &cptr_, &cidx_}) b->release();
-    for (DBuf<double> *b : {&G_, &L_, &tp_, &ypos_, &xpos_, &rhs_, &wbuf_}) b->release();
-    for (DBuf<int2> *b : {&t_rhs_, &t_y_, &t_w_, &t_s_, &t_x_}) b->release();
-    lv_rhs_.clear(), lv_y_.clear(), lv_w_.clear(), lv_s_.clear(), lv_x_.clear();
+    for (DBuf<double> *b : {&G_, &L_, &Lt_, &tp_, &ypos_, &xpos_, &rhs_, &wbuf_}) b->release();
+    for (DBuf<int2> *b : {&t_rhs_, &t_y_, &t_w_, &t_x_}) b->release();
+    lv_rhs_.clear(), lv_y_.clear(), lv_w_.clear(), lv_x_.clear();
 }
 
 k::FrontView SparseFactor::view() const
@@ -410,6 +410,7 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
     const auto t1 = clk::now();
     G_.resize(std::max<int64_t>(goff[F], 1));
     L_.resize(std::max<int64_t>(loff[F], 1));
+    Lt_.resize(std::max<int64_t>(loff[F], 1));
     {
         DBuf<double> upd(stack_peak), Fw(max_m2), Gt(max_pm), work(max_work);
         DBuf<int32_t> d_li, d_lj, d_rel;
@@ -437,6 +438,7 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
                 LSS_HIP(hipMemcpy2DAsync(upd.get() + uoff[f], size_t(u) * sizeof(double), Fw.get() + p * mf + p,
                                          size_t(mf) * sizeof(double), size_t(u) * sizeof(double), size_t(u),
                                          hipMemcpyDeviceToDevice, s));
+                k::sparse_transpose(u, p, L_.get() + loff[f], Lt_.get() + loff[f], s);
             }
         }
         LSS_HIP(hipStreamSynchronize(s));  // the workspaces are released at scope end
@@ -447,25 +449,23 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
     for (int32_t f = 0; f < F; ++f) H = std::max(H, T.height[f]);
     std::vector<std::vector<int32_t>> bylev(H + 1);
     for (int32_t f = 0; f < F; ++f) bylev[T.height[f]].push_back(f);
-    std::vector<int2> trhs, ty, tw, ts, tx;
-    for (auto *lv : {&lv_rhs_, &lv_y_, &lv_w_, &lv_s_, &lv_x_}) lv->assign(H + 2, 0);
+    std::vector<int2> trhs, ty, tw, tx;
+    for (auto *lv : {&lv_rhs_, &lv_y_, &lv_w_, &lv_x_}) lv->assign(H + 2, 0);
     for (int32_t h = 0; h <= H; ++h) {
         lv_rhs_[h] = int64_t(trhs.size()), lv_y_[h] = int64_t(ty.size()), lv_w_[h] = int64_t(tw.size());
-        lv_s_[h] = int64_t(ts.size()), lv_x_[h] = int64_t(tx.size());
+        lv_x_[h] = int64_t(tx.size());
         for (int32_t f : bylev[h]) {
             const int32_t p = T.np[f], u = int32_t(T.nu(f));
             for (int32_t a = 0; a < p + u; ++a) trhs.push_back(make_int2(f, a));
             for (int32_t i = 0; i < p; ++i) ty.push_back(make_int2(f, i)), tx.push_back(make_int2(f, i));
             for (int32_t a = 0; a < u; ++a) tw.push_back(make_int2(f, a));
-            for (int32_t ch = 0; ch < (p + 63) / 64; ++ch) ts.push_back(make_int2(f, ch));
         }
     }
     lv_rhs_[H + 1] = int64_t(trhs.size()), lv_y_[H + 1] = int64_t(ty.size()), lv_w_[H + 1] = int64_t(tw.size());
-    lv_s_[H + 1] = int64_t(ts.size()), lv_x_[H + 1] = int64_t(tx.size());
+    lv_x_[H + 1] = int64_t(tx.size());
     t_rhs_.upload(trhs.data(), trhs.size(), s);
     t_y_.upload(ty.data(), ty.size(), s);
     t_w_.upload(tw.data(), tw.size(), s);
-    t_s_.upload(ts.data(), ts.size(), s);
     t_x_.upload(tx.data(), tx.size(), s);
     order_.upload(T.order.data(), m, s);
     p0_.upload(T.p0.data(), F, s);
@@ -509,7 +509,7 @@ void SparseFactor::solve_add(const double *t, const int32_t *pdofs, double *z, f
                         ypos_.get(), m_, wbuf_.get(), wtot_, k, stop, s);
     }
     for (int h = NL - 1; h >= 0; --h) {
-        k::sparse_bwd_s(lv_s_[h + 1] - lv_s_[h], t_s_.get() + lv_s_[h], fv, L_.get(), xpos_.get(), ypos_.get(), m_,
+        k::sparse_bwd_s(lv_y_[h + 1] - lv_y_[h], t_y_.get() + lv_y_[h], fv, Lt_.get(), xpos_.get(), ypos_.get(), m_,
                         rhs_.get(), rtot_, k, stop, s);
         k::sparse_bwd_x(lv_x_[h + 1] - lv_x_[h], t_x_.get() + lv_x_[h], fv, G_.get(), rhs_.get(), rtot_,
                         xpos_.get(), m_, k, stop, s);

@@ -56,7 +56,7 @@ public:
     int32_t fronts() const { return nf_; }
     int32_t max_front() const { return max_front_; }
     int32_t levels() const { return lv_rhs_.empty() ? 0 : int32_t(lv_rhs_.size()) - 1; }  // tree heights 0 .. levels() - 1
-    int64_t factor_entries() const { return int64_t(G_.size() + L_.size()); }
+    int64_t factor_entries() const { return int64_t(G_.size() + L_.size()); }  // G11 + L21 (L21^T kept besides)
     int64_t stack_bytes() const { return stack_bytes_; }  // peak update-matrix storage of the factorization
     double t_symbolic = 0, t_numeric = 0;
 
@@ -65,10 +65,10 @@ private:
     int64_t rtot_ = 0, wtot_ = 0, stack_bytes_ = 0;
     DBuf<int32_t> order_, p0_, np_, nu_, uidx_;
     DBuf<int64_t> roff_, woff_, goff_, loff_, cptr_, cidx_;
-    DBuf<double> G_, L_;
+    DBuf<double> G_, L_, Lt_;  // G11 per front; L21 (u x p) and its transpose (p x u)
     // level task lists: tasks of level h at [lv_x[h], lv_x[h + 1])
-    DBuf<int2> t_rhs_, t_y_, t_w_, t_s_, t_x_;
-    std::vector<int64_t> lv_rhs_, lv_y_, lv_w_, lv_s_, lv_x_;
+    DBuf<int2> t_rhs_, t_y_, t_w_, t_x_;
+    std::vector<int64_t> lv_rhs_, lv_y_, lv_w_, lv_x_;
     DBuf<double> tp_, ypos_, xpos_, rhs_, wbuf_;
     k::FrontView view() const;
 };

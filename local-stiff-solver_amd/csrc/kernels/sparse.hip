@@ -129,9 +129,9 @@ __global__ __launch_bounds__(kT) void k_fwd_w(int64_t ntask, const int2 *__restr
         w[int64_t(c) * wtot + fv.woff[f] + a] = rhs[int64_t(c) * rtot + fv.roff[f] + p + a] - acc;
 }
 
-// wave per 64 pivot columns: s_j = y_j - sum_a L21(a, j) x[U_f(a)], a in row order
+// wave per pivot column j: s_j = y_j - sum_a L21(a, j) x[U_f(a)], from L21^T (p x u, row-major)
 __global__ __launch_bounds__(kT) void k_bwd_s(int64_t ntask, const int2 *__restrict__ tasks, FrontView fv,
-                                              const double *__restrict__ L, const double *__restrict__ xpos,
+                                              const double *__restrict__ Lt, const double *__restrict__ xpos,
                                               const double *__restrict__ ypos, int32_t m,
                                               double *__restrict__ sbuf, int64_t rtot, const int *__restrict__ stop)
 {
@@ -140,15 +140,28 @@ __global__ __launch_bounds__(kT) void k_bwd_s(int64_t ntask, const int2 *__restr
     if (t >= ntask) return;
     const int lane = threadIdx.x & 63, c = blockIdx.y;
     const int2 task = tasks[t];
-    const int f = task.x, p = fv.np[f], u = fv.nu[f];
-    const int j = task.y * 64 + lane;
-    if (j >= p) return;
-    const double *col = L + fv.loff[f] + j;
+    const int f = task.x, j = task.y, u = fv.nu[f];
+    const double *row = Lt + fv.loff[f] + int64_t(j) * u;
     const int32_t *ui = fv.uidx + fv.woff[f];
     const double *x = xpos + int64_t(c) * m;
     double acc = 0.0;
-    for (int a = 0; a < u; ++a) acc += col[int64_t(a) * p] * x[ui[a]];
-    sbuf[int64_t(c) * rtot + fv.roff[f] + j] = ypos[int64_t(c) * m + fv.p0[f] + j] - acc;
+    for (int a = lane; a < u; a += 64) acc += row[a] * x[ui[a]];
+    acc = dev::wave_sum(acc);
+    if (lane == 0) sbuf[int64_t(c) * rtot + fv.roff[f] + j] = ypos[int64_t(c) * m + fv.p0[f] + j] - acc;
+}
+
+// dst (c x r) = src (r x c)^T, row-major, 32 x 32 tiles through LDS
+__global__ __launch_bounds__(kT) void k_transpose(int64_t r, int64_t c, const double *__restrict__ src,
+                                                  double *__restrict__ dst)
+{
+    __shared__ double tile[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int64_t r0 = int64_t(blockIdx.y) * 32, c0 = int64_t(blockIdx.x) * 32;
+    for (int k = ty; k < 32; k += kT / 32)
+        if (r0 + k < r && c0 + tx < c) tile[k][tx] = src[(r0 + k) * c + c0 + tx];
+    __syncthreads();
+    for (int k = ty; k < 32; k += kT / 32)
+        if (c0 + k < c && r0 + tx < r) dst[(c0 + k) * r + r0 + tx] = tile[tx][k];
 }
 
 // wave per pivot row i: x_i = sum_{j >= i} G11(j, i) s_j (G11^T stored in the strict upper triangle)
@@ -246,11 +259,18 @@ void sparse_fwd_w(int64_t ntask, const int2 *tasks, const FrontView &fv, const d
     LSS_LAUNCH_CHECK();
 }
 
-void sparse_bwd_s(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *L, const double *xpos,
+void sparse_bwd_s(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *Lt, const double *xpos,
                   const double *ypos, int32_t m, double *sbuf, int64_t rtot, int k, const int *stop, hipStream_t s)
 {
     if (ntask == 0) return;
-    k_bwd_s<<<dim3(waves_grid(ntask), k), kT, 0, s>>>(ntask, tasks, fv, L, xpos, ypos, m, sbuf, rtot, stop);
+    k_bwd_s<<<dim3(waves_grid(ntask), k), kT, 0, s>>>(ntask, tasks, fv, Lt, xpos, ypos, m, sbuf, rtot, stop);
+    LSS_LAUNCH_CHECK();
+}
+
+void sparse_transpose(int64_t r, int64_t c, const double *src, double *dst, hipStream_t s)
+{
+    if (r == 0 || c == 0) return;
+    k_transpose<<<dim3(div_up(c, 32), div_up(r, 32)), kT, 0, s>>>(r, c, src, dst);
     LSS_LAUNCH_CHECK();
 }
 
