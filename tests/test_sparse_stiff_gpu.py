@@ -143,3 +143,65 @@ def test_mas_solver_sparse_units(gpu, kw):
     xd, xs = dense.solve(b), sparse.solve(b)
     assert abs(dense.get_info()["num_iterations"] - sparse.get_info()["num_iterations"]) <= 10
     assert np.linalg.norm(xs - xd) <= 1e-4 * np.linalg.norm(xd)
+
+
+def test_all_stiff_direct_at_size(gpu):
+    # 48^3 log-uniform-E elasticity (331 776 DOF), every DOF stiff: one unit, factored by
+    # the multifrontal path, so the hybrid PCG must stop at the direct solution
+    import torch
+
+    from lss_amd.solver import residual_norm
+
+    prob = problems.make("elastic_logE", N=48)
+    A = prob.csc()
+    n = A.n
+    dev = torch.device("cuda:0")
+    dA = [torch.from_numpy(a).to(dev) for a in (A.colptr, A.rowidx, A.vals)]
+    b = torch.from_numpy(prob.rhs(0)).to(dev)
+    x = torch.zeros(n, dtype=torch.float64, device=dev)
+    s = Solver.create(_cfg(3))
+    s.set_problematic_dofs(np.arange(n, dtype=np.int32))
+    s.factorize_device(n, *dA, stream=torch.cuda.current_stream())
+    sp_info = s.get_info()["stiff"]["sparse"]
+    assert sp_info["units"] == 1 and sp_info["dofs"] == [n] and sp_info["fronts"][0] > 1000
+    s.solve_device(b, x, stream=torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    info = s.get_info()
+    r, bn = residual_norm(*dA, b, x)
+    print(f"48^3 all-stiff: fronts {sp_info['fronts']} max front {sp_info['max_front']} factor entries "
+          f"{sp_info['factor_entries']} symbolic {sp_info['t_symbolic']} numeric {sp_info['t_numeric']}; "
+          f"{info['num_iterations']} iterations, true rel residual {r / bn:.2e}")
+    assert info["stop_reason"] == "converged_rel" and info["num_iterations"] <= 2
+    assert r / bn <= 1e-8
+
+
+@pytest.mark.parametrize("name,N,strategy", [("elastic_logE", 48, "elbow"), ("elastic_slivers", 64, "gmm")])
+def test_mas_solver_large_unit_at_size(gpu, name, N, strategy):
+    # the MAS solver's window-expanded sets at the workloads' stated sizes form one unit of
+    # 0.3-0.5 M DOF (above the dense limit): the additive correction goes through the
+    # multifrontal factor and the loop reaches the MAS solver's tolerance (1e-6)
+    import torch
+
+    from lss_amd.solver import residual_norm
+
+    prob = problems.make(name, N=N)
+    A = prob.csc()
+    n = A.n
+    dev = torch.device("cuda:0")
+    dA = [torch.from_numpy(a).to(dev) for a in (A.colptr, A.rowidx, A.vals)]
+    b = torch.from_numpy(prob.rhs(0)).to(dev)
+    x = torch.zeros(n, dtype=torch.float64, device=dev)
+    s = Solver.create({"solver": "MAS", "MAS": {"block_dim": 3, "max_iter": 3000, "use_bad_dof_precond": True,
+                                                "bad_dof_strategy": strategy}})
+    s.factorize_device(n, *dA, stream=torch.cuda.current_stream())
+    info = s.get_info()
+    sp_info = info["stiff"]["sparse"]
+    assert sp_info["units"] == 1 and sp_info["dofs"][0] > 16384
+    s.solve_device(b, x, stream=torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    info = s.get_info()
+    r, bn = residual_norm(*dA, b, x)
+    print(f"{name} {N}^3 MAS+{strategy}: unit {sp_info['dofs']} fronts {sp_info['fronts']} max front "
+          f"{sp_info['max_front']}; {info['solver_iter']} iterations, true rel residual {r / bn:.2e}")
+    assert info["solver_status"] == "Reach relative tolerance"
+    assert r / bn <= 2e-6
