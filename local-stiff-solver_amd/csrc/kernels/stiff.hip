@@ -279,65 +279,189 @@ __global__ __launch_bounds__(256) void k_explicit_inverse(const int32_t *__restr
 // Diagonal block of the blocked factorization of one large component: the
 // kb x kb block A(k0:k0+kb, k0:k0+kb) (row-major, leading dimension m, lower
 // triangle read) is factored A = L L^T in LDS; L goes back in place (strict
-// upper triangle of the block zeroed) and L^{-1} into the same block of G.
-// Rows of the LDS matrix are padded to DP = kb + 1 doubles.
+// upper triangle of the block zeroed) and L^{-1} into the lower triangle of the
+// same block of G.  The block is itself factored in sub-blocks of 32: wave 0
+// factors and inverts the 32 x 32 diagonal sub-block in registers (lane i holds
+// row i of L and column i of L^{-1}; the other lanes' entries arrive by readlane,
+// so no barrier inside the sub-block), then all waves form the panel below it
+// (L_IJ = A_IJ G_JJ^T) and the trailing update (A_II -= L_IJ L_IJ^T, 4 x 4 tiles);
+// the off-diagonal blocks of the inverse follow by distance from the diagonal,
+// G_IJ = -G_II sum_{K=J}^{I-1} L_IK G_KJ.  About 5 barriers per sub-block instead
+// of two per column.  LDS: the block with rows padded to DP = kb + 1 doubles
+// (lower triangle L, strict upper triangle G^T), G's diagonal, and three 32 x 32
+// work blocks.
 constexpr int kDenseNB = 128;
+constexpr int kDenseSB = 32;
+
+__device__ __forceinline__ double readlane_d(double v, int l)
+{
+    union {
+        double d;
+        int i[2];
+    } u;
+    u.d = v;
+    u.i[0] = __builtin_amdgcn_readlane(u.i[0], l);
+    u.i[1] = __builtin_amdgcn_readlane(u.i[1], l);
+    return u.d;
+}
+
+__host__ __device__ constexpr size_t dense_diag_lds(int kb)
+{
+    return (size_t(kb) * (kb + 1) + kb + 3 * kDenseSB * kDenseSB) * sizeof(double);
+}
+
 __global__ __launch_bounds__(256) void k_dense_diag(int64_t m, int k0, int kb, double *__restrict__ A,
                                                     double *__restrict__ G, int *__restrict__ status)
 {
+    constexpr int SB = kDenseSB;
     extern __shared__ double lds[];
     const int DP = kb + 1;
-    double *M = lds;
-    double *tmp = lds + int64_t(kb) * DP;
+    double *M = lds;                    // kb x DP
+    double *dg = M + int64_t(kb) * DP;  // diagonal of G
+    double *T = dg + kb;                // 3 work blocks of SB x SB
     __shared__ int bad;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    constexpr int NW = 4;
-    if (threadIdx.x == 0) bad = 0;
-    for (int t = threadIdx.x; t < kb * kb; t += 256) {
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid == 0) bad = 0;
+    for (int t = tid; t < kb * kb; t += 256) {
         const int i = t / kb, j = t - i * kb;
-        M[i * DP + j] = j <= i ? A[(k0 + i) * m + k0 + j] : 0.0;
+        if (j <= i) M[i * DP + j] = A[(k0 + i) * m + k0 + j];
     }
     __syncthreads();
-    for (int k = 0; k < kb; ++k) {
-        const double d = M[k * DP + k];
-        if (threadIdx.x == 0 && (!(d > 0.0) || !isfinite(d))) bad = 1;
-        const double rdk = 1.0 / sqrt(d);
-        for (int i = k + 1 + threadIdx.x; i < kb; i += 256) {
-            const double v = M[i * DP + k] * rdk;
-            M[i * DP + k] = v;
-            tmp[i] = v;
+    const int nsub = (kb + SB - 1) / SB;
+    for (int J = 0; J < nsub; ++J) {
+        const int j0 = J * SB, jb = min(SB, kb - j0);
+        if (wid == 0) {
+            double row[SB], g[SB];
+#pragma unroll
+            for (int c = 0; c < SB; ++c) row[c] = (lane < jb && c <= lane) ? M[(j0 + lane) * DP + j0 + c] : 0.0;
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < SB; ++k) {
+                if (k < jb) {
+                    const double dkk = readlane_d(row[k], k);
+                    ok = ok && dkk > 0.0 && isfinite(dkk);
+                    const double sq = sqrt(dkk);
+                    if (lane == k) row[k] = sq;
+                    else if (lane > k) row[k] = row[k] / sq;
+                    const double lk = row[k];
+#pragma unroll
+                    for (int j = k + 1; j < SB; ++j) {
+                        const double ljk = readlane_d(row[k], j);
+                        if (lane >= j) row[j] -= lk * ljk;
+                    }
+                }
+            }
+            // lane c: column c of G_JJ = L_JJ^{-1} by forward substitution
+#pragma unroll
+            for (int i = 0; i < SB; ++i) {
+                if (i < jb) {
+                    double acc = lane == i ? 1.0 : 0.0;
+#pragma unroll
+                    for (int k = 0; k < i; ++k) acc -= readlane_d(row[k], i) * g[k];
+                    g[i] = acc / readlane_d(row[i], i);
+                } else {
+                    g[i] = 0.0;
+                }
+            }
+            if (lane < jb) {
+#pragma unroll
+                for (int c = 0; c < SB; ++c) {
+                    if (c <= lane) M[(j0 + lane) * DP + j0 + c] = row[c];
+                    else if (c < jb) M[(j0 + lane) * DP + j0 + c] = g[c];  // G(c, lane) at (lane, c)
+                    if (c == lane) dg[j0 + c] = g[c];
+                }
+            }
+            if (lane == 0 && !ok) bad = 1;
         }
         __syncthreads();
-        if (threadIdx.x == 0) M[k * DP + k] = sqrt(d);
-        for (int i = k + 1 + wid; i < kb; i += NW) {
-            const double li = tmp[i];
-            for (int j = k + 1 + lane; j <= i; j += 64) M[i * DP + j] -= li * tmp[j];
+        const int r0 = j0 + jb;
+        if (r0 >= kb) continue;
+        // panel, one thread per row: L(r, j0 + c) = sum_{k <= c} A(r, j0 + k) G_JJ(c, k)
+        for (int r = r0 + tid; r < kb; r += 256) {
+            double a[SB];
+#pragma unroll
+            for (int k = 0; k < SB; ++k) a[k] = k < jb ? M[r * DP + j0 + k] : 0.0;
+#pragma unroll
+            for (int c = 0; c < SB; ++c) {
+                if (c < jb) {
+                    double acc = a[c] * dg[j0 + c];
+#pragma unroll
+                    for (int k = 0; k < c; ++k) acc += a[k] * M[(j0 + k) * DP + j0 + c];
+                    M[r * DP + j0 + c] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        // trailing update in 4 x 4 tiles of the lower triangle of rows/columns r0 .. kb - 1
+        const int nt = (kb - r0 + 3) / 4, ntiles = nt * (nt + 1) / 2;
+        for (int t = tid; t < ntiles; t += 256) {
+            int ti = int((sqrtf(8.0f * float(t) + 1.0f) - 1.0f) * 0.5f);
+            while (ti * (ti + 1) / 2 > t) --ti;
+            while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+            const int tj = t - ti * (ti + 1) / 2;
+            const int rb = r0 + 4 * ti, cb = r0 + 4 * tj;
+            double acc[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+            for (int k = 0; k < jb; ++k) {
+                double lr[4], lc[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    lr[u] = rb + u < kb ? M[(rb + u) * DP + j0 + k] : 0.0;
+                    lc[u] = cb + u < kb ? M[(cb + u) * DP + j0 + k] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[u][v] += lr[u] * lc[v];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = rb + u, c = cb + v;
+                    if (r < kb && c <= r) M[r * DP + c] -= acc[u][v];
+                }
         }
         __syncthreads();
     }
-    for (int t = threadIdx.x; t < kb * kb; t += 256) {
+    // off-diagonal blocks of G, by distance d = I - J from the diagonal
+    for (int d = 1; d < nsub; ++d) {
+        const int npair = nsub - d;
+        for (int e = tid; e < npair * SB * SB; e += 256) {
+            const int p = e / (SB * SB), a = (e / SB) % SB, b = e % SB;
+            const int ia = (p + d) * SB + a, jc = p * SB + b;
+            double acc = 0.0;
+            if (ia < kb && jc < kb) {
+                const int kend = min((p + d) * SB, kb);
+                for (int kk = jc; kk < kend; ++kk) {  // G(kk, jc) = 0 for kk < jc
+                    const double gk = kk == jc ? dg[kk] : M[jc * DP + kk];
+                    acc += M[ia * DP + kk] * gk;
+                }
+            }
+            T[p * SB * SB + a * SB + b] = acc;
+        }
+        __syncthreads();
+        for (int e = tid; e < npair * SB * SB; e += 256) {
+            const int p = e / (SB * SB), a = (e / SB) % SB, b = e % SB;
+            const int i0 = (p + d) * SB, ia = i0 + a, jc = p * SB + b;
+            if (ia < kb && jc < kb) {
+                double acc = dg[ia] * T[p * SB * SB + a * SB + b];
+                for (int q = 0; q < a; ++q) acc += M[(i0 + q) * DP + ia] * T[p * SB * SB + q * SB + b];
+                M[jc * DP + ia] = -acc;  // G(ia, jc) at (jc, ia)
+            }
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < kb * kb; t += 256) {
         const int i = t / kb, j = t - i * kb;
         A[(k0 + i) * m + k0 + j] = j <= i ? M[i * DP + j] : 0.0;
+        if (j < i) G[(k0 + i) * m + k0 + j] = M[j * DP + i];
+        else if (j == i) G[(k0 + i) * m + k0 + j] = dg[i];
     }
-    __syncthreads();
-    // L^{-1} in place, columns from the last to the first
-    for (int j = kb - 1; j >= 0; --j) {
-        const double gjj = 1.0 / M[j * DP + j];
-        for (int i = j + 1 + threadIdx.x; i < kb; i += 256) {
-            double acc = 0.0;
-            for (int q = j + 1; q <= i; ++q) acc += M[i * DP + q] * M[q * DP + j];
-            tmp[i] = -acc * gjj;
-        }
-        __syncthreads();
-        for (int i = j + 1 + threadIdx.x; i < kb; i += 256) M[i * DP + j] = tmp[i];
-        if (threadIdx.x == 0) M[j * DP + j] = gjj;
-        __syncthreads();
-    }
-    for (int t = threadIdx.x; t < kb * kb; t += 256) {
-        const int i = t / kb, j = t - i * kb;
-        if (j <= i) G[(k0 + i) * m + k0 + j] = M[i * DP + j];
-    }
-    if (threadIdx.x == 0 && bad) atomicOr(status, 2);
+    if (tid == 0 && bad) atomicOr(status, 2);
 }
 
 // Upper triangle of G := transpose of its lower triangle (32 x 32 tiles via LDS).
@@ -400,12 +524,11 @@ int dense_block_size() { return kDenseNB; }
 void dense_diag_block(int64_t m, int k0, int kb, double *A, double *G, int *status, hipStream_t s)
 {
     if (kb <= 0 || kb > kDenseNB) fail(LSS_ERR_ARG, "dense_diag_block: block size out of range");
-    const size_t lds = (size_t(kb) * (kb + 1) + kb) * sizeof(double);
+    const size_t lds = dense_diag_lds(kb);
     static bool attr = false;
     if (!attr) {
         LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_dense_diag),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    int((size_t(kDenseNB) * (kDenseNB + 1) + kDenseNB) * sizeof(double))));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(dense_diag_lds(kDenseNB))));
         attr = true;
     }
     k_dense_diag<<<1, 256, lds, s>>>(m, k0, kb, A, G, status);
