@@ -340,9 +340,9 @@ __global__ __launch_bounds__(256) void k_dense_diag(int64_t m, int k0, int kb, d
                 if (k < jb) {
                     const double dkk = readlane_d(row[k], k);
                     ok = ok && dkk > 0.0 && isfinite(dkk);
-                    const double sq = sqrt(dkk);
+                    const double sq = sqrt(dkk), rsq = 1.0 / sq;
                     if (lane == k) row[k] = sq;
-                    else if (lane > k) row[k] = row[k] / sq;
+                    else if (lane > k) row[k] *= rsq;
                     const double lk = row[k];
 #pragma unroll
                     for (int j = k + 1; j < SB; ++j) {
@@ -352,13 +352,17 @@ __global__ __launch_bounds__(256) void k_dense_diag(int64_t m, int k0, int kb, d
                 }
             }
             // lane c: column c of G_JJ = L_JJ^{-1} by forward substitution
+            double rdiag = 0.0;  // lane i: 1 / L_ii
+#pragma unroll
+            for (int c = 0; c < SB; ++c)
+                if (c == lane) rdiag = 1.0 / row[c];
 #pragma unroll
             for (int i = 0; i < SB; ++i) {
                 if (i < jb) {
                     double acc = lane == i ? 1.0 : 0.0;
 #pragma unroll
                     for (int k = 0; k < i; ++k) acc -= readlane_d(row[k], i) * g[k];
-                    g[i] = acc / readlane_d(row[i], i);
+                    g[i] = acc * readlane_d(rdiag, i);
                 } else {
                     g[i] = 0.0;
                 }
@@ -427,31 +431,69 @@ __global__ __launch_bounds__(256) void k_dense_diag(int64_t m, int k0, int kb, d
         }
         __syncthreads();
     }
-    // off-diagonal blocks of G, by distance d = I - J from the diagonal
+    // off-diagonal blocks of G, by distance d = I - J from the diagonal, in 4 x 4 tiles:
+    // T = sum_{K=J}^{I-1} L_IK G_KJ, then G_IJ = -G_II T
     for (int d = 1; d < nsub; ++d) {
-        const int npair = nsub - d;
-        for (int e = tid; e < npair * SB * SB; e += 256) {
-            const int p = e / (SB * SB), a = (e / SB) % SB, b = e % SB;
-            const int ia = (p + d) * SB + a, jc = p * SB + b;
-            double acc = 0.0;
-            if (ia < kb && jc < kb) {
-                const int kend = min((p + d) * SB, kb);
-                for (int kk = jc; kk < kend; ++kk) {  // G(kk, jc) = 0 for kk < jc
-                    const double gk = kk == jc ? dg[kk] : M[jc * DP + kk];
-                    acc += M[ia * DP + kk] * gk;
+        const int ntile = (nsub - d) * 64;  // 8 x 8 tiles per block pair
+        for (int t = tid; t < ntile; t += 256) {
+            const int pp = t >> 6, ti = (t >> 3) & 7, tj = t & 7;
+            const int ia0 = (pp + d) * SB + 4 * ti, jc0 = pp * SB + 4 * tj;
+            const int kend = min((pp + d) * SB, kb);
+            double acc[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+#pragma unroll 4
+            for (int kk = pp * SB; kk < kend; ++kk) {
+                double l[4], gk[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) l[u] = ia0 + u < kb ? M[(ia0 + u) * DP + kk] : 0.0;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int jc = jc0 + v;  // G(kk, jc): lower triangular, G^T stored above the diagonal
+                    gk[v] = (jc < kb && kk >= jc) ? (kk == jc ? dg[kk] : M[jc * DP + kk]) : 0.0;
                 }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[u][v] += l[u] * gk[v];
             }
-            T[p * SB * SB + a * SB + b] = acc;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) T[pp * SB * SB + (4 * ti + u) * SB + 4 * tj + v] = acc[u][v];
         }
         __syncthreads();
-        for (int e = tid; e < npair * SB * SB; e += 256) {
-            const int p = e / (SB * SB), a = (e / SB) % SB, b = e % SB;
-            const int i0 = (p + d) * SB, ia = i0 + a, jc = p * SB + b;
-            if (ia < kb && jc < kb) {
-                double acc = dg[ia] * T[p * SB * SB + a * SB + b];
-                for (int q = 0; q < a; ++q) acc += M[(i0 + q) * DP + ia] * T[p * SB * SB + q * SB + b];
-                M[jc * DP + ia] = -acc;  // G(ia, jc) at (jc, ia)
+        for (int t = tid; t < ntile; t += 256) {
+            const int pp = t >> 6, ti = (t >> 3) & 7, tj = t & 7;
+            const int i0 = (pp + d) * SB, jc0 = pp * SB + 4 * tj;
+            double acc[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+            for (int q = 0; q < 4 * ti + 4; ++q) {
+                double gq[4], tq[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int a = 4 * ti + u, ia = i0 + a;  // G(ia, i0 + q), q <= a
+                    gq[u] = (ia < kb && q <= a) ? (q == a ? dg[ia] : M[(i0 + q) * DP + ia]) : 0.0;
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) tq[v] = T[pp * SB * SB + q * SB + 4 * tj + v];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[u][v] += gq[u] * tq[v];
             }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int ia = i0 + 4 * ti + u, jc = jc0 + v;
+                    if (ia < kb && jc < kb) M[jc * DP + ia] = -acc[u][v];  // G(ia, jc) at (jc, ia)
+                }
         }
         __syncthreads();
     }
