@@ -1664,8 +1664,12 @@ std::string HybridSolver::info_json() const
     for (size_t u = 0; u < sparse_.size(); ++u) o << (u ? "," : "") << sparse_[u].max_front();
     o << "],\"factor_entries\":[";
     for (size_t u = 0; u < sparse_.size(); ++u) o << (u ? "," : "") << sparse_[u].factor_entries();
-    o << "],\"update_stack_bytes\":[";
+    o << "],\"update_bytes\":[";
     for (size_t u = 0; u < sparse_.size(); ++u) o << (u ? "," : "") << sparse_[u].stack_bytes();
+    o << "],\"batched_fronts\":[";
+    for (size_t u = 0; u < sparse_.size(); ++u) o << (u ? "," : "") << sparse_[u].batched_fronts();
+    o << "],\"front_groups\":[";
+    for (size_t u = 0; u < sparse_.size(); ++u) o << (u ? "," : "") << sparse_[u].front_groups();
     o << "],\"t_symbolic\":[";
     for (size_t u = 0; u < sparse_.size(); ++u) o << (u ? "," : "") << sparse_[u].t_symbolic;
     o << "],\"t_numeric\":[";

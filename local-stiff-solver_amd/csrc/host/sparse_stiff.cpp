@@ -11,16 +11,25 @@
 // are eliminated together); a front's update positions are its pivots' neighbours
 // and its children's update positions beyond its own pivots.
 //
-// Numeric factorization, children before parents, one front at a time on the stream:
-// zero the m x m front, write the original entries of its pivot columns, add the
-// children's update matrices (extend-add, children in order), and factor the first p
-// columns with the blocked dense path (DenseBlas::partial_factor_inverse), which also
-// leaves the Schur complement in the trailing block.  Kept per front: G11 = L11^{-1}
-// (with G11^T in its strict upper triangle) and L21.
+// The dissection runs on the quotient graph of indistinguishable vertices (equal closed
+// neighbourhoods: the DOFs of one mesh node), each group weighted by its size.
+//
+// Numeric factorization, level by level of the assembly tree (the fronts of a level are
+// independent): zero each m x m front, write the original entries of its pivot columns,
+// add the children's update matrices (extend-add, children in order), and factor the
+// first p columns, which leaves the Schur complement in the trailing block.  Fronts of up
+// to kBatchFront rows go through batched kernels, one launch per step for all fronts of
+// the level: a 128-column step is the diagonal block (factor + inverse, one workgroup per
+// front), the panel below it and the trailing update in 64 x 64 tiles; the off-diagonal
+// blocks of L11^{-1} follow by block distance.  Larger fronts take the blocked dense path
+// (DenseBlas::partial_factor_inverse, rocBLAS products) one at a time.  Kept per front:
+// G11 = L11^{-1} (with G11^T in its strict upper triangle), L21 and L21^T.
 #include "sparse_stiff.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
+#include <map>
 #include <numeric>
 #include <string>
 
@@ -35,16 +44,91 @@ double since(clk::time_point t0) { return std::chrono::duration<double>(clk::now
 
 // Largest front (rows) the numeric phase accepts: its dense workspace takes m^2 doubles.
 constexpr int32_t kMaxFront = 49152;
+// Fronts of at most this many rows are factored in level batches; the workspaces of one
+// batch (m^2 doubles per front) take at most kGroupWork doubles.
+constexpr int64_t kBatchFront = 4096;
+constexpr int64_t kGroupWork = int64_t(1) << 29;
 
+int64_t align_up(int64_t n) { return (n + 31) / 32 * 32; }  // 256-byte granules
+
+// First-fit placement of blocks in one array (offsets in doubles); peak() is the extent used.
+class FirstFit {
+public:
+    int64_t take(int64_t n)
+    {
+        if (n == 0) return 0;
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= n) {
+                const int64_t at = it->first, rest = it->second - n;
+                free_.erase(it);
+                if (rest) free_[at + n] = rest;
+                return at;
+            }
+        int64_t at = top_;
+        if (!free_.empty()) {  // a free block at the top grows into the new one
+            auto last = std::prev(free_.end());
+            if (last->first + last->second == top_) {
+                at = last->first;
+                free_.erase(last);
+            }
+        }
+        top_ = at + n;
+        peak_ = std::max(peak_, top_);
+        return at;
+    }
+    void release(int64_t at, int64_t n)
+    {
+        if (n == 0) return;
+        auto next = free_.lower_bound(at);
+        if (next != free_.end() && at + n == next->first) {
+            n += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == at) {
+                prev->second += n;
+                return;
+            }
+        }
+        free_[at] = n;
+    }
+    int64_t peak() const { return peak_; }
+
+private:
+    std::map<int64_t, int64_t> free_;
+    int64_t top_ = 0, peak_ = 0;
+};
+
+struct Range {
+    int64_t off = 0, n = 0;
+};
+// Launch ranges of one batch of fronts (into the task lists of the numeric phase).
+struct GroupPlan {
+    int64_t wsize = 0;                         // workspace doubles
+    Range asmb;                                // original entries
+    std::vector<Range> ext;                    // extend-add, one per child slot
+    std::vector<std::array<Range, 3>> steps;   // per 128-column step: diagonal block, panel, trailing tiles
+    std::vector<std::array<Range, 2>> rounds;  // per block distance of G11: T tiles, G tiles
+    Range cl, clt, cu, cm;                     // L21, L21^T, update matrix, G11 mirror
+};
+
+// Dissects a graph whose vertices carry weights (the sizes of groups of original
+// vertices); emitted fronts list the original vertices of their groups.
 struct Dissector {
-    const int32_t *ap, *ai;
-    int32_t leaf;
+    const int32_t *ap, *ai;   // quotient graph
+    const int32_t *mptr, *mem;  // members of each quotient vertex (original vertices, ascending)
+    int64_t leaf;
     FrontTree &T;
     std::vector<int32_t> mark, lev, vis;
     int32_t stamp = 0, vstamp = 0, next = 0;
 
-    int32_t emit(std::vector<int32_t> &verts, const std::vector<int32_t> &kids)
+    int64_t weight(int32_t v) const { return mptr[v + 1] - mptr[v]; }
+
+    int32_t emit(const std::vector<int32_t> &qverts, const std::vector<int32_t> &kids)
     {
+        std::vector<int32_t> verts;
+        for (int32_t v : qverts) verts.insert(verts.end(), mem + mptr[v], mem + mptr[v + 1]);
         std::sort(verts.begin(), verts.end());
         const int32_t f = T.fronts();
         T.p0.push_back(next);
@@ -84,7 +168,9 @@ struct Dissector {
     // returns the root fronts of the subgraph induced by V
     std::vector<int32_t> dissect(std::vector<int32_t> V)
     {
-        if (int64_t(V.size()) <= leaf) return {emit(V, {})};
+        int64_t nv = 0;
+        for (int32_t v : V) nv += weight(v);
+        if (nv <= leaf) return {emit(V, {})};
         const int32_t my = ++stamp;
         for (int32_t v : V) mark[v] = my;
         std::sort(V.begin(), V.end());
@@ -122,14 +208,14 @@ struct Dissector {
         const int32_t nlev = lev[q.back()] + 1;
         if (nlev < 3) return {emit(V, {})};
         std::vector<int64_t> cnt(nlev, 0);
-        for (int32_t v : q) ++cnt[lev[v]];
-        // the smallest level whose predecessors hold 35-65 % of the vertices (the one
+        for (int32_t v : q) cnt[lev[v]] += weight(v);
+        // the smallest level whose predecessors hold 35-65 % of the weight (the one
         // nearest to half among equals); else the level that reaches half
-        const int64_t nv = int64_t(V.size()), half = nv / 2;
+        const int64_t half = nv / 2;
         int64_t cum = 0, kcum = 0;
         int32_t k = -1, kh = -1;
         for (int32_t l = 1; l < nlev - 1; ++l) {
-            cum += cnt[l - 1];  // vertices before level l
+            cum += cnt[l - 1];  // weight before level l
             if (kh < 0 && cum + cnt[l] >= half) kh = l;
             if (20 * cum < 7 * nv || 20 * cum > 13 * nv) continue;
             if (k < 0 || cnt[l] < cnt[k] ||
@@ -162,6 +248,82 @@ struct Dissector {
     }
 };
 
+uint64_t mix64(uint64_t x)
+{
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+// Groups of indistinguishable vertices (equal closed neighbourhoods N(v) + v), numbered
+// by their smallest member: grp[v] = group of v; members listed ascending per group.
+int32_t indistinguishable(int32_t m, const int32_t *ap, const int32_t *ai, std::vector<int32_t> &grp,
+                          std::vector<int32_t> &mptr, std::vector<int32_t> &mem)
+{
+    std::vector<uint64_t> h(m);
+    for (int32_t v = 0; v < m; ++v) {
+        uint64_t x = mix64(uint64_t(v));
+        for (int32_t e = ap[v]; e < ap[v + 1]; ++e) x += mix64(uint64_t(ai[e]));
+        h[v] = x;
+    }
+    std::vector<int32_t> by(m);
+    std::iota(by.begin(), by.end(), 0);
+    std::sort(by.begin(), by.end(), [&](int32_t a, int32_t b) {
+        const int32_t da = ap[a + 1] - ap[a], db = ap[b + 1] - ap[b];
+        return h[a] != h[b] ? h[a] < h[b] : da != db ? da < db : a < b;
+    });
+    // closed neighbourhoods of u and v equal: same degree, each in the other's list, and
+    // the lists equal once each vertex's own entry takes the other's place
+    auto same = [&](int32_t u, int32_t v) {
+        if (ap[u + 1] - ap[u] != ap[v + 1] - ap[v]) return false;
+        if (!std::binary_search(ai + ap[u], ai + ap[u + 1], v)) return false;
+        int32_t i = ap[u], j = ap[v];
+        while (i < ap[u + 1] && j < ap[v + 1]) {
+            const int32_t a = ai[i] == v ? u : ai[i], b = ai[j] == u ? v : ai[j];
+            if (ai[i] == v) { ++i; continue; }  // v in N(u) matches u in N(v): skip both
+            if (ai[j] == u) { ++j; continue; }
+            if (a != b) return false;
+            ++i, ++j;
+        }
+        return true;
+    };
+    grp.assign(m, -1);
+    std::vector<int32_t> lead;  // smallest member per provisional group
+    for (int32_t i = 0; i < m;) {
+        int32_t e = i + 1;
+        while (e < m && h[by[e]] == h[by[i]] && ap[by[e] + 1] - ap[by[e]] == ap[by[i] + 1] - ap[by[i]]) ++e;
+        for (int32_t a = i; a < e; ++a) {
+            const int32_t u = by[a];
+            if (grp[u] >= 0) continue;
+            grp[u] = int32_t(lead.size());
+            lead.push_back(u);
+            for (int32_t b = a + 1; b < e; ++b)
+                if (grp[by[b]] < 0 && same(u, by[b])) grp[by[b]] = grp[u];
+        }
+        i = e;
+    }
+    // renumber groups by smallest member (u was the smallest: by[] is ascending within ties)
+    const int32_t ng = int32_t(lead.size());
+    std::vector<int32_t> rank(ng);
+    {
+        std::vector<int32_t> ord(ng);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return lead[a] < lead[b]; });
+        for (int32_t r = 0; r < ng; ++r) rank[ord[r]] = r;
+    }
+    mptr.assign(ng + 1, 0);
+    for (int32_t v = 0; v < m; ++v) {
+        grp[v] = rank[grp[v]];
+        ++mptr[grp[v] + 1];
+    }
+    for (int32_t g = 0; g < ng; ++g) mptr[g + 1] += mptr[g];
+    mem.assign(m, 0);
+    std::vector<int32_t> cur(mptr.begin(), mptr.end() - 1);
+    for (int32_t v = 0; v < m; ++v) mem[cur[grp[v]]++] = v;  // ascending per group
+    return ng;
+}
+
 }  // namespace
 
 FrontTree nested_dissection(int32_t m, const int32_t *adj_ptr, const int32_t *adj_idx, int32_t leaf)
@@ -170,12 +332,28 @@ FrontTree nested_dissection(int32_t m, const int32_t *adj_ptr, const int32_t *ad
     T.m = m;
     T.order.assign(m, 0);
     T.pos.assign(m, 0);
-    Dissector D{adj_ptr, adj_idx, std::max(leaf, 1), T, {}, {}, {}};
-    D.mark.assign(m, 0);
-    D.lev.assign(m, 0);
-    D.vis.assign(m, 0);
-    if (m > 0) {
-        std::vector<int32_t> all(m);
+    // dissect the quotient graph of indistinguishable vertices, weighted by group size
+    std::vector<int32_t> grp, mptr, mem, qptr(1, 0), qidx;
+    const int32_t ng = indistinguishable(m, adj_ptr, adj_idx, grp, mptr, mem);
+    {
+        std::vector<int32_t> nb;
+        for (int32_t g = 0; g < ng; ++g) {
+            const int32_t r = mem[mptr[g]];
+            nb.clear();
+            for (int32_t e = adj_ptr[r]; e < adj_ptr[r + 1]; ++e)
+                if (grp[adj_idx[e]] != g) nb.push_back(grp[adj_idx[e]]);
+            std::sort(nb.begin(), nb.end());
+            nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+            qidx.insert(qidx.end(), nb.begin(), nb.end());
+            qptr.push_back(int32_t(qidx.size()));
+        }
+    }
+    Dissector D{qptr.data(), qidx.data(), mptr.data(), mem.data(), std::max(leaf, 1), T, {}, {}, {}};
+    D.mark.assign(ng, 0);
+    D.lev.assign(ng, 0);
+    D.vis.assign(ng, 0);
+    if (ng > 0) {
+        std::vector<int32_t> all(ng);
         std::iota(all.begin(), all.end(), 0);
         D.dissect(std::move(all));
     }
@@ -215,7 +393,7 @@ FrontTree nested_dissection(int32_t m, const int32_t *adj_ptr, const int32_t *ad
 
 void SparseFactor::release()
 {
-    m_ = nf_ = max_front_ = 0;
+    m_ = nf_ = max_front_ = batched_ = groups_ = 0;
     rtot_ = wtot_ = 0;
     for (DBuf<int32_t> *b : {&order_, &p0_, &np_, &nu_, &uidx_}) b->release();
     for (DBuf<int64_t> *b : {&roff_, &woff_, &goff_, &loff_, &cptr_, &cidx_}) b->release();
@@ -284,24 +462,35 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
         }
     }
     std::vector<int32_t>().swap(rows);
-    // 3. symmetric adjacency, no self loops
+    // 3. symmetric adjacency, no self loops: each column's rows merged with the
+    //    transposed pattern (both ascending)
     std::vector<int32_t> aptr(m + 1, 0), aidx;
     {
-        std::vector<int64_t> deg(m + 1, 0);
-        for (int32_t v = 0; v < m; ++v)
-            for (int64_t i = eptr[v]; i < eptr[v + 1]; ++i)
-                if (erow[i] != v) ++deg[v + 1], ++deg[erow[i] + 1];
-        for (int32_t v = 0; v < m; ++v) deg[v + 1] += deg[v];
-        std::vector<int32_t> raw(deg[m]);
-        std::vector<int64_t> cur(deg.begin(), deg.end() - 1);
-        for (int32_t v = 0; v < m; ++v)
-            for (int64_t i = eptr[v]; i < eptr[v + 1]; ++i)
-                if (erow[i] != v) raw[cur[v]++] = erow[i], raw[cur[erow[i]]++] = v;
-        aidx.reserve(deg[m]);
+        std::vector<int64_t> tptr(m + 1, 0);
+        for (int64_t i = 0; i < eptr[m]; ++i) ++tptr[erow[i] + 1];
+        for (int32_t v = 0; v < m; ++v) tptr[v + 1] += tptr[v];
+        std::vector<int32_t> tidx(std::max<int64_t>(eptr[m], 1));
+        {
+            std::vector<int64_t> cur(tptr.begin(), tptr.end() - 1);
+            for (int32_t v = 0; v < m; ++v)
+                for (int64_t i = eptr[v]; i < eptr[v + 1]; ++i) tidx[cur[erow[i]]++] = v;
+        }
+        aidx.reserve(2 * eptr[m]);
         for (int32_t v = 0; v < m; ++v) {
-            std::sort(raw.begin() + deg[v], raw.begin() + deg[v + 1]);
-            const auto e = std::unique(raw.begin() + deg[v], raw.begin() + deg[v + 1]);
-            aidx.insert(aidx.end(), raw.begin() + deg[v], e);
+            int64_t i = eptr[v], j = tptr[v];
+            const int64_t ie = eptr[v + 1], je = tptr[v + 1];
+            while (i < ie || j < je) {
+                int32_t w;
+                if (j >= je || (i < ie && erow[i] < tidx[j])) {
+                    w = erow[i++];
+                } else if (i >= ie || tidx[j] < erow[i]) {
+                    w = tidx[j++];
+                } else {
+                    w = erow[i++];
+                    ++j;
+                }
+                if (w != v) aidx.push_back(w);
+            }
             aptr[v + 1] = int32_t(aidx.size());
         }
     }
@@ -309,8 +498,7 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
     const FrontTree T = nested_dissection(m, aptr.data(), aidx.data(), leaf);
     const int32_t F = T.fronts();
     nf_ = F;
-    std::vector<int64_t> roff(F + 1, 0), woff(F + 1, 0), goff(F + 1, 0), loff(F + 1, 0), uoff(F + 1, 0), toff(F + 1, 0);
-    int64_t max_m2 = 1, max_pm = 1, max_work = 1;
+    std::vector<int64_t> roff(F + 1, 0), woff(F + 1, 0), goff(F + 1, 0), loff(F + 1, 0), toff(F + 1, 0);
     for (int32_t f = 0; f < F; ++f) {
         const int64_t p = T.np[f], u = T.nu(f), mf = p + u;
         if (mf > kMaxFront)
@@ -321,82 +509,62 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
         woff[f + 1] = woff[f] + u;
         goff[f + 1] = goff[f] + p * p;
         loff[f + 1] = loff[f] + u * p;
-        max_m2 = std::max(max_m2, mf * mf);
-        max_pm = std::max(max_pm, p * mf);
-        max_work = std::max(max_work, DenseBlas::work_size(mf));
     }
     rtot_ = roff[F];
     wtot_ = woff[F];
-    // update matrices on a stack: in postorder a front's children's updates lie on top,
-    // contiguous and in child order; after its extend-add the front's own update takes
-    // their place (uoff[f]: its offset; the stack peaks at stack_peak doubles)
-    int64_t stack_peak = 1;
-    {
-        int64_t top = 0;
-        for (int32_t f = 0; f < F; ++f) {
-            int64_t base = top;
-            for (int32_t c : T.children[f]) base -= T.nu(c) * T.nu(c);
-            for (size_t i = 0, at = size_t(base); i < T.children[f].size(); ++i) {
-                const int32_t c = T.children[f][i];
-                if (uoff[c] != int64_t(at))
-                    fail(LSS_ERR_HIP, "sparse stiff factorization: update stack out of order (internal)");
-                at += size_t(T.nu(c) * T.nu(c));
-            }
-            uoff[f] = base;
-            top = base + T.nu(f) * T.nu(f);
-            stack_peak = std::max(stack_peak, top);
-        }
-    }
-    stack_bytes_ = stack_peak * int64_t(sizeof(double));
-    // front-local row of position q in front f
+    // front-local rows: pivots first, then the update positions (lmap holds the latter
+    // for one front at a time)
+    std::vector<int32_t> fof(m), lmap(m, -1);
+    for (int32_t f = 0; f < F; ++f)
+        for (int32_t q = T.p0[f]; q < T.p0[f] + T.np[f]; ++q) fof[q] = f;
+    auto map_front = [&](int32_t f, int32_t on) {
+        for (int64_t t = T.uptr[f]; t < T.uptr[f + 1]; ++t) lmap[T.uidx[t]] = on ? T.np[f] + int32_t(t - T.uptr[f]) : -1;
+    };
     auto loc = [&](int32_t f, int32_t q) -> int32_t {
-        const int32_t p1 = T.p0[f] + T.np[f];
-        if (q >= T.p0[f] && q < p1) return q - T.p0[f];
-        const auto b0 = T.uidx.begin() + T.uptr[f], b1 = T.uidx.begin() + T.uptr[f + 1];
-        const auto it = std::lower_bound(b0, b1, q);
-        if (q < T.p0[f] || it == b1 || *it != q)
-            fail(LSS_ERR_HIP, "sparse stiff factorization: position outside its front (internal)");
-        return T.np[f] + int32_t(it - b0);
+        const int32_t r = q >= T.p0[f] && q < T.p0[f] + T.np[f] ? q - T.p0[f] : lmap[q];
+        if (r < 0) fail(LSS_ERR_HIP, "sparse stiff factorization: position outside its front (internal)");
+        return r;
     };
-    // CSC entry of the pair (a, b): operator row max(a, b), column min(a, b)
-    auto entry = [&](int32_t a, int32_t b) -> int64_t {
-        const int32_t hi = std::max(a, b), lo = std::min(a, b);
-        const auto b0 = erow.begin() + eptr[hi], b1 = erow.begin() + eptr[hi + 1];
-        const auto it = std::lower_bound(b0, b1, lo);
-        return (it != b1 && *it == lo) ? esrc[it - erow.begin()] : -1;
-    };
-    // 5. original entries of every front's pivot columns (lower triangle of the front)
-    std::vector<int32_t> tli, tlj;
-    std::vector<int64_t> tsrc;
-    for (int32_t f = 0; f < F; ++f) {
-        toff[f] = int64_t(tli.size());
-        for (int32_t q = T.p0[f]; q < T.p0[f] + T.np[f]; ++q) {
-            const int32_t v = T.order[q];
-            const int64_t ed = entry(v, v);
-            if (ed >= 0) tli.push_back(q - T.p0[f]), tlj.push_back(q - T.p0[f]), tsrc.push_back(ed);
-            for (int32_t e = aptr[v]; e < aptr[v + 1]; ++e) {
-                const int32_t w = aidx[e], qw = T.pos[w];
-                if (qw < q) continue;
-                const int64_t ev = entry(v, w);
-                if (ev < 0) continue;
-                tli.push_back(loc(f, qw));
-                tlj.push_back(q - T.p0[f]);
-                tsrc.push_back(ev);
+    // 5. original entries: the pair {a, b} is read from the column of the later unit index
+    //    (its row the earlier one; the diagonal from its own column) and belongs to the front
+    //    that eliminates the earlier of its two positions: row li (the later position), column
+    //    lj (the earlier) of that front's lower triangle
+    for (int32_t c = 0; c < m; ++c)
+        for (int64_t i = eptr[c]; i < eptr[c + 1] && erow[i] <= c; ++i)
+            ++toff[fof[std::min(T.pos[erow[i]], T.pos[c])] + 1];
+    for (int32_t f = 0; f < F; ++f) toff[f + 1] += toff[f];
+    std::vector<int32_t> tli(std::max<int64_t>(toff[F], 1)), tlj(std::max<int64_t>(toff[F], 1));
+    std::vector<int64_t> tsrc(std::max<int64_t>(toff[F], 1));
+    {
+        std::vector<int64_t> cur(toff.begin(), toff.end() - 1);
+        for (int32_t c = 0; c < m; ++c)
+            for (int64_t i = eptr[c]; i < eptr[c + 1] && erow[i] <= c; ++i) {
+                const int32_t qa = T.pos[erow[i]], qb = T.pos[c], q1 = std::min(qa, qb), f = fof[q1];
+                const int64_t k = cur[f]++;
+                tli[k] = std::max(qa, qb);  // a position until the loop below
+                tlj[k] = q1 - T.p0[f];
+                tsrc[k] = esrc[i];
             }
-        }
     }
-    toff[F] = int64_t(tli.size());
+    for (int32_t f = 0; f < F; ++f) {
+        map_front(f, 1);
+        for (int64_t k = toff[f]; k < toff[f + 1]; ++k) tli[k] = loc(f, tli[k]);
+        map_front(f, 0);
+    }
     // 6. child -> parent row maps (at the child's woff) and, per parent row, the
     //    children's update entries that land there, children in order
     std::vector<int32_t> rel(std::max<int64_t>(wtot_, 1));
     std::vector<int64_t> cptr(rtot_ + 1, 0);
-    for (int32_t f = 0; f < F; ++f)
+    for (int32_t f = 0; f < F; ++f) {
+        map_front(f, 1);
         for (int32_t c : T.children[f])
             for (int64_t a = 0; a < T.nu(c); ++a) {
                 const int32_t r = loc(f, T.uidx[T.uptr[c] + a]);
                 rel[woff[c] + a] = r;
                 ++cptr[roff[f] + r + 1];
             }
+        map_front(f, 0);
+    }
     for (int64_t r = 0; r < rtot_; ++r) cptr[r + 1] += cptr[r];
     std::vector<int64_t> cidx(std::max<int64_t>(cptr[rtot_], 1));
     {
@@ -405,27 +573,305 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
             for (int32_t c : T.children[f])
                 for (int64_t a = 0; a < T.nu(c); ++a) cidx[cur[roff[f] + rel[woff[c] + a]]++] = woff[c] + a;
     }
+    // 7. plan of the numeric phase, level by level (tree heights; the fronts of one level
+    //    are independent).  Fronts of at most kBatchFront rows are factored together, in
+    //    groups whose workspaces fit kGroupWork doubles, one launch per step for the whole
+    //    group; larger fronts one at a time with the blocked dense path.  Update matrices
+    //    live in a pool from their front's factorization to their parent's assembly; the
+    //    offsets come from a first-fit plan in the order the stream reads and writes them.
+    int32_t H = 0;
+    for (int32_t f = 0; f < F; ++f) H = std::max(H, T.height[f]);
+    std::vector<std::vector<int32_t>> bylev(H + 1);
+    for (int32_t f = 0; f < F; ++f) bylev[T.height[f]].push_back(f);
+    const int32_t NB = k::dense_block_size();
+    auto mof = [&](int32_t f) { return int64_t(T.np[f]) + T.nu(f); };
+    auto usize = [&](int32_t f) { return align_up(T.nu(f) * T.nu(f)); };
+    std::vector<int64_t> wb(F, 0), upoff(F, 0);
+    FirstFit pool;
+    std::vector<int64_t> a_dst, a_src;
+    std::vector<k::ExtendTask> ext;
+    std::vector<k::DiagTask> dtask;
+    std::vector<k::PanelTask> ptask;
+    std::vector<k::TileTask> ttask;
+    std::vector<k::CopyTask> ctask;
+    std::vector<GroupPlan> groups;
+    std::vector<int32_t> stages;  // >= 0: group index; < 0: -1 - front (one-front path)
+    int64_t max_w = 1, max_slots = 1, max_m2 = 1, max_pm = 1, max_work = 1;
+    auto plan_group = [&](const std::vector<int32_t> &fr, GroupPlan &g) {
+        g.asmb.off = int64_t(a_dst.size());
+        for (int32_t f : fr)
+            for (int64_t t = toff[f]; t < toff[f + 1]; ++t) {
+                a_dst.push_back(wb[f] + int64_t(tli[t]) * mof(f) + tlj[t]);
+                a_src.push_back(tsrc[t]);
+            }
+        g.asmb.n = int64_t(a_dst.size()) - g.asmb.off;
+        size_t kmax = 0;
+        int32_t nbmax = 0;
+        for (int32_t f : fr) {
+            kmax = std::max(kmax, T.children[f].size());
+            nbmax = std::max(nbmax, (T.np[f] + NB - 1) / NB);
+        }
+        for (size_t j = 0; j < kmax; ++j) {  // children in order: one launch per child slot
+            Range r{int64_t(ext.size()), 0};
+            for (int32_t f : fr)
+                if (T.children[f].size() > j) {
+                    const int32_t c = T.children[f][j];
+                    for (int32_t a = 0; a < int32_t(T.nu(c)); ++a) ext.push_back({c, a});
+                }
+            r.n = int64_t(ext.size()) - r.off;
+            if (r.n) g.ext.push_back(r);
+        }
+        for (int32_t kk = 0; kk < nbmax; ++kk) {
+            const int32_t k0 = kk * NB;
+            std::array<Range, 3> st;
+            st[0].off = int64_t(dtask.size()), st[1].off = int64_t(ptask.size()), st[2].off = int64_t(ttask.size());
+            for (int32_t f : fr) {
+                const int32_t p = T.np[f], mf = int32_t(mof(f));
+                if (p <= k0) continue;
+                const int32_t kb = std::min(NB, p - k0), s0 = k0 + kb;
+                dtask.push_back({wb[f], goff[f], mf, p, k0, kb});
+                for (int32_t r = s0; r < mf; r += 32) ptask.push_back({wb[f], goff[f], mf, p, k0, kb, r, std::min(32, mf - r)});
+                for (int32_t i0 = s0; i0 < mf; i0 += 64)
+                    for (int32_t j0 = s0; j0 <= i0; j0 += 64)
+                        ttask.push_back({wb[f] + int64_t(i0) * mf + k0, wb[f] + int64_t(j0) * mf + k0,
+                                         wb[f] + int64_t(i0) * mf + j0, mf, mf, mf, std::min(64, mf - i0),
+                                         std::min(64, mf - j0), kb, i0 == j0 ? 1 : 0, 0});
+            }
+            st[0].n = int64_t(dtask.size()) - st[0].off, st[1].n = int64_t(ptask.size()) - st[1].off;
+            st[2].n = int64_t(ttask.size()) - st[2].off;
+            g.steps.push_back(st);
+        }
+        // off-diagonal blocks of G11 by distance d: T = sum_{K=J}^{I-1} L_IK G_KJ, G_IJ = -G_II T
+        for (int32_t d = 1; d < nbmax; ++d) {
+            std::array<Range, 2> rd;
+            int64_t slot = 0;
+            rd[0].off = int64_t(ttask.size());
+            std::vector<k::TileTask> gt;
+            for (int32_t f : fr) {
+                const int32_t p = T.np[f], mf = int32_t(mof(f)), nb = (p + NB - 1) / NB;
+                for (int32_t I = d; I < nb; ++I, ++slot) {
+                    const int32_t J = I - d, kbI = std::min(NB, p - I * NB);
+                    const int64_t tb = slot * NB * NB;
+                    for (int32_t ti = 0; ti < kbI; ti += 64)
+                        for (int32_t tj = 0; tj < NB; tj += 64) {
+                            ttask.push_back({wb[f] + int64_t(I * NB + ti) * mf + J * NB,
+                                             goff[f] + int64_t(J * NB) * p + J * NB + tj, tb + ti * NB + tj, mf, p,
+                                             NB, std::min(64, kbI - ti), 64, d * NB, 0, 0});
+                            gt.push_back({goff[f] + int64_t(I * NB + ti) * p + I * NB, tb + tj,
+                                          goff[f] + int64_t(I * NB + ti) * p + J * NB + tj, p, NB, p,
+                                          std::min(64, kbI - ti), 64, kbI, 0, 0});
+                        }
+                }
+            }
+            rd[0].n = int64_t(ttask.size()) - rd[0].off;
+            rd[1].off = int64_t(ttask.size());
+            ttask.insert(ttask.end(), gt.begin(), gt.end());
+            rd[1].n = int64_t(gt.size());
+            max_slots = std::max(max_slots, slot);
+            g.rounds.push_back(rd);
+        }
+        // outputs: L21, L21^T, the update matrix (into the pool), G11^T above G11
+        Range *cr[4] = {&g.cl, &g.clt, &g.cu, &g.cm};
+        for (int mode = 0; mode < 4; ++mode) {
+            cr[mode]->off = int64_t(ctask.size());
+            for (int32_t f : fr) {
+                const int32_t p = T.np[f], u = int32_t(T.nu(f)), mf = int32_t(mof(f));
+                if (mode < 2)
+                    for (int32_t a0 = 0; a0 < u; a0 += 32)
+                        for (int32_t j0 = 0; j0 < p; j0 += 32) {
+                            const int64_t src = wb[f] + int64_t(p + a0) * mf + j0;
+                            if (mode == 0)
+                                ctask.push_back({src, loff[f] + int64_t(a0) * p + j0, mf, p, std::min(32, u - a0),
+                                                 std::min(32, p - j0), 0, 0});
+                            else
+                                ctask.push_back({src, loff[f] + int64_t(j0) * u + a0, mf, u, std::min(32, u - a0),
+                                                 std::min(32, p - j0), 1, 0});
+                        }
+                else if (mode == 2)
+                    for (int32_t a0 = 0; a0 < u; a0 += 32)
+                        for (int32_t b0 = 0; b0 <= a0; b0 += 32)
+                            ctask.push_back({wb[f] + int64_t(p + a0) * mf + p + b0, upoff[f] + int64_t(a0) * u + b0,
+                                             mf, u, std::min(32, u - a0), std::min(32, u - b0), 0, 0});
+                else
+                    for (int32_t i0 = 0; i0 < p; i0 += 32)
+                        for (int32_t j0 = 0; j0 <= i0; j0 += 32)
+                            ctask.push_back({goff[f] + int64_t(i0) * p + j0, goff[f] + int64_t(j0) * p + i0, p, p,
+                                             std::min(32, p - i0), std::min(32, p - j0), i0 == j0 ? 2 : 1, 0});
+            }
+            cr[mode]->n = int64_t(ctask.size()) - cr[mode]->off;
+        }
+    };
+    for (int32_t h = 0; h <= H; ++h) {
+        std::vector<int32_t> small, big;
+        for (int32_t f : bylev[h]) (mof(f) <= kBatchFront ? small : big).push_back(f);
+        for (size_t i = 0; i < small.size();) {
+            std::vector<int32_t> fr;
+            int64_t w = 0;
+            while (i < small.size()) {
+                const int64_t sz = align_up(mof(small[i]) * mof(small[i]));
+                if (!fr.empty() && w + sz > kGroupWork) break;
+                wb[small[i]] = w;
+                w += sz;
+                fr.push_back(small[i++]);
+            }
+            for (int32_t f : fr)
+                for (int32_t c : T.children[f]) pool.release(upoff[c], usize(c));
+            for (int32_t f : fr) upoff[f] = pool.take(usize(f));
+            GroupPlan g;
+            g.wsize = w;
+            max_w = std::max(max_w, w);
+            plan_group(fr, g);
+            stages.push_back(int32_t(groups.size()));
+            groups.push_back(std::move(g));
+            batched_ += int32_t(fr.size());
+        }
+        for (int32_t f : big) {
+            for (int32_t c : T.children[f]) pool.release(upoff[c], usize(c));
+            upoff[f] = pool.take(usize(f));
+            stages.push_back(-1 - f);
+            const int64_t mf = mof(f);
+            max_m2 = std::max(max_m2, mf * mf);
+            max_pm = std::max(max_pm, int64_t(T.np[f]) * mf);
+            max_work = std::max(max_work, DenseBlas::work_size(mf));
+        }
+    }
+    // every task inside its buffers (an internal error otherwise, before any launch)
+    {
+        const int64_t nW = max_w, nG = goff[F], nL = loff[F], nT = max_slots * NB * NB, nU = pool.peak();
+        bool ok = true;
+        auto in = [&](int64_t first, int64_t last, int64_t n) { ok = ok && first >= 0 && last < n; };
+        auto tiles = [&](const Range &r, bool nt, int64_t na, int64_t nb, int64_t nc) {
+            for (int64_t i = r.off; i < r.off + r.n; ++i) {
+                const k::TileTask &t = ttask[i];
+                ok = ok && t.M > 0 && t.N > 0 && t.K > 0 && t.M <= 64 && t.N <= 64;
+                in(t.a, t.a + int64_t(t.M - 1) * t.lda + t.K - 1, na);
+                in(t.b, nt ? t.b + int64_t(t.N - 1) * t.ldb + t.K - 1 : t.b + int64_t(t.K - 1) * t.ldb + t.N - 1, nb);
+                in(t.c, t.c + int64_t(t.M - 1) * t.ldc + t.N - 1, nc);
+            }
+        };
+        auto copies = [&](const Range &r, int64_t ns, int64_t nd) {
+            for (int64_t i = r.off; i < r.off + r.n; ++i) {
+                const k::CopyTask &t = ctask[i];
+                ok = ok && t.rows > 0 && t.cols > 0 && t.rows <= 32 && t.cols <= 32;
+                in(t.src, t.src + int64_t(t.rows - 1) * t.lds + t.cols - 1, ns);
+                in(t.dst, t.mode == 0 ? t.dst + int64_t(t.rows - 1) * t.ldd + t.cols - 1
+                                      : t.dst + int64_t(t.cols - 1) * t.ldd + t.rows - 1, nd);
+            }
+        };
+        for (const GroupPlan &g : groups) {
+            ok = ok && g.wsize <= nW;
+            for (int64_t i = g.asmb.off; i < g.asmb.off + g.asmb.n; ++i) in(a_dst[i], a_dst[i], g.wsize);
+            for (const Range &r : g.ext)
+                for (int64_t i = r.off; i < r.off + r.n; ++i) {
+                    const int32_t c = ext[i].c, a = ext[i].a, f = c >= 0 && c < F ? T.parent[c] : -1;
+                    ok = ok && f >= 0 && a >= 0 && a < T.nu(c);
+                    if (!ok) break;
+                    in(upoff[c], upoff[c] + int64_t(a) * T.nu(c) + a, nU);
+                    in(wb[f], wb[f] + mof(f) * mof(f) - 1, g.wsize);
+                }
+            for (const auto &q : g.steps) {
+                for (int64_t i = q[0].off; i < q[0].off + q[0].n; ++i) {
+                    const k::DiagTask &t = dtask[i];
+                    const int64_t e = t.k0 + t.kb - 1;
+                    ok = ok && t.kb > 0 && t.kb <= NB;
+                    in(t.a, t.a + e * t.lda + e, g.wsize);
+                    in(t.g, t.g + e * t.ldg + e, nG);
+                }
+                for (int64_t i = q[1].off; i < q[1].off + q[1].n; ++i) {
+                    const k::PanelTask &t = ptask[i];
+                    const int64_t e = t.k0 + t.kb - 1;
+                    ok = ok && t.kb > 0 && t.kb <= NB && t.nr > 0 && t.nr <= 32;
+                    in(t.a, t.a + int64_t(t.r0 + t.nr - 1) * t.lda + e, g.wsize);
+                    in(t.g, t.g + e * t.ldg + e, nG);
+                }
+                tiles(q[2], true, g.wsize, g.wsize, g.wsize);
+            }
+            for (const auto &q : g.rounds) {
+                tiles(q[0], false, g.wsize, nG, nT);
+                tiles(q[1], false, nG, nT, nG);
+            }
+            copies(g.cl, g.wsize, nL);
+            copies(g.clt, g.wsize, nL);
+            copies(g.cu, g.wsize, nU);
+            copies(g.cm, nG, nG);
+        }
+        if (!ok) fail(LSS_ERR_HIP, "sparse stiff factorization: batched plan out of bounds (internal)");
+    }
+    groups_ = int32_t(groups.size());
+    stack_bytes_ = std::max<int64_t>(pool.peak(), 1) * int64_t(sizeof(double));
     t_symbolic = since(t0);
-    // 7. numeric factorization, children first
+    // 8. numeric factorization
     const auto t1 = clk::now();
+    np_.upload(T.np.data(), F, s);
+    {
+        std::vector<int32_t> nu(F);
+        for (int32_t f = 0; f < F; ++f) nu[f] = int32_t(T.nu(f));
+        nu_.upload(nu.data(), F, s);
+    }
+    woff_.upload(woff.data(), F + 1, s);
     G_.resize(std::max<int64_t>(goff[F], 1));
     L_.resize(std::max<int64_t>(loff[F], 1));
     Lt_.resize(std::max<int64_t>(loff[F], 1));
+    k::fill(goff[F], G_.get(), 0.0, s);  // the batched steps read G11's upper blocks as zeros
     {
-        DBuf<double> upd(stack_peak), Fw(max_m2), Gt(max_pm), work(max_work);
-        DBuf<int32_t> d_li, d_lj, d_rel;
-        DBuf<int64_t> d_src;
+        DBuf<double> upd(std::max<int64_t>(pool.peak(), 1)), W(groups.empty() ? 1 : max_w),
+            Tw(groups.empty() ? 1 : max_slots * NB * NB);
+        const bool any_big = groups.size() < stages.size();
+        DBuf<double> Fw(any_big ? max_m2 : 1), Gt(any_big ? max_pm : 1), work(any_big ? max_work : 1);
+        DBuf<int32_t> d_li, d_lj, d_rel, d_parent;
+        DBuf<int64_t> d_src, d_wb, d_upoff, d_adst, d_asrc;
+        DBuf<k::ExtendTask> d_ext;
+        DBuf<k::DiagTask> d_dt;
+        DBuf<k::PanelTask> d_pt;
+        DBuf<k::TileTask> d_tt;
+        DBuf<k::CopyTask> d_ct;
         d_li.upload(tli.data(), tli.size(), s);
         d_lj.upload(tlj.data(), tlj.size(), s);
         d_src.upload(tsrc.data(), tsrc.size(), s);
         d_rel.upload(rel.data(), rel.size(), s);
-        for (int32_t f = 0; f < F; ++f) {
+        d_parent.upload(T.parent.data(), F, s);
+        d_wb.upload(wb.data(), F, s);
+        d_upoff.upload(upoff.data(), F, s);
+        d_adst.upload(a_dst.data(), a_dst.size(), s);
+        d_asrc.upload(a_src.data(), a_src.size(), s);
+        d_ext.upload(ext.data(), ext.size(), s);
+        d_dt.upload(dtask.data(), dtask.size(), s);
+        d_pt.upload(ptask.data(), ptask.size(), s);
+        d_tt.upload(ttask.data(), ttask.size(), s);
+        d_ct.upload(ctask.data(), ctask.size(), s);
+        for (const int32_t st : stages) {
+            if (st >= 0) {
+                const GroupPlan &g = groups[st];
+                double *w = W.get();
+                k::fill(g.wsize, w, 0.0, s);
+                k::front_assemble_batch(g.asmb.n, d_adst.get() + g.asmb.off, d_asrc.get() + g.asmb.off, d_vals, w, s);
+                for (const Range &r : g.ext)
+                    k::front_extend_batch(r.n, d_ext.get() + r.off, d_parent.get(), d_wb.get(), np_.get(), nu_.get(),
+                                          woff_.get(), d_upoff.get(), d_rel.get(), upd.get(), w, s);
+                for (const auto &q : g.steps) {
+                    k::dense_diag_batch(int32_t(q[0].n), d_dt.get() + q[0].off, w, G_.get(), d_status, s);
+                    k::front_panel_batch(int32_t(q[1].n), d_pt.get() + q[1].off, w, G_.get(), s);
+                    k::front_gemm_batch(int32_t(q[2].n), d_tt.get() + q[2].off, true, -1.0, 1.0, w, w, w, s);
+                }
+                for (const auto &q : g.rounds) {
+                    k::front_gemm_batch(int32_t(q[0].n), d_tt.get() + q[0].off, false, 1.0, 0.0, w, G_.get(),
+                                        Tw.get(), s);
+                    k::front_gemm_batch(int32_t(q[1].n), d_tt.get() + q[1].off, false, -1.0, 0.0, G_.get(),
+                                        Tw.get(), G_.get(), s);
+                }
+                k::front_copy_batch(int32_t(g.cl.n), d_ct.get() + g.cl.off, w, L_.get(), s);
+                k::front_copy_batch(int32_t(g.clt.n), d_ct.get() + g.clt.off, w, Lt_.get(), s);
+                k::front_copy_batch(int32_t(g.cu.n), d_ct.get() + g.cu.off, w, upd.get(), s);
+                k::front_copy_batch(int32_t(g.cm.n), d_ct.get() + g.cm.off, G_.get(), G_.get(), s);
+                continue;
+            }
+            const int32_t f = -1 - st;
             const int64_t p = T.np[f], u = T.nu(f), mf = p + u;
             k::fill(mf * mf, Fw.get(), 0.0, s);
             k::sparse_assemble(toff[f + 1] - toff[f], d_li.get() + toff[f], d_lj.get() + toff[f],
                                d_src.get() + toff[f], d_vals, Fw.get(), mf, s);
             for (int32_t c : T.children[f])
-                k::sparse_extend_add(int32_t(T.nu(c)), d_rel.get() + woff[c], upd.get() + uoff[c], Fw.get(), mf, s);
+                k::sparse_extend_add(int32_t(T.nu(c)), d_rel.get() + woff[c], upd.get() + upoff[c], Fw.get(), mf, s);
             blas.partial_factor_inverse(mf, p, Fw.get(), Gt.get(), work.get(), d_status, s);
             LSS_HIP(hipMemcpy2DAsync(G_.get() + goff[f], size_t(p) * sizeof(double), Gt.get(),
                                      size_t(mf) * sizeof(double), size_t(p) * sizeof(double), size_t(p),
@@ -435,20 +881,16 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
                 LSS_HIP(hipMemcpy2DAsync(L_.get() + loff[f], size_t(p) * sizeof(double), Fw.get() + p * mf,
                                          size_t(mf) * sizeof(double), size_t(p) * sizeof(double), size_t(u),
                                          hipMemcpyDeviceToDevice, s));
-                LSS_HIP(hipMemcpy2DAsync(upd.get() + uoff[f], size_t(u) * sizeof(double), Fw.get() + p * mf + p,
+                LSS_HIP(hipMemcpy2DAsync(upd.get() + upoff[f], size_t(u) * sizeof(double), Fw.get() + p * mf + p,
                                          size_t(mf) * sizeof(double), size_t(u) * sizeof(double), size_t(u),
                                          hipMemcpyDeviceToDevice, s));
                 k::sparse_transpose(u, p, L_.get() + loff[f], Lt_.get() + loff[f], s);
             }
         }
-        LSS_HIP(hipStreamSynchronize(s));  // the workspaces are released at scope end
+        LSS_HIP(hipStreamSynchronize(s));  // the workspaces and task lists are released at scope end
     }
     t_numeric = since(t1);
-    // 8. solve tables: fronts by height, task lists per level
-    int32_t H = 0;
-    for (int32_t f = 0; f < F; ++f) H = std::max(H, T.height[f]);
-    std::vector<std::vector<int32_t>> bylev(H + 1);
-    for (int32_t f = 0; f < F; ++f) bylev[T.height[f]].push_back(f);
+    // 9. solve tables: task lists per tree level
     std::vector<int2> trhs, ty, tw, tx;
     for (auto *lv : {&lv_rhs_, &lv_y_, &lv_w_, &lv_x_}) lv->assign(H + 2, 0);
     for (int32_t h = 0; h <= H; ++h) {
@@ -469,12 +911,7 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
     t_x_.upload(tx.data(), tx.size(), s);
     order_.upload(T.order.data(), m, s);
     p0_.upload(T.p0.data(), F, s);
-    np_.upload(T.np.data(), F, s);
-    std::vector<int32_t> nu(F);
-    for (int32_t f = 0; f < F; ++f) nu[f] = int32_t(T.nu(f));
-    nu_.upload(nu.data(), F, s);
     roff_.upload(roff.data(), F + 1, s);
-    woff_.upload(woff.data(), F + 1, s);
     goff_.upload(goff.data(), F + 1, s);
     loff_.upload(loff.data(), F + 1, s);
     {

@@ -58,10 +58,12 @@ public:
     int32_t levels() const { return lv_rhs_.empty() ? 0 : int32_t(lv_rhs_.size()) - 1; }  // tree heights 0 .. levels() - 1
     int64_t factor_entries() const { return int64_t(G_.size() + L_.size()); }  // G11 + L21 (L21^T kept besides)
     int64_t stack_bytes() const { return stack_bytes_; }  // peak update-matrix storage of the factorization
+    int32_t batched_fronts() const { return batched_; }    // fronts factored in level batches
+    int32_t front_groups() const { return groups_; }       // batches
     double t_symbolic = 0, t_numeric = 0;
 
 private:
-    int32_t m_ = 0, nf_ = 0, max_front_ = 0;
+    int32_t m_ = 0, nf_ = 0, max_front_ = 0, batched_ = 0, groups_ = 0;
     int64_t rtot_ = 0, wtot_ = 0, stack_bytes_ = 0;
     DBuf<int32_t> order_, p0_, np_, nu_, uidx_;
     DBuf<int64_t> roff_, woff_, goff_, loff_, cptr_, cidx_;

@@ -6,7 +6,9 @@
 // take a dense Cholesky here; larger ones a multifrontal one: a nested-dissection
 // ordering and the front structures on the host, then per front (children first)
 // assembly of the original entries, extend-add of the children's update matrices,
-// and a partial blocked Cholesky (dense kernels + rocBLAS, host/sparse_stiff.cpp).
+// and a partial blocked Cholesky: the fronts of one tree level together through the
+// batched kernels below (task lists built on the host), large fronts one at a time
+// with the dense kernels and rocBLAS (host/sparse_stiff.cpp).
 // A front keeps G11 = L11^{-1} (with its transpose in the strict upper triangle)
 // and L21, so both triangular solves are products.  The solve walks the tree
 // level by level: forward, bottom-up, each front gathers its right-hand side and
@@ -199,6 +201,168 @@ __global__ __launch_bounds__(kT) void k_scatter_add(int32_t m, const int32_t *__
     if (z32) z32[d] = float(v);
 }
 
+// ---- batched factorization of a tree level's fronts ----
+
+__global__ __launch_bounds__(kT) void k_fb_assemble(int64_t ntrip, const int64_t *__restrict__ dst,
+                                                    const int64_t *__restrict__ src,
+                                                    const double *__restrict__ vals, double *__restrict__ W)
+{
+    const int64_t t = int64_t(blockIdx.x) * kT + threadIdx.x;
+    if (t >= ntrip) return;
+    W[dst[t]] = vals[src[t]];
+}
+
+// one wave per (child, update row); the children of one launch have distinct parents
+__global__ __launch_bounds__(kT) void k_fb_extend(int64_t ntask, const ExtendTask *__restrict__ tasks,
+                                                  const int32_t *__restrict__ parent, const int64_t *__restrict__ wb,
+                                                  const int32_t *__restrict__ np, const int32_t *__restrict__ nu,
+                                                  const int64_t *__restrict__ woff, const int64_t *__restrict__ uoff,
+                                                  const int32_t *__restrict__ rel, const double *__restrict__ U,
+                                                  double *__restrict__ W)
+{
+    const int64_t t = int64_t(blockIdx.x) * (kT / 64) + (threadIdx.x >> 6);
+    if (t >= ntask) return;
+    const int lane = threadIdx.x & 63;
+    const ExtendTask task = tasks[t];
+    const int32_t c = task.c, a = task.a, f = parent[c];
+    const int64_t m = int64_t(np[f]) + nu[f];
+    const int32_t *rc = rel + woff[c];
+    const double *row = U + uoff[c] + int64_t(a) * nu[c];
+    double *Fr = W + wb[f] + int64_t(rc[a]) * m;
+    for (int32_t b = lane; b <= a; b += 64) Fr[rc[b]] += row[b];
+}
+
+// panel rows r0 .. r0 + nr - 1 below diagonal block k0: L_rk = A_rk G_kk^T (G_kk lower triangular,
+// zero above its diagonal).  The workgroup holds its rows in LDS, so the product is in place.
+__global__ __launch_bounds__(kT) void k_fb_panel(const PanelTask *__restrict__ tasks, double *__restrict__ W,
+                                                 const double *__restrict__ G)
+{
+    constexpr int R = 32, NB = 128, KC = 16;
+    __shared__ double As[R][NB + 1];
+    __shared__ double Gs[KC][NB + 4];
+    const PanelTask t = tasks[blockIdx.x];
+    const int tid = threadIdx.x;
+    double *a = W + t.a + int64_t(t.r0) * t.lda + t.k0;
+    const double *g = G + t.g + int64_t(t.k0) * t.ldg + t.k0;
+    for (int q = tid; q < R * NB; q += kT) {
+        const int i = q >> 7, c = q & (NB - 1);
+        As[i][c] = (i < t.nr && c < t.kb) ? a[int64_t(i) * t.lda + c] : 0.0;
+    }
+    const int ty = tid >> 5, tx = tid & 31;  // rows 4 ty + u, columns tx + 32 v
+    double acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+    for (int k0 = 0; k0 < t.kb; k0 += KC) {
+        __syncthreads();
+        for (int q = tid; q < KC * NB; q += kT) {
+            const int c = q >> 4, kk = q & (KC - 1);
+            Gs[kk][c] = (c < t.kb && k0 + kk < t.kb) ? g[int64_t(c) * t.ldg + k0 + kk] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) av[u] = As[4 * ty + u][k0 + kk];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) bv[v] = Gs[kk][tx + 32 * v];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 4 * ty + u, c = tx + 32 * v;
+            if (i < t.nr && c < t.kb) a[int64_t(i) * t.lda + c] = acc[u][v];
+        }
+}
+
+// one 64 x 64 tile of C = beta C + alpha A op(B), K in chunks of 16 through LDS, 4 x 4 per thread
+template <bool NT>
+__global__ __launch_bounds__(kT) void k_fb_gemm(const TileTask *__restrict__ tasks, double alpha, double beta,
+                                                const double *__restrict__ A, const double *__restrict__ B,
+                                                double *__restrict__ C)
+{
+    constexpr int TM = 64, KC = 16, LD = TM + 4;
+    __shared__ double As[KC][LD], Bs[KC][LD];
+    const TileTask t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+    const double *a = A + t.a, *b = B + t.b;
+    double acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+    for (int k0 = 0; k0 < t.K; k0 += KC) {
+        for (int q = tid; q < TM * KC; q += kT) {
+            const int i = q >> 4, kk = q & (KC - 1);
+            As[kk][i] = (i < t.M && k0 + kk < t.K) ? a[int64_t(i) * t.lda + k0 + kk] : 0.0;
+        }
+        if (NT) {
+            for (int q = tid; q < TM * KC; q += kT) {
+                const int j = q >> 4, kk = q & (KC - 1);
+                Bs[kk][j] = (j < t.N && k0 + kk < t.K) ? b[int64_t(j) * t.ldb + k0 + kk] : 0.0;
+            }
+        } else {
+            for (int q = tid; q < TM * KC; q += kT) {
+                const int kk = q >> 6, j = q & (TM - 1);
+                Bs[kk][j] = (j < t.N && k0 + kk < t.K) ? b[int64_t(k0 + kk) * t.ldb + j] : 0.0;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) av[u] = As[kk][4 * ty + u];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) bv[v] = Bs[kk][4 * tx + v];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
+        }
+        __syncthreads();
+    }
+    double *c = C + t.c;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 4 * ty + u, j = 4 * tx + v;
+            if (i < t.M && j < t.N && (!t.lower || j <= i)) {
+                double *o = c + int64_t(i) * t.ldc + j;
+                *o = beta == 0.0 ? alpha * acc[u][v] : beta * *o + alpha * acc[u][v];
+            }
+        }
+}
+
+__global__ __launch_bounds__(kT) void k_fb_copy(const CopyTask *__restrict__ tasks, const double *__restrict__ src,
+                                                double *__restrict__ dst)
+{
+    __shared__ double tile[32][33];
+    const CopyTask t = tasks[blockIdx.x];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const double *s = src + t.src;
+    double *d = dst + t.dst;
+    if (t.mode == 0) {
+        for (int i = ty; i < t.rows; i += kT / 32)
+            if (tx < t.cols) d[int64_t(i) * t.ldd + tx] = s[int64_t(i) * t.lds + tx];
+        return;
+    }
+    for (int i = ty; i < 32; i += kT / 32) tile[i][tx] = (i < t.rows && tx < t.cols) ? s[int64_t(i) * t.lds + tx] : 0.0;
+    __syncthreads();
+    // destination row j (a source column), column i (a source row)
+    for (int j = ty; j < t.cols; j += kT / 32)
+        if (tx < t.rows && (t.mode == 1 || j < tx)) d[int64_t(j) * t.ldd + tx] = tile[tx][j];
+}
+
 int waves_grid(int64_t ntask) { return div_up(ntask, kT / 64); }
 
 }  // namespace
@@ -287,6 +451,46 @@ void sparse_scatter_add(int32_t m, const int32_t *order, const int32_t *pdofs, c
 {
     if (m == 0) return;
     k_scatter_add<<<dim3(div_up(m, kT), k), kT, 0, s>>>(m, order, pdofs, xpos, z, z32, ld, stop);
+    LSS_LAUNCH_CHECK();
+}
+
+void front_panel_batch(int32_t ntask, const PanelTask *tasks, double *W, const double *G, hipStream_t s)
+{
+    if (ntask == 0) return;
+    k_fb_panel<<<ntask, kT, 0, s>>>(tasks, W, G);
+    LSS_LAUNCH_CHECK();
+}
+
+void front_gemm_batch(int32_t ntask, const TileTask *tasks, bool nt, double alpha, double beta, const double *A,
+                      const double *B, double *C, hipStream_t s)
+{
+    if (ntask == 0) return;
+    if (nt) k_fb_gemm<true><<<ntask, kT, 0, s>>>(tasks, alpha, beta, A, B, C);
+    else k_fb_gemm<false><<<ntask, kT, 0, s>>>(tasks, alpha, beta, A, B, C);
+    LSS_LAUNCH_CHECK();
+}
+
+void front_copy_batch(int32_t ntask, const CopyTask *tasks, const double *src, double *dst, hipStream_t s)
+{
+    if (ntask == 0) return;
+    k_fb_copy<<<ntask, kT, 0, s>>>(tasks, src, dst);
+    LSS_LAUNCH_CHECK();
+}
+
+void front_extend_batch(int64_t ntask, const ExtendTask *tasks, const int32_t *parent, const int64_t *wb,
+                        const int32_t *np, const int32_t *nu, const int64_t *woff, const int64_t *uoff,
+                        const int32_t *rel, const double *U, double *W, hipStream_t s)
+{
+    if (ntask == 0) return;
+    k_fb_extend<<<waves_grid(ntask), kT, 0, s>>>(ntask, tasks, parent, wb, np, nu, woff, uoff, rel, U, W);
+    LSS_LAUNCH_CHECK();
+}
+
+void front_assemble_batch(int64_t ntrip, const int64_t *dst, const int64_t *src, const double *vals, double *W,
+                          hipStream_t s)
+{
+    if (ntrip == 0) return;
+    k_fb_assemble<<<div_up(ntrip, kT), kT, 0, s>>>(ntrip, dst, src, vals, W);
     LSS_LAUNCH_CHECK();
 }
 

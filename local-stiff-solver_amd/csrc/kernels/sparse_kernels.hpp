@@ -22,6 +22,43 @@ struct FrontView {
     const int64_t *cidx = nullptr;   // ... as indices into the update buffer, children in order
 };
 
+// Batched factorization of the fronts of one tree level (several fronts per launch).
+// Fronts live in a level workspace W (front f: m x m row-major at its own offset), G11
+// in the factor array G (p x p at goff); offsets below are element offsets.
+struct DiagTask {  // factor + invert the kb x kb diagonal block at (k0, k0) of one front
+    int64_t a, g;  // front in W, its G11 in G
+    int32_t lda, ldg, k0, kb;
+};
+struct PanelTask {  // rows r0 .. r0 + nr - 1 (nr <= 32) of the panel below block k0: A_rk := A_rk G_kk^T
+    int64_t a, g;
+    int32_t lda, ldg, k0, kb, r0, nr;
+};
+struct TileTask {  // one 64 x 64 tile: C = beta C + alpha A op(B), K inner; lower: write j <= i only
+    int64_t a, b, c;
+    int32_t lda, ldb, ldc, M, N, K, lower, pad;
+};
+struct CopyTask {  // one 32 x 32 tile: mode 0 copy, 1 transpose, 2 transpose of the strict lower part
+    int64_t src, dst;
+    int32_t lds, ldd, rows, cols, mode, pad;
+};
+struct ExtendTask {  // row a of child c's update matrix into its parent's front
+    int32_t c, a;
+};
+
+void dense_diag_batch(int32_t ntask, const DiagTask *tasks, double *W, double *G, int *status, hipStream_t s);
+void front_panel_batch(int32_t ntask, const PanelTask *tasks, double *W, const double *G, hipStream_t s);
+// NT: op(B) = B^T (B read by rows j), else op(B) = B (rows k); beta = 0 reads no C
+void front_gemm_batch(int32_t ntask, const TileTask *tasks, bool nt, double alpha, double beta, const double *A,
+                      const double *B, double *C, hipStream_t s);
+void front_copy_batch(int32_t ntask, const CopyTask *tasks, const double *src, double *dst, hipStream_t s);
+// W[wb[f] + rel_c[a] * m_f + rel_c[b]] += U[uoff[c] + a * u_c + b], b <= a, f = parent[c] (distinct parents)
+void front_extend_batch(int64_t ntask, const ExtendTask *tasks, const int32_t *parent, const int64_t *wb,
+                        const int32_t *np, const int32_t *nu, const int64_t *woff, const int64_t *uoff,
+                        const int32_t *rel, const double *U, double *W, hipStream_t s);
+// W[dst[t]] = vals[src[t]] (distinct destinations)
+void front_assemble_batch(int64_t ntrip, const int64_t *dst, const int64_t *src, const double *vals, double *W,
+                          hipStream_t s);
+
 // out[coff[v] + t] = idx[ptr[dofs[v]] + t]: the stored rows of the unit's columns.
 void sparse_gather_pattern(int32_t m, const int32_t *dofs, const int32_t *ptr, const int64_t *coff,
                            const int32_t *idx, int32_t *out, hipStream_t s);

@@ -12,6 +12,7 @@
 // (r - A z1) only on the stiff rows (the other rows of the reference's full
 // SpMV are never read) and adds x into z in place.
 #include "mas_kernels.hpp"
+#include "sparse_kernels.hpp"
 #include "../common/runtime.hpp"
 
 namespace lss::k {
@@ -310,8 +311,8 @@ __host__ __device__ constexpr size_t dense_diag_lds(int kb)
     return (size_t(kb) * (kb + 1) + kb + 3 * kDenseSB * kDenseSB) * sizeof(double);
 }
 
-__global__ __launch_bounds__(256) void k_dense_diag(int64_t m, int k0, int kb, double *__restrict__ A,
-                                                    double *__restrict__ G, int *__restrict__ status)
+__device__ __forceinline__ void dense_diag_run(int64_t lda, int k0, int kb, double *__restrict__ A, int64_t ldg,
+                                               double *__restrict__ G, int *__restrict__ status)
 {
     constexpr int SB = kDenseSB;
     extern __shared__ double lds[];
@@ -324,7 +325,7 @@ __global__ __launch_bounds__(256) void k_dense_diag(int64_t m, int k0, int kb, d
     if (tid == 0) bad = 0;
     for (int t = tid; t < kb * kb; t += 256) {
         const int i = t / kb, j = t - i * kb;
-        if (j <= i) M[i * DP + j] = A[(k0 + i) * m + k0 + j];
+        if (j <= i) M[i * DP + j] = A[(k0 + i) * lda + k0 + j];
     }
     __syncthreads();
     const int nsub = (kb + SB - 1) / SB;
@@ -499,11 +500,25 @@ __global__ __launch_bounds__(256) void k_dense_diag(int64_t m, int k0, int kb, d
     }
     for (int t = tid; t < kb * kb; t += 256) {
         const int i = t / kb, j = t - i * kb;
-        A[(k0 + i) * m + k0 + j] = j <= i ? M[i * DP + j] : 0.0;
-        if (j < i) G[(k0 + i) * m + k0 + j] = M[j * DP + i];
-        else if (j == i) G[(k0 + i) * m + k0 + j] = dg[i];
+        A[(k0 + i) * lda + k0 + j] = j <= i ? M[i * DP + j] : 0.0;
+        if (j < i) G[(k0 + i) * ldg + k0 + j] = M[j * DP + i];
+        else if (j == i) G[(k0 + i) * ldg + k0 + j] = dg[i];
     }
     if (tid == 0 && bad) atomicOr(status, 2);
+}
+
+__global__ __launch_bounds__(256) void k_dense_diag(int64_t m, int k0, int kb, double *__restrict__ A,
+                                                    double *__restrict__ G, int *__restrict__ status)
+{
+    dense_diag_run(m, k0, kb, A, m, G, status);
+}
+
+// one workgroup per task: the diagonal block of one front of a batch
+__global__ __launch_bounds__(256) void k_dense_diag_batch(const DiagTask *__restrict__ tasks, double *__restrict__ W,
+                                                          double *__restrict__ G, int *__restrict__ status)
+{
+    const DiagTask t = tasks[blockIdx.x];
+    dense_diag_run(t.lda, t.k0, t.kb, W + t.a, t.ldg, G + t.g, status);
 }
 
 // Upper triangle of G := transpose of its lower triangle (32 x 32 tiles via LDS).
@@ -574,6 +589,19 @@ void dense_diag_block(int64_t m, int k0, int kb, double *A, double *G, int *stat
         attr = true;
     }
     k_dense_diag<<<1, 256, lds, s>>>(m, k0, kb, A, G, status);
+    LSS_LAUNCH_CHECK();
+}
+
+void dense_diag_batch(int32_t ntask, const DiagTask *tasks, double *W, double *G, int *status, hipStream_t s)
+{
+    if (ntask == 0) return;
+    static bool attr = false;
+    if (!attr) {
+        LSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_dense_diag_batch),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(dense_diag_lds(kDenseNB))));
+        attr = true;
+    }
+    k_dense_diag_batch<<<ntask, 256, dense_diag_lds(kDenseNB), s>>>(tasks, W, G, status);
     LSS_LAUNCH_CHECK();
 }
 
