@@ -400,6 +400,7 @@ void SparseFactor::release()
     for (DBuf<double> *b : {&G_, &L_, &Lt_, &tp_, &ypos_, &xpos_, &rhs_, &wbuf_}) b->release();
     for (DBuf<int2> *b : {&t_rhs_, &t_y_, &t_w_, &t_x_}) b->release();
     lv_rhs_.clear(), lv_y_.clear(), lv_w_.clear(), lv_x_.clear();
+    lpr_y_.clear(), lpr_w_.clear(), lpr_s_.clear(), lpr_x_.clear();
 }
 
 k::FrontView SparseFactor::view() const
@@ -893,15 +894,32 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
     // 9. solve tables: task lists per tree level
     std::vector<int2> trhs, ty, tw, tx;
     for (auto *lv : {&lv_rhs_, &lv_y_, &lv_w_, &lv_x_}) lv->assign(H + 2, 0);
+    // lanes per row of a level's products: about sixteen entries per lane (4 .. 64 lanes), so
+    // short rows share a wave and its chain of table loads
+    auto lanes = [](int64_t len, int64_t rows) {
+        const int64_t per = rows ? (len / rows + 15) / 16 : 1;
+        int l = 4;
+        while (l < 64 && l < per) l *= 2;
+        return l;
+    };
+    for (auto *lp : {&lpr_y_, &lpr_w_, &lpr_s_, &lpr_x_}) lp->assign(H + 1, 64);
     for (int32_t h = 0; h <= H; ++h) {
         lv_rhs_[h] = int64_t(trhs.size()), lv_y_[h] = int64_t(ty.size()), lv_w_[h] = int64_t(tw.size());
         lv_x_[h] = int64_t(tx.size());
+        int64_t ly = 0, lw = 0, ls = 0;  // summed row lengths: G11 halves, L21 rows, L21^T rows
         for (int32_t f : bylev[h]) {
             const int32_t p = T.np[f], u = int32_t(T.nu(f));
             for (int32_t a = 0; a < p + u; ++a) trhs.push_back(make_int2(f, a));
             for (int32_t i = 0; i < p; ++i) ty.push_back(make_int2(f, i)), tx.push_back(make_int2(f, i));
             for (int32_t a = 0; a < u; ++a) tw.push_back(make_int2(f, a));
+            ly += int64_t(p) * (p + 1) / 2;
+            lw += int64_t(u) * p;
+            ls += int64_t(p) * u;
         }
+        const int64_t ny = int64_t(ty.size()) - lv_y_[h], nw = int64_t(tw.size()) - lv_w_[h];
+        lpr_y_[h] = lpr_x_[h] = lanes(ly, ny);
+        lpr_w_[h] = lanes(lw, nw);
+        lpr_s_[h] = lanes(ls, ny);
     }
     lv_rhs_[H + 1] = int64_t(trhs.size()), lv_y_[H + 1] = int64_t(ty.size()), lv_w_[H + 1] = int64_t(tw.size());
     lv_x_[H + 1] = int64_t(tx.size());
@@ -941,15 +959,18 @@ void SparseFactor::solve_add(const double *t, const int32_t *pdofs, double *z, f
         k::sparse_fwd_rhs(lv_rhs_[h + 1] - lv_rhs_[h], t_rhs_.get() + lv_rhs_[h], fv, tp_.get(), m_, wbuf_.get(),
                           wtot_, rhs_.get(), rtot_, k, stop, s);
         k::sparse_fwd_y(lv_y_[h + 1] - lv_y_[h], t_y_.get() + lv_y_[h], fv, G_.get(), rhs_.get(), rtot_,
-                        ypos_.get(), m_, k, stop, s);
+                        ypos_.get(), m_, k, stop, lpr_y_[h], s);
         k::sparse_fwd_w(lv_w_[h + 1] - lv_w_[h], t_w_.get() + lv_w_[h], fv, L_.get(), rhs_.get(), rtot_,
-                        ypos_.get(), m_, wbuf_.get(), wtot_, k, stop, s);
+                        ypos_.get(), m_, wbuf_.get(), wtot_, k, stop, lpr_w_[h], s);
     }
     for (int h = NL - 1; h >= 0; --h) {
-        k::sparse_bwd_s(lv_y_[h + 1] - lv_y_[h], t_y_.get() + lv_y_[h], fv, Lt_.get(), xpos_.get(), ypos_.get(), m_,
-                        rhs_.get(), rtot_, k, stop, s);
+        // x at the level's update positions into the update buffer (free after the forward sweep)
+        k::sparse_bwd_gather(lv_w_[h + 1] - lv_w_[h], t_w_.get() + lv_w_[h], fv, xpos_.get(), m_, wbuf_.get(), wtot_,
+                             k, stop, s);
+        k::sparse_bwd_s(lv_y_[h + 1] - lv_y_[h], t_y_.get() + lv_y_[h], fv, Lt_.get(), wbuf_.get(), wtot_,
+                        ypos_.get(), m_, rhs_.get(), rtot_, k, stop, lpr_s_[h], s);
         k::sparse_bwd_x(lv_x_[h + 1] - lv_x_[h], t_x_.get() + lv_x_[h], fv, G_.get(), rhs_.get(), rtot_,
-                        xpos_.get(), m_, k, stop, s);
+                        xpos_.get(), m_, k, stop, lpr_x_[h], s);
     }
     k::sparse_scatter_add(m_, order_.get(), pdofs, xpos_.get(), z, z32, ld, k, stop, s);
 }

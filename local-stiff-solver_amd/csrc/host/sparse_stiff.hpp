@@ -71,6 +71,7 @@ private:
     // level task lists: tasks of level h at [lv_x[h], lv_x[h + 1])
     DBuf<int2> t_rhs_, t_y_, t_w_, t_x_;
     std::vector<int64_t> lv_rhs_, lv_y_, lv_w_, lv_x_;
+    std::vector<int> lpr_y_, lpr_w_, lpr_s_, lpr_x_;  // lanes per row of each level's products
     DBuf<double> tp_, ypos_, xpos_, rhs_, wbuf_;
     k::FrontView view() const;
 };

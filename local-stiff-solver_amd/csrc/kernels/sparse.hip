@@ -25,6 +25,38 @@ namespace {
 
 constexpr int kT = 256;
 
+// Rows of the level steps go to groups of LPR lanes (4 .. 64, a power of two chosen per
+// level from the mean row length), summed by a fixed butterfly within the group.
+// sum_j a[j] v(j) over j = b + lane, b + lane + LPR, ... < e, in that order; each lane
+// issues its loads in batches of U (predicated) before the products, so a row of up to
+// U * LPR entries costs one memory round trip
+template <int LPR, typename V>
+__device__ __forceinline__ double group_dot(const double *__restrict__ a, int b, int e, int lane, V v)
+{
+    constexpr int U = 16;
+    double acc = 0.0;
+    for (int j0 = b + lane; j0 < e; j0 += LPR * U) {
+        double av[U], bv[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int j = j0 + q * LPR;
+            av[q] = j < e ? a[j] : 0.0;
+            bv[q] = j < e ? v(j) : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) acc += av[q] * bv[q];
+    }
+    return acc;
+}
+
+template <int LPR>
+__device__ __forceinline__ double group_sum(double v)
+{
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 __global__ __launch_bounds__(kT) void k_gather_pattern(int32_t m, const int32_t *__restrict__ dofs,
                                                        const int32_t *__restrict__ ptr,
                                                        const int64_t *__restrict__ coff,
@@ -91,64 +123,78 @@ __global__ __launch_bounds__(kT) void k_fwd_rhs(int64_t ntask, const int2 *__res
 }
 
 // wave per pivot row i: y_i = sum_{j <= i} G11(i, j) rhs_j
+template <int LPR>
 __global__ __launch_bounds__(kT) void k_fwd_y(int64_t ntask, const int2 *__restrict__ tasks, FrontView fv,
                                               const double *__restrict__ G, const double *__restrict__ rhs,
                                               int64_t rtot, double *__restrict__ ypos, int32_t m,
                                               const int *__restrict__ stop)
 {
     if (dev::all_stopped(stop, gridDim.y)) return;
-    const int64_t t = int64_t(blockIdx.x) * (kT / 64) + (threadIdx.x >> 6);
-    if (t >= ntask) return;
-    const int lane = threadIdx.x & 63, c = blockIdx.y;
+    const int64_t t = (int64_t(blockIdx.x) * kT + threadIdx.x) / LPR;
+    if (t >= ntask) return;  // a whole group
+    const int lane = threadIdx.x & (LPR - 1), c = blockIdx.y;
     const int2 task = tasks[t];
     const int f = task.x, i = task.y, p = fv.np[f];
     const double *row = G + fv.goff[f] + int64_t(i) * p;
     const double *v = rhs + int64_t(c) * rtot + fv.roff[f];
-    double acc = 0.0;
-    for (int j = lane; j <= i; j += 64) acc += row[j] * v[j];
-    acc = dev::wave_sum(acc);
+    double acc = group_dot<LPR>(row, 0, i + 1, lane, [&](int j) { return v[j]; });
+    acc = group_sum<LPR>(acc);
     if (lane == 0) ypos[int64_t(c) * m + fv.p0[f] + i] = acc;
 }
 
 // wave per update row a: w_a = rhs_{p + a} - sum_j L21(a, j) y_j
+template <int LPR>
 __global__ __launch_bounds__(kT) void k_fwd_w(int64_t ntask, const int2 *__restrict__ tasks, FrontView fv,
                                               const double *__restrict__ L, const double *__restrict__ rhs,
                                               int64_t rtot, const double *__restrict__ ypos, int32_t m,
                                               double *__restrict__ w, int64_t wtot, const int *__restrict__ stop)
 {
     if (dev::all_stopped(stop, gridDim.y)) return;
-    const int64_t t = int64_t(blockIdx.x) * (kT / 64) + (threadIdx.x >> 6);
-    if (t >= ntask) return;
-    const int lane = threadIdx.x & 63, c = blockIdx.y;
+    const int64_t t = (int64_t(blockIdx.x) * kT + threadIdx.x) / LPR;
+    if (t >= ntask) return;  // a whole group
+    const int lane = threadIdx.x & (LPR - 1), c = blockIdx.y;
     const int2 task = tasks[t];
     const int f = task.x, a = task.y, p = fv.np[f];
     const double *row = L + fv.loff[f] + int64_t(a) * p;
     const double *y = ypos + int64_t(c) * m + fv.p0[f];
-    double acc = 0.0;
-    for (int j = lane; j < p; j += 64) acc += row[j] * y[j];
-    acc = dev::wave_sum(acc);
+    double acc = group_dot<LPR>(row, 0, p, lane, [&](int j) { return y[j]; });
+    acc = group_sum<LPR>(acc);
     if (lane == 0)
         w[int64_t(c) * wtot + fv.woff[f] + a] = rhs[int64_t(c) * rtot + fv.roff[f] + p + a] - acc;
 }
 
-// wave per pivot column j: s_j = y_j - sum_a L21(a, j) x[U_f(a)], from L21^T (p x u, row-major)
+// xu[woff[f] + a] = x[U_f(a)]: the update positions' solution entries of a level's fronts,
+// gathered once (thread per update row) so that the products below stream contiguous data
+__global__ __launch_bounds__(kT) void k_bwd_gather(int64_t ntask, const int2 *__restrict__ tasks, FrontView fv,
+                                                   const double *__restrict__ xpos, int32_t m,
+                                                   double *__restrict__ xu, int64_t wtot, const int *__restrict__ stop)
+{
+    if (dev::all_stopped(stop, gridDim.y)) return;
+    const int64_t t = int64_t(blockIdx.x) * kT + threadIdx.x;
+    if (t >= ntask) return;
+    const int c = blockIdx.y;
+    const int2 task = tasks[t];
+    const int64_t q = fv.woff[task.x] + task.y;
+    xu[int64_t(c) * wtot + q] = xpos[int64_t(c) * m + fv.uidx[q]];
+}
+
+// row j of L21^T (p x u, row-major) per group: s_j = y_j - sum_a L21(a, j) x[U_f(a)]
+template <int LPR>
 __global__ __launch_bounds__(kT) void k_bwd_s(int64_t ntask, const int2 *__restrict__ tasks, FrontView fv,
-                                              const double *__restrict__ Lt, const double *__restrict__ xpos,
-                                              const double *__restrict__ ypos, int32_t m,
+                                              const double *__restrict__ Lt, const double *__restrict__ xu,
+                                              int64_t wtot, const double *__restrict__ ypos, int32_t m,
                                               double *__restrict__ sbuf, int64_t rtot, const int *__restrict__ stop)
 {
     if (dev::all_stopped(stop, gridDim.y)) return;
-    const int64_t t = int64_t(blockIdx.x) * (kT / 64) + (threadIdx.x >> 6);
-    if (t >= ntask) return;
-    const int lane = threadIdx.x & 63, c = blockIdx.y;
+    const int64_t t = (int64_t(blockIdx.x) * kT + threadIdx.x) / LPR;
+    if (t >= ntask) return;  // a whole group
+    const int lane = threadIdx.x & (LPR - 1), c = blockIdx.y;
     const int2 task = tasks[t];
     const int f = task.x, j = task.y, u = fv.nu[f];
     const double *row = Lt + fv.loff[f] + int64_t(j) * u;
-    const int32_t *ui = fv.uidx + fv.woff[f];
-    const double *x = xpos + int64_t(c) * m;
-    double acc = 0.0;
-    for (int a = lane; a < u; a += 64) acc += row[a] * x[ui[a]];
-    acc = dev::wave_sum(acc);
+    const double *x = xu + int64_t(c) * wtot + fv.woff[f];
+    double acc = group_dot<LPR>(row, 0, u, lane, [&](int a) { return x[a]; });
+    acc = group_sum<LPR>(acc);
     if (lane == 0) sbuf[int64_t(c) * rtot + fv.roff[f] + j] = ypos[int64_t(c) * m + fv.p0[f] + j] - acc;
 }
 
@@ -167,22 +213,22 @@ __global__ __launch_bounds__(kT) void k_transpose(int64_t r, int64_t c, const do
 }
 
 // wave per pivot row i: x_i = sum_{j >= i} G11(j, i) s_j (G11^T stored in the strict upper triangle)
+template <int LPR>
 __global__ __launch_bounds__(kT) void k_bwd_x(int64_t ntask, const int2 *__restrict__ tasks, FrontView fv,
                                               const double *__restrict__ G, const double *__restrict__ sbuf,
                                               int64_t rtot, double *__restrict__ xpos, int32_t m,
                                               const int *__restrict__ stop)
 {
     if (dev::all_stopped(stop, gridDim.y)) return;
-    const int64_t t = int64_t(blockIdx.x) * (kT / 64) + (threadIdx.x >> 6);
-    if (t >= ntask) return;
-    const int lane = threadIdx.x & 63, c = blockIdx.y;
+    const int64_t t = (int64_t(blockIdx.x) * kT + threadIdx.x) / LPR;
+    if (t >= ntask) return;  // a whole group
+    const int lane = threadIdx.x & (LPR - 1), c = blockIdx.y;
     const int2 task = tasks[t];
     const int f = task.x, i = task.y, p = fv.np[f];
     const double *row = G + fv.goff[f] + int64_t(i) * p;
     const double *v = sbuf + int64_t(c) * rtot + fv.roff[f];
-    double acc = 0.0;
-    for (int j = i + lane; j < p; j += 64) acc += row[j] * v[j];
-    acc = dev::wave_sum(acc);
+    double acc = group_dot<LPR>(row, i, p, lane, [&](int j) { return v[j]; });
+    acc = group_sum<LPR>(acc);
     if (lane == 0) xpos[int64_t(c) * m + fv.p0[f] + i] = acc;
 }
 
@@ -407,27 +453,60 @@ void sparse_fwd_rhs(int64_t ntask, const int2 *tasks, const FrontView &fv, const
 }
 
 void sparse_fwd_y(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *G, const double *rhs,
-                  int64_t rtot, double *ypos, int32_t m, int k, const int *stop, hipStream_t s)
+                  int64_t rtot, double *ypos, int32_t m, int k, const int *stop, int lpr, hipStream_t s)
 {
     if (ntask == 0) return;
-    k_fwd_y<<<dim3(waves_grid(ntask), k), kT, 0, s>>>(ntask, tasks, fv, G, rhs, rtot, ypos, m, stop);
+    const dim3 grid(div_up(ntask * lpr, kT), k);
+    switch (lpr) {
+        case 4: k_fwd_y<4><<<grid, kT, 0, s>>>(ntask, tasks, fv, G, rhs, rtot, ypos, m, stop); break;
+        case 8: k_fwd_y<8><<<grid, kT, 0, s>>>(ntask, tasks, fv, G, rhs, rtot, ypos, m, stop); break;
+        case 16: k_fwd_y<16><<<grid, kT, 0, s>>>(ntask, tasks, fv, G, rhs, rtot, ypos, m, stop); break;
+        case 32: k_fwd_y<32><<<grid, kT, 0, s>>>(ntask, tasks, fv, G, rhs, rtot, ypos, m, stop); break;
+        case 64: k_fwd_y<64><<<grid, kT, 0, s>>>(ntask, tasks, fv, G, rhs, rtot, ypos, m, stop); break;
+        default: fail(LSS_ERR_ARG, "sparse solve: lanes per row must be 4 .. 64");
+    }
     LSS_LAUNCH_CHECK();
 }
 
 void sparse_fwd_w(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *L, const double *rhs,
                   int64_t rtot, const double *ypos, int32_t m, double *w, int64_t wtot, int k, const int *stop,
-                  hipStream_t s)
+                  int lpr, hipStream_t s)
 {
     if (ntask == 0) return;
-    k_fwd_w<<<dim3(waves_grid(ntask), k), kT, 0, s>>>(ntask, tasks, fv, L, rhs, rtot, ypos, m, w, wtot, stop);
+    const dim3 grid(div_up(ntask * lpr, kT), k);
+    switch (lpr) {
+        case 4: k_fwd_w<4><<<grid, kT, 0, s>>>(ntask, tasks, fv, L, rhs, rtot, ypos, m, w, wtot, stop); break;
+        case 8: k_fwd_w<8><<<grid, kT, 0, s>>>(ntask, tasks, fv, L, rhs, rtot, ypos, m, w, wtot, stop); break;
+        case 16: k_fwd_w<16><<<grid, kT, 0, s>>>(ntask, tasks, fv, L, rhs, rtot, ypos, m, w, wtot, stop); break;
+        case 32: k_fwd_w<32><<<grid, kT, 0, s>>>(ntask, tasks, fv, L, rhs, rtot, ypos, m, w, wtot, stop); break;
+        case 64: k_fwd_w<64><<<grid, kT, 0, s>>>(ntask, tasks, fv, L, rhs, rtot, ypos, m, w, wtot, stop); break;
+        default: fail(LSS_ERR_ARG, "sparse solve: lanes per row must be 4 .. 64");
+    }
     LSS_LAUNCH_CHECK();
 }
 
-void sparse_bwd_s(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *Lt, const double *xpos,
-                  const double *ypos, int32_t m, double *sbuf, int64_t rtot, int k, const int *stop, hipStream_t s)
+void sparse_bwd_s(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *Lt, const double *xu,
+                  int64_t wtot, const double *ypos, int32_t m, double *sbuf, int64_t rtot, int k, const int *stop,
+                  int lpr, hipStream_t s)
 {
     if (ntask == 0) return;
-    k_bwd_s<<<dim3(waves_grid(ntask), k), kT, 0, s>>>(ntask, tasks, fv, Lt, xpos, ypos, m, sbuf, rtot, stop);
+    const dim3 grid(div_up(ntask * lpr, kT), k);
+    switch (lpr) {
+        case 4: k_bwd_s<4><<<grid, kT, 0, s>>>(ntask, tasks, fv, Lt, xu, wtot, ypos, m, sbuf, rtot, stop); break;
+        case 8: k_bwd_s<8><<<grid, kT, 0, s>>>(ntask, tasks, fv, Lt, xu, wtot, ypos, m, sbuf, rtot, stop); break;
+        case 16: k_bwd_s<16><<<grid, kT, 0, s>>>(ntask, tasks, fv, Lt, xu, wtot, ypos, m, sbuf, rtot, stop); break;
+        case 32: k_bwd_s<32><<<grid, kT, 0, s>>>(ntask, tasks, fv, Lt, xu, wtot, ypos, m, sbuf, rtot, stop); break;
+        case 64: k_bwd_s<64><<<grid, kT, 0, s>>>(ntask, tasks, fv, Lt, xu, wtot, ypos, m, sbuf, rtot, stop); break;
+        default: fail(LSS_ERR_ARG, "sparse solve: lanes per row must be 4 .. 64");
+    }
+    LSS_LAUNCH_CHECK();
+}
+
+void sparse_bwd_gather(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *xpos, int32_t m,
+                       double *xu, int64_t wtot, int k, const int *stop, hipStream_t s)
+{
+    if (ntask == 0) return;
+    k_bwd_gather<<<dim3(div_up(ntask, kT), k), kT, 0, s>>>(ntask, tasks, fv, xpos, m, xu, wtot, stop);
     LSS_LAUNCH_CHECK();
 }
 
@@ -439,10 +518,18 @@ void sparse_transpose(int64_t r, int64_t c, const double *src, double *dst, hipS
 }
 
 void sparse_bwd_x(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *G, const double *sbuf,
-                  int64_t rtot, double *xpos, int32_t m, int k, const int *stop, hipStream_t s)
+                  int64_t rtot, double *xpos, int32_t m, int k, const int *stop, int lpr, hipStream_t s)
 {
     if (ntask == 0) return;
-    k_bwd_x<<<dim3(waves_grid(ntask), k), kT, 0, s>>>(ntask, tasks, fv, G, sbuf, rtot, xpos, m, stop);
+    const dim3 grid(div_up(ntask * lpr, kT), k);
+    switch (lpr) {
+        case 4: k_bwd_x<4><<<grid, kT, 0, s>>>(ntask, tasks, fv, G, sbuf, rtot, xpos, m, stop); break;
+        case 8: k_bwd_x<8><<<grid, kT, 0, s>>>(ntask, tasks, fv, G, sbuf, rtot, xpos, m, stop); break;
+        case 16: k_bwd_x<16><<<grid, kT, 0, s>>>(ntask, tasks, fv, G, sbuf, rtot, xpos, m, stop); break;
+        case 32: k_bwd_x<32><<<grid, kT, 0, s>>>(ntask, tasks, fv, G, sbuf, rtot, xpos, m, stop); break;
+        case 64: k_bwd_x<64><<<grid, kT, 0, s>>>(ntask, tasks, fv, G, sbuf, rtot, xpos, m, stop); break;
+        default: fail(LSS_ERR_ARG, "sparse solve: lanes per row must be 4 .. 64");
+    }
     LSS_LAUNCH_CHECK();
 }
 

@@ -70,23 +70,28 @@ void sparse_extend_add(int32_t u, const int32_t *rel, const double *U, double *F
 
 // Solve (L L^T) x = t of one unit for k columns.  Vectors in elimination (position)
 // order: tp, ypos, xpos (stride m); rhs buffer (stride rtot), update buffer (stride wtot).
-// Level steps take task lists of (front, row) pairs (rows < m, < p, < u).
+// Level steps take task lists of (front, row) pairs (rows < m, < p, < u); the products
+// give each row a group of lpr lanes (4, 8, 16, 32 or 64).
 void sparse_permute_in(int32_t m, const int32_t *order, const double *t, int64_t ldt, double *tp, int k,
                        const int *stop, hipStream_t s);
 void sparse_fwd_rhs(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *tp, int32_t m,
                     const double *w, int64_t wtot, double *rhs, int64_t rtot, int k, const int *stop, hipStream_t s);
 void sparse_fwd_y(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *G, const double *rhs,
-                  int64_t rtot, double *ypos, int32_t m, int k, const int *stop, hipStream_t s);
+                  int64_t rtot, double *ypos, int32_t m, int k, const int *stop, int lpr, hipStream_t s);
 void sparse_fwd_w(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *L, const double *rhs,
                   int64_t rtot, const double *ypos, int32_t m, double *w, int64_t wtot, int k, const int *stop,
-                  hipStream_t s);
+                  int lpr, hipStream_t s);
+// xu (stride wtot) = x at the update positions of the tasks' fronts (tasks: (front, update row))
+void sparse_bwd_gather(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *xpos, int32_t m,
+                       double *xu, int64_t wtot, int k, const int *stop, hipStream_t s);
 // Lt: L21^T of every front (p x u, row-major, at loff); tasks: (front, pivot row) pairs
-void sparse_bwd_s(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *Lt, const double *xpos,
-                  const double *ypos, int32_t m, double *sbuf, int64_t rtot, int k, const int *stop, hipStream_t s);
+void sparse_bwd_s(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *Lt, const double *xu,
+                  int64_t wtot, const double *ypos, int32_t m, double *sbuf, int64_t rtot, int k, const int *stop,
+                  int lpr, hipStream_t s);
 // dst (c x r) = src (r x c)^T, both row-major
 void sparse_transpose(int64_t r, int64_t c, const double *src, double *dst, hipStream_t s);
 void sparse_bwd_x(int64_t ntask, const int2 *tasks, const FrontView &fv, const double *G, const double *sbuf,
-                  int64_t rtot, double *xpos, int32_t m, int k, const int *stop, hipStream_t s);
+                  int64_t rtot, double *xpos, int32_t m, int k, const int *stop, int lpr, hipStream_t s);
 // z[pdofs[order[q]]] += xpos[q] (and z32 = float of the new z when given), k columns at stride ld.
 void sparse_scatter_add(int32_t m, const int32_t *order, const int32_t *pdofs, const double *xpos, double *z,
                         float *z32, int64_t ld, int k, const int *stop, hipStream_t s);
