@@ -27,10 +27,13 @@
 #include "sparse_stiff.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <chrono>
 #include <map>
+#include <memory>
 #include <numeric>
+#include <parallel/algorithm>
 #include <string>
 
 #include "../kernels/kernels.hpp"
@@ -113,51 +116,66 @@ struct GroupPlan {
     Range cl, clt, cu, cm;                     // L21, L21^T, update matrix, G11 mirror
 };
 
+// A dissected subgraph: its fronts in postorder (quotient vertices; children as indices
+// into the same list) and the indices of its root fronts.
+struct Subtree {
+    std::vector<std::vector<int32_t>> verts, kids;
+    std::vector<int32_t> roots;
+    int32_t add(std::vector<int32_t> v, std::vector<int32_t> k)
+    {
+        verts.push_back(std::move(v));
+        kids.push_back(std::move(k));
+        return int32_t(verts.size()) - 1;
+    }
+    void append(Subtree &&o)  // o's fronts after this one's; o's roots join the roots
+    {
+        const int32_t off = int32_t(verts.size());
+        for (size_t f = 0; f < o.verts.size(); ++f) {
+            for (int32_t &c : o.kids[f]) c += off;
+            verts.push_back(std::move(o.verts[f]));
+            kids.push_back(std::move(o.kids[f]));
+        }
+        for (int32_t r : o.roots) roots.push_back(r + off);
+    }
+};
+
 // Dissects a graph whose vertices carry weights (the sizes of groups of original
-// vertices); emitted fronts list the original vertices of their groups.
+// vertices).  Disjoint subgraphs are dissected as concurrent tasks: each task reads and
+// writes the per-vertex arrays of its own vertices only, except `mark` of neighbours
+// (read to test membership; every subset has a stamp of its own), so the result does
+// not depend on the schedule.
 struct Dissector {
-    const int32_t *ap, *ai;   // quotient graph
-    const int32_t *mptr, *mem;  // members of each quotient vertex (original vertices, ascending)
+    const int32_t *ap, *ai;     // quotient graph
+    const int32_t *mptr;        // group sizes: mptr[v + 1] - mptr[v]
     int64_t leaf;
-    FrontTree &T;
     std::vector<int32_t> mark, lev, vis;
-    int32_t stamp = 0, vstamp = 0, next = 0;
+    std::atomic<int32_t> stamp{0};
 
     int64_t weight(int32_t v) const { return mptr[v + 1] - mptr[v]; }
+    int32_t marked(int32_t w) const { return __atomic_load_n(&mark[w], __ATOMIC_RELAXED); }
+    int32_t fresh() { return stamp.fetch_add(1, std::memory_order_relaxed) + 1; }
 
-    int32_t emit(const std::vector<int32_t> &qverts, const std::vector<int32_t> &kids)
+    static Subtree single(std::vector<int32_t> V)
     {
-        std::vector<int32_t> verts;
-        for (int32_t v : qverts) verts.insert(verts.end(), mem + mptr[v], mem + mptr[v + 1]);
-        std::sort(verts.begin(), verts.end());
-        const int32_t f = T.fronts();
-        T.p0.push_back(next);
-        T.np.push_back(int32_t(verts.size()));
-        T.parent.push_back(-1);
-        T.children.push_back(kids);
-        for (int32_t v : verts) {
-            T.pos[v] = next;
-            T.order[next] = v;
-            ++next;
-        }
-        for (int32_t c : kids) T.parent[c] = f;
-        return f;
+        Subtree t;
+        t.roots.push_back(t.add(std::move(V), {}));
+        return t;
     }
 
     // breadth-first search over the current subset from src: q = visit order, lev = levels
     void bfs(int32_t src, int32_t my, std::vector<int32_t> &q)
     {
-        ++vstamp;
+        const int32_t vs = fresh();
         q.clear();
         q.push_back(src);
-        vis[src] = vstamp;
+        vis[src] = vs;
         lev[src] = 0;
         for (size_t h = 0; h < q.size(); ++h) {
             const int32_t v = q[h];
             for (int32_t e = ap[v]; e < ap[v + 1]; ++e) {
                 const int32_t w = ai[e];
-                if (mark[w] == my && vis[w] != vstamp) {
-                    vis[w] = vstamp;
+                if (marked(w) == my && vis[w] != vs) {
+                    vis[w] = vs;
                     lev[w] = lev[v] + 1;
                     q.push_back(w);
                 }
@@ -165,18 +183,31 @@ struct Dissector {
         }
     }
 
-    // returns the root fronts of the subgraph induced by V
-    std::vector<int32_t> dissect(std::vector<int32_t> V)
+    // left and right, as tasks when the subgraphs are large
+    void both(std::vector<int32_t> &&l, std::vector<int32_t> &&r, Subtree &a, Subtree &b)
+    {
+        if (l.size() + r.size() > 4096) {
+#pragma omp task shared(a) firstprivate(l)
+            a = dissect(std::move(l));
+            b = dissect(std::move(r));
+#pragma omp taskwait
+        } else {
+            a = dissect(std::move(l));
+            b = dissect(std::move(r));
+        }
+    }
+
+    Subtree dissect(std::vector<int32_t> V)
     {
         int64_t nv = 0;
         for (int32_t v : V) nv += weight(v);
-        if (nv <= leaf) return {emit(V, {})};
-        const int32_t my = ++stamp;
-        for (int32_t v : V) mark[v] = my;
+        if (nv <= leaf) return single(std::move(V));
+        const int32_t my = fresh();
+        for (int32_t v : V) __atomic_store_n(&mark[v], my, __ATOMIC_RELAXED);
         std::sort(V.begin(), V.end());
         // connected components, in the order of their smallest vertex
         std::vector<std::vector<int32_t>> comps;
-        const int32_t cs = ++vstamp;
+        const int32_t cs = fresh();
         for (int32_t s0 : V) {
             if (vis[s0] == cs) continue;
             comps.emplace_back();
@@ -187,7 +218,7 @@ struct Dissector {
                 const int32_t v = c[h];
                 for (int32_t e = ap[v]; e < ap[v + 1]; ++e) {
                     const int32_t w = ai[e];
-                    if (mark[w] == my && vis[w] != cs) {
+                    if (marked(w) == my && vis[w] != cs) {
                         vis[w] = cs;
                         c.push_back(w);
                     }
@@ -195,18 +226,25 @@ struct Dissector {
             }
         }
         if (comps.size() > 1) {
-            std::vector<int32_t> roots;
-            for (auto &c : comps) {
-                const std::vector<int32_t> r = dissect(std::move(c));
-                roots.insert(roots.end(), r.begin(), r.end());
+            std::vector<Subtree> parts(comps.size());
+            for (size_t i = 0; i < comps.size(); ++i) {
+                if (comps[i].size() > 4096) {
+#pragma omp task shared(parts, comps) firstprivate(i)
+                    parts[i] = dissect(std::move(comps[i]));
+                } else {
+                    parts[i] = dissect(std::move(comps[i]));
+                }
             }
-            return roots;
+#pragma omp taskwait
+            Subtree t;
+            for (auto &p : parts) t.append(std::move(p));
+            return t;
         }
         std::vector<int32_t> q;
         bfs(V.front(), my, q);
         bfs(q.back(), my, q);
         const int32_t nlev = lev[q.back()] + 1;
-        if (nlev < 3) return {emit(V, {})};
+        if (nlev < 3) return single(std::move(V));
         std::vector<int64_t> cnt(nlev, 0);
         for (int32_t v : q) cnt[lev[v]] += weight(v);
         // the smallest level whose predecessors hold 35-65 % of the weight (the one
@@ -236,15 +274,17 @@ struct Dissector {
                 bool cut = false;
                 for (int32_t e = ap[v]; e < ap[v + 1] && !cut; ++e) {
                     const int32_t w = ai[e];
-                    cut = mark[w] == my && lev[w] == k + 1;
+                    cut = marked(w) == my && lev[w] == k + 1;
                 }
                 (cut ? sep : left).push_back(v);
             }
         }
-        std::vector<int32_t> kids = dissect(std::move(left));
-        const std::vector<int32_t> r = dissect(std::move(right));
-        kids.insert(kids.end(), r.begin(), r.end());
-        return {emit(sep, kids)};
+        Subtree a, b;
+        both(std::move(left), std::move(right), a, b);
+        a.append(std::move(b));
+        const std::vector<int32_t> kids = a.roots;
+        a.roots.assign(1, a.add(std::move(sep), kids));
+        return a;
     }
 };
 
@@ -262,6 +302,7 @@ int32_t indistinguishable(int32_t m, const int32_t *ap, const int32_t *ai, std::
                           std::vector<int32_t> &mptr, std::vector<int32_t> &mem)
 {
     std::vector<uint64_t> h(m);
+#pragma omp parallel for schedule(static)
     for (int32_t v = 0; v < m; ++v) {
         uint64_t x = mix64(uint64_t(v));
         for (int32_t e = ap[v]; e < ap[v + 1]; ++e) x += mix64(uint64_t(ai[e]));
@@ -269,7 +310,7 @@ int32_t indistinguishable(int32_t m, const int32_t *ap, const int32_t *ai, std::
     }
     std::vector<int32_t> by(m);
     std::iota(by.begin(), by.end(), 0);
-    std::sort(by.begin(), by.end(), [&](int32_t a, int32_t b) {
+    __gnu_parallel::sort(by.begin(), by.end(), [&](int32_t a, int32_t b) {
         const int32_t da = ap[a + 1] - ap[a], db = ap[b + 1] - ap[b];
         return h[a] != h[b] ? h[a] < h[b] : da != db ? da < db : a < b;
     });
@@ -333,61 +374,116 @@ FrontTree nested_dissection(int32_t m, const int32_t *adj_ptr, const int32_t *ad
     T.order.assign(m, 0);
     T.pos.assign(m, 0);
     // dissect the quotient graph of indistinguishable vertices, weighted by group size
-    std::vector<int32_t> grp, mptr, mem, qptr(1, 0), qidx;
+    std::vector<int32_t> grp, mptr, mem, qptr, qidx;
     const int32_t ng = indistinguishable(m, adj_ptr, adj_idx, grp, mptr, mem);
     {
-        std::vector<int32_t> nb;
-        for (int32_t g = 0; g < ng; ++g) {
+        // neighbour groups of each group's first member (all members have the same ones)
+        auto nbs = [&](int32_t g, std::vector<int32_t> &nb) {
             const int32_t r = mem[mptr[g]];
             nb.clear();
             for (int32_t e = adj_ptr[r]; e < adj_ptr[r + 1]; ++e)
                 if (grp[adj_idx[e]] != g) nb.push_back(grp[adj_idx[e]]);
             std::sort(nb.begin(), nb.end());
             nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
-            qidx.insert(qidx.end(), nb.begin(), nb.end());
-            qptr.push_back(int32_t(qidx.size()));
+        };
+        qptr.assign(size_t(ng) + 1, 0);
+#pragma omp parallel
+        {
+            std::vector<int32_t> nb;
+#pragma omp for schedule(dynamic, 1024)
+            for (int32_t g = 0; g < ng; ++g) {
+                nbs(g, nb);
+                qptr[g + 1] = int32_t(nb.size());
+            }
+        }
+        for (int32_t g = 0; g < ng; ++g) qptr[g + 1] += qptr[g];
+        qidx.resize(qptr[ng]);
+#pragma omp parallel
+        {
+            std::vector<int32_t> nb;
+#pragma omp for schedule(dynamic, 1024)
+            for (int32_t g = 0; g < ng; ++g) {
+                nbs(g, nb);
+                std::copy(nb.begin(), nb.end(), qidx.begin() + qptr[g]);
+            }
         }
     }
-    Dissector D{qptr.data(), qidx.data(), mptr.data(), mem.data(), std::max(leaf, 1), T, {}, {}, {}};
+    Dissector D{qptr.data(), qidx.data(), mptr.data(), std::max<int64_t>(leaf, 1), {}, {}, {}};
     D.mark.assign(ng, 0);
     D.lev.assign(ng, 0);
     D.vis.assign(ng, 0);
+    Subtree top;
     if (ng > 0) {
         std::vector<int32_t> all(ng);
         std::iota(all.begin(), all.end(), 0);
-        D.dissect(std::move(all));
+#pragma omp parallel
+#pragma omp single
+        top = D.dissect(std::move(all));
     }
+    // number the fronts' original vertices in postorder, ascending within a front
+    {
+        int32_t next = 0;
+        std::vector<int32_t> verts;
+        for (size_t f = 0; f < top.verts.size(); ++f) {
+            verts.clear();
+            for (int32_t v : top.verts[f]) verts.insert(verts.end(), mem.begin() + mptr[v], mem.begin() + mptr[v + 1]);
+            std::sort(verts.begin(), verts.end());
+            T.p0.push_back(next);
+            T.np.push_back(int32_t(verts.size()));
+            T.parent.push_back(-1);
+            T.children.push_back(top.kids[f]);
+            for (int32_t v : verts) {
+                T.pos[v] = next;
+                T.order[next] = v;
+                ++next;
+            }
+            for (int32_t c : top.kids[f]) T.parent[c] = int32_t(f);
+        }
+    }
+    // update positions: a front's pivots' neighbours and its children's update positions
+    // beyond its pivots; fronts of one height are independent
     const int32_t F = T.fronts();
-    T.uptr.assign(1, 0);
     T.height.assign(F, 0);
-    std::vector<int32_t> mk(m, -1), U;
+    int32_t H = 0;
     for (int32_t f = 0; f < F; ++f) {
-        const int32_t p1 = T.p0[f] + T.np[f];
-        U.clear();
-        for (int32_t q = T.p0[f]; q < p1; ++q) {
-            const int32_t v = T.order[q];
-            for (int32_t e = adj_ptr[v]; e < adj_ptr[v + 1]; ++e) {
-                const int32_t qw = T.pos[adj_idx[e]];
-                if (qw >= p1 && mk[qw] != f) {
-                    mk[qw] = f;
-                    U.push_back(qw);
-                }
-            }
-        }
-        for (int32_t c : T.children[f]) {
-            for (int64_t t = T.uptr[c]; t < T.uptr[c + 1]; ++t) {
-                const int32_t qq = T.uidx[t];
-                if (qq >= p1 && mk[qq] != f) {
-                    mk[qq] = f;
-                    U.push_back(qq);
-                }
-            }
-            T.height[f] = std::max(T.height[f], T.height[c] + 1);
-        }
-        std::sort(U.begin(), U.end());
-        T.uidx.insert(T.uidx.end(), U.begin(), U.end());
-        T.uptr.push_back(int64_t(T.uidx.size()));
+        for (int32_t c : T.children[f]) T.height[f] = std::max(T.height[f], T.height[c] + 1);
+        H = std::max(H, T.height[f]);
     }
+    std::vector<std::vector<int32_t>> bylev(size_t(H) + 1), U(F);
+    for (int32_t f = 0; f < F; ++f) bylev[T.height[f]].push_back(f);
+    for (int32_t h = 0; h <= H; ++h) {
+#pragma omp parallel
+        {
+            std::vector<int32_t> mk;
+#pragma omp for schedule(dynamic, 4)
+            for (size_t i = 0; i < bylev[h].size(); ++i) {
+                const int32_t f = bylev[h][i], p1 = T.p0[f] + T.np[f];
+                if (mk.empty()) mk.assign(m, -1);
+                std::vector<int32_t> &u = U[f];
+                for (int32_t q = T.p0[f]; q < p1; ++q) {
+                    const int32_t v = T.order[q];
+                    for (int32_t e = adj_ptr[v]; e < adj_ptr[v + 1]; ++e) {
+                        const int32_t qw = T.pos[adj_idx[e]];
+                        if (qw >= p1 && mk[qw] != f) {
+                            mk[qw] = f;
+                            u.push_back(qw);
+                        }
+                    }
+                }
+                for (int32_t c : T.children[f])
+                    for (int32_t qq : U[c])
+                        if (qq >= p1 && mk[qq] != f) {
+                            mk[qq] = f;
+                            u.push_back(qq);
+                        }
+                std::sort(u.begin(), u.end());
+            }
+        }
+    }
+    T.uptr.assign(size_t(F) + 1, 0);
+    for (int32_t f = 0; f < F; ++f) T.uptr[f + 1] = T.uptr[f] + int64_t(U[f].size());
+    T.uidx.resize(T.uptr[F]);
+    for (int32_t f = 0; f < F; ++f) std::copy(U[f].begin(), U[f].end(), T.uidx.begin() + T.uptr[f]);
     return T;
 }
 
@@ -441,25 +537,40 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
     std::vector<int64_t> eptr(m + 1, 0);
     std::vector<int32_t> erow;
     std::vector<int64_t> esrc;
-    erow.reserve(coff[m]);
-    esrc.reserve(coff[m]);
     {
-        std::vector<std::pair<int32_t, int64_t>> tmp;
+        // per column (in parallel) into the column's own input range, then compacted
+        // uninitialized: the parallel loop touches its own pages first
+        std::unique_ptr<int32_t[]> crow(new int32_t[std::max<int64_t>(coff[m], 1)]);
+        std::unique_ptr<int64_t[]> csrc(new int64_t[std::max<int64_t>(coff[m], 1)]);
+#pragma omp parallel
+        {
+            std::vector<std::pair<int32_t, int64_t>> tmp;
+#pragma omp for schedule(dynamic, 256)
+            for (int32_t v = 0; v < m; ++v) {
+                tmp.clear();
+                const int64_t base = h_ptr[dofs[v]];
+                for (int64_t t = coff[v]; t < coff[v + 1]; ++t) {
+                    const int32_t w = g2l[rows[t]];
+                    if (w >= 0) tmp.emplace_back(w, base + (t - coff[v]));
+                }
+                std::stable_sort(tmp.begin(), tmp.end(),
+                                 [](const auto &a, const auto &b) { return a.first < b.first; });
+                int64_t k = coff[v];
+                for (size_t i = 0; i < tmp.size(); ++i) {
+                    if (i + 1 < tmp.size() && tmp[i + 1].first == tmp[i].first) continue;
+                    crow[k] = tmp[i].first;
+                    csrc[k++] = tmp[i].second;
+                }
+                eptr[v + 1] = k - coff[v];
+            }
+        }
+        for (int32_t v = 0; v < m; ++v) eptr[v + 1] += eptr[v];
+        erow.resize(std::max<int64_t>(eptr[m], 1));
+        esrc.resize(std::max<int64_t>(eptr[m], 1));
+#pragma omp parallel for schedule(dynamic, 1024)
         for (int32_t v = 0; v < m; ++v) {
-            tmp.clear();
-            const int64_t base = h_ptr[dofs[v]];
-            for (int64_t t = coff[v]; t < coff[v + 1]; ++t) {
-                const int32_t w = g2l[rows[t]];
-                if (w >= 0) tmp.emplace_back(w, base + (t - coff[v]));
-            }
-            std::stable_sort(tmp.begin(), tmp.end(),
-                             [](const auto &a, const auto &b) { return a.first < b.first; });
-            for (size_t i = 0; i < tmp.size(); ++i) {
-                if (i + 1 < tmp.size() && tmp[i + 1].first == tmp[i].first) continue;
-                erow.push_back(tmp[i].first);
-                esrc.push_back(tmp[i].second);
-            }
-            eptr[v + 1] = int64_t(erow.size());
+            std::copy(crow.get() + coff[v], crow.get() + coff[v] + (eptr[v + 1] - eptr[v]), erow.begin() + eptr[v]);
+            std::copy(csrc.get() + coff[v], csrc.get() + coff[v] + (eptr[v + 1] - eptr[v]), esrc.begin() + eptr[v]);
         }
     }
     std::vector<int32_t>().swap(rows);
