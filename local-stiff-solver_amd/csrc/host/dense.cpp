@@ -49,13 +49,30 @@ void gemm_rm(rocblas_handle h, bool trans_b, int64_t r, int64_t c, int64_t k, do
 DenseBlas::~DenseBlas()
 {
     if (handle_) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(handle_));
+    if (handle2_) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(handle2_));
+    if (ev_col_) (void)hipEventDestroy(ev_col_);
+    if (ev_pan_) (void)hipEventDestroy(ev_pan_);
+    if (s2_) (void)hipStreamDestroy(s2_);
+}
+
+void DenseBlas::init()
+{
+    if (handle_) return;
+    rocblas_handle h = nullptr, h2 = nullptr;
+    rb_check(rocblas_create_handle(&h), "create_handle");
+    handle_ = h;
+    rb_check(rocblas_create_handle(&h2), "create_handle");
+    handle2_ = h2;
+    LSS_HIP(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking));
+    LSS_HIP(hipEventCreateWithFlags(&ev_col_, hipEventDisableTiming));
+    LSS_HIP(hipEventCreateWithFlags(&ev_pan_, hipEventDisableTiming));
 }
 
 int64_t DenseBlas::work_size(int64_t m)
 {
     const int64_t nb = k::dense_block_size();
     const int64_t half = m / 2 + nb;
-    return std::max(m * nb, half * half);
+    return std::max(2 * m * nb, half * half);  // two panels in flight; the inverse's products
 }
 
 void DenseBlas::factor_inverse(int64_t m, double *A, double *G, double *work, int *status, hipStream_t s)
@@ -67,32 +84,55 @@ void DenseBlas::factor_inverse(int64_t m, double *A, double *G, double *work, in
 void DenseBlas::partial_factor_inverse(int64_t m, int64_t p, double *A, double *G, double *work, int *status,
                                        hipStream_t s)
 {
-    if (!handle_) {
-        rocblas_handle h = nullptr;
-        rb_check(rocblas_create_handle(&h), "create_handle");
-        handle_ = h;
-    }
-    rocblas_handle h = static_cast<rocblas_handle>(handle_);
+    init();
+    rocblas_handle h = static_cast<rocblas_handle>(handle_), h2 = static_cast<rocblas_handle>(handle2_);
     rb_check(rocblas_set_stream(h, s), "set_stream");
+    rb_check(rocblas_set_stream(h2, s2_), "set_stream");
     const int64_t nb = k::dense_block_size();
     // the GEMMs read whole blocks of G (the diagonal blocks' and G11's upper parts included):
     // those entries must be zero, whatever the buffer held before
     k::fill(p * m, G, 0.0, s);
-    for (int64_t k0 = 0; k0 < p; k0 += nb) {
+    // panels alternate between two workspaces: one is read by the trailing update while the
+    // next is written
+    double *W[2] = {work, work + m * nb};
+    // diagonal block k0 (L_kk, G_kk) and the panel L_ik = A_ik G_kk^T into Wb (r x kb), then into A
+    auto panel = [&](rocblas_handle hh, hipStream_t ss, int64_t k0, double *Wb) {
         const int64_t kb = std::min(nb, p - k0), r = m - k0 - kb;
-        k::dense_diag_block(m, int(k0), int(kb), A, G, status, s);
-        if (r == 0) break;
-        // panel L_ik = A_ik G_kk^T into work (r x kb), then back into A
+        k::dense_diag_block(m, int(k0), int(kb), A, G, status, ss);
+        if (r == 0) return;
         double *Aik = A + (k0 + kb) * m + k0;
-        gemm_rm(h, true, r, kb, kb, 1.0, Aik, m, G + k0 * m + k0, m, 0.0, work, kb);
-        LSS_HIP(hipMemcpy2DAsync(Aik, size_t(m) * sizeof(double), work, size_t(kb) * sizeof(double),
-                                 size_t(kb) * sizeof(double), size_t(r), hipMemcpyDeviceToDevice, s));
-        // trailing lower triangle: A_22 -= L_i L_i^T.  Column-major view: the
-        // upper triangle of A_22^T, and L_i L_i^T = W^T W with W = work (kb x r).
+        gemm_rm(hh, true, r, kb, kb, 1.0, Aik, m, G + k0 * m + k0, m, 0.0, Wb, kb);
+        LSS_HIP(hipMemcpy2DAsync(Aik, size_t(m) * sizeof(double), Wb, size_t(kb) * sizeof(double),
+                                 size_t(kb) * sizeof(double), size_t(r), hipMemcpyDeviceToDevice, ss));
+    };
+    // trailing lower triangle from row/column c0: A(c0:, c0:) -= L L^T, L = rows of Wb from c0 (n x kb).
+    // Column-major view: the upper triangle of A^T, and L L^T = V^T V with V = the rows (kb x n).
+    auto syrk = [&](int64_t c0, int64_t n, int64_t kb, const double *L) {
         const double alpha = -1.0, beta = 1.0;
-        rb_check(rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_transpose, rocblas_int(r), rocblas_int(kb),
-                               &alpha, work, rocblas_int(kb), &beta, A + (k0 + kb) * m + k0 + kb, rocblas_int(m)),
+        rb_check(rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_transpose, rocblas_int(n), rocblas_int(kb),
+                               &alpha, L, rocblas_int(kb), &beta, A + c0 * m + c0, rocblas_int(m)),
                  "dsyrk");
+    };
+    panel(h, s, 0, W[0]);
+    int b = 0;
+    for (int64_t k0 = 0; k0 < p; k0 += nb, b ^= 1) {
+        const int64_t kb = std::min(nb, p - k0), r = m - k0 - kb, k1 = k0 + kb;
+        if (r == 0) break;
+        const double *Wk = W[b];
+        if (k1 >= p) {  // last pivot block: the update rows' Schur complement
+            syrk(k1, r, kb, Wk);
+            break;
+        }
+        // look-ahead: block column k+1 first (rows k1 .. m-1), then its diagonal block and
+        // panel on the second stream while the rest of this step's update runs here
+        const int64_t kb1 = std::min(nb, p - k1), k2 = k1 + kb1;
+        gemm_rm(h, true, r, kb1, kb, -1.0, Wk, kb, Wk, kb, 1.0, A + k1 * m + k1, m);
+        LSS_HIP(hipEventRecord(ev_col_, s));
+        LSS_HIP(hipStreamWaitEvent(s2_, ev_col_, 0));
+        panel(h2, s2_, k1, W[b ^ 1]);
+        LSS_HIP(hipEventRecord(ev_pan_, s2_));
+        if (m > k2) syrk(k2, m - k2, kb, Wk + kb1 * kb);
+        LSS_HIP(hipStreamWaitEvent(s, ev_pan_, 0));
     }
     inverse(0, p, m, A, G, work, s);
 }

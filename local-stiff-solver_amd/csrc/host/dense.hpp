@@ -29,7 +29,11 @@ public:
                                 hipStream_t s);
 
 private:
-    void *handle_ = nullptr;  // rocblas_handle
+    void *handle_ = nullptr;   // rocblas_handle on the caller's stream
+    void *handle2_ = nullptr;  // rocblas_handle on s2_: the next diagonal block and panel (look-ahead)
+    hipStream_t s2_ = nullptr;
+    hipEvent_t ev_col_ = nullptr, ev_pan_ = nullptr;
+    void init();
     void inverse(int64_t a, int64_t b, int64_t m, const double *L, double *G, double *work, hipStream_t s);
 };
 

@@ -118,13 +118,16 @@ struct GroupPlan {
 
 // A dissected subgraph: its fronts in postorder (quotient vertices; children as indices
 // into the same list) and the indices of its root fronts.
+// A front may stand for a subgraph whose dissection is deferred (job >= 0).
 struct Subtree {
     std::vector<std::vector<int32_t>> verts, kids;
+    std::vector<int32_t> job;
     std::vector<int32_t> roots;
-    int32_t add(std::vector<int32_t> v, std::vector<int32_t> k)
+    int32_t add(std::vector<int32_t> v, std::vector<int32_t> k, int32_t j = -1)
     {
         verts.push_back(std::move(v));
         kids.push_back(std::move(k));
+        job.push_back(j);
         return int32_t(verts.size()) - 1;
     }
     void append(Subtree &&o)  // o's fronts after this one's; o's roots join the roots
@@ -134,22 +137,26 @@ struct Subtree {
             for (int32_t &c : o.kids[f]) c += off;
             verts.push_back(std::move(o.verts[f]));
             kids.push_back(std::move(o.kids[f]));
+            job.push_back(o.job[f]);
         }
         for (int32_t r : o.roots) roots.push_back(r + off);
     }
 };
 
 // Dissects a graph whose vertices carry weights (the sizes of groups of original
-// vertices).  Disjoint subgraphs are dissected as concurrent tasks: each task reads and
-// writes the per-vertex arrays of its own vertices only, except `mark` of neighbours
-// (read to test membership; every subset has a stamp of its own), so the result does
-// not depend on the schedule.
+// vertices).  The top kParDepth levels of the recursion run first and leave the
+// subgraphs below them as jobs, which are then dissected in parallel (a dynamic loop)
+// and spliced back in place.  A job reads and writes the per-vertex arrays of its own
+// vertices only, except `mark` of neighbours (read to test membership; every subset has
+// a stamp of its own), so the result equals the sequential recursion.
 struct Dissector {
     const int32_t *ap, *ai;     // quotient graph
     const int32_t *mptr;        // group sizes: mptr[v + 1] - mptr[v]
     int64_t leaf;
     std::vector<int32_t> mark, lev, vis;
     std::atomic<int32_t> stamp{0};
+    static constexpr int kParDepth = 6;
+    std::vector<std::vector<int32_t>> jobs;  // deferred subgraphs (filled by the top levels only)
 
     int64_t weight(int32_t v) const { return mptr[v + 1] - mptr[v]; }
     int32_t marked(int32_t w) const { return __atomic_load_n(&mark[w], __ATOMIC_RELAXED); }
@@ -183,25 +190,19 @@ struct Dissector {
         }
     }
 
-    // left and right, as tasks when the subgraphs are large
-    void both(std::vector<int32_t> &&l, std::vector<int32_t> &&r, Subtree &a, Subtree &b)
-    {
-        if (l.size() + r.size() > 4096) {
-#pragma omp task shared(a) firstprivate(l)
-            a = dissect(std::move(l));
-            b = dissect(std::move(r));
-#pragma omp taskwait
-        } else {
-            a = dissect(std::move(l));
-            b = dissect(std::move(r));
-        }
-    }
-
-    Subtree dissect(std::vector<int32_t> V)
+    // depth >= 0: a top level (subgraphs at kParDepth become jobs); -1: inside a job
+    Subtree dissect(std::vector<int32_t> V, int depth)
     {
         int64_t nv = 0;
         for (int32_t v : V) nv += weight(v);
         if (nv <= leaf) return single(std::move(V));
+        if (depth >= kParDepth) {
+            Subtree t;
+            t.roots.push_back(t.add({}, {}, int32_t(jobs.size())));
+            jobs.push_back(std::move(V));
+            return t;
+        }
+        const int next = depth < 0 ? -1 : depth + 1;
         const int32_t my = fresh();
         for (int32_t v : V) __atomic_store_n(&mark[v], my, __ATOMIC_RELAXED);
         std::sort(V.begin(), V.end());
@@ -226,18 +227,8 @@ struct Dissector {
             }
         }
         if (comps.size() > 1) {
-            std::vector<Subtree> parts(comps.size());
-            for (size_t i = 0; i < comps.size(); ++i) {
-                if (comps[i].size() > 4096) {
-#pragma omp task shared(parts, comps) firstprivate(i)
-                    parts[i] = dissect(std::move(comps[i]));
-                } else {
-                    parts[i] = dissect(std::move(comps[i]));
-                }
-            }
-#pragma omp taskwait
             Subtree t;
-            for (auto &p : parts) t.append(std::move(p));
+            for (auto &c : comps) t.append(dissect(std::move(c), next));
             return t;
         }
         std::vector<int32_t> q;
@@ -279,9 +270,8 @@ struct Dissector {
                 (cut ? sep : left).push_back(v);
             }
         }
-        Subtree a, b;
-        both(std::move(left), std::move(right), a, b);
-        a.append(std::move(b));
+        Subtree a = dissect(std::move(left), next);
+        a.append(dissect(std::move(right), next));
         const std::vector<int32_t> kids = a.roots;
         a.roots.assign(1, a.add(std::move(sep), kids));
         return a;
@@ -416,9 +406,27 @@ FrontTree nested_dissection(int32_t m, const int32_t *adj_ptr, const int32_t *ad
     if (ng > 0) {
         std::vector<int32_t> all(ng);
         std::iota(all.begin(), all.end(), 0);
-#pragma omp parallel
-#pragma omp single
-        top = D.dissect(std::move(all));
+        Subtree head = D.dissect(std::move(all), 0);
+        const int32_t nj = int32_t(D.jobs.size());
+        std::vector<Subtree> done(nj);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int32_t j = 0; j < nj; ++j) done[j] = D.dissect(std::move(D.jobs[j]), -1);
+        // splice the jobs' fronts in place of their stand-ins (postorder is kept)
+        std::vector<std::vector<int32_t>> now(head.verts.size());  // stand-in -> its roots
+        for (size_t f = 0; f < head.verts.size(); ++f) {
+            if (head.job[f] >= 0) {
+                Subtree &d = done[head.job[f]];
+                const int32_t off = int32_t(top.verts.size());
+                for (int32_t r : d.roots) now[f].push_back(r + off);
+                d.roots.clear();
+                top.append(std::move(d));
+                continue;
+            }
+            std::vector<int32_t> kids;
+            for (int32_t c : head.kids[f]) kids.insert(kids.end(), now[c].begin(), now[c].end());
+            now[f].push_back(top.add(std::move(head.verts[f]), std::move(kids)));
+        }
+        for (int32_t r : head.roots) top.roots.insert(top.roots.end(), now[r].begin(), now[r].end());
     }
     // number the fronts' original vertices in postorder, ascending within a front
     {
