@@ -156,3 +156,22 @@ def test_edge_cases():
     G = _grid(5, 1, 27)
     assert np.array_equal(nested_dissection(G.indptr, G.indices, 8)["order"],
                           nested_dissection(G.indptr, G.indices, 8)["order"])
+
+
+def test_same_ordering_from_new_threads():
+    # the stiff factorization runs on a worker thread created per factorization; the
+    # dissection's parallel phase must give the main thread's ordering from any thread
+    import threading
+
+    G = _grid(20, 3, 27)
+    ref = nested_dissection(G.indptr, G.indices, 32)
+    assert ref["p0"].size > 64  # deep enough for the parallel phase below the top levels
+    out = []
+    for _ in range(3):
+        t = threading.Thread(target=lambda: out.append(nested_dissection(G.indptr, G.indices, 32)))
+        t.start()
+        t.join()
+    assert len(out) == 3
+    for r in out:
+        for key in ("order", "p0", "np", "parent", "struct_ptr", "struct_idx"):
+            assert np.array_equal(r[key], ref[key]), key
