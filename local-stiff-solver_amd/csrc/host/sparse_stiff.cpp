@@ -30,6 +30,7 @@
 #include <atomic>
 #include <array>
 #include <chrono>
+#include <functional>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -109,65 +110,38 @@ struct Range {
 // Launch ranges of one batch of fronts (into the task lists of the numeric phase).
 struct GroupPlan {
     int64_t wsize = 0;                         // workspace doubles
-    Range asmb;                                // original entries
+    Range fr;                                  // its fronts (in the front list)
     std::vector<Range> ext;                    // extend-add, one per child slot
     std::vector<std::array<Range, 3>> steps;   // per 128-column step: diagonal block, panel, trailing tiles
     std::vector<std::array<Range, 2>> rounds;  // per block distance of G11: T tiles, G tiles
     Range cl, clt, cu, cm;                     // L21, L21^T, update matrix, G11 mirror
 };
 
-// A dissected subgraph: its fronts in postorder (quotient vertices; children as indices
-// into the same list) and the indices of its root fronts.
-// A front may stand for a subgraph whose dissection is deferred (job >= 0).
-struct Subtree {
-    std::vector<std::vector<int32_t>> verts, kids;
-    std::vector<int32_t> job;
-    std::vector<int32_t> roots;
-    int32_t add(std::vector<int32_t> v, std::vector<int32_t> k, int32_t j = -1)
-    {
-        verts.push_back(std::move(v));
-        kids.push_back(std::move(k));
-        job.push_back(j);
-        return int32_t(verts.size()) - 1;
-    }
-    void append(Subtree &&o)  // o's fronts after this one's; o's roots join the roots
-    {
-        const int32_t off = int32_t(verts.size());
-        for (size_t f = 0; f < o.verts.size(); ++f) {
-            for (int32_t &c : o.kids[f]) c += off;
-            verts.push_back(std::move(o.verts[f]));
-            kids.push_back(std::move(o.kids[f]));
-            job.push_back(o.job[f]);
-        }
-        for (int32_t r : o.roots) roots.push_back(r + off);
-    }
-};
-
 // Dissects a graph whose vertices carry weights (the sizes of groups of original
-// vertices).  The top kParDepth levels of the recursion run first and leave the
-// subgraphs below them as jobs, which are then dissected in parallel (a dynamic loop)
-// and spliced back in place.  A job reads and writes the per-vertex arrays of its own
-// vertices only, except `mark` of neighbours (read to test membership; every subset has
-// a stamp of its own), so the result equals the sequential recursion.
+// vertices), breadth-first: all subgraphs of one depth are split by one parallel loop,
+// then the tree of splits is numbered in postorder, which gives the recursion's order.
+// A split reads and writes the per-vertex arrays of its own vertices only, except `mark`
+// of neighbours (read to test membership; every subset has a stamp of its own), so the
+// result does not depend on the schedule.
 struct Dissector {
     const int32_t *ap, *ai;     // quotient graph
     const int32_t *mptr;        // group sizes: mptr[v + 1] - mptr[v]
     int64_t leaf;
     std::vector<int32_t> mark, lev, vis;
     std::atomic<int32_t> stamp{0};
-    static constexpr int kParDepth = 6;
-    std::vector<std::vector<int32_t>> jobs;  // deferred subgraphs (filled by the top levels only)
+
+    // one node of the tree of splits: a front (kLeaf), the components of a disconnected
+    // subgraph (kParts), or a separator (verts) with its two sides (kSep)
+    enum Kind { kLeaf, kParts, kSep };
+    struct Split {
+        Kind kind = kLeaf;
+        std::vector<int32_t> verts;
+        std::vector<std::vector<int32_t>> parts;
+    };
 
     int64_t weight(int32_t v) const { return mptr[v + 1] - mptr[v]; }
     int32_t marked(int32_t w) const { return __atomic_load_n(&mark[w], __ATOMIC_RELAXED); }
     int32_t fresh() { return stamp.fetch_add(1, std::memory_order_relaxed) + 1; }
-
-    static Subtree single(std::vector<int32_t> V)
-    {
-        Subtree t;
-        t.roots.push_back(t.add(std::move(V), {}));
-        return t;
-    }
 
     // breadth-first search over the current subset from src: q = visit order, lev = levels
     void bfs(int32_t src, int32_t my, std::vector<int32_t> &q)
@@ -190,19 +164,15 @@ struct Dissector {
         }
     }
 
-    // depth >= 0: a top level (subgraphs at kParDepth become jobs); -1: inside a job
-    Subtree dissect(std::vector<int32_t> V, int depth)
+    Split split(std::vector<int32_t> V)
     {
+        Split out;
         int64_t nv = 0;
         for (int32_t v : V) nv += weight(v);
-        if (nv <= leaf) return single(std::move(V));
-        if (depth >= kParDepth) {
-            Subtree t;
-            t.roots.push_back(t.add({}, {}, int32_t(jobs.size())));
-            jobs.push_back(std::move(V));
-            return t;
+        if (nv <= leaf) {
+            out.verts = std::move(V);
+            return out;
         }
-        const int next = depth < 0 ? -1 : depth + 1;
         const int32_t my = fresh();
         for (int32_t v : V) __atomic_store_n(&mark[v], my, __ATOMIC_RELAXED);
         std::sort(V.begin(), V.end());
@@ -227,15 +197,18 @@ struct Dissector {
             }
         }
         if (comps.size() > 1) {
-            Subtree t;
-            for (auto &c : comps) t.append(dissect(std::move(c), next));
-            return t;
+            out.kind = kParts;
+            out.parts = std::move(comps);
+            return out;
         }
         std::vector<int32_t> q;
         bfs(V.front(), my, q);
         bfs(q.back(), my, q);
         const int32_t nlev = lev[q.back()] + 1;
-        if (nlev < 3) return single(std::move(V));
+        if (nlev < 3) {
+            out.verts = std::move(V);
+            return out;
+        }
         std::vector<int64_t> cnt(nlev, 0);
         for (int32_t v : q) cnt[lev[v]] += weight(v);
         // the smallest level whose predecessors hold 35-65 % of the weight (the one
@@ -254,7 +227,7 @@ struct Dissector {
             }
         }
         if (k < 0) k = kh < 0 ? 1 : kh;
-        std::vector<int32_t> left, right, sep;
+        std::vector<int32_t> left, right;
         for (int32_t v : q) {
             const int32_t l = lev[v];
             if (l < k) {
@@ -267,14 +240,58 @@ struct Dissector {
                     const int32_t w = ai[e];
                     cut = marked(w) == my && lev[w] == k + 1;
                 }
-                (cut ? sep : left).push_back(v);
+                (cut ? out.verts : left).push_back(v);
             }
         }
-        Subtree a = dissect(std::move(left), next);
-        a.append(dissect(std::move(right), next));
-        const std::vector<int32_t> kids = a.roots;
-        a.roots.assign(1, a.add(std::move(sep), kids));
-        return a;
+        out.kind = kSep;
+        out.parts.push_back(std::move(left));
+        out.parts.push_back(std::move(right));
+        return out;
+    }
+
+    // the fronts of the graph V in postorder: verts[f] (quotient vertices), kids[f]
+    void dissect(std::vector<int32_t> V, std::vector<std::vector<int32_t>> &verts,
+                 std::vector<std::vector<int32_t>> &kids)
+    {
+        struct Node {
+            Kind kind = kLeaf;
+            std::vector<int32_t> verts, child;
+        };
+        std::vector<Node> nodes(1);
+        std::vector<std::pair<int32_t, std::vector<int32_t>>> frontier, next;
+        frontier.emplace_back(0, std::move(V));
+        while (!frontier.empty()) {
+            std::vector<Split> res(frontier.size());
+#pragma omp parallel for schedule(dynamic, 1)
+            for (size_t i = 0; i < frontier.size(); ++i) res[i] = split(std::move(frontier[i].second));
+            next.clear();
+            for (size_t i = 0; i < frontier.size(); ++i) {
+                const int32_t id = frontier[i].first;
+                nodes[id].kind = res[i].kind;
+                nodes[id].verts = std::move(res[i].verts);
+                for (auto &part : res[i].parts) {
+                    const int32_t c = int32_t(nodes.size());
+                    nodes.emplace_back();
+                    nodes[id].child.push_back(c);
+                    next.emplace_back(c, std::move(part));
+                }
+            }
+            frontier.swap(next);
+        }
+        // postorder: the sides (or components) first, then the separator above them
+        std::function<std::vector<int32_t>(int32_t)> emit = [&](int32_t id) -> std::vector<int32_t> {
+            Node &nd = nodes[id];
+            std::vector<int32_t> roots;
+            for (int32_t c : nd.child) {
+                const std::vector<int32_t> r = emit(c);
+                roots.insert(roots.end(), r.begin(), r.end());
+            }
+            if (nd.kind == kParts) return roots;
+            verts.push_back(std::move(nd.verts));
+            kids.push_back(std::move(roots));
+            return {int32_t(verts.size()) - 1};
+        };
+        emit(0);
     }
 };
 
@@ -402,50 +419,30 @@ FrontTree nested_dissection(int32_t m, const int32_t *adj_ptr, const int32_t *ad
     D.mark.assign(ng, 0);
     D.lev.assign(ng, 0);
     D.vis.assign(ng, 0);
-    Subtree top;
+    std::vector<std::vector<int32_t>> fverts, fkids;
     if (ng > 0) {
         std::vector<int32_t> all(ng);
         std::iota(all.begin(), all.end(), 0);
-        Subtree head = D.dissect(std::move(all), 0);
-        const int32_t nj = int32_t(D.jobs.size());
-        std::vector<Subtree> done(nj);
-#pragma omp parallel for schedule(dynamic, 1)
-        for (int32_t j = 0; j < nj; ++j) done[j] = D.dissect(std::move(D.jobs[j]), -1);
-        // splice the jobs' fronts in place of their stand-ins (postorder is kept)
-        std::vector<std::vector<int32_t>> now(head.verts.size());  // stand-in -> its roots
-        for (size_t f = 0; f < head.verts.size(); ++f) {
-            if (head.job[f] >= 0) {
-                Subtree &d = done[head.job[f]];
-                const int32_t off = int32_t(top.verts.size());
-                for (int32_t r : d.roots) now[f].push_back(r + off);
-                d.roots.clear();
-                top.append(std::move(d));
-                continue;
-            }
-            std::vector<int32_t> kids;
-            for (int32_t c : head.kids[f]) kids.insert(kids.end(), now[c].begin(), now[c].end());
-            now[f].push_back(top.add(std::move(head.verts[f]), std::move(kids)));
-        }
-        for (int32_t r : head.roots) top.roots.insert(top.roots.end(), now[r].begin(), now[r].end());
+        D.dissect(std::move(all), fverts, fkids);
     }
     // number the fronts' original vertices in postorder, ascending within a front
     {
         int32_t next = 0;
         std::vector<int32_t> verts;
-        for (size_t f = 0; f < top.verts.size(); ++f) {
+        for (size_t f = 0; f < fverts.size(); ++f) {
             verts.clear();
-            for (int32_t v : top.verts[f]) verts.insert(verts.end(), mem.begin() + mptr[v], mem.begin() + mptr[v + 1]);
+            for (int32_t v : fverts[f]) verts.insert(verts.end(), mem.begin() + mptr[v], mem.begin() + mptr[v + 1]);
             std::sort(verts.begin(), verts.end());
             T.p0.push_back(next);
             T.np.push_back(int32_t(verts.size()));
             T.parent.push_back(-1);
-            T.children.push_back(top.kids[f]);
+            T.children.push_back(fkids[f]);
             for (int32_t v : verts) {
                 T.pos[v] = next;
                 T.order[next] = v;
                 ++next;
             }
-            for (int32_t c : top.kids[f]) T.parent[c] = int32_t(f);
+            for (int32_t c : fkids[f]) T.parent[c] = int32_t(f);
         }
     }
     // update positions: a front's pivots' neighbours and its children's update positions
@@ -595,9 +592,9 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
             for (int32_t v = 0; v < m; ++v)
                 for (int64_t i = eptr[v]; i < eptr[v + 1]; ++i) tidx[cur[erow[i]]++] = v;
         }
-        aidx.reserve(2 * eptr[m]);
-        for (int32_t v = 0; v < m; ++v) {
-            int64_t i = eptr[v], j = tptr[v];
+        // per vertex, the merged list (counted, then written at its offset; both in parallel)
+        auto merge = [&](int32_t v, int32_t *out) {
+            int64_t i = eptr[v], j = tptr[v], k = 0;
             const int64_t ie = eptr[v + 1], je = tptr[v + 1];
             while (i < ie || j < je) {
                 int32_t w;
@@ -609,10 +606,19 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
                     w = erow[i++];
                     ++j;
                 }
-                if (w != v) aidx.push_back(w);
+                if (w != v) {
+                    if (out) out[k] = w;
+                    ++k;
+                }
             }
-            aptr[v + 1] = int32_t(aidx.size());
-        }
+            return int32_t(k);
+        };
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int32_t v = 0; v < m; ++v) aptr[v + 1] = merge(v, nullptr);
+        for (int32_t v = 0; v < m; ++v) aptr[v + 1] += aptr[v];
+        aidx.resize(std::max<int32_t>(aptr[m], 1));
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int32_t v = 0; v < m; ++v) merge(v, aidx.data() + aptr[v]);
     }
     // 4. ordering and front structures
     const FrontTree T = nested_dissection(m, aptr.data(), aidx.data(), leaf);
@@ -708,7 +714,7 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
     auto usize = [&](int32_t f) { return align_up(T.nu(f) * T.nu(f)); };
     std::vector<int64_t> wb(F, 0), upoff(F, 0);
     FirstFit pool;
-    std::vector<int64_t> a_dst, a_src;
+    std::vector<int32_t> gfronts;  // the fronts of every batch, batch after batch
     std::vector<k::ExtendTask> ext;
     std::vector<k::DiagTask> dtask;
     std::vector<k::PanelTask> ptask;
@@ -718,13 +724,9 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
     std::vector<int32_t> stages;  // >= 0: group index; < 0: -1 - front (one-front path)
     int64_t max_w = 1, max_slots = 1, max_m2 = 1, max_pm = 1, max_work = 1;
     auto plan_group = [&](const std::vector<int32_t> &fr, GroupPlan &g) {
-        g.asmb.off = int64_t(a_dst.size());
-        for (int32_t f : fr)
-            for (int64_t t = toff[f]; t < toff[f + 1]; ++t) {
-                a_dst.push_back(wb[f] + int64_t(tli[t]) * mof(f) + tlj[t]);
-                a_src.push_back(tsrc[t]);
-            }
-        g.asmb.n = int64_t(a_dst.size()) - g.asmb.off;
+        g.fr.off = int64_t(gfronts.size());
+        gfronts.insert(gfronts.end(), fr.begin(), fr.end());
+        g.fr.n = int64_t(fr.size());
         size_t kmax = 0;
         int32_t nbmax = 0;
         for (int32_t f : fr) {
@@ -880,7 +882,10 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
         };
         for (const GroupPlan &g : groups) {
             ok = ok && g.wsize <= nW;
-            for (int64_t i = g.asmb.off; i < g.asmb.off + g.asmb.n; ++i) in(a_dst[i], a_dst[i], g.wsize);
+            for (int64_t i = g.fr.off; i < g.fr.off + g.fr.n; ++i) {
+                const int32_t f = gfronts[i];
+                in(wb[f], wb[f] + mof(f) * mof(f) - 1, g.wsize);
+            }
             for (const Range &r : g.ext)
                 for (int64_t i = r.off; i < r.off + r.n; ++i) {
                     const int32_t c = ext[i].c, a = ext[i].a, f = c >= 0 && c < F ? T.parent[c] : -1;
@@ -939,7 +944,8 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
         const bool any_big = groups.size() < stages.size();
         DBuf<double> Fw(any_big ? max_m2 : 1), Gt(any_big ? max_pm : 1), work(any_big ? max_work : 1);
         DBuf<int32_t> d_li, d_lj, d_rel, d_parent;
-        DBuf<int64_t> d_src, d_wb, d_upoff, d_adst, d_asrc;
+        DBuf<int64_t> d_src, d_wb, d_upoff, d_toff;
+        DBuf<int32_t> d_gfr;
         DBuf<k::ExtendTask> d_ext;
         DBuf<k::DiagTask> d_dt;
         DBuf<k::PanelTask> d_pt;
@@ -952,8 +958,8 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
         d_parent.upload(T.parent.data(), F, s);
         d_wb.upload(wb.data(), F, s);
         d_upoff.upload(upoff.data(), F, s);
-        d_adst.upload(a_dst.data(), a_dst.size(), s);
-        d_asrc.upload(a_src.data(), a_src.size(), s);
+        d_toff.upload(toff.data(), toff.size(), s);
+        d_gfr.upload(gfronts.data(), gfronts.size(), s);
         d_ext.upload(ext.data(), ext.size(), s);
         d_dt.upload(dtask.data(), dtask.size(), s);
         d_pt.upload(ptask.data(), ptask.size(), s);
@@ -964,7 +970,8 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
                 const GroupPlan &g = groups[st];
                 double *w = W.get();
                 k::fill(g.wsize, w, 0.0, s);
-                k::front_assemble_batch(g.asmb.n, d_adst.get() + g.asmb.off, d_asrc.get() + g.asmb.off, d_vals, w, s);
+                k::front_assemble_batch(int32_t(g.fr.n), d_gfr.get() + g.fr.off, d_toff.get(), d_li.get(), d_lj.get(),
+                                        d_src.get(), d_wb.get(), np_.get(), nu_.get(), d_vals, w, s);
                 for (const Range &r : g.ext)
                     k::front_extend_batch(r.n, d_ext.get() + r.off, d_parent.get(), d_wb.get(), np_.get(), nu_.get(),
                                           woff_.get(), d_upoff.get(), d_rel.get(), upd.get(), w, s);

@@ -249,13 +249,17 @@ __global__ __launch_bounds__(kT) void k_scatter_add(int32_t m, const int32_t *__
 
 // ---- batched factorization of a tree level's fronts ----
 
-__global__ __launch_bounds__(kT) void k_fb_assemble(int64_t ntrip, const int64_t *__restrict__ dst,
-                                                    const int64_t *__restrict__ src,
+// one workgroup per front of the list: its original entries (distinct positions of its lower triangle)
+__global__ __launch_bounds__(kT) void k_fb_assemble(const int32_t *__restrict__ fronts, const int64_t *__restrict__ toff,
+                                                    const int32_t *__restrict__ li, const int32_t *__restrict__ lj,
+                                                    const int64_t *__restrict__ src, const int64_t *__restrict__ wb,
+                                                    const int32_t *__restrict__ np, const int32_t *__restrict__ nu,
                                                     const double *__restrict__ vals, double *__restrict__ W)
 {
-    const int64_t t = int64_t(blockIdx.x) * kT + threadIdx.x;
-    if (t >= ntrip) return;
-    W[dst[t]] = vals[src[t]];
+    const int32_t f = fronts[blockIdx.x];
+    const int64_t m = int64_t(np[f]) + nu[f];
+    double *F = W + wb[f];
+    for (int64_t t = toff[f] + threadIdx.x; t < toff[f + 1]; t += kT) F[int64_t(li[t]) * m + lj[t]] = vals[src[t]];
 }
 
 // one wave per (child, update row); the children of one launch have distinct parents
@@ -573,11 +577,12 @@ void front_extend_batch(int64_t ntask, const ExtendTask *tasks, const int32_t *p
     LSS_LAUNCH_CHECK();
 }
 
-void front_assemble_batch(int64_t ntrip, const int64_t *dst, const int64_t *src, const double *vals, double *W,
-                          hipStream_t s)
+void front_assemble_batch(int32_t nfronts, const int32_t *fronts, const int64_t *toff, const int32_t *li,
+                          const int32_t *lj, const int64_t *src, const int64_t *wb, const int32_t *np,
+                          const int32_t *nu, const double *vals, double *W, hipStream_t s)
 {
-    if (ntrip == 0) return;
-    k_fb_assemble<<<div_up(ntrip, kT), kT, 0, s>>>(ntrip, dst, src, vals, W);
+    if (nfronts == 0) return;
+    k_fb_assemble<<<nfronts, kT, 0, s>>>(fronts, toff, li, lj, src, wb, np, nu, vals, W);
     LSS_LAUNCH_CHECK();
 }
 

@@ -55,9 +55,11 @@ void front_copy_batch(int32_t ntask, const CopyTask *tasks, const double *src, d
 void front_extend_batch(int64_t ntask, const ExtendTask *tasks, const int32_t *parent, const int64_t *wb,
                         const int32_t *np, const int32_t *nu, const int64_t *woff, const int64_t *uoff,
                         const int32_t *rel, const double *U, double *W, hipStream_t s);
-// W[dst[t]] = vals[src[t]] (distinct destinations)
-void front_assemble_batch(int64_t ntrip, const int64_t *dst, const int64_t *src, const double *vals, double *W,
-                          hipStream_t s);
+// the original entries of the listed fronts: W[wb[f] + li[t] * m_f + lj[t]] = vals[src[t]],
+// t in [toff[f], toff[f + 1]) (distinct positions per front)
+void front_assemble_batch(int32_t nfronts, const int32_t *fronts, const int64_t *toff, const int32_t *li,
+                          const int32_t *lj, const int64_t *src, const int64_t *wb, const int32_t *np,
+                          const int32_t *nu, const double *vals, double *W, hipStream_t s);
 
 // out[coff[v] + t] = idx[ptr[dofs[v]] + t]: the stored rows of the unit's columns.
 void sparse_gather_pattern(int32_t m, const int32_t *dofs, const int32_t *ptr, const int64_t *coff,
