@@ -107,6 +107,8 @@ Config parse_config(const char *text, const Config &base)
                 else if (k == "max_dense_component") c.max_dense_component = as_int(v, k);
                 else if (k == "stiff_factorization") c.stiff_factorization = as_str(v, k);
                 else if (k == "sparse_leaf_size") c.sparse_leaf_size = int(as_int(v, k));
+                else if (k == "sparse_batch_rows") c.sparse_batch_rows = as_int(v, k);
+                else if (k == "sparse_batch_bytes") c.sparse_batch_bytes = as_int(v, k);
                 else if (k == "profile_kernels") c.profile_kernels = as_bool(v, k);
                 else if (k == "partitioner") c.partitioner = as_str(v, k);
                 else if (k == "max_coarse_levels") c.max_coarse_levels = int(as_int(v, k));
@@ -149,6 +151,8 @@ Config parse_config(const char *text, const Config &base)
         else if (k == "max_dense_component") c.max_dense_component = as_int(v, k);
         else if (k == "stiff_factorization") c.stiff_factorization = as_str(v, k);
         else if (k == "sparse_leaf_size") c.sparse_leaf_size = int(as_int(v, k));
+        else if (k == "sparse_batch_rows") c.sparse_batch_rows = as_int(v, k);
+        else if (k == "sparse_batch_bytes") c.sparse_batch_bytes = as_int(v, k);
         else if (k == "profile_kernels") c.profile_kernels = as_bool(v, k);
         else if (k == "true_residual_check") c.true_residual_check = as_bool(v, k);
         else if (k == "partitioner") c.partitioner = as_str(v, k);
@@ -176,6 +180,8 @@ Config validate(Config c)
     if (c.stiff_factorization != "auto" && c.stiff_factorization != "dense" && c.stiff_factorization != "sparse")
         bad("stiff_factorization must be auto|dense|sparse");
     if (c.sparse_leaf_size < 1) bad("sparse_leaf_size must be >= 1");
+    if (c.sparse_batch_rows < 0) bad("sparse_batch_rows must be >= 0");
+    if (c.sparse_batch_bytes < 0) bad("sparse_batch_bytes must be >= 0");
     if (c.mfma_flush != 4 && c.mfma_flush != 8 && c.mfma_flush != 16 && c.mfma_flush != 32)
         bad("mfma_flush must be 4, 8, 16 or 32");
     if (c.check_every < 1) bad("check_every must be >= 1");

@@ -61,6 +61,8 @@ struct Config {
     // "sparse" = every unit outside the dense branch takes the sparse path
     std::string stiff_factorization = "auto";
     int sparse_leaf_size = 256;  // nested dissection: subgraphs this small are one front
+    int64_t sparse_batch_rows = 4096;                  // fronts up to this size are factored in level batches
+    int64_t sparse_batch_bytes = int64_t(4) << 30;     // workspace of one batch (its fronts' m^2 doubles)
     bool profile_kernels = false;       // time SpMV / MAS launches with HIP events
     bool true_residual_check = false;   // on convergence verify ||b - A x|| and restart PCG from it if needed
     std::string partitioner = "device";  // MAS partition: "device" (HIP kernels) | "host" (OpenMP)
@@ -97,7 +99,8 @@ struct Config {
                         bad_dof_kmeans_max_iterations, bad_dof_kmeans_expand_neighbors, bad_dof_gmm_jump_threshold,
                         bad_dof_gmm_tol, bad_dof_gmm_max_iterations, bad_dof_gmm_expand_neighbors, precision,
                         inner_spmv, base_preconditioner, damping, lanczos_steps, max_dense_component, partitioner,
-                        max_coarse_levels, partition_local_max, stiff_factorization, sparse_leaf_size);
+                        max_coarse_levels, partition_local_max, stiff_factorization, sparse_leaf_size,
+                        sparse_batch_rows, sparse_batch_bytes);
     }
     bool same_factorization(const Config &o) const { return factor_keys() == o.factor_keys(); }
 };

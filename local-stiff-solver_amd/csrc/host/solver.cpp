@@ -934,7 +934,8 @@ void HybridSolver::build_stiff(int32_t n, const int32_t *ptr, const int32_t *idx
         int64_t mmax = 0;
         d_status_.zero(s);
         for (size_t u = 0; u < sparse_units.size(); ++u) {
-            sparse_[u].factorize(n, sparse_units[u], h_ptr, ptr, idx, vals, cfg_.sparse_leaf_size, kMaxCols, dense_,
+            sparse_[u].factorize(n, sparse_units[u], h_ptr, ptr, idx, vals, cfg_.sparse_leaf_size,
+                                 cfg_.sparse_batch_rows, cfg_.sparse_batch_bytes, kMaxCols, dense_,
                                  d_status_.get(), s);
             mmax = std::max<int64_t>(mmax, int64_t(sparse_units[u].size()));
         }

@@ -18,7 +18,7 @@
 // independent): zero each m x m front, write the original entries of its pivot columns,
 // add the children's update matrices (extend-add, children in order), and factor the
 // first p columns, which leaves the Schur complement in the trailing block.  Fronts of up
-// to kBatchFront rows go through batched kernels, one launch per step for all fronts of
+// to batch_rows rows (4 096 by default) go through batched kernels, one launch per step for all fronts of
 // the level: a 128-column step is the diagonal block (factor + inverse, one workgroup per
 // front), the panel below it and the trailing update in 64 x 64 tiles; the off-diagonal
 // blocks of L11^{-1} follow by block distance.  Larger fronts take the blocked dense path
@@ -48,10 +48,6 @@ double since(clk::time_point t0) { return std::chrono::duration<double>(clk::now
 
 // Largest front (rows) the numeric phase accepts: its dense workspace takes m^2 doubles.
 constexpr int32_t kMaxFront = 49152;
-// Fronts of at most this many rows are factored in level batches; the workspaces of one
-// batch (m^2 doubles per front) take at most kGroupWork doubles.
-constexpr int64_t kBatchFront = 4096;
-constexpr int64_t kGroupWork = int64_t(1) << 29;
 
 int64_t align_up(int64_t n) { return (n + 31) / 32 * 32; }  // 256-byte granules
 
@@ -515,7 +511,8 @@ k::FrontView SparseFactor::view() const
 
 void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const std::vector<int32_t> &h_ptr,
                              const int32_t *d_ptr, const int32_t *d_idx, const double *d_vals, int32_t leaf,
-                             int max_cols, DenseBlas &blas, int *d_status, hipStream_t s)
+                             int64_t batch_rows, int64_t batch_bytes, int max_cols, DenseBlas &blas, int *d_status,
+                             hipStream_t s)
 {
     const auto t0 = clk::now();
     release();
@@ -700,8 +697,8 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
                 for (int64_t a = 0; a < T.nu(c); ++a) cidx[cur[roff[f] + rel[woff[c] + a]]++] = woff[c] + a;
     }
     // 7. plan of the numeric phase, level by level (tree heights; the fronts of one level
-    //    are independent).  Fronts of at most kBatchFront rows are factored together, in
-    //    groups whose workspaces fit kGroupWork doubles, one launch per step for the whole
+    //    are independent).  Fronts of at most batch_rows rows are factored together, in
+    //    groups whose workspaces fit batch_bytes, one launch per step for the whole
     //    group; larger fronts one at a time with the blocked dense path.  Update matrices
     //    live in a pool from their front's factorization to their parent's assembly; the
     //    offsets come from a first-fit plan in the order the stream reads and writes them.
@@ -825,13 +822,13 @@ void SparseFactor::factorize(int32_t n, const std::vector<int32_t> &dofs, const 
     };
     for (int32_t h = 0; h <= H; ++h) {
         std::vector<int32_t> small, big;
-        for (int32_t f : bylev[h]) (mof(f) <= kBatchFront ? small : big).push_back(f);
+        for (int32_t f : bylev[h]) (mof(f) <= batch_rows ? small : big).push_back(f);
         for (size_t i = 0; i < small.size();) {
             std::vector<int32_t> fr;
             int64_t w = 0;
             while (i < small.size()) {
                 const int64_t sz = align_up(mof(small[i]) * mof(small[i]));
-                if (!fr.empty() && w + sz > kGroupWork) break;
+                if (!fr.empty() && (w + sz) * int64_t(sizeof(double)) > batch_bytes) break;
                 wb[small[i]] = w;
                 w += sz;
                 fr.push_back(small[i++]);

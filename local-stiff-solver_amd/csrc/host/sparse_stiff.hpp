@@ -42,10 +42,13 @@ public:
     // Factor S A S^T for the unit `dofs` (ascending global DOF ids).  A: the operator as
     // CSC arrays on the device, column j = operator row j; h_ptr: a host copy of the
     // column pointers.  Entry (i, j), i > j in unit order, is taken from operator row
-    // dofs[i] (the lower triangle the dense path reads).  status |= 2 when not SPD.
+    // dofs[i] (the lower triangle the dense path reads).  status |= 2 when not SPD.  Fronts of
+    // at most batch_rows rows are factored in level batches whose workspaces take at most
+    // batch_bytes (a batch holds at least one front); larger ones one at a time.
     void factorize(int32_t n, const std::vector<int32_t> &dofs, const std::vector<int32_t> &h_ptr,
-                   const int32_t *d_ptr, const int32_t *d_idx, const double *d_vals, int32_t leaf, int max_cols,
-                   DenseBlas &blas, int *d_status, hipStream_t s);
+                   const int32_t *d_ptr, const int32_t *d_idx, const double *d_vals, int32_t leaf,
+                   int64_t batch_rows, int64_t batch_bytes, int max_cols, DenseBlas &blas, int *d_status,
+                   hipStream_t s);
     // z[pdofs[i]] += ((S A S^T)^{-1} t)[i] for k columns; t: k columns of m values (unit
     // order) at stride m, z (and z32 when given) at stride ld.
     void solve_add(const double *t, const int32_t *pdofs, double *z, float *z32, int64_t ld, int k, const int *stop,

@@ -49,6 +49,8 @@ def _create(doc):
     ('{"solver": "GPUHybrid", "GPUHybrid": {"base_preconditioner": "ILU"}}', "base_preconditioner"),
     ('{"solver": "GPUHybrid", "GPUHybrid": {"bank_apply": "wmma"}}', "bank_apply"),
     ('{"solver": "GPUHybrid", "GPUHybrid": {"bank_apply": "mfma", "mfma_flush": 6}}', "mfma_flush"),
+    ('{"solver": "GPUHybrid", "GPUHybrid": {"sparse_batch_rows": -1}}', "sparse_batch_rows"),
+    ('{"solver": "MAS", "MAS": {"sparse_batch_bytes": -5}}', "sparse_batch_bytes"),
 ])
 def test_bad_config_is_rejected(doc, frag):
     rc, msg = _create(doc)

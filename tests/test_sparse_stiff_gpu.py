@@ -5,7 +5,9 @@ GPUHybrid.cu:1231-1249 and 575).
     composed preconditioner is the exact inverse, so PCG stops within two iterations
     at the direct solution; any error in the ordering, the front structures, the
     assembly, the extend-add, the partial factorizations or the level-batched
-    triangular solves shows.  Leaf sizes from 8 (deep trees) to whole-graph fronts.
+    triangular solves shows.  Leaf sizes from 8 (deep trees) to whole-graph fronts; batch
+    limits that split tree levels into many batches, mix batched and one-at-a-time fronts
+    within a level, or batch nothing.
   * On GMM-selected units the sparse factor and the dense factor give the same
     preconditioner to rounding, and the same solve.
   * Units above max_dense_component take the sparse path by themselves ("auto") and
@@ -50,6 +52,40 @@ def test_all_stiff_sparse_is_a_direct_solve(gpu, name, N, leaf):
           f"{sp_info['max_front']} factor entries {sp_info['factor_entries']} iterations {info['num_iterations']}")
     assert info["stop_reason"] == "converged_rel"
     assert info["num_iterations"] <= 2
+    S = A.to_scipy().tocsc()
+    xd = spla.spsolve(S, b)
+    floor = 50 * np.finfo(np.float64).eps * np.linalg.norm(abs(S) @ np.abs(x))
+    assert np.linalg.norm(S @ x - b) <= max(1e-8 * np.linalg.norm(b), floor)
+    assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
+
+
+@pytest.mark.parametrize("rows,nbytes", [(200, 200_000), (100_000, 1), (0, 1 << 32)])
+def test_batch_limits_keep_the_direct_solve(gpu, rows, nbytes):
+    # small batch limits: tree levels split into several batches (one front per batch with a
+    # 1-byte workspace limit), batched and one-at-a-time fronts within a level, or no batches
+    prob = problems.make("elastic_logE", N=16)
+    A = prob.csc()
+    n = A.n
+    ref = Solver.create(_cfg(3, stiff_factorization="sparse", sparse_leaf_size=128))
+    s = Solver.create(_cfg(3, stiff_factorization="sparse", sparse_leaf_size=128, sparse_batch_rows=rows,
+                           sparse_batch_bytes=nbytes))
+    for x in (ref, s):
+        x.set_problematic_dofs(np.arange(n, dtype=np.int32))
+        x.factorize(A)
+    sp, spr = s.get_info()["stiff"]["sparse"], ref.get_info()["stiff"]["sparse"]
+    assert sp["fronts"] == spr["fronts"]
+    nb, ng = sp["batched_fronts"][0], sp["front_groups"][0]
+    print(f"rows {rows} bytes {nbytes}: {nb} of {sp['fronts'][0]} fronts batched in {ng} batches "
+          f"(defaults: {spr['batched_fronts'][0]} in {spr['front_groups'][0]})")
+    if rows == 0:
+        assert nb == 0 and ng == 0
+    elif nbytes == 1:
+        assert nb == sp["fronts"][0] and ng == nb
+    else:
+        assert 0 < nb < sp["fronts"][0] and ng > spr["front_groups"][0]
+    b = prob.rhs(2)
+    x = s.solve(b)
+    assert s.get_info()["num_iterations"] <= 2
     S = A.to_scipy().tocsc()
     xd = spla.spsolve(S, b)
     floor = 50 * np.finfo(np.float64).eps * np.linalg.norm(abs(S) @ np.abs(x))
