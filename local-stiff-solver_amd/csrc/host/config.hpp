@@ -60,7 +60,7 @@ struct Config {
     // units above max_dense_component: "auto" = multifrontal sparse Cholesky, "dense" = rejected;
     // "sparse" = every unit outside the dense branch takes the sparse path
     std::string stiff_factorization = "auto";
-    int sparse_leaf_size = 256;  // nested dissection: subgraphs this small are one front
+    int sparse_leaf_size = 128;  // nested dissection: subgraphs this small are one front
     int64_t sparse_batch_rows = 4096;                  // fronts up to this size are factored in level batches
     int64_t sparse_batch_bytes = int64_t(4) << 30;     // workspace of one batch (its fronts' m^2 doubles)
     bool profile_kernels = false;       // time SpMV / MAS launches with HIP events

@@ -304,7 +304,7 @@ def _csc_arrays(A):
             np.ascontiguousarray(vals, np.float64))
 
 
-def nested_dissection(adj_ptr, adj_idx, leaf: int = 256) -> dict:
+def nested_dissection(adj_ptr, adj_idx, leaf: int = 128) -> dict:
     """Ordering and front structure the sparse stiff factorization builds for a
     symmetric graph (lss_debug_nested_dissection; host only, no GPU needed).
     adj_ptr/adj_idx: CSR without self loops.  Returns order (position -> vertex) and,
