@@ -103,6 +103,8 @@ def main():
                     help="bank storage of the mixed path")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=JSON",
                     help="extra GPUHybrid configuration entry for both paths (experiments)")
+    ap.add_argument("--direct-max", type=int, default=1_000_000,
+                    help="largest system (DOF) whose accuracy reference is an FP64 sparse-direct GPU solve")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (HBM counter calibration)")
     ap.add_argument("--multi-rhs", type=int, default=0, metavar="K",
@@ -213,28 +215,57 @@ def main():
                                     "grad_norm": float(torch.linalg.norm(grad))}
             log(name, f"forward+adjoint {res[name]['adjoint']['s']:.3f} s")
         del s
-    # accuracy: the FP64 GPU path solved far past the benchmark tolerance is the
-    # reference solution (a sparse-direct FP64 solve of this size is out of reach of the
-    # host's direct solvers); for systems of up to 40 000 DOF, scipy's sparse LU
+    # accuracy: the reference solution is, for systems of up to 40 000 DOF, scipy's sparse
+    # LU; up to --direct-max DOF, an FP64 sparse-direct solve on the GPU (the multifrontal
+    # Cholesky of the stiff-unit path with every DOF in one unit, so the first PCG iterate is
+    # the direct solution); above, the FP64 GPU path solved far past the benchmark tolerance
     ref_kind = "FP64 GPU path, recurrence residual 1e-13"
     s = Solver.create({"solver": "GPUHybrid", "GPUHybrid": dict(paths["fp64"], relative_tolerance=1e-13)})
     s.factorize_device(n, d_ptr, d_idx, d_val, stream=stream)
     x.zero_()
     s.solve_device(d_b, x, stream=stream)
     ref_info = s.get_info()
-    x_ref = x.cpu().numpy().copy()
+    x_deep = x.cpu().numpy().copy()
+    x_ref = x_deep
     r_ref_dd, _ = residual_norm(cp_d, ri_d, va_d, d_b, x)
+    ref_iters = ref_info["num_iterations"]
     del s
     if n <= 40_000:
         import scipy.sparse.linalg as spla
 
         x_ref = spla.spsolve(A.to_scipy().tocsc(), b)
         ref_kind = "scipy sparse LU (SuperLU), FP64"
+        ref_iters = None
         r_ref_dd, _ = residual_norm(cp_d, ri_d, va_d, d_b, torch.from_numpy(x_ref).to(dev))
+    elif n <= args.direct_max:
+        g = {"block_size": prob.block_size, "relative_tolerance": 1e-13, "max_iter": 10,
+             "select_bad_dofs_from_l1_norm": False, "decompose_subdomains": False,
+             "stiff_factorization": "sparse"}
+        s = Solver.create({"solver": "GPUHybrid", "GPUHybrid": g})
+        s.set_problematic_dofs(np.arange(n, dtype=np.int32))
+        t0 = time.perf_counter()
+        s.factorize_device(n, d_ptr, d_idx, d_val, stream=stream)
+        x.zero_()
+        s.solve_device(d_b, x, stream=stream)
+        torch.cuda.synchronize()
+        t_direct = time.perf_counter() - t0
+        dinfo = s.get_info()
+        sp = dinfo["stiff"]["sparse"]
+        x_ref = x.cpu().numpy().copy()
+        r_ref_dd, _ = residual_norm(cp_d, ri_d, va_d, d_b, x)
+        ref_iters = dinfo["num_iterations"]
+        ref_kind = (f"FP64 sparse direct on the GPU (multifrontal Cholesky, every DOF one unit: "
+                    f"{sp['fronts'][0]} fronts, largest {sp['max_front'][0]} rows, {sp['factor_entries'][0]:.3e} "
+                    f"factor entries, {t_direct:.2f} s; PCG iterations {dinfo['num_iterations']})")
+        log("direct reference", ref_kind)
+        del s
     accuracy = {"reference": ref_kind, "reference_rel_residual_dd": r_ref_dd / bnorm,
-                "reference_iterations": ref_info["num_iterations"]}
+                "reference_iterations": ref_iters}
     for name in ("mixed", "fp64"):
         accuracy[f"x_{name}_vs_ref"] = float(np.linalg.norm(res[name]["x"] - x_ref) / np.linalg.norm(x_ref))
+    if x_ref is not x_deep:  # the FP64 path iterated to 1e-13, against the direct reference
+        accuracy["x_fp64_deep_vs_ref"] = float(np.linalg.norm(x_deep - x_ref) / np.linalg.norm(x_ref))
+        accuracy["fp64_deep_iterations"] = ref_info["num_iterations"]
     multi = None
     if args.multi_rhs > 1:
         # solve phase only: one factorization, K right-hand sides fused vs one by one
