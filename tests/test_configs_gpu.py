@@ -210,8 +210,6 @@ def test_precond_symmetric_positive_at_size(system):
 def test_fused_two_columns_at_size(system, precision):
     import torch
 
-    if system["N"] < 128:
-        pytest.skip("fused columns are checked at the two contact sizes here (small sizes: test_multi_rhs)")
     prob = system["prob"]
     if precision == "bf16":
         s = Solver.create(_cfg(system["bs"], precision="bf16"))
